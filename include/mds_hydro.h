@@ -1,0 +1,206 @@
+/*
+ * mds_hydro.h -- C-ABI of the MI355X explicit hydrodynamics sweep.
+ *
+ * One context (mds_ctx) owns the device state of one or more independent
+ * problems (a batch) and advances all of them with the reference's SSP-RK2
+ * step: boundaries -> EOS -> sources -> fluxes -> push, twice per step
+ * (step.c:28-129), with the reference's dt policy (step.c:163-203,
+ * main.c:228-252) evaluated on the device.
+ *
+ * Conventions
+ *   - every call returns 0 on success, a negative MDS_E* code on failure;
+ *     mds_last_error() gives a message; the library never exits the process.
+ *   - primitive records follow the reference (decs.h:641-646):
+ *       0 rho, 1 u (internal energy density), 2..4 contravariant coordinate
+ *       velocities v^1..v^3 (io.c:176-186 applies scale factors on output only)
+ *   - host arrays are SoA over interior zones, [nvars][n3][n2][n1], i fastest.
+ *   - one HIP stream per context; mds_run() is asynchronous and performs no
+ *     host synchronisation (t, dt and the step counter live in device memory),
+ *     so it can be enqueued repeatedly or captured.
+ *   - no torch types cross this boundary: plain pointers and sizes only.
+ *
+ * Reference interface replaced by each entry point is cited per declaration.
+ */
+#ifndef MDS_HYDRO_H
+#define MDS_HYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDS_NG 2                 /* ghost depth of the reconstruction stencil, decs.h:61 */
+
+/* precision of the device path */
+#define MDS_FP64       0         /* FP64 end to end, reference operation order */
+#define MDS_FP32_BULK  1         /* FP32 state/reconstruction/Riemann, FP64 update and geometry */
+
+/* boundary codes: same numbers and meaning as decs.h:64-72 */
+#define MDS_BC_OUTFLOW   0
+#define MDS_BC_REFLECT   1
+#define MDS_BC_PERIODIC  3
+
+#define MDS_GEOM_CARTESIAN 0
+#define MDS_GEOM_SPHERICAL 1     /* 1D radial columns in this release */
+
+#define MDS_RCOORD_UNIFORM 0     /* r = |x|     (metric_spherical_general.c:149-152) */
+#define MDS_RCOORD_LOG     1     /* r = exp(x)  (metrics/metric_spherical_logr.c:3-38) */
+
+/* error codes */
+#define MDS_OK          0
+#define MDS_EINVAL     -1
+#define MDS_ENOMEM     -2
+#define MDS_EDEVICE    -3
+#define MDS_ENOTIMPL   -4
+
+/* Per-zone geometry record of a 1D spherical column (one record per zone,
+ * n1 + 2*MDS_NG records, zone -NG first), restating the fields of zone_geom
+ * (decs.h:144-153) that the 1D sweep reads. */
+enum {
+  MDS_ZG_VOLUME = 0,       /* geom.volume                         */
+  MDS_ZG_AREA0,            /* geom.area[0] (lower radial face)    */
+  MDS_ZG_GCOV00,           /* geom.gcov[0][0..2] (zone average)   */
+  MDS_ZG_GCOV01,
+  MDS_ZG_GCOV02,
+  MDS_ZG_GCON0,            /* geom.gcon[0..2]                     */
+  MDS_ZG_GCON1,
+  MDS_ZG_GCON2,
+  MDS_ZG_GCOVF0,           /* geom.gcov[1][0..2] (lower face)     */
+  MDS_ZG_GCOVF1,
+  MDS_ZG_GCOVF2,
+  MDS_ZG_SCALEF,           /* geom.scale[1][0]                    */
+  MDS_ZG_SCALE0,           /* geom.scale[0][0]                    */
+  MDS_ZG_C000,             /* non-zero connection coefficients    */
+  MDS_ZG_C011,             /*   conn[l][m][n] -> C<l><m><n>        */
+  MDS_ZG_C101,
+  MDS_ZG_C110,
+  MDS_ZG_C111,
+  MDS_ZG_C202,
+  MDS_ZG_C220,
+  MDS_ZG_C212,
+  MDS_ZG_C221,
+  MDS_ZG_C022,
+  MDS_ZG_C122,
+  MDS_ZG_COUNT
+};
+
+/* Reconstruction coefficients of one zone for the FP32-bulk path, in a
+ * well-conditioned local form of para_recon (reconstruct.c:127-213):
+ *   first fit   al = a + CLM*dqm + CLP*dqp,  ar = a + CRM*dqm + CRP*dqp
+ *   refit       b2 = (a - al - (ar-al)*MU1) * KINV,  b1 = (ar - al) - b2,
+ *               interior extremum at sigma* = -b1/(2 b2) in (0,1)
+ *   relocation  al' = ar + (a-ar)*INUR  (extremum to right face)
+ *               ar' = al + (a-al)*IMU2  (extremum to left face)
+ * They are evaluated in extended precision from the same rounded volume
+ * moments the FP64 path uses. */
+enum { MDS_RC_CLM = 0, MDS_RC_CLP, MDS_RC_CRM, MDS_RC_CRP, MDS_RC_MU1, MDS_RC_KINV,
+       MDS_RC_INUR, MDS_RC_IMU2, MDS_RC_COUNT };
+
+/* Grid and geometry of one problem.  Built by mds_grid_build() (which follows
+ * the reference's init_coords/init_interp_vol/init_volume/init_area/init_gcov/
+ * init_gcon/init_scale/init_conn, geometry.c:4-62) or supplied by the caller. */
+typedef struct mds_grid {
+  int ndim;                /* 1..3 */
+  int n[3];                /* interior zones (1 for inactive directions) */
+  int geom;                /* MDS_GEOM_* */
+  int rcoord;              /* MDS_RCOORD_* (spherical) */
+  double startx[3];
+  double dx[3];
+  /* volume moments per direction: alpha = int x^2 dV, beta = int x dV,
+   * Gamma = int dV (per unit transverse measure), length n[d] + 2*NG,
+   * zone -NG first (reference alpha0/beta0/Gamma0, alpha1s..., decs.h:360-364) */
+  double *alpha[3], *beta[3], *Gamma[3];
+  /* FP32-bulk reconstruction coefficients per direction: (n[d]+2*NG) * MDS_RC_COUNT */
+  double *rcoef[3];
+  /* Cartesian: uniform volume and areas (metric_cartesian.c:134-187) */
+  double volume;
+  double area[3];
+  /* spherical 1D: (n[0]+2*NG) * MDS_ZG_COUNT per-zone records, else NULL */
+  double *zone;
+} mds_grid;
+
+/* Build the reference geometry for a grid.  geom/rcoord/ndim/n/startx/dx
+ * must be set on input; allocates the arrays (release with mds_grid_free). */
+int mds_grid_build(mds_grid *g);
+void mds_grid_free(mds_grid *g);
+
+/* Physics and boundary description of one problem. */
+typedef struct mds_problem {
+  const mds_grid *grid;    /* copied into the context at create */
+  int bc_lo[3], bc_hi[3];  /* MDS_BC_* per direction (all variables) */
+  double gam;              /* gamma-law index (eos_gamma.c:5-12) */
+  double g0[3];            /* fixed gravity, contravariant (gravity.c:46-64); zeros = none */
+  int nvel;                /* velocity components carried: 0 = auto (all non-zero ones) */
+} mds_problem;
+
+/* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */
+typedef struct mds_run_params {
+  double cfl;              /* 0.9 */
+  double dt_init_safety;   /* 0.1 */
+  double dt_grow_safety;   /* 1.1 */
+  double dtmin;            /* 1e-200 */
+  double dtmax;            /* 1e200 */
+  double initial_dt;       /* 1e100 */
+  double tmax;             /* problems freeze once t >= tmax */
+} mds_run_params;
+
+typedef struct mds_ctx mds_ctx;
+
+/* Default run parameters (io.c:5018-5041). */
+void mds_default_run_params(mds_run_params *rp);
+
+/* Create a context holding nprob problems.  Replaces the reference's
+ * allocate_memory + init_geometry + device residency (main.c:78-165,
+ * utils.c:1500-1630).  precision: MDS_FP64 or MDS_FP32_BULK. */
+int mds_create(const mds_problem *probs, int nprob, const mds_run_params *rp,
+               int precision, mds_ctx **out);
+int mds_destroy(mds_ctx *ctx);
+const char *mds_last_error(const mds_ctx *ctx);
+
+/* Host <-> device copies of the primitive state (reference: omp target
+ * update to/from, main.c:180, step.c:16-24).  prims: nvars*n3*n2*n1 doubles. */
+int mds_upload_prims(mds_ctx *ctx, int prob, const double *prims, int nvars);
+int mds_download_prims(mds_ctx *ctx, int prob, double *prims, int nvars);
+
+/* Initial EOS fill and first dt estimate input (main.c:181, eos.c:4-18);
+ * also resets t, the step counter and the dt history of every problem. */
+int mds_update_eos(mds_ctx *ctx);
+
+/* Enqueue nsteps iterations of the reference main loop body for every active
+ * problem: estimate_dt (step.c:163-203) + dt policy (main.c:228-252) +
+ * step() (step.c:28-129).  Asynchronous; no host synchronisation. */
+int mds_run(mds_ctx *ctx, int nsteps);
+/* Same, replayed from a captured hipGraph of one step (captured on first use). */
+int mds_run_graph(mds_ctx *ctx, int nsteps);
+int mds_sync(mds_ctx *ctx);
+
+/* Clock of a problem: time, last dt, steps taken (reference globals t, dt, istep). */
+int mds_get_clock(mds_ctx *ctx, int prob, double *t, double *dt, int *istep);
+
+/* Compensated (Neumaier) FP64 totals of rho*V and E*V over interior zones
+ * (reference calculate_total_mass/energy, utils.c:2337-2366, 2581-2613). */
+int mds_conserved_totals(mds_ctx *ctx, int prob, double out[2]);
+
+/* Diagnostic counters since mds_update_eos: [0] radial faces tagged as shocks
+ * by HLLC, [1] transverse faces solved with HLLE. */
+int mds_counters(mds_ctx *ctx, long long out[2]);
+
+/* The HIP stream (hipStream_t) all work of the context is enqueued on. */
+void *mds_stream(mds_ctx *ctx);
+
+/* Pointwise parity entry points: run the device reconstruction / Riemann
+ * solvers on host arrays (copied in and out).  Replace para_recon
+ * (reconstruct.c:127-213) and riemann_flux_HLLC/HLLE (riemann.c:178-499).
+ *   recon: ncell pencils of 3 values a[3*c+q], moments mom[9*c + 3*q + {0:alpha,1:beta,2:Gamma}],
+ *          faces x[2*c+{0,1}]  ->  al[c], ar[c]
+ *   riemann: states pl/pr[7*f + {rho,u,v1,v2,v3,P,Gamma}], face geometry
+ *          fg[5*f + {area, scale, gcov0, gcov1, gcov2}], dir  ->  flux[5*f+..], sig[f], isshock[f] */
+int mds_recon_points(int precision, int ncell, const double *a, const double *mom,
+                     const double *x, const double *rcoef, double *al, double *ar);
+int mds_riemann_points(int precision, int solver /*0 HLLC, 1 HLLE*/, int nface,
+                       const double *pl, const double *pr, const double *fg, int dir,
+                       double *flux, double *sig, int *isshock);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

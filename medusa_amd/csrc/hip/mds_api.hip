@@ -1,0 +1,712 @@
+// mds_api.hip -- C-ABI of the hydro sweep: context, device residency, the
+// device-side dt policy, hipGraph replay, compensated totals and the
+// pointwise parity entry points.  See include/mds_hydro.h.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdarg.h>
+#include <vector>
+#include <new>
+#include <algorithm>
+#include "mds_internal.h"
+#include "mds_physics.h"
+
+
+extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, const Sweep1DArgs *a,
+                                         const void *sp, const void *sph_, int stage, hipStream_t st);
+extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a, const void *sp,
+                                         const void *sph, int stage, hipStream_t st);
+
+struct mds_ctx {
+  int precision = 0;
+  int nprob = 0;
+  int ndim = 1;
+  int nv = 1;
+  int sph = 0;
+  long long nzone = 0;
+  RunParams rp{};
+  double tmax = 0.0;
+  hipStream_t st = nullptr;
+  // host copies
+  std::vector<DevProb> hprob;
+  std::vector<long long> zstart;        // prefix sums of interior zones (nprob+1)
+  std::vector<int> nvars;
+  // device
+  void *p[5] = {nullptr}, *ph[5] = {nullptr};
+  DevProb *dprob = nullptr;
+  DevClock *dclock = nullptr;
+  int2 *dseg = nullptr;
+  int nseg = 0;
+  unsigned char *dflags = nullptr;
+  unsigned long long *dcount = nullptr;
+  long long *dzstart = nullptr;
+  std::vector<void *> geo_allocs;
+  int chunk2d = 64;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  char err[512] = {0};
+};
+
+static int set_err(mds_ctx *c, int code, const char *fmt, ...)
+{
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err((ctx), MDS_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// small device kernels
+// ---------------------------------------------------------------------------
+
+// estimate_dt tail + dt policy (step.c:192-198, main.c:206-251), one thread
+// per problem.  The clock is advanced here (t += dt, istep++: main.c:263-264)
+// since the stage kernels only read dt.
+__global__ void dt_policy_kernel(DevClock *ck, int nprob, RunParams rp)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  DevClock c = ck[p];
+  if (!c.active) return;
+  if (!(c.t < c.tmax)) { ck[p].active = 0; return; }
+  double md = __longlong_as_double((long long)c.mindt);
+  if (c.istep == 0) md = (md < rp.initial_dt) ? md : rp.initial_dt;
+  md = (md < rp.dtmax) ? md : rp.dtmax;
+  md *= rp.cfl;
+  double dt = md;
+  if (dt < rp.dtmin) { ck[p].status = 1; ck[p].active = 0; return; }
+  if (c.istep == 0) dt *= rp.dt_init_safety;
+  else { double g = c.dtold * rp.dt_grow_safety; dt = (g < dt) ? g : dt; }
+  c.dtold = dt;
+  if (c.t + 1 * dt > c.tmax) dt = (1.000000001 * (c.tmax - c.t)) / 1;
+  if (dt < rp.dtmin) dt = rp.dtmin;
+  c.dt = dt;
+  c.t += dt;
+  c.istep += 1;
+  c.mindt = dt_key(1.0e100);   // step() resets min_dt (step.c:49)
+  ck[p] = c;
+}
+
+// first estimate_dt zone bound from the initial state (main.c:181 then step.c:177-187)
+template <class R>
+__global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long long *zstart, int nprob,
+                               StateArrays<R> sp)
+{
+  long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= zstart[nprob]) return;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (zstart[mid] <= z) lo = mid; else hi = mid - 1;
+  }
+  const DevProb &pb = prob[lo];
+  long long q = z - zstart[lo];
+  double rho = sp.v[0][pb.off + q], uu = sp.v[1][pb.off + q];
+  double press = (pb.gam - 1.0) * uu;
+  double cs = sqrt((pb.gam * press) / rho);
+  int i = (int)(q % pb.n[0]);
+  double loc = 0.0;
+  for (int d = 0; d < pb.ndim; d++) {
+    double sc = 1.0;
+    if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
+    double v = sp.v[2 + d][pb.off + q];
+    loc += (fabs(v) + (cs / sc)) / pb.dx[d];
+  }
+  double b = 1.0 / loc;
+  atomicMin(&ck[lo].mindt, dt_key(b));
+}
+
+// Neumaier-compensated block partials of rho*V and E*V over one problem.
+template <class R>
+__global__ void totals_kernel(const DevProb *prob, int p, long long nz, int nv, StateArrays<R> sp,
+                              double *partial)
+{
+  const DevProb &pb = prob[p];
+  double s[2] = {0.0, 0.0}, cmp[2] = {0.0, 0.0};
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nz;
+       q += (long long)gridDim.x * blockDim.x) {
+    double rho = sp.v[0][pb.off + q], uu = sp.v[1][pb.off + q];
+    double vol = pb.volume;
+    double g[3] = {1.0, 1.0, 1.0};
+    if (pb.sph) {
+      const double *zg = pb.zone + (size_t)(q + MDS_NG) * MDS_ZG_COUNT;
+      vol = zg[MDS_ZG_VOLUME];
+      g[0] = zg[MDS_ZG_GCOV00]; g[1] = zg[MDS_ZG_GCOV01]; g[2] = zg[MDS_ZG_GCOV02];
+    }
+    double vsq = 0.0;
+    for (int d = 0; d < nv; d++) { double v = sp.v[2 + d][pb.off + q]; vsq += v * (g[d] * v); }
+    double val[2] = {rho * vol, (((0.5 * rho) * vsq) + uu) * vol};
+    for (int k = 0; k < 2; k++) {
+      double t = s[k] + val[k];
+      if (fabs(s[k]) >= fabs(val[k])) cmp[k] += (s[k] - t) + val[k];
+      else cmp[k] += (val[k] - t) + s[k];
+      s[k] = t;
+    }
+  }
+  __shared__ double sh[4][256];
+  sh[0][threadIdx.x] = s[0]; sh[1][threadIdx.x] = cmp[0];
+  sh[2][threadIdx.x] = s[1]; sh[3][threadIdx.x] = cmp[1];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double S[2] = {0, 0}, C[2] = {0, 0};
+    for (int t = 0; t < (int)blockDim.x; t++)
+      for (int k = 0; k < 2; k++) {
+        double vals[2] = {sh[2 * k][t], sh[2 * k + 1][t]};
+        for (int m = 0; m < 2; m++) {
+          double tt = S[k] + vals[m];
+          if (fabs(S[k]) >= fabs(vals[m])) C[k] += (S[k] - tt) + vals[m];
+          else C[k] += (vals[m] - tt) + S[k];
+          S[k] = tt;
+        }
+      }
+    partial[4 * blockIdx.x + 0] = S[0];
+    partial[4 * blockIdx.x + 1] = C[0];
+    partial[4 * blockIdx.x + 2] = S[1];
+    partial[4 * blockIdx.x + 3] = C[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static size_t rsize(const mds_ctx *c) { return c->precision == MDS_FP32_BULK ? sizeof(float) : sizeof(double); }
+
+template <class R>
+static StateArrays<R> arrays(void *const v[5])
+{
+  StateArrays<R> s;
+  for (int k = 0; k < 5; k++) s.v[k] = (R *)v[k];
+  return s;
+}
+
+static int upload_geo(mds_ctx *c, const void *src, size_t bytes, const void **dst)
+{
+  void *d = nullptr;
+  HIPCHK(c, hipMalloc(&d, bytes));
+  c->geo_allocs.push_back(d);
+  HIPCHK(c, hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+  *dst = d;
+  return 0;
+}
+
+extern "C" void mds_default_run_params(mds_run_params *rp)
+{
+  rp->cfl = 0.9;
+  rp->dt_init_safety = 0.1;
+  rp->dt_grow_safety = 1.1;
+  rp->dtmin = 1.0e-200;
+  rp->dtmax = 1.0e200;
+  rp->initial_dt = 1.0e100;
+  rp->tmax = 0.0;
+}
+
+extern "C" const char *mds_last_error(const mds_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+extern "C" void *mds_stream(mds_ctx *ctx) { return ctx ? (void *)ctx->st : nullptr; }
+
+extern "C" int mds_destroy(mds_ctx *c)
+{
+  if (!c) return MDS_OK;
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  for (int k = 0; k < 5; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
+  for (void *g : c->geo_allocs) hipFree(g);
+  if (c->dprob) hipFree(c->dprob);
+  if (c->dclock) hipFree(c->dclock);
+  if (c->dseg) hipFree(c->dseg);
+  if (c->dflags) hipFree(c->dflags);
+  if (c->dcount) hipFree(c->dcount);
+  if (c->dzstart) hipFree(c->dzstart);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return MDS_OK;
+}
+
+static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const mds_run_params *rp)
+{
+  c->nprob = nprob;
+  c->rp.cfl = rp->cfl;
+  c->rp.dt_init_safety = rp->dt_init_safety;
+  c->rp.dt_grow_safety = rp->dt_grow_safety;
+  c->rp.dtmin = rp->dtmin;
+  c->rp.dtmax = rp->dtmax;
+  c->rp.initial_dt = rp->initial_dt;
+  c->tmax = rp->tmax;
+  c->ndim = probs[0].grid->ndim;
+  c->sph = probs[0].grid->geom == MDS_GEOM_SPHERICAL;
+  c->nv = probs[0].nvel ? probs[0].nvel : (c->sph ? 3 : c->ndim);
+  if (c->ndim > 1 && nprob != 1) return set_err(c, MDS_ENOTIMPL, "batches are supported for 1D columns only");
+  if (c->ndim == 3) return set_err(c, MDS_ENOTIMPL, "3D sweep not available in this build");
+  if (c->ndim == 1 && c->nv == 2) c->nv = 3;
+  if (c->ndim == 2 && c->nv < 2) c->nv = 2;
+  if (c->sph && c->ndim != 1) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D only");
+  HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  c->zstart.assign(nprob + 1, 0);
+  c->hprob.resize(nprob);
+  for (int p = 0; p < nprob; p++) {
+    const mds_problem &P = probs[p];
+    const mds_grid *g = P.grid;
+    if (!g || g->ndim != c->ndim || (g->geom == MDS_GEOM_SPHERICAL) != (c->sph != 0))
+      return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same ndim and geometry", p);
+    int nvp = P.nvel ? P.nvel : (c->sph ? 3 : c->ndim);
+    if (c->ndim == 1 && nvp == 2) nvp = 3;
+    if (c->ndim == 2 && nvp < 2) nvp = 2;
+    if (nvp != c->nv) return set_err(c, MDS_EINVAL, "problem %d: velocity component count differs in the batch", p);
+    for (int d = 0; d < g->ndim; d++) {
+      if (g->n[d] < 4) return set_err(c, MDS_EINVAL, "problem %d: need at least 4 zones per active direction", p);
+      for (int side = 0; side < 2; side++) {
+        int b = side ? P.bc_hi[d] : P.bc_lo[d];
+        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC)
+          return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
+      }
+    }
+    long long nz = (long long)g->n[0] * (g->ndim > 1 ? g->n[1] : 1) * (g->ndim > 2 ? g->n[2] : 1);
+    c->zstart[p + 1] = c->zstart[p] + nz;
+    DevProb &d = c->hprob[p];
+    memset(&d, 0, sizeof(d));
+    d.off = c->zstart[p];
+    d.ndim = g->ndim;
+    d.sph = c->sph;
+    d.gam = P.gam;
+    for (int k = 0; k < 3; k++) {
+      d.n[k] = k < g->ndim ? g->n[k] : 1;
+      d.bc_lo[k] = P.bc_lo[k];
+      d.bc_hi[k] = P.bc_hi[k];
+      d.g0[k] = P.g0[k];
+      d.startx[k] = g->startx[k];
+      d.dx[k] = g->dx[k];
+      d.area[k] = g->area[k];
+    }
+    d.volume = g->volume;
+    for (int k = 0; k < g->ndim; k++) {
+      int nzk = g->n[k] + 2 * MDS_NG;
+      std::vector<double> mom(3 * (size_t)nzk);
+      std::vector<float> rcf((size_t)nzk * MDS_RC_COUNT);
+      for (int q = 0; q < nzk; q++) {
+        mom[3 * q + 0] = g->alpha[k][q];
+        mom[3 * q + 1] = g->beta[k][q];
+        mom[3 * q + 2] = g->Gamma[k][q];
+      }
+      for (size_t q = 0; q < rcf.size(); q++) rcf[q] = (float)g->rcoef[k][q];
+      int rc1 = upload_geo(c, mom.data(), mom.size() * sizeof(double), (const void **)&d.mom[k]);
+      if (rc1) return rc1;
+      rc1 = upload_geo(c, rcf.data(), rcf.size() * sizeof(float), (const void **)&d.rc[k]);
+      if (rc1) return rc1;
+    }
+    if (c->sph) {
+      if (!g->zone) return set_err(c, MDS_EINVAL, "problem %d: spherical grid without zone records", p);
+      int rc1 = upload_geo(c, g->zone, (size_t)(g->n[0] + 2 * MDS_NG) * MDS_ZG_COUNT * sizeof(double),
+                           (const void **)&d.zone);
+      if (rc1) return rc1;
+    }
+  }
+  c->nzone = c->zstart[nprob];
+  size_t rs = rsize(c);
+  for (int k = 0; k < 5; k++) {
+    HIPCHK(c, hipMalloc(&c->p[k], (size_t)c->nzone * rs));
+    HIPCHK(c, hipMalloc(&c->ph[k], (size_t)c->nzone * rs));
+    HIPCHK(c, hipMemset(c->p[k], 0, (size_t)c->nzone * rs));
+    HIPCHK(c, hipMemset(c->ph[k], 0, (size_t)c->nzone * rs));
+  }
+  HIPCHK(c, hipMalloc(&c->dprob, sizeof(DevProb) * nprob));
+  HIPCHK(c, hipMemcpy(c->dprob, c->hprob.data(), sizeof(DevProb) * nprob, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMalloc(&c->dclock, sizeof(DevClock) * nprob));
+  HIPCHK(c, hipMemset(c->dclock, 0, sizeof(DevClock) * nprob));
+  HIPCHK(c, hipMalloc(&c->dcount, 4 * sizeof(unsigned long long)));
+  HIPCHK(c, hipMemset(c->dcount, 0, 4 * sizeof(unsigned long long)));
+  HIPCHK(c, hipMalloc(&c->dzstart, sizeof(long long) * (nprob + 1)));
+  HIPCHK(c, hipMemcpy(c->dzstart, c->zstart.data(), sizeof(long long) * (nprob + 1), hipMemcpyHostToDevice));
+  if (c->ndim == 1) {
+    // segment table, longest problems first so the tail of the launch is short
+    std::vector<int> order(nprob);
+    for (int p = 0; p < nprob; p++) order[p] = p;
+    std::vector<int2> seg;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return c->hprob[a].n[0] > c->hprob[b].n[0]; });
+    for (int p : order)
+      for (int s = 0; s < c->hprob[p].n[0]; s += MDS_SEG1D) seg.push_back(make_int2(p, s));
+    c->nseg = (int)seg.size();
+    HIPCHK(c, hipMalloc(&c->dseg, sizeof(int2) * seg.size()));
+    HIPCHK(c, hipMemcpy(c->dseg, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(c, hipMalloc(&c->dflags, (size_t)c->nzone));
+    HIPCHK(c, hipMemset(c->dflags, 0, (size_t)c->nzone));
+  }
+  return MDS_OK;
+}
+
+extern "C" int mds_create(const mds_problem *probs, int nprob, const mds_run_params *rp, int precision,
+                          mds_ctx **out)
+{
+  if (!out) return MDS_EINVAL;
+  *out = nullptr;
+  if (!probs || nprob < 1 || !rp) return MDS_EINVAL;
+  if (precision != MDS_FP64 && precision != MDS_FP32_BULK) return MDS_EINVAL;
+  mds_ctx *c = new (std::nothrow) mds_ctx();
+  if (!c) return MDS_ENOMEM;
+  c->precision = precision;
+  int rc = create_impl(c, probs, nprob, rp);
+  if (rc) {
+    // keep the message for the caller: print it, since the context is released
+    fprintf(stderr, "mds_create: %s\n", c->err);
+    mds_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MDS_OK;
+}
+
+extern "C" int mds_upload_prims(mds_ctx *c, int prob, const double *prims, int nvars)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !prims || nvars < 2 + c->nv) return MDS_EINVAL;
+  long long nz = c->zstart[prob + 1] - c->zstart[prob];
+  long long off = c->zstart[prob];
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int k = 0; k < 2 + c->nv; k++) {
+    const double *src = prims + (size_t)k * nz;
+    if (c->precision == MDS_FP32_BULK) {
+      std::vector<float> tmp(nz);
+      for (long long q = 0; q < nz; q++) tmp[q] = (float)src[q];
+      HIPCHK(c, hipMemcpy((float *)c->p[k] + off, tmp.data(), nz * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(c, hipMemcpy((double *)c->p[k] + off, src, nz * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  return MDS_OK;
+}
+
+extern "C" int mds_download_prims(mds_ctx *c, int prob, double *prims, int nvars)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !prims || nvars < 2) return MDS_EINVAL;
+  long long nz = c->zstart[prob + 1] - c->zstart[prob];
+  long long off = c->zstart[prob];
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int k = 0; k < nvars && k < 5; k++) {
+    double *dst = prims + (size_t)k * nz;
+    if (k >= 2 + c->nv) { for (long long q = 0; q < nz; q++) dst[q] = 0.0; continue; }
+    if (c->precision == MDS_FP32_BULK) {
+      std::vector<float> tmp(nz);
+      HIPCHK(c, hipMemcpy(tmp.data(), (float *)c->p[k] + off, nz * sizeof(float), hipMemcpyDeviceToHost));
+      for (long long q = 0; q < nz; q++) dst[q] = tmp[q];
+    } else {
+      HIPCHK(c, hipMemcpy(dst, (double *)c->p[k] + off, nz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  for (int k = 5; k < nvars; k++)
+    for (long long q = 0; q < nz; q++) prims[(size_t)k * nz + q] = 0.0;
+  return MDS_OK;
+}
+
+extern "C" int mds_update_eos(mds_ctx *c)
+{
+  if (!c) return MDS_EINVAL;
+  std::vector<DevClock> ck(c->nprob);
+  for (int p = 0; p < c->nprob; p++) {
+    memset(&ck[p], 0, sizeof(DevClock));
+    ck[p].t = 0.0;
+    ck[p].dt = 0.0;
+    ck[p].dtold = 1.0e100;
+    ck[p].tmax = c->tmax;
+    ck[p].mindt = (unsigned long long)0x7FF0000000000000ull;   // +inf until the zone pass
+    ck[p].istep = 0;
+    ck[p].active = 1;
+  }
+  // estimate_dt's first call starts from min_dt = 1e100 (step.c:171-174)
+  {
+    double big = 1.0e100;
+    unsigned long long k;
+    memcpy(&k, &big, sizeof(k));
+    for (int p = 0; p < c->nprob; p++) ck[p].mindt = k;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->dclock, ck.data(), sizeof(DevClock) * c->nprob, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(c->dcount, 0, 4 * sizeof(unsigned long long), c->st));
+  dim3 block(256), grid((unsigned)((c->nzone + 255) / 256));
+  if (c->precision == MDS_FP32_BULK) {
+    hipLaunchKernelGGL(init_dt_kernel<float>, grid, block, 0, c->st, c->dprob, c->dclock, c->dzstart, c->nprob,
+                       arrays<float>(c->p));
+  } else {
+    hipLaunchKernelGGL(init_dt_kernel<double>, grid, block, 0, c->st, c->dprob, c->dclock, c->dzstart,
+                       c->nprob, arrays<double>(c->p));
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDS_OK;
+}
+
+static int enqueue_step(mds_ctx *c)
+{
+  int fp32 = c->precision == MDS_FP32_BULK;
+  hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
+                     c->rp);
+  HIPCHK(c, hipGetLastError());
+  if (c->ndim == 1) {
+    Sweep1DArgs a;
+    a.seg = c->dseg;
+    a.nseg = c->nseg;
+    a.prob = c->dprob;
+    a.clock = c->dclock;
+    a.counters = c->dcount;
+    a.rp = c->rp;
+    for (int stage = 0; stage < 2; stage++) {
+      if (fp32) {
+        StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+        HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
+      } else {
+        StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+        HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
+      }
+    }
+  } else {
+    Sweep2DArgs a;
+    a.prob = c->dprob;
+    a.clock = c->dclock;
+    a.counters = c->dcount;
+    a.flags = c->dflags;
+    a.chunk = c->chunk2d;
+    a.nstrip = (c->hprob[0].n[0] + MDS_SEG1D - 1) / MDS_SEG1D;
+    a.nchunk = (c->hprob[0].n[1] + a.chunk - 1) / a.chunk;
+    for (int stage = 0; stage < 2; stage++) {
+      if (fp32) {
+        StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+        HIPCHK(c, mds_launch_sweep2d(1, c->nv, &a, &sp, &sh, stage, c->st));
+      } else {
+        StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+        HIPCHK(c, mds_launch_sweep2d(0, c->nv, &a, &sp, &sh, stage, c->st));
+      }
+    }
+  }
+  return MDS_OK;
+}
+
+extern "C" int mds_run(mds_ctx *c, int nsteps)
+{
+  if (!c || nsteps < 0) return MDS_EINVAL;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = enqueue_step(c);
+    if (rc) return rc;
+  }
+  return MDS_OK;
+}
+
+extern "C" int mds_run_graph(mds_ctx *c, int nsteps)
+{
+  if (!c || nsteps < 0) return MDS_EINVAL;
+  if (!c->gexec) {
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->st, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+  return MDS_OK;
+}
+
+extern "C" int mds_sync(mds_ctx *c)
+{
+  if (!c) return MDS_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDS_OK;
+}
+
+extern "C" int mds_get_clock(mds_ctx *c, int prob, double *t, double *dt, int *istep)
+{
+  if (!c || prob < 0 || prob >= c->nprob) return MDS_EINVAL;
+  DevClock ck;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(&ck, c->dclock + prob, sizeof(ck), hipMemcpyDeviceToHost));
+  if (t) *t = ck.t;
+  if (dt) *dt = ck.dt;
+  if (istep) *istep = ck.istep;
+  if (ck.status) return set_err(c, MDS_EINVAL, "problem %d: dt dropped below dtmin", prob);
+  return MDS_OK;
+}
+
+extern "C" int mds_counters(mds_ctx *c, long long out[2])
+{
+  if (!c || !out) return MDS_EINVAL;
+  unsigned long long h[4];
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(h, c->dcount, sizeof(h), hipMemcpyDeviceToHost));
+  out[0] = (long long)h[0];
+  out[1] = (long long)h[1];
+  return MDS_OK;
+}
+
+extern "C" int mds_conserved_totals(mds_ctx *c, int prob, double out[2])
+{
+  if (!c || prob < 0 || prob >= c->nprob || !out) return MDS_EINVAL;
+  long long nz = c->zstart[prob + 1] - c->zstart[prob];
+  int nblk = 256;
+  double *dpart = nullptr;
+  HIPCHK(c, hipMalloc(&dpart, sizeof(double) * 4 * nblk));
+  if (c->precision == MDS_FP32_BULK)
+    hipLaunchKernelGGL(totals_kernel<float>, dim3(nblk), dim3(256), 0, c->st, c->dprob, prob, nz, c->nv,
+                       arrays<float>(c->p), dpart);
+  else
+    hipLaunchKernelGGL(totals_kernel<double>, dim3(nblk), dim3(256), 0, c->st, c->dprob, prob, nz, c->nv,
+                       arrays<double>(c->p), dpart);
+  std::vector<double> h(4 * nblk);
+  hipError_t e = hipMemcpyAsync(h.data(), dpart, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  hipFree(dpart);
+  if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "totals: %s", hipGetErrorString(e));
+  for (int k = 0; k < 2; k++) {
+    double S = 0.0, C = 0.0;
+    for (int b = 0; b < nblk; b++)
+      for (int m = 0; m < 2; m++) {
+        double v = h[4 * b + 2 * k + m];
+        double t = S + v;
+        if (fabs(S) >= fabs(v)) C += (S - t) + v; else C += (v - t) + S;
+        S = t;
+      }
+    out[k] = S + C;
+  }
+  return MDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pointwise parity entry points
+// ---------------------------------------------------------------------------
+__global__ void recon_points_x64(int n, const double *a, const double *mom, const double *x, double *al,
+                                 double *ar)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  ReconGeo64 g;
+  g.am = mom[9 * c + 0]; g.bm = mom[9 * c + 1]; g.gm = mom[9 * c + 2];
+  g.a0 = mom[9 * c + 3]; g.b0 = mom[9 * c + 4]; g.g0 = mom[9 * c + 5];
+  g.ap = mom[9 * c + 6]; g.bp = mom[9 * c + 7]; g.gp = mom[9 * c + 8];
+  g.xlo = x[2 * c];
+  g.xhi = x[2 * c + 1];
+  recon_geo_x64(g);
+  para_recon_x64(a[3 * c], a[3 * c + 1], a[3 * c + 2], g, al[c], ar[c]);
+}
+
+__global__ void recon_points_f32(int n, const double *a, const double *rc, double *al, double *ar)
+{
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  ReconGeo32 g;
+  const double *r = rc + (size_t)c * MDS_RC_COUNT;
+  g.clm = (float)r[MDS_RC_CLM]; g.clp = (float)r[MDS_RC_CLP]; g.crm = (float)r[MDS_RC_CRM];
+  g.crp = (float)r[MDS_RC_CRP]; g.mu1 = (float)r[MDS_RC_MU1]; g.kinv = (float)r[MDS_RC_KINV];
+  g.inur = (float)r[MDS_RC_INUR]; g.imu2 = (float)r[MDS_RC_IMU2];
+  float l, rr;
+  para_recon_f32((float)a[3 * c], (float)a[3 * c + 1], (float)a[3 * c + 2], g, l, rr);
+  al[c] = l;
+  ar[c] = rr;
+}
+
+template <class R>
+__global__ void riemann_points(int solver, int n, const double *pl, const double *pr, const double *fg, int dir,
+                               double *flux, double *sig, int *shock)
+{
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  PState<R, 3> L, Rt;
+  L.rho = (R)pl[7 * f + 0]; L.uu = (R)pl[7 * f + 1];
+  L.v[0] = (R)pl[7 * f + 2]; L.v[1] = (R)pl[7 * f + 3]; L.v[2] = (R)pl[7 * f + 4];
+  L.P = (R)pl[7 * f + 5]; L.G = (R)pl[7 * f + 6];
+  Rt.rho = (R)pr[7 * f + 0]; Rt.uu = (R)pr[7 * f + 1];
+  Rt.v[0] = (R)pr[7 * f + 2]; Rt.v[1] = (R)pr[7 * f + 3]; Rt.v[2] = (R)pr[7 * f + 4];
+  Rt.P = (R)pr[7 * f + 5]; Rt.G = (R)pr[7 * f + 6];
+  FaceGeo<R> g;
+  g.area = (R)fg[5 * f + 0];
+  g.scale = (R)fg[5 * f + 1];
+  g.gcov[0] = (R)fg[5 * f + 2]; g.gcov[1] = (R)fg[5 * f + 3]; g.gcov[2] = (R)fg[5 * f + 4];
+  int sh = 0;
+  if constexpr (sizeof(R) == 8) {
+    double F[5], s;
+    if (solver == 0) sh = hllc_x64<3, false>(L, Rt, g, dir, F, s);
+    else hlle_x64<3, false>(L, Rt, g, dir, F, s);
+    for (int q = 0; q < 5; q++) flux[5 * f + q] = F[q];
+    sig[f] = s;
+  } else {
+    float F[5] = {0, 0, 0, 0, 0}, s = 0.0f;
+    FaceGeo<float> g1 = g;
+    g1.area = 1.0f;
+    if (g.area < 1.0e-16f) { for (int q = 0; q < 5; q++) F[q] = 0.0f; s = 1.0e-30f; }
+    else if (solver == 0) sh = hllc_f32<3, false>(L, Rt, g1, dir, F, s);
+    else hlle_f32<3, false>(L, Rt, g1, dir, F, s);
+    for (int q = 0; q < 5; q++) flux[5 * f + q] = fg[5 * f + 0] * (double)F[q];
+    sig[f] = s;
+  }
+  shock[f] = sh;
+}
+
+template <class T>
+static hipError_t dcopy(T **d, const T *h, size_t n)
+{
+  hipError_t e = hipMalloc((void **)d, n * sizeof(T) + 8);
+  if (e != hipSuccess) return e;
+  if (h) e = hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+extern "C" int mds_recon_points(int precision, int ncell, const double *a, const double *mom, const double *x,
+                                const double *rcoef, double *al, double *ar)
+{
+  if (ncell <= 0 || !a || !al || !ar) return MDS_EINVAL;
+  double *da = nullptr, *dm = nullptr, *dx = nullptr, *drc = nullptr, *dl = nullptr, *dr = nullptr;
+  hipError_t e = dcopy(&da, a, 3 * (size_t)ncell);
+  if (e == hipSuccess) e = dcopy(&dl, (const double *)nullptr, (size_t)ncell);
+  if (e == hipSuccess) e = dcopy(&dr, (const double *)nullptr, (size_t)ncell);
+  dim3 b(128), g((ncell + 127) / 128);
+  if (e == hipSuccess) {
+    if (precision == MDS_FP64) {
+      if (!mom || !x) e = hipErrorInvalidValue;
+      if (e == hipSuccess) e = dcopy(&dm, mom, 9 * (size_t)ncell);
+      if (e == hipSuccess) e = dcopy(&dx, x, 2 * (size_t)ncell);
+      if (e == hipSuccess) hipLaunchKernelGGL(recon_points_x64, g, b, 0, 0, ncell, da, dm, dx, dl, dr);
+    } else {
+      if (!rcoef) e = hipErrorInvalidValue;
+      if (e == hipSuccess) e = dcopy(&drc, rcoef, (size_t)MDS_RC_COUNT * ncell);
+      if (e == hipSuccess) hipLaunchKernelGGL(recon_points_f32, g, b, 0, 0, ncell, da, drc, dl, dr);
+    }
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(al, dl, ncell * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ar, dr, ncell * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(da); hipFree(dm); hipFree(dx); hipFree(drc); hipFree(dl); hipFree(dr);
+  return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}
+
+extern "C" int mds_riemann_points(int precision, int solver, int nface, const double *pl, const double *pr,
+                                  const double *fg, int dir, double *flux, double *sig, int *isshock)
+{
+  if (nface <= 0 || !pl || !pr || !fg || !flux || !sig || !isshock || dir < 0 || dir > 2) return MDS_EINVAL;
+  double *dpl = nullptr, *dpr = nullptr, *dfg = nullptr, *dfl = nullptr, *dsg = nullptr;
+  int *dsh = nullptr;
+  hipError_t e = dcopy(&dpl, pl, 7 * (size_t)nface);
+  if (e == hipSuccess) e = dcopy(&dpr, pr, 7 * (size_t)nface);
+  if (e == hipSuccess) e = dcopy(&dfg, fg, 5 * (size_t)nface);
+  if (e == hipSuccess) e = dcopy(&dfl, (const double *)nullptr, 5 * (size_t)nface);
+  if (e == hipSuccess) e = dcopy(&dsg, (const double *)nullptr, (size_t)nface);
+  if (e == hipSuccess) e = dcopy(&dsh, (const int *)nullptr, (size_t)nface);
+  dim3 b(128), g((nface + 127) / 128);
+  if (e == hipSuccess) {
+    if (precision == MDS_FP64)
+      hipLaunchKernelGGL(riemann_points<double>, g, b, 0, 0, solver, nface, dpl, dpr, dfg, dir, dfl, dsg, dsh);
+    else
+      hipLaunchKernelGGL(riemann_points<float>, g, b, 0, 0, solver, nface, dpl, dpr, dfg, dir, dfl, dsg, dsh);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(flux, dfl, 5 * (size_t)nface * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(sig, dsg, (size_t)nface * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(isshock, dsh, (size_t)nface * sizeof(int), hipMemcpyDeviceToHost);
+  hipFree(dpl); hipFree(dpr); hipFree(dfg); hipFree(dfl); hipFree(dsg); hipFree(dsh);
+  return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}

@@ -1,0 +1,487 @@
+// mds_physics.h -- pointwise device physics of the hydro sweep (gfx950).
+//
+// Two families:
+//   *_x64  FP64, the reference's operation order (reconstruct.c:127-213,
+//          riemann.c:4-62,178-499, fluid.c:4-53, rad_fluid.c:164-192,
+//          step.c:205-223).  Built with -ffp-contract=off so that every
+//          product and sum rounds exactly as in the reference CPU build.
+//   *_f32  FP32-bulk: the same algorithms in single precision with the
+//          well-conditioned local reconstruction (MDS_RC_* coefficients),
+//          hardware reciprocal / square root, FMA contraction allowed.
+//
+// Velocity components beyond NV are identically zero and skipped; every
+// skipped term is an exact +-0 contribution in the reference arithmetic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MDS_DEV __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// geometry of one face as seen by a Riemann solve: area, scale[1+dir][dir],
+// gcov[1+dir][0..2].  CART: unit metric, so every multiply/divide by scale or
+// gcov is the identity and is dropped at compile time.
+template <class R>
+struct FaceGeo {
+  R area;
+  R scale;
+  R gcov[3];
+};
+
+template <class R, int NV>
+struct PState {          // reconstructed primitive state on one side of a face
+  R rho, uu, v[3], P, G;
+};
+
+template <class R, int NV>
+struct Flux {
+  R f[5];                // rho, E, m1..m3 (area weighted)
+};
+
+// ---------------------------------------------------------------------------
+// FP64, reference operation order
+// ---------------------------------------------------------------------------
+
+// para_recon for one zone (reconstruct.c:142-201).  am, a0, ap: zone means;
+// al_m..: volume moments alpha/beta/Gamma of zones i-1, i, i+1; xlo, xhi:
+// faces of zone i.  den and den2 depend on the geometry only; the reference
+// recomputes them per variable with identical operands, so hoisting them is
+// bit-exact.
+struct ReconGeo64 {
+  double am, a0, ap;     // alpha
+  double bm, b0, bp;     // beta
+  double gm, g0, gp;     // Gamma
+  double xlo, xhi;
+  double den, den2;
+};
+
+MDS_DEV void recon_geo_x64(ReconGeo64 &g)
+{
+  double t012 = (g.am * g.b0) * g.gp;
+  double t021 = (g.am * g.bp) * g.g0;
+  double t102 = (g.a0 * g.bm) * g.gp;
+  double t120 = (g.a0 * g.bp) * g.gm;
+  double t201 = (g.ap * g.bm) * g.g0;
+  double t210 = (g.ap * g.b0) * g.gm;
+  g.den = 1.0 / (((((t012 + t120) + t201) - t021) - t102) - t210);
+  double d2 = (g.xlo - g.xhi) * ((g.a0 + ((g.g0 * g.xlo) * g.xhi)) - (g.b0 * (g.xlo + g.xhi)));
+  g.den2 = 1.0 / d2;
+}
+
+MDS_DEV void para_recon_x64(double qm, double q0, double qp, const ReconGeo64 &g,
+                            double &al, double &ar)
+{
+  double dqm = q0 - qm;
+  double dqp = qp - q0;
+  if (dqm * dqp <= 0.0) { al = q0; ar = q0; return; }
+  double dqc = qp - qm;
+  double A = ((((dqm * g.bp) * g.gm) * g.g0) + g.gp * (((dqp * g.bm) * g.g0) - ((dqc * g.b0) * g.gm))) * g.den;
+  double B = (-((((dqm * g.ap) * g.gm) * g.g0) + g.gp * (((dqp * g.am) * g.g0) - ((dqc * g.a0) * g.gm)))) * g.den;
+  double C = q0 - ((g.a0 * A) + (g.b0 * B)) / g.g0;
+  double l = g.xlo * ((A * g.xlo) + B) + C;
+  double r = g.xhi * ((A * g.xhi) + B) + C;
+  double mean = (0.75 * qm) + (0.25 * q0);
+  if (((q0 - l) * (l - mean)) < 0.0) l = mean;
+  mean = (0.75 * qp) + (0.25 * q0);
+  if (((mean - r) * (r - q0)) < 0.0) r = mean;
+  double A2 = ((g.b0 * (r - l)) + g.g0 * (((q0 * (g.xlo - g.xhi)) - (r * g.xlo)) + (l * g.xhi))) * g.den2;
+  double B2 = (((g.a0 * (l - r)) + ((g.g0 * (r - q0)) * (g.xlo * g.xlo))) + ((g.g0 * (q0 - l)) * (g.xhi * g.xhi))) * g.den2;
+  double xz = (-B2) / ((2.0 * A2) + 1.0e-16);
+  if (__builtin_isinf(xz) || __builtin_isnan(xz)) xz = -1.0e-200;
+  double xmid = 0.5 * (g.xlo + g.xhi);
+  if (xz > g.xlo && xz < g.xhi) {
+    if (xz > xmid) {
+      double A3 = (g.g0 * (q0 - r)) / (g.a0 + (g.xhi * ((g.g0 * g.xhi) - (2.0 * g.b0))));
+      double B3 = (-2.0 * A3) * g.xhi;
+      double C3 = q0 - ((g.a0 * A3) + (g.b0 * B3)) / g.g0;
+      l = g.xlo * ((A3 * g.xlo) + B3) + C3;
+    } else {
+      double A3 = (g.g0 * (q0 - l)) / (g.a0 + (g.xlo * ((g.g0 * g.xlo) - (2.0 * g.b0))));
+      double B3 = (-2.0 * A3) * g.xlo;
+      double C3 = q0 - ((g.a0 * A3) + (g.b0 * B3)) / g.g0;
+      r = g.xhi * ((A3 * g.xhi) + B3) + C3;
+    }
+  }
+  al = l;
+  ar = r;
+}
+
+// wave-speed estimate shared by HLLC/HLLE (riemann.c:210-303, 368-429)
+template <bool HLLC_FLOOR>
+MDS_DEV void wave_speeds_x64(double rl, double rr, double p_l, double p_r, double G_l, double G_r,
+                             double u_l, double u_r, double cs_l, double cs_r,
+                             double &sl, double &sr, double &ss, double &p_s, int &isshock)
+{
+  double p_max, p_min;
+  if (p_l > p_r) { p_max = p_l; p_min = p_r; } else { p_max = p_r; p_min = p_l; }
+  double pratio = p_max / p_min;
+  isshock = ((p_max - p_min) / p_min > 0.5) ? 1 : 0;
+  p_s = (0.5 * (p_l + p_r)) - (((0.125 * (u_r - u_l)) * (rl + rr)) * (cs_l + cs_r));
+  if (HLLC_FLOOR) p_s = (0 > p_s) ? 0.0 : p_s;
+  else p_s = (p_s > 0.0) ? p_s : 0.0;
+  if (pratio < 2.0 && p_s >= p_min && p_s <= p_max) {
+  } else if (p_s < p_min) {
+    double G_avg = 0.5 * (G_l + G_r);
+    double plr = pow(p_l / p_r, (G_avg - 1.0) / (2.0 * G_avg));
+    double s0 = ((((plr * u_l) / cs_l) + (u_r / cs_r)) + ((2.0 * (plr - 1.0)) / (G_avg - 1.0)))
+                / ((plr / cs_l) + (1.0 / cs_r));
+    p_s = 0.5 * ((p_l * pow(1.0 + (((G_l - 1.0) / (2.0 * cs_l)) * (u_l - s0)), (2.0 * G_l) / (G_l - 1.0)))
+                 + (p_r * pow(1.0 + (((G_r - 1.0) / (2.0 * cs_r)) * (s0 - u_r)), (2.0 * G_r) / (G_r - 1.0))));
+    p_s = (p_s > 0.0) ? p_s : 0.0;
+  } else {
+    double a_l = 2.0 / ((G_l + 1.0) * rl);
+    double a_r = 2.0 / ((G_r + 1.0) * rr);
+    double b_l = ((G_l - 1.0) / (G_l + 1.0)) * p_l;
+    double b_r = ((G_r - 1.0) / (G_r + 1.0)) * p_r;
+    double g_l = sqrt(a_l / (p_s + b_l));
+    double g_r = sqrt(a_r / (p_s + b_r));
+    p_s = (((g_l * p_l) + (g_r * p_r)) - (u_r - u_l)) / (g_l + g_r);
+    p_s = (p_s > 0.0) ? p_s : 0.0;
+  }
+  double q_l = (p_s <= p_l) ? 1.0 : sqrt(1.0 + (((G_l + 1.0) / (2.0 * G_l)) * ((p_s / p_l) - 1.0)));
+  double q_r = (p_s <= p_r) ? 1.0 : sqrt(1.0 + (((G_r + 1.0) / (2.0 * G_r)) * ((p_s / p_r) - 1.0)));
+  sl = u_l - (cs_l * q_l);
+  sr = u_r + (cs_r * q_r);
+  ss = (((p_r - p_l) + ((rl * u_l) * (sl - u_l))) - ((rr * u_r) * (sr - u_r)))
+       / ((rl * (sl - u_l)) - (rr * (sr - u_r)));
+}
+
+// conserved state and physical flux of one side (riemann.c:4-37)
+template <int NV, bool CART>
+MDS_DEV void p_uflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, int dir,
+                         double u[5], double f[5])
+{
+  double vcov[3];
+  for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
+  double v = p.v[dir];
+  u[0] = p.rho;
+  double vsq = 0.0;
+  for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
+  double etot = ((0.5 * u[0]) * vsq) + p.uu;
+  u[1] = etot;
+  for (int d = 0; d < NV; d++) u[2 + d] = u[0] * vcov[d];
+  f[0] = u[0] * v;
+  f[1] = v * (etot + p.P);
+  for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
+  f[2 + dir] += p.P;
+}
+
+// area-weighted physical flux (riemann.c:39-62)
+template <int NV, bool CART>
+MDS_DEV void pflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, int dir, double f[5])
+{
+  double vcov[3];
+  for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
+  double v = p.v[dir];
+  f[0] = (g.area * p.rho) * v;
+  for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
+  f[2 + dir] += g.area * p.P;
+  double vsq = 0.0;
+  for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
+  double etot = ((0.5 * p.rho) * vsq) + p.uu;
+  f[1] = (g.area * v) * (etot + p.P);
+}
+
+// HLLC (riemann.c:328-499).  Returns isshock.
+template <int NV, bool CART>
+MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
+                     const FaceGeo<double> &g, int dir, double flux[5], double &sig)
+{
+  if (g.area < 1.0e-16) {
+    sig = 1.0e-200;
+    for (int q = 0; q < 2 + NV; q++) flux[q] = 0.0;
+    return 0;
+  }
+  double u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
+  double u_r = CART ? Rt.v[dir] : Rt.v[dir] * g.scale;
+  double cs_l = sqrt((L.G * L.P) / L.rho);
+  double cs_r = sqrt((Rt.G * Rt.P) / Rt.rho);
+  double sl, sr, ss, p_s;
+  int isshock;
+  wave_speeds_x64<true>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  if (ss > sr) sr = 1.001 * ss;
+  if (ss < sl) sl = 1.001 * ss;
+  double smax = (fabs(sl) > fabs(sr)) ? fabs(sl) : fabs(sr);
+  sig = CART ? smax : smax / g.scale;
+  if (sl > 0.0) { pflux_x64<NV, CART>(L, g, dir, flux); return isshock; }
+  if (sr < 0.0) { pflux_x64<NV, CART>(Rt, g, dir, flux); return isshock; }
+  double u[5], fx[5], Us[5], vs[3];
+  double slr;
+  const PState<double, NV> &K = (ss > 0.0) ? L : Rt;
+  if (ss > 0.0) {
+    p_uflux_x64<NV, CART>(L, g, dir, u, fx);
+    Us[0] = (L.rho * (sl - u_l)) / (sl - ss);
+    Us[1] = (((u[1] * (sl - u_l)) - (L.P * u_l)) + (p_s * ss)) / (sl - ss);
+    slr = CART ? sl : sl / g.scale;
+  } else {
+    p_uflux_x64<NV, CART>(Rt, g, dir, u, fx);
+    Us[0] = (Rt.rho * (sr - u_r)) / (sr - ss);
+    Us[1] = (((u[1] * (u_r - sr)) + (Rt.P * u_r)) - (p_s * ss)) / (ss - sr);
+    slr = CART ? sr : sr / g.scale;
+  }
+  for (int d = 0; d < NV; d++) vs[d] = K.v[d];
+  vs[dir] = CART ? ss : ss / g.scale;
+  for (int d = 0; d < NV; d++) Us[2 + d] = (CART ? vs[d] : g.gcov[d] * vs[d]) * Us[0];
+  for (int q = 0; q < 2 + NV; q++) flux[q] = g.area * (fx[q] + (slr * (Us[q] - u[q])));
+  return isshock;
+}
+
+// HLLE (riemann.c:178-325)
+template <int NV, bool CART>
+MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
+                      const FaceGeo<double> &g, int dir, double flux[5], double &sig)
+{
+  if (g.area < 1.0e-16) {
+    sig = 1.0e-200;
+    for (int q = 0; q < 2 + NV; q++) flux[q] = 0.0;
+    return;
+  }
+  double u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
+  double u_r = CART ? Rt.v[dir] : Rt.v[dir] * g.scale;
+  double cs_l = sqrt((L.G * L.P) / L.rho);
+  double cs_r = sqrt((Rt.G * Rt.P) / Rt.rho);
+  double sl, sr, ss, p_s;
+  int isshock;
+  wave_speeds_x64<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  double ul[5], ur[5], fl[5], fr[5];
+  p_uflux_x64<NV, CART>(L, g, dir, ul, fl);
+  p_uflux_x64<NV, CART>(Rt, g, dir, ur, fr);
+  double srs = CART ? sr : sr / g.scale;
+  double sls = CART ? sl : sl / g.scale;
+  double cmax = (srs > 0.0) ? srs : 0.0;
+  double cmin = fabs((sls < 0.0) ? sls : 0.0);
+  sig = (cmax > cmin) ? cmax : cmin;
+  for (int q = 0; q < 2 + NV; q++)
+    flux[q] = (g.area * (((cmin * fr[q]) + (cmax * fl[q])) - ((cmax * cmin) * (ur[q] - ul[q])))) / (cmax + cmin);
+}
+
+// fluid.c:4-25
+template <int NV, bool CART>
+MDS_DEV void p_to_u_x64(double rho, double uu, const double v[3], const double gcov[3], double u[5])
+{
+  double vcov[3];
+  for (int d = 0; d < NV; d++) vcov[d] = CART ? v[d] : gcov[d] * v[d];
+  u[0] = rho;
+  double vsq = 0.0;
+  for (int d = 0; d < NV; d++) {
+    vsq += v[d] * vcov[d];
+    u[2 + d] = u[0] * vcov[d];
+  }
+  u[1] = ((0.5 * u[0]) * vsq) + uu;
+}
+
+// fluid.c:27-53 (no floors)
+template <int NV, bool CART>
+MDS_DEV void u_to_p_x64(double u[5], const double gcon[3], double &rho, double &uu, double v[3])
+{
+  rho = u[0];
+  for (int d = 0; d < NV; d++) u[2 + d] /= u[0];
+  for (int d = 0; d < NV; d++) v[d] = CART ? u[2 + d] : gcon[d] * u[2 + d];
+  double vsq = 0.0;
+  for (int d = 0; d < NV; d++) vsq += u[2 + d] * v[d];
+  uu = u[1] - ((0.5 * rho) * vsq);
+}
+
+// step.c:205-223
+template <int NV>
+MDS_DEV double dt_source_control_x64(double uu, const double v[3], const double u[5], const double s[5])
+{
+  const double alpha = 0.02;
+  double mdt = 1.0e100;
+  double local_dt = (alpha * u[0]) / (fabs(s[0]) + 1.0e-16);
+  mdt = (mdt < local_dt) ? mdt : local_dt;
+  local_dt = (alpha * u[1]) / (fabs(s[1]) + 1.0e-16);
+  mdt = (mdt < local_dt) ? mdt : local_dt;
+  double momterm = 0.0;
+  for (int d = 0; d < NV; d++) momterm += v[d] * s[2 + d];
+  local_dt = (2.0 * uu) / (fabs(fabs(s[1]) - fabs(momterm)) + 1.0e-16);
+  mdt = (mdt < local_dt) ? mdt : local_dt;
+  return mdt;
+}
+
+// ---------------------------------------------------------------------------
+// FP32-bulk
+// ---------------------------------------------------------------------------
+struct ReconGeo32 {
+  float clm, clp, crm, crp, mu1, kinv, inur, imu2;
+};
+
+MDS_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+MDS_DEV float fsqrt(float x) { return __builtin_sqrtf(x); }
+
+// para_recon in local coordinates: identical branch structure, well-conditioned.
+MDS_DEV void para_recon_f32(float qm, float q0, float qp, const ReconGeo32 &g, float &al, float &ar)
+{
+#pragma clang fp contract(fast)
+  float dqm = q0 - qm;
+  float dqp = qp - q0;
+  if (dqm * dqp <= 0.0f) { al = q0; ar = q0; return; }
+  float l = q0 + g.clm * dqm + g.clp * dqp;
+  float r = q0 + g.crm * dqm + g.crp * dqp;
+  float mean = 0.75f * qm + 0.25f * q0;
+  if ((q0 - l) * (l - mean) < 0.0f) l = mean;
+  mean = 0.75f * qp + 0.25f * q0;
+  if ((mean - r) * (r - q0) < 0.0f) r = mean;
+  float b2 = (q0 - l - (r - l) * g.mu1) * g.kinv;
+  float b1 = (r - l) - b2;
+  // extremum sigma* = -b1/(2 b2) strictly inside (0,1)?
+  float sz = -b1 * frcp(2.0f * b2);
+  if (b2 != 0.0f && sz > 0.0f && sz < 1.0f) {
+    if (sz > 0.5f) l = r + (q0 - r) * g.inur;
+    else r = l + (q0 - l) * g.imu2;
+  }
+  al = l;
+  ar = r;
+}
+
+template <bool HLLC_FLOOR>
+MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l, float G_r,
+                             float u_l, float u_r, float cs_l, float cs_r,
+                             float &sl, float &sr, float &ss, float &p_s, int &isshock)
+{
+#pragma clang fp contract(fast)
+  float p_max = fmaxf(p_l, p_r), p_min = fminf(p_l, p_r);
+  float ipmin = frcp(p_min);
+  float pratio = p_max * ipmin;
+  isshock = ((p_max - p_min) * ipmin > 0.5f) ? 1 : 0;
+  p_s = 0.5f * (p_l + p_r) - 0.125f * (u_r - u_l) * (rl + rr) * (cs_l + cs_r);
+  p_s = fmaxf(p_s, 0.0f);
+  if (pratio < 2.0f && p_s >= p_min && p_s <= p_max) {
+  } else if (p_s < p_min) {
+    float G_avg = 0.5f * (G_l + G_r);
+    float plr = __builtin_powf(p_l * frcp(p_r), (G_avg - 1.0f) * frcp(2.0f * G_avg));
+    float icl = frcp(cs_l), icr = frcp(cs_r);
+    float s0 = (plr * u_l * icl + u_r * icr + 2.0f * (plr - 1.0f) * frcp(G_avg - 1.0f)) * frcp(plr * icl + icr);
+    p_s = 0.5f * (p_l * __builtin_powf(1.0f + (G_l - 1.0f) * 0.5f * icl * (u_l - s0), 2.0f * G_l * frcp(G_l - 1.0f))
+                  + p_r * __builtin_powf(1.0f + (G_r - 1.0f) * 0.5f * icr * (s0 - u_r), 2.0f * G_r * frcp(G_r - 1.0f)));
+    p_s = fmaxf(p_s, 0.0f);
+  } else {
+    float a_l = 2.0f * frcp((G_l + 1.0f) * rl);
+    float a_r = 2.0f * frcp((G_r + 1.0f) * rr);
+    float b_l = (G_l - 1.0f) * frcp(G_l + 1.0f) * p_l;
+    float b_r = (G_r - 1.0f) * frcp(G_r + 1.0f) * p_r;
+    float g_l = fsqrt(a_l * frcp(p_s + b_l));
+    float g_r = fsqrt(a_r * frcp(p_s + b_r));
+    p_s = (g_l * p_l + g_r * p_r - (u_r - u_l)) * frcp(g_l + g_r);
+    p_s = fmaxf(p_s, 0.0f);
+  }
+  float q_l = (p_s <= p_l) ? 1.0f : fsqrt(1.0f + (G_l + 1.0f) * frcp(2.0f * G_l) * (p_s * frcp(p_l) - 1.0f));
+  float q_r = (p_s <= p_r) ? 1.0f : fsqrt(1.0f + (G_r + 1.0f) * frcp(2.0f * G_r) * (p_s * frcp(p_r) - 1.0f));
+  sl = u_l - cs_l * q_l;
+  sr = u_r + cs_r * q_r;
+  ss = (p_r - p_l + rl * u_l * (sl - u_l) - rr * u_r * (sr - u_r)) * frcp(rl * (sl - u_l) - rr * (sr - u_r));
+}
+
+// Per-unit-area fluxes in FP32 (the FP64 caller applies the FP64 area).
+template <int NV, bool CART>
+MDS_DEV void pflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float f[5])
+{
+#pragma clang fp contract(fast)
+  float vcov[3];
+  for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
+  float v = p.v[dir];
+  f[0] = p.rho * v;
+  for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
+  f[2 + dir] += p.P;
+  float vsq = 0.0f;
+  for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
+  f[1] = v * (0.5f * p.rho * vsq + p.uu + p.P);
+}
+
+template <int NV, bool CART>
+MDS_DEV void p_uflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float u[5], float f[5])
+{
+#pragma clang fp contract(fast)
+  float vcov[3];
+  for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
+  float v = p.v[dir];
+  u[0] = p.rho;
+  float vsq = 0.0f;
+  for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
+  float etot = 0.5f * p.rho * vsq + p.uu;
+  u[1] = etot;
+  for (int d = 0; d < NV; d++) u[2 + d] = p.rho * vcov[d];
+  f[0] = p.rho * v;
+  f[1] = v * (etot + p.P);
+  for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
+  f[2 + dir] += p.P;
+}
+
+// HLLC in FP32; flux per unit area.  Returns isshock.
+template <int NV, bool CART>
+MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
+                     const FaceGeo<float> &g, int dir, float flux[5], float &sig)
+{
+#pragma clang fp contract(fast)
+  float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
+  float u_r = CART ? Rt.v[dir] : Rt.v[dir] * g.scale;
+  float cs_l = fsqrt(L.G * L.P * frcp(L.rho));
+  float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
+  float sl, sr, ss, p_s;
+  int isshock;
+  wave_speeds_f32<true>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  if (ss > sr) sr = 1.001f * ss;
+  if (ss < sl) sl = 1.001f * ss;
+  float iscale = CART ? 1.0f : frcp(g.scale);
+  sig = fmaxf(fabsf(sl), fabsf(sr)) * iscale;
+  if (sl > 0.0f) { pflux_f32<NV, CART>(L, g, dir, flux); return isshock; }
+  if (sr < 0.0f) { pflux_f32<NV, CART>(Rt, g, dir, flux); return isshock; }
+  float u[5], fx[5], Us[5];
+  bool left = ss > 0.0f;
+  const PState<float, NV> &K = left ? L : Rt;
+  float sk = left ? sl : sr, uk = left ? u_l : u_r;
+  p_uflux_f32<NV, CART>(K, g, dir, u, fx);
+  float iden = frcp(sk - ss);
+  Us[0] = K.rho * (sk - uk) * iden;
+  Us[1] = (u[1] * (sk - uk) - K.P * uk + p_s * ss) * iden;
+  float slr = sk * iscale;
+  for (int d = 0; d < NV; d++) {
+    float vs = (d == dir) ? ss * iscale : K.v[d];
+    Us[2 + d] = (CART ? vs : g.gcov[d] * vs) * Us[0];
+  }
+  for (int q = 0; q < 2 + NV; q++) flux[q] = fx[q] + slr * (Us[q] - u[q]);
+  return isshock;
+}
+
+template <int NV, bool CART>
+MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
+                      const FaceGeo<float> &g, int dir, float flux[5], float &sig)
+{
+#pragma clang fp contract(fast)
+  float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
+  float u_r = CART ? Rt.v[dir] : Rt.v[dir] * g.scale;
+  float cs_l = fsqrt(L.G * L.P * frcp(L.rho));
+  float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
+  float sl, sr, ss, p_s;
+  int isshock;
+  wave_speeds_f32<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  float ul[5], ur[5], fl[5], fr[5];
+  p_uflux_f32<NV, CART>(L, g, dir, ul, fl);
+  p_uflux_f32<NV, CART>(Rt, g, dir, ur, fr);
+  float iscale = CART ? 1.0f : frcp(g.scale);
+  float cmax = fmaxf(sr * iscale, 0.0f);
+  float cmin = fabsf(fminf(sl * iscale, 0.0f));
+  sig = fmaxf(cmax, cmin);
+  float iden = frcp(cmax + cmin);
+  for (int q = 0; q < 2 + NV; q++) flux[q] = (cmin * fr[q] + cmax * fl[q] - cmax * cmin * (ur[q] - ul[q])) * iden;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+// positive double -> order-preserving 64-bit key for atomicMin
+MDS_DEV unsigned long long dt_key(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+template <class T>
+MDS_DEV T lane_from_below(T v) { return __shfl_up(v, 1, 64); }     // lane l <- lane l-1
+template <class T>
+MDS_DEV T lane_from_above(T v) { return __shfl_down(v, 1, 64); }   // lane l <- lane l+1
+
+MDS_DEV double wave_min(double v)
+{
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  return v;
+}

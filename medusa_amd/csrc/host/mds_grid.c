@@ -1,0 +1,264 @@
+/*
+ * mds_grid.c -- grid and geometry of one problem (host side of the product).
+ *
+ * Follows the reference's geometry initialisation (geometry.c:4-62):
+ *   Cartesian  metrics/metric_cartesian.c:47-105 (moments), :134-187 (volume, area),
+ *              unit metric and zero connection (:189-295)
+ *   spherical  metrics/metric_spherical_general.c:558-603 (coords, moments),
+ *              :708-820 (volume, area), :837-929 (connection), :941-1051
+ *              (gcov, gcon, scale), 1D columns (x1 in [-1,1], dx2 = 2 pi),
+ *              POLYTH theta map with the io.c:5014-5015 defaults,
+ *              radial map r=|x| (:149-152) or r=exp(x) (metric_spherical_logr.c)
+ *   Romberg    utils.c:14-62
+ * and adds the FP32-bulk reconstruction coefficients (MDS_RC_*), evaluated in
+ * long double from the same rounded FP64 volume moments.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../../include/mds_hydro.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846264338327
+#endif
+#define NG MDS_NG
+
+/* ---- Romberg, utils.c:14-62 ---- */
+typedef double (*integrand)(double, const void *);
+
+static double trap_stage(double x0, double x1, int stage, integrand f, const void *ctx)
+{
+  unsigned long long nseg = 1 << stage;
+  double h = (x1 - x0) / nseg;
+  if (stage == 0) return (0.5 * h) * (f(x0, ctx) + f(x1, ctx));
+  double acc = 0.0;
+  for (unsigned long long q = 0; q < (unsigned long long)(1 << (stage - 1)); q++)
+    acc += f(x0 + (2 * q + 1) * h, ctx);
+  return h * acc;
+}
+
+static double romberg(double x0, double x1, integrand f, const void *ctx)
+{
+  double row[16];
+  row[0] = trap_stage(x0, x1, 0, f, ctx);
+  for (int st = 1; st <= 15; st++) {
+    double prev = row[0];
+    row[0] = (0.5 * prev) + trap_stage(x0, x1, st, f, ctx);
+    for (int m = 1; m <= st; m++) {
+      unsigned long long w = 1 << (2 * m);
+      double keep = row[m];
+      row[m] = (1.0 / (w - 1)) * ((w * row[m - 1]) - prev);
+      prev = keep;
+    }
+    if (fabs((row[st] - row[st - 1]) / row[st]) < 1.e-15 || (st > 2 && fabs(row[st]) < 1.e-15))
+      return row[st];
+  }
+  return row[15];
+}
+
+/* ---- spherical maps ---- */
+typedef struct { int rlog; double pnorm, pxt, palpha; } maps;
+
+static double rx(const maps *m, double x) { return m->rlog ? exp(x) : fabs(x); }
+static double drx(const maps *m, double x) { return m->rlog ? exp(x) : 1.0; }
+static double d2rx(const maps *m, double x) { return m->rlog ? exp(x) : 0.0; }
+static double fold(double x) { if (x < -1.0) x = -2.0 - x; if (x > 1.0) x = 2.0 - x; return x; }
+static double thx(const maps *m, double x)
+{
+  x = fold(x);
+  return ((m->pnorm * x) * (1.0 + (pow(x / m->pxt, m->palpha) / (m->palpha + 1.0)))) + (0.5 * M_PI);
+}
+static double dthx(const maps *m, double x) { x = fold(x); return m->pnorm * (1.0 + pow(x / m->pxt, m->palpha)); }
+static double d2thx(const maps *m, double x)
+{
+  x = fold(x);
+  return ((m->pnorm * m->palpha) * pow(x / m->pxt, m->palpha - 1.0)) / m->pxt;
+}
+static double i_r2dr(double x, const void *c) { const maps *m = c; return pow(rx(m, x), 2) * drx(m, x); }
+static double i_xr1(double x, const void *c) { const maps *m = c; return (x * pow(rx(m, x), 2)) * drx(m, x); }
+static double i_xr2(double x, const void *c) { const maps *m = c; return ((x * x) * pow(rx(m, x), 2)) * drx(m, x); }
+static double i_r2d2r(double x, const void *c) { const maps *m = c; return pow(rx(m, x), 2) * d2rx(m, x); }
+static double i_r3(double x, const void *c) { const maps *m = c; return pow(rx(m, x), 3); }
+static double i_r2dr3(double x, const void *c) { const maps *m = c; return pow(rx(m, x), 2) * pow(drx(m, x), 3); }
+static double i_rdr2(double x, const void *c) { const maps *m = c; return rx(m, x) * pow(drx(m, x), 2); }
+static double i_sdth(double x, const void *c) { const maps *m = c; return sin(thx(m, x)) * dthx(m, x); }
+static double i_sdth3(double x, const void *c) { const maps *m = c; return sin(thx(m, x)) * pow(dthx(m, x), 3); }
+static double i_s3dth(double x, const void *c) { const maps *m = c; return pow(sin(thx(m, x)), 3) * dthx(m, x); }
+static double i_sd2th(double x, const void *c) { const maps *m = c; return sin(thx(m, x)) * d2thx(m, x); }
+static double i_cdth2(double x, const void *c) { const maps *m = c; return cos(thx(m, x)) * pow(dthx(m, x), 2); }
+
+/* FP32-bulk reconstruction coefficients of zone q (pencil index with NG
+ * offset) from the rounded moments, in long double. */
+static void recon_coeffs(const double *al, const double *be, const double *ga,
+                         double startx, double dx, int nz, double *rc)
+{
+  for (int q = 0; q < nz; q++) {
+    double *o = &rc[q * MDS_RC_COUNT];
+    for (int k = 0; k < MDS_RC_COUNT; k++) o[k] = 0.0;
+    long double xlo = (long double)(startx + (q - NG) * dx);
+    long double xhi = (long double)(startx + (q + 1 - NG) * dx);
+    long double h = xhi - xlo;
+    long double a0 = al[q], b0 = be[q], g0 = ga[q];
+    long double m1 = b0 / g0, m2 = a0 / g0;            /* <x>, <x^2> of the zone */
+    long double mu1 = (m1 - xlo) / h;
+    long double mu2 = ((m2 - 2.0L * xlo * m1) + xlo * xlo) / (h * h);
+    long double nur = ((m2 - 2.0L * xhi * m1) + xhi * xhi) / (h * h);
+    o[MDS_RC_MU1] = (double)mu1;
+    o[MDS_RC_KINV] = (double)(1.0L / (mu2 - mu1));
+    o[MDS_RC_INUR] = (double)(1.0L / nur);
+    o[MDS_RC_IMU2] = (double)(1.0L / mu2);
+    if (q == 0 || q == nz - 1) continue;   /* no neighbours: flat in the first fit */
+    long double am = al[q - 1], ap = al[q + 1], bm = be[q - 1], bp = be[q + 1], gm = ga[q - 1], gp = ga[q + 1];
+    long double det = am * b0 * gp + a0 * bp * gm + ap * bm * g0 - am * bp * g0 - a0 * bm * gp - ap * b0 * gm;
+    long double den = 1.0L / det;
+    /* Cramer coefficients of A and B in dqm and dqp (dqc = dqm + dqp), reconstruct.c:157-159 */
+    long double Am = (bp * gm * g0 - gp * b0 * gm) * den;
+    long double Ap = (gp * bm * g0 - gp * b0 * gm) * den;
+    long double Bm = -(ap * gm * g0 - gp * a0 * gm) * den;
+    long double Bp = -(gp * am * g0 - gp * a0 * gm) * den;
+    long double X2L = xlo * xlo - m2, X1L = xlo - m1;
+    long double X2R = xhi * xhi - m2, X1R = xhi - m1;
+    o[MDS_RC_CLM] = (double)(Am * X2L + Bm * X1L);
+    o[MDS_RC_CLP] = (double)(Ap * X2L + Bp * X1L);
+    o[MDS_RC_CRM] = (double)(Am * X2R + Bm * X1R);
+    o[MDS_RC_CRP] = (double)(Ap * X2R + Bp * X1R);
+  }
+}
+
+static int alloc_dir(mds_grid *g, int d)
+{
+  int nz = g->n[d] + 2 * NG;
+  g->alpha[d] = calloc(nz, sizeof(double));
+  g->beta[d] = calloc(nz, sizeof(double));
+  g->Gamma[d] = calloc(nz, sizeof(double));
+  g->rcoef[d] = calloc((size_t)nz * MDS_RC_COUNT, sizeof(double));
+  return (g->alpha[d] && g->beta[d] && g->Gamma[d] && g->rcoef[d]) ? 0 : MDS_ENOMEM;
+}
+
+static int build_cartesian(mds_grid *g)
+{
+  for (int d = 0; d < 3; d++) {
+    if (alloc_dir(g, d)) return MDS_ENOMEM;
+    for (int i = -NG; i < g->n[d] + NG; i++) {
+      double x0 = g->startx[d] + i * g->dx[d];
+      double x1 = x0 + g->dx[d];
+      g->alpha[d][i + NG] = ((x1 * x1 * x1) - (x0 * x0 * x0)) / 3.0;
+      g->beta[d][i + NG] = ((x1 * x1) - (x0 * x0)) / 2.0;
+      g->Gamma[d][i + NG] = g->dx[d];
+    }
+    recon_coeffs(g->alpha[d], g->beta[d], g->Gamma[d], g->startx[d], g->dx[d], g->n[d] + 2 * NG, g->rcoef[d]);
+  }
+  g->volume = 1.0;
+  for (int d = 0; d < g->ndim; d++) g->volume *= g->dx[d];
+  for (int loc = 0; loc < 3; loc++) {
+    g->area[loc] = 0.0;
+    if (loc >= g->ndim) continue;
+    g->area[loc] = 1.0;
+    for (int d = 0; d < g->ndim; d++) g->area[loc] *= g->dx[d];
+    g->area[loc] /= g->dx[loc];
+  }
+  g->zone = NULL;
+  return 0;
+}
+
+static int build_spherical1d(mds_grid *g)
+{
+  if (g->ndim != 1) return MDS_ENOTIMPL;
+  maps m;
+  m.rlog = (g->rcoord == MDS_RCOORD_LOG);
+  m.pxt = 1.2;
+  m.palpha = 4;
+  m.pnorm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m.palpha + 1.0)) * (1.0 / pow(m.pxt, m.palpha))));
+  const double dphi = 2.0 * M_PI, x1lo = -1.0, x1hi = 1.0;
+  int n = g->n[0], nz = n + 2 * NG;
+  for (int d = 0; d < 3; d++)
+    if (alloc_dir(g, d)) return MDS_ENOMEM;
+  /* inactive directions: unit moments so that nothing downstream divides by 0 */
+  for (int d = 1; d < 3; d++)
+    for (int q = 0; q < g->n[d] + 2 * NG; q++) g->Gamma[d][q] = 1.0;
+  for (int i = -NG; i < n + NG; i++) {
+    double x0 = g->startx[0] + i * g->dx[0];
+    double x1 = g->startx[0] + (i + 1) * g->dx[0];
+    g->Gamma[0][i + NG] = romberg(x0, x1, i_r2dr, &m);
+    g->beta[0][i + NG] = romberg(x0, x1, i_xr1, &m);
+    g->alpha[0][i + NG] = romberg(x0, x1, i_xr2, &m);
+  }
+  recon_coeffs(g->alpha[0], g->beta[0], g->Gamma[0], g->startx[0], g->dx[0], nz, g->rcoef[0]);
+
+  g->zone = calloc((size_t)nz * MDS_ZG_COUNT, sizeof(double));
+  if (!g->zone) return MDS_ENOMEM;
+  double sth = romberg(x1lo, x1hi, i_sdth, &m);
+  double vthf = sth;
+  double th_s3 = romberg(x1lo, x1hi, i_sdth3, &m);
+  double th_s3c = romberg(x1lo, x1hi, i_s3dth, &m);
+  double th_sd2 = romberg(x1lo, x1hi, i_sd2th, &m);
+  double th_cd2 = romberg(x1lo, x1hi, i_cdth2, &m);
+  for (int i = -NG; i < n + NG; i++) {
+    double *z = &g->zone[(size_t)(i + NG) * MDS_ZG_COUNT];
+    double xl = g->startx[0] + i * g->dx[0];
+    double xh = g->startx[0] + (i + 1) * g->dx[0];
+    double r2dr = romberg(xl, xh, i_r2dr, &m);
+    z[MDS_ZG_VOLUME] = ((r2dr * sth) * 1) * dphi;
+    z[MDS_ZG_AREA0] = (i >= 0 && i <= n) ? (i_r2dr(xl, &m) * sth) * (1 * dphi) : 0.0;
+    double vrf = r2dr;
+    double r5 = 0.2 * (pow(rx(&m, xh), 5) - pow(rx(&m, xl), 5));
+    double gc[3], gf[3];
+    gc[0] = romberg(xl, xh, i_r2dr3, &m) / vrf;
+    gc[1] = fabs(((r5 / vrf) * th_s3) / vthf);
+    gc[2] = fabs(((r5 / vrf) * th_s3c) / vthf);
+    gf[0] = pow(drx(&m, xl), 2);
+    gf[1] = (pow(rx(&m, xl), 2) * th_s3) / vthf;
+    gf[2] = (pow(rx(&m, xl), 2) * th_s3c) / vthf;
+    for (int d = 0; d < 3; d++) {
+      z[MDS_ZG_GCOV00 + d] = gc[d];
+      z[MDS_ZG_GCOVF0 + d] = gf[d];
+      z[MDS_ZG_GCON0 + d] = (fabs(gc[d]) > 1.0e-50) ? 1.0 / gc[d] : 0.0;
+    }
+    z[MDS_ZG_SCALEF] = sqrt(gf[0]);
+    z[MDS_ZG_SCALE0] = sqrt(gc[0]);
+    if (i >= 0 && i < n) {
+      double vr = romberg(xl, xl + g->dx[0], i_r2dr, &m);
+      double c101 = romberg(xl, xh, i_rdr2, &m) / vr;
+      double c212 = th_cd2 / vthf;
+      z[MDS_ZG_C000] = romberg(xl, xh, i_r2d2r, &m) / vr;
+      z[MDS_ZG_C011] = -(romberg(xl, xh, i_r3, &m) / vr) * (th_s3 / vthf);
+      z[MDS_ZG_C101] = c101;
+      z[MDS_ZG_C110] = c101;
+      z[MDS_ZG_C202] = c101;
+      z[MDS_ZG_C220] = c101;
+      z[MDS_ZG_C212] = c212;
+      z[MDS_ZG_C221] = c212;
+      z[MDS_ZG_C022] = ((-c101) * gc[2]) / gc[0];
+      z[MDS_ZG_C122] = ((-c212) * gc[2]) / gc[1];
+      z[MDS_ZG_C111] = th_sd2 / vthf;
+    }
+  }
+  g->volume = 0.0;
+  g->area[0] = g->area[1] = g->area[2] = 0.0;
+  return 0;
+}
+
+int mds_grid_build(mds_grid *g)
+{
+  if (!g || g->ndim < 1 || g->ndim > 3) return MDS_EINVAL;
+  for (int d = 0; d < 3; d++) {
+    if (d >= g->ndim) { g->n[d] = 1; }
+    if (g->n[d] < 1) return MDS_EINVAL;
+    g->alpha[d] = g->beta[d] = g->Gamma[d] = g->rcoef[d] = NULL;
+  }
+  g->zone = NULL;
+  int rc = (g->geom == MDS_GEOM_SPHERICAL) ? build_spherical1d(g) : build_cartesian(g);
+  if (rc) mds_grid_free(g);
+  return rc;
+}
+
+void mds_grid_free(mds_grid *g)
+{
+  if (!g) return;
+  for (int d = 0; d < 3; d++) {
+    free(g->alpha[d]); free(g->beta[d]); free(g->Gamma[d]); free(g->rcoef[d]);
+    g->alpha[d] = g->beta[d] = g->Gamma[d] = g->rcoef[d] = NULL;
+  }
+  free(g->zone);
+  g->zone = NULL;
+}

@@ -1,0 +1,46 @@
+"""Benchmark configurations (grids, boundaries, initial states).
+
+Each builder returns (grid, problem_dict, prims) where problem_dict is what
+medusa_amd.mds.Context expects and prims is the [5][n3][n2][n1] initial state.
+"""
+import math
+
+from . import mds
+
+
+def sod(n=1024, gam=1.4):
+    """1D Sod shock tube on [0,1], outflow both ends (problems/init_sod.c)."""
+    g = mds.GridBuilder(1, [n], [0.0], [1.0 / n])
+    prob = dict(grid=g, bc_lo=[mds.BC_OUTFLOW], bc_hi=[mds.BC_OUTFLOW], gam=gam)
+    return g, prob, mds.init_sod(g, gam)
+
+
+def kh(n=2048, gam=5.0 / 3.0, seed=0x5EED0002):
+    """2D Kelvin-Helmholtz on the periodic unit square."""
+    g = mds.GridBuilder(2, [n, n], [0.0, 0.0], [1.0 / n, 1.0 / n])
+    per = [mds.BC_PERIODIC, mds.BC_PERIODIC]
+    prob = dict(grid=g, bc_lo=per, bc_hi=per, gam=gam)
+    return g, prob, mds.init_kh(g, gam, seed=seed)
+
+
+def rt(n=512, gam=1.4, seed=0x5EED0003):
+    """3D Rayleigh-Taylor on [-0.25,0.25]^2 x [-0.75,0.75], periodic x,y, reflecting z."""
+    g = mds.GridBuilder(3, [n, n, n], [-0.25, -0.25, -0.75], [0.5 / n, 0.5 / n, 1.5 / n])
+    prob = dict(grid=g, bc_lo=[mds.BC_PERIODIC, mds.BC_PERIODIC, mds.BC_REFLECT],
+                bc_hi=[mds.BC_PERIODIC, mds.BC_PERIODIC, mds.BC_REFLECT], gam=gam, g0=[0.0, 0.0, -0.1])
+    return g, prob, mds.init_rt(g, gam, seed=seed)
+
+
+def sph_riemann(n, gam, rho_in, p_in, r0, frho, fp, rmin=1.0e5, rmax=1.0e9):
+    """One spherical Riemann problem on a log-r grid, reflecting inner and outflow outer boundary."""
+    x0, x1 = math.log(rmin), math.log(rmax)
+    g = mds.GridBuilder(1, [int(n)], [x0], [(x1 - x0) / int(n)], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
+    prob = dict(grid=g, bc_lo=[mds.BC_REFLECT], bc_hi=[mds.BC_OUTFLOW], gam=gam)
+    return g, prob, mds.init_sph_riemann(g, gam, rho_in, p_in, r0, frho, fp)
+
+
+def ensemble(nprob=2048, nmin=256, nmax=8192, seed=0x5EED0004):
+    """The batched ensemble of spherical Riemann problems; returns (params, list of (grid, prob, prims))."""
+    par = mds.ensemble_params(nprob, nmin, nmax, seed)
+    out = [sph_riemann(*row) for row in par]
+    return par, out

@@ -1,0 +1,132 @@
+/*
+ * oracle.h -- CPU oracle for the explicit hydrodynamics sweep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, single-threaded, strict-IEEE
+ * restatement of the reference algorithm (MEDUSA/Fornax: step.c, reconstruct.c,
+ * riemann.c, fluid.c, rad_fluid.c, bc.c, gravity.c, eos/gamma/eos_gamma.c,
+ * metrics/metric_cartesian.c, metrics/metric_spherical_general.c, utils.c).
+ * It is used only by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py, always as the checker -- never as the product path.
+ *
+ * Every function cites the reference file:line it restates.  The arithmetic
+ * follows the reference operation order so that a strict-IEEE build
+ * (-O2 -fno-fast-math -ffp-contract=off) is a bit-level model of the
+ * reference CPU build without -ffast-math.
+ *
+ * Pinning (what checks each function independently of this restatement) is
+ * listed in the header comment of each function and in tests/test_oracle_*.py.
+ */
+#ifndef MDS_ORACLE_H
+#define MDS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NG 2          /* ghost zones, decs.h:61 */
+#define ORC_NSIZE 20      /* pencil row capacity, decs.h:36 */
+
+/* primitive mnemonics, decs.h:641-646 */
+#define ORC_RHO 0
+#define ORC_UU  1
+#define ORC_U1  2
+#define ORC_U2  3
+#define ORC_U3  4
+#define ORC_YE  5
+/* conserved mnemonics, decs.h:650-654 */
+#define ORC_ETOT 1
+
+/* boundary codes, decs.h:64-72 */
+#define ORC_BC_OUTFLOW  0
+#define ORC_BC_REFLECT  1
+#define ORC_BC_EXTRAP   2
+#define ORC_BC_PERIODIC 3
+
+#define ORC_GEOM_CARTESIAN 0
+#define ORC_GEOM_SPHERICAL 1
+
+/* radial map r(x) for spherical grids */
+#define ORC_RCOORD_UNIFORM 0   /* r = |x|       (metric_spherical_general.c:149-152) */
+#define ORC_RCOORD_LOG     1   /* r = exp(x)    (metrics/metric_spherical_logr.c:3-38) */
+
+/* Geometry of one zone, restating zone_geom (decs.h:144-153) for NDIM<=3.
+ * Index 0 of gcov/scale is the zone centre, 1+d the lower face in direction d. */
+typedef struct {
+  double area[3];
+  double conn[3][3][3];
+  double gcov[4][3];
+  double gcon[3];
+  double scale[4][3];
+  double volume;
+} orc_zone_geom;
+
+typedef struct {
+  int ndim;
+  int n[3];
+  int geom;            /* ORC_GEOM_* */
+  int rcoord;          /* ORC_RCOORD_* (spherical only) */
+  double startx[3];
+  double dx[3];
+  int bc_lo[3], bc_hi[3];
+  int nvars;           /* 5 (+ncomp) */
+  int interp_order;    /* hydro_interp_order, io.c:5054 (3 = para_recon) */
+  double gam;          /* gamma-law index */
+  double cfl;
+  double dt_init_safety, dt_grow_safety, dtmin, dtmax, initial_dt;
+  double g0[3];        /* fixed gravity (contravariant), gravity.c:46-64; all zero = none */
+  int use_fixed_grav;
+} orc_config;
+
+typedef struct orc_sim orc_sim;
+
+/* ---- pointwise kernels (reconstruct.c, riemann.c, fluid.c) ---- */
+void orc_para_recon(const double *a, double *al, double *ar,
+                    const double *alpha, const double *beta, const double *Gamma,
+                    const double *x, int istart, int istop);
+void orc_lin_recon(const double *a, double *al, double *ar,
+                   const double *beta, const double *Gamma,
+                   const double *x, int istart, int istop);
+/* face geometry needed by a Riemann solve: area[dir], gcov[1+dir][*], scale[1+dir][dir] */
+int orc_riemann_hllc(int nhydro, int nvars, const double *pl, const double *pr,
+                     const orc_zone_geom *g, int dir,
+                     double *flux, double *sig_speed, double *vriemann);
+int orc_riemann_hlle(int nhydro, int nvars, const double *pl, const double *pr,
+                     const orc_zone_geom *g, int dir,
+                     double *flux, double *sig_speed, double *vriemann);
+void orc_p_to_u(const double *p, double *u, const double *gcov, int nhydro);
+void orc_u_to_p(double *u, double *p, const double *gcon, int nhydro);
+void orc_geom_src(const double *p, double press, double *src,
+                  const orc_zone_geom *g, int nvars, int geom);
+double orc_dt_source_control(const double *p, const double *u, const double *s);
+
+/* ---- geometry (metric_cartesian.c, metric_spherical_general.c, utils.c) ---- */
+double orc_romb(double x0, double x1, double (*f)(double, const void *), const void *ctx);
+
+/* ---- simulation ---- */
+orc_sim *orc_create(const orc_config *cfg);
+void orc_destroy(orc_sim *s);
+/* prims: SoA over interior zones, [nvars][n3][n2][n1] with i fastest */
+void orc_set_prims(orc_sim *s, const double *prims);
+void orc_get_prims(const orc_sim *s, double *prims);
+/* geometry export for cross-checks: per zone incl. ghosts, natural order */
+int orc_zone_count_all(const orc_sim *s);
+const orc_zone_geom *orc_geometry(const orc_sim *s);
+/* volume moments per direction, length n[d]+2*NG (index 0 = zone -NG) */
+const double *orc_moment(const orc_sim *s, int which /*0 alpha,1 beta,2 Gamma*/, int dir);
+void orc_update_eos_all(orc_sim *s);           /* main.c:181 */
+double orc_estimate_dt(orc_sim *s);             /* step.c:163-203 */
+void orc_step(orc_sim *s);                      /* step.c:28-129 */
+/* main.c:206-313 loop: runs until t>=tmax or max_steps; returns steps taken */
+int orc_run(orc_sim *s, double tmax, int max_steps);
+double orc_time(const orc_sim *s);
+double orc_last_dt(const orc_sim *s);
+int orc_istep(const orc_sim *s);
+long orc_shock_face_count(const orc_sim *s);    /* radial faces HLLC tagged as shocks */
+long orc_hlle_face_count(const orc_sim *s);     /* transverse faces solved with HLLE */
+/* compensated (Neumaier) sums of conserved mass and total energy over interior zones */
+void orc_conserved_totals(const orc_sim *s, double out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

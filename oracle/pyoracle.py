@@ -1,0 +1,219 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker.  Builds oracle/liboracle.so on first use if it is
+missing (gcc, strict IEEE) so that the CPU test suite needs no manual step.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+RHO, UU, U1, U2, U3, YE = 0, 1, 2, 3, 4, 5
+BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC = 0, 1, 2, 3
+GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
+RCOORD_UNIFORM, RCOORD_LOG = 0, 1
+
+
+class ZoneGeom(ctypes.Structure):
+    _fields_ = [("area", ctypes.c_double * 3),
+                ("conn", ctypes.c_double * 27),
+                ("gcov", ctypes.c_double * 12),
+                ("gcon", ctypes.c_double * 3),
+                ("scale", ctypes.c_double * 12),
+                ("volume", ctypes.c_double)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int),
+                ("n", ctypes.c_int * 3),
+                ("geom", ctypes.c_int),
+                ("rcoord", ctypes.c_int),
+                ("startx", ctypes.c_double * 3),
+                ("dx", ctypes.c_double * 3),
+                ("bc_lo", ctypes.c_int * 3),
+                ("bc_hi", ctypes.c_int * 3),
+                ("nvars", ctypes.c_int),
+                ("interp_order", ctypes.c_int),
+                ("gam", ctypes.c_double),
+                ("cfl", ctypes.c_double),
+                ("dt_init_safety", ctypes.c_double),
+                ("dt_grow_safety", ctypes.c_double),
+                ("dtmin", ctypes.c_double),
+                ("dtmax", ctypes.c_double),
+                ("initial_dt", ctypes.c_double),
+                ("g0", ctypes.c_double * 3),
+                ("use_fixed_grav", ctypes.c_int)]
+
+
+def _load(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    path = os.path.join(HERE, name)
+    srcs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith((".c", ".h"))]
+    if (not os.path.exists(path)) or any(os.path.getmtime(s) > os.path.getmtime(path) for s in srcs):
+        subprocess.check_call(["make", "-s", "-C", HERE, name])
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.orc_create.restype = ctypes.c_void_p
+    lib.orc_create.argtypes = [ctypes.POINTER(Config)]
+    lib.orc_destroy.argtypes = [ctypes.c_void_p]
+    lib.orc_set_prims.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_get_prims.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    lib.orc_run.restype = ctypes.c_int
+    lib.orc_step.argtypes = [ctypes.c_void_p]
+    lib.orc_estimate_dt.argtypes = [ctypes.c_void_p]
+    lib.orc_estimate_dt.restype = ctypes.c_double
+    lib.orc_update_eos_all.argtypes = [ctypes.c_void_p]
+    for fn in ("orc_time", "orc_last_dt"):
+        getattr(lib, fn).argtypes = [ctypes.c_void_p]
+        getattr(lib, fn).restype = ctypes.c_double
+    lib.orc_istep.argtypes = [ctypes.c_void_p]
+    lib.orc_istep.restype = ctypes.c_int
+    for fn in ("orc_shock_face_count", "orc_hlle_face_count"):
+        getattr(lib, fn).argtypes = [ctypes.c_void_p]
+        getattr(lib, fn).restype = ctypes.c_long
+    lib.orc_zone_count_all.argtypes = [ctypes.c_void_p]
+    lib.orc_zone_count_all.restype = ctypes.c_int
+    lib.orc_geometry.argtypes = [ctypes.c_void_p]
+    lib.orc_geometry.restype = ctypes.POINTER(ZoneGeom)
+    lib.orc_moment.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.orc_moment.restype = dp
+    lib.orc_conserved_totals.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_para_recon.argtypes = [dp, dp, dp, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int]
+    lib.orc_lin_recon.argtypes = [dp, dp, dp, dp, dp, dp, ctypes.c_int, ctypes.c_int]
+    for fn in ("orc_riemann_hllc", "orc_riemann_hlle"):
+        f = getattr(lib, fn)
+        f.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.POINTER(ZoneGeom), ctypes.c_int, dp, dp, dp]
+        f.restype = ctypes.c_int
+    lib.orc_p_to_u.argtypes = [dp, dp, dp, ctypes.c_int]
+    lib.orc_u_to_p.argtypes = [dp, dp, dp, ctypes.c_int]
+    lib.orc_geom_src.argtypes = [dp, ctypes.c_double, dp, ctypes.POINTER(ZoneGeom), ctypes.c_int, ctypes.c_int]
+    lib.orc_dt_source_control.argtypes = [dp, dp, dp]
+    lib.orc_dt_source_control.restype = ctypes.c_double
+    return lib
+
+
+_LIBS = {}
+
+
+def lib(fast=False):
+    if fast not in _LIBS:
+        _LIBS[fast] = _load(fast)
+    return _LIBS[fast]
+
+
+def ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
+                geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, interp_order=3,
+                g0=(0.0, 0.0, 0.0), dt_init_safety=0.1, dt_grow_safety=1.1,
+                dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100):
+    c = Config()
+    c.ndim = ndim
+    n = list(n) + [1] * (3 - len(n))
+    for d in range(3):
+        c.n[d] = int(n[d])
+        c.startx[d] = float(startx[d]) if d < len(startx) else 0.0
+        c.dx[d] = float(dx[d]) if d < len(dx) else 1.0
+        c.bc_lo[d] = int(bc_lo[d]) if d < len(bc_lo) else 0
+        c.bc_hi[d] = int(bc_hi[d]) if d < len(bc_hi) else 0
+        c.g0[d] = float(g0[d])
+    c.geom = geom
+    c.rcoord = rcoord
+    c.nvars = nvars
+    c.interp_order = interp_order
+    c.gam = gam
+    c.cfl = cfl
+    c.dt_init_safety = dt_init_safety
+    c.dt_grow_safety = dt_grow_safety
+    c.dtmin = dtmin
+    c.dtmax = dtmax
+    c.initial_dt = initial_dt
+    c.use_fixed_grav = int(any(v != 0.0 for v in g0))
+    return c
+
+
+class Sim:
+    """One oracle simulation (a restatement of the reference's globals + main loop)."""
+
+    def __init__(self, cfg, fast=False):
+        self.lib = lib(fast)
+        self.cfg = cfg
+        self.h = self.lib.orc_create(ctypes.byref(cfg))
+        self.n = [cfg.n[d] if d < cfg.ndim else 1 for d in range(3)]
+        self.nvars = cfg.nvars
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.nvars, self.n[2], self.n[1], self.n[0])
+
+    def set_prims(self, prims):
+        a = np.ascontiguousarray(prims, dtype=np.float64).reshape(self.shape)
+        self.lib.orc_set_prims(self.h, ptr(a))
+
+    def get_prims(self):
+        a = np.zeros(self.shape, dtype=np.float64)
+        self.lib.orc_get_prims(self.h, ptr(a))
+        return a
+
+    def run(self, tmax=1.0e300, max_steps=2 ** 31 - 1):
+        return self.lib.orc_run(self.h, tmax, max_steps)
+
+    @property
+    def t(self):
+        return self.lib.orc_time(self.h)
+
+    @property
+    def dt(self):
+        return self.lib.orc_last_dt(self.h)
+
+    @property
+    def istep(self):
+        return self.lib.orc_istep(self.h)
+
+    def shock_faces(self):
+        return self.lib.orc_shock_face_count(self.h)
+
+    def hlle_faces(self):
+        return self.lib.orc_hlle_face_count(self.h)
+
+    def totals(self):
+        out = np.zeros(2)
+        self.lib.orc_conserved_totals(self.h, ptr(out))
+        return out
+
+    def geometry(self):
+        nall = self.lib.orc_zone_count_all(self.h)
+        g = self.lib.orc_geometry(self.h)
+        arr = np.ctypeslib.as_array(ctypes.cast(g, ctypes.POINTER(ctypes.c_double)),
+                                    shape=(nall, ctypes.sizeof(ZoneGeom) // 8))
+        return arr.copy()
+
+    def moments(self, d):
+        ng = self.cfg.n[d] + 4 if d < self.cfg.ndim else 5
+        out = []
+        for q in range(3):
+            p = self.lib.orc_moment(self.h, q, d)
+            out.append(np.ctypeslib.as_array(p, shape=(ng,)).copy())
+        return out
+
+
+# geometry record field offsets (in doubles) inside ZoneGeom
+GEOM_AREA = 0
+GEOM_CONN = 3
+GEOM_GCOV = 30
+GEOM_GCON = 42
+GEOM_SCALE = 45
+GEOM_VOLUME = 57

@@ -1,0 +1,45 @@
+"""Helpers shared by the parity tests: oracle configs mirroring a device
+problem, and the reference's face-weighted relative L1 convention
+(tst/regression/scripts/utils/comparison.py:9-27: sum |v| * face width)."""
+import numpy as np
+
+from oracle import pyoracle as po
+from medusa_amd import mds
+
+
+def oracle_for(prob, rp_cfl=0.5, **kw):
+    g = prob["grid"]
+    nd = g.ndim
+    cfg = po.make_config(
+        nd, g.n[:nd], [g.g.startx[d] for d in range(nd)], [g.g.dx[d] for d in range(nd)],
+        list(prob["bc_lo"]), list(prob["bc_hi"]), prob["gam"], cfl=rp_cfl,
+        geom=po.GEOM_SPHERICAL if g.g.geom == mds.GEOM_SPHERICAL else po.GEOM_CARTESIAN,
+        rcoord=g.g.rcoord, g0=tuple(prob.get("g0", (0.0, 0.0, 0.0))), **kw)
+    return po.Sim(cfg)
+
+
+def l1rel(a, b, w=None):
+    """face-weighted relative L1 of a against b (w: zone widths, default uniform)."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    if w is None:
+        w = np.ones_like(b)
+    w = np.broadcast_to(np.asarray(w, dtype=np.float64), a.shape).ravel() if np.ndim(w) else w
+    den = np.sum(np.abs(b) * w)
+    num = np.sum(np.abs(a - b) * w)
+    return num / den if den > 0 else num
+
+
+def pressure(prims, gam):
+    return (gam - 1.0) * prims[1]
+
+
+def compare(dev, ref, gam, w=None):
+    """Relative L1 of density, pressure and speed between two prim arrays."""
+    sp_dev = np.sqrt(np.sum(dev[2:5] ** 2, axis=0))
+    sp_ref = np.sqrt(np.sum(ref[2:5] ** 2, axis=0))
+    return {
+        "rho": l1rel(dev[0], ref[0], w),
+        "p": l1rel(pressure(dev, gam), pressure(ref, gam), w),
+        "v": l1rel(sp_dev, sp_ref, w),
+    }

@@ -1,0 +1,250 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+FP64 path: bit-level restatement of the reference order, so the bar is the
+north-star tolerance of ~1e-12 relative L1 (FMA and ordering only); in
+practice most results are bit-identical.  FP32-bulk path: 1e-5 relative L1 in
+density against the FP64 oracle.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from medusa_amd import mds, problems
+from parity_util import oracle_for, compare, l1rel
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1.0e-12
+FP32_RHO_TOL = 1.0e-5
+
+
+def say(*a):
+    print("[gpu-parity]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_recon(a3, mom9, x2):
+    L = po.lib()
+    n = a3.shape[0]
+    al = np.zeros(n)
+    ar = np.zeros(n)
+    for c in range(n):
+        a = np.array([a3[c, 0], a3[c, 1], a3[c, 2], 0.0])
+        alpha = np.array([mom9[c, 0], mom9[c, 3], mom9[c, 6], 1.0])
+        beta = np.array([mom9[c, 1], mom9[c, 4], mom9[c, 7], 1.0])
+        gam = np.array([mom9[c, 2], mom9[c, 5], mom9[c, 8], 1.0])
+        x = np.array([0.0, x2[c, 0], x2[c, 1], 0.0])
+        l = np.zeros(4)
+        r = np.zeros(4)
+        L.orc_para_recon(po.ptr(a), po.ptr(l), po.ptr(r), po.ptr(alpha), po.ptr(beta), po.ptr(gam), po.ptr(x), -1, -1)
+        al[c] = l[1]
+        ar[c] = r[1]
+    return al, ar
+
+
+def _pencils(grid, d, rng, ncell):
+    nz = grid.n[d] + 4
+    mom = grid.moments(d)
+    rc = grid.rcoef(d)
+    idx = rng.integers(2, nz - 2, size=ncell)
+    m9 = np.stack([np.stack([mom[0][idx + k], mom[1][idx + k], mom[2][idx + k]], 1) for k in (-1, 0, 1)], 1)
+    m9 = m9.reshape(ncell, 9)
+    x0 = grid.g.startx[d] + (idx - 2) * grid.g.dx[d]
+    x1 = grid.g.startx[d] + (idx - 1) * grid.g.dx[d]
+    x2 = np.stack([x0, x1], 1)
+    kind = rng.integers(0, 3, size=ncell)
+    base = rng.uniform(0.5, 2.0, size=ncell)
+    slope = rng.uniform(-0.3, 0.3, size=ncell)
+    curv = rng.uniform(-0.3, 0.3, size=ncell)
+    k = np.array([-1.0, 0.0, 1.0])
+    a3 = base[:, None] + slope[:, None] * k[None, :] + curv[:, None] * k[None, :] ** 2
+    noisy = kind == 2
+    a3[noisy] = rng.uniform(0.5, 2.0, size=(noisy.sum(), 3))
+    return a3, m9, x2, rc[idx]
+
+
+def test_recon_points_fp64_matches_oracle():
+    rng = np.random.default_rng(7)
+    for label, g, d in (("cartesian x~[0,1] n=2048", mds.GridBuilder(1, [2048], [0.0], [1.0 / 2048]), 0),
+                        ("cartesian z~[-0.75,0.75] n=512", mds.GridBuilder(1, [512], [-0.75], [1.5 / 512]), 0),
+                        ("log-r spherical n=700",
+                         problems.sph_riemann(700, 1.4, 1e9, 1e27, 1e7, 10, 10)[0], 0)):
+        a3, m9, x2, _ = _pencils(g, d, rng, 3000)
+        say("recon FP64", label)
+        al, ar = mds.recon_points(mds.FP64, a3, mom=m9, x=x2)
+        ol, orr = _oracle_recon(a3, m9, x2)
+        scale = np.abs(a3).max(1)
+        err = max(np.max(np.abs(al - ol) / scale), np.max(np.abs(ar - orr) / scale))
+        exact = np.mean((al == ol) & (ar == orr))
+        say(f"  max rel diff {err:.3e}, bit-identical fraction {exact:.4f}")
+        assert err <= 1e-13
+        assert exact > 0.99
+
+
+def test_recon_points_fp32_close_to_oracle():
+    rng = np.random.default_rng(11)
+    g = mds.GridBuilder(1, [2048], [0.0], [1.0 / 2048])
+    a3, m9, x2, rc = _pencils(g, 0, rng, 3000)
+    say("recon FP32 cartesian n=2048")
+    al, ar = mds.recon_points(mds.FP32_BULK, a3, rcoef=rc)
+    ol, orr = _oracle_recon(a3, m9, x2)
+    scale = np.abs(a3).max(1)
+    err = np.maximum(np.abs(al - ol), np.abs(ar - orr)) / scale
+    # FP32 rounding of the state (6e-8) amplified by the limiter branches
+    say(f"  median {np.median(err):.3e}  99.9% {np.quantile(err, 0.999):.3e}")
+    assert np.quantile(err, 0.999) < 1e-5
+
+
+def _random_states(rng, n, wide=False):
+    pl = np.zeros((n, 7))
+    pr = np.zeros((n, 7))
+    for p in (pl, pr):
+        p[:, 0] = 10 ** rng.uniform(-1, 1, n) if wide else rng.uniform(0.1, 2.0, n)
+        p[:, 5] = 10 ** rng.uniform(-1, 1, n) if wide else rng.uniform(0.1, 2.0, n)
+        p[:, 2:5] = rng.uniform(-2.0, 2.0, (n, 3))
+        p[:, 6] = 1.4
+        p[:, 1] = p[:, 5] / (p[:, 6] - 1.0)
+    return pl, pr
+
+
+def _oracle_riemann(solver, pl, pr, fg, d):
+    L = po.lib()
+    fn = L.orc_riemann_hllc if solver == 0 else L.orc_riemann_hlle
+    n = pl.shape[0]
+    flux = np.zeros((n, 5))
+    sig = np.zeros(n)
+    sh = np.zeros(n, dtype=int)
+    g = po.ZoneGeom()
+    for f in range(n):
+        g.area[d] = fg[f, 0]
+        for q in range(3):
+            g.gcov[(1 + d) * 3 + q] = fg[f, 2 + q]
+        g.scale[(1 + d) * 3 + d] = fg[f, 1]
+        a = np.ascontiguousarray(pl[f])
+        b = np.ascontiguousarray(pr[f])
+        fl = np.zeros(5)
+        s = np.zeros(1)
+        vr = np.zeros(3)
+        sh[f] = fn(5, 5, po.ptr(a), po.ptr(b), ctypes.byref(g), d, po.ptr(fl), po.ptr(s), po.ptr(vr))
+        flux[f] = fl
+        sig[f] = s[0]
+    return flux, sig, sh
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_riemann_points_fp64_matches_oracle(solver):
+    rng = np.random.default_rng(3 + solver)
+    n = 4000
+    pl, pr = _random_states(rng, n, wide=True)
+    fg = np.zeros((n, 5))
+    fg[:, 0] = rng.uniform(0.5, 2.0, n)
+    fg[:, 1] = rng.uniform(0.5, 2.0, n)
+    fg[:, 2:5] = rng.uniform(0.5, 2.0, (n, 3))
+    fg[: n // 4, 1] = 1.0
+    fg[: n // 4, 2:5] = 1.0
+    for d in range(3):
+        say(f"riemann FP64 solver={'HLLC' if solver == 0 else 'HLLE'} dir={d}")
+        f_gpu, s_gpu, sh_gpu = mds.riemann_points(mds.FP64, solver, pl, pr, fg, d)
+        f_orc, s_orc, sh_orc = _oracle_riemann(solver, pl, pr, fg, d)
+        scale = np.abs(f_orc).max(1, keepdims=True) + 1e-300
+        ferr = np.max(np.abs(f_gpu - f_orc) / scale)
+        serr = np.max(np.abs(s_gpu - s_orc) / np.abs(s_orc))
+        exact = np.mean(np.all(f_gpu == f_orc, axis=1))
+        say(f"  flux max rel {ferr:.3e} sig max rel {serr:.3e} bit-identical {exact:.4f}")
+        assert ferr < 1e-12 and serr < 1e-12
+        if solver == 0:
+            assert np.array_equal(sh_gpu, sh_orc)
+
+
+def _run_pair(prob, prims, precision, nsteps=None, tmax=None, cfl=0.5):
+    """Run the device path and the oracle on the same problem; return both final states."""
+    rp = mds.run_params(cfl=cfl, tmax=tmax if tmax is not None else 1.0e300)
+    ctx = mds.Context([prob], precision=precision, rp=rp)
+    ctx.upload(prims)
+    ctx.update_eos()
+    sim = oracle_for(prob, rp_cfl=cfl)
+    sim.set_prims(prims)
+    if tmax is not None:
+        ctx.run_to()
+        sim.run(tmax)
+    else:
+        ctx.run(nsteps)
+        sim.run(max_steps=nsteps)
+    return ctx, sim
+
+
+def test_sod_c1_fp64_full_run():
+    g, prob, prims = problems.sod(1024)
+    say("C1 Sod 1024 FP64 to t=0.2")
+    ctx, sim = _run_pair(prob, prims, mds.FP64, tmax=0.2)
+    t, dt, ist = ctx.clock()
+    dev = ctx.download()
+    ref = sim.get_prims()
+    err = compare(dev, ref, 1.4)
+    say(f"  device steps {ist} t={t!r}; oracle steps {sim.istep} t={sim.t!r}; L1 {err}")
+    assert ist == sim.istep
+    assert abs(t - sim.t) <= 1e-14
+    for k, v in err.items():
+        assert v <= FP64_TOL, (k, v)
+    m0 = np.sum(prims[0]) / 1024
+    m1 = ctx.totals()[0]
+    assert abs(m1 - m0) / m0 < 1e-14
+
+
+def test_kh_fp64_small_parity():
+    g, prob, prims = problems.kh(128)
+    say("KH 128^2 FP64 30 steps")
+    ctx, sim = _run_pair(prob, prims, mds.FP64, nsteps=30)
+    dev = ctx.download()
+    ref = sim.get_prims()
+    err = compare(dev, ref, 5.0 / 3.0)
+    t, dt, ist = ctx.clock()
+    say(f"  L1 {err}; device t={t!r} oracle t={sim.t!r}; HLLE faces {ctx.counters()} vs oracle {sim.hlle_faces()}")
+    assert abs(t - sim.t) <= 1e-15
+    for k, v in err.items():
+        assert v <= FP64_TOL, (k, v)
+    assert ctx.counters()[1] == sim.hlle_faces()
+
+
+def test_kh_fp32_small_within_tolerance():
+    g, prob, prims = problems.kh(128)
+    say("KH 128^2 FP32-bulk 30 steps")
+    ctx, sim = _run_pair(prob, prims, mds.FP32_BULK, nsteps=30)
+    dev = ctx.download()
+    ref = sim.get_prims()
+    err = compare(dev, ref, 5.0 / 3.0)
+    say(f"  L1 {err}")
+    assert err["rho"] <= FP32_RHO_TOL
+
+
+def test_sph_riemann_fp64_parity():
+    par, ens = problems.ensemble(3, 256, 600)
+    for k, (g, prob, prims) in enumerate(ens):
+        say(f"spherical Riemann problem {k} n={g.n[0]} FP64 40 steps")
+        ctx, sim = _run_pair(prob, prims, mds.FP64, nsteps=40)
+        dev = ctx.download()
+        ref = sim.get_prims()
+        w = g.zone()[2:-2, 0]
+        err = compare(dev, ref, prob["gam"], w=w)
+        say(f"  L1 {err}")
+        for kk, v in err.items():
+            assert v <= FP64_TOL, (kk, v)
+
+
+def test_graph_replay_matches_eager():
+    g, prob, prims = problems.kh(96)
+    say("KH 96^2 FP64 eager vs graph, 12 steps")
+    outs = []
+    for graph in (False, True):
+        ctx = mds.Context([prob], precision=mds.FP64, rp=mds.run_params(cfl=0.5))
+        ctx.upload(prims)
+        ctx.update_eos()
+        for _ in range(3):
+            ctx.run(4, graph=graph)
+            ctx.sync()
+        outs.append((ctx.download(), ctx.clock()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
