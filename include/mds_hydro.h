@@ -173,6 +173,31 @@ int mds_run(mds_ctx *ctx, int nsteps);
 int mds_run_graph(mds_ctx *ctx, int nsteps);
 int mds_sync(mds_ctx *ctx);
 
+/* Launch-shape parameters of the sweep kernels (no counterpart in the
+ * reference, whose OpenMP runtime picks launch shapes).  Chosen by
+ * mds_autotune() or set explicitly; every value gives identical results. */
+#define MDS_TILE_ROWS2D  0       /* rows marched by one wavefront in the 2D sweep */
+int mds_set_tile(mds_ctx *ctx, int param, int value);
+int mds_get_tile(mds_ctx *ctx, int param, int *value);
+
+/* Time each candidate launch shape on the context's current state for
+ * nsteps steps and keep the fastest; the state, clocks and counters are
+ * restored afterwards, so the simulation is unaffected.  best_ms (optional)
+ * receives the best time per step. */
+int mds_autotune(mds_ctx *ctx, int nsteps, double *best_ms);
+
+/* Measurement variant of mds_run (no counterpart in the reference, which
+ * times with host wall-clock timers, timer.c:34-48): enqueues nsteps steps
+ * eagerly with HIP events between the kernels on the context stream and
+ * waits.  ms[0] = elapsed time of the whole region, ms[1] = summed duration
+ * of the first-stage sweep kernels, ms[2] = of the second-stage sweep kernels. */
+int mds_run_timed(mds_ctx *ctx, int nsteps, double ms[3]);
+
+/* Device copy kernel over nbytes (rounded down to 16 B) repeated reps times;
+ * returns the mean duration in ms.  Used to calibrate HBM counters and to
+ * measure the achievable copy bandwidth on the device. */
+int mds_copy_probe(long long nbytes, int reps, double *ms);
+
 /* Clock of a problem: time, last dt, steps taken (reference globals t, dt, istep). */
 int mds_get_clock(mds_ctx *ctx, int prob, double *t, double *dt, int *istep);
 

@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <vector>
 #include <new>
 #include <algorithm>
@@ -41,7 +42,7 @@ struct mds_ctx {
   unsigned long long *dcount = nullptr;
   long long *dzstart = nullptr;
   std::vector<void *> geo_allocs;
-  int chunk2d = 64;
+  int chunk2d = 24;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   char err[512] = {0};
@@ -311,6 +312,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     }
   }
   c->nzone = c->zstart[nprob];
+  if (const char *e = getenv("MDS_TILE_ROWS2D")) c->chunk2d = atoi(e) > 0 ? atoi(e) : c->chunk2d;
   size_t rs = rsize(c);
   for (int k = 0; k < 5; k++) {
     HIPCHK(c, hipMalloc(&c->p[k], (size_t)c->nzone * rs));
@@ -442,12 +444,9 @@ extern "C" int mds_update_eos(mds_ctx *c)
   return MDS_OK;
 }
 
-static int enqueue_step(mds_ctx *c)
+// one RK stage of every problem: the fused sweep kernel
+static int enqueue_stage(mds_ctx *c, int stage, int fp32)
 {
-  int fp32 = c->precision == MDS_FP32_BULK;
-  hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
-                     c->rp);
-  HIPCHK(c, hipGetLastError());
   if (c->ndim == 1) {
     Sweep1DArgs a;
     a.seg = c->dseg;
@@ -456,14 +455,12 @@ static int enqueue_step(mds_ctx *c)
     a.clock = c->dclock;
     a.counters = c->dcount;
     a.rp = c->rp;
-    for (int stage = 0; stage < 2; stage++) {
-      if (fp32) {
-        StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
-        HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
-      } else {
-        StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-        HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
-      }
+    if (fp32) {
+      StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+      HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
+    } else {
+      StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+      HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
     }
   } else {
     Sweep2DArgs a;
@@ -474,15 +471,27 @@ static int enqueue_step(mds_ctx *c)
     a.chunk = c->chunk2d;
     a.nstrip = (c->hprob[0].n[0] + MDS_SEG1D - 1) / MDS_SEG1D;
     a.nchunk = (c->hprob[0].n[1] + a.chunk - 1) / a.chunk;
-    for (int stage = 0; stage < 2; stage++) {
-      if (fp32) {
-        StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
-        HIPCHK(c, mds_launch_sweep2d(1, c->nv, &a, &sp, &sh, stage, c->st));
-      } else {
-        StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-        HIPCHK(c, mds_launch_sweep2d(0, c->nv, &a, &sp, &sh, stage, c->st));
-      }
+    if (fp32) {
+      StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+      HIPCHK(c, mds_launch_sweep2d(1, c->nv, &a, &sp, &sh, stage, c->st));
+    } else {
+      StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+      HIPCHK(c, mds_launch_sweep2d(0, c->nv, &a, &sp, &sh, stage, c->st));
     }
+  }
+  return MDS_OK;
+}
+
+// estimate_dt + dt policy, then step(): two RK stages (step.c:28-129)
+static int enqueue_step(mds_ctx *c)
+{
+  int fp32 = c->precision == MDS_FP32_BULK;
+  hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
+                     c->rp);
+  HIPCHK(c, hipGetLastError());
+  for (int stage = 0; stage < 2; stage++) {
+    int rc = enqueue_stage(c, stage, fp32);
+    if (rc) return rc;
   }
   return MDS_OK;
 }
@@ -512,6 +521,185 @@ extern "C" int mds_run_graph(mds_ctx *c, int nsteps)
   }
   for (int s = 0; s < nsteps; s++) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
   return MDS_OK;
+}
+
+extern "C" int mds_run_timed(mds_ctx *c, int nsteps, double ms[3])
+{
+  if (!c || nsteps < 1 || !ms) return MDS_EINVAL;
+  std::vector<hipEvent_t> ev(3 * (size_t)nsteps + 1, nullptr);
+  int rc = MDS_OK;
+  for (auto &e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) { rc = set_err(c, MDS_EDEVICE, "hipEventCreate failed"); break; }
+  }
+  if (rc == MDS_OK) {
+    int fp32 = c->precision == MDS_FP32_BULK;
+    hipError_t e = hipEventRecord(ev[0], c->st);
+    for (int s = 0; s < nsteps && e == hipSuccess && rc == MDS_OK; s++) {
+      hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
+                         c->rp);
+      for (int stage = 0; stage < 2 && e == hipSuccess; stage++) {
+        e = hipEventRecord(ev[3 * s + 1 + stage], c->st);
+        if (e != hipSuccess) break;
+        rc = enqueue_stage(c, stage, fp32);
+        if (rc) break;
+      }
+      if (e == hipSuccess && rc == MDS_OK) e = hipEventRecord(ev[3 * s + 3], c->st);
+    }
+    if (e == hipSuccess && rc == MDS_OK) e = hipEventSynchronize(ev[3 * (size_t)nsteps]);
+    if (e != hipSuccess && rc == MDS_OK) rc = set_err(c, MDS_EDEVICE, "timed run: %s", hipGetErrorString(e));
+    if (rc == MDS_OK) {
+      float f = 0.0f;
+      double tot[3] = {0.0, 0.0, 0.0};
+      hipEventElapsedTime(&f, ev[0], ev[3 * (size_t)nsteps]);
+      tot[0] = f;
+      for (int s = 0; s < nsteps; s++) {
+        hipEventElapsedTime(&f, ev[3 * s + 1], ev[3 * s + 2]); tot[1] += f;
+        hipEventElapsedTime(&f, ev[3 * s + 2], ev[3 * s + 3]); tot[2] += f;
+      }
+      ms[0] = tot[0]; ms[1] = tot[1]; ms[2] = tot[2];
+    }
+  }
+  for (auto &e : ev) if (e) hipEventDestroy(e);
+  return rc;
+}
+
+__global__ void copy_probe_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n)
+{
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int mds_copy_probe(long long nbytes, int reps, double *ms)
+{
+  if (nbytes < 16 || reps < 1 || !ms) return MDS_EINVAL;
+  long long n = nbytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, n * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, n * 16);
+  if (e == hipSuccess) e = hipMemset(a, 0, n * 16);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  dim3 blk(256), grd((unsigned)std::min<long long>((n + 255) / 256, (long long)ncu * 32));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(copy_probe_kernel, grd, blk, 0, 0, a, b, n);   // warm up
+    e = hipEventRecord(e0, 0);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; r++) hipLaunchKernelGGL(copy_probe_kernel, grd, blk, 0, 0, a, b, n);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float f = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+  if (e == hipSuccess) *ms = f / reps;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}
+
+extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
+{
+  if (!c) return MDS_EINVAL;
+  if (param == MDS_TILE_ROWS2D) {
+    if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "rows per wavefront out of range: %d", value);
+    if (value != c->chunk2d) {
+      c->chunk2d = value;
+      if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+      if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    }
+    return MDS_OK;
+  }
+  return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
+}
+
+extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
+{
+  if (!c || !value) return MDS_EINVAL;
+  if (param == MDS_TILE_ROWS2D) { *value = c->chunk2d; return MDS_OK; }
+  return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
+}
+
+// save/restore of the evolving device state around autotuning trials
+struct Snapshot {
+  std::vector<void *> buf;
+  void *clock = nullptr, *count = nullptr, *flags = nullptr;
+};
+
+static void snap_free(Snapshot &s)
+{
+  for (void *b : s.buf) hipFree(b);
+  s.buf.clear();
+  if (s.clock) hipFree(s.clock);
+  if (s.count) hipFree(s.count);
+  if (s.flags) hipFree(s.flags);
+  s.clock = s.count = s.flags = nullptr;
+}
+
+static int snap_alloc(mds_ctx *c, Snapshot &s)
+{
+  size_t bytes = (size_t)c->nzone * rsize(c);
+  for (int k = 0; k < 10; k++) {
+    void *b = nullptr;
+    HIPCHK(c, hipMalloc(&b, bytes));
+    s.buf.push_back(b);
+  }
+  HIPCHK(c, hipMalloc(&s.clock, sizeof(DevClock) * c->nprob));
+  HIPCHK(c, hipMalloc(&s.count, 4 * sizeof(unsigned long long)));
+  if (c->dflags) HIPCHK(c, hipMalloc(&s.flags, (size_t)c->nzone));
+  return MDS_OK;
+}
+
+// save = true: context -> snapshot; false: snapshot -> context
+static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
+{
+  size_t bytes = (size_t)c->nzone * rsize(c);
+  for (int k = 0; k < 5; k++) {
+    void *dst = save ? s.buf[k] : c->p[k], *src = save ? c->p[k] : s.buf[k];
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+    dst = save ? s.buf[5 + k] : c->ph[k];
+    src = save ? c->ph[k] : s.buf[5 + k];
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(save ? s.clock : (void *)c->dclock, save ? (void *)c->dclock : s.clock,
+                           sizeof(DevClock) * c->nprob, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(save ? s.count : (void *)c->dcount, save ? (void *)c->dcount : s.count,
+                           4 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, c->st));
+  if (c->dflags)
+    HIPCHK(c, hipMemcpyAsync(save ? s.flags : (void *)c->dflags, save ? (void *)c->dflags : s.flags,
+                             (size_t)c->nzone, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return MDS_OK;
+}
+
+extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
+{
+  if (!c || nsteps < 1) return MDS_EINVAL;
+  if (c->ndim != 2) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // only the 2D sweep has a free shape
+  static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 128};
+  Snapshot s;
+  int rc = snap_alloc(c, s);
+  if (rc == MDS_OK) rc = snap_copy(c, s, true);
+  int best = c->chunk2d;
+  double bms = 1e300;
+  for (int v : cand) {
+    if (rc || v > c->hprob[0].n[1]) break;
+    c->chunk2d = v;
+    double ms[3];
+    rc = mds_run_timed(c, 1, ms);      // warm the shape
+    if (rc == MDS_OK) rc = mds_run_timed(c, nsteps, ms);
+    if (rc == MDS_OK && ms[0] / nsteps < bms) { bms = ms[0] / nsteps; best = v; }
+    if (rc == MDS_OK) rc = snap_copy(c, s, false);
+  }
+  snap_free(s);
+  c->chunk2d = best;
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  if (best_ms) *best_ms = bms;
+  return rc;
 }
 
 extern "C" int mds_sync(mds_ctx *c)

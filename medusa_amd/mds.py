@@ -18,6 +18,7 @@ BC_OUTFLOW, BC_REFLECT, BC_PERIODIC = 0, 1, 3
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG = 0, 1
 RC_COUNT = 8
+TILE_ROWS2D = 0
 ZG_COUNT = 24
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -63,7 +64,10 @@ _LIB = None
 EXPORTED = ["mds_grid_build", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_conserved_totals", "mds_counters",
-            "mds_stream", "mds_recon_points", "mds_riemann_points"]
+            "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
+            "mds_set_tile", "mds_get_tile", "mds_autotune",
+            "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
+            "mds_init_sph_riemann", "mds_ensemble_params"]
 
 
 def lib():
@@ -97,6 +101,11 @@ def lib():
     L.mds_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.mds_stream.argtypes = [vp]
     L.mds_stream.restype = vp
+    L.mds_run_timed.argtypes = [vp, ctypes.c_int, _dp]
+    L.mds_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, _dp]
+    L.mds_set_tile.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.mds_get_tile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.mds_autotune.argtypes = [vp, ctypes.c_int, _dp]
     L.mds_recon_points.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mds_riemann_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_int)]
@@ -212,6 +221,26 @@ class Context:
         fn = lib().mds_run_graph if graph else lib().mds_run
         self._check(fn(self.h, nsteps), "run")
 
+    def run_timed(self, nsteps):
+        """Eager run with HIP events between kernels: (total_ms, stage1_kernels_ms, stage2_kernels_ms)."""
+        out = np.zeros(3)
+        self._check(lib().mds_run_timed(self.h, nsteps, _ptr(out)), "run_timed")
+        return tuple(out)
+
+    def set_tile(self, param, value):
+        self._check(lib().mds_set_tile(self.h, param, value), "set_tile")
+
+    def get_tile(self, param):
+        v = ctypes.c_int()
+        self._check(lib().mds_get_tile(self.h, param, ctypes.byref(v)), "get_tile")
+        return v.value
+
+    def autotune(self, nsteps=3):
+        """Pick the fastest launch shape on the current state (state is restored)."""
+        ms = np.zeros(1)
+        self._check(lib().mds_autotune(self.h, nsteps, _ptr(ms)), "autotune")
+        return float(ms[0])
+
     def sync(self):
         self._check(lib().mds_sync(self.h), "sync")
 
@@ -240,6 +269,14 @@ class Context:
             if all(self.clock(p)[0] >= self.rp.tmax for p in range(self.nprob)):
                 break
         return done
+
+
+def copy_probe_ms(nbytes, reps=20):
+    """Mean duration (ms) of a device copy kernel over nbytes."""
+    out = np.zeros(1)
+    if lib().mds_copy_probe(int(nbytes), reps, _ptr(out)):
+        raise RuntimeError("mds_copy_probe failed")
+    return float(out[0])
 
 
 def recon_points(precision, a, mom=None, x=None, rcoef=None):
