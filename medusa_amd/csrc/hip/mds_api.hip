@@ -17,6 +17,8 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, const Sweep1
                                          const void *sp, const void *sph_, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a, const void *sp,
                                          const void *sph, int stage, hipStream_t st);
+extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, const Sweep3DArgs *a, const void *sp,
+                                         const void *sph, int stage, hipStream_t st);
 
 struct mds_ctx {
   int precision = 0;
@@ -43,6 +45,8 @@ struct mds_ctx {
   long long *dzstart = nullptr;
   std::vector<void *> geo_allocs;
   int chunk2d = 24;
+  int ty3d = 4;
+  int kc3d = 16;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   char err[512] = {0};
@@ -104,7 +108,8 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
                                StateArrays<R> sp)
 {
   long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (z >= zstart[nprob]) return;
+  bool live = z < zstart[nprob];
+  if (!live) z = zstart[nprob] - 1;       // keep every lane active for the wave reduction
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -124,7 +129,14 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
     loc += (fabs(v) + (cs / sc)) / pb.dx[d];
   }
   double b = 1.0 / loc;
-  atomicMin(&ck[lo].mindt, dt_key(b));
+  // one atomic per wavefront when the whole wavefront belongs to one problem
+  int p0 = __shfl(lo, 0, 64);
+  if (__all(lo == p0)) {
+    double m = wave_min(b);
+    if ((threadIdx.x & 63) == 0) atomicMin(&ck[lo].mindt, dt_key(m));
+  } else {
+    atomicMin(&ck[lo].mindt, dt_key(b));
+  }
 }
 
 // Neumaier-compensated block partials of rho*V and E*V over one problem.
@@ -247,9 +259,9 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   c->sph = probs[0].grid->geom == MDS_GEOM_SPHERICAL;
   c->nv = probs[0].nvel ? probs[0].nvel : (c->sph ? 3 : c->ndim);
   if (c->ndim > 1 && nprob != 1) return set_err(c, MDS_ENOTIMPL, "batches are supported for 1D columns only");
-  if (c->ndim == 3) return set_err(c, MDS_ENOTIMPL, "3D sweep not available in this build");
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
+  if (c->ndim == 3) c->nv = 3;
   if (c->sph && c->ndim != 1) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D only");
   HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   c->zstart.assign(nprob + 1, 0);
@@ -262,6 +274,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     int nvp = P.nvel ? P.nvel : (c->sph ? 3 : c->ndim);
     if (c->ndim == 1 && nvp == 2) nvp = 3;
     if (c->ndim == 2 && nvp < 2) nvp = 2;
+    if (c->ndim == 3) nvp = 3;
     if (nvp != c->nv) return set_err(c, MDS_EINVAL, "problem %d: velocity component count differs in the batch", p);
     for (int d = 0; d < g->ndim; d++) {
       if (g->n[d] < 4) return set_err(c, MDS_EINVAL, "problem %d: need at least 4 zones per active direction", p);
@@ -313,6 +326,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   }
   c->nzone = c->zstart[nprob];
   if (const char *e = getenv("MDS_TILE_ROWS2D")) c->chunk2d = atoi(e) > 0 ? atoi(e) : c->chunk2d;
+  if (c->ndim == 3 && c->hprob[0].n[2] < c->kc3d) c->kc3d = c->hprob[0].n[2];
   size_t rs = rsize(c);
   for (int k = 0; k < 5; k++) {
     HIPCHK(c, hipMalloc(&c->p[k], (size_t)c->nzone * rs));
@@ -462,6 +476,25 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
       StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
       HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
     }
+  } else if (c->ndim == 3) {
+    Sweep3DArgs a;
+    const DevProb &h = c->hprob[0];
+    a.prob = c->dprob;
+    a.clock = c->dclock;
+    a.counters = c->dcount;
+    a.flags = c->dflags;
+    a.nstrip = (h.n[0] + MDS_SEG1D - 1) / MDS_SEG1D;
+    a.ntile = (h.n[1] + c->ty3d - 1) / c->ty3d;
+    a.chunk = c->kc3d;
+    a.nchunk = (h.n[2] + a.chunk - 1) / a.chunk;
+    a.src = (h.g0[0] != 0.0 || h.g0[1] != 0.0 || h.g0[2] != 0.0);
+    if (fp32) {
+      StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+      HIPCHK(c, mds_launch_sweep3d(1, c->ty3d, &a, &sp, &sh, stage, c->st));
+    } else {
+      StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+      HIPCHK(c, mds_launch_sweep3d(0, c->ty3d, &a, &sp, &sh, stage, c->st));
+    }
   } else {
     Sweep2DArgs a;
     a.prob = c->dprob;
@@ -471,6 +504,7 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.chunk = c->chunk2d;
     a.nstrip = (c->hprob[0].n[0] + MDS_SEG1D - 1) / MDS_SEG1D;
     a.nchunk = (c->hprob[0].n[1] + a.chunk - 1) / a.chunk;
+    a.src = (c->hprob[0].g0[0] != 0.0 || c->hprob[0].g0[1] != 0.0 || c->hprob[0].g0[2] != 0.0);
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       HIPCHK(c, mds_launch_sweep2d(1, c->nv, &a, &sp, &sh, stage, c->st));
@@ -604,22 +638,33 @@ extern "C" int mds_copy_probe(long long nbytes, int reps, double *ms)
 extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
 {
   if (!c) return MDS_EINVAL;
+  int *slot = nullptr;
   if (param == MDS_TILE_ROWS2D) {
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "rows per wavefront out of range: %d", value);
-    if (value != c->chunk2d) {
-      c->chunk2d = value;
-      if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-      if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-    }
-    return MDS_OK;
+    slot = &c->chunk2d;
+  } else if (param == MDS_TILE_TY3D) {
+    if (value != 4 && value != 12) return set_err(c, MDS_EINVAL, "3D tile rows must be 4 or 12: %d", value);
+    slot = &c->ty3d;
+  } else if (param == MDS_TILE_KC3D) {
+    if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "planes per workgroup out of range: %d", value);
+    slot = &c->kc3d;
+  } else {
+    return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
   }
-  return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
+  if (*slot != value) {
+    *slot = value;
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  }
+  return MDS_OK;
 }
 
 extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
 {
   if (!c || !value) return MDS_EINVAL;
   if (param == MDS_TILE_ROWS2D) { *value = c->chunk2d; return MDS_OK; }
+  if (param == MDS_TILE_TY3D) { *value = c->ty3d; return MDS_OK; }
+  if (param == MDS_TILE_KC3D) { *value = c->kc3d; return MDS_OK; }
   return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
 }
 
@@ -678,16 +723,26 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
-  if (c->ndim != 2) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // only the 2D sweep has a free shape
-  static const int cand[] = {4, 6, 8, 12, 16, 24, 32, 48, 64, 128};
+  if (c->ndim == 1) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // the 1D sweep has a fixed shape
+  // candidate shapes: 2D rows per wavefront; 3D (rows per workgroup, planes per workgroup)
+  std::vector<std::pair<int, int>> cand;
+  if (c->ndim == 2) {
+    for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
+      if (v <= c->hprob[0].n[1]) cand.push_back({v, 0});
+  } else {
+    for (int ty : {4, 12})
+      for (int kc : {8, 16, 32, 64})
+        if (kc <= c->hprob[0].n[2]) cand.push_back({ty, kc});
+  }
   Snapshot s;
   int rc = snap_alloc(c, s);
   if (rc == MDS_OK) rc = snap_copy(c, s, true);
-  int best = c->chunk2d;
+  std::pair<int, int> best = c->ndim == 2 ? std::make_pair(c->chunk2d, 0) : std::make_pair(c->ty3d, c->kc3d);
   double bms = 1e300;
-  for (int v : cand) {
-    if (rc || v > c->hprob[0].n[1]) break;
-    c->chunk2d = v;
+  for (auto v : cand) {
+    if (rc) break;
+    if (c->ndim == 2) c->chunk2d = v.first;
+    else { c->ty3d = v.first; c->kc3d = v.second; }
     double ms[3];
     rc = mds_run_timed(c, 1, ms);      // warm the shape
     if (rc == MDS_OK) rc = mds_run_timed(c, nsteps, ms);
@@ -695,7 +750,8 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     if (rc == MDS_OK) rc = snap_copy(c, s, false);
   }
   snap_free(s);
-  c->chunk2d = best;
+  if (c->ndim == 2) c->chunk2d = best.first;
+  else { c->ty3d = best.first; c->kc3d = best.second; }
   if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
   if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   if (best_ms) *best_ms = bms;

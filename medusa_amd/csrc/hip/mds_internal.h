@@ -54,6 +54,16 @@ struct Sweep2DArgs {
   unsigned long long *counters;
   unsigned char *flags;    // stage-1 shock tags, one byte per zone
   int nstrip, nchunk, chunk;
+  int src;                 // source terms present (fixed gravity)
+};
+
+struct Sweep3DArgs {
+  const DevProb *prob;     // single problem
+  DevClock *clock;
+  unsigned long long *counters;
+  unsigned char *flags;    // stage-1 shock tags, one byte per zone
+  int nstrip, ntile, nchunk, chunk;
+  int src;                 // source terms present (fixed gravity)
 };
 
 #define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D sweep

@@ -299,6 +299,11 @@ MDS_DEV double dt_source_control_x64(double uu, const double v[3], const double 
   return mdt;
 }
 
+// dt_source_control with reciprocals instead of divisions (FP32-bulk path,
+// where the limit only needs to be close to the reference value)
+template <int NV>
+MDS_DEV double dt_source_control_fast(double uu, const double v[3], const double u[5], const double s[5]);
+
 // ---------------------------------------------------------------------------
 // FP32-bulk
 // ---------------------------------------------------------------------------
@@ -472,10 +477,46 @@ MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
 // positive double -> order-preserving 64-bit key for atomicMin
 MDS_DEV unsigned long long dt_key(double v) { return (unsigned long long)__double_as_longlong(v); }
 
-template <class T>
-MDS_DEV T lane_from_below(T v) { return __shfl_up(v, 1, 64); }     // lane l <- lane l-1
-template <class T>
-MDS_DEV T lane_from_above(T v) { return __shfl_down(v, 1, 64); }   // lane l <- lane l+1
+// Cross-lane neighbours with whole-wave DPP shifts (GFX9 wave_shr:1 /
+// wave_shl:1): lane l reads lane l-1 (from_below) or l+1 (from_above); the
+// edge lane keeps its own value.  Must be called with every lane active.
+MDS_DEV int dpp_shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+MDS_DEV int dpp_shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+MDS_DEV float lane_from_below(float v) { return __int_as_float(dpp_shr1(__float_as_int(v))); }
+MDS_DEV float lane_from_above(float v) { return __int_as_float(dpp_shl1(__float_as_int(v))); }
+MDS_DEV int lane_from_below(int v) { return dpp_shr1(v); }
+MDS_DEV int lane_from_above(int v) { return dpp_shl1(v); }
+MDS_DEV double lane_from_below(double v)
+{
+  long long b = __double_as_longlong(v);
+  int lo = dpp_shr1((int)(b & 0xffffffffll)), hi = dpp_shr1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+MDS_DEV double lane_from_above(double v)
+{
+  long long b = __double_as_longlong(v);
+  int lo = dpp_shl1((int)(b & 0xffffffffll)), hi = dpp_shl1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/x to full double precision: hardware estimate + one Newton step
+MDS_DEV double drcp(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+MDS_DEV double wave_max(double v)
+{
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
 
 MDS_DEV double wave_min(double v)
 {
@@ -484,4 +525,16 @@ MDS_DEV double wave_min(double v)
     v = (o < v) ? o : v;
   }
   return v;
+}
+
+template <int NV>
+MDS_DEV double dt_source_control_fast(double uu, const double v[3], const double u[5], const double s[5])
+{
+  double mdt = (0.02 * u[0]) * drcp(fabs(s[0]) + 1.0e-16);
+  double t = (0.02 * u[1]) * drcp(fabs(s[1]) + 1.0e-16);
+  mdt = fmin(mdt, t);
+  double momterm = 0.0;
+  for (int d = 0; d < NV; d++) momterm += v[d] * s[2 + d];
+  t = (2.0 * uu) * drcp(fabs(fabs(s[1]) - fabs(momterm)) + 1.0e-16);
+  return fmin(mdt, t);
 }

@@ -19,6 +19,8 @@ GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG = 0, 1
 RC_COUNT = 8
 TILE_ROWS2D = 0
+TILE_TY3D = 1
+TILE_KC3D = 2
 ZG_COUNT = 24
 
 _dp = ctypes.POINTER(ctypes.c_double)

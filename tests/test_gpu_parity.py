@@ -255,3 +255,90 @@ def test_graph_replay_matches_eager():
         outs.append((ctx.download(), ctx.clock()))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert outs[0][1] == outs[1][1]
+
+
+def _shock_tube_nd(ndim, n=24, seed=5):
+    """Sod tube along x replicated in y (and z), with a small transverse
+    velocity so that HLLE and HLLC give different transverse fluxes: the
+    radial-face shock tags (step.c:711-717) decide the solver."""
+    ns = [n] * ndim
+    g = mds.GridBuilder(ndim, ns, [0.0] * ndim, [1.0 / n] * ndim)
+    bc = [mds.BC_OUTFLOW] + [mds.BC_PERIODIC] * (ndim - 1)
+    prob = dict(grid=g, bc_lo=bc, bc_hi=bc, gam=1.4)
+    p = np.zeros((5, g.n[2], g.n[1], g.n[0]))
+    x = (np.arange(n) + 0.5) / n
+    p[0] = np.where(x < 0.5, 1.0, 0.125)[None, None, :]
+    p[1] = (np.where(x < 0.5, 1.0, 0.1) / 0.4)[None, None, :]
+    rng = np.random.default_rng(seed)
+    p[3] = 0.05 * rng.uniform(-1, 1, size=p[3].shape)
+    if ndim == 3:
+        p[4] = 0.05 * rng.uniform(-1, 1, size=p[4].shape)
+    return g, prob, p
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_shock_tags_select_hlle_like_oracle(ndim):
+    g, prob, prims = _shock_tube_nd(ndim)
+    say(f"{ndim}D shock tube 24^{ndim} FP64 12 steps (HLLE/HLLC selection by shock tags)")
+    ctx, sim = _run_pair(prob, prims, mds.FP64, nsteps=12)
+    dev = ctx.download()
+    ref = sim.get_prims()
+    err = compare(dev, ref, 1.4)
+    cnt = ctx.counters()
+    say(f"  L1 {err}; device shock/hlle faces {cnt}; oracle {sim.shock_faces()}/{sim.hlle_faces()}")
+    assert sim.shock_faces() > 0
+    assert cnt[0] == sim.shock_faces() and cnt[1] == sim.hlle_faces()
+    for k, v in err.items():
+        assert v <= FP64_TOL, (k, v)
+
+
+def test_rt3d_fp64_parity():
+    g, prob, prims = problems.rt(16)
+    say("RT 3D 16^3 FP64 10 steps (gravity, reflecting z)")
+    ctx, sim = _run_pair(prob, prims, mds.FP64, nsteps=10)
+    dev = ctx.download()
+    ref = sim.get_prims()
+    err = compare(dev, ref, 1.4)
+    t, dt, ist = ctx.clock()
+    say(f"  L1 {err}; t {t!r} vs {sim.t!r}; hlle faces {ctx.counters()[1]} vs {sim.hlle_faces()}")
+    assert abs(t - sim.t) <= 1e-15
+    assert ctx.counters()[1] == sim.hlle_faces()
+    for k, v in err.items():
+        assert v <= FP64_TOL, (k, v)
+
+
+def test_rt3d_fp32_within_tolerance():
+    g, prob, prims = problems.rt(16)
+    say("RT 3D 16^3 FP32-bulk 10 steps")
+    ctx, sim = _run_pair(prob, prims, mds.FP32_BULK, nsteps=10)
+    err = compare(ctx.download(), sim.get_prims(), 1.4)
+    say(f"  L1 {err}")
+    assert err["rho"] <= FP32_RHO_TOL
+
+
+@pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
+def test_results_independent_of_launch_shape(precision):
+    outs = []
+    g, prob, prims = problems.kh(100)
+    for rows in (3, 7, 24):
+        ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+        ctx.set_tile(mds.TILE_ROWS2D, rows)
+        ctx.upload(prims)
+        ctx.update_eos()
+        ctx.run(6)
+        outs.append(ctx.download())
+    g3, prob3, prims3 = problems.rt(20)
+    outs3 = []
+    for ty, kc in ((4, 3), (12, 8), (4, 20)):
+        ctx = mds.Context([prob3], precision=precision, rp=mds.run_params(cfl=0.5))
+        ctx.set_tile(mds.TILE_TY3D, ty)
+        ctx.set_tile(mds.TILE_KC3D, kc)
+        ctx.upload(prims3)
+        ctx.update_eos()
+        ctx.run(4)
+        outs3.append(ctx.download())
+    say(f"launch-shape invariance precision={precision}")
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    for o in outs3[1:]:
+        assert np.array_equal(o, outs3[0])
