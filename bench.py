@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
-"""Headline benchmark of the explicit hydro sweep on one MI355X.
+"""Benchmark of the explicit hydro sweep on one MI355X.
 
-Workload: 2D Kelvin-Helmholtz, 2048 x 2048 periodic box, gamma = 5/3,
-volume-moment PPM + HLLC/HLLE, SSP-RK2, CFL 0.5 (synthetic seeded initial
-state).  A step is one full SSP-RK2 step of every zone (both stages,
-boundaries, EOS, sources and the device-side dt policy included).
+Default workload (the headline): 2D Kelvin-Helmholtz, 2048 x 2048 periodic
+box, gamma = 5/3, volume-moment PPM + HLLC/HLLE, SSP-RK2, CFL 0.5, seeded
+synthetic initial state.  A step is one full SSP-RK2 step of every zone
+(both stages, boundaries, EOS, sources and the device-side dt policy).
+Other workloads (--workload): sod1d (1024 zones, to t=0.2), rt3d (512^3,
+gravity, 100 steps), ensemble (2048 independent spherical columns of
+256-8192 zones, 500 steps, one launch per stage for the whole batch).
 
 Prints ONE JSON line:
-  value      zone-updates/s of the FP32-bulk path (the headline)
-  fp64       the FP64 path on the same workload and step count
-  roofline   dominant kernel (the FP32-bulk 2D stage kernel): algorithmic
-             bytes per launch / average launch duration measured with HIP
-             events on the context stream inside the timed region
+  value         zone-updates/s of the FP32-bulk path (FP64 for sod1d)
+  fp64          the FP64 path on the same workload and step count
+  roofline      dominant kernel (the headline path's stage kernel):
+                algorithmic bytes per launch / average launch duration,
+                measured with HIP events on the context stream inside the
+                timed region
   cpu_baseline  the CPU oracle (reference-flag build, 1 thread) on a
-             bounded number of steps of the same grid
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                bounded sample of the same workload
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kh2d]
 """
 import argparse
 import json
@@ -28,71 +32,119 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor figure
-BYTES_PER_ZONE_UPDATE = {"fp32": 5 * 4 * 4, "fp64": 5 * 4 * 8}   # 5 state passes x NVAR=4 x size
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def l1rel(a, b):
-    den = np.sum(np.abs(b))
-    return float(np.sum(np.abs(a - b)) / den) if den > 0 else float(np.sum(np.abs(a - b)))
+def l1rel(a, b, w=None):
+    w = 1.0 if w is None else w
+    den = np.sum(np.abs(b) * w)
+    return float(np.sum(np.abs(a - b) * w) / den) if den > 0 else float(np.sum(np.abs(a - b) * w))
 
 
-def run_path(mds, prob, prims, precision, steps, warmup, nz):
-    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
-    ctx.upload(prims)
+# --------------------------------------------------------------------------
+# workloads: (problems, NVAR, default steps, label, kernel name)
+# --------------------------------------------------------------------------
+def workload(name, n):
+    from medusa_amd import problems
+    if name == "kh2d":
+        n = n or 2048
+        g, prob, prims = problems.kh(n)
+        return dict(probs=[(prob, prims)], nvar=4, steps=200, zones=n * n, gam=[5.0 / 3.0],
+                    label=f"kh2d {n}x{n} periodic, gamma=5/3, PPM+HLLC, SSP-RK2, CFL 0.5",
+                    kernel={"fp32": "sweep2d<float,2,false>", "fp64": "sweep2d<double,2,false>"}, n=n)
+    if name == "rt3d":
+        n = n or 512
+        g, prob, prims = problems.rt(n)
+        return dict(probs=[(prob, prims)], nvar=5, steps=100, zones=n ** 3, gam=[1.4],
+                    label=f"rt3d {n}^3, g=-0.1 z, periodic x,y, reflecting z, gamma=1.4, CFL 0.5",
+                    kernel={"fp32": "sweep3d<float,3,true,TY>", "fp64": "sweep3d<double,3,true,TY>"}, n=n)
+    if name == "sod1d":
+        n = n or 1024
+        g, prob, prims = problems.sod(n)
+        return dict(probs=[(prob, prims)], nvar=3, steps=909, zones=n, gam=[1.4],
+                    label=f"sod1d {n} zones, outflow, gamma=1.4, CFL 0.5, 909 steps (t=0.2)",
+                    kernel={"fp32": "sweep1d_f32<1,false>", "fp64": "sweep1d_x64<1,false>"}, n=n)
+    if name == "ensemble":
+        nprob = n or 2048
+        par, ens = problems.ensemble(nprob)
+        zones = int(sum(int(r[0]) for r in par))
+        return dict(probs=[(e[1], e[2]) for e in ens], nvar=3, steps=500, zones=zones, gam=list(par[:, 1]),
+                    label=f"ensemble of {nprob} spherical log-r Riemann columns, 256-8192 zones (seeded), "
+                          f"{zones} zones, per-problem dt, 500 steps",
+                    kernel={"fp32": "sweep1d_f32<1,true>", "fp64": "sweep1d_x64<3,true>"}, n=nprob)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_path(mds, W, precision, steps, warmup):
+    probs = [p for p, _ in W["probs"]]
+    ctx = mds.Context(probs, precision=precision, rp=mds.run_params(cfl=0.5))
+    for k, (_, prims) in enumerate(W["probs"]):
+        ctx.upload(prims, k)
     ctx.update_eos()
     tuned_ms = ctx.autotune(3)          # launch-shape autotuner (state is restored afterwards)
-    rows = ctx.get_tile(mds.TILE_ROWS2D)
-    m0 = ctx.totals()
+    m0 = np.array([ctx.totals(k) for k in range(len(probs))])
     if warmup:
         ctx.run(warmup, graph=True)
         ctx.sync()
     tot_ms, st1_ms, st2_ms = ctx.run_timed(steps)
-    final = ctx.download()
-    m1 = ctx.totals()
-    t, dt, ist = ctx.clock()
-    # graph replay rate on top of the timed region (informational)
+    finals = [ctx.download(k) for k in range(len(probs))]
+    m1 = np.array([ctx.totals(k) for k in range(len(probs))])
     g0 = time.perf_counter()
-    ctx.run(steps, graph=True)
+    ctx.run(steps, graph=True)          # graph replay rate on top of the timed region (informational)
     ctx.sync()
     g1 = time.perf_counter()
-    launches = 2 * steps
-    avg_launch_ms = (st1_ms + st2_ms) / launches
-    return dict(
-        ctx=ctx, final=final,
-        ms_per_step=tot_ms / steps,
-        value=nz * steps / (tot_ms * 1e-3),
-        graph_value=nz * steps / (g1 - g0),
-        avg_launch_ms=avg_launch_ms,
-        stage_ms=(st1_ms / steps, st2_ms / steps),
-        mass_drift=float((m1[0] - m0[0]) / m0[0]),
-        energy_drift=float((m1[1] - m0[1]) / m0[1]),
-        t=t, istep=ist, counters=ctx.counters(), rows=rows, tuned_ms=tuned_ms)
+    zones = W["zones"]
+    drift = np.max(np.abs((m1[:, 0] - m0[:, 0]) / m0[:, 0]))
+    edrift = np.max(np.abs((m1[:, 1] - m0[:, 1]) / m0[:, 1]))
+    return dict(finals=finals, ms_per_step=tot_ms / steps, value=zones * steps / (tot_ms * 1e-3),
+                graph_value=zones * steps / (g1 - g0), avg_launch_ms=(st1_ms + st2_ms) / (2 * steps),
+                stage_ms=(st1_ms / steps, st2_ms / steps), mass_drift=float(drift), energy_drift=float(edrift),
+                counters=ctx.counters(), tuned_ms=tuned_ms,
+                tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D)])
 
 
-def cpu_baseline(n, steps):
-    """CPU oracle, reference optimisation flags (-O3 -ffast-math), 1 thread, same grid."""
+def cpu_baseline(name, W, steps):
+    """CPU oracle, reference optimisation flags (-O3 -ffast-math), 1 thread."""
     from oracle import pyoracle as po
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from parity_util import oracle_for
     from medusa_amd import problems
-    g, prob, prims = problems.kh(n)
-    cfg = po.make_config(2, [n, n], [0.0, 0.0], [1.0 / n, 1.0 / n], [3, 3], [3, 3], 5.0 / 3.0, cfl=0.5)
-    sim = po.Sim(cfg, fast=True)
-    sim.set_prims(prims)
-    t0 = time.perf_counter()
-    sim.run(max_steps=steps)
-    el = time.perf_counter() - t0
-    return dict(value=n * n * steps / el, unit="zone-updates/s", cores=1, kind="port",
-                sample=f"KH {n}x{n}, first {steps} SSP-RK2 steps from the same initial state, "
-                       f"1 thread, oracle built with -O3 -ffast-math -ftree-vectorize -fpeel-loops; "
-                       f"{el:.1f} s")
+    if name == "rt3d":
+        # bounded sample: a 64 x 64 x n column of the same zone shape and state
+        n = W["n"]
+        g, prob, prims = problems.rt(n)
+        sub = np.ascontiguousarray(prims[:, :, :64, :64])
+        from medusa_amd import mds
+        g2 = mds.GridBuilder(3, [64, 64, n], [-0.25, -0.25, -0.75], [0.5 / n, 0.5 / n, 1.5 / n])
+        prob2 = dict(prob, grid=g2)
+        sample = [(prob2, sub)]
+        what = f"64x64x{n} column of the rt3d grid (same zone shape and state), periodic x,y"
+    elif name == "ensemble":
+        sample = W["probs"][:64]
+        what = "first 64 columns of the ensemble"
+    else:
+        sample = W["probs"]
+        what = "the full grid"
+    zones = 0
+    el = 0.0
+    for prob, prims in sample:
+        sim = oracle_for(prob, fast=True)
+        sim.set_prims(prims)
+        t0 = time.perf_counter()
+        sim.run(max_steps=steps)
+        el += time.perf_counter() - t0
+        zones += int(np.prod(prims.shape[1:]))
+    return dict(value=zones * steps / el, unit="zone-updates/s", cores=1, kind="port",
+                sample=f"{what}, {steps} SSP-RK2 steps from the initial state, 1 thread, oracle built with "
+                       f"-O3 -ffast-math -ftree-vectorize -fpeel-loops; {el:.1f} s")
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the FP32 stage kernel from the committed counter summary, if present."""
-    p = os.path.join(ROOT, "profiles", "kh2d_2048_sweep2d_f32_traffic.json")
+def traffic_from_profile(name, path):
+    """HBM bytes per launch from the committed counter summary, if present."""
+    p = os.path.join(ROOT, "profiles", f"{name}_{path}_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -102,9 +154,10 @@ def traffic_from_profile():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--workload", default="kh2d", choices=["kh2d", "rt3d", "sod1d", "ensemble"])
+    ap.add_argument("--n", type=int, default=None, help="grid size (problem count for ensemble)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
@@ -114,47 +167,54 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    from medusa_amd import mds, problems
+    from medusa_amd import mds
     if not os.path.exists(mds.LIB_PATH):
         from medusa_amd import build
         build.build(verbose=False)
 
-    n = args.n
-    nz = n * n
-    log(f"[bench] KH {n}x{n}: building grid and initial state")
-    g, prob, prims = problems.kh(n)
+    log(f"[bench] workload {args.workload}: building grids and initial state")
+    W = workload(args.workload, args.n)
+    steps = args.steps if args.steps is not None else W["steps"]
+    zones = W["zones"]
+    paths = [("fp64", mds.FP64), ("fp32", mds.FP32_BULK)]
+    if args.workload == "sod1d" and not args.only:
+        paths = [("fp64", mds.FP64)]
     res = {}
-    for name, prec in (("fp64", mds.FP64), ("fp32", mds.FP32_BULK)):
+    for name, prec in paths:
         if args.only and args.only != name:
             continue
-        log(f"[bench] {name}: warmup {args.warmup} steps, timed {args.steps} steps")
-        res[name] = run_path(mds, prob, prims, prec, args.steps, args.warmup, nz)
-        r = res[name]
-        log(f"[bench] {name}: {r['value']:.4e} zone-updates/s, {r['ms_per_step']:.3f} ms/step, "
+        log(f"[bench] {name}: warmup {args.warmup} steps, timed {steps} steps")
+        res[name] = r = run_path(mds, W, prec, steps, args.warmup)
+        log(f"[bench] {name}: {r['value']:.4e} zone-updates/s, {r['ms_per_step']:.4f} ms/step, "
             f"graph {r['graph_value']:.4e}, launch {r['avg_launch_ms'] * 1e3:.1f} us, "
-            f"mass drift {r['mass_drift']:.2e}, autotuned rows/wave {r['rows']}")
+            f"mass drift {r['mass_drift']:.2e}, tiles {r['tiles']}")
 
-    head = res.get("fp32") or res.get("fp64")
     head_name = "fp32" if "fp32" in res else "fp64"
+    head = res[head_name]
     err = None
     if "fp32" in res and "fp64" in res:
-        a, b = res["fp32"]["final"], res["fp64"]["final"]
-        gam = 5.0 / 3.0
-        err = dict(rho=l1rel(a[0], b[0]), p=l1rel((gam - 1) * a[1], (gam - 1) * b[1]),
-                   v=l1rel(np.hypot(a[2], a[3]), np.hypot(b[2], b[3])))
-        log(f"[bench] FP32-bulk vs FP64 relative L1 after {args.warmup + args.steps} steps: {err}")
+        a, b = res["fp32"]["finals"], res["fp64"]["finals"]
+        e = {"rho": [], "p": [], "v": []}
+        for k, (fa, fb) in enumerate(zip(a, b)):
+            gam = W["gam"][k] if k < len(W["gam"]) else W["gam"][0]
+            e["rho"].append(l1rel(fa[0], fb[0]))
+            e["p"].append(l1rel((gam - 1) * fa[1], (gam - 1) * fb[1]))
+            e["v"].append(l1rel(np.sqrt(np.sum(fa[2:5] ** 2, 0)), np.sqrt(np.sum(fb[2:5] ** 2, 0))))
+        err = {k: float(np.max(v)) for k, v in e.items()}
+        log(f"[bench] FP32-bulk vs FP64 relative L1 (max over problems) after {args.warmup + steps} steps: {err}")
 
-    bpl = BYTES_PER_ZONE_UPDATE[head_name] / 2.0 * nz       # algorithmic bytes per stage launch
+    bpz = 5 * W["nvar"] * (4 if head_name == "fp32" else 8)      # algorithmic bytes per zone-update
+    bpl = bpz / 2.0 * zones                                        # per stage launch
     achieved = bpl / (head["avg_launch_ms"] * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(),
-                    kernel=f"sweep2d<{'float' if head_name == 'fp32' else 'double'},2>",
-                    bytes_per_launch=bpl, avg_launch_us=round(head["avg_launch_ms"] * 1e3, 2))
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.workload, head_name),
+                    kernel=W["kernel"][head_name], bytes_per_launch=bpl,
+                    avg_launch_us=round(head["avg_launch_ms"] * 1e3, 2))
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU baseline: oracle on {n}x{n} for {args.cpu_steps} steps")
-        cpu = cpu_baseline(n, args.cpu_steps)
+        log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
+        cpu = cpu_baseline(args.workload, W, args.cpu_steps)
         log(f"[bench] CPU baseline {cpu['value']:.4e} zone-updates/s")
 
     out = {
@@ -162,7 +222,7 @@ def main():
         "value": head["value"],
         "unit": "zone-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
@@ -170,9 +230,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if head_name == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"kh2d {n}x{n} periodic, gamma=5/3, PPM+HLLC, SSP-RK2, CFL 0.5",
-                   "zones": nz, "path": "FP32-bulk" if head_name == "fp32" else "FP64",
-                   "parallelism": "single-gpu"},
+        "config": {"workload": W["label"], "zones": zones,
+                   "path": "FP32-bulk" if head_name == "fp32" else "FP64", "parallelism": "single-gpu"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -180,12 +239,12 @@ def main():
         r = res["fp64"]
         out["fp64"] = dict(value=r["value"], ms_per_step=r["ms_per_step"], graph_value=r["graph_value"],
                            avg_launch_us=r["avg_launch_ms"] * 1e3,
-                           achieved_gbs=BYTES_PER_ZONE_UPDATE["fp64"] / 2.0 * nz / (r["avg_launch_ms"] * 1e-3) / 1e9,
-                           mass_drift=r["mass_drift"], energy_drift=r["energy_drift"])
+                           achieved_gbs=5 * W["nvar"] * 8 / 2.0 * zones / (r["avg_launch_ms"] * 1e-3) / 1e9,
+                           mass_drift=r["mass_drift"], energy_drift=r["energy_drift"], tiles=r["tiles"])
     if "fp32" in res:
         r = res["fp32"]
         out["fp32"] = dict(graph_value=r["graph_value"], mass_drift=r["mass_drift"],
-                           energy_drift=r["energy_drift"], stage_ms=r["stage_ms"])
+                           energy_drift=r["energy_drift"], stage_ms=r["stage_ms"], tiles=r["tiles"])
     if "fp32" in res and "fp64" in res:
         out["speedup_fp32_over_fp64"] = res["fp32"]["value"] / res["fp64"]["value"]
         out["l1_fp32_vs_fp64"] = err

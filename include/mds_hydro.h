@@ -122,6 +122,8 @@ typedef struct mds_grid {
  * must be set on input; allocates the arrays (release with mds_grid_free). */
 int mds_grid_build(mds_grid *g);
 void mds_grid_free(mds_grid *g);
+/* Build count independent grids (host threads); returns the lowest error code. */
+int mds_grid_build_batch(mds_grid *g, int count);
 
 /* Physics and boundary description of one problem. */
 typedef struct mds_problem {
@@ -129,7 +131,9 @@ typedef struct mds_problem {
   int bc_lo[3], bc_hi[3];  /* MDS_BC_* per direction (all variables) */
   double gam;              /* gamma-law index (eos_gamma.c:5-12) */
   double g0[3];            /* fixed gravity, contravariant (gravity.c:46-64); zeros = none */
-  int nvel;                /* velocity components carried: 0 = auto (all non-zero ones) */
+  int nvel;                /* velocity components carried; 0 = auto: ndim, except FP64
+                              spherical columns, which carry 3 (the connection terms give
+                              the transverse components round-off sources) */
 } mds_problem;
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */

@@ -28,7 +28,7 @@ CC = shutil.which("gcc") or "cc"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics", "-Wno-unused-result", "-Wno-unused-value"]
-C_FLAGS = ["-O2", "-std=c99", "-fPIC", "-fno-fast-math", "-ffp-contract=off", "-Wall",
+C_FLAGS = ["-O2", "-std=c99", "-fPIC", "-fno-fast-math", "-ffp-contract=off", "-fopenmp", "-Wall",
            "-Wno-unused-function"]
 
 
@@ -68,7 +68,7 @@ def build(verbose=True, jobs=8):
     if failed:
         raise RuntimeError("compile failed: " + " ".join(failed[0]))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lm"], verbose)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lgomp", "-lm"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

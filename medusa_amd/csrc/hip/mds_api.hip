@@ -257,12 +257,22 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   c->tmax = rp->tmax;
   c->ndim = probs[0].grid->ndim;
   c->sph = probs[0].grid->geom == MDS_GEOM_SPHERICAL;
-  c->nv = probs[0].nvel ? probs[0].nvel : (c->sph ? 3 : c->ndim);
+  // auto: FP64 spherical columns carry all three velocity components (the
+  // connection terms give the transverse ones round-off sources, so the
+  // reference evolves them); FP32-bulk carries the ndim physical ones
+  auto auto_nv = [&](const mds_problem &P) {
+    if (P.nvel) return P.nvel;
+    bool s = P.grid && P.grid->geom == MDS_GEOM_SPHERICAL;
+    return (s && c->precision == MDS_FP64) ? 3 : (P.grid ? P.grid->ndim : 1);
+  };
+  c->nv = auto_nv(probs[0]);
   if (c->ndim > 1 && nprob != 1) return set_err(c, MDS_ENOTIMPL, "batches are supported for 1D columns only");
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
   if (c->ndim == 3) c->nv = 3;
   if (c->sph && c->ndim != 1) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D only");
+  if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
+    return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
   HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   c->zstart.assign(nprob + 1, 0);
   c->hprob.resize(nprob);
@@ -271,7 +281,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     const mds_grid *g = P.grid;
     if (!g || g->ndim != c->ndim || (g->geom == MDS_GEOM_SPHERICAL) != (c->sph != 0))
       return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same ndim and geometry", p);
-    int nvp = P.nvel ? P.nvel : (c->sph ? 3 : c->ndim);
+    int nvp = auto_nv(P);
     if (c->ndim == 1 && nvp == 2) nvp = 3;
     if (c->ndim == 2 && nvp < 2) nvp = 2;
     if (c->ndim == 3) nvp = 3;

@@ -132,8 +132,11 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
   double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, sig = 1.0;
   int isshock = hllc_x64<NV, !SPH>(L, Rt, fg, 0, F, sig);
   const bool face_ok = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
-  double lim = face_ok ? pb.dx[0] / fabs(sig) : 1.0e300;
-  if (face_ok && isshock && lane < 62) atomicAdd(&a.counters[0], 1ull);
+  // CFL limits, reduced per wavefront before the single division (min of
+  // dx/|s| = dx/max|s|, min of 1/loc = 1/max loc; bit-identical)
+  double smax = face_ok ? fabs(sig) : 0.0;
+  double locmax = 0.0, srcnum = 1.0e300, lim_src = 1.0e300;
+  unsigned long long nsh = __popcll(__ballot(face_ok && isshock && lane < 62));
 
   // flux through the upper face from the lane above
   double Fp[5];
@@ -186,8 +189,16 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     double U[5], Uh[5];
     p_to_u_x64<NV, !SPH>(rn, un, vn, gcov0, U);
     p_to_u_x64<NV, !SPH>(rho, uu, v, gcov0, Uh);
-    double dsrc = dt_source_control_x64<NV>(un, vn, U, src);
-    lim = (lim < dsrc) ? lim : dsrc;
+    if (SPH || pb.g0[0] != 0.0) {
+      double dsrc = dt_source_control_x64<NV>(un, vn, U, src);
+      lim_src = (lim_src < dsrc) ? lim_src : dsrc;
+    } else {
+      // no sources: the three quotients share the denominator 1e-16 (step.c:212-219)
+      double m0 = 0.02 * U[0], m1 = 0.02 * U[1], m2 = 2.0 * un;
+      double m = (m0 < m1) ? m0 : m1;
+      m = (m < m2) ? m : m2;
+      srcnum = m;
+    }
 
     const double vol = SPH ? zg[MDS_ZG_VOLUME] : pb.volume;
     const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
@@ -208,13 +219,19 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
       // the eos array as last filled (from this stage's input state)
       double cs = sqrt((gam * press) / rho);
       double sc0 = SPH ? zg[MDS_ZG_SCALE0] : 1.0;
-      double loc = (fabs(vo[0]) + (SPH ? cs / sc0 : cs)) / pb.dx[0];
-      double zl = 1.0 / loc;
-      lim = (lim < zl) ? lim : zl;
+      locmax = (fabs(vo[0]) + (SPH ? cs / sc0 : cs)) / pb.dx[0];
     }
   }
-  double wmin = wave_min(lim);
-  if (lane == 0 && wmin < 1.0e300) atomicMin(&ck->mindt, dt_key(wmin));
+  double wsmax = wave_max(smax), wloc = wave_max(locmax), wsrc = wave_min(srcnum), wlim = wave_min(lim_src);
+  if (lane == 0) {
+    double lim = 1.0e300;
+    if (wsmax > 0.0) lim = fmin(lim, pb.dx[0] / wsmax);
+    if (wloc > 0.0) lim = fmin(lim, 1.0 / wloc);
+    if (wsrc < 1.0e300) lim = fmin(lim, wsrc / (0.0 + 1.0e-16));
+    lim = fmin(lim, wlim);
+    if (lim < 1.0e300) atomicMin(&ck->mindt, dt_key(lim));
+    if (nsh) atomicAdd(&a.counters[0], nsh);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -295,8 +312,9 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
   if (area >= 1.0e-16) isshock = hllc_f32<NV, !SPH>(L, Rt, fg, 0, Ff, sig);
   else sig = 1.0e-30f;
   const bool face_ok = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
-  double lim = face_ok ? pb.dx[0] / fabs((double)sig) : 1.0e300;
-  if (face_ok && isshock && lane < 62) atomicAdd(&a.counters[0], 1ull);
+  double smax = face_ok ? (double)fabsf(sig) : 0.0;
+  double locmax = 0.0, lim_src = 1.0e300;
+  unsigned long long nsh = __popcll(__ballot(face_ok && isshock && lane < 62));
   double F[5];
   for (int k = 0; k < 2 + NV; k++) F[k] = area * (double)Ff[k];
   double Fp[5];
@@ -348,31 +366,43 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     double U[5], Uh[5];
     p_to_u_x64<NV, !SPH>(rn, un, vn, gcov0, U);
     p_to_u_x64<NV, !SPH>(drho, duu, dv, gcov0, Uh);
-    double dsrc = dt_source_control_x64<NV>(un, vn, U, src);
-    lim = (lim < dsrc) ? lim : dsrc;
+    lim_src = dt_source_control_fast<NV>(un, vn, U, src);
     const double vol = SPH ? zg[MDS_ZG_VOLUME] : pb.volume;
-    const double ivol = 1.0 / vol;
+    const double ivol = drcp(vol);
     const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
     for (int k = 0; k < 2 + NV; k++) {
       double ddt = (Fp[k] - F[k]) * ivol - src[k];
       U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
     }
-    double ro, uo, vo[3];
-    u_to_p_x64<NV, !SPH>(U, gcon, ro, uo, vo);
+    double ro = U[0], uo, vo[3] = {0.0, 0.0, 0.0};
+    {
+      double ri = drcp(U[0]), vsq = 0.0;
+      for (int d = 0; d < NV; d++) {
+        double m = U[2 + d] * ri;           // covariant velocity
+        vo[d] = SPH ? gcon[d] * m : m;
+        vsq += m * vo[d];
+      }
+      uo = U[1] - 0.5 * ro * vsq;
+    }
     StateArrays<float> &out = (stage == 0) ? sph : sp;
     out.v[0][off + c] = (float)ro;
     out.v[1][off + c] = (float)uo;
     for (int d = 0; d < NV; d++) out.v[2 + d][off + c] = (float)vo[d];
     if (stage == 1) {
-      double cs = sqrt((pb.gam * dpress) / drho);
+      float cs = fsqrt((float)pb.gam * press * frcp(rho));
       double sc0 = SPH ? zg[MDS_ZG_SCALE0] : 1.0;
-      double loc = (fabs(vo[0]) + cs / sc0) / pb.dx[0];
-      double zl = 1.0 / loc;
-      lim = (lim < zl) ? lim : zl;
+      locmax = (fabs(vo[0]) + (double)cs * (SPH ? drcp(sc0) : 1.0)) * (1.0 / pb.dx[0]);
     }
   }
-  double wmin = wave_min(lim);
-  if (lane == 0 && wmin < 1.0e300) atomicMin(&ck->mindt, dt_key(wmin));
+  double wsmax = wave_max(smax), wloc = wave_max(locmax), wlim = wave_min(lim_src);
+  if (lane == 0) {
+    double lim = 1.0e300;
+    if (wsmax > 0.0) lim = fmin(lim, pb.dx[0] / wsmax);
+    if (wloc > 0.0) lim = fmin(lim, 1.0 / wloc);
+    lim = fmin(lim, wlim);
+    if (lim < 1.0e300) atomicMin(&ck->mindt, dt_key(lim));
+    if (nsh) atomicAdd(&a.counters[0], nsh);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -404,7 +434,9 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, const Sweep1
   } else {
     const StateArrays<float> &p = *(const StateArrays<float> *)sp;
     const StateArrays<float> &h = *(const StateArrays<float> *)sph_;
-    if (sph) {
+    if (sph && nv == 1) {
+      hipLaunchKernelGGL((sweep1d_f32<1, true>), grid, block, 0, st, *a, p, h, stage);
+    } else if (sph) {
       hipLaunchKernelGGL((sweep1d_f32<3, true>), grid, block, 0, st, *a, p, h, stage);
     } else if (nv == 1) {
       hipLaunchKernelGGL((sweep1d_f32<1, false>), grid, block, 0, st, *a, p, h, stage);

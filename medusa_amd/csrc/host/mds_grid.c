@@ -252,6 +252,18 @@ int mds_grid_build(mds_grid *g)
   return rc;
 }
 
+int mds_grid_build_batch(mds_grid *g, int count)
+{
+  if (!g || count < 0) return MDS_EINVAL;
+  int rc = MDS_OK;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : rc)
+  for (int q = 0; q < count; q++) {
+    int r = mds_grid_build(&g[q]);
+    if (r < rc) rc = r;
+  }
+  return rc;
+}
+
 void mds_grid_free(mds_grid *g)
 {
   if (!g) return;

@@ -63,7 +63,7 @@ class RunParams(ctypes.Structure):
 
 _LIB = None
 
-EXPORTED = ["mds_grid_build", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
+EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
@@ -83,6 +83,7 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     L.mds_grid_build.argtypes = [ctypes.POINTER(Grid)]
+    L.mds_grid_build_batch.argtypes = [ctypes.POINTER(Grid), ctypes.c_int]
     L.mds_grid_free.argtypes = [ctypes.POINTER(Grid)]
     L.mds_grid_free.restype = None
     L.mds_default_run_params.argtypes = [ctypes.POINTER(RunParams)]
@@ -120,24 +121,48 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def _grid_desc(ndim, n, startx, dx, geom, rcoord, g=None):
+    g = g if g is not None else Grid()
+    g.ndim = ndim
+    n = list(n) + [1] * (3 - len(n))
+    for d in range(3):
+        g.n[d] = int(n[d])
+        g.startx[d] = float(startx[d]) if d < len(startx) else 0.0
+        g.dx[d] = float(dx[d]) if d < len(dx) else 1.0
+    g.geom = geom
+    g.rcoord = rcoord
+    return g
+
+
 class GridBuilder:
     """Owns an mds_grid built by mds_grid_build (the reference geometry)."""
 
-    def __init__(self, ndim, n, startx, dx, geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM):
-        self.g = Grid()
-        self.g.ndim = ndim
-        n = list(n) + [1] * (3 - len(n))
-        for d in range(3):
-            self.g.n[d] = int(n[d])
-            self.g.startx[d] = float(startx[d]) if d < len(startx) else 0.0
-            self.g.dx[d] = float(dx[d]) if d < len(dx) else 1.0
-        self.g.geom = geom
-        self.g.rcoord = rcoord
-        rc = lib().mds_grid_build(ctypes.byref(self.g))
-        if rc:
-            raise RuntimeError(f"mds_grid_build failed ({rc})")
-        self.ndim = ndim
+    def __init__(self, ndim, n, startx, dx, geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, _built=None):
+        if _built is not None:
+            self.g = _built
+        else:
+            self.g = _grid_desc(ndim, n, startx, dx, geom, rcoord)
+            rc = lib().mds_grid_build(ctypes.byref(self.g))
+            if rc:
+                raise RuntimeError(f"mds_grid_build failed ({rc})")
+        self.ndim = self.g.ndim
         self.n = [self.g.n[d] for d in range(3)]
+
+    @staticmethod
+    def batch(descs):
+        """Build many grids with host threads: descs = [(ndim, n, startx, dx, geom, rcoord), ...]."""
+        arr = (Grid * len(descs))()
+        for k, d in enumerate(descs):
+            _grid_desc(*d, g=arr[k])
+        rc = lib().mds_grid_build_batch(arr, len(descs))
+        if rc:
+            raise RuntimeError(f"mds_grid_build_batch failed ({rc})")
+        out = []
+        for k in range(len(descs)):
+            g = Grid()
+            ctypes.pointer(g)[0] = arr[k]
+            out.append(GridBuilder(0, [], [], [], _built=g))
+        return out
 
     def __del__(self):
         if getattr(self, "g", None) is not None and _LIB is not None:

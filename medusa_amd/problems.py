@@ -39,8 +39,16 @@ def sph_riemann(n, gam, rho_in, p_in, r0, frho, fp, rmin=1.0e5, rmax=1.0e9):
     return g, prob, mds.init_sph_riemann(g, gam, rho_in, p_in, r0, frho, fp)
 
 
-def ensemble(nprob=2048, nmin=256, nmax=8192, seed=0x5EED0004):
-    """The batched ensemble of spherical Riemann problems; returns (params, list of (grid, prob, prims))."""
+def ensemble(nprob=2048, nmin=256, nmax=8192, seed=0x5EED0004, rmin=1.0e5, rmax=1.0e9):
+    """The batched ensemble of spherical Riemann problems; returns (params, list of (grid, prob, prims)).
+    Geometry is built with host threads (mds_grid_build_batch)."""
     par = mds.ensemble_params(nprob, nmin, nmax, seed)
-    out = [sph_riemann(*row) for row in par]
+    x0, x1 = math.log(rmin), math.log(rmax)
+    descs = [(1, [int(r[0])], [x0], [(x1 - x0) / int(r[0])], mds.GEOM_SPHERICAL, mds.RCOORD_LOG) for r in par]
+    grids = mds.GridBuilder.batch(descs)
+    out = []
+    for g, r in zip(grids, par):
+        n, gam, rho_in, p_in, r0, frho, fp = r
+        prob = dict(grid=g, bc_lo=[mds.BC_REFLECT], bc_hi=[mds.BC_OUTFLOW], gam=gam)
+        out.append((g, prob, mds.init_sph_riemann(g, gam, rho_in, p_in, r0, frho, fp)))
     return par, out

@@ -7,15 +7,19 @@ from oracle import pyoracle as po
 from medusa_amd import mds
 
 
-def oracle_for(prob, rp_cfl=0.5, **kw):
+def oracle_config(prob, rp_cfl=0.5, **kw):
+    """oracle configuration mirroring a device problem description"""
     g = prob["grid"]
     nd = g.ndim
-    cfg = po.make_config(
+    return po.make_config(
         nd, g.n[:nd], [g.g.startx[d] for d in range(nd)], [g.g.dx[d] for d in range(nd)],
         list(prob["bc_lo"]), list(prob["bc_hi"]), prob["gam"], cfl=rp_cfl,
         geom=po.GEOM_SPHERICAL if g.g.geom == mds.GEOM_SPHERICAL else po.GEOM_CARTESIAN,
         rcoord=g.g.rcoord, g0=tuple(prob.get("g0", (0.0, 0.0, 0.0))), **kw)
-    return po.Sim(cfg)
+
+
+def oracle_for(prob, rp_cfl=0.5, fast=False, **kw):
+    return po.Sim(oracle_config(prob, rp_cfl, **kw), fast=fast)
 
 
 def l1rel(a, b, w=None):
