@@ -78,13 +78,19 @@ def workload(name, n):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_path(mds, W, precision, steps, warmup):
+def run_path(mds, W, precision, steps, warmup, tiles=None):
     probs = [p for p, _ in W["probs"]]
     ctx = mds.Context(probs, precision=precision, rp=mds.run_params(cfl=0.5))
     for k, (_, prims) in enumerate(W["probs"]):
         ctx.upload(prims, k)
     ctx.update_eos()
-    tuned_ms = ctx.autotune(3)          # launch-shape autotuner (state is restored afterwards)
+    if tiles:
+        for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D), tiles):
+            if t != mds.TILE_FUSE1D or ctx.ndim == 1:
+                ctx.set_tile(t, v)
+        tuned_ms = None
+    else:
+        tuned_ms = ctx.autotune(3)      # launch-shape autotuner (state is restored afterwards)
     m0 = np.array([ctx.totals(k) for k in range(len(probs))])
     if warmup:
         ctx.run(warmup, graph=True)
@@ -97,17 +103,22 @@ def run_path(mds, W, precision, steps, warmup):
     ctx.sync()
     g1 = time.perf_counter()
     zones = W["zones"]
+    # sweep launches per step: one two-stage launch for 1D columns without periodic x, else one per stage
+    launches = 1 if (ctx.ndim == 1 and ctx.get_tile(mds.TILE_FUSE1D)) else 2
     drift = np.max(np.abs((m1[:, 0] - m0[:, 0]) / m0[:, 0]))
     edrift = np.max(np.abs((m1[:, 1] - m0[:, 1]) / m0[:, 1]))
     return dict(finals=finals, ms_per_step=tot_ms / steps, value=zones * steps / (tot_ms * 1e-3),
-                graph_value=zones * steps / (g1 - g0), avg_launch_ms=(st1_ms + st2_ms) / (2 * steps),
+                graph_value=zones * steps / (g1 - g0), avg_launch_ms=(st1_ms + st2_ms) / (launches * steps),
+                launches=launches,
                 stage_ms=(st1_ms / steps, st2_ms / steps), mass_drift=float(drift), energy_drift=float(edrift),
                 counters=ctx.counters(), tuned_ms=tuned_ms,
-                tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D)])
+                tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D)])
 
 
-def cpu_baseline(name, W, steps):
-    """CPU oracle, reference optimisation flags (-O3 -ffast-math), 1 thread."""
+def cpu_baseline(name, W, budget_s):
+    """CPU oracle, reference optimisation flags (-O3 -ffast-math), 1 thread,
+    on a bounded sample: one step of every sample problem to calibrate, then
+    as many more steps as fill about budget_s seconds."""
     from oracle import pyoracle as po
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from parity_util import oracle_for
@@ -128,15 +139,22 @@ def cpu_baseline(name, W, steps):
     else:
         sample = W["probs"]
         what = "the full grid"
-    zones = 0
-    el = 0.0
+    sims, zones = [], 0
     for prob, prims in sample:
         sim = oracle_for(prob, fast=True)
         sim.set_prims(prims)
-        t0 = time.perf_counter()
-        sim.run(max_steps=steps)
-        el += time.perf_counter() - t0
+        sims.append(sim)
         zones += int(np.prod(prims.shape[1:]))
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.run(max_steps=1)
+    t1 = time.perf_counter() - t0
+    more = int(max(0, min(W["steps"] - 1, round(budget_s / max(t1, 1e-9)) - 1)))
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.run(max_steps=more)
+    el = t1 + (time.perf_counter() - t0)
+    steps = 1 + more
     return dict(value=zones * steps / el, unit="zone-updates/s", cores=1, kind="port",
                 sample=f"{what}, {steps} SSP-RK2 steps from the initial state, 1 thread, oracle built with "
                        f"-O3 -ffast-math -ftree-vectorize -fpeel-loops; {el:.1f} s")
@@ -158,7 +176,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="kh2d", choices=["kh2d", "rt3d", "sod1d", "ensemble"])
     ap.add_argument("--n", type=int, default=None, help="grid size (problem count for ensemble)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time budget")
+    ap.add_argument("--tiles", default=None,
+                    help="ROWS2D,TY3D,KC3D[,FUSE1D] launch shape to use instead of autotuning (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=None, metavar="BYTES",
+                    help="only run the device copy kernel over BYTES (counter calibration) and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     args = ap.parse_args()
@@ -172,6 +194,13 @@ def main():
         from medusa_amd import build
         build.build(verbose=False)
 
+    if args.copy_probe:
+        ms = mds.copy_probe_ms(args.copy_probe, 5)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms,
+                          "GB/s": 2.0 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    tiles = [int(t) for t in args.tiles.split(",")] if args.tiles else None
+
     log(f"[bench] workload {args.workload}: building grids and initial state")
     W = workload(args.workload, args.n)
     steps = args.steps if args.steps is not None else W["steps"]
@@ -184,7 +213,7 @@ def main():
         if args.only and args.only != name:
             continue
         log(f"[bench] {name}: warmup {args.warmup} steps, timed {steps} steps")
-        res[name] = r = run_path(mds, W, prec, steps, args.warmup)
+        res[name] = r = run_path(mds, W, prec, steps, args.warmup, tiles)
         log(f"[bench] {name}: {r['value']:.4e} zone-updates/s, {r['ms_per_step']:.4f} ms/step, "
             f"graph {r['graph_value']:.4e}, launch {r['avg_launch_ms'] * 1e3:.1f} us, "
             f"mass drift {r['mass_drift']:.2e}, tiles {r['tiles']}")
@@ -204,7 +233,7 @@ def main():
         log(f"[bench] FP32-bulk vs FP64 relative L1 (max over problems) after {args.warmup + steps} steps: {err}")
 
     bpz = 5 * W["nvar"] * (4 if head_name == "fp32" else 8)      # algorithmic bytes per zone-update
-    bpl = bpz / 2.0 * zones                                        # per stage launch
+    bpl = bpz / head["launches"] * zones                           # per sweep launch
     achieved = bpl / (head["avg_launch_ms"] * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.workload, head_name),
@@ -213,8 +242,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU baseline: oracle, {args.cpu_steps} steps")
-        cpu = cpu_baseline(args.workload, W, args.cpu_steps)
+        log(f"[bench] CPU baseline: oracle, about {args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(args.workload, W, args.cpu_seconds)
         log(f"[bench] CPU baseline {cpu['value']:.4e} zone-updates/s")
 
     out = {
@@ -239,7 +268,7 @@ def main():
         r = res["fp64"]
         out["fp64"] = dict(value=r["value"], ms_per_step=r["ms_per_step"], graph_value=r["graph_value"],
                            avg_launch_us=r["avg_launch_ms"] * 1e3,
-                           achieved_gbs=5 * W["nvar"] * 8 / 2.0 * zones / (r["avg_launch_ms"] * 1e-3) / 1e9,
+                           achieved_gbs=5 * W["nvar"] * 8 / r["launches"] * zones / (r["avg_launch_ms"] * 1e-3) / 1e9,
                            mass_drift=r["mass_drift"], energy_drift=r["energy_drift"], tiles=r["tiles"])
     if "fp32" in res:
         r = res["fp32"]

@@ -133,7 +133,8 @@ typedef struct mds_problem {
   double g0[3];            /* fixed gravity, contravariant (gravity.c:46-64); zeros = none */
   int nvel;                /* velocity components carried; 0 = auto: ndim, except FP64
                               spherical columns, which carry 3 (the connection terms give
-                              the transverse components round-off sources) */
+                              the transverse components round-off sources); FP32-bulk
+                              spherical columns carry the radial component only */
 } mds_problem;
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */
@@ -183,6 +184,8 @@ int mds_sync(mds_ctx *ctx);
 #define MDS_TILE_ROWS2D  0       /* rows marched by one wavefront in the 2D sweep */
 #define MDS_TILE_TY3D    1       /* output rows per workgroup in the 3D sweep (4 or 12) */
 #define MDS_TILE_KC3D    2       /* planes marched by one workgroup in the 3D sweep */
+#define MDS_TILE_FUSE1D  3       /* 1D: 1 = both RK stages of a step in one launch (default
+                                    when no problem is periodic in x), 0 = one launch per stage */
 int mds_set_tile(mds_ctx *ctx, int param, int value);
 int mds_get_tile(mds_ctx *ctx, int param, int *value);
 

@@ -38,8 +38,13 @@ struct mds_ctx {
   void *p[5] = {nullptr}, *ph[5] = {nullptr};
   DevProb *dprob = nullptr;
   DevClock *dclock = nullptr;
-  int2 *dseg = nullptr;
+  int2 *dseg = nullptr;            // 1D segment table of the single-stage kernels
   int nseg = 0;
+  int2 *dseg2 = nullptr;           // 1D segment table of the two-stage kernel
+  int nseg2 = 0;
+  int can_fuse1d = 0;              // 1D and no problem periodic in x
+  int fuse1d = 0;                  // use the two-stage 1D kernel (p -> ph, then swap)
+  int par = 0;                     // buffer parity: which allocation p currently names
   unsigned char *dflags = nullptr;
   unsigned long long *dcount = nullptr;
   long long *dzstart = nullptr;
@@ -47,10 +52,18 @@ struct mds_ctx {
   int chunk2d = 24;
   int ty3d = 4;
   int kc3d = 16;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};        // one step, per buffer parity
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   char err[512] = {0};
 };
+
+static void drop_graphs(mds_ctx *c)
+{
+  for (int k = 0; k < 2; k++) {
+    if (c->gexec[k]) { hipGraphExecDestroy(c->gexec[k]); c->gexec[k] = nullptr; }
+    if (c->graph[k]) { hipGraphDestroy(c->graph[k]); c->graph[k] = nullptr; }
+  }
+}
 
 static int set_err(mds_ctx *c, int code, const char *fmt, ...)
 {
@@ -230,13 +243,13 @@ extern "C" void *mds_stream(mds_ctx *ctx) { return ctx ? (void *)ctx->st : nullp
 extern "C" int mds_destroy(mds_ctx *c)
 {
   if (!c) return MDS_OK;
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
-  if (c->graph) hipGraphDestroy(c->graph);
+  drop_graphs(c);
   for (int k = 0; k < 5; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
   for (void *g : c->geo_allocs) hipFree(g);
   if (c->dprob) hipFree(c->dprob);
   if (c->dclock) hipFree(c->dclock);
   if (c->dseg) hipFree(c->dseg);
+  if (c->dseg2) hipFree(c->dseg2);
   if (c->dflags) hipFree(c->dflags);
   if (c->dcount) hipFree(c->dcount);
   if (c->dzstart) hipFree(c->dzstart);
@@ -273,6 +286,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->sph && c->ndim != 1) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D only");
   if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
+  if (c->sph && c->precision == MDS_FP32_BULK && c->nv != 1)
+    return set_err(c, MDS_ENOTIMPL, "FP32-bulk spherical columns carry the radial velocity only");
   HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   c->zstart.assign(nprob + 1, 0);
   c->hprob.resize(nprob);
@@ -329,9 +344,29 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     }
     if (c->sph) {
       if (!g->zone) return set_err(c, MDS_EINVAL, "problem %d: spherical grid without zone records", p);
-      int rc1 = upload_geo(c, g->zone, (size_t)(g->n[0] + 2 * MDS_NG) * MDS_ZG_COUNT * sizeof(double),
-                           (const void **)&d.zone);
+      const int nzq = g->n[0] + 2 * MDS_NG;
+      int rc1 = upload_geo(c, g->zone, (size_t)nzq * MDS_ZG_COUNT * sizeof(double), (const void **)&d.zone);
       if (rc1) return rc1;
+      if (c->precision == MDS_FP32_BULK) {
+        std::vector<ZoneF32> z32((size_t)nzq);
+        for (int q = 0; q < nzq; q++) {
+          const double *zg = g->zone + (size_t)q * MDS_ZG_COUNT;
+          ZoneF32 &z = z32[q];
+          memset(&z, 0, sizeof(z));
+          z.vol = zg[MDS_ZG_VOLUME];
+          z.area = zg[MDS_ZG_AREA0];
+          z.gcov0 = zg[MDS_ZG_GCOV00];
+          z.gcon0 = zg[MDS_ZG_GCON0];
+          z.cp = (zg[MDS_ZG_C000] + zg[MDS_ZG_C101]) + zg[MDS_ZG_C202];
+          z.c000 = (float)zg[MDS_ZG_C000];
+          z.isc0 = (float)(1.0 / zg[MDS_ZG_SCALE0]);
+          z.scalef = (float)zg[MDS_ZG_SCALEF];
+          z.gcovf0 = (float)zg[MDS_ZG_GCOVF0];
+          for (int k = 0; k < MDS_RC_COUNT; k++) z.rc[k] = (float)g->rcoef[0][(size_t)q * MDS_RC_COUNT + k];
+        }
+        rc1 = upload_geo(c, z32.data(), z32.size() * sizeof(ZoneF32), (const void **)&d.z32);
+        if (rc1) return rc1;
+      }
     }
   }
   c->nzone = c->zstart[nprob];
@@ -363,6 +398,17 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     c->nseg = (int)seg.size();
     HIPCHK(c, hipMalloc(&c->dseg, sizeof(int2) * seg.size()));
     HIPCHK(c, hipMemcpy(c->dseg, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
+    seg.clear();
+    for (int p : order)
+      for (int s = 0; s < c->hprob[p].n[0]; s += MDS_SEG1D2) seg.push_back(make_int2(p, s));
+    c->nseg2 = (int)seg.size();
+    HIPCHK(c, hipMalloc(&c->dseg2, sizeof(int2) * seg.size()));
+    HIPCHK(c, hipMemcpy(c->dseg2, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
+    // the two-stage kernel needs every ghost source zone inside the wave
+    c->can_fuse1d = 1;
+    for (int p = 0; p < nprob; p++)
+      if (c->hprob[p].bc_lo[0] == MDS_BC_PERIODIC || c->hprob[p].bc_hi[0] == MDS_BC_PERIODIC) c->can_fuse1d = 0;
+    c->fuse1d = c->can_fuse1d;
   } else {
     HIPCHK(c, hipMalloc(&c->dflags, (size_t)c->nzone));
     HIPCHK(c, hipMemset(c->dflags, 0, (size_t)c->nzone));
@@ -526,18 +572,61 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
   return MDS_OK;
 }
 
-// estimate_dt + dt policy, then step(): two RK stages (step.c:28-129)
-static int enqueue_step(mds_ctx *c)
+// both RK stages of every 1D problem in one launch: p -> ph (the caller swaps)
+static int enqueue_fused1d(mds_ctx *c)
 {
+  Sweep1DArgs a;
+  a.seg = c->dseg2;
+  a.nseg = c->nseg2;
+  a.prob = c->dprob;
+  a.clock = c->dclock;
+  a.counters = c->dcount;
+  a.rp = c->rp;
   int fp32 = c->precision == MDS_FP32_BULK;
+  if (fp32) {
+    StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+    HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, -1, c->st));
+  } else {
+    StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+    HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, -1, c->st));
+  }
+  return MDS_OK;
+}
+
+// after a two-stage launch the new state is in ph: swap the buffer names
+static void finish_step(mds_ctx *c)
+{
+  if (!c->fuse1d) return;
+  for (int k = 0; k < 5; k++) std::swap(c->p[k], c->ph[k]);
+  c->par ^= 1;
+}
+
+static void enqueue_dt(mds_ctx *c)
+{
   hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
                      c->rp);
+}
+
+// estimate_dt + dt policy, then step(): two RK stages (step.c:28-129);
+// the kernels only (the caller runs finish_step once they are enqueued)
+static int enqueue_step_kernels(mds_ctx *c)
+{
+  int fp32 = c->precision == MDS_FP32_BULK;
+  enqueue_dt(c);
   HIPCHK(c, hipGetLastError());
+  if (c->fuse1d) return enqueue_fused1d(c);
   for (int stage = 0; stage < 2; stage++) {
     int rc = enqueue_stage(c, stage, fp32);
     if (rc) return rc;
   }
   return MDS_OK;
+}
+
+static int enqueue_step(mds_ctx *c)
+{
+  int rc = enqueue_step_kernels(c);
+  if (rc == MDS_OK) finish_step(c);
+  return rc;
 }
 
 extern "C" int mds_run(mds_ctx *c, int nsteps)
@@ -553,17 +642,22 @@ extern "C" int mds_run(mds_ctx *c, int nsteps)
 extern "C" int mds_run_graph(mds_ctx *c, int nsteps)
 {
   if (!c || nsteps < 0) return MDS_EINVAL;
-  if (!c->gexec) {
-    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->st, &g);
-    if (rc) { if (g) hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    c->graph = g;
-    HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  for (int s = 0; s < nsteps; s++) {
+    // one graph per buffer parity: the two-stage 1D step alternates p and ph
+    const int k = c->fuse1d ? c->par : 0;
+    if (!c->gexec[k]) {
+      HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step_kernels(c);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->st, &g);
+      if (rc) { if (g) hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+      c->graph[k] = g;
+      HIPCHK(c, hipGraphInstantiate(&c->gexec[k], c->graph[k], nullptr, nullptr, 0));
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec[k], c->st));
+    finish_step(c);
   }
-  for (int s = 0; s < nsteps; s++) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
   return MDS_OK;
 }
 
@@ -579,13 +673,20 @@ extern "C" int mds_run_timed(mds_ctx *c, int nsteps, double ms[3])
     int fp32 = c->precision == MDS_FP32_BULK;
     hipError_t e = hipEventRecord(ev[0], c->st);
     for (int s = 0; s < nsteps && e == hipSuccess && rc == MDS_OK; s++) {
-      hipLaunchKernelGGL(dt_policy_kernel, dim3((c->nprob + 127) / 128), dim3(128), 0, c->st, c->dclock, c->nprob,
-                         c->rp);
-      for (int stage = 0; stage < 2 && e == hipSuccess; stage++) {
-        e = hipEventRecord(ev[3 * s + 1 + stage], c->st);
-        if (e != hipSuccess) break;
-        rc = enqueue_stage(c, stage, fp32);
-        if (rc) break;
+      enqueue_dt(c);
+      if (c->fuse1d) {
+        // one two-stage launch: ms[1] gets its duration, ms[2] stays ~0
+        e = hipEventRecord(ev[3 * s + 1], c->st);
+        if (e == hipSuccess) rc = enqueue_fused1d(c);
+        if (e == hipSuccess && rc == MDS_OK) e = hipEventRecord(ev[3 * s + 2], c->st);
+        if (e == hipSuccess && rc == MDS_OK) finish_step(c);
+      } else {
+        for (int stage = 0; stage < 2 && e == hipSuccess; stage++) {
+          e = hipEventRecord(ev[3 * s + 1 + stage], c->st);
+          if (e != hipSuccess) break;
+          rc = enqueue_stage(c, stage, fp32);
+          if (rc) break;
+        }
       }
       if (e == hipSuccess && rc == MDS_OK) e = hipEventRecord(ev[3 * s + 3], c->st);
     }
@@ -658,13 +759,17 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
   } else if (param == MDS_TILE_KC3D) {
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "planes per workgroup out of range: %d", value);
     slot = &c->kc3d;
+  } else if (param == MDS_TILE_FUSE1D) {
+    if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "two-stage 1D switch must be 0 or 1: %d", value);
+    if (value && !c->can_fuse1d)
+      return set_err(c, MDS_EINVAL, "the two-stage 1D kernel needs 1D problems without periodic x boundaries");
+    slot = &c->fuse1d;
   } else {
     return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
   }
   if (*slot != value) {
     *slot = value;
-    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    drop_graphs(c);
   }
   return MDS_OK;
 }
@@ -675,6 +780,7 @@ extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
   if (param == MDS_TILE_ROWS2D) { *value = c->chunk2d; return MDS_OK; }
   if (param == MDS_TILE_TY3D) { *value = c->ty3d; return MDS_OK; }
   if (param == MDS_TILE_KC3D) { *value = c->kc3d; return MDS_OK; }
+  if (param == MDS_TILE_FUSE1D) { *value = c->fuse1d; return MDS_OK; }
   return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
 }
 
@@ -733,10 +839,13 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
-  if (c->ndim == 1) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // the 1D sweep has a fixed shape
-  // candidate shapes: 2D rows per wavefront; 3D (rows per workgroup, planes per workgroup)
+  // candidate shapes: 1D single/two-stage kernels; 2D rows per wavefront;
+  // 3D (rows per workgroup, planes per workgroup)
   std::vector<std::pair<int, int>> cand;
-  if (c->ndim == 2) {
+  if (c->ndim == 1) {
+    cand.push_back({0, 0});
+    if (c->can_fuse1d) cand.push_back({1, 0});
+  } else if (c->ndim == 2) {
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
       if (v <= c->hprob[0].n[1]) cand.push_back({v, 0});
   } else {
@@ -747,11 +856,13 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
   Snapshot s;
   int rc = snap_alloc(c, s);
   if (rc == MDS_OK) rc = snap_copy(c, s, true);
-  std::pair<int, int> best = c->ndim == 2 ? std::make_pair(c->chunk2d, 0) : std::make_pair(c->ty3d, c->kc3d);
+  std::pair<int, int> best = c->ndim == 1 ? std::make_pair(c->fuse1d, 0)
+                             : c->ndim == 2 ? std::make_pair(c->chunk2d, 0) : std::make_pair(c->ty3d, c->kc3d);
   double bms = 1e300;
   for (auto v : cand) {
     if (rc) break;
-    if (c->ndim == 2) c->chunk2d = v.first;
+    if (c->ndim == 1) c->fuse1d = v.first;
+    else if (c->ndim == 2) c->chunk2d = v.first;
     else { c->ty3d = v.first; c->kc3d = v.second; }
     double ms[3];
     rc = mds_run_timed(c, 1, ms);      // warm the shape
@@ -760,10 +871,10 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     if (rc == MDS_OK) rc = snap_copy(c, s, false);
   }
   snap_free(s);
-  if (c->ndim == 2) c->chunk2d = best.first;
+  if (c->ndim == 1) c->fuse1d = best.first;
+  else if (c->ndim == 2) c->chunk2d = best.first;
   else { c->ty3d = best.first; c->kc3d = best.second; }
-  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+  drop_graphs(c);
   if (best_ms) *best_ms = bms;
   return rc;
 }

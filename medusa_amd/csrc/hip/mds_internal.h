@@ -4,6 +4,24 @@
 #include <hip/hip_runtime.h>
 #include "../../../include/mds_hydro.h"
 
+// Compact per-zone geometry of a spherical column for the FP32-bulk path:
+// the FP64 factors the update needs (volume, lower-face area, zone metric,
+// the combined pressure coefficient of the connection terms) and the FP32
+// factors the Riemann solve and the CFL bound need, plus the reconstruction
+// coefficients.  96 bytes, read with six 16-byte loads.
+struct __align__(16) ZoneF32 {
+  double vol;              // zone_geom.volume
+  double area;             // area[0] of the lower radial face
+  double gcov0, gcon0;     // zone-averaged gcov[0][0], gcon[0]
+  double cp;               // conn[0][0][0] + conn[1][0][1] + conn[2][0][2]
+  float c000;              // conn[0][0][0]
+  float isc0;              // 1 / scale[0][0]
+  float scalef, gcovf0;    // lower face: scale[1][0], gcov[1][0]
+  float rc[MDS_RC_COUNT];  // FP32 reconstruction coefficients
+  float pad[2];
+};
+static_assert(sizeof(ZoneF32) == 96, "ZoneF32 layout");
+
 // One problem as the kernels see it.
 struct DevProb {
   long long off;           // first interior zone in the packed state arrays
@@ -18,6 +36,7 @@ struct DevProb {
   const double *mom[3];    // per direction (n[d]+2NG)*3: alpha, beta, Gamma per zone
   const float *rc[3];      // per direction (n[d]+2NG)*MDS_RC_COUNT
   const double *zone;      // spherical: (n[0]+2NG)*MDS_ZG_COUNT
+  const ZoneF32 *z32;      // spherical, FP32-bulk: n[0]+2NG records
 };
 
 // Device-resident clock of one problem (the reference globals t, dt, istep,
@@ -66,4 +85,5 @@ struct Sweep3DArgs {
   int src;                 // source terms present (fixed gravity)
 };
 
-#define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D sweep
+#define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D stage sweep
+#define MDS_SEG1D2 56      // output zones per 64-lane wave in the 1D two-stage sweep

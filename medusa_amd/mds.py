@@ -21,6 +21,7 @@ RC_COUNT = 8
 TILE_ROWS2D = 0
 TILE_TY3D = 1
 TILE_KC3D = 2
+TILE_FUSE1D = 3
 ZG_COUNT = 24
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -218,6 +219,7 @@ class Context:
         self.h = h
         self.precision = precision
         self.nprob = len(problems)
+        self.ndim = problems[0]["grid"].g.ndim
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:

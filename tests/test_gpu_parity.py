@@ -342,3 +342,82 @@ def test_results_independent_of_launch_shape(precision):
         assert np.array_equal(o, outs[0])
     for o in outs3[1:]:
         assert np.array_equal(o, outs3[0])
+
+
+def _batch_ctx(ens, precision, fuse, tmax=None):
+    rp = mds.run_params(cfl=0.5) if tmax is None else mds.run_params(cfl=0.5, tmax=tmax)
+    ctx = mds.Context([e[1] for e in ens], precision=precision, rp=rp)
+    ctx.set_tile(mds.TILE_FUSE1D, fuse)
+    for k, e in enumerate(ens):
+        ctx.upload(e[2], k)
+    ctx.update_eos()
+    return ctx
+
+
+@pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
+def test_two_stage_1d_kernel_matches_single_stage(precision):
+    """The two-stage 1D kernel (one launch per step, buffers swapped) gives
+    the same bits as the one-launch-per-stage kernels, over odd step counts,
+    graph replays of both buffer parities and segment boundaries."""
+    par, ens = problems.ensemble(5, 60, 300, seed=77)
+    g, prob, prims = problems.sod(224)          # 224 = 4 two-stage segments exactly
+    ens = list(ens)
+    for case, batch in (("ensemble", ens), ("sod224", [(g, prob, prims)])):
+        outs = []
+        for fuse in (0, 1):
+            say(f"{case} precision={precision} two-stage={fuse}")
+            ctx = _batch_ctx(batch, precision, fuse)
+            ctx.run(7)
+            ctx.run(6, graph=True)
+            ctx.run(3)
+            ctx.run(1, graph=True)
+            outs.append(([ctx.download(k) for k in range(len(batch))],
+                         [ctx.clock(k) for k in range(len(batch))], ctx.counters()))
+        for a, b in zip(outs[0][0], outs[1][0]):
+            assert np.array_equal(a, b)
+        assert outs[0][1] == outs[1][1]
+        assert outs[0][2] == outs[1][2]
+
+
+def test_batched_columns_fp64_match_oracle_through_freeze():
+    """A batch of spherical columns with per-problem dt, run until every one
+    reaches tmax (the ones that get there first are carried over unchanged):
+    each column equals its own oracle run."""
+    par, ens = problems.ensemble(6, 64, 400, seed=99)
+    ts = []
+    for g, prob, prims in ens:
+        sim = oracle_for(prob)
+        sim.set_prims(prims)
+        sim.run(max_steps=25)
+        ts.append(sim.t)
+    tmax = float(np.median(ts))
+    say(f"6 columns FP64 to tmax={tmax:.4e} (median of the 25-step times)")
+    ctx = _batch_ctx(ens, mds.FP64, 1, tmax=tmax)
+    ctx.run_to(batch=8)
+    for k, (g, prob, prims) in enumerate(ens):
+        sim = oracle_for(prob)
+        sim.set_prims(prims)
+        sim.run(tmax=tmax)
+        dev = ctx.download(k)
+        t, dt, ist = ctx.clock(k)
+        err = compare(dev, sim.get_prims(), prob["gam"], w=g.zone()[2:-2, 0])
+        say(f"  column {k} n={g.n[0]}: steps {ist} vs {sim.istep}, t {t!r} vs {sim.t!r}, L1 {err}")
+        assert ist == sim.istep and t == sim.t
+        for kk, v in err.items():
+            assert v <= FP64_TOL, (k, kk, v)
+
+
+def test_batched_columns_fp32_within_tolerance():
+    par, ens = problems.ensemble(6, 64, 800, seed=123)
+    say("6 columns FP32-bulk, 120 steps, against the FP64 oracle")
+    ctx = _batch_ctx(ens, mds.FP32_BULK, 1)
+    ctx.run(120, graph=True)
+    for k, (g, prob, prims) in enumerate(ens):
+        sim = oracle_for(prob)
+        sim.set_prims(prims)
+        sim.run(max_steps=120)
+        w = g.zone()[2:-2, 0]
+        err = compare(ctx.download(k), sim.get_prims(), prob["gam"], w=w)
+        say(f"  column {k} n={g.n[0]}: L1 {err}")
+        assert err["rho"] <= FP32_RHO_TOL
+        assert err["p"] <= 10 * FP32_RHO_TOL
