@@ -1,0 +1,71 @@
+/*
+ * mds_eos.h -- tabulated equation of state of the hydro sweep.
+ *
+ * The table follows the layout of the reference's Burrows table
+ * (eos/burrows/eos_stuff.c:7, TAB_ELEM: record, Ye, log rho, log T with T
+ * fastest) and its use (eos_given_rex_tianshu, eos_stuff.c:507-650):
+ * bilinear weights in (log10 rho, Ye), the temperature bracket of the
+ * specific internal energy along log T, the exact inverse of the piecewise
+ * linear e(log T), then P, Gamma_1 and cs = sqrt(Gamma_1 P / rho) at the
+ * recovered temperature.  Values are stored in FP32; the inversion runs in
+ * FP64.  The reference walks the bracket from the previous temperature; the
+ * device finds the same bracket by bisection over the T nodes (e is strictly
+ * increasing in T), so no temperature state is carried.  Off-table (rho, Ye)
+ * are clamped to the table edges (the reference reads outside the table).
+ *
+ * The table of this repository is synthetic: a seeded smooth analytic
+ * electron-degeneracy + thermal electrons + ideal ions + radiation model,
+ * evaluated in extended precision at the nodes (no real table can be loaded
+ * offline).
+ */
+#ifndef MDS_EOS_H
+#define MDS_EOS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* records of the table (reference record numbers in brackets) */
+#define MDS_EOS_E       0   /* specific internal energy e [erg/g], linear (record 1) */
+#define MDS_EOS_LNP     1   /* ln P [dyn/cm^2] (record 2, log-transformed at load) */
+#define MDS_EOS_GAM1    2   /* adiabatic index Gamma_1 (record 12) */
+#define MDS_EOS_NFIELD  3
+
+typedef struct mds_eos_table {
+  int nr, nt, ny;                        /* nodes in log10 rho, log10 T [K], Ye */
+  double lr0, lr1;                       /* first and last log10 rho node */
+  double lt0, lt1;                       /* first and last log10 T node */
+  double ye0, ye1;                       /* first and last Ye node */
+  float *data;                           /* MDS_EOS_NFIELD * ny * nr * nt, T fastest */
+} mds_eos_table;
+
+/* Default synthetic table dimensions: 180 x 180 x 50 over log10 rho in
+ * [3,15], log10 T in [9,12], Ye in [0.05,0.55]. */
+void mds_eos_table_default(mds_eos_table *t);
+
+/* Model parameters drawn from the seeded stream (uniform jitter k = 0,1,2):
+ * out = {A (mean ion mass number), K53 (degenerate-electron constant, cgs),
+ *        rho_x (relativistic transition of rho*Ye, g/cm^3)}. */
+void mds_eos_model_params(unsigned long long seed, double out[3]);
+
+/* Evaluate the analytic model in extended precision at one point:
+ * out = {P, e, Gamma_1}. */
+void mds_eos_model_eval(const double prm[3], double rho, double temp, double ye, double out[3]);
+
+/* Fill t->data (allocated here; dimensions and ranges must be set) from the
+ * model with parameters mds_eos_model_params(seed).  Host threads. */
+int mds_eos_table_build(mds_eos_table *t, unsigned long long seed);
+void mds_eos_table_free(mds_eos_table *t);
+
+/* Pointwise device EOS (parity entry point; replaces eos_fill /
+ * eos_given_rex_per_zone, eos_stuff.c:768-901): for each of n states
+ * (rho, u = rho*e, Ye) -> P, cs, T [K], Gamma_1.  precision: MDS_FP64 (all
+ * FP64 after the FP32 table reads) or MDS_FP32_BULK (FP64 inversion, FP32
+ * interpolation of ln P increments and Gamma_1).  Any output may be NULL. */
+int mds_eos_points(int precision, const mds_eos_table *t, int n, const double *rho, const double *u,
+                   const double *ye, double *press, double *cs, double *temp, double *gam1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,119 @@
+/*
+ * mds_eos_table.c -- synthetic tabulated EOS (host C).
+ *
+ * Model, per unit mass, at density rho, temperature T and electron fraction
+ * Ye (cgs, extended precision):
+ *   degenerate electrons  P_d = K53 (rho Ye)^(5/3) / (1 + x),  x = (rho Ye / rho_x)^(1/3)
+ *                         e_d = P_d / (rho (g_d - 1)),  g_d = 5/3 - (1/3) x / (1 + x)
+ *   thermal electrons     P_e = n_e k T * T / (T + T_F),  T_F = 3e9 K (rho Ye / 1e6)^(2/3) / (1 + x)
+ *                         e_e = 1.5 P_e / rho
+ *   ions                  P_i = rho k T / (A m_u),  e_i = 1.5 P_i / rho
+ *   radiation             P_r = a T^4 / 3,  e_r = a T^4 / rho
+ * P and e are sums of the four; e is strictly increasing in T.  Gamma_1 =
+ * chi_rho + P chi_T^2 / (rho T c_v) from central differences of the model.
+ * A, K53 and rho_x carry a seeded +-10%, +-5%, +-10% jitter.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../../include/mds_hydro.h"
+#include "../../../include/mds_host.h"
+#include "../../../include/mds_eos.h"
+
+void mds_eos_table_default(mds_eos_table *t)
+{
+  memset(t, 0, sizeof(*t));
+  t->nr = 180;  t->lr0 = 3.0;  t->lr1 = 15.0;
+  t->nt = 180;  t->lt0 = 9.0;  t->lt1 = 12.0;
+  t->ny = 50;   t->ye0 = 0.05; t->ye1 = 0.55;
+}
+
+void mds_eos_model_params(unsigned long long seed, double out[3])
+{
+  double u0 = 2.0 * mds_uniform(seed, 0) - 1.0;
+  double u1 = 2.0 * mds_uniform(seed, 1) - 1.0;
+  double u2 = 2.0 * mds_uniform(seed, 2) - 1.0;
+  out[0] = 56.0 * (1.0 + 0.10 * u0);
+  out[1] = 1.0036e13 * (1.0 + 0.05 * u1);
+  out[2] = 2.0e6 * (1.0 + 0.10 * u2);
+}
+
+static void model(const double prm[3], long double rho, long double T, long double ye, long double *P,
+                  long double *e)
+{
+  const long double kB = 1.380649e-16L, mu = 1.66053906660e-24L, arad = 7.565723e-15L;
+  const long double A = prm[0], K53 = prm[1], rhox = prm[2];
+  const long double rye = rho * ye;
+  const long double x = cbrtl(rye / rhox);
+  const long double Pd = K53 * powl(rye, 5.0L / 3.0L) / (1.0L + x);
+  const long double gd = 5.0L / 3.0L - (x / (1.0L + x)) / 3.0L;
+  const long double ed = Pd / (rho * (gd - 1.0L));
+  const long double TF = 3.0e9L * powl(rye / 1.0e6L, 2.0L / 3.0L) / (1.0L + x);
+  const long double ne = rye / mu;
+  const long double Pe = ne * kB * T * (T / (T + TF));
+  const long double Pi = rho * kB * T / (A * mu);
+  const long double T4 = (T * T) * (T * T);
+  const long double Pr = arad * T4 / 3.0L;
+  *P = ((Pd + Pe) + Pi) + Pr;
+  *e = ((ed + 1.5L * Pe / rho) + 1.5L * Pi / rho) + arad * T4 / rho;
+}
+
+void mds_eos_model_eval(const double prm[3], double rho, double temp, double ye, double out[3])
+{
+  long double P, e, Pp, Pm, ep, em, d;
+  const long double h = 1.0e-5L;
+  model(prm, rho, temp, ye, &P, &e);
+  // chi_rho = dlnP/dlnrho at fixed T
+  model(prm, rho * expl(h), temp, ye, &Pp, &d);
+  model(prm, rho * expl(-h), temp, ye, &Pm, &d);
+  const long double chr = (logl(Pp) - logl(Pm)) / (2.0L * h);
+  // chi_T = dlnP/dlnT and c_v = de/dT at fixed rho
+  model(prm, rho, temp * expl(h), ye, &Pp, &ep);
+  model(prm, rho, temp * expl(-h), ye, &Pm, &em);
+  const long double cht = (logl(Pp) - logl(Pm)) / (2.0L * h);
+  const long double cv = (ep - em) / ((long double)temp * (expl(h) - expl(-h)));
+  const long double g1 = chr + P * cht * cht / ((long double)rho * (long double)temp * cv);
+  out[0] = (double)P;
+  out[1] = (double)e;
+  out[2] = (double)g1;
+}
+
+int mds_eos_table_build(mds_eos_table *t, unsigned long long seed)
+{
+  if (!t || t->nr < 2 || t->nt < 3 || t->ny < 2) return MDS_EINVAL;
+  if (!(t->lr1 > t->lr0) || !(t->lt1 > t->lt0) || !(t->ye1 > t->ye0)) return MDS_EINVAL;
+  const size_t rec = (size_t)t->ny * t->nr * t->nt;
+  t->data = (float *)malloc(MDS_EOS_NFIELD * rec * sizeof(float));
+  if (!t->data) return MDS_ENOMEM;
+  double prm[3];
+  mds_eos_model_params(seed, prm);
+  const double dlr = (t->lr1 - t->lr0) / (t->nr - 1);
+  const double dlt = (t->lt1 - t->lt0) / (t->nt - 1);
+  const double dye = (t->ye1 - t->ye0) / (t->ny - 1);
+  const int ny = t->ny, nr = t->nr, nt = t->nt;
+  float *data = t->data;
+#pragma omp parallel for schedule(static) collapse(2)
+  for (int jy = 0; jy < ny; jy++) {
+    for (int jr = 0; jr < nr; jr++) {
+      const double ye = t->ye0 + jy * dye;
+      const double rho = pow(10.0, t->lr0 + jr * dlr);
+      for (int jt = 0; jt < nt; jt++) {
+        const double temp = pow(10.0, t->lt0 + jt * dlt);
+        double v[3];
+        mds_eos_model_eval(prm, rho, temp, ye, v);
+        const size_t q = ((size_t)jy * nr + jr) * nt + jt;
+        data[MDS_EOS_E * rec + q] = (float)v[1];
+        data[MDS_EOS_LNP * rec + q] = (float)log(v[0]);
+        data[MDS_EOS_GAM1 * rec + q] = (float)v[2];
+      }
+    }
+  }
+  return MDS_OK;
+}
+
+void mds_eos_table_free(mds_eos_table *t)
+{
+  if (!t) return;
+  free(t->data);
+  t->data = NULL;
+}

@@ -62,6 +62,14 @@ class RunParams(ctypes.Structure):
                 ("tmax", ctypes.c_double)]
 
 
+class EosTableDesc(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("nt", ctypes.c_int), ("ny", ctypes.c_int),
+                ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
+                ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
+                ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
+                ("data", ctypes.POINTER(ctypes.c_float))]
+
+
 _LIB = None
 
 EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
@@ -70,7 +78,9 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
             "mds_set_tile", "mds_get_tile", "mds_autotune",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
-            "mds_init_sph_riemann", "mds_ensemble_params"]
+            "mds_init_sph_riemann", "mds_ensemble_params",
+            "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
+            "mds_eos_table_free", "mds_eos_points"]
 
 
 def lib():
@@ -113,6 +123,17 @@ def lib():
     L.mds_recon_points.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mds_riemann_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.mds_eos_table_default.argtypes = [ctypes.POINTER(EosTableDesc)]
+    L.mds_eos_table_default.restype = None
+    L.mds_eos_model_params.argtypes = [ctypes.c_ulonglong, _dp]
+    L.mds_eos_model_params.restype = None
+    L.mds_eos_model_eval.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+    L.mds_eos_model_eval.restype = None
+    L.mds_eos_table_build.argtypes = [ctypes.POINTER(EosTableDesc), ctypes.c_ulonglong]
+    L.mds_eos_table_free.argtypes = [ctypes.POINTER(EosTableDesc)]
+    L.mds_eos_table_free.restype = None
+    L.mds_eos_points.argtypes = [ctypes.c_int, ctypes.POINTER(EosTableDesc), ctypes.c_int, _dp, _dp, _dp,
+                                 _dp, _dp, _dp, _dp]
     _LIB = L
     return L
 
@@ -392,3 +413,68 @@ def ensemble_params(nprob, nmin=256, nmax=8192, seed=0x5EED0004):
     if _host().mds_ensemble_params(nprob, nmin, nmax, seed, _ptr(out)):
         raise RuntimeError("mds_ensemble_params failed")
     return out
+
+
+# --------------------------------------------------------------------------
+# tabulated EOS
+# --------------------------------------------------------------------------
+EOS_SEED = 0x5EED0005
+
+
+class EosTable:
+    """Synthetic tabulated EOS (mds_eos_table): host FP32 records
+    [MDS_EOS_NFIELD][ny][nr][nt] (e, ln P, Gamma_1)."""
+
+    NFIELD = 3
+
+    def __init__(self, seed=EOS_SEED, **dims):
+        self.t = EosTableDesc()
+        lib().mds_eos_table_default(ctypes.byref(self.t))
+        for k, v in dims.items():
+            setattr(self.t, k, v)
+        self.seed = seed
+        rc = lib().mds_eos_table_build(ctypes.byref(self.t), seed)
+        if rc:
+            raise RuntimeError(f"mds_eos_table_build failed ({rc})")
+        n = self.NFIELD * self.t.ny * self.t.nr * self.t.nt
+        self.data = np.ctypeslib.as_array(self.t.data, shape=(n,)).reshape(self.NFIELD, self.t.ny, self.t.nr,
+                                                                           self.t.nt)
+
+    def __del__(self):
+        if getattr(self, "t", None) is not None and _LIB is not None and self.t.data:
+            _LIB.mds_eos_table_free(ctypes.byref(self.t))
+
+    def nodes(self):
+        """(log10 rho, log10 T, Ye) node coordinates."""
+        t = self.t
+        return (t.lr0 + np.arange(t.nr) * ((t.lr1 - t.lr0) / (t.nr - 1)),
+                t.lt0 + np.arange(t.nt) * ((t.lt1 - t.lt0) / (t.nt - 1)),
+                t.ye0 + np.arange(t.ny) * ((t.ye1 - t.ye0) / (t.ny - 1)))
+
+
+def eos_model_params(seed=EOS_SEED):
+    out = np.zeros(3)
+    lib().mds_eos_model_params(seed, _ptr(out))
+    return out
+
+
+def eos_model_eval(prm, rho, temp, ye):
+    """(P, e, Gamma_1) of the analytic model at one point (extended precision)."""
+    out = np.zeros(3)
+    prm = np.ascontiguousarray(prm, dtype=np.float64)
+    lib().mds_eos_model_eval(_ptr(prm), rho, temp, ye, _ptr(out))
+    return out
+
+
+def eos_points(precision, table, rho, u, ye):
+    """Device EOS fill of arrays of states: (P, cs, T, Gamma_1)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    ye = np.ascontiguousarray(ye, dtype=np.float64)
+    n = rho.size
+    outs = [np.zeros(n) for _ in range(4)]
+    rc = lib().mds_eos_points(precision, ctypes.byref(table.t), n, _ptr(rho), _ptr(u), _ptr(ye),
+                              *[_ptr(o) for o in outs])
+    if rc:
+        raise RuntimeError(f"mds_eos_points failed ({rc})")
+    return tuple(outs)

@@ -79,6 +79,21 @@ typedef struct {
 
 typedef struct orc_sim orc_sim;
 
+/* tabulated EOS table (layout of eos/burrows/eos_stuff.c:7: record, Ye,
+ * rho, T with T fastest; records 0 e, 1 ln P, 2 Gamma_1), FP32 values */
+typedef struct {
+  int nr, nt, ny;
+  double lr0, lr1, lt0, lt1, ye0, ye1;
+  const float *data;
+} orc_eos_tab;
+
+/* eos_stuff.c:507-650: out = {P, cs, T, Gamma_1}; returns off-table bits
+ * (1 below, 2 above the energy range) */
+int orc_eos_walk(const orc_eos_tab *t, double rho, double u, double ye, double temp_guess, double out[4]);
+/* eos_stellar_collapse.c:294-369 on the same interpolant: log10 T of the root */
+int orc_eos_root(const orc_eos_tab *t, double rho, double u, double ye, double lt_guess, double tol,
+                 double *lt_out, int *newton_iters);
+
 /* ---- pointwise kernels (reconstruct.c, riemann.c, fluid.c) ---- */
 void orc_para_recon(const double *a, double *al, double *ar,
                     const double *alpha, const double *beta, const double *Gamma,

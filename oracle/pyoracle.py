@@ -94,10 +94,49 @@ def _load(fast=False):
     lib.orc_geom_src.argtypes = [dp, ctypes.c_double, dp, ctypes.POINTER(ZoneGeom), ctypes.c_int, ctypes.c_int]
     lib.orc_dt_source_control.argtypes = [dp, dp, dp]
     lib.orc_dt_source_control.restype = ctypes.c_double
+    lib.orc_eos_walk.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, dp]
+    lib.orc_eos_walk.restype = ctypes.c_int
+    lib.orc_eos_root.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_int)]
+    lib.orc_eos_root.restype = ctypes.c_int
     return lib
 
 
 _LIBS = {}
+
+
+class EosTab(ctypes.Structure):
+    """orc_eos_tab: dimensions, node ranges and the FP32 records."""
+    _fields_ = [("nr", ctypes.c_int), ("nt", ctypes.c_int), ("ny", ctypes.c_int),
+                ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
+                ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
+                ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
+                ("data", ctypes.POINTER(ctypes.c_float))]
+
+
+def eos_tab(dims, data):
+    """Describe a float32 table [3][ny][nr][nt]; dims: dict of nr, nt, ny and the ranges."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    t = EosTab(**{k: dims[k] for k in ("nr", "nt", "ny", "lr0", "lr1", "lt0", "lt1", "ye0", "ye1")})
+    t.data = data.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    t._keep = data
+    return t
+
+
+def eos_walk(tab, rho, u, ye, temp_guess=0.0):
+    """eos_stuff.c:507-650 restated: (P, cs, T, Gamma_1), status."""
+    out = np.zeros(4)
+    st = lib().orc_eos_walk(ctypes.byref(tab), rho, u, ye, temp_guess, ptr(out))
+    return out, st
+
+
+def eos_root(tab, rho, u, ye, lt_guess, tol=1e-10):
+    """eos_stellar_collapse.c:294-369 restated on the same interpolant: (log10 T, status, newton iterations)."""
+    lt = np.zeros(1)
+    it = ctypes.c_int()
+    st = lib().orc_eos_root(ctypes.byref(tab), rho, u, ye, lt_guess, tol, ptr(lt), ctypes.byref(it))
+    return float(lt[0]), st, it.value
 
 
 def lib(fast=False):
