@@ -135,6 +135,11 @@ typedef struct mds_problem {
                               spherical columns, which carry 3 (the connection terms give
                               the transverse components round-off sources); FP32-bulk
                               spherical columns carry the radial component only */
+  const struct mds_eos_table *eos;  /* tabulated EOS (mds_eos.h), NULL = gamma law; with a table
+                              the state carries Ye as a sixth variable (decs.h:646).
+                              Spherical columns only; every problem of a batch shares it */
+  int monopole;            /* Newtonian monopole self-gravity (gravity.c:145-498);
+                              spherical columns only */
 } mds_problem;
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */

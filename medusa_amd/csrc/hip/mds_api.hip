@@ -11,10 +11,15 @@
 #include <algorithm>
 #include "mds_internal.h"
 #include "mds_physics.h"
+#include "mds_eos.h"
+#include "../../../include/mds_eos.h"
 
 
-extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, const Sweep1DArgs *a,
+extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int mono, const Sweep1DArgs *a,
                                          const void *sp, const void *sph_, int stage, hipStream_t st);
+extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
+                                          hipStream_t s);
+hipError_t mds_eos_to_device(const mds_eos_table *t, DevEos *d, void **alloc);
 extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a, const void *sp,
                                          const void *sph, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, const Sweep3DArgs *a, const void *sp,
@@ -35,7 +40,13 @@ struct mds_ctx {
   std::vector<long long> zstart;        // prefix sums of interior zones (nprob+1)
   std::vector<int> nvars;
   // device
-  void *p[5] = {nullptr}, *ph[5] = {nullptr};
+  void *p[6] = {nullptr}, *ph[6] = {nullptr};    // rho, u, v1..v3, Ye (Ye with a table EOS)
+  int nstate = 5;                  // state arrays per copy (6 with a table EOS)
+  int tab = 0;                     // tabulated EOS (Ye carried)
+  DevEos deos{};
+  void *eos_alloc = nullptr;
+  int mono = 0;                    // monopole gravity
+  double *dgrav = nullptr;         // monopole field at zone centres (FP64), packed like the state
   DevProb *dprob = nullptr;
   DevClock *dclock = nullptr;
   int2 *dseg = nullptr;            // 1D segment table of the single-stage kernels
@@ -118,7 +129,7 @@ __global__ void dt_policy_kernel(DevClock *ck, int nprob, RunParams rp)
 // first estimate_dt zone bound from the initial state (main.c:181 then step.c:177-187)
 template <class R>
 __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long long *zstart, int nprob,
-                               StateArrays<R> sp)
+                               StateArrays<R> sp, int tab, DevEos eos)
 {
   long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool live = z < zstart[nprob];
@@ -131,8 +142,14 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
   const DevProb &pb = prob[lo];
   long long q = z - zstart[lo];
   double rho = sp.v[0][pb.off + q], uu = sp.v[1][pb.off + q];
-  double press = (pb.gam - 1.0) * uu;
-  double cs = sqrt((pb.gam * press) / rho);
+  double cs;
+  if (tab) {
+    double press, gam1, lt;
+    eos_fill_x64(eos, rho, uu, (double)sp.v[5][pb.off + q], press, gam1, cs, lt);
+  } else {
+    double press = (pb.gam - 1.0) * uu;
+    cs = sqrt((pb.gam * press) / rho);
+  }
   int i = (int)(q % pb.n[0]);
   double loc = 0.0;
   for (int d = 0; d < pb.ndim; d++) {
@@ -211,7 +228,7 @@ template <class R>
 static StateArrays<R> arrays(void *const v[5])
 {
   StateArrays<R> s;
-  for (int k = 0; k < 5; k++) s.v[k] = (R *)v[k];
+  for (int k = 0; k < 6; k++) s.v[k] = (R *)v[k];
   return s;
 }
 
@@ -244,7 +261,9 @@ extern "C" int mds_destroy(mds_ctx *c)
 {
   if (!c) return MDS_OK;
   drop_graphs(c);
-  for (int k = 0; k < 5; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
+  for (int k = 0; k < 6; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
+  if (c->eos_alloc) hipFree(c->eos_alloc);
+  if (c->dgrav) hipFree(c->dgrav);
   for (void *g : c->geo_allocs) hipFree(g);
   if (c->dprob) hipFree(c->dprob);
   if (c->dclock) hipFree(c->dclock);
@@ -288,6 +307,20 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
   if (c->sph && c->precision == MDS_FP32_BULK && c->nv != 1)
     return set_err(c, MDS_ENOTIMPL, "FP32-bulk spherical columns carry the radial velocity only");
+  c->tab = probs[0].eos != nullptr;
+  c->mono = probs[0].monopole != 0;
+  for (int p = 1; p < nprob; p++)
+    if ((probs[p].eos != probs[0].eos) || ((probs[p].monopole != 0) != (c->mono != 0)))
+      return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same EOS and gravity", p);
+  if ((c->tab || c->mono) && !(c->sph && c->ndim == 1))
+    return set_err(c, MDS_ENOTIMPL, "tabulated EOS and monopole gravity are supported for spherical columns");
+  if (c->tab) {
+    const mds_eos_table *t = probs[0].eos;
+    if (!t->data || t->nr < 2 || t->nt < 3 || t->ny < 2 || t->nt - 1 > (1 << 20))
+      return set_err(c, MDS_EINVAL, "EOS table without data or with bad dimensions");
+    HIPCHK(c, mds_eos_to_device(t, &c->deos, &c->eos_alloc));
+    c->nstate = 6;
+  }
   HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   c->zstart.assign(nprob + 1, 0);
   c->hprob.resize(nprob);
@@ -316,6 +349,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     d.off = c->zstart[p];
     d.ndim = g->ndim;
     d.sph = c->sph;
+    d.rcoord = g->rcoord;
     d.gam = P.gam;
     for (int k = 0; k < 3; k++) {
       d.n[k] = k < g->ndim ? g->n[k] : 1;
@@ -363,6 +397,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
           z.scalef = (float)zg[MDS_ZG_SCALEF];
           z.gcovf0 = (float)zg[MDS_ZG_GCOVF0];
           for (int k = 0; k < MDS_RC_COUNT; k++) z.rc[k] = (float)g->rcoef[0][(size_t)q * MDS_RC_COUNT + k];
+          const double xlo = g->startx[0] + (q - MDS_NG) * g->dx[0];
+          z.dcen = (float)(g->beta[0][q] / g->Gamma[0][q] - xlo);
         }
         rc1 = upload_geo(c, z32.data(), z32.size() * sizeof(ZoneF32), (const void **)&d.z32);
         if (rc1) return rc1;
@@ -373,11 +409,15 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (const char *e = getenv("MDS_TILE_ROWS2D")) c->chunk2d = atoi(e) > 0 ? atoi(e) : c->chunk2d;
   if (c->ndim == 3 && c->hprob[0].n[2] < c->kc3d) c->kc3d = c->hprob[0].n[2];
   size_t rs = rsize(c);
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < c->nstate; k++) {
     HIPCHK(c, hipMalloc(&c->p[k], (size_t)c->nzone * rs));
     HIPCHK(c, hipMalloc(&c->ph[k], (size_t)c->nzone * rs));
     HIPCHK(c, hipMemset(c->p[k], 0, (size_t)c->nzone * rs));
     HIPCHK(c, hipMemset(c->ph[k], 0, (size_t)c->nzone * rs));
+  }
+  if (c->mono) {
+    HIPCHK(c, hipMalloc(&c->dgrav, sizeof(double) * (size_t)c->nzone));
+    HIPCHK(c, hipMemset(c->dgrav, 0, sizeof(double) * (size_t)c->nzone));
   }
   HIPCHK(c, hipMalloc(&c->dprob, sizeof(DevProb) * nprob));
   HIPCHK(c, hipMemcpy(c->dprob, c->hprob.data(), sizeof(DevProb) * nprob, hipMemcpyHostToDevice));
@@ -404,8 +444,9 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     c->nseg2 = (int)seg.size();
     HIPCHK(c, hipMalloc(&c->dseg2, sizeof(int2) * seg.size()));
     HIPCHK(c, hipMemcpy(c->dseg2, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
-    // the two-stage kernel needs every ghost source zone inside the wave
-    c->can_fuse1d = 1;
+    // the two-stage kernel needs every ghost source zone inside the wave, and
+    // no field that depends on the whole column between the stages
+    c->can_fuse1d = !c->mono;
     for (int p = 0; p < nprob; p++)
       if (c->hprob[p].bc_lo[0] == MDS_BC_PERIODIC || c->hprob[p].bc_hi[0] == MDS_BC_PERIODIC) c->can_fuse1d = 0;
     c->fuse1d = c->can_fuse1d;
@@ -440,10 +481,12 @@ extern "C" int mds_create(const mds_problem *probs, int nprob, const mds_run_par
 extern "C" int mds_upload_prims(mds_ctx *c, int prob, const double *prims, int nvars)
 {
   if (!c || prob < 0 || prob >= c->nprob || !prims || nvars < 2 + c->nv) return MDS_EINVAL;
+  if (c->tab && nvars < 6) return set_err(c, MDS_EINVAL, "a table EOS context needs 6 variables (Ye last)");
   long long nz = c->zstart[prob + 1] - c->zstart[prob];
   long long off = c->zstart[prob];
   HIPCHK(c, hipStreamSynchronize(c->st));
-  for (int k = 0; k < 2 + c->nv; k++) {
+  for (int k = 0; k < c->nstate; k++) {
+    if (k >= 2 + c->nv && k < 5) continue;
     const double *src = prims + (size_t)k * nz;
     if (c->precision == MDS_FP32_BULK) {
       std::vector<float> tmp(nz);
@@ -462,9 +505,9 @@ extern "C" int mds_download_prims(mds_ctx *c, int prob, double *prims, int nvars
   long long nz = c->zstart[prob + 1] - c->zstart[prob];
   long long off = c->zstart[prob];
   HIPCHK(c, hipStreamSynchronize(c->st));
-  for (int k = 0; k < nvars && k < 5; k++) {
+  for (int k = 0; k < nvars && k < c->nstate; k++) {
     double *dst = prims + (size_t)k * nz;
-    if (k >= 2 + c->nv) { for (long long q = 0; q < nz; q++) dst[q] = 0.0; continue; }
+    if (k >= 2 + c->nv && k < 5) { for (long long q = 0; q < nz; q++) dst[q] = 0.0; continue; }
     if (c->precision == MDS_FP32_BULK) {
       std::vector<float> tmp(nz);
       HIPCHK(c, hipMemcpy(tmp.data(), (float *)c->p[k] + off, nz * sizeof(float), hipMemcpyDeviceToHost));
@@ -473,7 +516,7 @@ extern "C" int mds_download_prims(mds_ctx *c, int prob, double *prims, int nvars
       HIPCHK(c, hipMemcpy(dst, (double *)c->p[k] + off, nz * sizeof(double), hipMemcpyDeviceToHost));
     }
   }
-  for (int k = 5; k < nvars; k++)
+  for (int k = c->nstate; k < nvars; k++)
     for (long long q = 0; q < nz; q++) prims[(size_t)k * nz + q] = 0.0;
   return MDS_OK;
 }
@@ -504,10 +547,10 @@ extern "C" int mds_update_eos(mds_ctx *c)
   dim3 block(256), grid((unsigned)((c->nzone + 255) / 256));
   if (c->precision == MDS_FP32_BULK) {
     hipLaunchKernelGGL(init_dt_kernel<float>, grid, block, 0, c->st, c->dprob, c->dclock, c->dzstart, c->nprob,
-                       arrays<float>(c->p));
+                       arrays<float>(c->p), c->tab, c->deos);
   } else {
     hipLaunchKernelGGL(init_dt_kernel<double>, grid, block, 0, c->st, c->dprob, c->dclock, c->dzstart,
-                       c->nprob, arrays<double>(c->p));
+                       c->nprob, arrays<double>(c->p), c->tab, c->deos);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -525,12 +568,17 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.clock = c->dclock;
     a.counters = c->dcount;
     a.rp = c->rp;
+    a.eos = c->deos;
+    a.grav = c->dgrav;
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
-      HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
+      // monopole field of the stage input state (gravity.c:145-251 at p_to_phys_src_start)
+      if (c->mono) HIPCHK(c, mds_launch_monopole(1, c->nprob, c->dprob, stage == 0 ? &sp : &sh, c->dgrav, c->st));
+      HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, c->tab, c->mono, &a, &sp, &sh, stage, c->st));
     } else {
       StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-      HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, stage, c->st));
+      if (c->mono) HIPCHK(c, mds_launch_monopole(0, c->nprob, c->dprob, stage == 0 ? &sp : &sh, c->dgrav, c->st));
+      HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, c->tab, c->mono, &a, &sp, &sh, stage, c->st));
     }
   } else if (c->ndim == 3) {
     Sweep3DArgs a;
@@ -582,13 +630,15 @@ static int enqueue_fused1d(mds_ctx *c)
   a.clock = c->dclock;
   a.counters = c->dcount;
   a.rp = c->rp;
+  a.eos = c->deos;
+  a.grav = nullptr;
   int fp32 = c->precision == MDS_FP32_BULK;
   if (fp32) {
     StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
-    HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, &a, &sp, &sh, -1, c->st));
+    HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, c->tab, 0, &a, &sp, &sh, -1, c->st));
   } else {
     StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-    HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, &a, &sp, &sh, -1, c->st));
+    HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, c->tab, 0, &a, &sp, &sh, -1, c->st));
   }
   return MDS_OK;
 }
@@ -597,7 +647,7 @@ static int enqueue_fused1d(mds_ctx *c)
 static void finish_step(mds_ctx *c)
 {
   if (!c->fuse1d) return;
-  for (int k = 0; k < 5; k++) std::swap(c->p[k], c->ph[k]);
+  for (int k = 0; k < 6; k++) std::swap(c->p[k], c->ph[k]);
   c->par ^= 1;
 }
 
@@ -803,7 +853,7 @@ static void snap_free(Snapshot &s)
 static int snap_alloc(mds_ctx *c, Snapshot &s)
 {
   size_t bytes = (size_t)c->nzone * rsize(c);
-  for (int k = 0; k < 10; k++) {
+  for (int k = 0; k < 2 * c->nstate; k++) {
     void *b = nullptr;
     HIPCHK(c, hipMalloc(&b, bytes));
     s.buf.push_back(b);
@@ -818,11 +868,11 @@ static int snap_alloc(mds_ctx *c, Snapshot &s)
 static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 {
   size_t bytes = (size_t)c->nzone * rsize(c);
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < c->nstate; k++) {
     void *dst = save ? s.buf[k] : c->p[k], *src = save ? c->p[k] : s.buf[k];
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
-    dst = save ? s.buf[5 + k] : c->ph[k];
-    src = save ? c->ph[k] : s.buf[5 + k];
+    dst = save ? s.buf[c->nstate + k] : c->ph[k];
+    src = save ? c->ph[k] : s.buf[c->nstate + k];
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
   }
   HIPCHK(c, hipMemcpyAsync(save ? s.clock : (void *)c->dclock, save ? (void *)c->dclock : s.clock,

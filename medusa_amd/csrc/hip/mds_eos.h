@@ -9,14 +9,8 @@
 // the inversion is FP64; the FP32-bulk variant interpolates ln P increments
 // and Gamma_1 in FP32.
 #pragma once
+#include "mds_internal.h"
 #include "mds_physics.h"
-
-struct DevEos {
-  int nr, nt, ny;
-  double lr0, lr1, lt0, lt1, ye0, ye1;
-  double dlr, dlt, dye;          // node spacings, (last - first) / (n - 1)
-  const float *e, *lnp, *g1;     // record bases (MDS_EOS_E, _LNP, _GAM1)
-};
 
 // bilinear weights in (log10 rho, Ye) and the four T rows they touch
 struct EosW {
@@ -27,13 +21,15 @@ struct EosW {
 MDS_DEV EosW eos_weights(const DevEos &t, double rho, double ye)
 {
   EosW w;
+  // clamp to the table (a NaN lands on the first node: the indices below
+  // stay inside the table whatever the state)
   double rl = log10(rho);
-  rl = rl < t.lr0 ? t.lr0 : (rl > t.lr1 ? t.lr1 : rl);
-  ye = ye < t.ye0 ? t.ye0 : (ye > t.ye1 ? t.ye1 : ye);
+  rl = !(rl >= t.lr0) ? t.lr0 : (rl > t.lr1 ? t.lr1 : rl);
+  ye = !(ye >= t.ye0) ? t.ye0 : (ye > t.ye1 ? t.ye1 : ye);
   int jr = (int)((rl - t.lr0) / t.dlr);
   int jy = (int)((ye - t.ye0) / t.dye);
-  jr = jr > t.nr - 1 ? t.nr - 1 : jr;
-  jy = jy > t.ny - 1 ? t.ny - 1 : jy;
+  jr = jr > t.nr - 1 ? t.nr - 1 : (jr < 0 ? 0 : jr);
+  jy = jy > t.ny - 1 ? t.ny - 1 : (jy < 0 ? 0 : jy);
   int jyp = jy + 1, jrp = jr + 1;
   double drho = (rl - (t.lr0 + jr * t.dlr)) / t.dlr;
   double dye = (ye - (t.ye0 + jy * t.dye)) / t.dye;

@@ -18,9 +18,18 @@ struct __align__(16) ZoneF32 {
   float isc0;              // 1 / scale[0][0]
   float scalef, gcovf0;    // lower face: scale[1][0], gcov[1][0]
   float rc[MDS_RC_COUNT];  // FP32 reconstruction coefficients
-  float pad[2];
+  float dcen;              // volume centroid beta/Gamma minus the lower face x
+  float pad;
 };
 static_assert(sizeof(ZoneF32) == 96, "ZoneF32 layout");
+
+// Tabulated EOS as the kernels see it (records of mds_eos_table in HBM).
+struct DevEos {
+  int nr, nt, ny;
+  double lr0, lr1, lt0, lt1, ye0, ye1;
+  double dlr, dlt, dye;          // node spacings, (last - first) / (n - 1)
+  const float *e, *lnp, *g1;     // record bases (MDS_EOS_E, _LNP, _GAM1)
+};
 
 // One problem as the kernels see it.
 struct DevProb {
@@ -29,6 +38,7 @@ struct DevProb {
   int ndim;
   int bc_lo[3], bc_hi[3];
   int sph;                 // spherical 1D column
+  int rcoord;              // radial map (MDS_RCOORD_*)
   double gam;
   double g0[3];            // fixed gravity (contravariant)
   double startx[3], dx[3];
@@ -55,7 +65,7 @@ struct RunParams {
 
 template <class R>
 struct StateArrays {
-  R *v[5];                 // rho, u, v1, v2, v3 (packed over problems)
+  R *v[6];                 // rho, u, v1, v2, v3, Ye (packed over problems; Ye with a table EOS)
 };
 
 struct Sweep1DArgs {
@@ -65,6 +75,8 @@ struct Sweep1DArgs {
   DevClock *clock;
   unsigned long long *counters;
   RunParams rp;
+  DevEos eos;              // tabulated EOS (when the batch has one)
+  const double *grav;      // monopole gravity at zone centres, packed like the state
 };
 
 struct Sweep2DArgs {

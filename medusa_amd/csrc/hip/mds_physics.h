@@ -31,7 +31,11 @@ struct FaceGeo {
 template <class R, int NV>
 struct PState {          // reconstructed primitive state on one side of a face
   R rho, uu, v[3], P, G;
+  R ye;                  // passive scalar (electron fraction), used when NS = 1
 };
+
+// slot of the passive scalar in conserved/flux records (decs.h:646: YE after U3)
+#define MDS_YI 5
 
 template <class R, int NV>
 struct Flux {
@@ -147,9 +151,9 @@ MDS_DEV void wave_speeds_x64(double rl, double rr, double p_l, double p_r, doubl
 }
 
 // conserved state and physical flux of one side (riemann.c:4-37)
-template <int NV, bool CART>
+template <int NV, bool CART, int NS = 0>
 MDS_DEV void p_uflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, int dir,
-                         double u[5], double f[5])
+                         double u[6], double f[6])
 {
   double vcov[3];
   for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
@@ -160,15 +164,17 @@ MDS_DEV void p_uflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, 
   double etot = ((0.5 * u[0]) * vsq) + p.uu;
   u[1] = etot;
   for (int d = 0; d < NV; d++) u[2 + d] = u[0] * vcov[d];
+  if (NS) u[MDS_YI] = u[0] * p.ye;
   f[0] = u[0] * v;
   f[1] = v * (etot + p.P);
   for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
   f[2 + dir] += p.P;
+  if (NS) f[MDS_YI] = f[0] * p.ye;
 }
 
 // area-weighted physical flux (riemann.c:39-62)
-template <int NV, bool CART>
-MDS_DEV void pflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, int dir, double f[5])
+template <int NV, bool CART, int NS = 0>
+MDS_DEV void pflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, int dir, double f[6])
 {
   double vcov[3];
   for (int d = 0; d < NV; d++) vcov[d] = CART ? p.v[d] : g.gcov[d] * p.v[d];
@@ -180,16 +186,18 @@ MDS_DEV void pflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, in
   for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
   double etot = ((0.5 * p.rho) * vsq) + p.uu;
   f[1] = (g.area * v) * (etot + p.P);
+  if (NS) f[MDS_YI] = f[0] * p.ye;
 }
 
 // HLLC (riemann.c:328-499).  Returns isshock.
-template <int NV, bool CART>
+template <int NV, bool CART, int NS = 0>
 MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
-                     const FaceGeo<double> &g, int dir, double flux[5], double &sig)
+                     const FaceGeo<double> &g, int dir, double flux[6], double &sig)
 {
   if (g.area < 1.0e-16) {
     sig = 1.0e-200;
     for (int q = 0; q < 2 + NV; q++) flux[q] = 0.0;
+    if (NS) flux[MDS_YI] = 0.0;
     return 0;
   }
   double u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -203,18 +211,18 @@ MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   if (ss < sl) sl = 1.001 * ss;
   double smax = (fabs(sl) > fabs(sr)) ? fabs(sl) : fabs(sr);
   sig = CART ? smax : smax / g.scale;
-  if (sl > 0.0) { pflux_x64<NV, CART>(L, g, dir, flux); return isshock; }
-  if (sr < 0.0) { pflux_x64<NV, CART>(Rt, g, dir, flux); return isshock; }
-  double u[5], fx[5], Us[5], vs[3];
+  if (sl > 0.0) { pflux_x64<NV, CART, NS>(L, g, dir, flux); return isshock; }
+  if (sr < 0.0) { pflux_x64<NV, CART, NS>(Rt, g, dir, flux); return isshock; }
+  double u[6], fx[6], Us[6], vs[3];
   double slr;
   const PState<double, NV> &K = (ss > 0.0) ? L : Rt;
   if (ss > 0.0) {
-    p_uflux_x64<NV, CART>(L, g, dir, u, fx);
+    p_uflux_x64<NV, CART, NS>(L, g, dir, u, fx);
     Us[0] = (L.rho * (sl - u_l)) / (sl - ss);
     Us[1] = (((u[1] * (sl - u_l)) - (L.P * u_l)) + (p_s * ss)) / (sl - ss);
     slr = CART ? sl : sl / g.scale;
   } else {
-    p_uflux_x64<NV, CART>(Rt, g, dir, u, fx);
+    p_uflux_x64<NV, CART, NS>(Rt, g, dir, u, fx);
     Us[0] = (Rt.rho * (sr - u_r)) / (sr - ss);
     Us[1] = (((u[1] * (u_r - sr)) + (Rt.P * u_r)) - (p_s * ss)) / (ss - sr);
     slr = CART ? sr : sr / g.scale;
@@ -222,18 +230,21 @@ MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   for (int d = 0; d < NV; d++) vs[d] = K.v[d];
   vs[dir] = CART ? ss : ss / g.scale;
   for (int d = 0; d < NV; d++) Us[2 + d] = (CART ? vs[d] : g.gcov[d] * vs[d]) * Us[0];
+  if (NS) Us[MDS_YI] = Us[0] * K.ye;
   for (int q = 0; q < 2 + NV; q++) flux[q] = g.area * (fx[q] + (slr * (Us[q] - u[q])));
+  if (NS) flux[MDS_YI] = g.area * (fx[MDS_YI] + (slr * (Us[MDS_YI] - u[MDS_YI])));
   return isshock;
 }
 
 // HLLE (riemann.c:178-325)
-template <int NV, bool CART>
+template <int NV, bool CART, int NS = 0>
 MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
-                      const FaceGeo<double> &g, int dir, double flux[5], double &sig)
+                      const FaceGeo<double> &g, int dir, double flux[6], double &sig)
 {
   if (g.area < 1.0e-16) {
     sig = 1.0e-200;
     for (int q = 0; q < 2 + NV; q++) flux[q] = 0.0;
+    if (NS) flux[MDS_YI] = 0.0;
     return;
   }
   double u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -243,9 +254,9 @@ MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   double sl, sr, ss, p_s;
   int isshock;
   wave_speeds_x64<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
-  double ul[5], ur[5], fl[5], fr[5];
-  p_uflux_x64<NV, CART>(L, g, dir, ul, fl);
-  p_uflux_x64<NV, CART>(Rt, g, dir, ur, fr);
+  double ul[6], ur[6], fl[6], fr[6];
+  p_uflux_x64<NV, CART, NS>(L, g, dir, ul, fl);
+  p_uflux_x64<NV, CART, NS>(Rt, g, dir, ur, fr);
   double srs = CART ? sr : sr / g.scale;
   double sls = CART ? sl : sl / g.scale;
   double cmax = (srs > 0.0) ? srs : 0.0;
@@ -253,6 +264,10 @@ MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   sig = (cmax > cmin) ? cmax : cmin;
   for (int q = 0; q < 2 + NV; q++)
     flux[q] = (g.area * (((cmin * fr[q]) + (cmax * fl[q])) - ((cmax * cmin) * (ur[q] - ul[q])))) / (cmax + cmin);
+  if (NS) {
+    const int q = MDS_YI;
+    flux[q] = (g.area * (((cmin * fr[q]) + (cmax * fl[q])) - ((cmax * cmin) * (ur[q] - ul[q])))) / (cmax + cmin);
+  }
 }
 
 // fluid.c:4-25
@@ -378,8 +393,8 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
 }
 
 // Per-unit-area fluxes in FP32 (the FP64 caller applies the FP64 area).
-template <int NV, bool CART>
-MDS_DEV void pflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float f[5])
+template <int NV, bool CART, int NS = 0>
+MDS_DEV void pflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float f[6])
 {
 #pragma clang fp contract(fast)
   float vcov[3];
@@ -391,10 +406,11 @@ MDS_DEV void pflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int 
   float vsq = 0.0f;
   for (int d = 0; d < NV; d++) vsq += p.v[d] * vcov[d];
   f[1] = v * (0.5f * p.rho * vsq + p.uu + p.P);
+  if (NS) f[MDS_YI] = f[0] * p.ye;
 }
 
-template <int NV, bool CART>
-MDS_DEV void p_uflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float u[5], float f[5])
+template <int NV, bool CART, int NS = 0>
+MDS_DEV void p_uflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, int dir, float u[6], float f[6])
 {
 #pragma clang fp contract(fast)
   float vcov[3];
@@ -410,12 +426,13 @@ MDS_DEV void p_uflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, in
   f[1] = v * (etot + p.P);
   for (int d = 0; d < NV; d++) f[2 + d] = f[0] * vcov[d];
   f[2 + dir] += p.P;
+  if (NS) { u[MDS_YI] = p.rho * p.ye; f[MDS_YI] = f[0] * p.ye; }
 }
 
 // HLLC in FP32; flux per unit area.  Returns isshock.
-template <int NV, bool CART>
+template <int NV, bool CART, int NS = 0>
 MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
-                     const FaceGeo<float> &g, int dir, float flux[5], float &sig)
+                     const FaceGeo<float> &g, int dir, float flux[6], float &sig)
 {
 #pragma clang fp contract(fast)
   float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -429,9 +446,9 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   if (ss < sl) sl = 1.001f * ss;
   float iscale = CART ? 1.0f : frcp(g.scale);
   sig = fmaxf(fabsf(sl), fabsf(sr)) * iscale;
-  if (sl > 0.0f) { pflux_f32<NV, CART>(L, g, dir, flux); return isshock; }
-  if (sr < 0.0f) { pflux_f32<NV, CART>(Rt, g, dir, flux); return isshock; }
-  float u[5], fx[5], Us[5];
+  if (sl > 0.0f) { pflux_f32<NV, CART, NS>(L, g, dir, flux); return isshock; }
+  if (sr < 0.0f) { pflux_f32<NV, CART, NS>(Rt, g, dir, flux); return isshock; }
+  float u[6], fx[6], Us[6];
   bool left = ss > 0.0f;
   const PState<float, NV> &K = left ? L : Rt;
   float sk = left ? sl : sr, uk = left ? u_l : u_r;
@@ -445,12 +462,13 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
     Us[2 + d] = (CART ? vs : g.gcov[d] * vs) * Us[0];
   }
   for (int q = 0; q < 2 + NV; q++) flux[q] = fx[q] + slr * (Us[q] - u[q]);
+  if (NS) flux[MDS_YI] = flux[0] * K.ye;     // passive scalar: upwind by the contact
   return isshock;
 }
 
-template <int NV, bool CART>
+template <int NV, bool CART, int NS = 0>
 MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
-                      const FaceGeo<float> &g, int dir, float flux[5], float &sig)
+                      const FaceGeo<float> &g, int dir, float flux[6], float &sig)
 {
 #pragma clang fp contract(fast)
   float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -460,15 +478,19 @@ MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   float sl, sr, ss, p_s;
   int isshock;
   wave_speeds_f32<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
-  float ul[5], ur[5], fl[5], fr[5];
-  p_uflux_f32<NV, CART>(L, g, dir, ul, fl);
-  p_uflux_f32<NV, CART>(Rt, g, dir, ur, fr);
+  float ul[6], ur[6], fl[6], fr[6];
+  p_uflux_f32<NV, CART, NS>(L, g, dir, ul, fl);
+  p_uflux_f32<NV, CART, NS>(Rt, g, dir, ur, fr);
   float iscale = CART ? 1.0f : frcp(g.scale);
   float cmax = fmaxf(sr * iscale, 0.0f);
   float cmin = fabsf(fminf(sl * iscale, 0.0f));
   sig = fmaxf(cmax, cmin);
   float iden = frcp(cmax + cmin);
   for (int q = 0; q < 2 + NV; q++) flux[q] = (cmin * fr[q] + cmax * fl[q] - cmax * cmin * (ur[q] - ul[q])) * iden;
+  if (NS) {
+    const int q = MDS_YI;
+    flux[q] = (cmin * fr[q] + cmax * fl[q] - cmax * cmin * (ur[q] - ul[q])) * iden;
+  }
 }
 
 // ---------------------------------------------------------------------------

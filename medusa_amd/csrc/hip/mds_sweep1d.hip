@@ -4,30 +4,36 @@
 // lane l holds zone c0 - H + l:
 //   lane loads its zone (boundary ghosts are mapped onto interior zones at
 //   load time, bc.c:705-1421 semantics, so no ghost storage exists)
-//   -> gamma-law EOS in registers (eos_gamma.c:45-57)
+//   -> EOS in registers: gamma law (eos_gamma.c:45-57) or the table
+//      (eos_stuff.c:507-650, see mds_eos.h)
 //   -> reconstruction of its own zone, neighbours by DPP lane shifts
 //      (once per zone instead of once per face as in step.c:500-547)
-//   -> HLLC at its lower face (riemann.c:328-499)
-//   -> flux divergence, sources and the RK update (step.c:327-478)
+//   -> HLLC at its lower face (riemann.c:328-499), Ye as a passive scalar
+//   -> flux divergence, sources (geometric, fixed or monopole gravity) and
+//      the RK update (step.c:327-478)
 //   -> u -> p; wave-level reductions of the CFL limits, one atomicMin per
 //      wave into the problem's device clock (step.c:560, 393-394).
 // Two kernel families:
 //   sweep1d_*   one RK stage per launch, H = 2, 60 zones per wave
-//               (p -> ph, then p, ph -> p; any boundary codes)
+//               (p -> ph, then p, ph -> p; any boundary codes; monopole
+//               gravity, whose field comes from a separate per-stage kernel)
 //   sweep1d2_*  both RK stages of a step in one launch, H = 4, 56 zones per
 //               wave: stage 0 leaves 60 valid lanes, the boundary ghosts of
 //               the intermediate state are refilled from them by lane
 //               permutes, stage 1 produces the 56 owned zones.  The state is
 //               read and written once per step (in -> out; the caller swaps
-//               the buffers).  Used when no problem is periodic in x.
+//               the buffers).  Used when no problem is periodic in x and
+//               there is no monopole gravity.
 // Both families run the same per-lane stage code, so they give identical
 // results.
 // Reference loops replaced: prim_to_fluxes1 (step.c:482-567) + push
 // (step.c:327-478) + reset_boundaries_gpu for 1D (bc.c:705-1030) +
 // update_eos/update_eos_ghost (eos.c:4-32) + get_geom_src_terms
-// (rad_fluid.c:195-214) + fixed_gravity (gravity.c:46-64).
+// (rad_fluid.c:195-214) + fixed_gravity (gravity.c:46-64) + the source part
+// of monopole_gravity_finish (gravity.c:432-498).
 #include "mds_internal.h"
 #include "mds_physics.h"
+#include "mds_eos.h"
 
 namespace {
 
@@ -56,6 +62,12 @@ MDS_DEV int clamp_q(int c, int n)
   int q = c + MDS_NG;
   return q < 0 ? 0 : (q > n + 2 * MDS_NG - 1 ? n + 2 * MDS_NG - 1 : q);
 }
+
+// primitive record of one zone (rho, u, v^1..v^3, Ye)
+template <class R>
+struct Prim {
+  R rho, uu, v[3], ye;
+};
 
 // CFL and source limits gathered by one lane over the stages it runs.
 struct Lim1D {
@@ -94,35 +106,40 @@ MDS_DEV void commit_limits(const DevProb &pb, DevClock *ck, unsigned long long *
 // holding their source zones (the reference's ghost fill between the stages,
 // step.c:99).  Interior lanes permute onto themselves.  Non-periodic codes
 // only: then every source zone lies in this wave.
-template <int NV, class R>
-MDS_DEV void ghost_permute(int c, int n, int c0, int lo, int hi, R &rho, R &uu, R v[3])
+template <int NV, int TAB, class R>
+MDS_DEV void ghost_permute(int c, int n, int c0, int lo, int hi, Prim<R> &q)
 {
   bool flip;
   const int s = bc_src(c, n, lo, hi, flip);
   int src_lane = s - c0 + 2 * MDS_NG;
   src_lane = src_lane < 0 ? 0 : (src_lane > 63 ? 63 : src_lane);
-  rho = __shfl(rho, src_lane, 64);
-  uu = __shfl(uu, src_lane, 64);
-  for (int d = 0; d < NV; d++) v[d] = __shfl(v[d], src_lane, 64);
-  if (flip) v[0] = -v[0];
+  q.rho = __shfl(q.rho, src_lane, 64);
+  q.uu = __shfl(q.uu, src_lane, 64);
+  for (int d = 0; d < NV; d++) q.v[d] = __shfl(q.v[d], src_lane, 64);
+  if (TAB) q.ye = __shfl(q.ye, src_lane, 64);
+  if (flip) q.v[0] = -q.v[0];
 }
 
-template <int NV, class R>
-MDS_DEV void load_prims(const StateArrays<R> &in, long long off, int s, bool flip, R &rho, R &uu, R v[3])
+template <int NV, int TAB, class R>
+MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, bool flip)
 {
-  rho = in.v[0][off + s];
-  uu = in.v[1][off + s];
-  for (int d = 0; d < 3; d++) v[d] = R(0);
-  for (int d = 0; d < NV; d++) v[d] = in.v[2 + d][off + s];
-  if (flip) v[0] = -v[0];
+  Prim<R> q;
+  q.rho = in.v[0][off + s];
+  q.uu = in.v[1][off + s];
+  for (int d = 0; d < 3; d++) q.v[d] = R(0);
+  for (int d = 0; d < NV; d++) q.v[d] = in.v[2 + d][off + s];
+  q.ye = TAB ? in.v[5][off + s] : R(0);
+  if (flip) q.v[0] = -q.v[0];
+  return q;
 }
 
-template <int NV, class R>
-MDS_DEV void store_prims(StateArrays<R> &out, long long off, int c, R rho, R uu, const R v[3])
+template <int NV, int TAB, class R>
+MDS_DEV void store_prims(StateArrays<R> &out, long long off, int c, const Prim<R> &q)
 {
-  out.v[0][off + c] = rho;
-  out.v[1][off + c] = uu;
-  for (int d = 0; d < NV; d++) out.v[2 + d][off + c] = v[d];
+  out.v[0][off + c] = q.rho;
+  out.v[1][off + c] = q.uu;
+  for (int d = 0; d < NV; d++) out.v[2 + d][off + c] = q.v[d];
+  if (TAB) out.v[5][off + c] = q.ye;
 }
 
 // ---------------------------------------------------------------------------
@@ -172,26 +189,44 @@ MDS_DEV void load_geo64(const DevProb &pb, int c, int n, Geo64 &g)
   }
 }
 
-// One RK stage of one lane.  (rho, uu, v): stage input at this lane's zone;
-// U0: conserved form of U^n (sim_p) at this zone, (un, vn) its primitives.
-// Outputs (ro, uo, vo) are the input where the update is not valid.
-template <int NV, bool SPH>
-MDS_DEV void stage_x64(const DevProb &pb, const Geo64 &g, int stage, double dt, double rho, double uu,
-                       const double v[3], const double U0[5], double un, const double vn[3], Roles r,
-                       double &ro, double &uo, double vo[3], Lim1D &lim)
+// conserved form (fluid.c:4-25), Ye in slot MDS_YI
+template <int NV, bool SPH, int TAB>
+MDS_DEV void conserved_x64(const Prim<double> &q, const double gcov[3], double U[6])
+{
+  p_to_u_x64<NV, !SPH>(q.rho, q.uu, q.v, gcov, U);
+  U[MDS_YI] = TAB ? U[0] * q.ye : 0.0;
+}
+
+// One RK stage of one lane.  q: stage input at this lane's zone; U0: the
+// conserved form of U^n (sim_p) at this zone, qn its primitives; gz: monopole
+// gravity at the zone centre.  The output is the input where the update is
+// not valid.
+template <int NV, bool SPH, int TAB, int MONO>
+MDS_DEV Prim<double> stage_x64(const DevProb &pb, const DevEos &eos, const Geo64 &g, int stage, double dt,
+                               const Prim<double> &q, const double U0[6], const Prim<double> &qn, double gz,
+                               Roles r, Lim1D &lim)
 {
   const double gam = pb.gam;
-  const double press = (gam - 1.0) * uu;
-
-  // reconstruct rho, u, v^d, P (Gamma is uniform: para_recon returns it flat)
-  double al[3 + NV], ar[3 + NV];
+  double press, gam1, cs = 0.0;
+  if (TAB) {
+    double lt;
+    eos_fill_x64(eos, q.rho, q.uu, q.ye, press, gam1, cs, lt);
+  } else {
+    press = (gam - 1.0) * q.uu;
+    gam1 = gam;
+  }
+  // reconstruct rho, u, v^d, (Ye), P, (Gamma_1); a uniform Gamma would
+  // reconstruct flat (para_recon returns it unchanged)
+  constexpr int NQ = 3 + NV + 2 * TAB;
+  double al[NQ], ar[NQ];
   {
-    double qv[3 + NV];
-    qv[0] = rho; qv[1] = uu;
-    for (int d = 0; d < NV; d++) qv[2 + d] = v[d];
-    qv[2 + NV] = press;
+    double qv[NQ];
+    qv[0] = q.rho; qv[1] = q.uu;
+    for (int d = 0; d < NV; d++) qv[2 + d] = q.v[d];
+    if (TAB) { qv[2 + NV] = q.ye; qv[3 + NV] = press; qv[4 + NV] = gam1; }
+    else qv[2 + NV] = press;
 #pragma unroll
-    for (int k = 0; k < 3 + NV; k++) {
+    for (int k = 0; k < NQ; k++) {
       double qm = lane_from_below(qv[k]);
       double qp = lane_from_above(qv[k]);
       para_recon_x64(qm, qv[k], qp, g.rg, al[k], ar[k]);
@@ -203,31 +238,39 @@ MDS_DEV void stage_x64(const DevProb &pb, const Geo64 &g, int stage, double dt, 
   L.uu = lane_from_below(ar[1]);   Rt.uu = al[1];
   for (int d = 0; d < NV; d++) { L.v[d] = lane_from_below(ar[2 + d]); Rt.v[d] = al[2 + d]; }
   for (int d = NV; d < 3; d++) { L.v[d] = 0.0; Rt.v[d] = 0.0; }
-  L.P = lane_from_below(ar[2 + NV]); Rt.P = al[2 + NV];
-  L.G = gam; Rt.G = gam;
+  if (TAB) {
+    L.ye = lane_from_below(ar[2 + NV]); Rt.ye = al[2 + NV];
+    L.P = lane_from_below(ar[3 + NV]);  Rt.P = al[3 + NV];
+    L.G = lane_from_below(ar[4 + NV]);  Rt.G = al[4 + NV];
+  } else {
+    L.ye = Rt.ye = 0.0;
+    L.P = lane_from_below(ar[2 + NV]); Rt.P = al[2 + NV];
+    L.G = gam; Rt.G = gam;
+  }
 
-  double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, sig = 1.0;
-  int isshock = hllc_x64<NV, !SPH>(L, Rt, g.fg, 0, F, sig);
+  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, sig = 1.0;
+  int isshock = hllc_x64<NV, !SPH, TAB>(L, Rt, g.fg, 0, F, sig);
   // CFL limits are reduced per wavefront before the single division
   // (min of dx/|s| = dx/max|s|, min of 1/loc = 1/max loc; bit-identical)
   if (r.face) lim.smax = fmax(lim.smax, fabs(sig));
   lim.nsh += __popcll(__ballot(r.count && isshock));
 
   // flux through the upper face from the lane above
-  double Fp[5];
+  double Fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 2 + NV; k++) Fp[k] = lane_from_above(F[k]);
+  if (TAB) Fp[MDS_YI] = lane_from_above(F[MDS_YI]);
+  const double area_hi = MONO ? lane_from_above(g.fg.area) : 0.0;
 
-  ro = rho; uo = uu;
-  for (int d = 0; d < 3; d++) vo[d] = v[d];
-  if (!r.out) return;
+  Prim<double> o = q;
+  if (!r.out) return o;
   const double *zg = g.zg;
   // sources from the stage input state (rad_fluid.c:195-214, gravity.c:46-64)
-  double src[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (SPH) {
     double vc[3], T[3][3];
-    for (int d = 0; d < 3; d++) vc[d] = g.gcov0[d] * v[d];
+    for (int d = 0; d < 3; d++) vc[d] = g.gcov0[d] * q.v[d];
     for (int l = 0; l < 3; l++) {
-      for (int m = 0; m < 3; m++) T[l][m] = (rho * v[m]) * vc[l];
+      for (int m = 0; m < 3; m++) T[l][m] = (q.rho * q.v[m]) * vc[l];
       T[l][l] += press;
     }
     // non-zero conn[l][m][n] in the reference's (l, n) accumulation order
@@ -248,18 +291,30 @@ MDS_DEV void stage_x64(const DevProb &pb, const Geo64 &g, int stage, double dt, 
     }
   }
   if (pb.g0[0] != 0.0) {
-    src[1] += rho * ((v[0] * pb.g0[0]) * g.gcov0[0]);
-    src[2] += (rho * pb.g0[0]) * g.gcov0[0];
+    src[1] += q.rho * ((q.v[0] * pb.g0[0]) * g.gcov0[0]);
+    src[2] += (q.rho * pb.g0[0]) * g.gcov0[0];
   }
-  double U[5], Uh[5];
-  for (int k = 0; k < 5; k++) U[k] = U0[k];
-  p_to_u_x64<NV, !SPH>(rho, uu, v, g.gcov0, Uh);
-  if (SPH || pb.g0[0] != 0.0) {
-    double dsrc = dt_source_control_x64<NV>(un, vn, U, src);
+  if (MONO) {
+    // gravity.c:475-486: rho g, and the mass flux per area, linear between
+    // the faces, at the volume centroid times g
+    const double xl = g.rg.xlo;
+    double vr = Fp[0];
+    vr /= (area_hi + 1.0e-16);
+    src[2] += q.rho * gz;
+    const double vl = F[0] / (g.fg.area + 1.0e-16);
+    const double A = (vr - vl) / pb.dx[0];
+    const double B = vl - A * xl;
+    src[1] += ((A * g.rg.b0) / g.rg.g0 + B) * gz;
+  }
+  double U[6], Uh[6];
+  for (int k = 0; k < 6; k++) U[k] = U0[k];
+  conserved_x64<NV, SPH, TAB>(q, g.gcov0, Uh);
+  if (SPH || pb.g0[0] != 0.0 || MONO) {
+    double dsrc = dt_source_control_x64<NV>(qn.uu, qn.v, U, src);
     lim.lim_src = (lim.lim_src < dsrc) ? lim.lim_src : dsrc;
   } else {
     // no sources: the three quotients share the denominator 1e-16 (step.c:212-219)
-    double m0 = 0.02 * U[0], m1 = 0.02 * U[1], m2 = 2.0 * un;
+    double m0 = 0.02 * U[0], m1 = 0.02 * U[1], m2 = 2.0 * qn.uu;
     double m = (m0 < m1) ? m0 : m1;
     m = (m < m2) ? m : m2;
     lim.srcnum = (lim.srcnum < m) ? lim.srcnum : m;
@@ -270,21 +325,29 @@ MDS_DEV void stage_x64(const DevProb &pb, const Geo64 &g, int stage, double dt, 
     double ddt = (divF / g.vol) - src[k];
     U[k] = ((pf * U[k]) + (phf * Uh[k])) - ((ff * dt) * ddt);
   }
-  u_to_p_x64<NV, !SPH>(U, g.gcon, ro, uo, vo);
+  if (TAB) {
+    const int k = MDS_YI;
+    double divF = (0.0 + Fp[k]) - F[k];
+    double ddt = (divF / g.vol) - src[k];
+    U[k] = ((pf * U[k]) + (phf * Uh[k])) - ((ff * dt) * ddt);
+  }
+  u_to_p_x64<NV, !SPH>(U, g.gcon, o.rho, o.uu, o.v);
+  if (TAB) o.ye = U[MDS_YI] / U[0];
   if (stage == 1) {
     // estimate_dt zone bound (step.c:177-187): new velocity, sound speed of
     // the eos array as last filled (from this stage's input state)
-    double cs = sqrt((gam * press) / rho);
+    if (!TAB) cs = sqrt((gam * press) / q.rho);
     double sc0 = SPH ? zg[MDS_ZG_SCALE0] : 1.0;
-    double loc = (fabs(vo[0]) + (SPH ? cs / sc0 : cs)) / pb.dx[0];
+    double loc = (fabs(o.v[0]) + (SPH ? cs / sc0 : cs)) / pb.dx[0];
     lim.locmax = fmax(lim.locmax, loc);
   }
+  return o;
 }
 
 }  // namespace
 
 // FP64 single-stage kernel: stage 0 reads p, writes ph; stage 1 reads p, ph, writes p.
-template <int NV, bool SPH>
+template <int NV, bool SPH, int TAB, int MONO>
 __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<double> sp,
                                                    StateArrays<double> sph, int stage)
 {
@@ -303,8 +366,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
   const StateArrays<double> &in = (stage == 0) ? sp : sph;
   bool flip;
   const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  double rho, uu, v[3];
-  load_prims<NV>(in, off, s, flip, rho, uu, v);
+  const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
   Geo64 g;
   load_geo64<SPH>(pb, c, n, g);
 
@@ -313,23 +375,23 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
   r.count = r.face && (lane <= 61 || c == n);
   r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
   // U^n of this zone: the stage input itself in stage 0, sim_p in stage 1
-  double rn = rho, un = uu, vn[3] = {v[0], v[1], v[2]};
-  if (stage == 1 && r.out) load_prims<NV>(sp, off, c, false, rn, un, vn);
-  double U0[5];
-  p_to_u_x64<NV, !SPH>(rn, un, vn, g.gcov0, U0);
+  Prim<double> qn = q;
+  if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, false);
+  double U0[6];
+  conserved_x64<NV, SPH, TAB>(qn, g.gcov0, U0);
+  const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
   Lim1D lim;
-  double ro, uo, vo[3];
-  stage_x64<NV, SPH>(pb, g, stage, dt, rho, uu, v, U0, un, vn, r, ro, uo, vo, lim);
+  const Prim<double> o = stage_x64<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
   if (r.out) {
     StateArrays<double> &out = (stage == 0) ? sph : sp;
-    store_prims<NV>(out, off, c, ro, uo, vo);
+    store_prims<NV, TAB>(out, off, c, o);
   }
   commit_limits(pb, ck, a.counters, lane, lim);
 }
 
 // FP64 two-stage kernel: reads in (U^n), writes out (U^{n+1}).
-template <int NV, bool SPH>
+template <int NV, bool SPH, int TAB>
 __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<double> in,
                                                     StateArrays<double> out)
 {
@@ -346,36 +408,29 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
   const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
   if (!ck->active) {
     // frozen problem: carry its state over to the output buffer
-    if (own) {
-      double rho, uu, v[3];
-      load_prims<NV>(in, off, c, false, rho, uu, v);
-      store_prims<NV>(out, off, c, rho, uu, v);
-    }
+    if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
     return;
   }
   const double dt = ck->dt;
   bool flip;
   const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  double rho, uu, v[3];
-  load_prims<NV>(in, off, s, flip, rho, uu, v);
+  const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
   Geo64 g;
   load_geo64<SPH>(pb, c, n, g);
-  double U0[5];
-  p_to_u_x64<NV, !SPH>(rho, uu, v, g.gcov0, U0);
+  double U0[6];
+  conserved_x64<NV, SPH, TAB>(q, g.gcov0, U0);
 
   const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
   const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
   Lim1D lim;
   // stage 0: U^n -> U* on lanes 2..61
   const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
-  double rh, uh, vh[3];
-  stage_x64<NV, SPH>(pb, g, 0, dt, rho, uu, v, U0, uu, v, r0, rh, uh, vh, lim);
-  ghost_permute<NV>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], rh, uh, vh);
+  Prim<double> h = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, q, 0.0, r0, lim);
+  ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
   // stage 1: U^n, U* -> U^{n+1} on lanes 4..59
   const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
-  double ro, uo, vo[3];
-  stage_x64<NV, SPH>(pb, g, 1, dt, rh, uh, vh, U0, uu, v, r1, ro, uo, vo, lim);
-  if (own) store_prims<NV>(out, off, c, ro, uo, vo);
+  const Prim<double> o = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, q, 0.0, r1, lim);
+  if (own) store_prims<NV, TAB>(out, off, c, o);
   commit_limits(pb, ck, a.counters, lane, lim);
 }
 
@@ -389,7 +444,7 @@ struct Geo32 {
   double area;              // lower face (FP64)
   FaceGeo<float> fg;        // FP32 face metric for the Riemann solve (unit area)
   double vol, gcov0, gcon0, cp;
-  float c000, isc0;
+  float c000, isc0, dcen;
 };
 
 template <bool SPH>
@@ -411,6 +466,7 @@ MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
     g.cp = z.cp;
     g.c000 = z.c000;
     g.isc0 = z.isc0;
+    g.dcen = z.dcen;
   } else {
     rc = pb.rc[0] + (size_t)q * MDS_RC_COUNT;
     g.area = pb.area[0];
@@ -421,40 +477,53 @@ MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
     g.cp = 0.0;
     g.c000 = 0.0f;
     g.isc0 = 1.0f;
+    g.dcen = 0.5f * (float)pb.dx[0];
   }
   g.rg.clm = rc[MDS_RC_CLM]; g.rg.clp = rc[MDS_RC_CLP]; g.rg.crm = rc[MDS_RC_CRM]; g.rg.crp = rc[MDS_RC_CRP];
   g.rg.mu1 = rc[MDS_RC_MU1]; g.rg.kinv = rc[MDS_RC_KINV]; g.rg.inur = rc[MDS_RC_INUR]; g.rg.imu2 = rc[MDS_RC_IMU2];
 }
 
 // U^n in conserved form (FP64) and its primitives as doubles.
-template <int NV, bool SPH>
-MDS_DEV void conserved_f32(const Geo32 &g, float rho, float uu, const float v[3], double U0[5], double &un,
-                           double vn[3])
+template <int NV, bool SPH, int TAB>
+MDS_DEV void conserved_f32(const Geo32 &g, const Prim<float> &q, double U0[6], Prim<double> &qn)
 {
-  un = uu;
-  for (int d = 0; d < 3; d++) vn[d] = v[d];
+  qn.rho = q.rho; qn.uu = q.uu; qn.ye = q.ye;
+  for (int d = 0; d < 3; d++) qn.v[d] = q.v[d];
   const double gcov[3] = {g.gcov0, 1.0, 1.0};
-  p_to_u_x64<NV, !SPH>((double)rho, un, vn, gcov, U0);
+  p_to_u_x64<NV, !SPH>(qn.rho, qn.uu, qn.v, gcov, U0);
+  U0[MDS_YI] = TAB ? U0[0] * qn.ye : 0.0;
 }
 
-// One FP32-bulk RK stage of one lane: reconstruction and Riemann fluxes in
-// FP32 (per unit area), area-weighted fluxes, sources, the update and u -> p
-// in FP64.  (rho, uu, v): stage input; U0: conserved U^n; (un, vn) its prims.
-template <int NV, bool SPH>
-MDS_DEV void stage_f32(const DevProb &pb, const Geo32 &g, int stage, double dt, float rho, float uu,
-                       const float v[3], const double U0[5], double un, const double vn[3], Roles r,
-                       float &ro, float &uo, float vo[3], Lim1D &lim)
+// One FP32-bulk RK stage of one lane: EOS table interpolation,
+// reconstruction and Riemann fluxes in FP32 (per unit area), area-weighted
+// fluxes, sources, the update and u -> p in FP64.
+template <int NV, bool SPH, int TAB, int MONO>
+MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 &g, int stage, double dt,
+                              const Prim<float> &q, const double U0[6], const Prim<double> &qn, double gz,
+                              Roles r, Lim1D &lim)
 {
   const float gam = (float)pb.gam;
-  const float press = (gam - 1.0f) * uu;
-  float al[3 + NV], ar[3 + NV];
+  float press, gam1, cs = 0.0f;
+  double dpress;
+  if (TAB) {
+    double lt;
+    eos_fill_f32(eos, (double)q.rho, (double)q.uu, (double)q.ye, dpress, gam1, cs, lt);
+    press = (float)dpress;
+  } else {
+    press = (gam - 1.0f) * q.uu;
+    dpress = press;
+    gam1 = gam;
+  }
+  constexpr int NQ = 3 + NV + 2 * TAB;
+  float al[NQ], ar[NQ];
   {
-    float qv[3 + NV];
-    qv[0] = rho; qv[1] = uu;
-    for (int d = 0; d < NV; d++) qv[2 + d] = v[d];
-    qv[2 + NV] = press;
+    float qv[NQ];
+    qv[0] = q.rho; qv[1] = q.uu;
+    for (int d = 0; d < NV; d++) qv[2 + d] = q.v[d];
+    if (TAB) { qv[2 + NV] = q.ye; qv[3 + NV] = press; qv[4 + NV] = gam1; }
+    else qv[2 + NV] = press;
 #pragma unroll
-    for (int k = 0; k < 3 + NV; k++) {
+    for (int k = 0; k < NQ; k++) {
       float qm = lane_from_below(qv[k]);
       float qp = lane_from_above(qv[k]);
       para_recon_f32(qm, qv[k], qp, g.rg, al[k], ar[k]);
@@ -465,25 +534,34 @@ MDS_DEV void stage_f32(const DevProb &pb, const Geo32 &g, int stage, double dt, 
   L.uu = lane_from_below(ar[1]);   Rt.uu = al[1];
   for (int d = 0; d < NV; d++) { L.v[d] = lane_from_below(ar[2 + d]); Rt.v[d] = al[2 + d]; }
   for (int d = NV; d < 3; d++) { L.v[d] = 0.0f; Rt.v[d] = 0.0f; }
-  L.P = lane_from_below(ar[2 + NV]); Rt.P = al[2 + NV];
-  L.G = gam; Rt.G = gam;
+  if (TAB) {
+    L.ye = lane_from_below(ar[2 + NV]); Rt.ye = al[2 + NV];
+    L.P = lane_from_below(ar[3 + NV]);  Rt.P = al[3 + NV];
+    L.G = lane_from_below(ar[4 + NV]);  Rt.G = al[4 + NV];
+  } else {
+    L.ye = Rt.ye = 0.0f;
+    L.P = lane_from_below(ar[2 + NV]); Rt.P = al[2 + NV];
+    L.G = gam; Rt.G = gam;
+  }
 
-  float Ff[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, sig = 1.0f;
+  float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, sig = 1.0f;
   int isshock = 0;
-  if (g.area >= 1.0e-16) isshock = hllc_f32<NV, !SPH>(L, Rt, g.fg, 0, Ff, sig);
+  if (g.area >= 1.0e-16) isshock = hllc_f32<NV, !SPH, TAB>(L, Rt, g.fg, 0, Ff, sig);
   else sig = 1.0e-30f;
   if (r.face) lim.smax = fmax(lim.smax, (double)fabsf(sig));
   lim.nsh += __popcll(__ballot(r.count && isshock));
-  double F[5], Fp[5];
+  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 2 + NV; k++) F[k] = g.area * (double)Ff[k];
+  if (TAB) F[MDS_YI] = g.area * (double)Ff[MDS_YI];
   for (int k = 0; k < 2 + NV; k++) Fp[k] = lane_from_above(F[k]);
+  if (TAB) Fp[MDS_YI] = lane_from_above(F[MDS_YI]);
+  const double area_hi = MONO ? lane_from_above(g.area) : 0.0;
 
-  ro = rho; uo = uu;
-  for (int d = 0; d < 3; d++) vo[d] = v[d];
-  if (!r.out) return;
-  const double drho = rho, dpress = press;
-  double dv[3] = {v[0], v[1], v[2]};
-  double src[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  Prim<float> o = q;
+  if (!r.out) return o;
+  const double drho = q.rho;
+  double dv[3] = {q.v[0], q.v[1], q.v[2]};
+  double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (SPH) {
     // radial column, v^1 only: T[0][0] conn000 + P (conn101 + conn202)
     src[2] = ((drho * dv[0]) * (g.gcov0 * dv[0])) * (double)g.c000 + dpress * g.cp;
@@ -492,15 +570,28 @@ MDS_DEV void stage_f32(const DevProb &pb, const Geo32 &g, int stage, double dt, 
     src[1] += drho * ((dv[0] * pb.g0[0]) * g.gcov0);
     src[2] += (drho * pb.g0[0]) * g.gcov0;
   }
+  if (MONO) {
+    // gravity.c:475-486, the linear mass flux per area evaluated at the
+    // centroid offset from the lower face
+    const double vl = F[0] / (g.area + 1.0e-16), vr = Fp[0] / (area_hi + 1.0e-16);
+    src[2] += drho * gz;
+    src[1] += (vl + ((vr - vl) / pb.dx[0]) * (double)g.dcen) * gz;
+  }
   const double gcov[3] = {g.gcov0, 1.0, 1.0};
-  double U[5], Uh[5];
-  for (int k = 0; k < 5; k++) U[k] = U0[k];
-  p_to_u_x64<NV, !SPH>(drho, (double)uu, dv, gcov, Uh);
-  lim.lim_src = fmin(lim.lim_src, dt_source_control_fast<NV>(un, vn, U, src));
+  double U[6], Uh[6];
+  for (int k = 0; k < 6; k++) U[k] = U0[k];
+  p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
+  Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
+  lim.lim_src = fmin(lim.lim_src, dt_source_control_fast<NV>(qn.uu, qn.v, U, src));
   const double ivol = drcp(g.vol);
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
   for (int k = 0; k < 2 + NV; k++) {
     double ddt = (Fp[k] - F[k]) * ivol - src[k];
+    U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
+  }
+  if (TAB) {
+    const int k = MDS_YI;
+    double ddt = (Fp[k] - F[k]) * ivol;
     U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
   }
   double vout[3] = {0.0, 0.0, 0.0}, vsq = 0.0;
@@ -511,19 +602,21 @@ MDS_DEV void stage_f32(const DevProb &pb, const Geo32 &g, int stage, double dt, 
     vsq += m * vout[d];
   }
   const double uout = U[1] - 0.5 * U[0] * vsq;
-  ro = (float)U[0];
-  uo = (float)uout;
-  for (int d = 0; d < NV; d++) vo[d] = (float)vout[d];
+  o.rho = (float)U[0];
+  o.uu = (float)uout;
+  for (int d = 0; d < NV; d++) o.v[d] = (float)vout[d];
+  if (TAB) o.ye = (float)(U[MDS_YI] * ri);
   if (stage == 1) {
-    float cs = fsqrt(gam * press * frcp(rho));
+    if (!TAB) cs = fsqrt(gam * press * frcp(q.rho));
     double loc = (fabs(vout[0]) + (double)(cs * g.isc0)) * (1.0 / pb.dx[0]);
     lim.locmax = fmax(lim.locmax, loc);
   }
+  return o;
 }
 
 }  // namespace
 
-template <int NV, bool SPH>
+template <int NV, bool SPH, int TAB, int MONO>
 __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<float> sp,
                                                    StateArrays<float> sph, int stage)
 {
@@ -542,30 +635,30 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
   const StateArrays<float> &in = (stage == 0) ? sp : sph;
   bool flip;
   const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  float rho, uu, v[3];
-  load_prims<NV>(in, off, s, flip, rho, uu, v);
+  const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
   Geo32 g;
   load_geo32<SPH>(pb, c, n, g);
   Roles r;
   r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
   r.count = r.face && (lane <= 61 || c == n);
   r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
-  float rn = rho, unf = uu, vnf[3] = {v[0], v[1], v[2]};
-  if (stage == 1 && r.out) load_prims<NV>(sp, off, c, false, rn, unf, vnf);
-  double U0[5], un, vn[3];
-  conserved_f32<NV, SPH>(g, rn, unf, vnf, U0, un, vn);
+  Prim<float> qnf = q;
+  if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, false);
+  double U0[6];
+  Prim<double> qn;
+  conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
+  const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
   Lim1D lim;
-  float ro, uo, vo[3];
-  stage_f32<NV, SPH>(pb, g, stage, dt, rho, uu, v, U0, un, vn, r, ro, uo, vo, lim);
+  const Prim<float> o = stage_f32<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
   if (r.out) {
     StateArrays<float> &out = (stage == 0) ? sph : sp;
-    store_prims<NV>(out, off, c, ro, uo, vo);
+    store_prims<NV, TAB>(out, off, c, o);
   }
   commit_limits(pb, ck, a.counters, lane, lim);
 }
 
-template <int NV, bool SPH>
+template <int NV, bool SPH, int TAB>
 __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<float> in,
                                                     StateArrays<float> out)
 {
@@ -581,74 +674,148 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
   const long long off = pb.off;
   const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
   if (!ck->active) {
-    if (own) {
-      float rho, uu, v[3];
-      load_prims<NV>(in, off, c, false, rho, uu, v);
-      store_prims<NV>(out, off, c, rho, uu, v);
-    }
+    if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
     return;
   }
   const double dt = ck->dt;
   bool flip;
   const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  float rho, uu, v[3];
-  load_prims<NV>(in, off, s, flip, rho, uu, v);
+  const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
   Geo32 g;
   load_geo32<SPH>(pb, c, n, g);
-  double U0[5], un, vn[3];
-  conserved_f32<NV, SPH>(g, rho, uu, v, U0, un, vn);
+  double U0[6];
+  Prim<double> qn;
+  conserved_f32<NV, SPH, TAB>(g, q, U0, qn);
 
   const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
   const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
   Lim1D lim;
   const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
-  float rh, uh, vh[3];
-  stage_f32<NV, SPH>(pb, g, 0, dt, rho, uu, v, U0, un, vn, r0, rh, uh, vh, lim);
-  ghost_permute<NV>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], rh, uh, vh);
+  Prim<float> h = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, qn, 0.0, r0, lim);
+  ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
   const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
-  float ro, uo, vo[3];
-  stage_f32<NV, SPH>(pb, g, 1, dt, rh, uh, vh, U0, un, vn, r1, ro, uo, vo, lim);
-  if (own) store_prims<NV>(out, off, c, ro, uo, vo);
+  const Prim<float> o = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, qn, 0.0, r1, lim);
+  if (own) store_prims<NV, TAB>(out, off, c, o);
   commit_limits(pb, ck, a.counters, lane, lim);
+}
+
+// ---------------------------------------------------------------------------
+// monopole gravity field (gravity.c:145-251 start, 420-450 finish), one
+// workgroup per problem: shell masses V rho of the stage state, enclosed
+// mass by a block scan, g = -G M / r^2 dr/dx at the faces, linear in x
+// across each zone evaluated at the volume centroid beta/Gamma.
+// ---------------------------------------------------------------------------
+#define MDS_MONO_THREADS 256
+
+template <class R>
+__global__ __launch_bounds__(MDS_MONO_THREADS) void monopole_kernel(const DevProb *prob, StateArrays<R> st,
+                                                                     double *grav)
+{
+  const DevProb &pb = prob[blockIdx.x];
+  const int n = pb.n[0];
+  const int t = threadIdx.x;
+  const int chunk = (n + MDS_MONO_THREADS - 1) / MDS_MONO_THREADS;
+  const int i0 = t * chunk < n ? t * chunk : n, i1 = (t + 1) * chunk < n ? (t + 1) * chunk : n;
+  const long long off = pb.off;
+  double part = 0.0;
+  for (int i = i0; i < i1; i++)
+    part = part + pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_VOLUME] * (double)st.v[0][off + i];
+  // exclusive block scan of the chunk masses
+  __shared__ double sm[MDS_MONO_THREADS];
+  sm[t] = part;
+  __syncthreads();
+  for (int o = 1; o < MDS_MONO_THREADS; o <<= 1) {
+    double v = t >= o ? sm[t - o] : 0.0;
+    __syncthreads();
+    sm[t] += v;
+    __syncthreads();
+  }
+  double mr = sm[t] - part;
+  const bool logr = pb.rcoord == MDS_RCOORD_LOG;
+  const double GN = 6.67384e-8;
+  auto edge = [&](int ii, double m) {
+    if (ii == 0) return 0.0;
+    const double xe = pb.startx[0] + ii * pb.dx[0];
+    const double r = logr ? exp(xe) : fabs(xe);
+    const double drdx = logr ? exp(xe) : 1.0;
+    return ((-GN * m) / (r * r)) * drdx;
+  };
+  double ge = edge(i0, mr);
+  for (int i = i0; i < i1; i++) {
+    mr = mr + pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_VOLUME] * (double)st.v[0][off + i];
+    const double gn = edge(i + 1, mr);
+    const double xl = pb.startx[0] + i * pb.dx[0];
+    const double A = (gn - ge) / pb.dx[0];
+    const double B = ge - A * xl;
+    const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
+    grav[off + i] = (A * be) / ga + B;
+    ge = gn;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
+                                          hipStream_t s)
+{
+  if (fp32)
+    hipLaunchKernelGGL(monopole_kernel<float>, dim3(nprob), dim3(MDS_MONO_THREADS), 0, s, prob,
+                       *(const StateArrays<float> *)st, grav);
+  else
+    hipLaunchKernelGGL(monopole_kernel<double>, dim3(nprob), dim3(MDS_MONO_THREADS), 0, s, prob,
+                       *(const StateArrays<double> *)st, grav);
+  return hipGetLastError();
+}
+
 // stage 0/1: single-stage kernels on (p, ph); stage -1: the two-stage kernel
 // with sp = input buffer and sph_ = output buffer (a->seg must then be the
-// MDS_SEG1D2 segment table).
-extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, const Sweep1DArgs *a,
-                                         const void *sp, const void *sph_, int stage,
-                                         hipStream_t st)
+// MDS_SEG1D2 segment table).  tab: tabulated EOS with Ye; mono: monopole
+// gravity (single-stage kernels only, a->grav filled for the stage input).
+extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int mono, const Sweep1DArgs *a,
+                                         const void *sp, const void *sph_, int stage, hipStream_t st)
 {
   const int waves_per_block = 4;
   dim3 block(64 * waves_per_block);
   dim3 grid((a->nseg + waves_per_block - 1) / waves_per_block);
+  if ((tab || mono) && !sph) return hipErrorInvalidValue;
+  if (mono && stage < 0) return hipErrorInvalidValue;
+#define MDS_L1(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h, stage)
+#define MDS_L2(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h)
   if (!fp32) {
     const StateArrays<double> &p = *(const StateArrays<double> *)sp;
     const StateArrays<double> &h = *(const StateArrays<double> *)sph_;
     if (stage < 0) {
-      if (sph) hipLaunchKernelGGL((sweep1d2_x64<3, true>), grid, block, 0, st, *a, p, h);
-      else if (nv == 1) hipLaunchKernelGGL((sweep1d2_x64<1, false>), grid, block, 0, st, *a, p, h);
-      else hipLaunchKernelGGL((sweep1d2_x64<3, false>), grid, block, 0, st, *a, p, h);
+      if (sph && tab) MDS_L2(sweep1d2_x64, 3, true, 1);
+      else if (sph) MDS_L2(sweep1d2_x64, 3, true, 0);
+      else if (nv == 1) MDS_L2(sweep1d2_x64, 1, false, 0);
+      else MDS_L2(sweep1d2_x64, 3, false, 0);
     } else {
-      if (sph) hipLaunchKernelGGL((sweep1d_x64<3, true>), grid, block, 0, st, *a, p, h, stage);
-      else if (nv == 1) hipLaunchKernelGGL((sweep1d_x64<1, false>), grid, block, 0, st, *a, p, h, stage);
-      else hipLaunchKernelGGL((sweep1d_x64<3, false>), grid, block, 0, st, *a, p, h, stage);
+      if (sph && tab && mono) MDS_L1(sweep1d_x64, 3, true, 1, 1);
+      else if (sph && tab) MDS_L1(sweep1d_x64, 3, true, 1, 0);
+      else if (sph && mono) MDS_L1(sweep1d_x64, 3, true, 0, 1);
+      else if (sph) MDS_L1(sweep1d_x64, 3, true, 0, 0);
+      else if (nv == 1) MDS_L1(sweep1d_x64, 1, false, 0, 0);
+      else MDS_L1(sweep1d_x64, 3, false, 0, 0);
     }
   } else {
     const StateArrays<float> &p = *(const StateArrays<float> *)sp;
     const StateArrays<float> &h = *(const StateArrays<float> *)sph_;
     if (stage < 0) {
-      if (sph) hipLaunchKernelGGL((sweep1d2_f32<1, true>), grid, block, 0, st, *a, p, h);
-      else if (nv == 1) hipLaunchKernelGGL((sweep1d2_f32<1, false>), grid, block, 0, st, *a, p, h);
-      else hipLaunchKernelGGL((sweep1d2_f32<3, false>), grid, block, 0, st, *a, p, h);
+      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1);
+      else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0);
+      else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0);
+      else MDS_L2(sweep1d2_f32, 3, false, 0);
     } else {
-      if (sph) hipLaunchKernelGGL((sweep1d_f32<1, true>), grid, block, 0, st, *a, p, h, stage);
-      else if (nv == 1) hipLaunchKernelGGL((sweep1d_f32<1, false>), grid, block, 0, st, *a, p, h, stage);
-      else hipLaunchKernelGGL((sweep1d_f32<3, false>), grid, block, 0, st, *a, p, h, stage);
+      if (sph && tab && mono) MDS_L1(sweep1d_f32, 1, true, 1, 1);
+      else if (sph && tab) MDS_L1(sweep1d_f32, 1, true, 1, 0);
+      else if (sph && mono) MDS_L1(sweep1d_f32, 1, true, 0, 1);
+      else if (sph) MDS_L1(sweep1d_f32, 1, true, 0, 0);
+      else if (nv == 1) MDS_L1(sweep1d_f32, 1, false, 0, 0);
+      else MDS_L1(sweep1d_f32, 3, false, 0, 0);
     }
   }
+#undef MDS_L1
+#undef MDS_L2
   return hipGetLastError();
 }

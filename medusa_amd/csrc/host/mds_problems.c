@@ -4,7 +4,9 @@
  */
 #include <math.h>
 #include <stddef.h>
+#include <stdlib.h>
 #include "../../../include/mds_host.h"
+#include "../../../include/mds_eos.h"
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846264338327
@@ -143,5 +145,130 @@ int mds_ensemble_params(int nprob, int nmin, int nmax, unsigned long long seed, 
     o[5] = pow(10.0, 1.0 + 2.0 * mds_uniform(seed, k++));
     o[6] = pow(10.0, 1.0 + 2.0 * mds_uniform(seed, k++));
   }
+  return MDS_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Polytropic star (intent of problems/init_ccsn.c:51-394 for a synthetic
+ * progenitor): Lane-Emden solution of index n, rho = rho_c theta^n,
+ * P = K rho^(1+1/n) inside the first zero, (rho_atm, K rho_atm^(1+1/n))
+ * outside; seeded lognormal density perturbations rho *= exp(sigma z) at
+ * constant pressure; Ye = 0.45 + 0.05 (1 - (rho/rho_c)^(1/3)).  With a table
+ * the temperature solves P_model(rho, T, Ye) = P (bisection in log T over
+ * the table's range less 2% at each end, clamped there) and u = rho e_model;
+ * without one,
+ * u = P / (gam - 1). */
+
+#define LE_H 1.0e-4L
+#define LE_MAX 200000
+
+static int lane_emden(long double n, long double *theta, long double *dtheta, int cap)
+{
+  /* RK4 in xi for y = (theta, theta'); series start at xi = h */
+  long double h = LE_H, xi = h;
+  long double y0 = 1.0L - h * h / 6.0L + n * h * h * h * h / 120.0L, y1 = -h / 3.0L + n * h * h * h / 30.0L;
+  theta[0] = 1.0L; dtheta[0] = 0.0L;
+  theta[1] = y0; dtheta[1] = y1;
+  int k = 1;
+  while (k + 1 < cap) {
+    long double a0 = y1, b0 = -2.0L / xi * y1 - powl(fmaxl(y0, 0.0L), n);
+    long double ya = y0 + 0.5L * h * a0, yb = y1 + 0.5L * h * b0, xa = xi + 0.5L * h;
+    long double a1 = yb, b1 = -2.0L / xa * yb - powl(fmaxl(ya, 0.0L), n);
+    ya = y0 + 0.5L * h * a1; yb = y1 + 0.5L * h * b1;
+    long double a2 = yb, b2 = -2.0L / xa * yb - powl(fmaxl(ya, 0.0L), n);
+    ya = y0 + h * a2; yb = y1 + h * b2;
+    long double a3 = yb, b3 = -2.0L / (xi + h) * yb - powl(fmaxl(ya, 0.0L), n);
+    y0 += h / 6.0L * (a0 + 2 * a1 + 2 * a2 + a3);
+    y1 += h / 6.0L * (b0 + 2 * b1 + 2 * b2 + b3);
+    xi += h;
+    k++;
+    theta[k] = y0; dtheta[k] = y1;
+    if (y0 <= 0.0L) break;
+  }
+  return k;   /* theta[k] <= 0: first zero in (xi_{k-1}, xi_k] */
+}
+
+static double temp_for_pressure(const double prm[3], const mds_eos_table *t, double rho, double ye, double P)
+{
+  /* keep 2% of the log T range clear of the table edges */
+  double lo = t->lt0 + 0.02 * (t->lt1 - t->lt0), hi = t->lt1 - 0.02 * (t->lt1 - t->lt0), v[3];
+  mds_eos_model_eval(prm, rho, pow(10.0, lo), ye, v);
+  if (v[0] >= P) return pow(10.0, lo);
+  mds_eos_model_eval(prm, rho, pow(10.0, hi), ye, v);
+  if (v[0] <= P) return pow(10.0, hi);
+  for (int it = 0; it < 80; it++) {
+    double mid = 0.5 * (lo + hi);
+    mds_eos_model_eval(prm, rho, pow(10.0, mid), ye, v);
+    if (v[0] < P) lo = mid; else hi = mid;
+  }
+  return pow(10.0, 0.5 * (lo + hi));
+}
+
+int mds_init_polytrope(const mds_grid *g, double n_index, double rho_c, double K, double rho_atm, double gam,
+                       const mds_eos_table *tab, unsigned long long eos_seed, double sigma,
+                       unsigned long long seed, double *prims)
+{
+  if (!g || !prims || g->geom != MDS_GEOM_SPHERICAL || !(rho_c > 0.0) || !(K > 0.0)) return MDS_EINVAL;
+  long double *th = (long double *)malloc(sizeof(long double) * LE_MAX);
+  long double *dth = (long double *)malloc(sizeof(long double) * LE_MAX);
+  if (!th || !dth) { free(th); free(dth); return MDS_ENOMEM; }
+  const int kz = lane_emden(n_index, th, dth, LE_MAX);
+  const double GN = 6.67384e-8;
+  const double alpha = sqrt((n_index + 1.0) * K * pow(rho_c, 1.0 / n_index - 1.0) / (4.0 * M_PI * GN));
+  const double xi1 = (double)((kz - 1) * LE_H + LE_H * th[kz - 1] / (th[kz - 1] - th[kz]));
+  double prm[3] = {0.0, 0.0, 0.0};
+  if (tab) mds_eos_model_params(eos_seed, prm);
+  const size_t nz = nzones(g);
+  const int n1 = g->n[0];
+  const int nv = 6;
+  for (size_t z = 0; z < nz; z++) {
+    const int i = (int)(z % (size_t)n1);
+    const double xc = g->beta[0][i + MDS_NG] / g->Gamma[0][i + MDS_NG];
+    const double r = g->rcoord == MDS_RCOORD_LOG ? exp(xc) : fabs(xc);
+    const double xi = r / alpha;
+    double rho, P;
+    if (xi < xi1) {
+      const double s = xi / (double)LE_H;
+      int k = (int)s;
+      if (k >= kz) k = kz - 1;
+      const double f = s - k;
+      /* cubic Hermite on theta, theta' */
+      const double h = (double)LE_H, t0 = (double)th[k], t1 = (double)th[k + 1];
+      const double d0 = (double)dth[k] * h, d1 = (double)dth[k + 1] * h;
+      const double f2 = f * f, f3 = f2 * f;
+      double theta = (2 * f3 - 3 * f2 + 1) * t0 + (f3 - 2 * f2 + f) * d0 + (-2 * f3 + 3 * f2) * t1 + (f3 - f2) * d1;
+      if (theta < 0.0) theta = 0.0;
+      rho = rho_c * pow(theta, n_index);
+      if (rho < rho_atm) rho = rho_atm;
+    } else {
+      rho = rho_atm;
+    }
+    P = K * pow(rho, 1.0 + 1.0 / n_index);
+    if (sigma > 0.0) {
+      double u1 = 1.0 - mds_uniform(seed, 2 * (unsigned long long)z);
+      double u2 = mds_uniform(seed, 2 * (unsigned long long)z + 1);
+      double zz = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+      rho *= exp(sigma * zz);
+    }
+    const double ye = 0.45 + 0.05 * (1.0 - cbrt(fmin(rho / rho_c, 1.0)));
+    double u;
+    if (tab) {
+      const double T = temp_for_pressure(prm, tab, rho, ye, P);
+      double v[3];
+      mds_eos_model_eval(prm, rho, T, ye, v);
+      u = rho * v[1];
+    } else {
+      u = P / (gam - 1.0);
+    }
+    prims[0 * nz + z] = rho;
+    prims[1 * nz + z] = u;
+    prims[2 * nz + z] = 0.0;
+    prims[3 * nz + z] = 0.0;
+    prims[4 * nz + z] = 0.0;
+    prims[5 * nz + z] = ye;
+  }
+  (void)nv;
+  free(th);
+  free(dth);
   return MDS_OK;
 }

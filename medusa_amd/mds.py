@@ -43,13 +43,23 @@ class Grid(ctypes.Structure):
                 ("zone", _dp)]
 
 
+class EosTableDesc(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("nt", ctypes.c_int), ("ny", ctypes.c_int),
+                ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
+                ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
+                ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
+                ("data", ctypes.POINTER(ctypes.c_float))]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("grid", ctypes.POINTER(Grid)),
                 ("bc_lo", ctypes.c_int * 3),
                 ("bc_hi", ctypes.c_int * 3),
                 ("gam", ctypes.c_double),
                 ("g0", ctypes.c_double * 3),
-                ("nvel", ctypes.c_int)]
+                ("nvel", ctypes.c_int),
+                ("eos", ctypes.POINTER(EosTableDesc)),
+                ("monopole", ctypes.c_int)]
 
 
 class RunParams(ctypes.Structure):
@@ -62,14 +72,6 @@ class RunParams(ctypes.Structure):
                 ("tmax", ctypes.c_double)]
 
 
-class EosTableDesc(ctypes.Structure):
-    _fields_ = [("nr", ctypes.c_int), ("nt", ctypes.c_int), ("ny", ctypes.c_int),
-                ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
-                ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
-                ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
-                ("data", ctypes.POINTER(ctypes.c_float))]
-
-
 _LIB = None
 
 EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
@@ -80,7 +82,7 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
-            "mds_eos_table_free", "mds_eos_points"]
+            "mds_eos_table_free", "mds_eos_points", "mds_init_polytrope"]
 
 
 def lib():
@@ -232,6 +234,11 @@ class Context:
                 arr[k].g0[d] = p.get("g0", [0.0, 0.0, 0.0])[d]
             arr[k].gam = p["gam"]
             arr[k].nvel = p.get("nvel", 0)
+            if p.get("eos") is not None:
+                arr[k].eos = ctypes.pointer(p["eos"].t)
+            arr[k].monopole = int(bool(p.get("monopole", False)))
+        self._tables = [p.get("eos") for p in problems]     # keep the tables alive
+        self.nstate = 6 if problems[0].get("eos") is not None else 5
         self.rp = rp if rp is not None else run_params()
         h = ctypes.c_void_p()
         rc = L.mds_create(arr, len(problems), ctypes.byref(self.rp), precision, ctypes.byref(h))
@@ -253,15 +260,15 @@ class Context:
 
     def _shape(self, prob):
         g = self.grids[prob]
-        return (5, g.n[2], g.n[1], g.n[0])
+        return (self.nstate, g.n[2], g.n[1], g.n[0])
 
     def upload(self, prims, prob=0):
-        a = np.ascontiguousarray(prims, dtype=np.float64).reshape(self._shape(prob))
-        self._check(lib().mds_upload_prims(self.h, prob, _ptr(a), 5), "upload")
+        a = np.ascontiguousarray(np.asarray(prims, dtype=np.float64)[:self.nstate]).reshape(self._shape(prob))
+        self._check(lib().mds_upload_prims(self.h, prob, _ptr(a), self.nstate), "upload")
 
     def download(self, prob=0):
         a = np.zeros(self._shape(prob), dtype=np.float64)
-        self._check(lib().mds_download_prims(self.h, prob, _ptr(a), 5), "download")
+        self._check(lib().mds_download_prims(self.h, prob, _ptr(a), self.nstate), "download")
         return a
 
     def update_eos(self):
@@ -370,6 +377,9 @@ def _host():
         L.mds_init_rt.argtypes = [ctypes.POINTER(Grid)] + [ctypes.c_double] * 7 + [ctypes.c_ulonglong, _dp]
         L.mds_init_sph_riemann.argtypes = [ctypes.POINTER(Grid)] + [ctypes.c_double] * 6 + [_dp]
         L.mds_ensemble_params.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _dp]
+        L.mds_init_polytrope.argtypes = ([ctypes.POINTER(Grid)] + [ctypes.c_double] * 5
+                                         + [ctypes.POINTER(EosTableDesc), ctypes.c_ulonglong, ctypes.c_double,
+                                            ctypes.c_ulonglong, _dp])
         L.mds_uniform.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong]
         L.mds_uniform.restype = ctypes.c_double
         L._host_ready = True
@@ -405,6 +415,18 @@ def init_sph_riemann(grid, gam, rho_in, p_in, r0, frho, fp):
     p = _state(grid)
     if _host().mds_init_sph_riemann(ctypes.byref(grid.g), gam, rho_in, p_in, r0, frho, fp, _ptr(p)):
         raise RuntimeError("mds_init_sph_riemann failed")
+    return p
+
+
+def init_polytrope(grid, n_index=3.0, rho_c=1.0e10, K=6.0e14, rho_atm=1.0e4, gam=4.0 / 3.0, table=None,
+                   sigma=0.01, seed=0x5EED0006):
+    """Polytropic star: prims [6][n3][n2][n1] (rho, u, v1, v2, v3, Ye)."""
+    p = np.zeros((6, grid.n[2], grid.n[1], grid.n[0]))
+    tp = ctypes.byref(table.t) if table is not None else None
+    es = table.seed if table is not None else 0
+    if _host().mds_init_polytrope(ctypes.byref(grid.g), n_index, rho_c, K, rho_atm, gam, tp, es, sigma, seed,
+                                  _ptr(p)):
+        raise RuntimeError("mds_init_polytrope failed")
     return p
 
 

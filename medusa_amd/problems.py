@@ -52,3 +52,13 @@ def ensemble(nprob=2048, nmin=256, nmax=8192, seed=0x5EED0004, rmin=1.0e5, rmax=
         prob = dict(grid=g, bc_lo=[mds.BC_REFLECT], bc_hi=[mds.BC_OUTFLOW], gam=gam)
         out.append((g, prob, mds.init_sph_riemann(g, gam, rho_in, p_in, r0, frho, fp)))
     return par, out
+
+
+def star1d(n=640, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, rmin=1.0e5, rmax=2.0e9, **kw):
+    """A polytropic star (n=3, rho_c = 1e10) on a 1D log-r column with monopole
+    gravity, reflecting inner and outflow outer boundary.  With a table
+    (mds.EosTable) the EOS is tabulated and Ye is carried; otherwise gamma-law."""
+    x0, x1 = math.log(rmin), math.log(rmax)
+    g = mds.GridBuilder(1, [int(n)], [x0], [(x1 - x0) / int(n)], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
+    prob = dict(grid=g, bc_lo=[mds.BC_REFLECT], bc_hi=[mds.BC_OUTFLOW], gam=gam, monopole=True, eos=table)
+    return g, prob, mds.init_polytrope(g, gam=gam, table=table, sigma=sigma, seed=seed, **kw)

@@ -75,13 +75,15 @@ typedef struct {
   double dt_init_safety, dt_grow_safety, dtmin, dtmax, initial_dt;
   double g0[3];        /* fixed gravity (contravariant), gravity.c:46-64; all zero = none */
   int use_fixed_grav;
+  int use_monopole;    /* Newtonian monopole gravity, gravity.c:145-251, 432-498 */
+  const struct orc_eos_tab *eos_tab;   /* tabulated EOS (NULL: gamma law); nvars 6 carries Ye */
 } orc_config;
 
 typedef struct orc_sim orc_sim;
 
 /* tabulated EOS table (layout of eos/burrows/eos_stuff.c:7: record, Ye,
  * rho, T with T fastest; records 0 e, 1 ln P, 2 Gamma_1), FP32 values */
-typedef struct {
+typedef struct orc_eos_tab {
   int nr, nt, ny;
   double lr0, lr1, lt0, lt1, ye0, ye1;
   const float *data;
@@ -140,6 +142,8 @@ long orc_shock_face_count(const orc_sim *s);    /* radial faces HLLC tagged as s
 long orc_hlle_face_count(const orc_sim *s);     /* transverse faces solved with HLLE */
 /* compensated (Neumaier) sums of conserved mass and total energy over interior zones */
 void orc_conserved_totals(const orc_sim *s, double out[2]);
+/* monopole gravity at the n1+1 radial faces from the current state (gravity.c:420-425) */
+void orc_monopole_edges(const orc_sim *s, double *grav);
 
 #ifdef __cplusplus
 }

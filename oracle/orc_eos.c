@@ -38,14 +38,16 @@ static orc_w weights(const orc_eos_tab *t, double rho, double ye)
   const double dr = (t->lr1 - t->lr0) / (t->nr - 1);
   const double dy = (t->ye1 - t->ye0) / (t->ny - 1);
   double rl = log10(rho);
-  if (rl < t->lr0) rl = t->lr0;
+  if (!(rl >= t->lr0)) rl = t->lr0;     /* also catches NaN */
   if (rl > t->lr1) rl = t->lr1;
-  if (ye < t->ye0) ye = t->ye0;
+  if (!(ye >= t->ye0)) ye = t->ye0;
   if (ye > t->ye1) ye = t->ye1;
   w.jr = (int)((rl - t->lr0) / dr);
   w.jy = (int)((ye - t->ye0) / dy);
   if (w.jr > t->nr - 1) w.jr = t->nr - 1;
   if (w.jy > t->ny - 1) w.jy = t->ny - 1;
+  if (w.jr < 0) w.jr = 0;
+  if (w.jy < 0) w.jy = 0;
   w.jyp = w.jy + 1;
   w.jrp = w.jr + 1;
   double drho = (rl - (t->lr0 + w.jr * dr)) / dr;
@@ -90,9 +92,8 @@ int orc_eos_walk(const orc_eos_tab *t, double rho, double u, double ye, double t
   const double dt = (t->lt1 - t->lt0) / (t->nt - 1);
   const double e = u / rho;
   double temp = temp_guess < 1.0e-10 ? 1.0 : temp_guess;       /* eos_stuff.c:528 */
-  int jt = (int)((log10(temp) - t->lt0) / dt);
-  if (jt < 0) jt = 0;
-  if (jt > t->nt - 2) jt = t->nt - 2;
+  const double st = (log10(temp) - t->lt0) / dt;
+  int jt = !(st >= 0.0) ? 0 : (st > t->nt - 2 ? t->nt - 2 : (int)st);
   int status = 0;
   /* walk down to the last node below e (eos_stuff.c:563-577) */
   double elo, ehi = -1e99;

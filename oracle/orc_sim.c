@@ -44,7 +44,9 @@
 #define NEOS 5
 #define E_PRESS 0
 #define E_CS 1
+#define E_TEMP 2
 #define E_GAMMA 4
+#define GNEWT 6.67384e-8     /* constants.h:3 */
 
 struct orc_sim {
   orc_config c;
@@ -397,13 +399,24 @@ static void reset_boundaries(orc_sim *s, double *P)
   }
 }
 
-/* eos_gamma.c:45-57 over every zone (interior and ghosts, eos.c:4-32) */
+/* eos_gamma.c:45-57, or the table (eos_stuff.c:507-650, walk started from
+ * the temperature of the previous fill), over every zone (interior and
+ * ghosts, eos.c:4-32) */
 static void update_eos(orc_sim *s, const double *P)
 {
   double gam = s->c.gam;
   for (long z = 0; z < s->nall; z++) {
     const double *p = &P[z * s->nvars];
     double *e = &s->eos[z * NEOS];
+    if (s->c.eos_tab) {
+      double out[4];
+      orc_eos_walk(s->c.eos_tab, p[ORC_RHO], p[ORC_UU], p[ORC_YE], e[E_TEMP], out);
+      e[E_PRESS] = out[0];
+      e[E_CS] = out[1];
+      e[E_TEMP] = out[2];
+      e[E_GAMMA] = out[3];
+      continue;
+    }
     double press = (gam - 1.0) * p[ORC_UU];
     e[E_PRESS] = press;
     e[E_CS] = sqrt((gam * press) / p[ORC_RHO]);
@@ -545,6 +558,63 @@ static void push(orc_sim *s, double dtpush, const double *p, const double *ph, d
   s->min_dt = OMIN(s->min_dt, min_dt_src);
 }
 
+/* Newtonian monopole gravity on radial shells (gravity.c:145-251 start and
+ * 254-498 finish without GR_MONOPOLE, 1D: one zone per shell): shell masses
+ * from the stage state, enclosed mass at the faces, g = -G M / r^2 dr/dx at
+ * the faces, linear in x across the zone evaluated at the volume centroid
+ * beta/Gamma; momentum source rho g, energy source (linear mass flux per
+ * area between the faces at the centroid) * g.  Needs this stage's fluxes. */
+/* face values of g (gravity.c:420-425): grav[0] = 0, grav[ii] = -G mr[ii] / r^2 dr/dx */
+static void monopole_edges(const orc_sim *s, const double *P, double *grav)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], nv = s->nvars;
+  sphmap m = {c->rcoord, 0.0, 0.0, 0.0};
+  double mr = 0.0;
+  grav[0] = 0.0;
+  for (int ii = 1; ii <= n1; ii++) {
+    double ms = 0.0;
+    for (int k = 0; k < c->n[2]; k++)
+      for (int j = 0; j < c->n[1]; j++) {
+        long z = zid(s, ii - 1, j, k);
+        ms += s->geom[z].volume * P[z * nv + ORC_RHO];
+      }
+    mr = mr + ms;
+    double xe = c->startx[0] + ii * c->dx[0];
+    grav[ii] = -GNEWT * mr / OSQR(r_of_x(&m, xe)) * dr_dx(&m, xe);
+  }
+}
+
+void orc_monopole_edges(const orc_sim *s, double *grav) { monopole_edges(s, s->p, grav); }
+
+static void monopole(orc_sim *s, const double *P)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], nv = s->nvars;
+  double *grav = calloc(n1 + 1, sizeof(double));
+  monopole_edges(s, P, grav);
+  for (int ii = 0; ii < n1; ii++) {
+    double xl = c->startx[0] + ii * c->dx[0];
+    double A = (grav[ii + 1] - grav[ii]) / c->dx[0];
+    double B = grav[ii] - A * xl;
+    const double be = s->mom[1][0][ii + ORC_NG], ga = s->mom[2][0][ii + ORC_NG];
+    double gz = (A * be) / ga + B;
+    for (int k = 0; k < c->n[2]; k++)
+      for (int j = 0; j < c->n[1]; j++) {
+        long z = zid(s, ii, j, k), zp = zid(s, ii + 1, j, k);
+        double vr = s->flux[0][zp * nv + ORC_RHO];
+        double area = s->geom[zp].area[0] + 1.0e-16;
+        vr /= area;
+        s->src[z * nv + ORC_U1] += P[z * nv + ORC_RHO] * gz;
+        double vl = s->flux[0][z * nv + ORC_RHO] / (s->geom[z].area[0] + 1.0e-16);
+        double Av = (vr - vl) / c->dx[0];
+        double Bv = vl - Av * xl;
+        s->src[z * nv + ORC_ETOT] += ((Av * be) / ga + Bv) * gz;
+      }
+  }
+  free(grav);
+}
+
 static void stage(orc_sim *s, int st)
 {
   double *P = st == 0 ? s->p : s->ph;
@@ -552,6 +622,7 @@ static void stage(orc_sim *s, int st)
   update_eos(s, P);
   sources(s, P);
   for (int d = 0; d < s->c.ndim; d++) sweep_dir(s, P, d);
+  if (s->c.use_monopole) monopole(s, P);
   if (st == 0) push(s, s->dt, s->p, s->p, s->ph, 0);
   else push(s, s->dt, s->p, s->ph, s->p, 1);
 }

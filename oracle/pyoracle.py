@@ -46,7 +46,9 @@ class Config(ctypes.Structure):
                 ("dtmax", ctypes.c_double),
                 ("initial_dt", ctypes.c_double),
                 ("g0", ctypes.c_double * 3),
-                ("use_fixed_grav", ctypes.c_int)]
+                ("use_fixed_grav", ctypes.c_int),
+                ("use_monopole", ctypes.c_int),
+                ("eos_tab", ctypes.c_void_p)]
 
 
 def _load(fast=False):
@@ -94,6 +96,8 @@ def _load(fast=False):
     lib.orc_geom_src.argtypes = [dp, ctypes.c_double, dp, ctypes.POINTER(ZoneGeom), ctypes.c_int, ctypes.c_int]
     lib.orc_dt_source_control.argtypes = [dp, dp, dp]
     lib.orc_dt_source_control.restype = ctypes.c_double
+    lib.orc_monopole_edges.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_monopole_edges.restype = None
     lib.orc_eos_walk.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, dp]
     lib.orc_eos_walk.restype = ctypes.c_int
@@ -153,7 +157,8 @@ def ptr(a):
 def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
                 geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, interp_order=3,
                 g0=(0.0, 0.0, 0.0), dt_init_safety=0.1, dt_grow_safety=1.1,
-                dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100):
+                dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100, monopole=False, eos_tab=None):
+    """eos_tab: an EosTab (tabulated EOS; nvars must be 6 to carry Ye)."""
     c = Config()
     c.ndim = ndim
     n = list(n) + [1] * (3 - len(n))
@@ -176,6 +181,10 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
     c.dtmax = dtmax
     c.initial_dt = initial_dt
     c.use_fixed_grav = int(any(v != 0.0 for v in g0))
+    c.use_monopole = int(bool(monopole))
+    if eos_tab is not None:
+        c.eos_tab = ctypes.cast(ctypes.pointer(eos_tab), ctypes.c_void_p)
+        c._eos_keep = eos_tab
     return c
 
 
@@ -206,6 +215,11 @@ class Sim:
         a = np.zeros(self.shape, dtype=np.float64)
         self.lib.orc_get_prims(self.h, ptr(a))
         return a
+
+    def monopole_edges(self):
+        out = np.zeros(self.n[0] + 1)
+        self.lib.orc_monopole_edges(self.h, ptr(out))
+        return out
 
     def run(self, tmax=1.0e300, max_steps=2 ** 31 - 1):
         return self.lib.orc_run(self.h, tmax, max_steps)
