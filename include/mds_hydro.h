@@ -38,9 +38,10 @@ extern "C" {
 #define MDS_BC_OUTFLOW   0
 #define MDS_BC_REFLECT   1
 #define MDS_BC_PERIODIC  3
+#define MDS_BC_SPHERICAL_ORIGIN 5 /* theta boundary of 3D spherical grids: across the pole */
 
 #define MDS_GEOM_CARTESIAN 0
-#define MDS_GEOM_SPHERICAL 1     /* 1D radial columns in this release */
+#define MDS_GEOM_SPHERICAL 1     /* 1D radial columns or 3D (r, theta, phi) grids */
 
 #define MDS_RCOORD_UNIFORM 0     /* r = |x|     (metric_spherical_general.c:149-152) */
 #define MDS_RCOORD_LOG     1     /* r = exp(x)  (metrics/metric_spherical_logr.c:3-38) */
@@ -83,6 +84,26 @@ enum {
   MDS_ZG_COUNT
 };
 
+/* Geometry record of a 3D spherical (r, theta, phi) grid, one per (i, j)
+ * (the metric does not depend on phi): (n1 + 2*NG) * (n2 + 2*NG) records,
+ * i fastest, zone (-NG, -NG) first; restating zone_geom (decs.h:144-153) of
+ * metric_spherical_general.c:708-1051 on a uniform angular grid. */
+enum {
+  MDS_ZS_VOLUME = 0,
+  MDS_ZS_AREA0, MDS_ZS_AREA1, MDS_ZS_AREA2,        /* area[0..2] of the lower faces   */
+  MDS_ZS_GCOV0,                                    /* gcov[0][0..2] (zone average)    */
+  MDS_ZS_GCOVF0 = MDS_ZS_GCOV0 + 3,                /* gcov[1][0..2] (radial face)     */
+  MDS_ZS_GCOVF1 = MDS_ZS_GCOVF0 + 3,               /* gcov[2][0..2] (theta face)      */
+  MDS_ZS_GCON = MDS_ZS_GCOVF1 + 3,                 /* gcon[0..2]                      */
+  MDS_ZS_SCALE0 = MDS_ZS_GCON + 3,                 /* scale[0][0..2] (= scale[3][..]) */
+  MDS_ZS_SCALEF0 = MDS_ZS_SCALE0 + 3,              /* scale[1][0]                     */
+  MDS_ZS_SCALEF1,                                  /* scale[2][1]                     */
+  MDS_ZS_C000, MDS_ZS_C011, MDS_ZS_C101, MDS_ZS_C110, MDS_ZS_C202, MDS_ZS_C220,
+  MDS_ZS_C212, MDS_ZS_C221, MDS_ZS_C022, MDS_ZS_C122, MDS_ZS_C111,
+  MDS_ZS_STH, MDS_ZS_CTH,                          /* sin, cos of theta at the theta centroid */
+  MDS_ZS_COUNT
+};
+
 /* Reconstruction coefficients of one zone for the FP32-bulk path, in a
  * well-conditioned local form of para_recon (reconstruct.c:127-213):
  *   first fit   al = a + CLM*dqm + CLP*dqp,  ar = a + CRM*dqm + CRP*dqp
@@ -114,7 +135,10 @@ typedef struct mds_grid {
   /* Cartesian: uniform volume and areas (metric_cartesian.c:134-187) */
   double volume;
   double area[3];
-  /* spherical 1D: (n[0]+2*NG) * MDS_ZG_COUNT per-zone records, else NULL */
+  /* spherical: 1D (n[0]+2*NG) * MDS_ZG_COUNT per-zone records, 3D
+   * (n[0]+2*NG) * (n[1]+2*NG) * MDS_ZS_COUNT per-(i,j) records; else NULL.
+   * 3D spherical grids span theta over x1 in [-1,1] (startx[1] = -1,
+   * dx[1] = 2/n2, POLYTH map) and phi over [0, 2 pi). */
   double *zone;
 } mds_grid;
 

@@ -2,6 +2,8 @@
 
   host C   medusa_amd/csrc/host/*.c   gcc, strict IEEE (no FMA contraction)
   HIP      medusa_amd/csrc/hip/*.hip  hipcc --offload-arch=gfx950
+  driver   medusa_amd/csrc/driver/mds_driver.c -> medusa_amd/lib/mds_driver, a C
+           program in the shape of the reference main loop, linked to libmds.so
 
 FP64 kernels are compiled with -ffp-contract=off so they round exactly like
 the reference CPU build; the FP32-bulk functions re-enable contraction locally
@@ -70,6 +72,12 @@ def build(verbose=True, jobs=8):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lgomp", "-lm"], verbose)
     os.replace(tmp, LIB)
+    # the C host program in the shape of the reference main loop, on the C-ABI only
+    drv = os.path.join(CSRC, "driver", "mds_driver.c")
+    exe = os.path.join(LIBDIR, "mds_driver")
+    _run([CC, "-O2", "-std=c99", "-Wall", drv, "-o", exe + ".tmp", f"-L{LIBDIR}", "-lmds", "-lm",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

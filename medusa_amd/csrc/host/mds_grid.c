@@ -238,6 +238,130 @@ static int build_spherical1d(mds_grid *g)
   return 0;
 }
 
+static double i_xth1(double x, const void *c) { const maps *m = c; return (x * sin(thx(m, x))) * dthx(m, x); }
+static double i_xth2(double x, const void *c) { const maps *m = c; return ((x * x) * sin(thx(m, x))) * dthx(m, x); }
+
+/* 3D spherical grid on a uniform angular mesh (DJS = DKS = 1):
+ * metric_spherical_general.c:592-669 (moments), :708-740 (volume),
+ * :745-833 (areas, zero on the axis), :837-929 (connection with the
+ * pressure-balanced Gamma^theta_thetatheta), :941-1051 (gcov, gcon, scale). */
+static int build_spherical3d(mds_grid *g)
+{
+  maps m;
+  m.rlog = (g->rcoord == MDS_RCOORD_LOG);
+  m.pxt = 1.2;
+  m.palpha = 4;
+  m.pnorm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m.palpha + 1.0)) * (1.0 / pow(m.pxt, m.palpha))));
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
+  const int N1 = n1 + 2 * NG, N2 = n2 + 2 * NG;
+  for (int d = 0; d < 3; d++)
+    if (alloc_dir(g, d)) return MDS_ENOMEM;
+  for (int i = -NG; i < n1 + NG; i++) {
+    double x0 = g->startx[0] + i * g->dx[0], x1 = g->startx[0] + (i + 1) * g->dx[0];
+    g->Gamma[0][i + NG] = romberg(x0, x1, i_r2dr, &m);
+    g->beta[0][i + NG] = romberg(x0, x1, i_xr1, &m);
+    g->alpha[0][i + NG] = romberg(x0, x1, i_xr2, &m);
+  }
+  for (int j = -NG; j < n2 + NG; j++) {
+    double x0 = g->startx[1] + j * 1 * g->dx[1], x1 = g->startx[1] + (j + 1) * 1 * g->dx[1];
+    g->Gamma[1][j + NG] = romberg(x0, x1, i_sdth, &m);
+    g->beta[1][j + NG] = romberg(x0, x1, i_xth1, &m);
+    g->alpha[1][j + NG] = romberg(x0, x1, i_xth2, &m);
+  }
+  for (int k = -NG; k < n3 + NG; k++) {
+    double x0 = g->startx[2] + k * 1 * g->dx[2], x1 = g->startx[2] + (k + 1) * 1 * g->dx[2];
+    g->Gamma[2][k + NG] = 1 * g->dx[2];
+    g->beta[2][k + NG] = ((x1 * x1) - (x0 * x0)) / 2.0;
+    g->alpha[2][k + NG] = ((x1 * x1 * x1) - (x0 * x0 * x0)) / 3.0;
+  }
+  for (int d = 0; d < 3; d++)
+    recon_coeffs(g->alpha[d], g->beta[d], g->Gamma[d], g->startx[d], g->dx[d], g->n[d] + 2 * NG, g->rcoef[d]);
+
+  g->zone = calloc((size_t)N1 * N2 * MDS_ZS_COUNT, sizeof(double));
+  if (!g->zone) return MDS_ENOMEM;
+  const double dph = g->dx[2];
+  const int top_axis = fabs((g->startx[1] + n2 * g->dx[1]) - 1) < 1.0e-15;
+  for (int i = -NG; i < n1 + NG; i++) {
+    double xl = g->startx[0] + i * g->dx[0], xh = g->startx[0] + (i + 1) * g->dx[0];
+    double r2dr = romberg(xl, xh, i_r2dr, &m);
+    double vrf = r2dr;
+    double r2dr3 = romberg(xl, xh, i_r2dr3, &m);
+    double r5 = 0.2 * (pow(rx(&m, xh), 5) - pow(rx(&m, xl), 5));
+    for (int j = -NG; j < n2 + NG; j++) {
+      double tl = g->startx[1] + j * 1 * g->dx[1], th = g->startx[1] + (j + 1) * 1 * g->dx[1];
+      double sth = romberg(tl, th, i_sdth, &m);
+      double vthf = sth;
+      double th_s3 = romberg(tl, th, i_sdth3, &m);
+      double th_s3c = romberg(tl, th, i_s3dth, &m);
+      double *z = &g->zone[((size_t)(j + NG) * N1 + (i + NG)) * MDS_ZS_COUNT];
+      z[MDS_ZS_VOLUME] = ((r2dr * sth) * 1) * dph;
+      if (i >= 0 && i <= n1 && j >= 0 && j <= n2) {
+        double dph_int = 1 * dph;
+        z[MDS_ZS_AREA0] = (i_r2dr(xl, &m) * sth) * dph_int;
+        z[MDS_ZS_AREA1] = (r2dr * i_sdth(tl, &m)) * dph_int;
+        z[MDS_ZS_AREA2] = r2dr * sth;
+        if ((j == 0 && g->startx[1] == -1) || (j == n2 && top_axis)) z[MDS_ZS_AREA1] = 0.0;
+      }
+      double gc[3], gf0[3], gf1[3];
+      gc[0] = r2dr3 / vrf;
+      gc[1] = fabs(((r5 / vrf) * th_s3) / vthf);
+      gc[2] = fabs(((r5 / vrf) * th_s3c) / vthf);
+      gf0[0] = pow(drx(&m, xl), 2);
+      gf0[1] = (pow(rx(&m, xl), 2) * th_s3) / vthf;
+      gf0[2] = (pow(rx(&m, xl), 2) * th_s3c) / vthf;
+      gf1[0] = gc[0];
+      gf1[1] = fabs((r5 / vrf) * pow(dthx(&m, tl), 2));
+      gf1[2] = fabs((r5 / vrf) * pow(sin(thx(&m, tl)), 2));
+      for (int d = 0; d < 3; d++) {
+        z[MDS_ZS_GCOV0 + d] = gc[d];
+        z[MDS_ZS_GCOVF0 + d] = gf0[d];
+        z[MDS_ZS_GCOVF1 + d] = gf1[d];
+        z[MDS_ZS_GCON + d] = (fabs(gc[d]) > 1.0e-50) ? 1.0 / gc[d] : 0.0;
+        z[MDS_ZS_SCALE0 + d] = sqrt(gc[d]);
+      }
+      z[MDS_ZS_SCALEF0] = sqrt(gf0[0]);
+      z[MDS_ZS_SCALEF1] = sqrt(gf1[1]);
+      const double thc = thx(&m, g->beta[1][j + NG] / g->Gamma[1][j + NG]);
+      z[MDS_ZS_STH] = sin(thc);
+      z[MDS_ZS_CTH] = cos(thc);
+    }
+  }
+  /* connection coefficients of interior zones (areas above are complete) */
+  for (int i = 0; i < n1; i++) {
+    double xl = g->startx[0] + i * g->dx[0], xh = g->startx[0] + (i + 1) * g->dx[0];
+    double r2d2r = romberg(xl, xh, i_r2d2r, &m);
+    double r3i = romberg(xl, xh, i_r3, &m);
+    double rdr2 = romberg(xl, xh, i_rdr2, &m);
+    double vr = romberg(xl, xl + g->dx[0], i_r2dr, &m);
+    for (int j = 0; j < n2; j++) {
+      double tl = g->startx[1] + j * 1 * g->dx[1], th = g->startx[1] + (j + 1) * 1 * g->dx[1];
+      double vthf = romberg(tl, th, i_sdth, &m);
+      double th_s3 = romberg(tl, th, i_sdth3, &m);
+      double th_cd2 = romberg(tl, th, i_cdth2, &m);
+      double *z = &g->zone[((size_t)(j + NG) * N1 + (i + NG)) * MDS_ZS_COUNT];
+      const double *zu = &g->zone[((size_t)(j + 1 + NG) * N1 + (i + NG)) * MDS_ZS_COUNT];
+      double c101 = rdr2 / vr;
+      double c212 = th_cd2 / vthf;
+      z[MDS_ZS_C000] = r2d2r / vr;
+      z[MDS_ZS_C011] = -(r3i / vr) * (th_s3 / vthf);
+      z[MDS_ZS_C101] = c101;
+      z[MDS_ZS_C110] = c101;
+      z[MDS_ZS_C202] = c101;
+      z[MDS_ZS_C220] = c101;
+      z[MDS_ZS_C212] = c212;
+      z[MDS_ZS_C221] = c212;
+      z[MDS_ZS_C022] = ((-c101) * z[MDS_ZS_GCOV0 + 2]) / z[MDS_ZS_GCOV0 + 0];
+      z[MDS_ZS_C122] = ((-c212) * z[MDS_ZS_GCOV0 + 2]) / z[MDS_ZS_GCOV0 + 1];
+      double dA = zu[MDS_ZS_AREA1];
+      dA -= z[MDS_ZS_AREA1];
+      z[MDS_ZS_C111] = dA / z[MDS_ZS_VOLUME] - c212;
+    }
+  }
+  g->volume = 0.0;
+  g->area[0] = g->area[1] = g->area[2] = 0.0;
+  return 0;
+}
+
 int mds_grid_build(mds_grid *g)
 {
   if (!g || g->ndim < 1 || g->ndim > 3) return MDS_EINVAL;
@@ -247,7 +371,10 @@ int mds_grid_build(mds_grid *g)
     g->alpha[d] = g->beta[d] = g->Gamma[d] = g->rcoef[d] = NULL;
   }
   g->zone = NULL;
-  int rc = (g->geom == MDS_GEOM_SPHERICAL) ? build_spherical1d(g) : build_cartesian(g);
+  int rc;
+  if (g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3) rc = build_spherical3d(g);
+  else if (g->geom == MDS_GEOM_SPHERICAL) rc = build_spherical1d(g);
+  else rc = build_cartesian(g);
   if (rc) mds_grid_free(g);
   return rc;
 }

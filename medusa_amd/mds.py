@@ -23,6 +23,13 @@ TILE_TY3D = 1
 TILE_KC3D = 2
 TILE_FUSE1D = 3
 ZG_COUNT = 24
+# 3D spherical per-(i,j) record (MDS_ZS_*)
+(ZS_VOLUME, ZS_AREA0, ZS_AREA1, ZS_AREA2, ZS_GCOV0) = range(5)
+ZS_GCOVF0, ZS_GCOVF1, ZS_GCON, ZS_SCALE0 = 7, 10, 13, 16
+ZS_SCALEF0, ZS_SCALEF1 = 19, 20
+(ZS_C000, ZS_C011, ZS_C101, ZS_C110, ZS_C202, ZS_C220, ZS_C212, ZS_C221, ZS_C022, ZS_C122,
+ ZS_C111, ZS_STH, ZS_CTH, ZS_COUNT) = range(21, 35)
+BC_SPHERICAL_ORIGIN = 5
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -202,9 +209,13 @@ class GridBuilder:
         return np.ctypeslib.as_array(self.g.rcoef[d], shape=(nz * RC_COUNT,)).reshape(nz, RC_COUNT).copy()
 
     def zone(self):
+        """1D spherical: [n1+2NG][MDS_ZG_COUNT]; 3D spherical: [n2+2NG][n1+2NG][MDS_ZS_COUNT]."""
         if not self.g.zone:
             return None
         nz = self.n[0] + 2 * NG
+        if self.ndim == 3:
+            nj = self.n[1] + 2 * NG
+            return np.ctypeslib.as_array(self.g.zone, shape=(nj * nz * ZS_COUNT,)).reshape(nj, nz, ZS_COUNT).copy()
         return np.ctypeslib.as_array(self.g.zone, shape=(nz * ZG_COUNT,)).reshape(nz, ZG_COUNT).copy()
 
 

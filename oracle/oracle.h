@@ -41,6 +41,7 @@ extern "C" {
 #define ORC_BC_REFLECT  1
 #define ORC_BC_EXTRAP   2
 #define ORC_BC_PERIODIC 3
+#define ORC_BC_SPHERICAL_ORIGIN 5   /* theta: across the pole (phi + pi); decs.h:69 */
 
 #define ORC_GEOM_CARTESIAN 0
 #define ORC_GEOM_SPHERICAL 1

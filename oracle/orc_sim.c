@@ -143,6 +143,16 @@ static double f_sthdth_dx3(double x, const void *c) { const sphmap *m = c; retur
 static double f_sth3dth_dx(double x, const void *c) { const sphmap *m = c; return pow(sin(th_of_x(m, x)), 3) * dth_dx(m, x); }
 static double f_sthd2th_dx2(double x, const void *c) { const sphmap *m = c; return sin(th_of_x(m, x)) * d2th_dx2(m, x); }
 static double f_cthdth_dx2(double x, const void *c) { const sphmap *m = c; return cos(th_of_x(m, x)) * pow(dth_dx(m, x), 2); }
+static double f_xth1g(double x, const void *c) { const sphmap *m = c; return (x * sin(th_of_x(m, x))) * dth_dx(m, x); }
+static double f_xth2g(double x, const void *c) { const sphmap *m = c; return ((x * x) * sin(th_of_x(m, x))) * dth_dx(m, x); }
+
+static void sphmap_init(sphmap *m, int rcoord)
+{
+  m->rcoord = rcoord;
+  m->poly_xt = 1.2;
+  m->poly_alpha = 4;
+  m->poly_norm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m->poly_alpha + 1.0)) * (1.0 / pow(m->poly_xt, m->poly_alpha))));
+}
 
 static void build_geometry_cartesian(orc_sim *s)
 {
@@ -176,6 +186,120 @@ static void build_geometry_cartesian(orc_sim *s)
     for (int loc = 0; loc < 4; loc++)
       for (int d = 0; d < 3; d++) { g->gcov[loc][d] = 1.0; g->scale[loc][d] = 1.0; }
     for (int d = 0; d < 3; d++) g->gcon[d] = 1.0;
+  }
+}
+
+/* 2D/3D spherical (r, theta, phi) geometry on a uniform (non-dendritic) grid,
+ * metric_spherical_general.c:592-1051 with DJS = DKS = 1: x1 in [-1,1] through
+ * the POLYTH theta map, phi = x2 (dx[2] = 2 pi in 2D, init_coords :558-590).
+ * Volume, metric and scale for every zone (ghosts included; theta ghosts
+ * reflect through th_of_x), areas on the faces the reference loops over
+ * (i <= n1, j <= n2) with the axis faces forced to zero (:808-833), and the
+ * connection coefficients of interior zones with the pressure-balanced
+ * Gamma^theta_theta_theta = dA/dV - Gamma^phi_theta_phi (:905-915). */
+static void build_geometry_spherical(orc_sim *s)
+{
+  const orc_config *c = &s->c;
+  sphmap m;
+  sphmap_init(&m, c->rcoord);
+  const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2];
+  const double dph = c->ndim == 3 ? c->dx[2] : 2.0 * M_PI;
+  const int nn[3] = {n1, n2, n3};
+  for (int d = 0; d < 3; d++)
+    for (int q = 0; q < 3; q++) s->mom[q][d] = calloc(nn[d] + 2 * ORC_NG, sizeof(double));
+  /* init_interp_vol :592-669 */
+  for (int i = -ORC_NG; i < n1 + ORC_NG; i++) {
+    double x0 = c->startx[0] + i * c->dx[0], x1 = c->startx[0] + (i + 1) * c->dx[0];
+    s->mom[2][0][i + ORC_NG] = orc_romb(x0, x1, f_r2dr_dx, &m);
+    s->mom[1][0][i + ORC_NG] = orc_romb(x0, x1, f_xr1g, &m);
+    s->mom[0][0][i + ORC_NG] = orc_romb(x0, x1, f_xr2g, &m);
+  }
+  for (int j = -ORC_NG; j < n2 + ORC_NG; j++) {
+    double x0 = c->startx[1] + j * 1 * c->dx[1], x1 = c->startx[1] + (j + 1) * 1 * c->dx[1];
+    s->mom[2][1][j + ORC_NG] = orc_romb(x0, x1, f_sthdth_dx, &m);
+    s->mom[1][1][j + ORC_NG] = orc_romb(x0, x1, f_xth1g, &m);
+    s->mom[0][1][j + ORC_NG] = orc_romb(x0, x1, f_xth2g, &m);
+  }
+  for (int k = -ORC_NG; k < n3 + ORC_NG; k++) {
+    double x0 = c->startx[2] + k * 1 * c->dx[2], x1 = c->startx[2] + (k + 1) * 1 * c->dx[2];
+    s->mom[2][2][k + ORC_NG] = 1 * c->dx[2];
+    s->mom[1][2][k + ORC_NG] = (OSQR(x1) - OSQR(x0)) / 2.0;
+    s->mom[0][2][k + ORC_NG] = (OCUBE(x1) - OCUBE(x0)) / 3.0;
+  }
+  const int kg = c->ndim == 3 ? ORC_NG : 0;
+  for (int i = -ORC_NG; i < n1 + ORC_NG; i++) {
+    double x0lo = c->startx[0] + i * c->dx[0], x0hi = c->startx[0] + (i + 1) * c->dx[0];
+    double r2dr_int = orc_romb(x0lo, x0hi, f_r2dr_dx, &m);
+    double vrf = r2dr_int;
+    double rint_r2dr_dx3 = orc_romb(x0lo, x0hi, f_r2dr_dx3, &m);
+    double r5 = 0.2 * (pow(r_of_x(&m, x0hi), 5) - pow(r_of_x(&m, x0lo), 5));
+    for (int j = -ORC_NG; j < n2 + ORC_NG; j++) {
+      double x1lo = c->startx[1] + j * 1 * c->dx[1], x1hi = c->startx[1] + (j + 1) * 1 * c->dx[1];
+      double sthdth_int = orc_romb(x1lo, x1hi, f_sthdth_dx, &m);
+      double vthf = sthdth_int;
+      double thint_sthdth_dx3 = orc_romb(x1lo, x1hi, f_sthdth_dx3, &m);
+      double thint_sth3dth_dx = orc_romb(x1lo, x1hi, f_sth3dth_dx, &m);
+      orc_zone_geom g;
+      memset(&g, 0, sizeof(g));
+      g.volume = ((r2dr_int * sthdth_int) * 1) * dph;                          /* :708-740 */
+      if (i >= 0 && i <= n1 && j >= 0 && j <= n2) {                            /* :745-806 */
+        double dph_int = 1 * dph;
+        g.area[0] = (f_r2dr_dx(x0lo, &m) * sthdth_int) * dph_int;
+        g.area[1] = (r2dr_int * f_sthdth_dx(x1lo, &m)) * dph_int;
+        if (c->ndim == 3) g.area[2] = r2dr_int * sthdth_int;
+        if ((j == 0 && c->startx[1] == -1) || (j == n2 && fabs((c->startx[1] + n2 * c->dx[1]) - 1) < 1.0e-15))
+          g.area[1] = 0.0;                                                     /* :808-833 */
+      }
+      /* init_gcov :941-1010 */
+      g.gcov[0][0] = rint_r2dr_dx3 / vrf;
+      g.gcov[0][1] = fabs(((r5 / vrf) * thint_sthdth_dx3) / vthf);
+      g.gcov[0][2] = fabs(((r5 / vrf) * thint_sth3dth_dx) / vthf);
+      g.gcov[1][0] = pow(dr_dx(&m, x0lo), 2);
+      g.gcov[1][1] = (pow(r_of_x(&m, x0lo), 2) * thint_sthdth_dx3) / vthf;
+      g.gcov[1][2] = (pow(r_of_x(&m, x0lo), 2) * thint_sth3dth_dx) / vthf;
+      g.gcov[2][0] = g.gcov[0][0];
+      g.gcov[2][1] = fabs((r5 / vrf) * pow(dth_dx(&m, x1lo), 2));
+      g.gcov[2][2] = fabs((r5 / vrf) * pow(sin(th_of_x(&m, x1lo)), 2));
+      if (c->ndim == 3)
+        for (int d = 0; d < 3; d++) g.gcov[3][d] = g.gcov[0][d];
+      for (int d = 0; d < 3; d++)                                              /* :1016-1032 */
+        g.gcon[d] = (fabs(g.gcov[0][d]) > 1.0e-50) ? 1.0 / g.gcov[0][d] : 0.0;
+      for (int loc = 0; loc <= c->ndim; loc++)                                 /* :1037-1051 */
+        for (int d = 0; d < 3; d++) g.scale[loc][d] = sqrt(g.gcov[loc][d]);
+      for (int k = -kg; k < n3 + kg; k++) s->geom[zid(s, i, j, k)] = g;
+    }
+  }
+  /* init_conn :837-929, interior zones (after volumes and areas) */
+  for (int i = 0; i < n1; i++) {
+    double x0lo = c->startx[0] + i * c->dx[0], x0hi = c->startx[0] + (i + 1) * c->dx[0];
+    double rint_r2d2r_dx2 = orc_romb(x0lo, x0hi, f_r2d2r_dx2, &m);
+    double rint_r3 = orc_romb(x0lo, x0hi, f_r3, &m);
+    double rint_rdr_dx2 = orc_romb(x0lo, x0hi, f_rdr_dx2, &m);
+    double vrf = orc_romb(x0lo, x0lo + c->dx[0], f_r2dr_dx, &m);      /* Vr(x0lo) */
+    for (int j = 0; j < n2; j++) {
+      double x1lo = c->startx[1] + j * 1 * c->dx[1], x1hi = c->startx[1] + (j + 1) * 1 * c->dx[1];
+      double vthf = orc_romb(x1lo, x1hi, f_sthdth_dx, &m);
+      double thint_sthdth_dx3 = orc_romb(x1lo, x1hi, f_sthdth_dx3, &m);
+      double thint_cthdth_dx2 = orc_romb(x1lo, x1hi, f_cthdth_dx2, &m);
+      for (int k = 0; k < n3; k++) {
+        orc_zone_geom *g = &s->geom[zid(s, i, j, k)];
+        memset(g->conn, 0, sizeof(g->conn));
+        g->conn[0][0][0] = rint_r2d2r_dx2 / vrf;
+        g->conn[0][1][1] = -(rint_r3 / vrf) * (thint_sthdth_dx3 / vthf);
+        g->conn[1][0][1] = rint_rdr_dx2 / vrf;
+        g->conn[1][1][0] = g->conn[1][0][1];
+        g->conn[2][0][2] = g->conn[1][0][1];
+        g->conn[2][2][0] = g->conn[2][0][2];
+        g->conn[2][1][2] = thint_cthdth_dx2 / vthf;
+        g->conn[2][2][1] = g->conn[2][1][2];
+        g->conn[0][2][2] = ((-g->conn[2][2][0]) * g->gcov[0][2]) / g->gcov[0][0];
+        g->conn[1][2][2] = ((-g->conn[2][2][1]) * g->gcov[0][2]) / g->gcov[0][1];
+        double dA = s->geom[zid(s, i, j + 1, k)].area[1];
+        dA -= g->area[1];
+        double dV = g->volume;
+        g->conn[1][1][1] = dA / dV - g->conn[2][1][2];
+      }
+    }
   }
 }
 
@@ -278,7 +402,8 @@ orc_sim *orc_create(const orc_config *cfg)
   s->shock = calloc(s->nall, 1);
   s->geom = calloc(s->nall, sizeof(orc_zone_geom));
   for (int d = 0; d < 3; d++) s->flux[d] = calloc(s->nall * s->nvars, sizeof(double));
-  if (cfg->geom == ORC_GEOM_SPHERICAL) build_geometry_spherical1d(s);
+  if (cfg->geom == ORC_GEOM_SPHERICAL && cfg->ndim == 1) build_geometry_spherical1d(s);
+  else if (cfg->geom == ORC_GEOM_SPHERICAL) build_geometry_spherical(s);
   else build_geometry_cartesian(s);
   s->t = 0.0;
   s->istep = 0;
@@ -348,8 +473,14 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
     for (int a = 0; a < c->n[o1]; a++)
       for (int gidx = 0; gidx < ORC_NG; gidx++) {
         int ig = side ? nd + gidx : -ORC_NG + gidx;   /* ghost index along d */
-        int src = 0, flip = 0, extrap = 0;
+        int src = 0, flip = 0, extrap = 0, pole = 0;
         switch (code) {
+          case ORC_BC_SPHERICAL_ORIGIN:
+            /* theta boundary across the pole: mirror in j, phi + pi, flip v2 and v3
+             * (bc.c:1094-1121, 1235-1262); other directions are not supported */
+            src = side ? 2 * nd - ig - 1 : -1 - ig;
+            pole = (d == 1);
+            break;
           case ORC_BC_OUTFLOW:
             src = side ? nd - 1 : 0;
             break;
@@ -370,6 +501,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
         int idx[3], sidx[3];
         idx[d] = ig; idx[o1] = a; idx[o2] = b;
         sidx[d] = src; sidx[o1] = a; sidx[o2] = b;
+        if (pole) sidx[2] = (idx[2] + c->n[2] / 2) % c->n[2];
         double *dst = &P[zid(s, idx[0], idx[1], idx[2]) * nv];
         if (extrap) {
           int i0, i1;
@@ -387,6 +519,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           const double *sp = &P[zid(s, sidx[0], sidx[1], sidx[2]) * nv];
           for (int v = 0; v < nv; v++) dst[v] = sp[v];
           if (flip) dst[ORC_U1 + d] *= -1.0;
+          if (pole) { dst[ORC_U2] *= -1.0; dst[ORC_U3] *= -1.0; }
         }
       }
 }
@@ -461,6 +594,47 @@ static void interp_pencil(orc_sim *s, double pen[][2 * ORC_NG], double pl[][2 * 
   }
 }
 
+/* Polar-axis treatment of the phi faces in the last theta row
+ * (step.c:1299-1397): first-order states, with the velocities of zones k-1
+ * and k carried through Cartesian components (rotation at the zone centre)
+ * into the frame of the face at phi_k.  Only the face's two states are
+ * needed: pright of zone k-1 and pleft of zone k. */
+static void polar_states(orc_sim *s, const double *P, int i, int j, int k, double pen[][2 * ORC_NG],
+                         double pleft[][2 * ORC_NG], double pright[][2 * ORC_NG])
+{
+  const orc_config *c = &s->c;
+  sphmap m;
+  sphmap_init(&m, c->rcoord);
+  const int nv = s->nvars;
+  const double th = th_of_x(&m, s->mom[1][1][j + ORC_NG] / s->mom[2][1][j + ORC_NG]);
+  const double sth = sin(th), cth = cos(th);
+  for (int kt = k - 1; kt <= k; kt++) {
+    long z = zid(s, i, j, kt);
+    const orc_zone_geom *g = &s->geom[z];
+    double ph = c->startx[2] + (kt + 0.5) * 1 * c->dx[2];
+    double sph = sin(ph), cph = cos(ph);
+    double lam[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
+    double v[3];
+    for (int l = 0; l < 3; l++) {
+      v[l] = 0.0;
+      for (int mm = 0; mm < 3; mm++) v[l] += lam[l][mm] * P[z * nv + ORC_U1 + mm] * g->scale[0][mm];
+    }
+    /* the face at phi_k: the lower face of zone k, the upper face of zone k-1 */
+    ph = c->startx[2] + k * 1 * c->dx[2];
+    sph = sin(ph);
+    cph = cos(ph);
+    double lf[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
+    const double *scf = s->geom[zid(s, i, j, k)].scale[3];
+    const int q = ORC_NG + kt - k;
+    double (*dst)[2 * ORC_NG] = (kt == k) ? pleft : pright;
+    for (int vv = 0; vv < s->ninterp; vv++) dst[vv][q] = pen[vv][q];
+    for (int l = 0; l < 3; l++) {
+      dst[ORC_U1 + l][q] = 0.0;
+      for (int mm = 0; mm < 3; mm++) dst[ORC_U1 + l][q] += lf[mm][l] * v[mm] / scf[l];
+    }
+  }
+}
+
 /* One direction of the flux sweep: step.c:500-562 (d=0), 736-760 (d=1),
  * 1266-1421 (d=2).  Face (i,j,k) is the lower face of zone (i,j,k). */
 static void sweep_dir(orc_sim *s, const double *P, int d)
@@ -488,6 +662,8 @@ static void sweep_dir(orc_sim *s, const double *P, int d)
           pen[nv + 1][q] = s->eos[z * NEOS + E_GAMMA];
         }
         interp_pencil(s, pen, pleft, pright, al, be, ga, x);
+        if (d == 2 && c->geom == ORC_GEOM_SPHERICAL && j == c->n[1] - 1)
+          polar_states(s, P, i, j, k, pen, pleft, pright);
         double pl[ORC_NSIZE], pr[ORC_NSIZE], flux[ORC_NSIZE], vr[3], ss;
         for (int vv = 0; vv < s->ninterp; vv++) {
           pl[vv] = pright[vv][ORC_NG - 1];
