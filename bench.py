@@ -75,7 +75,22 @@ def workload(name, n):
                     label=f"ensemble of {nprob} spherical log-r Riemann columns, 256-8192 zones (seeded), "
                           f"{zones} zones, per-problem dt, 500 steps",
                     kernel={"fp32": "sweep1d_f32<1,true>", "fp64": "sweep1d_x64<3,true>"}, n=nprob)
+    if name == "star3d":
+        n1 = n or 640
+        n2, n3 = max(8, n1 // 5), max(8, (2 * n1) // 5)
+        T = mds_table()
+        g, prob, prims = problems.star3d(n1, n2, n3, table=T)
+        return dict(probs=[(prob, prims)], nvar=6, steps=200, zones=n1 * n2 * n3, gam=[4.0 / 3.0], table=T,
+                    label=f"star3d {n1}x{n2}x{n3} spherical (log r 1e5-2e9 cm, uniform angular mesh), "
+                          f"n=3 polytrope rho_c=1e10, synthetic FP32 EOS table 180x180x50, monopole gravity",
+                    kernel={"fp32": "sph3d stage: eos + 3 flux + update kernels (float state)",
+                            "fp64": "sph3d stage: eos + 3 flux + update kernels (double state)"}, n=n1)
     raise SystemExit(f"unknown workload {name}")
+
+
+def mds_table():
+    from medusa_amd import mds
+    return mds.EosTable()
 
 
 def run_path(mds, W, precision, steps, warmup, tiles=None):
@@ -136,12 +151,25 @@ def cpu_baseline(name, W, budget_s):
     elif name == "ensemble":
         sample = W["probs"][:64]
         what = "first 64 columns of the ensemble"
+    elif name == "star3d":
+        n1 = W["n"]
+        g, prob, prims = problems.star3d(n1, 8, 8, table=W["table"])
+        sample = [(prob, prims)]
+        what = f"the same star on a {n1}x8x8 spherical grid"
     else:
         sample = W["probs"]
         what = "the full grid"
     sims, zones = [], 0
     for prob, prims in sample:
-        sim = oracle_for(prob, fast=True)
+        if prob.get("eos") is not None or prob.get("monopole"):
+            kw = dict(nvars=6, monopole=bool(prob.get("monopole")))
+            if prob.get("eos") is not None:
+                t = prob["eos"].t
+                kw["eos_tab"] = po.eos_tab(dict(nr=t.nr, nt=t.nt, ny=t.ny, lr0=t.lr0, lr1=t.lr1, lt0=t.lt0,
+                                                lt1=t.lt1, ye0=t.ye0, ye1=t.ye1), prob["eos"].data)
+            sim = oracle_for(prob, fast=True, **kw)
+        else:
+            sim = oracle_for(prob, fast=True)
         sim.set_prims(prims)
         sims.append(sim)
         zones += int(np.prod(prims.shape[1:]))
@@ -174,7 +202,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="kh2d", choices=["kh2d", "rt3d", "sod1d", "ensemble"])
+    ap.add_argument("--workload", default="kh2d", choices=["kh2d", "rt3d", "sod1d", "ensemble", "star3d"])
     ap.add_argument("--n", type=int, default=None, help="grid size (problem count for ensemble)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time budget")
     ap.add_argument("--tiles", default=None,

@@ -50,14 +50,15 @@ int mds_init_sph_riemann(const mds_grid *g, double gam, double rho_in, double p_
  * fp = 10^U(1,3).  out[7*p + {0..6}] = {n, gam, rho_in, p_in, r0, frho, fp}. */
 int mds_ensemble_params(int nprob, int nmin, int nmax, unsigned long long seed, double *out);
 
-/* Polytropic star on a spherical grid (intent of problems/init_ccsn.c:51-394
- * with a synthetic progenitor): Lane-Emden index n_index, rho = rho_c
- * theta^n, P = K rho^(1+1/n) inside the first zero and (rho_atm,
- * K rho_atm^(1+1/n)) outside, at rest; density perturbed by exp(sigma z),
- * z standard normal (Box-Muller on the seeded stream, two draws per zone),
- * at constant pressure; Ye = 0.45 + 0.05 (1 - (rho/rho_c)^(1/3)).  With a
- * table, u = rho e_model(T) with T solving P_model = P (eos_seed: the
- * table's model parameters); without, u = P / (gam - 1).  prims holds six
+/* Polytropic star on a spherical grid (1D column or 3D), intent of
+ * problems/init_ccsn.c:51-394 with a synthetic progenitor: Lane-Emden index
+ * n_index on the radial zones, rho = rho_c theta^n, P = K rho^(1+1/n) inside
+ * the first zero and (rho_atm, K rho_atm^(1+1/n)) outside, at rest; Ye =
+ * 0.45 + 0.05 (1 - (rho/rho_c)^(1/3)); specific internal energy e from the
+ * table's model at the T that gives P (eos_seed: the table's model
+ * parameters), or P / ((gam - 1) rho) without a table.  Each zone's density
+ * is then multiplied by exp(sigma z), z standard normal (Box-Muller on the
+ * seeded stream, two draws per zone), at constant e and Ye.  prims holds six
  * variables (rho, u, v1, v2, v3, Ye) over the grid's zones. */
 int mds_init_polytrope(const mds_grid *g, double n_index, double rho_c, double K, double rho_atm, double gam,
                        const mds_eos_table *tab, unsigned long long eos_seed, double sigma,

@@ -20,6 +20,9 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
 extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
                                           hipStream_t s);
 hipError_t mds_eos_to_device(const mds_eos_table *t, DevEos *d, void **alloc);
+extern "C" hipError_t mds_launch_sph3d(const Sph3Launch *L, const DevProb *pb, DevClock *clock,
+                                       unsigned long long *counters, const DevEos *eos, const void *sp,
+                                       const void *sph, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a, const void *sp,
                                          const void *sph, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, const Sweep3DArgs *a, const void *sp,
@@ -47,6 +50,8 @@ struct mds_ctx {
   void *eos_alloc = nullptr;
   int mono = 0;                    // monopole gravity
   double *dgrav = nullptr;         // monopole field at zone centres (FP64), packed like the state
+  int sph3 = 0;                    // 3D spherical grid (direction-split stage)
+  Sph3Launch s3{};                 // its work buffers
   DevProb *dprob = nullptr;
   DevClock *dclock = nullptr;
   int2 *dseg = nullptr;            // 1D segment table of the single-stage kernels
@@ -151,10 +156,13 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
     cs = sqrt((pb.gam * press) / rho);
   }
   int i = (int)(q % pb.n[0]);
+  int j = (int)((q / pb.n[0]) % pb.n[1]);
   double loc = 0.0;
   for (int d = 0; d < pb.ndim; d++) {
     double sc = 1.0;
-    if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
+    if (pb.sph && pb.ndim == 3)
+      sc = pb.zone[((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT + MDS_ZS_SCALE0 + d];
+    else if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
     double v = sp.v[2 + d][pb.off + q];
     loc += (fabs(v) + (cs / sc)) / pb.dx[d];
   }
@@ -181,7 +189,12 @@ __global__ void totals_kernel(const DevProb *prob, int p, long long nz, int nv, 
     double rho = sp.v[0][pb.off + q], uu = sp.v[1][pb.off + q];
     double vol = pb.volume;
     double g[3] = {1.0, 1.0, 1.0};
-    if (pb.sph) {
+    if (pb.sph && pb.ndim == 3) {
+      const int i = (int)(q % pb.n[0]), j = (int)((q / pb.n[0]) % pb.n[1]);
+      const double *zs = pb.zone + ((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT;
+      vol = zs[MDS_ZS_VOLUME];
+      g[0] = zs[MDS_ZS_GCOV0]; g[1] = zs[MDS_ZS_GCOV0 + 1]; g[2] = zs[MDS_ZS_GCOV0 + 2];
+    } else if (pb.sph) {
       const double *zg = pb.zone + (size_t)(q + MDS_NG) * MDS_ZG_COUNT;
       vol = zg[MDS_ZG_VOLUME];
       g[0] = zg[MDS_ZG_GCOV00]; g[1] = zg[MDS_ZG_GCOV01]; g[2] = zg[MDS_ZG_GCOV02];
@@ -264,6 +277,12 @@ extern "C" int mds_destroy(mds_ctx *c)
   for (int k = 0; k < 6; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
   if (c->eos_alloc) hipFree(c->eos_alloc);
   if (c->dgrav) hipFree(c->dgrav);
+  for (int d = 0; d < 3; d++) if (c->s3.F[d]) hipFree(c->s3.F[d]);
+  if (c->s3.P) hipFree(c->s3.P);
+  if (c->s3.G) hipFree(c->s3.G);
+  if (c->s3.CS) hipFree(c->s3.CS);
+  if (c->s3.ms) hipFree(c->s3.ms);
+  if (c->s3.grav) hipFree(c->s3.grav);
   for (void *g : c->geo_allocs) hipFree(g);
   if (c->dprob) hipFree(c->dprob);
   if (c->dclock) hipFree(c->dclock);
@@ -302,18 +321,19 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
   if (c->ndim == 3) c->nv = 3;
-  if (c->sph && c->ndim != 1) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D only");
+  if (c->sph && c->ndim == 2) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D or 3D");
+  c->sph3 = c->sph && c->ndim == 3;
   if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
-  if (c->sph && c->precision == MDS_FP32_BULK && c->nv != 1)
+  if (c->sph && !c->sph3 && c->precision == MDS_FP32_BULK && c->nv != 1)
     return set_err(c, MDS_ENOTIMPL, "FP32-bulk spherical columns carry the radial velocity only");
   c->tab = probs[0].eos != nullptr;
   c->mono = probs[0].monopole != 0;
   for (int p = 1; p < nprob; p++)
     if ((probs[p].eos != probs[0].eos) || ((probs[p].monopole != 0) != (c->mono != 0)))
       return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same EOS and gravity", p);
-  if ((c->tab || c->mono) && !(c->sph && c->ndim == 1))
-    return set_err(c, MDS_ENOTIMPL, "tabulated EOS and monopole gravity are supported for spherical columns");
+  if ((c->tab || c->mono) && !c->sph)
+    return set_err(c, MDS_ENOTIMPL, "tabulated EOS and monopole gravity are supported on spherical grids");
   if (c->tab) {
     const mds_eos_table *t = probs[0].eos;
     if (!t->data || t->nr < 2 || t->nt < 3 || t->ny < 2 || t->nt - 1 > (1 << 20))
@@ -338,7 +358,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       if (g->n[d] < 4) return set_err(c, MDS_EINVAL, "problem %d: need at least 4 zones per active direction", p);
       for (int side = 0; side < 2; side++) {
         int b = side ? P.bc_hi[d] : P.bc_lo[d];
-        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC)
+        const bool pole = c->sph3 && d == 1 && b == MDS_BC_SPHERICAL_ORIGIN && (g->n[2] % 2) == 0;
+        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole)
           return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
       }
     }
@@ -379,9 +400,11 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     if (c->sph) {
       if (!g->zone) return set_err(c, MDS_EINVAL, "problem %d: spherical grid without zone records", p);
       const int nzq = g->n[0] + 2 * MDS_NG;
-      int rc1 = upload_geo(c, g->zone, (size_t)nzq * MDS_ZG_COUNT * sizeof(double), (const void **)&d.zone);
+      const size_t zbytes = c->sph3 ? (size_t)nzq * (g->n[1] + 2 * MDS_NG) * MDS_ZS_COUNT * sizeof(double)
+                                    : (size_t)nzq * MDS_ZG_COUNT * sizeof(double);
+      int rc1 = upload_geo(c, g->zone, zbytes, (const void **)&d.zone);
       if (rc1) return rc1;
-      if (c->precision == MDS_FP32_BULK) {
+      if (c->precision == MDS_FP32_BULK && !c->sph3) {
         std::vector<ZoneF32> z32((size_t)nzq);
         for (int q = 0; q < nzq; q++) {
           const double *zg = g->zone + (size_t)q * MDS_ZG_COUNT;
@@ -415,9 +438,31 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     HIPCHK(c, hipMemset(c->p[k], 0, (size_t)c->nzone * rs));
     HIPCHK(c, hipMemset(c->ph[k], 0, (size_t)c->nzone * rs));
   }
-  if (c->mono) {
+  if (c->mono && !c->sph3) {
     HIPCHK(c, hipMalloc(&c->dgrav, sizeof(double) * (size_t)c->nzone));
     HIPCHK(c, hipMemset(c->dgrav, 0, sizeof(double) * (size_t)c->nzone));
+  }
+  if (c->sph3) {
+    const DevProb &h = c->hprob[0];
+    Sph3Launch &L = c->s3;
+    L.fp32 = c->precision == MDS_FP32_BULK;
+    L.tab = c->tab;
+    L.mono = c->mono;
+    const int nslot = c->tab ? 6 : 5;
+    for (int d = 0; d < 3; d++) {
+      L.nf[d] = (long long)(h.n[0] + (d == 0)) * (h.n[1] + (d == 1)) * (h.n[2] + (d == 2));
+      HIPCHK(c, hipMalloc(&L.F[d], sizeof(double) * nslot * (size_t)L.nf[d]));
+      HIPCHK(c, hipMemset(L.F[d], 0, sizeof(double) * nslot * (size_t)L.nf[d]));
+    }
+    if (c->tab) {
+      HIPCHK(c, hipMalloc(&L.P, rs * (size_t)c->nzone));
+      HIPCHK(c, hipMalloc(&L.G, rs * (size_t)c->nzone));
+      HIPCHK(c, hipMalloc(&L.CS, rs * (size_t)c->nzone));
+    }
+    if (c->mono) {
+      HIPCHK(c, hipMalloc(&L.ms, sizeof(double) * h.n[0]));
+      HIPCHK(c, hipMalloc(&L.grav, sizeof(double) * h.n[0]));
+    }
   }
   HIPCHK(c, hipMalloc(&c->dprob, sizeof(DevProb) * nprob));
   HIPCHK(c, hipMemcpy(c->dprob, c->hprob.data(), sizeof(DevProb) * nprob, hipMemcpyHostToDevice));
@@ -579,6 +624,15 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
       StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
       if (c->mono) HIPCHK(c, mds_launch_monopole(0, c->nprob, c->dprob, stage == 0 ? &sp : &sh, c->dgrav, c->st));
       HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, c->tab, c->mono, &a, &sp, &sh, stage, c->st));
+    }
+  } else if (c->sph3) {
+    c->s3.flags = c->dflags;
+    if (fp32) {
+      StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
+      HIPCHK(c, mds_launch_sph3d(&c->s3, &c->hprob[0], c->dclock, c->dcount, &c->deos, &sp, &sh, stage, c->st));
+    } else {
+      StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
+      HIPCHK(c, mds_launch_sph3d(&c->s3, &c->hprob[0], c->dclock, c->dcount, &c->deos, &sp, &sh, stage, c->st));
     }
   } else if (c->ndim == 3) {
     Sweep3DArgs a;
@@ -889,6 +943,7 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
+  if (c->sph3) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // fixed shape: one thread per face/zone
   // candidate shapes: 1D single/two-stage kernels; 2D rows per wavefront;
   // 3D (rows per workgroup, planes per workgroup)
   std::vector<std::pair<int, int>> cand;

@@ -97,5 +97,15 @@ struct Sweep3DArgs {
   int src;                 // source terms present (fixed gravity)
 };
 
+// buffers of the direction-split 3D spherical stage
+struct Sph3Launch {
+  int fp32, tab, mono;
+  double *F[3];            // face fluxes per direction, [slot][face]
+  long long nf[3];         // faces per direction
+  void *P, *G, *CS;        // table EOS arrays (state precision)
+  double *ms, *grav;       // shell masses and monopole field (n1 each)
+  unsigned char *flags;    // shock tags
+};
+
 #define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D stage sweep
 #define MDS_SEG1D2 56      // output zones per 64-lane wave in the 1D two-stage sweep

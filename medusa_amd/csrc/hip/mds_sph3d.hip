@@ -1,0 +1,531 @@
+// mds_sph3d.hip -- RK stage of a 3D spherical (r, theta, phi) grid (gfx950).
+//
+// Direction-split, as the reference's prim_to_fluxes3 + push (step.c:1088-1430,
+// 327-478): per stage
+//   eos_kernel       tabulated EOS of every zone (eos.c:4-32; skipped for a
+//                    gamma law, which is evaluated in place)
+//   shell kernels    monopole gravity per radial shell (gravity.c:145-498)
+//   flux<0>          radial faces: HLLC, shock tags on zones i, i-1, i-2
+//   flux<1>, flux<2> theta and phi faces: HLLE next to a tagged zone, else
+//                    HLLC (fluid.c:189-210); the phi faces of the last theta
+//                    row use the polar-axis states of step.c:1299-1397
+//   update_kernel    geometric and gravity sources, flux divergence, the RK
+//                    update, u -> p and the dt limits (step.c:327-478)
+// One thread per face or zone, i fastest.  Ghost zones are not stored: every
+// stencil index outside the grid is mapped onto the zone the reference's
+// ghost fill copies there (bc.c:705-1421): radial REFLECT/OUTFLOW, theta
+// across the pole (SPHERICAL_ORIGIN: mirror in j, phi + pi, v2 and v3
+// flipped) or REFLECT/OUTFLOW, phi PERIODIC.  Face fluxes are kept in FP64
+// (area weighted) for both precisions.
+#include "mds_internal.h"
+#include "mds_physics.h"
+#include "mds_eos.h"
+
+namespace {
+
+struct Sph3Args {
+  DevProb pb;
+  DevClock *clock;
+  unsigned long long *counters;
+  unsigned char *flags;       // 1: no shock tag, 0: tagged (interior zones)
+  DevEos eos;
+  double *F[3];               // face fluxes [slot][face] per direction, slot 5 = Ye
+  long long nf[3];            // faces per direction
+  const void *P, *G, *CS;     // table EOS arrays (state precision)
+  const double *grav;         // monopole field per radial shell, or null
+};
+
+MDS_DEV long long zid3(const DevProb &pb, int i, int j, int k)
+{
+  return (long long)i + (long long)pb.n[0] * ((long long)j + (long long)pb.n[1] * k);
+}
+
+MDS_DEV long long fid3(const DevProb &pb, int D, int i, int j, int k)
+{
+  const long long a = pb.n[0] + (D == 0), b = pb.n[1] + (D == 1);
+  return (long long)i + a * ((long long)j + b * k);
+}
+
+MDS_DEV const double *rec3(const DevProb &pb, int i, int j)
+{
+  const int N1 = pb.n[0] + 2 * MDS_NG;
+  return pb.zone + ((size_t)(j + MDS_NG) * N1 + (i + MDS_NG)) * MDS_ZS_COUNT;
+}
+
+// Map a stencil zone onto the zone the ghost fill copies there; sgn gets the
+// velocity sign flips.
+MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3])
+{
+  sgn[0] = sgn[1] = sgn[2] = 1.0;
+  int *c = D == 0 ? &i : (D == 1 ? &j : &k);
+  const int n = pb.n[D];
+  if (*c >= 0 && *c < n) return;
+  const int side = *c >= n;
+  const int code = side ? pb.bc_hi[D] : pb.bc_lo[D];
+  const int ig = *c;
+  if (code == MDS_BC_PERIODIC) {
+    *c = side ? ig - n : ig + n;
+  } else if (code == MDS_BC_REFLECT) {
+    if (D == 2) *c = side ? n - 1 : 0;                 // bc.c:1285-1300: copy, not mirror
+    else *c = side ? 2 * n - 1 - ig : -1 - ig;
+    sgn[D] = -1.0;
+  } else if (code == MDS_BC_SPHERICAL_ORIGIN && D == 1) {
+    *c = side ? 2 * n - 1 - ig : -1 - ig;
+    k = (k + pb.n[2] / 2) % pb.n[2];
+    sgn[1] = sgn[2] = -1.0;
+  } else {
+    *c = side ? n - 1 : 0;
+  }
+}
+
+template <class R>
+MDS_DEV void zone_state(const Sph3Args &a, const StateArrays<R> &in, long long z, int tab, double gam, double q[8])
+{
+  // q = rho, u, v1, v2, v3, Ye, P, Gamma
+  q[0] = in.v[0][z];
+  q[1] = in.v[1][z];
+  for (int d = 0; d < 3; d++) q[2 + d] = in.v[2 + d][z];
+  if (tab) {
+    q[5] = in.v[5][z];
+    q[6] = ((const R *)a.P)[z];
+    q[7] = ((const R *)a.G)[z];
+  } else {
+    q[5] = 0.0;
+    q[6] = (gam - 1.0) * q[1];
+    q[7] = gam;
+  }
+}
+
+// Polar-axis states of the phi face at phi_k between zones k-1 (L) and k (R):
+// velocities rotated through Cartesian components (step.c:1299-1397)
+MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, double vl[3], double vr[3])
+{
+  const double sth = rc[MDS_ZS_STH], cth = rc[MDS_ZS_CTH];
+  double *vv[2] = {vl, vr};
+  for (int s = 0; s < 2; s++) {
+    const int kt = k - 1 + s;
+    double ph = pb.startx[2] + (kt + 0.5) * 1 * pb.dx[2];
+    double sph = sin(ph), cph = cos(ph);
+    const double lam[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
+    double v[3];
+    for (int l = 0; l < 3; l++) {
+      v[l] = 0.0;
+      for (int m = 0; m < 3; m++) v[l] += lam[l][m] * vv[s][m] * rc[MDS_ZS_SCALE0 + m];
+    }
+    ph = pb.startx[2] + k * 1 * pb.dx[2];
+    sph = sin(ph);
+    cph = cos(ph);
+    const double lf[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
+    for (int l = 0; l < 3; l++) {
+      double o = 0.0;
+      for (int m = 0; m < 3; m++) o += lf[m][l] * v[m] / rc[MDS_ZS_SCALE0 + l];
+      vv[s][l] = o;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// per-step shock-tag reset and the table EOS
+// ---------------------------------------------------------------------------
+__global__ void sph3_reset_flags(unsigned char *flags, long long n)
+{
+  long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z < n) flags[z] = 1;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void sph3_eos_kernel(Sph3Args a, StateArrays<R> in)
+{
+  const DevProb &pb = a.pb;
+  const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
+  const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  double press, lt;
+  if (sizeof(R) == 8) {
+    double g1, cs;
+    eos_fill_x64(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt);
+    ((R *)a.P)[z] = (R)press;
+    ((R *)a.G)[z] = (R)g1;
+    ((R *)a.CS)[z] = (R)cs;
+  } else {
+    float g1, cs;
+    eos_fill_f32(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt);
+    ((R *)a.P)[z] = (R)press;
+    ((R *)a.G)[z] = (R)g1;
+    ((R *)a.CS)[z] = (R)cs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// monopole gravity on radial shells: masses, then enclosed mass and the
+// zone-centred field (gravity.c:229-233, 420-450)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(256) void sph3_shell_mass(Sph3Args a, StateArrays<R> in, double *ms)
+{
+  const DevProb &pb = a.pb;
+  const int i = blockIdx.x;
+  const int nj = pb.n[1], nk = pb.n[2];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < nj * nk; t += blockDim.x) {
+    const int j = t % nj, k = t / nj;
+    s += rec3(pb, i, j)[MDS_ZS_VOLUME] * (double)in.v[0][zid3(pb, i, j, k)];
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ms[i] = sh[0];
+}
+
+__global__ void sph3_shell_field(Sph3Args a, const double *ms, double *grav)
+{
+  // one thread: n1 is a few thousand at most
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const DevProb &pb = a.pb;
+  const bool logr = pb.rcoord == MDS_RCOORD_LOG;
+  const double GN = 6.67384e-8;
+  double mr = 0.0, ge = 0.0;
+  for (int i = 0; i < pb.n[0]; i++) {
+    mr = mr + ms[i];
+    const double xe = pb.startx[0] + (i + 1) * pb.dx[0];
+    const double r = logr ? exp(xe) : fabs(xe);
+    const double gn = ((-GN * mr) / (r * r)) * (logr ? exp(xe) : 1.0);
+    const double xl = pb.startx[0] + i * pb.dx[0];
+    const double A = (gn - ge) / pb.dx[0];
+    const double B = ge - A * xl;
+    const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
+    grav[i] = (A * be) / ga + B;
+    ge = gn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// face fluxes in direction D
+// ---------------------------------------------------------------------------
+template <class R, int D, int TAB>
+__global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0] + (D == 0), n2 = pb.n[1] + (D == 1), n3 = pb.n[2] + (D == 2);
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = f < (long long)n1 * n2 * n3;
+  const long long fl = live ? f : 0;
+  const int fi = (int)(fl % n1), fj = (int)((fl / n1) % n2), fk = (int)(fl / ((long long)n1 * n2));
+  DevClock *ck = a.clock;
+  const bool active = ck->active;
+  const double gam = pb.gam;
+  // the 4-zone pencil along D
+  double q[4][8];
+  for (int s = 0; s < 4; s++) {
+    int i = fi, j = fj, k = fk;
+    if (D == 0) i += s - 2;
+    else if (D == 1) j += s - 2;
+    else k += s - 2;
+    double sgn[3];
+    map_zone(pb, D, i, j, k, sgn);
+    zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q[s]);
+    for (int d = 0; d < 3; d++) q[s][2 + d] *= sgn[d];
+  }
+  const int fd = D == 0 ? fi : (D == 1 ? fj : fk);
+  const int ri = D == 0 ? fi : fi, rj = D == 1 ? fj : fj;
+  const double *rc = rec3(pb, ri, rj);
+  FaceGeo<double> fg;
+  if (D == 0) {
+    fg.area = rc[MDS_ZS_AREA0];
+    fg.scale = rc[MDS_ZS_SCALEF0];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF0 + d];
+  } else if (D == 1) {
+    fg.area = rc[MDS_ZS_AREA1];
+    fg.scale = rc[MDS_ZS_SCALEF1];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF1 + d];
+  } else {
+    fg.area = rc[MDS_ZS_AREA2];
+    fg.scale = rc[MDS_ZS_SCALE0 + 2];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
+  }
+  // transverse faces: HLLE next to a tagged zone (ghosts count as tagged)
+  bool hlle = false;
+  if (D > 0) {
+    auto tagged = [&](int i, int j, int k) {
+      if (i < 0 || j < 0 || k < 0 || i >= pb.n[0] || j >= pb.n[1] || k >= pb.n[2]) return true;
+      return a.flags[zid3(pb, i, j, k)] == 0;
+    };
+    hlle = tagged(fi, fj, fk) || tagged(fi - (D == 0), fj - (D == 1), fk - (D == 2));
+  }
+  const bool polar = (D == 2) && (fj == pb.n[1] - 1);
+  constexpr int NQ = 5 + 2 * TAB + (TAB ? 1 : 1);   // rho u v1 v2 v3 [Ye] P G
+  int isshock = 0;
+  double sig = 0.0, F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (sizeof(R) == 8) {
+    // FP64: reference operation order
+    const double *mom = pb.mom[D];
+    const int b0 = fd - 2 + MDS_NG;
+    ReconGeo64 gl, gr;     // zones fd-1 (left of the face) and fd (right)
+    gl.am = mom[3 * b0]; gl.bm = mom[3 * b0 + 1]; gl.gm = mom[3 * b0 + 2];
+    gl.a0 = mom[3 * (b0 + 1)]; gl.b0 = mom[3 * (b0 + 1) + 1]; gl.g0 = mom[3 * (b0 + 1) + 2];
+    gl.ap = mom[3 * (b0 + 2)]; gl.bp = mom[3 * (b0 + 2) + 1]; gl.gp = mom[3 * (b0 + 2) + 2];
+    gr.am = gl.a0; gr.bm = gl.b0; gr.gm = gl.g0;
+    gr.a0 = gl.ap; gr.b0 = gl.bp; gr.g0 = gl.gp;
+    gr.ap = mom[3 * (b0 + 3)]; gr.bp = mom[3 * (b0 + 3) + 1]; gr.gp = mom[3 * (b0 + 3) + 2];
+    gl.xlo = pb.startx[D] + (fd - 1) * pb.dx[D];
+    gl.xhi = pb.startx[D] + fd * pb.dx[D];
+    gr.xlo = pb.startx[D] + fd * pb.dx[D];
+    gr.xhi = pb.startx[D] + (fd + 1) * pb.dx[D];
+    recon_geo_x64(gl);
+    recon_geo_x64(gr);
+    const int vars[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    double Lq[8], Rq[8];
+    for (int t = 0; t < 8; t++) {
+      const int v = vars[t];
+      if (!TAB && (v == 5 || v == 7)) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
+      if (polar) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
+      double al, ar, al2, ar2;
+      para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
+      para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+      Lq[v] = ar;
+      Rq[v] = al2;
+    }
+    if (polar) {
+      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
+      polar_rotate(pb, rc, fk, vl, vr);
+      for (int d = 0; d < 3; d++) { Lq[2 + d] = vl[d]; Rq[2 + d] = vr[d]; }
+    }
+    PState<double, 3> L, Rt;
+    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
+    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
+    Rt.G = Rq[7];
+    if (D == 0 || !hlle) isshock = hllc_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
+    else hlle_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
+  } else {
+    // FP32-bulk: reconstruction and fluxes per unit area in FP32
+    const float *rcf = pb.rc[D];
+    const int b0 = fd - 1 + MDS_NG;
+    ReconGeo32 gl, gr;
+    const float *cl = rcf + (size_t)b0 * MDS_RC_COUNT, *cr = rcf + (size_t)(b0 + 1) * MDS_RC_COUNT;
+    gl.clm = cl[MDS_RC_CLM]; gl.clp = cl[MDS_RC_CLP]; gl.crm = cl[MDS_RC_CRM]; gl.crp = cl[MDS_RC_CRP];
+    gl.mu1 = cl[MDS_RC_MU1]; gl.kinv = cl[MDS_RC_KINV]; gl.inur = cl[MDS_RC_INUR]; gl.imu2 = cl[MDS_RC_IMU2];
+    gr.clm = cr[MDS_RC_CLM]; gr.clp = cr[MDS_RC_CLP]; gr.crm = cr[MDS_RC_CRM]; gr.crp = cr[MDS_RC_CRP];
+    gr.mu1 = cr[MDS_RC_MU1]; gr.kinv = cr[MDS_RC_KINV]; gr.inur = cr[MDS_RC_INUR]; gr.imu2 = cr[MDS_RC_IMU2];
+    float Lq[8], Rq[8];
+    for (int v = 0; v < 8; v++) {
+      if ((!TAB && (v == 5 || v == 7)) || polar) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; continue; }
+      float al, ar, al2, ar2;
+      para_recon_f32((float)q[0][v], (float)q[1][v], (float)q[2][v], gl, al, ar);
+      para_recon_f32((float)q[1][v], (float)q[2][v], (float)q[3][v], gr, al2, ar2);
+      Lq[v] = ar;
+      Rq[v] = al2;
+    }
+    if (polar) {
+      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
+      polar_rotate(pb, rc, fk, vl, vr);
+      for (int d = 0; d < 3; d++) { Lq[2 + d] = (float)vl[d]; Rq[2 + d] = (float)vr[d]; }
+    }
+    PState<float, 3> L, Rt;
+    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
+    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
+    Rt.G = Rq[7];
+    FaceGeo<float> g32;
+    g32.area = 1.0f;
+    g32.scale = (float)fg.scale;
+    for (int d = 0; d < 3; d++) g32.gcov[d] = (float)fg.gcov[d];
+    float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, s32 = 0.0f;
+    if (fg.area >= 1.0e-16) {
+      if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
+      else hlle_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
+      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
+      sig = s32;
+    } else {
+      sig = 1.0e-30;
+    }
+  }
+  (void)NQ;
+  if (live && active) {
+    for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
+    if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
+    if (D == 0 && isshock) {
+      for (int s = 0; s < 3; s++) {
+        const int ii = fi - s;
+        if (ii >= 0 && ii < pb.n[0] && fj < pb.n[1] && fk < pb.n[2]) a.flags[zid3(pb, ii, fj, fk)] = 0;
+      }
+    }
+  }
+  // CFL: min dx/|s| over faces = dx / max |s| (one division per wave)
+  double smax = (live && active) ? fabs(sig) : 0.0;
+  const double wmax = wave_max(smax);
+  const unsigned long long nsh = __popcll(__ballot(live && active && D == 0 && isshock));
+  const unsigned long long nhl = __popcll(__ballot(live && active && D > 0 && hlle));
+  if ((threadIdx.x & 63) == 0) {
+    if (wmax > 0.0) atomicMin(&ck->mindt, dt_key(pb.dx[D] / wmax));
+    if (nsh) atomicAdd(&a.counters[0], nsh);
+    if (nhl) atomicAdd(&a.counters[1], nhl);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zone update: sources, flux divergence, RK combination, u -> p, dt limits
+// ---------------------------------------------------------------------------
+template <class R, int TAB, int MONO>
+__global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp, StateArrays<R> sph, int stage)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = z < (long long)n1 * n2 * n3;
+  const long long zl = live ? z : 0;
+  const int i = (int)(zl % n1), j = (int)((zl / n1) % n2), k = (int)(zl / ((long long)n1 * n2));
+  DevClock *ck = a.clock;
+  const bool active = ck->active;
+  const double dt = ck->dt;
+  const double gam = pb.gam;
+  const StateArrays<R> &in = stage == 0 ? sp : sph;
+  double q[8], qn[8];
+  zone_state<R>(a, in, zl, TAB, gam, q);
+  if (stage == 1) zone_state<R>(a, sp, zl, TAB, gam, qn);
+  else for (int t = 0; t < 8; t++) qn[t] = q[t];
+  const double *rc = rec3(pb, i, j);
+  double gcov[3], gcon[3];
+  for (int d = 0; d < 3; d++) { gcov[d] = rc[MDS_ZS_GCOV0 + d]; gcon[d] = rc[MDS_ZS_GCON + d]; }
+  // geometric sources (rad_fluid.c:195-214): the 11 non-zero conn[l][m][n]
+  // in the reference's (l, n) order per m
+  double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double press = q[6];
+  {
+    double vc[3], T[3][3];
+    for (int d = 0; d < 3; d++) vc[d] = gcov[d] * q[2 + d];
+    for (int l = 0; l < 3; l++) {
+      for (int m = 0; m < 3; m++) T[l][m] = (q[0] * q[2 + m]) * vc[l];
+      T[l][l] += press;
+    }
+    src[2] += T[0][0] * rc[MDS_ZS_C000];
+    src[2] += T[1][1] * rc[MDS_ZS_C101];
+    src[2] += T[2][2] * rc[MDS_ZS_C202];
+    src[3] += T[0][1] * rc[MDS_ZS_C011];
+    src[3] += T[1][0] * rc[MDS_ZS_C110];
+    src[3] += T[1][1] * rc[MDS_ZS_C111];
+    src[3] += T[2][2] * rc[MDS_ZS_C212];
+    src[4] += T[0][2] * rc[MDS_ZS_C022];
+    src[4] += T[1][2] * rc[MDS_ZS_C122];
+    src[4] += T[2][0] * rc[MDS_ZS_C220];
+    src[4] += T[2][1] * rc[MDS_ZS_C221];
+  }
+  // fluxes through the six faces
+  const long long f0 = fid3(pb, 0, i, j, k), f0p = fid3(pb, 0, i + 1, j, k);
+  const long long f1 = fid3(pb, 1, i, j, k), f1p = fid3(pb, 1, i, j + 1, k);
+  const long long f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
+  if (MONO) {
+    // gravity.c:475-486
+    const double gz = a.grav[i];
+    const double xl = pb.startx[0] + i * pb.dx[0];
+    double vr = a.F[0][f0p];
+    vr /= (rec3(pb, i + 1, j)[MDS_ZS_AREA0] + 1.0e-16);
+    src[2] += q[0] * gz;
+    const double vl = a.F[0][f0] / (rc[MDS_ZS_AREA0] + 1.0e-16);
+    const double A = (vr - vl) / pb.dx[0];
+    const double B = vl - A * xl;
+    const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
+    src[1] += ((A * be) / ga + B) * gz;
+  }
+  double U[6], Uh[6];
+  p_to_u_x64<3, false>(qn[0], qn[1], &qn[2], gcov, U);
+  p_to_u_x64<3, false>(q[0], q[1], &q[2], gcov, Uh);
+  U[MDS_YI] = TAB ? U[0] * qn[5] : 0.0;
+  Uh[MDS_YI] = TAB ? Uh[0] * q[5] : 0.0;
+  const double lim_src = dt_source_control_x64<3>(qn[1], &qn[2], U, src);
+  const double vol = rc[MDS_ZS_VOLUME];
+  const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
+  const int nslot = TAB ? 6 : 5;
+  for (int v = 0; v < nslot; v++) {
+    const size_t o0 = (size_t)v * a.nf[0], o1 = (size_t)v * a.nf[1], o2 = (size_t)v * a.nf[2];
+    double divF = 0.0;
+    divF += a.F[0][o0 + f0p];
+    divF -= a.F[0][o0 + f0];
+    divF += a.F[1][o1 + f1p];
+    divF -= a.F[1][o1 + f1];
+    divF += a.F[2][o2 + f2p] - a.F[2][o2 + f2];
+    const double ddt = (divF / vol) - src[v];
+    U[v] = ((pf * U[v]) + (phf * Uh[v])) - ((ff * dt) * ddt);
+  }
+  double ro, uo, vo[3];
+  u_to_p_x64<3, false>(U, gcon, ro, uo, vo);
+  double locmax = 0.0;
+  if (stage == 1) {
+    // estimate_dt zone bound (step.c:177-187) with the stage's sound speed
+    double cs = TAB ? (double)((const R *)a.CS)[zl] : sqrt((gam * press) / q[0]);
+    double loc = 0.0;
+    for (int d = 0; d < 3; d++) loc += (fabs(vo[d]) + (cs / rc[MDS_ZS_SCALE0 + d])) / pb.dx[d];
+    locmax = loc;
+  }
+  if (live && active) {
+    StateArrays<R> &out = stage == 0 ? sph : sp;
+    out.v[0][z] = (R)ro;
+    out.v[1][z] = (R)uo;
+    for (int d = 0; d < 3; d++) out.v[2 + d][z] = (R)vo[d];
+    if (TAB) out.v[5][z] = (R)(U[MDS_YI] / U[0]);
+  }
+  const double wl = wave_min((live && active) ? lim_src : 1.0e300);
+  const double wm = wave_max((live && active) ? locmax : 0.0);
+  if ((threadIdx.x & 63) == 0) {
+    double l = wl;
+    if (wm > 0.0) l = fmin(l, 1.0 / wm);
+    if (l < 1.0e300) atomicMin(&ck->mindt, dt_key(l));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launcher: one RK stage
+// ---------------------------------------------------------------------------
+template <class R>
+static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *clock, unsigned long long *counters,
+                             const DevEos &eos, const StateArrays<R> &sp, const StateArrays<R> &sph, int stage,
+                             hipStream_t st)
+{
+  Sph3Args a;
+  a.pb = pb;
+  a.clock = clock;
+  a.counters = counters;
+  a.flags = L.flags;
+  a.eos = eos;
+  for (int d = 0; d < 3; d++) { a.F[d] = L.F[d]; a.nf[d] = L.nf[d]; }
+  a.P = L.P; a.G = L.G; a.CS = L.CS;
+  a.grav = L.grav;
+  const StateArrays<R> &in = stage == 0 ? sp : sph;
+  const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
+  const dim3 blk(256);
+  if (stage == 0)
+    hipLaunchKernelGGL(sph3_reset_flags, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, L.flags, nz);
+  if (L.tab) hipLaunchKernelGGL(sph3_eos_kernel<R>, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, a, in);
+  if (L.mono) {
+    hipLaunchKernelGGL(sph3_shell_mass<R>, dim3(pb.n[0]), blk, 0, st, a, in, L.ms);
+    hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(64), 0, st, a, L.ms, L.grav);
+  }
+#define MDS_FLUX(D)                                                                              \
+  if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1>), dim3((unsigned)((L.nf[D] + 255) / 256)), blk, 0, st, a, in); \
+  else hipLaunchKernelGGL((sph3_flux<R, D, 0>), dim3((unsigned)((L.nf[D] + 255) / 256)), blk, 0, st, a, in);
+  MDS_FLUX(0)
+  MDS_FLUX(1)
+  MDS_FLUX(2)
+#undef MDS_FLUX
+  const dim3 grd((unsigned)((nz + 255) / 256));
+  if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1>), grd, blk, 0, st, a, sp, sph, stage);
+  else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0>), grd, blk, 0, st, a, sp, sph, stage);
+  else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1>), grd, blk, 0, st, a, sp, sph, stage);
+  else hipLaunchKernelGGL((sph3_update<R, 0, 0>), grd, blk, 0, st, a, sp, sph, stage);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t mds_launch_sph3d(const Sph3Launch *L, const DevProb *pb, DevClock *clock,
+                                       unsigned long long *counters, const DevEos *eos, const void *sp,
+                                       const void *sph, int stage, hipStream_t st)
+{
+  if (L->fp32)
+    return stage_impl<float>(*L, *pb, clock, counters, *eos, *(const StateArrays<float> *)sp,
+                             *(const StateArrays<float> *)sph, stage, st);
+  return stage_impl<double>(*L, *pb, clock, counters, *eos, *(const StateArrays<double> *)sp,
+                            *(const StateArrays<double> *)sph, stage, st);
+}

@@ -211,22 +211,26 @@ int mds_init_polytrope(const mds_grid *g, double n_index, double rho_c, double K
   if (!g || !prims || g->geom != MDS_GEOM_SPHERICAL || !(rho_c > 0.0) || !(K > 0.0)) return MDS_EINVAL;
   long double *th = (long double *)malloc(sizeof(long double) * LE_MAX);
   long double *dth = (long double *)malloc(sizeof(long double) * LE_MAX);
-  if (!th || !dth) { free(th); free(dth); return MDS_ENOMEM; }
+  const int n1 = g->n[0];
+  double *rho0 = (double *)malloc(sizeof(double) * n1), *e0 = (double *)malloc(sizeof(double) * n1);
+  double *ye0 = (double *)malloc(sizeof(double) * n1);
+  if (!th || !dth || !rho0 || !e0 || !ye0) {
+    free(th); free(dth); free(rho0); free(e0); free(ye0);
+    return MDS_ENOMEM;
+  }
   const int kz = lane_emden(n_index, th, dth, LE_MAX);
   const double GN = 6.67384e-8;
   const double alpha = sqrt((n_index + 1.0) * K * pow(rho_c, 1.0 / n_index - 1.0) / (4.0 * M_PI * GN));
   const double xi1 = (double)((kz - 1) * LE_H + LE_H * th[kz - 1] / (th[kz - 1] - th[kz]));
   double prm[3] = {0.0, 0.0, 0.0};
   if (tab) mds_eos_model_params(eos_seed, prm);
-  const size_t nz = nzones(g);
-  const int n1 = g->n[0];
-  const int nv = 6;
-  for (size_t z = 0; z < nz; z++) {
-    const int i = (int)(z % (size_t)n1);
+  /* the unperturbed star on the radial zones */
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < n1; i++) {
     const double xc = g->beta[0][i + MDS_NG] / g->Gamma[0][i + MDS_NG];
     const double r = g->rcoord == MDS_RCOORD_LOG ? exp(xc) : fabs(xc);
     const double xi = r / alpha;
-    double rho, P;
+    double rho;
     if (xi < xi1) {
       const double s = xi / (double)LE_H;
       int k = (int)s;
@@ -243,32 +247,42 @@ int mds_init_polytrope(const mds_grid *g, double n_index, double rho_c, double K
     } else {
       rho = rho_atm;
     }
-    P = K * pow(rho, 1.0 + 1.0 / n_index);
-    if (sigma > 0.0) {
-      double u1 = 1.0 - mds_uniform(seed, 2 * (unsigned long long)z);
-      double u2 = mds_uniform(seed, 2 * (unsigned long long)z + 1);
-      double zz = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
-      rho *= exp(sigma * zz);
-    }
+    const double P = K * pow(rho, 1.0 + 1.0 / n_index);
     const double ye = 0.45 + 0.05 * (1.0 - cbrt(fmin(rho / rho_c, 1.0)));
-    double u;
+    double e;
     if (tab) {
       const double T = temp_for_pressure(prm, tab, rho, ye, P);
       double v[3];
       mds_eos_model_eval(prm, rho, T, ye, v);
-      u = rho * v[1];
+      e = v[1];
     } else {
-      u = P / (gam - 1.0);
+      e = P / ((gam - 1.0) * rho);
     }
+    rho0[i] = rho;
+    e0[i] = e;
+    ye0[i] = ye;
+  }
+  /* zones: radial profile times the lognormal density factor at constant
+   * specific internal energy and Ye */
+  const size_t nz = nzones(g);
+#pragma omp parallel for schedule(static)
+  for (long long zz = 0; zz < (long long)nz; zz++) {
+    const size_t z = (size_t)zz;
+    const int i = (int)(z % (size_t)n1);
+    double fac = 1.0;
+    if (sigma > 0.0) {
+      double u1 = 1.0 - mds_uniform(seed, 2 * (unsigned long long)z);
+      double u2 = mds_uniform(seed, 2 * (unsigned long long)z + 1);
+      fac = exp(sigma * (sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2)));
+    }
+    const double rho = rho0[i] * fac;
     prims[0 * nz + z] = rho;
-    prims[1 * nz + z] = u;
+    prims[1 * nz + z] = rho * e0[i];
     prims[2 * nz + z] = 0.0;
     prims[3 * nz + z] = 0.0;
     prims[4 * nz + z] = 0.0;
-    prims[5 * nz + z] = ye;
+    prims[5 * nz + z] = ye0[i];
   }
-  (void)nv;
-  free(th);
-  free(dth);
+  free(th); free(dth); free(rho0); free(e0); free(ye0);
   return MDS_OK;
 }

@@ -62,3 +62,25 @@ def star1d(n=640, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, rmin=1
     g = mds.GridBuilder(1, [int(n)], [x0], [(x1 - x0) / int(n)], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
     prob = dict(grid=g, bc_lo=[mds.BC_REFLECT], bc_hi=[mds.BC_OUTFLOW], gam=gam, monopole=True, eos=table)
     return g, prob, mds.init_polytrope(g, gam=gam, table=table, sigma=sigma, seed=seed, **kw)
+
+
+def sph3d_grid(n1, n2, n3, rmin=1.0e5, rmax=2.0e9):
+    """3D spherical grid: log r over [rmin, rmax], theta over x1 in [-1, 1]
+    (POLYTH map), phi over [0, 2 pi)."""
+    x0, x1 = math.log(rmin), math.log(rmax)
+    return mds.GridBuilder(3, [int(n1), int(n2), int(n3)], [x0, -1.0, 0.0],
+                           [(x1 - x0) / int(n1), 2.0 / int(n2), 2.0 * math.pi / int(n3)],
+                           geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
+
+
+SPH3D_BC_LO = [mds.BC_REFLECT, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC]
+SPH3D_BC_HI = [mds.BC_OUTFLOW, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC]
+
+
+def star3d(n1=640, n2=128, n3=256, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, monopole=True, **kw):
+    """The polytropic star (n=3, rho_c = 1e10) on a 3D spherical grid with
+    monopole gravity; reflecting inner radius, outflow outer radius, theta
+    across the poles, phi periodic.  Uniform angular mesh (not dendritic)."""
+    g = sph3d_grid(n1, n2, n3)
+    prob = dict(grid=g, bc_lo=list(SPH3D_BC_LO), bc_hi=list(SPH3D_BC_HI), gam=gam, monopole=monopole, eos=table)
+    return g, prob, mds.init_polytrope(g, gam=gam, table=table, sigma=sigma, seed=seed, **kw)

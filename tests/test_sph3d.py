@@ -119,3 +119,120 @@ def test_host_geometry_matches_oracle_bitwise():
                     assert h[getattr(mds, "ZS_" + nm)] == conn[idx], (i, j, nm)
                 nz = np.count_nonzero(conn)
                 assert nz <= 11
+
+
+# ---------------------------------------------------------------------------
+# device path against the oracle
+# ---------------------------------------------------------------------------
+def _oracle_sph3d(prob, table=None, monopole=True):
+    import sys
+    from parity_util import oracle_config
+    kw = dict(nvars=6, monopole=monopole)
+    if table is not None:
+        t = table.t
+        kw["eos_tab"] = po.eos_tab(dict(nr=t.nr, nt=t.nt, ny=t.ny, lr0=t.lr0, lr1=t.lr1, lt0=t.lt0, lt1=t.lt1,
+                                        ye0=t.ye0, ye1=t.ye1), table.data)
+    return po.Sim(oracle_config(prob, 0.5, **kw))
+
+
+def _rel(a, b, w):
+    return float(np.sum(np.abs(a - b) * w) / np.sum(np.abs(b) * w))
+
+
+def _blast(n):
+    from medusa_amd import mds, problems
+    g = problems.sph3d_grid(*n)
+    prob = dict(grid=g, bc_lo=list(problems.SPH3D_BC_LO), bc_hi=list(problems.SPH3D_BC_HI), gam=5.0 / 3.0)
+    prims = np.zeros((6, n[2], n[1], n[0]))
+    prims[0] = 1.0e8
+    prims[1] = 1.0e26
+    rng = np.random.default_rng(3)
+    prims[0] *= 1.0 + 0.01 * rng.uniform(-1, 1, prims[0].shape)
+    # over-pressured blob off the axis and off the centre, in the last theta rows too
+    prims[1][:, n[1] - 3:, n[0] // 3:n[0] // 3 + 3] *= 10.0
+    prims[1][n[2] // 4:n[2] // 4 + 2, 2:4, n[0] // 2:n[0] // 2 + 2] *= 20.0
+    prims[5] = 0.5
+    return g, prob, prims
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sph3d_blast_matches_oracle(precision):
+    from medusa_amd import mds
+    n = (24, 12, 8)
+    g, prob, prims = _blast(n)
+    print(f"[sph3d] blast {n} precision={precision}, 12 steps", flush=True)
+    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    ctx.run(12)
+    dev = ctx.download()
+    sim = _oracle_sph3d(prob, monopole=False)
+    sim.set_prims(prims)
+    sim.run(max_steps=12)
+    ref = sim.get_prims()
+    vol = np.broadcast_to(g.zone()[2:-2, 2:-2, mds.ZS_VOLUME][None], ref[0].shape)
+    errs = {"rho": _rel(dev[0], ref[0], vol), "u": _rel(dev[1], ref[1], vol)}
+    cs = np.sqrt(5.0 / 3.0 * (2.0 / 3.0) * ref[1] / ref[0])
+    sc = g.zone()[2:-2, 2:-2, mds.ZS_SCALE0:mds.ZS_SCALE0 + 3]
+    for d in range(3):
+        errs[f"v{d + 1}/cs"] = float(np.sum(np.abs(dev[2 + d] - ref[2 + d]) * sc[None, :, :, d] * vol)
+                                     / np.sum(cs * vol))
+    t, dt, ist = ctx.clock()
+    hl = ctx.counters()
+    print(f"[sph3d]   L1 {errs}; t {t!r} vs {sim.t!r}; shocks/HLLE {hl} vs {sim.shock_faces()}/{sim.hlle_faces()}",
+          flush=True)
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+    if precision == 0:
+        assert ist == sim.istep and abs(t / sim.t - 1) < 1e-13
+        assert hl[0] == sim.shock_faces() and hl[1] == sim.hlle_faces()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_table", [False, True])
+def test_sph3d_star_fp64_matches_oracle(use_table):
+    from medusa_amd import mds, problems
+    T = mds.EosTable() if use_table else None
+    n = (32, 8, 8)
+    g, prob, prims = problems.star3d(*n, table=T)
+    print(f"[sph3d] star {n} table={use_table} FP64, 10 steps", flush=True)
+    ctx = mds.Context([prob], precision=mds.FP64, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    ctx.run(10, graph=True)
+    dev = ctx.download()
+    sim = _oracle_sph3d(prob, T)
+    sim.set_prims(prims)
+    sim.run(max_steps=10)
+    ref = sim.get_prims()
+    vol = np.broadcast_to(g.zone()[2:-2, 2:-2, mds.ZS_VOLUME][None], ref[0].shape)
+    errs = {"rho": _rel(dev[0], ref[0], vol), "u": _rel(dev[1], ref[1], vol)}
+    if use_table:
+        errs["ye"] = _rel(dev[5], ref[5], vol)
+    print(f"[sph3d]   L1 {errs}", flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-12, (k, v)
+
+
+@pytest.mark.gpu
+def test_sph3d_uniform_gas_stays_at_rest_on_device():
+    from medusa_amd import mds, problems
+    n = (16, 8, 8)
+    g = problems.sph3d_grid(*n)
+    prob = dict(grid=g, bc_lo=list(problems.SPH3D_BC_LO), bc_hi=list(problems.SPH3D_BC_HI), gam=5.0 / 3.0)
+    prims = np.zeros((5, n[2], n[1], n[0]))
+    prims[0] = 1.0e8
+    prims[1] = 1.0e26
+    for precision in (0, 1):
+        ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+        ctx.upload(prims)
+        ctx.update_eos()
+        ctx.run(10)
+        out = ctx.download()
+        cs = math.sqrt(5.0 / 3.0 * (2.0 / 3.0) * 1.0e26 / 1.0e8)
+        sc = g.zone()[2:-2, 2:-2, mds.ZS_SCALE0:mds.ZS_SCALE0 + 3]
+        vmax = max(float(np.max(np.abs(out[2 + d]) * sc[None, :, :, d])) for d in range(3))
+        print(f"[sph3d] uniform gas precision={precision}: max |v|/cs {vmax / cs:.2e}", flush=True)
+        assert vmax / cs < (1e-12 if precision == 0 else 1e-5)
