@@ -37,6 +37,20 @@ struct PState {          // reconstructed primitive state on one side of a face
 // slot of the passive scalar in conserved/flux records (decs.h:646: YE after U3)
 #define MDS_YI 5
 
+// field-wise select of one of two states
+template <class R, int NV>
+__device__ __forceinline__ PState<R, NV> pick_state(bool first, const PState<R, NV> &a, const PState<R, NV> &b)
+{
+  PState<R, NV> k;
+  k.rho = first ? a.rho : b.rho;
+  k.uu = first ? a.uu : b.uu;
+  for (int d = 0; d < 3; d++) k.v[d] = first ? a.v[d] : b.v[d];
+  k.P = first ? a.P : b.P;
+  k.G = first ? a.G : b.G;
+  k.ye = first ? a.ye : b.ye;
+  return k;
+}
+
 template <class R, int NV>
 struct Flux {
   R f[5];                // rho, E, m1..m3 (area weighted)
@@ -215,7 +229,9 @@ MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   if (sr < 0.0) { pflux_x64<NV, CART, NS>(Rt, g, dir, flux); return isshock; }
   double u[6], fx[6], Us[6], vs[3];
   double slr;
-  const PState<double, NV> &K = (ss > 0.0) ? L : Rt;
+  // the side upwind of the contact, selected by value (a reference to one of
+  // two locals would force both into scratch memory)
+  const PState<double, NV> K = pick_state(ss > 0.0, L, Rt);
   if (ss > 0.0) {
     p_uflux_x64<NV, CART, NS>(L, g, dir, u, fx);
     Us[0] = (L.rho * (sl - u_l)) / (sl - ss);
@@ -450,7 +466,7 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   if (sr < 0.0f) { pflux_f32<NV, CART, NS>(Rt, g, dir, flux); return isshock; }
   float u[6], fx[6], Us[6];
   bool left = ss > 0.0f;
-  const PState<float, NV> &K = left ? L : Rt;
+  const PState<float, NV> K = pick_state(left, L, Rt);
   float sk = left ? sl : sr, uk = left ? u_l : u_r;
   p_uflux_f32<NV, CART>(K, g, dir, u, fx);
   float iden = frcp(sk - ss);

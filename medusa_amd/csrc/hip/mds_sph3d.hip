@@ -279,17 +279,20 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
     gr.xhi = pb.startx[D] + (fd + 1) * pb.dx[D];
     recon_geo_x64(gl);
     recon_geo_x64(gr);
-    const int vars[8] = {0, 1, 2, 3, 4, 5, 6, 7};
     double Lq[8], Rq[8];
-    for (int t = 0; t < 8; t++) {
-      const int v = vars[t];
-      if (!TAB && (v == 5 || v == 7)) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
-      if (polar) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
-      double al, ar, al2, ar2;
-      para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
-      para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
-      Lq[v] = ar;
-      Rq[v] = al2;
+    if (polar) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        if (!TAB && (v == 5 || v == 7)) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
+        double al, ar, al2, ar2;
+        para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
+        para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+        Lq[v] = ar;
+        Rq[v] = al2;
+      }
     }
     if (polar) {
       double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
@@ -313,13 +316,19 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
     gr.clm = cr[MDS_RC_CLM]; gr.clp = cr[MDS_RC_CLP]; gr.crm = cr[MDS_RC_CRM]; gr.crp = cr[MDS_RC_CRP];
     gr.mu1 = cr[MDS_RC_MU1]; gr.kinv = cr[MDS_RC_KINV]; gr.inur = cr[MDS_RC_INUR]; gr.imu2 = cr[MDS_RC_IMU2];
     float Lq[8], Rq[8];
-    for (int v = 0; v < 8; v++) {
-      if ((!TAB && (v == 5 || v == 7)) || polar) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; continue; }
-      float al, ar, al2, ar2;
-      para_recon_f32((float)q[0][v], (float)q[1][v], (float)q[2][v], gl, al, ar);
-      para_recon_f32((float)q[1][v], (float)q[2][v], (float)q[3][v], gr, al2, ar2);
-      Lq[v] = ar;
-      Rq[v] = al2;
+    if (polar) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        if (!TAB && (v == 5 || v == 7)) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; continue; }
+        float al, ar, al2, ar2;
+        para_recon_f32((float)q[0][v], (float)q[1][v], (float)q[2][v], gl, al, ar);
+        para_recon_f32((float)q[1][v], (float)q[2][v], (float)q[3][v], gr, al2, ar2);
+        Lq[v] = ar;
+        Rq[v] = al2;
+      }
     }
     if (polar) {
       double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
