@@ -49,6 +49,7 @@ extern "C" {
 /* radial map r(x) for spherical grids */
 #define ORC_RCOORD_UNIFORM 0   /* r = |x|       (metric_spherical_general.c:149-152) */
 #define ORC_RCOORD_LOG     1   /* r = exp(x)    (metrics/metric_spherical_logr.c:3-38) */
+#define ORC_RCOORD_SINH    2   /* r = rtrans sinh(|x|/rtrans) (metric_spherical_general.c:155-161) */
 
 /* Geometry of one zone, restating zone_geom (decs.h:144-153) for NDIM<=3.
  * Index 0 of gcov/scale is the zone centre, 1+d the lower face in direction d. */
@@ -78,6 +79,8 @@ typedef struct {
   int use_fixed_grav;
   int use_monopole;    /* Newtonian monopole gravity, gravity.c:145-251, 432-498 */
   const struct orc_eos_tab *eos_tab;   /* tabulated EOS (NULL: gamma law); nvars 6 carries Ye */
+  double rtrans;       /* ORC_RCOORD_SINH scale */
+  int dendritic;       /* 3D spherical: dendritic angular coarsening (utils.c:181-590) */
 } orc_config;
 
 typedef struct orc_sim orc_sim;
@@ -145,6 +148,10 @@ long orc_hlle_face_count(const orc_sim *s);     /* transverse faces solved with 
 void orc_conserved_totals(const orc_sim *s, double out[2]);
 /* monopole gravity at the n1+1 radial faces from the current state (gravity.c:420-425) */
 void orc_monopole_edges(const orc_sim *s, double *grav);
+/* dendritic levels: nj[n1], nk[n1*n2] (0 beyond nj[i]); returns maxjlev*256 + maxklev */
+int orc_dendritic_levels(const orc_sim *s, int *nj, int *nk);
+/* theta (dir 0) / phi (dir 1) moments of refinement level lev (length n + 2 NG), NULL if none */
+const double *orc_level_moment(const orc_sim *s, int which, int dir, int lev);
 
 #ifdef __cplusplus
 }

@@ -62,7 +62,30 @@ struct orc_sim {
   double t, dt, dtold, min_dt;
   int istep, firstc;
   long nshock, nhlle;
+  /* dendritic levels (utils.c:181-590); NULL on uniform grids (every level 1) */
+  int *dj;             /* [i + NG], i in [-NG, n1 + NG) */
+  int *dk;             /* [(i + NG) * (n2 + 2 NG) + j + NG], j in [-NG, n2/dj + NG) */
+  int maxjlev, maxklev;
+  /* theta (0) and phi (1) moments per level l at l * (n + 2 NG), index 0 = zone -NG
+   * (alpha1s/beta1s/Gamma1s, alpha2s/..., metric_spherical_general.c:592-669) */
+  double *lmom[3][2];
+  unsigned char *live; /* zones the dendritic sweeps read: interior and ghosts */
 };
+
+/* DJS/DKS and the level-local zone counts (decs.h:569-590) */
+static int djs(const orc_sim *s, int i) { return s->dj ? s->dj[i + ORC_NG] : 1; }
+static int dks(const orc_sim *s, int i, int j)
+{
+  return s->dk ? s->dk[(long)(i + ORC_NG) * (s->c.n[1] + 2 * ORC_NG) + j + ORC_NG] : 1;
+}
+static int njs(const orc_sim *s, int i) { return s->c.n[1] / djs(s, i); }
+static int nks(const orc_sim *s, int i, int j) { return s->c.n[2] / dks(s, i, j); }
+static int lev_of(int f)
+{
+  int l = 0;
+  while (f >>= 1) l++;
+  return l;
+}
 
 static long zid(const orc_sim *s, int i, int j, int k)
 {
@@ -107,11 +130,27 @@ double orc_romb(double x0, double x1, double (*f)(double, const void *), const v
 /* spherical maps: r(x) (uniform: metric_spherical_general.c:149-152; log:
  * metric_spherical_logr.c) and the POLYTH theta map with the io.c defaults
  * poly_xt=1.2, poly_alpha=4 (io.c:5014-5015, metric_spherical_general.c:222-243). */
-typedef struct { int rcoord; double poly_norm, poly_xt, poly_alpha; } sphmap;
+typedef struct { int rcoord; double poly_norm, poly_xt, poly_alpha, rtrans; } sphmap;
 
-static double r_of_x(const sphmap *m, double x) { return m->rcoord == ORC_RCOORD_LOG ? exp(x) : fabs(x); }
-static double dr_dx(const sphmap *m, double x) { return m->rcoord == ORC_RCOORD_LOG ? exp(x) : 1.0; }
-static double d2r_dx2(const sphmap *m, double x) { return m->rcoord == ORC_RCOORD_LOG ? exp(x) : 0.0; }
+/* sinh map: metric_spherical_general.c:159-161, 186-188, 206-208 */
+static double r_of_x(const sphmap *m, double x)
+{
+  if (m->rcoord == ORC_RCOORD_LOG) return exp(x);
+  if (m->rcoord == ORC_RCOORD_SINH) return m->rtrans * sinh(fabs(x) / m->rtrans);
+  return fabs(x);
+}
+static double dr_dx(const sphmap *m, double x)
+{
+  if (m->rcoord == ORC_RCOORD_LOG) return exp(x);
+  if (m->rcoord == ORC_RCOORD_SINH) return cosh(fabs(x) / m->rtrans);
+  return 1.0;
+}
+static double d2r_dx2(const sphmap *m, double x)
+{
+  if (m->rcoord == ORC_RCOORD_LOG) return exp(x);
+  if (m->rcoord == ORC_RCOORD_SINH) return sinh(fabs(x) / m->rtrans) / m->rtrans;
+  return 0.0;
+}
 static double th_of_x(const sphmap *m, double x)
 {
   if (x < -1.0) x = -2.0 - x;
@@ -146,9 +185,10 @@ static double f_cthdth_dx2(double x, const void *c) { const sphmap *m = c; retur
 static double f_xth1g(double x, const void *c) { const sphmap *m = c; return (x * sin(th_of_x(m, x))) * dth_dx(m, x); }
 static double f_xth2g(double x, const void *c) { const sphmap *m = c; return ((x * x) * sin(th_of_x(m, x))) * dth_dx(m, x); }
 
-static void sphmap_init(sphmap *m, int rcoord)
+static void sphmap_init(sphmap *m, int rcoord, double rtrans)
 {
   m->rcoord = rcoord;
+  m->rtrans = rtrans;
   m->poly_xt = 1.2;
   m->poly_alpha = 4;
   m->poly_norm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m->poly_alpha + 1.0)) * (1.0 / pow(m->poly_xt, m->poly_alpha))));
@@ -189,66 +229,174 @@ static void build_geometry_cartesian(orc_sim *s)
   }
 }
 
-/* 2D/3D spherical (r, theta, phi) geometry on a uniform (non-dendritic) grid,
- * metric_spherical_general.c:592-1051 with DJS = DKS = 1: x1 in [-1,1] through
- * the POLYTH theta map, phi = x2 (dx[2] = 2 pi in 2D, init_coords :558-590).
- * Volume, metric and scale for every zone (ghosts included; theta ghosts
- * reflect through th_of_x), areas on the faces the reference loops over
- * (i <= n1, j <= n2) with the axis faces forced to zero (:808-833), and the
- * connection coefficients of interior zones with the pressure-balanced
- * Gamma^theta_theta_theta = dA/dV - Gamma^phi_theta_phi (:905-915). */
-static void build_geometry_spherical(orc_sim *s)
+/* Dendritic levels: compute_dj (utils.c:181-233), compute_dk (:240-368) and
+ * the 2:1 checks of init_dendritic (:540-579).  Returns 0, or -1 when the
+ * reference's check finds a 4:1 jump (it exits there) or when its fix-up of
+ * a 1/2:1 jump would change a level (the fix-up leaves the ghost rows
+ * unchanged, so such grids are rejected here). */
+static int build_levels(orc_sim *s, const sphmap *m)
 {
   const orc_config *c = &s->c;
+  const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2], NG = ORC_NG;
+  const int S1 = n2 + 2 * NG;
+  const double MAXA = 0.5;                      /* MAX_LOG2_ASPECT, utils.c:12 */
+  int *nj = calloc(n1 + 2 * NG, sizeof(int));
+  int *nk = calloc((size_t)(n1 + 2 * NG) * S1, sizeof(int));
+  s->dj = calloc(n1 + 2 * NG, sizeof(int));
+  s->dk = calloc((size_t)(n1 + 2 * NG) * S1, sizeof(int));
+#define NJ(i) nj[(i) + NG]
+#define NK(i, j) nk[(long)((i) + NG) * S1 + (j) + NG]
+#define DK(i, j) s->dk[(long)((i) + NG) * S1 + (j) + NG]
+  /* compute_dj */
+  NJ(0) = 2;
+  const double dth = th_of_x(m, c->startx[1] + n2 * c->dx[1]) - th_of_x(m, c->startx[1]);
+  for (int i = 1; i < n1; i++) {
+    double r0 = r_of_x(m, c->startx[0] + (i + 0.0) * c->dx[0]);
+    double rc = r_of_x(m, c->startx[0] + (i + 0.5) * c->dx[0]);
+    double r1 = r_of_x(m, c->startx[0] + (i + 1.0) * c->dx[0]);
+    double ideal = rc * dth / (r1 - r0);
+    NJ(i) = NJ(i - 1);
+    if (log2(ideal / NJ(i)) > MAXA) NJ(i) *= 2;
+    NJ(i) = OMIN(OMAX(NJ(i), NJ(i - 1)), n2);
+  }
+  for (int i = -NG; i < 0; i++) NJ(i) = NJ(-i - 1);
+  for (int i = n1; i < n1 + NG; i++) NJ(i) = NJ(n1 - 1);
+  for (int i = -NG; i < n1 + NG; i++) s->dj[i + NG] = n2 / NJ(i);
+  /* compute_dk */
+  const int nk_min = 4, nk_max = n3;
+  const double dph = n3 * c->dx[2];
+  for (int i = 0; i < n1; i++) {
+    double r0 = r_of_x(m, c->startx[0] + (i + 0.0) * c->dx[0]);
+    double rc = r_of_x(m, c->startx[0] + (i + 0.5) * c->dx[0]);
+    double r1 = r_of_x(m, c->startx[0] + (i + 1.0) * c->dx[0]);
+    double dr = r1 - r0;
+    NK(i, 0) = nk_min;
+    for (int j = 1; j < NJ(i); j++) {
+      double thc = th_of_x(m, c->startx[1] + (j + 0.5) * s->dj[i + NG] * c->dx[1]);
+      double ideal = rc * sin(thc) * dph / dr;
+      if (j < NJ(i) / 2) {
+        NK(i, j) = NK(i, j - 1);
+        if (log2(ideal / NK(i, j)) > MAXA) NK(i, j) *= 2;
+        if (i == 0) NK(i, j) = OMIN(NK(i, j), nk_max);
+        else if (NJ(i - 1) == NJ(i)) NK(i, j) = OMIN(OMAX(NK(i, j), NK(i - 1, j)), nk_max);
+        else NK(i, j) = OMIN(OMAX(NK(i, j), NK(i - 1, j / 2)), nk_max);
+      } else {
+        NK(i, j) = NK(i, NJ(i) - 1 - j);
+      }
+    }
+  }
+  for (int i = -NG; i < 0; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = NK(-i - 1, j);
+  for (int i = n1; i < n1 + NG; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = NK(n1 - 1, j);
+  for (int i = -NG; i < n1 + NG; i++) {
+    for (int j = -NG; j < 0; j++) NK(i, j) = NK(i, -j - 1);
+    for (int j = NJ(i); j < NJ(i) + NG; j++) NK(i, j) = NK(i, 2 * NJ(i) - 1 - j);
+    for (int j = -NG; j < NJ(i) + NG; j++) DK(i, j) = n3 / NK(i, j);
+  }
+  int bad = 0;
+  for (int i0 = 0; i0 < n1; i0++) {
+    const int i1 = i0 + 1;
+    if (s->dj[i0 + NG] / s->dj[i1 + NG] != 2) continue;
+    for (int jj = 0; jj < NJ(i0); jj++) {
+      const int j10 = jj * s->dj[i0 + NG] / s->dj[i1 + NG], j11 = j10 + 1;
+      const int a = DK(i0, jj) / DK(i1, j10), b = DK(i0, jj) / DK(i1, j11);
+      if (a == 4 || b == 4 || a == 0 || b == 0) bad = 1;
+    }
+  }
+  s->maxjlev = lev_of(s->dj[NG]);
+  s->maxklev = 0;
+  for (int i = -NG; i < n1 + NG; i++)
+    for (int j = -NG; j < NJ(i) + NG; j++) s->maxklev = OMAX(s->maxklev, lev_of(DK(i, j)));
+#undef NJ
+#undef NK
+#undef DK
+  free(nj);
+  free(nk);
+  return bad ? -1 : 0;
+}
+
+/* 2D/3D spherical geometry, metric_spherical_general.c:592-1051: moments
+ * per level (:592-669), volumes (:708-740), areas on the faces the reference
+ * loops over (i <= n1, j <= nj) with the axis faces forced to zero
+ * (:745-833), gcov/gcon/scale (:941-1051, ghosts included, the theta
+ * integrals reflect through th_of_x), and the connection coefficients of
+ * interior zones with the pressure-balanced Gamma^theta_thetatheta
+ * dA/dV - Gamma^phi_thetaphi, dA taken across a dendritic k-level change as
+ * the sum of the two finer faces or half the coarser one (:837-929).
+ * Returns -1 if the dendritic levels are rejected (build_levels). */
+static int build_geometry_spherical(orc_sim *s)
+{
+  const orc_config *c = &s->c;
+  const int NG = ORC_NG;
   sphmap m;
-  sphmap_init(&m, c->rcoord);
+  sphmap_init(&m, c->rcoord, c->rtrans);
   const int n1 = c->n[0], n2 = c->n[1], n3 = c->n[2];
   const double dph = c->ndim == 3 ? c->dx[2] : 2.0 * M_PI;
-  const int nn[3] = {n1, n2, n3};
-  for (int d = 0; d < 3; d++)
-    for (int q = 0; q < 3; q++) s->mom[q][d] = calloc(nn[d] + 2 * ORC_NG, sizeof(double));
+  int rc = 0;
+  s->maxjlev = s->maxklev = 0;
+  if (c->dendritic) rc = build_levels(s, &m);
+  for (int q = 0; q < 3; q++) {
+    s->mom[q][0] = calloc(n1 + 2 * NG, sizeof(double));
+    s->lmom[q][0] = calloc((size_t)(s->maxjlev + 1) * (n2 + 2 * NG), sizeof(double));
+    s->lmom[q][1] = calloc((size_t)(s->maxklev + 1) * (n3 + 2 * NG), sizeof(double));
+    s->mom[q][1] = s->lmom[q][0];
+    s->mom[q][2] = s->lmom[q][1];
+  }
   /* init_interp_vol :592-669 */
-  for (int i = -ORC_NG; i < n1 + ORC_NG; i++) {
+  for (int i = -NG; i < n1 + NG; i++) {
     double x0 = c->startx[0] + i * c->dx[0], x1 = c->startx[0] + (i + 1) * c->dx[0];
-    s->mom[2][0][i + ORC_NG] = orc_romb(x0, x1, f_r2dr_dx, &m);
-    s->mom[1][0][i + ORC_NG] = orc_romb(x0, x1, f_xr1g, &m);
-    s->mom[0][0][i + ORC_NG] = orc_romb(x0, x1, f_xr2g, &m);
+    s->mom[2][0][i + NG] = orc_romb(x0, x1, f_r2dr_dx, &m);
+    s->mom[1][0][i + NG] = orc_romb(x0, x1, f_xr1g, &m);
+    s->mom[0][0][i + NG] = orc_romb(x0, x1, f_xr2g, &m);
   }
-  for (int j = -ORC_NG; j < n2 + ORC_NG; j++) {
-    double x0 = c->startx[1] + j * 1 * c->dx[1], x1 = c->startx[1] + (j + 1) * 1 * c->dx[1];
-    s->mom[2][1][j + ORC_NG] = orc_romb(x0, x1, f_sthdth_dx, &m);
-    s->mom[1][1][j + ORC_NG] = orc_romb(x0, x1, f_xth1g, &m);
-    s->mom[0][1][j + ORC_NG] = orc_romb(x0, x1, f_xth2g, &m);
+  for (int lev = 0; lev <= s->maxjlev; lev++) {
+    const int sc = 1 << lev;
+    double *G = s->lmom[2][0] + (size_t)lev * (n2 + 2 * NG), *B = s->lmom[1][0] + (size_t)lev * (n2 + 2 * NG);
+    double *A = s->lmom[0][0] + (size_t)lev * (n2 + 2 * NG);
+    for (int j = -NG; j < n2 / sc + NG; j++) {
+      double x0 = c->startx[1] + j * sc * c->dx[1], x1 = c->startx[1] + (j + 1) * sc * c->dx[1];
+      G[j + NG] = orc_romb(x0, x1, f_sthdth_dx, &m);
+      B[j + NG] = orc_romb(x0, x1, f_xth1g, &m);
+      A[j + NG] = orc_romb(x0, x1, f_xth2g, &m);
+    }
   }
-  for (int k = -ORC_NG; k < n3 + ORC_NG; k++) {
-    double x0 = c->startx[2] + k * 1 * c->dx[2], x1 = c->startx[2] + (k + 1) * 1 * c->dx[2];
-    s->mom[2][2][k + ORC_NG] = 1 * c->dx[2];
-    s->mom[1][2][k + ORC_NG] = (OSQR(x1) - OSQR(x0)) / 2.0;
-    s->mom[0][2][k + ORC_NG] = (OCUBE(x1) - OCUBE(x0)) / 3.0;
+  for (int lev = 0; lev <= s->maxklev; lev++) {
+    const int sc = 1 << lev;
+    double *G = s->lmom[2][1] + (size_t)lev * (n3 + 2 * NG), *B = s->lmom[1][1] + (size_t)lev * (n3 + 2 * NG);
+    double *A = s->lmom[0][1] + (size_t)lev * (n3 + 2 * NG);
+    for (int k = -NG; k < n3 / sc + NG; k++) {
+      double x0 = c->startx[2] + k * sc * c->dx[2], x1 = c->startx[2] + (k + 1) * sc * c->dx[2];
+      G[k + NG] = sc * c->dx[2];
+      B[k + NG] = (OSQR(x1) - OSQR(x0)) / 2.0;
+      A[k + NG] = (OCUBE(x1) - OCUBE(x0)) / 3.0;
+    }
   }
-  const int kg = c->ndim == 3 ? ORC_NG : 0;
-  for (int i = -ORC_NG; i < n1 + ORC_NG; i++) {
+  const int kg = c->ndim == 3 ? NG : 0;
+  const int top_axis = fabs((c->startx[1] + n2 * c->dx[1]) - 1) < 1.0e-15;
+  for (int i = -NG; i < n1 + NG; i++) {
     double x0lo = c->startx[0] + i * c->dx[0], x0hi = c->startx[0] + (i + 1) * c->dx[0];
     double r2dr_int = orc_romb(x0lo, x0hi, f_r2dr_dx, &m);
     double vrf = r2dr_int;
     double rint_r2dr_dx3 = orc_romb(x0lo, x0hi, f_r2dr_dx3, &m);
     double r5 = 0.2 * (pow(r_of_x(&m, x0hi), 5) - pow(r_of_x(&m, x0lo), 5));
-    for (int j = -ORC_NG; j < n2 + ORC_NG; j++) {
-      double x1lo = c->startx[1] + j * 1 * c->dx[1], x1hi = c->startx[1] + (j + 1) * 1 * c->dx[1];
+    const int nj = njs(s, i);
+    for (int j = -NG; j < nj + NG; j++) {
+      double x1lo = c->startx[1] + j * djs(s, i) * c->dx[1], x1hi = c->startx[1] + (j + 1) * djs(s, i) * c->dx[1];
       double sthdth_int = orc_romb(x1lo, x1hi, f_sthdth_dx, &m);
       double vthf = sthdth_int;
       double thint_sthdth_dx3 = orc_romb(x1lo, x1hi, f_sthdth_dx3, &m);
       double thint_sth3dth_dx = orc_romb(x1lo, x1hi, f_sth3dth_dx, &m);
+      const int dkz = c->ndim == 3 ? dks(s, i, j) : 1;
       orc_zone_geom g;
       memset(&g, 0, sizeof(g));
-      g.volume = ((r2dr_int * sthdth_int) * 1) * dph;                          /* :708-740 */
-      if (i >= 0 && i <= n1 && j >= 0 && j <= n2) {                            /* :745-806 */
-        double dph_int = 1 * dph;
+      g.volume = ((r2dr_int * sthdth_int) * dkz) * dph;                        /* :708-740 */
+      if (i >= 0 && i <= n1 && j >= 0 && j <= nj) {                            /* :745-806 */
+        double dph_int = dkz * dph;
         g.area[0] = (f_r2dr_dx(x0lo, &m) * sthdth_int) * dph_int;
         g.area[1] = (r2dr_int * f_sthdth_dx(x1lo, &m)) * dph_int;
         if (c->ndim == 3) g.area[2] = r2dr_int * sthdth_int;
-        if ((j == 0 && c->startx[1] == -1) || (j == n2 && fabs((c->startx[1] + n2 * c->dx[1]) - 1) < 1.0e-15))
-          g.area[1] = 0.0;                                                     /* :808-833 */
+        if ((j == 0 && c->startx[1] == -1) || (j == nj && top_axis)) g.area[1] = 0.0;   /* :808-833 */
       }
       /* init_gcov :941-1010 */
       g.gcov[0][0] = rint_r2dr_dx3 / vrf;
@@ -266,7 +414,8 @@ static void build_geometry_spherical(orc_sim *s)
         g.gcon[d] = (fabs(g.gcov[0][d]) > 1.0e-50) ? 1.0 / g.gcov[0][d] : 0.0;
       for (int loc = 0; loc <= c->ndim; loc++)                                 /* :1037-1051 */
         for (int d = 0; d < 3; d++) g.scale[loc][d] = sqrt(g.gcov[loc][d]);
-      for (int k = -kg; k < n3 + kg; k++) s->geom[zid(s, i, j, k)] = g;
+      const int nk = c->ndim == 3 ? nks(s, i, j) : 1;
+      for (int k = -kg; k < nk + kg; k++) s->geom[zid(s, i, j, k)] = g;
     }
   }
   /* init_conn :837-929, interior zones (after volumes and areas) */
@@ -276,12 +425,13 @@ static void build_geometry_spherical(orc_sim *s)
     double rint_r3 = orc_romb(x0lo, x0hi, f_r3, &m);
     double rint_rdr_dx2 = orc_romb(x0lo, x0hi, f_rdr_dx2, &m);
     double vrf = orc_romb(x0lo, x0lo + c->dx[0], f_r2dr_dx, &m);      /* Vr(x0lo) */
-    for (int j = 0; j < n2; j++) {
-      double x1lo = c->startx[1] + j * 1 * c->dx[1], x1hi = c->startx[1] + (j + 1) * 1 * c->dx[1];
+    for (int j = 0; j < njs(s, i); j++) {
+      double x1lo = c->startx[1] + j * djs(s, i) * c->dx[1], x1hi = c->startx[1] + (j + 1) * djs(s, i) * c->dx[1];
       double vthf = orc_romb(x1lo, x1hi, f_sthdth_dx, &m);
       double thint_sthdth_dx3 = orc_romb(x1lo, x1hi, f_sthdth_dx3, &m);
       double thint_cthdth_dx2 = orc_romb(x1lo, x1hi, f_cthdth_dx2, &m);
-      for (int k = 0; k < n3; k++) {
+      const int nk = c->ndim == 3 ? nks(s, i, j) : 1;
+      for (int k = 0; k < nk; k++) {
         orc_zone_geom *g = &s->geom[zid(s, i, j, k)];
         memset(g->conn, 0, sizeof(g->conn));
         g->conn[0][0][0] = rint_r2d2r_dx2 / vrf;
@@ -294,13 +444,21 @@ static void build_geometry_spherical(orc_sim *s)
         g->conn[2][2][1] = g->conn[2][1][2];
         g->conn[0][2][2] = ((-g->conn[2][2][0]) * g->gcov[0][2]) / g->gcov[0][0];
         g->conn[1][2][2] = ((-g->conn[2][2][1]) * g->gcov[0][2]) / g->gcov[0][1];
-        double dA = s->geom[zid(s, i, j + 1, k)].area[1];
+        double dA;
+        const int dk0 = c->ndim == 3 ? dks(s, i, j) : 1, dk1 = c->ndim == 3 ? dks(s, i, j + 1) : 1;
+        if (dk1 < dk0)
+          dA = s->geom[zid(s, i, j + 1, 2 * k)].area[1] + s->geom[zid(s, i, j + 1, 2 * k + 1)].area[1];
+        else if (dk1 > dk0)
+          dA = 0.5 * s->geom[zid(s, i, j + 1, k / 2)].area[1];
+        else
+          dA = s->geom[zid(s, i, j + 1, k)].area[1];
         dA -= g->area[1];
         double dV = g->volume;
         g->conn[1][1][1] = dA / dV - g->conn[2][1][2];
       }
     }
   }
+  return rc;
 }
 
 /* 1D spherical geometry, metric_spherical_general.c:558-1051 with NDIM=1
@@ -309,10 +467,7 @@ static void build_geometry_spherical1d(orc_sim *s)
 {
   const orc_config *c = &s->c;
   sphmap m;
-  m.rcoord = c->rcoord;
-  m.poly_xt = 1.2;
-  m.poly_alpha = 4;
-  m.poly_norm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m.poly_alpha + 1.0)) * (1.0 / pow(m.poly_xt, m.poly_alpha))));
+  sphmap_init(&m, c->rcoord, c->rtrans);
   const double dx2 = 2.0 * M_PI;
   const double x1lo = -1.0, x1hi = 1.0;
   int n = c->n[0];
@@ -402,9 +557,24 @@ orc_sim *orc_create(const orc_config *cfg)
   s->shock = calloc(s->nall, 1);
   s->geom = calloc(s->nall, sizeof(orc_zone_geom));
   for (int d = 0; d < 3; d++) s->flux[d] = calloc(s->nall * s->nvars, sizeof(double));
+  if (cfg->dendritic && !(cfg->geom == ORC_GEOM_SPHERICAL && cfg->ndim == 3)) {
+    orc_destroy(s);
+    return NULL;
+  }
   if (cfg->geom == ORC_GEOM_SPHERICAL && cfg->ndim == 1) build_geometry_spherical1d(s);
-  else if (cfg->geom == ORC_GEOM_SPHERICAL) build_geometry_spherical(s);
-  else build_geometry_cartesian(s);
+  else if (cfg->geom == ORC_GEOM_SPHERICAL) {
+    if (build_geometry_spherical(s)) {
+      orc_destroy(s);
+      return NULL;
+    }
+  } else build_geometry_cartesian(s);
+  if (s->dj) {
+    /* zones read by the dendritic sweeps: each level's interior and NG ghosts */
+    s->live = calloc(s->nall, 1);
+    for (int i = -ORC_NG; i < s->c.n[0] + ORC_NG; i++)
+      for (int j = -ORC_NG; j < njs(s, i) + ORC_NG; j++)
+        for (int k = -ORC_NG; k < nks(s, i, j) + ORC_NG; k++) s->live[zid(s, i, j, k)] = 1;
+  }
   s->t = 0.0;
   s->istep = 0;
   s->firstc = 1;
@@ -417,13 +587,28 @@ void orc_destroy(orc_sim *s)
 {
   if (!s) return;
   free(s->p); free(s->ph); free(s->src); free(s->eos); free(s->shock); free(s->geom);
-  for (int d = 0; d < 3; d++) {
-    free(s->flux[d]);
-    for (int q = 0; q < 3; q++) free(s->mom[q][d]);
+  for (int d = 0; d < 3; d++) free(s->flux[d]);
+  for (int q = 0; q < 3; q++) {
+    if (s->lmom[q][0] || s->lmom[q][1]) {
+      free(s->mom[q][0]);
+      free(s->lmom[q][0]);
+      free(s->lmom[q][1]);
+    } else {
+      for (int d = 0; d < 3; d++) free(s->mom[q][d]);
+    }
   }
+  free(s->dj); free(s->dk); free(s->live);
   free(s);
 }
 
+/* zone (i,j,k) of the level-local index space is a real zone */
+static int zone_valid(const orc_sim *s, int i, int j, int k)
+{
+  return j < njs(s, i) && k < nks(s, i, j);
+}
+
+/* Dendritic grids keep the (i, j, k) storage of the uniform grid with j and
+ * k level-local; positions beyond a level's nj / nk are not zones. */
 void orc_set_prims(orc_sim *s, const double *prims)
 {
   const int *n = s->c.n;
@@ -431,6 +616,7 @@ void orc_set_prims(orc_sim *s, const double *prims)
   for (int k = 0; k < n[2]; k++)
     for (int j = 0; j < n[1]; j++)
       for (int i = 0; i < n[0]; i++) {
+        if (!zone_valid(s, i, j, k)) continue;
         long z = zid(s, i, j, k);
         long q = i + (long)n[0] * (j + (long)n[1] * k);
         for (int v = 0; v < s->nvars; v++) s->p[z * s->nvars + v] = prims[v * ncell + q];
@@ -446,6 +632,10 @@ void orc_get_prims(const orc_sim *s, double *prims)
       for (int i = 0; i < n[0]; i++) {
         long z = zid(s, i, j, k);
         long q = i + (long)n[0] * (j + (long)n[1] * k);
+        if (!zone_valid(s, i, j, k)) {
+          for (int v = 0; v < s->nvars; v++) prims[v * ncell + q] = 0.0;
+          continue;
+        }
         for (int v = 0; v < s->nvars; v++) prims[v * ncell + q] = s->p[z * s->nvars + v];
       }
 }
@@ -539,6 +729,7 @@ static void update_eos(orc_sim *s, const double *P)
 {
   double gam = s->c.gam;
   for (long z = 0; z < s->nall; z++) {
+    if (s->live && !s->live[z]) continue;
     const double *p = &P[z * s->nvars];
     double *e = &s->eos[z * NEOS];
     if (s->c.eos_tab) {
@@ -566,6 +757,7 @@ static void sources(orc_sim *s, const double *P)
   for (int k = 0; k < c->n[2]; k++)
     for (int j = 0; j < c->n[1]; j++)
       for (int i = 0; i < c->n[0]; i++) {
+        if (!zone_valid(s, i, j, k)) continue;
         long z = zid(s, i, j, k);
         const double *p = &P[z * s->nvars];
         double *src = &s->src[z * s->nvars];
@@ -604,14 +796,16 @@ static void polar_states(orc_sim *s, const double *P, int i, int j, int k, doubl
 {
   const orc_config *c = &s->c;
   sphmap m;
-  sphmap_init(&m, c->rcoord);
+  sphmap_init(&m, c->rcoord, c->rtrans);
   const int nv = s->nvars;
-  const double th = th_of_x(&m, s->mom[1][1][j + ORC_NG] / s->mom[2][1][j + ORC_NG]);
+  const size_t lo = (size_t)lev_of(djs(s, i)) * (c->n[1] + 2 * ORC_NG) + j + ORC_NG;
+  const double th = th_of_x(&m, s->lmom[1][0][lo] / s->lmom[2][0][lo]);
   const double sth = sin(th), cth = cos(th);
+  const int dk = dks(s, i, j);
   for (int kt = k - 1; kt <= k; kt++) {
     long z = zid(s, i, j, kt);
     const orc_zone_geom *g = &s->geom[z];
-    double ph = c->startx[2] + (kt + 0.5) * 1 * c->dx[2];
+    double ph = c->startx[2] + (kt + 0.5) * dk * c->dx[2];
     double sph = sin(ph), cph = cos(ph);
     double lam[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
     double v[3];
@@ -620,7 +814,7 @@ static void polar_states(orc_sim *s, const double *P, int i, int j, int k, doubl
       for (int mm = 0; mm < 3; mm++) v[l] += lam[l][mm] * P[z * nv + ORC_U1 + mm] * g->scale[0][mm];
     }
     /* the face at phi_k: the lower face of zone k, the upper face of zone k-1 */
-    ph = c->startx[2] + k * 1 * c->dx[2];
+    ph = c->startx[2] + k * dk * c->dx[2];
     sph = sin(ph);
     cph = cos(ph);
     double lf[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
@@ -633,6 +827,398 @@ static void polar_states(orc_sim *s, const double *P, int i, int j, int k, doubl
       for (int mm = 0; mm < 3; mm++) dst[ORC_U1 + l][q] += lf[mm][l] * v[mm] / scf[l];
     }
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Dendritic 3D spherical grids */
+
+/* bc.c:705-1421 on a dendritic grid: radial REFLECT/OUTFLOW, theta across
+ * the pole (SPHERICAL_ORIGIN), phi PERIODIC, each at the level of the ghost
+ * zone's own row.  Filled in the order radial, theta, phi; the theta fill
+ * also covers the radial ghost zones and the phi fill every row including
+ * the ghost rows, so that the corner ghosts the prolongation stencils reach
+ * hold the composition of the fills (the reference's loops leave corners
+ * unset). */
+static void fill_ghost_dend(orc_sim *s, double *P)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], NG = ORC_NG, nv = s->nvars;
+  for (int side = 0; side < 2; side++) {
+    const int code = side ? c->bc_hi[0] : c->bc_lo[0];
+    for (int gi = 0; gi < NG; gi++) {
+      const int i = side ? n1 + gi : -NG + gi;
+      for (int j = 0; j < njs(s, i); j++)
+        for (int k = 0; k < nks(s, i, j); k++) {
+          int ip, jp = j, kp = k;
+          if (code == ORC_BC_REFLECT) {
+            ip = side ? 2 * n1 - 1 - i : -1 - i;
+            jp = j * djs(s, i) / djs(s, ip);
+            kp = k * dks(s, i, j) / dks(s, ip, jp);
+          } else {
+            ip = side ? n1 - 1 : 0;
+          }
+          double *dst = &P[zid(s, i, j, k) * nv];
+          const double *src = &P[zid(s, ip, jp, kp) * nv];
+          for (int v = 0; v < nv; v++) dst[v] = src[v];
+          if (code == ORC_BC_REFLECT) dst[ORC_U1] *= -1.0;
+        }
+    }
+  }
+  for (int i = -NG; i < n1 + NG; i++) {
+    const int nj = njs(s, i);
+    for (int side = 0; side < 2; side++)
+      for (int gj = 0; gj < NG; gj++) {
+        const int j = side ? nj + gj : -NG + gj;
+        const int jp = side ? 2 * nj - 1 - j : -1 - j;
+        const int nkp = nks(s, i, jp);
+        for (int k = 0; k < nks(s, i, j); k++) {
+          const int kp = (k + nkp / 2) % nkp;
+          double *dst = &P[zid(s, i, j, k) * nv];
+          const double *src = &P[zid(s, i, jp, kp) * nv];
+          for (int v = 0; v < nv; v++) dst[v] = src[v];
+          dst[ORC_U2] *= -1.0;
+          dst[ORC_U3] *= -1.0;
+        }
+      }
+  }
+  for (int i = -NG; i < n1 + NG; i++)
+    for (int j = -NG; j < njs(s, i) + NG; j++) {
+      const int nk = nks(s, i, j);
+      for (int gk = 0; gk < NG; gk++) {
+        const int klo = -NG + gk, khi = nk + gk;
+        double *dl = &P[zid(s, i, j, klo) * nv], *dh = &P[zid(s, i, j, khi) * nv];
+        const double *sl = &P[zid(s, i, j, klo + nk) * nv], *sh = &P[zid(s, i, j, khi - nk) * nv];
+        for (int v = 0; v < nv; v++) {
+          dl[v] = sl[v];
+          dh[v] = sh[v];
+        }
+      }
+    }
+}
+
+#define OSIGN(a) ((a) < 0.0 ? -1.0 : 1.0)
+#define OMINMOD(a, b) (0.5 * (OSIGN(a) + OSIGN(b)) * OMIN(fabs(a), fabs(b)))   /* decs.h:629-630 */
+
+static const double *lev_arr(const orc_sim *s, int which, int dir, int lev)
+{
+  const int n = dir == 0 ? s->c.n[1] : s->c.n[2];
+  return s->lmom[which][dir] + (size_t)lev * (n + 2 * ORC_NG) + ORC_NG;
+}
+
+/* zone values of a pencil row: nvars primitives, then P and Gamma_1 */
+static void zone_row(const orc_sim *s, const double *P, int i, int j, int k, double *pp)
+{
+  const long z = zid(s, i, j, k);
+  for (int v = 0; v < s->nvars; v++) pp[v] = P[z * s->nvars + v];
+  pp[s->nvars] = s->eos[z * NEOS + E_PRESS];
+  pp[s->nvars + 1] = s->eos[z * NEOS + E_GAMMA];
+}
+
+/* kslopes0, step.c:783-840: limited k-slopes of row (ii,jj) evaluated at the
+ * phi centroid of zone kp of row (ip,jp) */
+static void kslopes0(const orc_sim *s, const double *P, int ip, int jp, int kp, int ii, int jj, double *pp)
+{
+  const int nv = s->nvars;
+  const int lp = lev_of(dks(s, ip, jp)), l = lev_of(dks(s, ii, jj));
+  const double *Bp = lev_arr(s, 1, 1, lp), *Gp = lev_arr(s, 2, 1, lp);
+  const double *B = lev_arr(s, 1, 1, l), *G = lev_arr(s, 2, 1, l);
+  const double x2p = Bp[kp] / Gp[kp];
+  const int kk = kp * dks(s, ip, jp) / dks(s, ii, jj);
+  const double x20 = B[kk - 1] / G[kk - 1], x21 = B[kk] / G[kk], x22 = B[kk + 1] / G[kk + 1];
+  const double idx2m = 1.0 / (x21 - x20), idx2p = 1.0 / (x22 - x21), idx2c = 1.0 / (x22 - x20);
+  double a0[ORC_NSIZE], a1[ORC_NSIZE], a2[ORC_NSIZE];
+  zone_row(s, P, ii, jj, kk - 1, a0);
+  zone_row(s, P, ii, jj, kk, a1);
+  zone_row(s, P, ii, jj, kk + 1, a2);
+  for (int v = 0; v < nv + 2; v++) {
+    const double sm = (a1[v] - a0[v]) * idx2m, sp = (a2[v] - a1[v]) * idx2p, sc = (a2[v] - a0[v]) * idx2c;
+    const double sg = OMINMOD(2.0 * OMINMOD(sm, sp), sc);
+    pp[v] = a1[v] + sg * (x2p - x21);
+  }
+}
+
+/* prolongate0, step.c:905-951: the value of radial level ii at the (theta,
+ * phi) centroid of zone (i,j,k): k-slopes on rows jj-1..jj+1, then a limited
+ * theta slope */
+static void prolongate0(const orc_sim *s, const double *P, int i, int j, int k, int ii, double *pp)
+{
+  const int nv = s->nvars;
+  const int jj = j * djs(s, i) / djs(s, ii);
+  const int lp = lev_of(djs(s, i)), l = lev_of(djs(s, ii));
+  const double *Bp = lev_arr(s, 1, 0, lp), *Gp = lev_arr(s, 2, 0, lp);
+  const double *B = lev_arr(s, 1, 0, l), *G = lev_arr(s, 2, 0, l);
+  const double x1p = Bp[j] / Gp[j];
+  double p0[ORC_NSIZE], p1[ORC_NSIZE], p2[ORC_NSIZE];
+  kslopes0(s, P, i, j, k, ii, jj - 1, p0);
+  kslopes0(s, P, i, j, k, ii, jj, p1);
+  kslopes0(s, P, i, j, k, ii, jj + 1, p2);
+  const double x10 = B[jj - 1] / G[jj - 1], x11 = B[jj] / G[jj], x12 = B[jj + 1] / G[jj + 1];
+  const double idx1m = 1.0 / (x11 - x10), idx1p = 1.0 / (x12 - x11), idx1c = 1.0 / (x12 - x10);
+  for (int v = 0; v < nv + 2; v++) {
+    const double sm = (p1[v] - p0[v]) * idx1m, sp = (p2[v] - p1[v]) * idx1p, sc = (p2[v] - p0[v]) * idx1c;
+    const double sg = OMINMOD(2.0 * OMINMOD(sm, sp), sc);
+    pp[v] = p1[v] + sg * (x1p - x11);
+  }
+}
+
+/* restrict0, step.c:1012-1049: volume-weighted mean of the finer zones of
+ * level ii under zone (i,j,k).  The last pencil entry averages the primitive
+ * at the Gamma_1 index of the eos record (p[...][GAMMA], step.c:1048), i.e.
+ * v^3, as the reference does. */
+static void restrict0(const orc_sim *s, const double *P, int i, int j, int k, int ii, double *pp)
+{
+  const int nv = s->nvars;
+  const int njp = djs(s, i) / djs(s, ii), jp = j * njp;
+  double vol_sum = 0.0;
+  for (int jj = jp; jj < jp + njp; jj++) {
+    const int nkp = dks(s, i, j) / dks(s, ii, jj), kp = k * nkp;
+    for (int kk = kp; kk < kp + nkp; kk++) vol_sum += s->geom[zid(s, ii, jj, kk)].volume;
+  }
+  vol_sum = 1.0 / vol_sum;
+  for (int v = 0; v < nv + 2; v++) pp[v] = 0.0;
+  for (int jj = jp; jj < jp + njp; jj++) {
+    const int nkp = dks(s, i, j) / dks(s, ii, jj), kp = k * nkp;
+    for (int kk = kp; kk < kp + nkp; kk++) {
+      const long z = zid(s, ii, jj, kk);
+      const double vol_frac = s->geom[z].volume * vol_sum;
+      for (int v = 0; v < nv; v++) pp[v] += P[z * nv + v] * vol_frac;
+      pp[nv] += s->eos[z * NEOS + E_PRESS] * vol_frac;
+      pp[nv + 1] += P[z * nv + E_GAMMA] * vol_frac;
+    }
+  }
+}
+
+/* kslopes1, step.c:842-896: row jj evaluated at the phi centroid of the
+ * theta face's zone kp, the face at the finer k-level of rows j-1 and j */
+static void kslopes1(const orc_sim *s, const double *P, int i, int j, int kp, int jj, double *pp)
+{
+  const int nv = s->nvars;
+  const int dkf = OMIN(dks(s, i, j), dks(s, i, j - 1));
+  const int lp = lev_of(dkf), l = lev_of(dks(s, i, jj));
+  const double *Bp = lev_arr(s, 1, 1, lp), *Gp = lev_arr(s, 2, 1, lp);
+  const double *B = lev_arr(s, 1, 1, l), *G = lev_arr(s, 2, 1, l);
+  const double x2p = Bp[kp] / Gp[kp];
+  const int kk = kp * dkf / dks(s, i, jj);
+  const double x20 = B[kk - 1] / G[kk - 1], x21 = B[kk] / G[kk], x22 = B[kk + 1] / G[kk + 1];
+  const double idx2m = 1.0 / (x21 - x20), idx2p = 1.0 / (x22 - x21), idx2c = 1.0 / (x22 - x20);
+  double a0[ORC_NSIZE], a1[ORC_NSIZE], a2[ORC_NSIZE];
+  zone_row(s, P, i, jj, kk - 1, a0);
+  zone_row(s, P, i, jj, kk, a1);
+  zone_row(s, P, i, jj, kk + 1, a2);
+  for (int v = 0; v < nv + 2; v++) {
+    const double sm = (a1[v] - a0[v]) * idx2m, sp = (a2[v] - a1[v]) * idx2p, sc = (a2[v] - a0[v]) * idx2c;
+    const double sg = OMINMOD(2.0 * OMINMOD(sm, sp), sc);
+    pp[v] = a1[v] + sg * (x2p - x21);
+  }
+}
+
+/* prolongate1 (step.c:962-1010) and restrict1 (:1051-1085) */
+static void theta_row(const orc_sim *s, const double *P, int i, int j, int kp, int jj, double *pp)
+{
+  const int nv = s->nvars;
+  const int dkf = OMIN(dks(s, i, j), dks(s, i, j - 1)), dkr = dks(s, i, jj);
+  if (dkr < dkf) {
+    const int nkp = dkf / dkr, k0 = kp * nkp;
+    double vol_sum = 0.0;
+    for (int kk = k0; kk < k0 + nkp; kk++) vol_sum += s->geom[zid(s, i, jj, kk)].volume;
+    vol_sum = 1.0 / vol_sum;
+    for (int v = 0; v < nv + 2; v++) pp[v] = 0.0;
+    for (int kk = k0; kk < k0 + nkp; kk++) {
+      const long z = zid(s, i, jj, kk);
+      const double vol_frac = s->geom[z].volume * vol_sum;
+      for (int v = 0; v < nv; v++) pp[v] += P[z * nv + v] * vol_frac;
+      pp[nv] += s->eos[z * NEOS + E_PRESS] * vol_frac;
+      pp[nv + 1] += s->eos[z * NEOS + E_GAMMA] * vol_frac;
+    }
+  } else if (dkr == dkf) {
+    zone_row(s, P, i, jj, kp * dkf / dkr, pp);
+  } else {
+    kslopes1(s, P, i, j, kp, jj, pp);
+  }
+}
+
+/* transverse_shock, fluid.c:189-210 */
+static int transverse_shock_dend(const orc_sim *s, int i, int j, int k, int dir)
+{
+  if (dir == 1) {
+    const int nkm = dks(s, i, j) / dks(s, i, j - 1), km = k * nkm;
+    int orflag = (s->shock[zid(s, i, j, k)] == 0);
+    for (int kk = km; kk < km + nkm; kk++) orflag = (orflag || s->shock[zid(s, i, j - 1, kk)] == 0);
+    return orflag;
+  }
+  return s->shock[zid(s, i, j, k)] == 0 || s->shock[zid(s, i, j, k - 1)] == 0;
+}
+
+static void riemann_store(orc_sim *s, int d, int isT, double *pl, double *pr, long zg, long zf, double width,
+                          int *isshock)
+{
+  double flux[ORC_NSIZE], vr[3], ss;
+  const int nv = s->nvars;
+  const orc_zone_geom *g = &s->geom[zg];
+  *isshock = 0;
+  if (isT) {
+    orc_riemann_hlle(s->nhydro, nv, pl, pr, g, d, flux, &ss, vr);
+    s->nhlle++;
+  } else {
+    *isshock = orc_riemann_hllc(s->nhydro, nv, pl, pr, g, d, flux, &ss, vr);
+  }
+  for (int vv = 0; vv < s->nhydro; vv++) s->flux[d][zf * nv + vv] = flux[vv];
+  s->min_dt = OMIN(s->min_dt, width / fabs(ss));
+}
+
+/* prim_to_fluxes3 on a dendritic grid, step.c:1159-1199 (radial faces),
+ * 1201-1265 (theta faces at the finer k-level of the two rows), 1266-1421
+ * (phi faces, polar-axis states in the last theta row) */
+static void sweep_dend(orc_sim *s, const double *P)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], nv = s->nvars, NG = ORC_NG;
+  double pen[ORC_NSIZE][2 * NG], pleft[ORC_NSIZE][2 * NG], pright[ORC_NSIZE][2 * NG];
+  double al[2 * NG], be[2 * NG], ga[2 * NG], x[2 * NG], pp[ORC_NSIZE], pl[ORC_NSIZE], pr[ORC_NSIZE];
+  for (int i = 0; i <= n1; i++)
+    for (int j = 0; j < njs(s, i); j++)
+      for (int k = 0; k < nks(s, i, j); k++) {
+        for (int q = 0; q < 2 * NG; q++) {
+          const int ii = i - NG + q;
+          al[q] = s->mom[0][0][ii + NG];
+          be[q] = s->mom[1][0][ii + NG];
+          ga[q] = s->mom[2][0][ii + NG];
+          x[q] = c->startx[0] + ii * c->dx[0];
+          if (djs(s, ii) < djs(s, i)) restrict0(s, P, i, j, k, ii, pp);
+          else prolongate0(s, P, i, j, k, ii, pp);
+          for (int vv = 0; vv < nv + 2; vv++) pen[vv][q] = pp[vv];
+        }
+        interp_pencil(s, pen, pleft, pright, al, be, ga, x);
+        for (int vv = 0; vv < nv + 2; vv++) {
+          pl[vv] = pright[vv][NG - 1];
+          pr[vv] = pleft[vv][NG];
+        }
+        int isshock;
+        const long z = zid(s, i, j, k);
+        riemann_store(s, 0, 0, pl, pr, z, z, c->dx[0], &isshock);
+        if (isshock) {
+          s->nshock++;
+          s->shock[z] = 0;
+          int jm = j * djs(s, i) / djs(s, i - 1), km = k * dks(s, i, j) / dks(s, i - 1, jm);
+          s->shock[zid(s, i - 1, jm, km)] = 0;
+          jm = j * djs(s, i) / djs(s, i - 2);
+          km = k * dks(s, i, j) / dks(s, i - 2, jm);
+          s->shock[zid(s, i - 2, jm, km)] = 0;
+        }
+      }
+  for (int i = 0; i < n1; i++) {
+    const int lj = lev_of(djs(s, i));
+    const double *A = lev_arr(s, 0, 0, lj), *B = lev_arr(s, 1, 0, lj), *G = lev_arr(s, 2, 0, lj);
+    for (int j = 0; j <= njs(s, i); j++)
+      for (int k = 0; k < nks(s, i, j); k++) {
+        const int dkj = dks(s, i, j), dkm = dks(s, i, j - 1), dkf = OMIN(dkj, dkm);
+        const int nkp = OMAX(1, dkj / dkf), kp = k * OMAX(1, dkj / dkm);
+        for (int kk = kp; kk < kp + nkp; kk++) {
+          for (int q = 0; q < 2 * NG; q++) {
+            const int jj = j - NG + q;
+            al[q] = A[jj];
+            be[q] = B[jj];
+            ga[q] = G[jj];
+            x[q] = c->startx[1] + jj * djs(s, i) * c->dx[1];
+            theta_row(s, P, i, j, kk, jj, pp);
+            for (int vv = 0; vv < nv + 2; vv++) pen[vv][q] = pp[vv];
+          }
+          interp_pencil(s, pen, pleft, pright, al, be, ga, x);
+          for (int vv = 0; vv < nv + 2; vv++) {
+            pl[vv] = pright[vv][NG - 1];
+            pr[vv] = pleft[vv][NG];
+          }
+          int isshock;
+          riemann_store(s, 1, transverse_shock_dend(s, i, j, k, 1), pl, pr, zid(s, i, j, k), zid(s, i, j, kk),
+                        djs(s, i) * c->dx[1], &isshock);
+        }
+      }
+  }
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < njs(s, i); j++) {
+      const int dk = dks(s, i, j), lk = lev_of(dk);
+      const double *A = lev_arr(s, 0, 1, lk), *B = lev_arr(s, 1, 1, lk), *G = lev_arr(s, 2, 1, lk);
+      for (int k = 0; k <= nks(s, i, j); k++) {
+        for (int q = 0; q < 2 * NG; q++) {
+          const int kt = k - NG + q;
+          x[q] = c->startx[2] + kt * dk * c->dx[2];
+          ga[q] = G[kt];
+          be[q] = B[kt];
+          al[q] = A[kt];
+          zone_row(s, P, i, j, kt, pp);
+          for (int vv = 0; vv < nv + 2; vv++) pen[vv][q] = pp[vv];
+        }
+        interp_pencil(s, pen, pleft, pright, al, be, ga, x);
+        if (j == njs(s, i) - 1) polar_states(s, P, i, j, k, pen, pleft, pright);
+        for (int vv = 0; vv < nv + 2; vv++) {
+          pl[vv] = pright[vv][NG - 1];
+          pr[vv] = pleft[vv][NG];
+        }
+        int isshock;
+        const long z = zid(s, i, j, k);
+        riemann_store(s, 2, transverse_shock_dend(s, i, j, k, 2), pl, pr, z, z, dk * c->dx[2], &isshock);
+      }
+    }
+}
+
+/* push on a dendritic grid (step.c:374-470): radial fan-in over the finer
+ * outer zones, theta faces with the half-area and two-face rules, phi faces */
+static void push_dend(orc_sim *s, double dtpush, const double *p, const double *ph, double *pf, int stage)
+{
+  const orc_config *c = &s->c;
+  const int nv = s->nvars;
+  double pfact, phfact, ffact;
+  if (stage == 0) { pfact = 1; phfact = 0; ffact = 1; }
+  else { pfact = 0.5; phfact = 0.5; ffact = 0.5; }
+  double min_dt_src = 1.0e100;
+  const double *F0 = s->flux[0], *F1 = s->flux[1], *F2 = s->flux[2];
+  for (int ii = 0; ii < c->n[0]; ii++)
+    for (int jj = 0; jj < njs(s, ii); jj++)
+      for (int kk = 0; kk < nks(s, ii, jj); kk++) {
+        const long z = zid(s, ii, jj, kk);
+        const orc_zone_geom *gm = &s->geom[z];
+        double u[ORC_NSIZE], uh[ORC_NSIZE], divF[ORC_NSIZE];
+        orc_p_to_u(&p[z * nv], u, gm->gcov[0], s->nhydro);
+        orc_p_to_u(&ph[z * nv], uh, gm->gcov[0], s->nhydro);
+        double local_dt = orc_dt_source_control(&p[z * nv], u, &s->src[z * nv]);
+        min_dt_src = OMIN(min_dt_src, local_dt);
+        for (int vv = 0; vv < nv; vv++) divF[vv] = 0.0;
+        const int jp = jj * djs(s, ii) / djs(s, ii + 1);
+        const int njp = (djs(s, ii + 1) < djs(s, ii)) ? djs(s, ii) / djs(s, ii + 1) : 1;
+        for (int j = jp; j < jp + njp; j++) {
+          const int nkp = dks(s, ii, jj) / dks(s, ii + 1, j), kp = kk * nkp;
+          for (int k = kp; k < kp + nkp; k++) {
+            const long zf = zid(s, ii + 1, j, k);
+            for (int vv = 0; vv < nv; vv++) divF[vv] += F0[zf * nv + vv];
+          }
+        }
+        for (int vv = 0; vv < nv; vv++) divF[vv] -= F0[z * nv + vv];
+        const int dk0 = dks(s, ii, jj), dkp = dks(s, ii, jj + 1), dkm = dks(s, ii, jj - 1);
+        if (dkp > dk0) {
+          const long zf = zid(s, ii, jj + 1, kk);
+          for (int vv = 0; vv < nv; vv++) divF[vv] += 0.5 * F1[zf * nv + vv];
+        } else if (dkp < dk0) {
+          const long za = zid(s, ii, jj + 1, 2 * kk), zb = zid(s, ii, jj + 1, 2 * kk + 1);
+          for (int vv = 0; vv < nv; vv++) divF[vv] += F1[za * nv + vv] + F1[zb * nv + vv];
+        } else {
+          const long zf = zid(s, ii, jj + 1, kk);
+          for (int vv = 0; vv < nv; vv++) divF[vv] += F1[zf * nv + vv];
+        }
+        if (dkm < dk0) {
+          const long za = zid(s, ii, jj, 2 * kk), zb = zid(s, ii, jj, 2 * kk + 1);
+          for (int vv = 0; vv < nv; vv++) divF[vv] -= 0.5 * F1[za * nv + vv] + 0.5 * F1[zb * nv + vv];
+        } else {
+          for (int vv = 0; vv < nv; vv++) divF[vv] -= F1[z * nv + vv];
+        }
+        const long z2 = zid(s, ii, jj, kk + 1);
+        for (int vv = 0; vv < nv; vv++) divF[vv] += F2[z2 * nv + vv] - F2[z * nv + vv];
+        for (int vv = 0; vv < nv; vv++) {
+          double ddt = (divF[vv] / gm->volume) - s->src[z * nv + vv];
+          u[vv] = ((pfact * u[vv]) + (phfact * uh[vv])) - ((ffact * dtpush) * ddt);
+        }
+        orc_u_to_p(u, &pf[z * nv], gm->gcon, s->nhydro);
+      }
+  s->min_dt = OMIN(s->min_dt, min_dt_src);
 }
 
 /* One direction of the flux sweep: step.c:500-562 (d=0), 736-760 (d=1),
@@ -745,13 +1331,15 @@ static void monopole_edges(const orc_sim *s, const double *P, double *grav)
 {
   const orc_config *c = &s->c;
   const int n1 = c->n[0], nv = s->nvars;
-  sphmap m = {c->rcoord, 0.0, 0.0, 0.0};
+  sphmap m;
+  sphmap_init(&m, c->rcoord, c->rtrans);
   double mr = 0.0;
   grav[0] = 0.0;
   for (int ii = 1; ii <= n1; ii++) {
     double ms = 0.0;
     for (int k = 0; k < c->n[2]; k++)
       for (int j = 0; j < c->n[1]; j++) {
+        if (!zone_valid(s, ii - 1, j, k)) continue;
         long z = zid(s, ii - 1, j, k);
         ms += s->geom[z].volume * P[z * nv + ORC_RHO];
       }
@@ -777,9 +1365,18 @@ static void monopole(orc_sim *s, const double *P)
     double gz = (A * be) / ga + B;
     for (int k = 0; k < c->n[2]; k++)
       for (int j = 0; j < c->n[1]; j++) {
-        long z = zid(s, ii, j, k), zp = zid(s, ii + 1, j, k);
-        double vr = s->flux[0][zp * nv + ORC_RHO];
-        double area = s->geom[zp].area[0] + 1.0e-16;
+        if (!zone_valid(s, ii, j, k)) continue;
+        long z = zid(s, ii, j, k);
+        double vr = 0.0, area = 0.0;
+        const int njp = djs(s, ii) / djs(s, ii + 1), jp = j * njp;
+        for (int jj = jp; jj < jp + njp; jj++) {
+          const int nkp = dks(s, ii, j) / dks(s, ii + 1, jj), kp = k * nkp;
+          for (int kk = kp; kk < kp + nkp; kk++) {
+            const long zp = zid(s, ii + 1, jj, kk);
+            vr += s->flux[0][zp * nv + ORC_RHO];
+            area += s->geom[zp].area[0] + 1.0e-16;
+          }
+        }
         vr /= area;
         s->src[z * nv + ORC_U1] += P[z * nv + ORC_RHO] * gz;
         double vl = s->flux[0][z * nv + ORC_RHO] / (s->geom[z].area[0] + 1.0e-16);
@@ -794,12 +1391,18 @@ static void monopole(orc_sim *s, const double *P)
 static void stage(orc_sim *s, int st)
 {
   double *P = st == 0 ? s->p : s->ph;
-  reset_boundaries(s, P);
+  if (s->dj) fill_ghost_dend(s, P);
+  else reset_boundaries(s, P);
   update_eos(s, P);
   sources(s, P);
-  for (int d = 0; d < s->c.ndim; d++) sweep_dir(s, P, d);
+  if (s->dj) sweep_dend(s, P);
+  else
+    for (int d = 0; d < s->c.ndim; d++) sweep_dir(s, P, d);
   if (s->c.use_monopole) monopole(s, P);
-  if (st == 0) push(s, s->dt, s->p, s->p, s->ph, 0);
+  if (s->dj) {
+    if (st == 0) push_dend(s, s->dt, s->p, s->p, s->ph, 0);
+    else push_dend(s, s->dt, s->p, s->ph, s->p, 1);
+  } else if (st == 0) push(s, s->dt, s->p, s->p, s->ph, 0);
   else push(s, s->dt, s->p, s->ph, s->p, 1);
 }
 
@@ -811,7 +1414,8 @@ void orc_step(orc_sim *s)
   if (c->ndim > 1) {   /* set_shock_flag(0): interior zones only (fluid.c:82-89) */
     for (int k = 0; k < c->n[2]; k++)
       for (int j = 0; j < c->n[1]; j++)
-        for (int i = 0; i < c->n[0]; i++) s->shock[zid(s, i, j, k)] = -1;
+        for (int i = 0; i < c->n[0]; i++)
+          if (zone_valid(s, i, j, k)) s->shock[zid(s, i, j, k)] = -1;
   }
   stage(s, 0);
   stage(s, 1);
@@ -826,10 +1430,13 @@ double orc_estimate_dt(orc_sim *s)
   for (int k = 0; k < c->n[2]; k++)
     for (int j = 0; j < c->n[1]; j++)
       for (int i = 0; i < c->n[0]; i++) {
+        if (!zone_valid(s, i, j, k)) continue;
         long z = zid(s, i, j, k);
         double local_dt = 0.0;
         for (int d = 0; d < c->ndim; d++) {
           double local_dx = c->dx[d];
+          if (d == 1) local_dx *= djs(s, i);
+          if (d == 2) local_dx *= dks(s, i, j);
           local_dt += (fabs(s->p[z * s->nvars + ORC_U1 + d]) + (s->eos[z * NEOS + E_CS] / s->geom[z].scale[0][d])) / local_dx;
           s->min_dt = OMIN(s->min_dt, 1.0 / local_dt);
         }
@@ -870,6 +1477,7 @@ void orc_conserved_totals(const orc_sim *s, double out[2])
   for (int k = 0; k < c->n[2]; k++)
     for (int j = 0; j < c->n[1]; j++)
       for (int i = 0; i < c->n[0]; i++) {
+        if (!zone_valid(s, i, j, k)) continue;
         long z = zid(s, i, j, k);
         double u[ORC_NSIZE];
         orc_p_to_u(&s->p[z * s->nvars], u, s->geom[z].gcov[0], s->nhydro);
@@ -883,4 +1491,25 @@ void orc_conserved_totals(const orc_sim *s, double out[2])
       }
   out[0] = sum[0] + comp[0];
   out[1] = sum[1] + comp[1];
+}
+
+/* dendritic level counts: nj[i] (n1 entries) and nk[i*n2 + j] for j < nj[i]
+ * (0 beyond); uniform grids report n2 and n3.  Returns the maximum theta
+ * and phi refinement levels as maxjlev * 256 + maxklev. */
+int orc_dendritic_levels(const orc_sim *s, int *nj, int *nk)
+{
+  const int n1 = s->c.n[0], n2 = s->c.n[1];
+  for (int i = 0; i < n1; i++) {
+    nj[i] = njs(s, i);
+    for (int j = 0; j < n2; j++) nk[(long)i * n2 + j] = j < njs(s, i) ? nks(s, i, j) : 0;
+  }
+  return s->maxjlev * 256 + s->maxklev;
+}
+
+/* theta (dir 0) or phi (dir 1) moments of refinement level lev, length n + 2 NG */
+const double *orc_level_moment(const orc_sim *s, int which, int dir, int lev)
+{
+  const int maxl = dir == 0 ? s->maxjlev : s->maxklev;
+  if (lev < 0 || lev > maxl || !s->lmom[which][dir]) return 0;
+  return lev_arr(s, which, dir, lev) - ORC_NG;
 }

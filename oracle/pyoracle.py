@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 RHO, UU, U1, U2, U3, YE = 0, 1, 2, 3, 4, 5
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC = 0, 1, 2, 3
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
-RCOORD_UNIFORM, RCOORD_LOG = 0, 1
+RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
+BC_SPHERICAL_ORIGIN = 5
 
 
 class ZoneGeom(ctypes.Structure):
@@ -48,7 +49,9 @@ class Config(ctypes.Structure):
                 ("g0", ctypes.c_double * 3),
                 ("use_fixed_grav", ctypes.c_int),
                 ("use_monopole", ctypes.c_int),
-                ("eos_tab", ctypes.c_void_p)]
+                ("eos_tab", ctypes.c_void_p),
+                ("rtrans", ctypes.c_double),
+                ("dendritic", ctypes.c_int)]
 
 
 def _load(fast=False):
@@ -75,6 +78,10 @@ def _load(fast=False):
         getattr(lib, fn).restype = ctypes.c_double
     lib.orc_istep.argtypes = [ctypes.c_void_p]
     lib.orc_istep.restype = ctypes.c_int
+    lib.orc_dendritic_levels.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.orc_dendritic_levels.restype = ctypes.c_int
+    lib.orc_level_moment.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_level_moment.restype = ctypes.POINTER(ctypes.c_double)
     for fn in ("orc_shock_face_count", "orc_hlle_face_count"):
         getattr(lib, fn).argtypes = [ctypes.c_void_p]
         getattr(lib, fn).restype = ctypes.c_long
@@ -157,8 +164,10 @@ def ptr(a):
 def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
                 geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, interp_order=3,
                 g0=(0.0, 0.0, 0.0), dt_init_safety=0.1, dt_grow_safety=1.1,
-                dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100, monopole=False, eos_tab=None):
-    """eos_tab: an EosTab (tabulated EOS; nvars must be 6 to carry Ye)."""
+                dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100, monopole=False, eos_tab=None,
+                rtrans=0.0, dendritic=False):
+    """eos_tab: an EosTab (tabulated EOS; nvars must be 6 to carry Ye).
+    rtrans: scale of the sinh radial map; dendritic: 3D spherical dendritic grid."""
     c = Config()
     c.ndim = ndim
     n = list(n) + [1] * (3 - len(n))
@@ -182,6 +191,8 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
     c.initial_dt = initial_dt
     c.use_fixed_grav = int(any(v != 0.0 for v in g0))
     c.use_monopole = int(bool(monopole))
+    c.rtrans = float(rtrans)
+    c.dendritic = int(bool(dendritic))
     if eos_tab is not None:
         c.eos_tab = ctypes.cast(ctypes.pointer(eos_tab), ctypes.c_void_p)
         c._eos_keep = eos_tab
@@ -195,6 +206,8 @@ class Sim:
         self.lib = lib(fast)
         self.cfg = cfg
         self.h = self.lib.orc_create(ctypes.byref(cfg))
+        if not self.h:
+            raise ValueError("oracle rejected the configuration (dendritic levels or geometry)")
         self.n = [cfg.n[d] if d < cfg.ndim else 1 for d in range(3)]
         self.nvars = cfg.nvars
 
@@ -253,6 +266,26 @@ class Sim:
         arr = np.ctypeslib.as_array(ctypes.cast(g, ctypes.POINTER(ctypes.c_double)),
                                     shape=(nall, ctypes.sizeof(ZoneGeom) // 8))
         return arr.copy()
+
+    def levels(self):
+        """Dendritic level counts: (nj[n1], nk[n1, n2] with 0 beyond nj[i], maxjlev, maxklev)."""
+        n1, n2 = self.n[0], self.n[1]
+        nj = np.zeros(n1, dtype=np.int32)
+        nk = np.zeros((n1, n2), dtype=np.int32)
+        r = self.lib.orc_dendritic_levels(self.h, nj.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          nk.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return nj, nk, r // 256, r % 256
+
+    def level_moments(self, d, lev):
+        """theta (d=1) or phi (d=2) moments [alpha, beta, Gamma] of refinement level lev."""
+        ng = self.cfg.n[d] + 4
+        out = []
+        for q in range(3):
+            p = self.lib.orc_level_moment(self.h, q, d - 1, lev)
+            if not p:
+                return None
+            out.append(np.ctypeslib.as_array(p, shape=(ng,)).copy())
+        return out
 
     def moments(self, d):
         ng = self.cfg.n[d] + 4 if d < self.cfg.ndim else 5
