@@ -45,6 +45,7 @@ extern "C" {
 
 #define MDS_RCOORD_UNIFORM 0     /* r = |x|     (metric_spherical_general.c:149-152) */
 #define MDS_RCOORD_LOG     1     /* r = exp(x)  (metrics/metric_spherical_logr.c:3-38) */
+#define MDS_RCOORD_SINH    2     /* r = rtrans sinh(|x|/rtrans) (metric_spherical_general.c:155-161, 186-208) */
 
 /* error codes */
 #define MDS_OK          0
@@ -140,6 +141,24 @@ typedef struct mds_grid {
    * 3D spherical grids span theta over x1 in [-1,1] (startx[1] = -1,
    * dx[1] = 2/n2, POLYTH map) and phi over [0, 2 pi). */
   double *zone;
+  /* input: scale of MDS_RCOORD_SINH */
+  double rtrans;
+  /* input: 3D spherical dendritic grid (utils.c:181-590): theta zones
+   * coarsened by dj(i) = n2/nj(i) near the origin, phi zones by
+   * dk(i,j) = n3/nk(i,j) near the poles, 2:1 between neighbours.  Zone
+   * (i, j, k) then uses level-local j < nj(i) and k < nk(i, j); per-(i, j)
+   * records (zone) and state arrays keep the uniform grid's (i, j, k)
+   * positions, and positions beyond a level's count are not zones.  Needs
+   * n2 and n3 powers of two (n3 >= 4), theta over [-1, 1]. */
+  int dendritic;
+  /* built for dendritic grids, else NULL / 0: */
+  int *dj;                 /* n[0] + 2*NG, zone -NG first */
+  int *dk;                 /* (n[0] + 2*NG) x (n[1] + 2*NG), [i + NG][j + NG], ghosts included */
+  int maxjlev, maxklev;    /* finest level l has dj or dk = 2^l */
+  /* theta (0) and phi (1) moments and FP32-bulk coefficients per level l
+   * (dj or dk = 2^l), level l at offset l * (n + 2*NG) (times MDS_RC_COUNT
+   * for lrcoef); level 0 equals alpha/beta/Gamma/rcoef[1 or 2] */
+  double *lalpha[2], *lbeta[2], *lGamma[2], *lrcoef[2];
 } mds_grid;
 
 /* Build the reference geometry for a grid.  geom/rcoord/ndim/n/startx/dx

@@ -157,6 +157,7 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
   }
   int i = (int)(q % pb.n[0]);
   int j = (int)((q / pb.n[0]) % pb.n[1]);
+  const int kz = (int)(q / ((long long)pb.n[0] * pb.n[1]));
   double loc = 0.0;
   for (int d = 0; d < pb.ndim; d++) {
     double sc = 1.0;
@@ -164,9 +165,13 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
       sc = pb.zone[((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT + MDS_ZS_SCALE0 + d];
     else if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
     double v = sp.v[2 + d][pb.off + q];
-    loc += (fabs(v) + (cs / sc)) / pb.dx[d];
+    double ldx = pb.dx[d];                                  // DJS / DKS widths (step.c:179-181)
+    if (pb.lj && d == 1) ldx *= (double)(1 << dend_lj(pb, i));
+    if (pb.lj && d == 2 && dend_valid(pb, i, j, 0)) ldx *= (double)(1 << dend_lk(pb, i, j));
+    loc += (fabs(v) + (cs / sc)) / ldx;
   }
   double b = 1.0 / loc;
+  if (pb.lj && !dend_valid(pb, i, j, kz)) b = 1.0e300;
   // one atomic per wavefront when the whole wavefront belongs to one problem
   int p0 = __shfl(lo, 0, 64);
   if (__all(lo == p0)) {
@@ -189,6 +194,9 @@ __global__ void totals_kernel(const DevProb *prob, int p, long long nz, int nv, 
     double rho = sp.v[0][pb.off + q], uu = sp.v[1][pb.off + q];
     double vol = pb.volume;
     double g[3] = {1.0, 1.0, 1.0};
+    if (pb.lj && !dend_valid(pb, (int)(q % pb.n[0]), (int)((q / pb.n[0]) % pb.n[1]),
+                             (int)(q / ((long long)pb.n[0] * pb.n[1]))))
+      continue;
     if (pb.sph && pb.ndim == 3) {
       const int i = (int)(q % pb.n[0]), j = (int)((q / pb.n[0]) % pb.n[1]);
       const double *zs = pb.zone + ((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT;
@@ -296,6 +304,48 @@ extern "C" int mds_destroy(mds_ctx *c)
   return MDS_OK;
 }
 
+// Radial and theta faces of a dendritic grid whose 4-zone pencils cross a
+// level change (prolongation / restriction, step.c:1159-1265), as (i, j, k, 0)
+// with k the face's own phi index; every other face of the grid is handled by
+// the regular flux kernels.
+static int build_dendritic_lists(mds_ctx *c, const mds_grid *g, Sph3Launch &L)
+{
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2], NG = MDS_NG, S1 = n2 + 2 * NG;
+  auto dj = [&](int i) { return g->dj[i + NG]; };
+  auto dk = [&](int i, int j) { return g->dk[(size_t)(i + NG) * S1 + j + NG]; };
+  std::vector<int4> rad, th;
+  for (int i = 0; i <= n1; i++) {
+    const int nj = n2 / dj(i);
+    bool jreg = true;
+    for (int s = -2; s < 2; s++) jreg = jreg && dj(i + s) == dj(i);
+    for (int j = 0; j < nj; j++) {
+      bool reg = jreg;
+      for (int s = -2; s < 2 && reg; s++) reg = dk(i + s, j) == dk(i, j);
+      if (reg) continue;
+      for (int k = 0; k < n3 / dk(i, j); k++) rad.push_back(make_int4(i, j, k, 0));
+    }
+  }
+  for (int i = 0; i < n1; i++) {
+    const int nj = n2 / dj(i);
+    for (int j = 0; j <= nj; j++) {
+      const int dkf = dk(i, j) < dk(i, j - 1) ? dk(i, j) : dk(i, j - 1);
+      bool reg = true;
+      for (int s = -2; s < 2; s++) reg = reg && dk(i, j + s) == dkf;
+      if (reg) continue;
+      for (int k = 0; k < n3 / dkf; k++) th.push_back(make_int4(i, j, k, 0));
+    }
+  }
+  std::vector<int4> *lists[2] = {&rad, &th};
+  for (int t = 0; t < 2; t++) {
+    L.nirr[t] = (long long)lists[t]->size();
+    L.irr[t] = nullptr;
+    if (!L.nirr[t]) continue;
+    int rc1 = upload_geo(c, lists[t]->data(), lists[t]->size() * sizeof(int4), (const void **)&L.irr[t]);
+    if (rc1) return rc1;
+  }
+  return 0;
+}
+
 static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const mds_run_params *rp)
 {
   c->nprob = nprob;
@@ -363,6 +413,15 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
           return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
       }
     }
+    if (g->dendritic) {
+      if (!c->sph3 || !g->dj || !g->dk) return set_err(c, MDS_EINVAL, "problem %d: dendritic grids are built 3D spherical grids", p);
+      if (P.bc_lo[0] != MDS_BC_REFLECT || (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT) ||
+          P.bc_lo[1] != MDS_BC_SPHERICAL_ORIGIN || P.bc_hi[1] != MDS_BC_SPHERICAL_ORIGIN ||
+          P.bc_lo[2] != MDS_BC_PERIODIC || P.bc_hi[2] != MDS_BC_PERIODIC)
+        return set_err(c, MDS_ENOTIMPL,
+                       "problem %d: dendritic grids need REFLECT inner / OUTFLOW or REFLECT outer radius, the pole in "
+                       "theta and periodic phi", p);
+    }
     long long nz = (long long)g->n[0] * (g->ndim > 1 ? g->n[1] : 1) * (g->ndim > 2 ? g->n[2] : 1);
     c->zstart[p + 1] = c->zstart[p] + nz;
     DevProb &d = c->hprob[p];
@@ -371,6 +430,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     d.ndim = g->ndim;
     d.sph = c->sph;
     d.rcoord = g->rcoord;
+    d.rtrans = g->rtrans;
     d.gam = P.gam;
     for (int k = 0; k < 3; k++) {
       d.n[k] = k < g->ndim ? g->n[k] : 1;
@@ -396,6 +456,34 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       if (rc1) return rc1;
       rc1 = upload_geo(c, rcf.data(), rcf.size() * sizeof(float), (const void **)&d.rc[k]);
       if (rc1) return rc1;
+    }
+    if (g->dendritic) {
+      // level tables (log2 of dj, dk) and the theta / phi moments of every level
+      const int N1 = g->n[0] + 2 * MDS_NG, N2 = g->n[1] + 2 * MDS_NG;
+      std::vector<unsigned char> lj(N1), lk((size_t)N1 * N2, 0);
+      auto lg = [](int f) { int l = 0; while (f >>= 1) l++; return l; };
+      for (int q = 0; q < N1; q++) lj[q] = (unsigned char)lg(g->dj[q]);
+      for (size_t q = 0; q < lk.size(); q++) lk[q] = g->dk[q] > 0 ? (unsigned char)lg(g->dk[q]) : 0;
+      int rc1 = upload_geo(c, lj.data(), lj.size(), (const void **)&d.lj);
+      if (rc1) return rc1;
+      rc1 = upload_geo(c, lk.data(), lk.size(), (const void **)&d.lk);
+      if (rc1) return rc1;
+      const int nlev[2] = {g->maxjlev + 1, g->maxklev + 1};
+      for (int t = 0; t < 2; t++) {
+        const int nzk = g->n[t + 1] + 2 * MDS_NG;
+        std::vector<double> mom(3 * (size_t)nzk * nlev[t]);
+        std::vector<float> rcf((size_t)nzk * MDS_RC_COUNT * nlev[t]);
+        for (size_t q = 0; q < (size_t)nzk * nlev[t]; q++) {
+          mom[3 * q + 0] = g->lalpha[t][q];
+          mom[3 * q + 1] = g->lbeta[t][q];
+          mom[3 * q + 2] = g->lGamma[t][q];
+        }
+        for (size_t q = 0; q < rcf.size(); q++) rcf[q] = (float)g->lrcoef[t][q];
+        rc1 = upload_geo(c, mom.data(), mom.size() * sizeof(double), (const void **)&d.lmom[t]);
+        if (rc1) return rc1;
+        rc1 = upload_geo(c, rcf.data(), rcf.size() * sizeof(float), (const void **)&d.lrc[t]);
+        if (rc1) return rc1;
+      }
     }
     if (c->sph) {
       if (!g->zone) return set_err(c, MDS_EINVAL, "problem %d: spherical grid without zone records", p);
@@ -462,6 +550,12 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     if (c->mono) {
       HIPCHK(c, hipMalloc(&L.ms, sizeof(double) * h.n[0]));
       HIPCHK(c, hipMalloc(&L.grav, sizeof(double) * h.n[0]));
+    }
+    const mds_grid *g = probs[0].grid;
+    if (g->dendritic) {
+      L.dend = 1;
+      int rc1 = build_dendritic_lists(c, g, L);
+      if (rc1) return rc1;
     }
   }
   HIPCHK(c, hipMalloc(&c->dprob, sizeof(DevProb) * nprob));

@@ -39,6 +39,7 @@ struct DevProb {
   int bc_lo[3], bc_hi[3];
   int sph;                 // spherical 1D column
   int rcoord;              // radial map (MDS_RCOORD_*)
+  double rtrans;           // MDS_RCOORD_SINH scale
   double gam;
   double g0[3];            // fixed gravity (contravariant)
   double startx[3], dx[3];
@@ -47,7 +48,25 @@ struct DevProb {
   const float *rc[3];      // per direction (n[d]+2NG)*MDS_RC_COUNT
   const double *zone;      // spherical: (n[0]+2NG)*MDS_ZG_COUNT
   const ZoneF32 *z32;      // spherical, FP32-bulk: n[0]+2NG records
+  // dendritic 3D spherical grids (null otherwise): log2 of the theta and phi
+  // coarsening, ghost-extended: lj[i + NG], lk[(i + NG) * (n[1] + 2NG) + j + NG]
+  const unsigned char *lj, *lk;
+  // theta (0) and phi (1) moments per level l at l * (n + 2NG) * 3 (alpha,
+  // beta, Gamma per zone) and FP32 coefficients at l * (n + 2NG) * MDS_RC_COUNT
+  const double *lmom[2];
+  const float *lrc[2];
 };
+
+// level-local counts of a dendritic grid (uniform grids: level 0 everywhere)
+__device__ __forceinline__ int dend_lj(const DevProb &pb, int i) { return pb.lj ? pb.lj[i + MDS_NG] : 0; }
+__device__ __forceinline__ int dend_lk(const DevProb &pb, int i, int j)
+{
+  return pb.lk ? pb.lk[(size_t)(i + MDS_NG) * (pb.n[1] + 2 * MDS_NG) + (j + MDS_NG)] : 0;
+}
+__device__ __forceinline__ bool dend_valid(const DevProb &pb, int i, int j, int k)
+{
+  return !pb.lj || (j < (pb.n[1] >> dend_lj(pb, i)) && k < (pb.n[2] >> dend_lk(pb, i, j)));
+}
 
 // Device-resident clock of one problem (the reference globals t, dt, istep,
 // min_dt and main.c's dtold).
@@ -105,6 +124,9 @@ struct Sph3Launch {
   void *P, *G, *CS;        // table EOS arrays (state precision)
   double *ms, *grav;       // shell masses and monopole field (n1 each)
   unsigned char *flags;    // shock tags
+  int dend;                // dendritic grid
+  const int4 *irr[2];      // dendritic: radial and theta faces whose pencils cross a level change
+  long long nirr[2];       // (i, j, k, 0) per face; k is the face's own phi index
 };
 
 #define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D stage sweep

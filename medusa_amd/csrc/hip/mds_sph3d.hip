@@ -1,4 +1,5 @@
-// mds_sph3d.hip -- RK stage of a 3D spherical (r, theta, phi) grid (gfx950).
+// mds_sph3d.hip -- RK stage of a 3D spherical (r, theta, phi) grid (gfx950),
+// uniform or dendritic.
 //
 // Direction-split, as the reference's prim_to_fluxes3 + push (step.c:1088-1430,
 // 327-478): per stage
@@ -17,6 +18,15 @@
 // across the pole (SPHERICAL_ORIGIN: mirror in j, phi + pi, v2 and v3
 // flipped) or REFLECT/OUTFLOW, phi PERIODIC.  Face fluxes are kept in FP64
 // (area weighted) for both precisions.
+//
+// Dendritic grids (utils.c:181-590) keep the (i, j, k) storage with
+// level-local j < nj(i), k < nk(i, j).  The flux kernels above handle the
+// faces whose 4-zone pencil stays on one level; the radial and theta faces
+// whose pencils cross a level change are listed at set-up and done by
+// flux_irr with the reference's prolongation (kslopes0/1, prolongate0/1,
+// limited linear) and volume-weighted restriction (restrict0/1),
+// step.c:783-1085.  The update gathers the finer faces of a coarser zone
+// (the fan-in of step.c:374-433).
 #include "mds_internal.h"
 #include "mds_physics.h"
 #include "mds_eos.h"
@@ -52,13 +62,32 @@ MDS_DEV const double *rec3(const DevProb &pb, int i, int j)
   return pb.zone + ((size_t)(j + MDS_NG) * N1 + (i + MDS_NG)) * MDS_ZS_COUNT;
 }
 
-// Map a stencil zone onto the zone the ghost fill copies there; sgn gets the
-// velocity sign flips.
+// zone counts of radial zone i and of row (i, j) at their levels
+MDS_DEV int nj_of(const DevProb &pb, int i) { return pb.n[1] >> dend_lj(pb, i); }
+MDS_DEV int nk_of(const DevProb &pb, int i, int j) { return pb.n[2] >> dend_lk(pb, i, j); }
+
+// index idx of level `from` expressed on level `to` (idx * 2^from / 2^to)
+MDS_DEV int rescale(int idx, int from, int to) { return from >= to ? idx << (from - to) : idx >> (to - from); }
+
+// volume moments (alpha, beta, Gamma per zone, zone -NG first) of direction D
+// at refinement level lev, and the FP32 coefficients
+MDS_DEV const double *mom_of(const DevProb &pb, int D, int lev)
+{
+  return (D == 0 || !pb.lj) ? pb.mom[D] : pb.lmom[D - 1] + (size_t)lev * (pb.n[D] + 2 * MDS_NG) * 3;
+}
+MDS_DEV const float *rc_of(const DevProb &pb, int D, int lev)
+{
+  return (D == 0 || !pb.lj) ? pb.rc[D] : pb.lrc[D - 1] + (size_t)lev * (pb.n[D] + 2 * MDS_NG) * MDS_RC_COUNT;
+}
+
+// Map a stencil zone that leaves the grid in direction D onto the zone the
+// ghost fill copies there; sgn gets the velocity sign flips.  Counts are those
+// of the zone's level.
 MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3])
 {
   sgn[0] = sgn[1] = sgn[2] = 1.0;
   int *c = D == 0 ? &i : (D == 1 ? &j : &k);
-  const int n = pb.n[D];
+  const int n = D == 0 ? pb.n[0] : (D == 1 ? nj_of(pb, i) : nk_of(pb, i, j));
   if (*c >= 0 && *c < n) return;
   const int side = *c >= n;
   const int code = side ? pb.bc_hi[D] : pb.bc_lo[D];
@@ -71,10 +100,31 @@ MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double s
     sgn[D] = -1.0;
   } else if (code == MDS_BC_SPHERICAL_ORIGIN && D == 1) {
     *c = side ? 2 * n - 1 - ig : -1 - ig;
-    k = (k + pb.n[2] / 2) % pb.n[2];
+    const int nk = nk_of(pb, i, j);
+    k = (k + nk / 2) % nk;
     sgn[1] = sgn[2] = -1.0;
   } else {
     *c = side ? n - 1 : 0;
+  }
+}
+
+// Any stencil zone of a dendritic grid, corners included: phi wrap within its
+// row, then the radial fill, then the theta fill -- the composition of the
+// reference's fills (its corner ghosts are never filled).
+MDS_DEV void map_any(const DevProb &pb, int &i, int &j, int &k, double sgn[3])
+{
+  sgn[0] = sgn[1] = sgn[2] = 1.0;
+  const int nk = nk_of(pb, i, j);
+  if (k < 0) k += nk;
+  else if (k >= nk) k -= nk;
+  double s[3];
+  if (i < 0 || i >= pb.n[0]) {
+    map_zone(pb, 0, i, j, k, s);
+    sgn[0] *= s[0];
+  }
+  if (j < 0 || j >= nj_of(pb, i)) {
+    map_zone(pb, 1, i, j, k, s);
+    for (int d = 0; d < 3; d++) sgn[d] *= s[d];
   }
 }
 
@@ -96,15 +146,26 @@ MDS_DEV void zone_state(const Sph3Args &a, const StateArrays<R> &in, long long z
   }
 }
 
+// the pencil values of any (possibly ghost or corner) zone
+template <class R>
+MDS_DEV void any_state(const Sph3Args &a, const StateArrays<R> &in, int i, int j, int k, int tab, double q[8])
+{
+  double sgn[3];
+  map_any(a.pb, i, j, k, sgn);
+  zone_state<R>(a, in, zid3(a.pb, i, j, k), tab, a.pb.gam, q);
+  for (int d = 0; d < 3; d++) q[2 + d] *= sgn[d];
+}
+
 // Polar-axis states of the phi face at phi_k between zones k-1 (L) and k (R):
-// velocities rotated through Cartesian components (step.c:1299-1397)
-MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, double vl[3], double vr[3])
+// velocities rotated through Cartesian components (step.c:1299-1397); dk is
+// the row's phi coarsening
+MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, int dk, double vl[3], double vr[3])
 {
   const double sth = rc[MDS_ZS_STH], cth = rc[MDS_ZS_CTH];
   double *vv[2] = {vl, vr};
   for (int s = 0; s < 2; s++) {
     const int kt = k - 1 + s;
-    double ph = pb.startx[2] + (kt + 0.5) * 1 * pb.dx[2];
+    double ph = pb.startx[2] + (kt + 0.5) * dk * pb.dx[2];
     double sph = sin(ph), cph = cos(ph);
     const double lam[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
     double v[3];
@@ -112,7 +173,7 @@ MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, double vl[
       v[l] = 0.0;
       for (int m = 0; m < 3; m++) v[l] += lam[l][m] * vv[s][m] * rc[MDS_ZS_SCALE0 + m];
     }
-    ph = pb.startx[2] + k * 1 * pb.dx[2];
+    ph = pb.startx[2] + (k * dk) * pb.dx[2];
     sph = sin(ph);
     cph = cos(ph);
     const double lf[3][3] = {{sth * cph, cth * cph, -sph}, {sth * sph, cth * sph, cph}, {cth, -sth, 0.0}};
@@ -121,6 +182,275 @@ MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, double vl[
       for (int m = 0; m < 3; m++) o += lf[m][l] * v[m] / rc[MDS_ZS_SCALE0 + l];
       vv[s][l] = o;
     }
+  }
+}
+
+MDS_DEV double minmod(double a, double b)
+{
+  // decs.h:629-630
+  const double sa = a < 0.0 ? -1.0 : 1.0, sb = b < 0.0 ? -1.0 : 1.0;
+  return (0.5 * (sa + sb)) * fmin(fabs(a), fabs(b));
+}
+
+// Reconstruction and Riemann solve of one face from its 4-zone pencil q
+// (zones fd-2 .. fd+1 along D, level lev).  GREC: reconstruct Gamma_1 of a
+// gamma law too (a restricted radial pencil carries v^3 in that slot).
+// hlle: transverse face next to a tagged zone.  Returns isshock.
+template <class R, int D, int TAB, int GREC>
+MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
+                      const double *rc, bool hlle, double F[6], double &sig)
+{
+  FaceGeo<double> fg;
+  if (D == 0) {
+    fg.area = rc[MDS_ZS_AREA0];
+    fg.scale = rc[MDS_ZS_SCALEF0];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF0 + d];
+  } else if (D == 1) {
+    fg.area = rc[MDS_ZS_AREA1];
+    fg.scale = rc[MDS_ZS_SCALEF1];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF1 + d];
+  } else {
+    fg.area = rc[MDS_ZS_AREA2];
+    fg.scale = rc[MDS_ZS_SCALE0 + 2];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
+  }
+  int isshock = 0;
+  sig = 0.0;
+  for (int v = 0; v < 6; v++) F[v] = 0.0;
+  const int s = 1 << lev;
+  if (sizeof(R) == 8) {
+    // FP64: reference operation order
+    const double *mom = mom_of(pb, D, lev);
+    const int b0 = fd - 2 + MDS_NG;
+    ReconGeo64 gl, gr;     // zones fd-1 (left of the face) and fd (right)
+    gl.am = mom[3 * b0]; gl.bm = mom[3 * b0 + 1]; gl.gm = mom[3 * b0 + 2];
+    gl.a0 = mom[3 * (b0 + 1)]; gl.b0 = mom[3 * (b0 + 1) + 1]; gl.g0 = mom[3 * (b0 + 1) + 2];
+    gl.ap = mom[3 * (b0 + 2)]; gl.bp = mom[3 * (b0 + 2) + 1]; gl.gp = mom[3 * (b0 + 2) + 2];
+    gr.am = gl.a0; gr.bm = gl.b0; gr.gm = gl.g0;
+    gr.a0 = gl.ap; gr.b0 = gl.bp; gr.g0 = gl.gp;
+    gr.ap = mom[3 * (b0 + 3)]; gr.bp = mom[3 * (b0 + 3) + 1]; gr.gp = mom[3 * (b0 + 3) + 2];
+    gl.xlo = pb.startx[D] + ((fd - 1) * s) * pb.dx[D];
+    gl.xhi = pb.startx[D] + (fd * s) * pb.dx[D];
+    gr.xlo = gl.xhi;
+    gr.xhi = pb.startx[D] + ((fd + 1) * s) * pb.dx[D];
+    recon_geo_x64(gl);
+    recon_geo_x64(gr);
+    double Lq[8], Rq[8];
+    if (polar) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        if (!TAB && (v == 5 || (v == 7 && !GREC))) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
+        double al, ar, al2, ar2;
+        para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
+        para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+        Lq[v] = ar;
+        Rq[v] = al2;
+      }
+    }
+    if (polar) {
+      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
+      polar_rotate(pb, rc, fk, dk, vl, vr);
+      for (int d = 0; d < 3; d++) { Lq[2 + d] = vl[d]; Rq[2 + d] = vr[d]; }
+    }
+    PState<double, 3> L, Rt;
+    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
+    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
+    Rt.G = Rq[7];
+    if (D == 0 || !hlle) isshock = hllc_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
+    else hlle_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
+  } else {
+    // FP32-bulk: reconstruction and fluxes per unit area in FP32
+    const float *rcf = rc_of(pb, D, lev);
+    const int b0 = fd - 1 + MDS_NG;
+    ReconGeo32 gl, gr;
+    const float *cl = rcf + (size_t)b0 * MDS_RC_COUNT, *cr = rcf + (size_t)(b0 + 1) * MDS_RC_COUNT;
+    gl.clm = cl[MDS_RC_CLM]; gl.clp = cl[MDS_RC_CLP]; gl.crm = cl[MDS_RC_CRM]; gl.crp = cl[MDS_RC_CRP];
+    gl.mu1 = cl[MDS_RC_MU1]; gl.kinv = cl[MDS_RC_KINV]; gl.inur = cl[MDS_RC_INUR]; gl.imu2 = cl[MDS_RC_IMU2];
+    gr.clm = cr[MDS_RC_CLM]; gr.clp = cr[MDS_RC_CLP]; gr.crm = cr[MDS_RC_CRM]; gr.crp = cr[MDS_RC_CRP];
+    gr.mu1 = cr[MDS_RC_MU1]; gr.kinv = cr[MDS_RC_KINV]; gr.inur = cr[MDS_RC_INUR]; gr.imu2 = cr[MDS_RC_IMU2];
+    float Lq[8], Rq[8];
+    if (polar) {
+#pragma unroll
+      for (int v = 0; v < 8; v++) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        if (!TAB && (v == 5 || (v == 7 && !GREC))) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; continue; }
+        float al, ar, al2, ar2;
+        para_recon_f32((float)q[0][v], (float)q[1][v], (float)q[2][v], gl, al, ar);
+        para_recon_f32((float)q[1][v], (float)q[2][v], (float)q[3][v], gr, al2, ar2);
+        Lq[v] = ar;
+        Rq[v] = al2;
+      }
+    }
+    if (polar) {
+      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
+      polar_rotate(pb, rc, fk, dk, vl, vr);
+      for (int d = 0; d < 3; d++) { Lq[2 + d] = (float)vl[d]; Rq[2 + d] = (float)vr[d]; }
+    }
+    PState<float, 3> L, Rt;
+    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
+    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
+    Rt.G = Rq[7];
+    FaceGeo<float> g32;
+    g32.area = 1.0f;
+    g32.scale = (float)fg.scale;
+    for (int d = 0; d < 3; d++) g32.gcov[d] = (float)fg.gcov[d];
+    float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, s32 = 0.0f;
+    if (fg.area >= 1.0e-16) {
+      if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
+      else hlle_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
+      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
+      sig = s32;
+    } else {
+      sig = 1.0e-30;
+    }
+  }
+  return isshock;
+}
+
+// shock-tag test of an interior zone (ghosts and non-zones count as tagged)
+MDS_DEV bool tagged(const Sph3Args &a, int i, int j, int k)
+{
+  const DevProb &pb = a.pb;
+  if (i < 0 || j < 0 || k < 0 || i >= pb.n[0] || j >= nj_of(pb, i) || k >= nk_of(pb, i, j)) return true;
+  return a.flags[zid3(pb, i, j, k)] == 0;
+}
+
+// radial shock tags on zones i, i-1, i-2 at their levels (step.c:1190-1198)
+MDS_DEV void tag_radial(const Sph3Args &a, int i, int j, int k)
+{
+  const DevProb &pb = a.pb;
+  for (int s = 0; s < 3; s++) {
+    const int ii = i - s;
+    if (ii < 0 || ii >= pb.n[0]) continue;
+    const int jm = rescale(j, dend_lj(pb, i), dend_lj(pb, ii));
+    const int km = rescale(k, dend_lk(pb, i, j), dend_lk(pb, ii, jm));
+    if (jm < nj_of(pb, ii) && km < nk_of(pb, ii, jm)) a.flags[zid3(pb, ii, jm, km)] = 0;
+  }
+}
+
+// per-wave CFL (min width/|s| over the wave's faces) and counters
+MDS_DEV void face_dt(const Sph3Args &a, bool ok, double width, double sig, bool shock, bool hl, int D)
+{
+  DevClock *ck = a.clock;
+  const double w = (ok && sig != 0.0) ? width / fabs(sig) : 1.0e300;
+  const double wmin = wave_min(w);
+  const unsigned long long nsh = __popcll(__ballot(ok && D == 0 && shock));
+  const unsigned long long nhl = __popcll(__ballot(ok && D > 0 && hl));
+  if ((threadIdx.x & 63) == 0) {
+    if (wmin < 1.0e300) atomicMin(&ck->mindt, dt_key(wmin));
+    if (nsh) atomicAdd(&a.counters[0], nsh);
+    if (nhl) atomicAdd(&a.counters[1], nhl);
+  }
+}
+
+// kslopes0 (step.c:783-840) / kslopes1 (:842-896): row (ii, jj) evaluated
+// with a limited phi slope at the phi centroid of index kt on level lt
+template <class R, int TAB>
+MDS_DEV void kslopes(const Sph3Args &a, const StateArrays<R> &in, int lt, int kt, int ii, int jj, double pp[8])
+{
+  const DevProb &pb = a.pb;
+  const int l = dend_lk(pb, ii, jj);
+  const double *mp = mom_of(pb, 2, lt) + 3 * (kt + MDS_NG);
+  const double x2p = mp[1] / mp[2];
+  const int kk = rescale(kt, lt, l);
+  const double *m = mom_of(pb, 2, l) + 3 * (kk + MDS_NG);
+  const double x20 = m[-2] / m[-1], x21 = m[1] / m[2], x22 = m[4] / m[5];
+  const double idx2m = 1.0 / (x21 - x20), idx2p = 1.0 / (x22 - x21), idx2c = 1.0 / (x22 - x20);
+  double a0[8], a1[8], a2[8];
+  any_state<R>(a, in, ii, jj, kk - 1, TAB, a0);
+  any_state<R>(a, in, ii, jj, kk, TAB, a1);
+  any_state<R>(a, in, ii, jj, kk + 1, TAB, a2);
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    const double sm = (a1[v] - a0[v]) * idx2m, sp = (a2[v] - a1[v]) * idx2p, sc = (a2[v] - a0[v]) * idx2c;
+    const double sg = minmod(2.0 * minmod(sm, sp), sc);
+    pp[v] = a1[v] + sg * (x2p - x21);
+  }
+}
+
+// radial pencil entry of level ii for the face of zone (i, j, k): restrict0
+// (step.c:1012-1049, whose last entry averages the primitive at the eos
+// record's Gamma index, i.e. v^3, as the reference does) or prolongate0
+// (:905-951)
+template <class R, int TAB>
+MDS_DEV void radial_entry(const Sph3Args &a, const StateArrays<R> &in, int i, int j, int k, int ii, double pp[8])
+{
+  const DevProb &pb = a.pb;
+  const int li = dend_lj(pb, i), lii = dend_lj(pb, ii), lk0 = dend_lk(pb, i, j);
+  if (lii < li) {
+    const int njp = 1 << (li - lii), jp = j * njp;
+    double vol_sum = 0.0;
+    for (int jj = jp; jj < jp + njp; jj++) {
+      const int l = dend_lk(pb, ii, jj);
+      const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
+      const double vol = rec3(pb, ii, jj)[MDS_ZS_VOLUME];
+      for (int kk = 0; kk < nkp; kk++) vol_sum += vol;
+    }
+    vol_sum = 1.0 / vol_sum;
+    for (int v = 0; v < 8; v++) pp[v] = 0.0;
+    for (int jj = jp; jj < jp + njp; jj++) {
+      const int l = dend_lk(pb, ii, jj);
+      const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
+      const double vol_frac = rec3(pb, ii, jj)[MDS_ZS_VOLUME] * vol_sum;
+      for (int kk = k * nkp; kk < k * nkp + nkp; kk++) {
+        double q[8];
+        any_state<R>(a, in, ii, jj, kk, TAB, q);
+#pragma unroll
+        for (int v = 0; v < 7; v++) pp[v] += q[v] * vol_frac;
+        pp[7] += q[4] * vol_frac;
+      }
+    }
+    return;
+  }
+  const int jj = rescale(j, li, lii);
+  const double *mp = mom_of(pb, 1, li) + 3 * (j + MDS_NG);
+  const double x1p = mp[1] / mp[2];
+  double p0[8], p1[8], p2[8];
+  kslopes<R, TAB>(a, in, lk0, k, ii, jj - 1, p0);
+  kslopes<R, TAB>(a, in, lk0, k, ii, jj, p1);
+  kslopes<R, TAB>(a, in, lk0, k, ii, jj + 1, p2);
+  const double *m = mom_of(pb, 1, lii) + 3 * (jj + MDS_NG);
+  const double x10 = m[-2] / m[-1], x11 = m[1] / m[2], x12 = m[4] / m[5];
+  const double idx1m = 1.0 / (x11 - x10), idx1p = 1.0 / (x12 - x11), idx1c = 1.0 / (x12 - x10);
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    const double sm = (p1[v] - p0[v]) * idx1m, sp = (p2[v] - p1[v]) * idx1p, sc = (p2[v] - p0[v]) * idx1c;
+    const double sg = minmod(2.0 * minmod(sm, sp), sc);
+    pp[v] = p1[v] + sg * (x1p - x11);
+  }
+}
+
+// theta pencil entry of row jj for the face (i, j) at phi index kf of level
+// lf = the finer of rows j-1, j: restrict1 (step.c:1051-1085), a copy, or
+// prolongate1 through kslopes1 (:962-1010)
+template <class R, int TAB>
+MDS_DEV void theta_entry(const Sph3Args &a, const StateArrays<R> &in, int i, int lf, int kf, int jj, double pp[8])
+{
+  const DevProb &pb = a.pb;
+  const int lr = dend_lk(pb, i, jj);
+  if (lr < lf) {
+    const int nkp = 1 << (lf - lr), k0 = kf * nkp;
+    const double vol = rec3(pb, i, jj)[MDS_ZS_VOLUME];
+    double vol_sum = 0.0;
+    for (int kk = 0; kk < nkp; kk++) vol_sum += vol;
+    vol_sum = 1.0 / vol_sum;
+    const double vol_frac = vol * vol_sum;
+    for (int v = 0; v < 8; v++) pp[v] = 0.0;
+    for (int kk = k0; kk < k0 + nkp; kk++) {
+      double q[8];
+      any_state<R>(a, in, i, jj, kk, TAB, q);
+#pragma unroll
+      for (int v = 0; v < 8; v++) pp[v] += q[v] * vol_frac;
+    }
+  } else if (lr == lf) {
+    any_state<R>(a, in, i, jj, kf, TAB, pp);
+  } else {
+    kslopes<R, TAB>(a, in, lf, kf, i, jj, pp);
   }
 }
 
@@ -142,6 +472,8 @@ __global__ __launch_bounds__(256) void sph3_eos_kernel(Sph3Args a, StateArrays<R
   const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
   const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
+  if (pb.lj && !dend_valid(pb, (int)(z % pb.n[0]), (int)((z / pb.n[0]) % pb.n[1]), (int)(z / ((long long)pb.n[0] * pb.n[1]))))
+    return;
   double press, lt;
   if (sizeof(R) == 8) {
     double g1, cs;
@@ -167,10 +499,12 @@ __global__ __launch_bounds__(256) void sph3_shell_mass(Sph3Args a, StateArrays<R
 {
   const DevProb &pb = a.pb;
   const int i = blockIdx.x;
-  const int nj = pb.n[1], nk = pb.n[2];
+  const int n2 = pb.n[1], n3 = pb.n[2];
+  const int nj = nj_of(pb, i);
   double s = 0.0;
-  for (int t = threadIdx.x; t < nj * nk; t += blockDim.x) {
-    const int j = t % nj, k = t / nj;
+  for (int t = threadIdx.x; t < n2 * n3; t += blockDim.x) {
+    const int j = t % n2, k = t / n2;
+    if (j >= nj || k >= nk_of(pb, i, j)) continue;
     s += rec3(pb, i, j)[MDS_ZS_VOLUME] * (double)in.v[0][zid3(pb, i, j, k)];
   }
   __shared__ double sh[256];
@@ -188,14 +522,23 @@ __global__ void sph3_shell_field(Sph3Args a, const double *ms, double *grav)
   // one thread: n1 is a few thousand at most
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const DevProb &pb = a.pb;
-  const bool logr = pb.rcoord == MDS_RCOORD_LOG;
   const double GN = 6.67384e-8;
   double mr = 0.0, ge = 0.0;
   for (int i = 0; i < pb.n[0]; i++) {
     mr = mr + ms[i];
     const double xe = pb.startx[0] + (i + 1) * pb.dx[0];
-    const double r = logr ? exp(xe) : fabs(xe);
-    const double gn = ((-GN * mr) / (r * r)) * (logr ? exp(xe) : 1.0);
+    double r, drdx;
+    if (pb.rcoord == MDS_RCOORD_LOG) {
+      r = exp(xe);
+      drdx = exp(xe);
+    } else if (pb.rcoord == MDS_RCOORD_SINH) {
+      r = pb.rtrans * sinh(fabs(xe) / pb.rtrans);
+      drdx = cosh(fabs(xe) / pb.rtrans);
+    } else {
+      r = fabs(xe);
+      drdx = 1.0;
+    }
+    const double gn = ((-GN * mr) / (r * r)) * drdx;
     const double xl = pb.startx[0] + i * pb.dx[0];
     const double A = (gn - ge) / pb.dx[0];
     const double B = ge - A * xl;
@@ -206,24 +549,49 @@ __global__ void sph3_shell_field(Sph3Args a, const double *ms, double *grav)
 }
 
 // ---------------------------------------------------------------------------
-// face fluxes in direction D
+// face fluxes in direction D (on dendritic grids, the faces whose pencil
+// stays on one level)
 // ---------------------------------------------------------------------------
-template <class R, int D, int TAB>
+template <class R, int D, int TAB, int DEND>
 __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
 {
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0] + (D == 0), n2 = pb.n[1] + (D == 1), n3 = pb.n[2] + (D == 2);
   const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = f < (long long)n1 * n2 * n3;
+  bool live = f < (long long)n1 * n2 * n3;
   const long long fl = live ? f : 0;
   const int fi = (int)(fl % n1), fj = (int)((fl / n1) % n2), fk = (int)(fl / ((long long)n1 * n2));
   DevClock *ck = a.clock;
   const bool active = ck->active;
   const double gam = pb.gam;
+  int lev = 0, dk = 1;
+  if (DEND) {
+    const int lj = dend_lj(pb, fi), nj = pb.n[1] >> lj;
+    if (D == 0) {
+      const int lk = fj < nj ? dend_lk(pb, fi, fj) : 0;
+      live = live && fj < nj && fk < (pb.n[2] >> lk);
+      for (int s = -2; s < 2 && live; s++)
+        if (dend_lj(pb, fi + s) != lj || dend_lk(pb, fi + s, fj) != lk) live = false;
+    } else if (D == 1) {
+      live = live && fj <= nj;
+      const int lk = live ? dend_lk(pb, fi, fj) : 0;
+      live = live && fk < (pb.n[2] >> lk);
+      for (int s = -2; s < 2 && live; s++)
+        if (dend_lk(pb, fi, fj + s) != lk) live = false;
+      lev = lj;
+    } else {
+      live = live && fj < nj;
+      const int lk = live ? dend_lk(pb, fi, fj) : 0;
+      live = live && fk <= (pb.n[2] >> lk);
+      lev = lk;
+      dk = 1 << lk;
+    }
+  }
+  const int ci = live ? fi : 0, cj = live ? fj : 0, ck3 = live ? fk : 0;
   // the 4-zone pencil along D
   double q[4][8];
   for (int s = 0; s < 4; s++) {
-    int i = fi, j = fj, k = fk;
+    int i = ci, j = cj, k = ck3;
     if (D == 0) i += s - 2;
     else if (D == 1) j += s - 2;
     else k += s - 2;
@@ -232,137 +600,31 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
     zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q[s]);
     for (int d = 0; d < 3; d++) q[s][2 + d] *= sgn[d];
   }
-  const int fd = D == 0 ? fi : (D == 1 ? fj : fk);
-  const int ri = D == 0 ? fi : fi, rj = D == 1 ? fj : fj;
-  const double *rc = rec3(pb, ri, rj);
-  FaceGeo<double> fg;
-  if (D == 0) {
-    fg.area = rc[MDS_ZS_AREA0];
-    fg.scale = rc[MDS_ZS_SCALEF0];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF0 + d];
-  } else if (D == 1) {
-    fg.area = rc[MDS_ZS_AREA1];
-    fg.scale = rc[MDS_ZS_SCALEF1];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF1 + d];
-  } else {
-    fg.area = rc[MDS_ZS_AREA2];
-    fg.scale = rc[MDS_ZS_SCALE0 + 2];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
-  }
+  const int fd = D == 0 ? ci : (D == 1 ? cj : ck3);
+  const double *rc = rec3(pb, ci, cj);
   // transverse faces: HLLE next to a tagged zone (ghosts count as tagged)
   bool hlle = false;
-  if (D > 0) {
-    auto tagged = [&](int i, int j, int k) {
-      if (i < 0 || j < 0 || k < 0 || i >= pb.n[0] || j >= pb.n[1] || k >= pb.n[2]) return true;
-      return a.flags[zid3(pb, i, j, k)] == 0;
-    };
-    hlle = tagged(fi, fj, fk) || tagged(fi - (D == 0), fj - (D == 1), fk - (D == 2));
-  }
-  const bool polar = (D == 2) && (fj == pb.n[1] - 1);
-  constexpr int NQ = 5 + 2 * TAB + (TAB ? 1 : 1);   // rho u v1 v2 v3 [Ye] P G
-  int isshock = 0;
-  double sig = 0.0, F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (sizeof(R) == 8) {
-    // FP64: reference operation order
-    const double *mom = pb.mom[D];
-    const int b0 = fd - 2 + MDS_NG;
-    ReconGeo64 gl, gr;     // zones fd-1 (left of the face) and fd (right)
-    gl.am = mom[3 * b0]; gl.bm = mom[3 * b0 + 1]; gl.gm = mom[3 * b0 + 2];
-    gl.a0 = mom[3 * (b0 + 1)]; gl.b0 = mom[3 * (b0 + 1) + 1]; gl.g0 = mom[3 * (b0 + 1) + 2];
-    gl.ap = mom[3 * (b0 + 2)]; gl.bp = mom[3 * (b0 + 2) + 1]; gl.gp = mom[3 * (b0 + 2) + 2];
-    gr.am = gl.a0; gr.bm = gl.b0; gr.gm = gl.g0;
-    gr.a0 = gl.ap; gr.b0 = gl.bp; gr.g0 = gl.gp;
-    gr.ap = mom[3 * (b0 + 3)]; gr.bp = mom[3 * (b0 + 3) + 1]; gr.gp = mom[3 * (b0 + 3) + 2];
-    gl.xlo = pb.startx[D] + (fd - 1) * pb.dx[D];
-    gl.xhi = pb.startx[D] + fd * pb.dx[D];
-    gr.xlo = pb.startx[D] + fd * pb.dx[D];
-    gr.xhi = pb.startx[D] + (fd + 1) * pb.dx[D];
-    recon_geo_x64(gl);
-    recon_geo_x64(gr);
-    double Lq[8], Rq[8];
-    if (polar) {
-#pragma unroll
-      for (int v = 0; v < 8; v++) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 8; v++) {
-        if (!TAB && (v == 5 || v == 7)) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
-        double al, ar, al2, ar2;
-        para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
-        para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
-        Lq[v] = ar;
-        Rq[v] = al2;
-      }
-    }
-    if (polar) {
-      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
-      polar_rotate(pb, rc, fk, vl, vr);
-      for (int d = 0; d < 3; d++) { Lq[2 + d] = vl[d]; Rq[2 + d] = vr[d]; }
-    }
-    PState<double, 3> L, Rt;
-    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
-    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
-    Rt.G = Rq[7];
-    if (D == 0 || !hlle) isshock = hllc_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
-    else hlle_x64<3, false, TAB>(L, Rt, fg, D, F, sig);
-  } else {
-    // FP32-bulk: reconstruction and fluxes per unit area in FP32
-    const float *rcf = pb.rc[D];
-    const int b0 = fd - 1 + MDS_NG;
-    ReconGeo32 gl, gr;
-    const float *cl = rcf + (size_t)b0 * MDS_RC_COUNT, *cr = rcf + (size_t)(b0 + 1) * MDS_RC_COUNT;
-    gl.clm = cl[MDS_RC_CLM]; gl.clp = cl[MDS_RC_CLP]; gl.crm = cl[MDS_RC_CRM]; gl.crp = cl[MDS_RC_CRP];
-    gl.mu1 = cl[MDS_RC_MU1]; gl.kinv = cl[MDS_RC_KINV]; gl.inur = cl[MDS_RC_INUR]; gl.imu2 = cl[MDS_RC_IMU2];
-    gr.clm = cr[MDS_RC_CLM]; gr.clp = cr[MDS_RC_CLP]; gr.crm = cr[MDS_RC_CRM]; gr.crp = cr[MDS_RC_CRP];
-    gr.mu1 = cr[MDS_RC_MU1]; gr.kinv = cr[MDS_RC_KINV]; gr.inur = cr[MDS_RC_INUR]; gr.imu2 = cr[MDS_RC_IMU2];
-    float Lq[8], Rq[8];
-    if (polar) {
-#pragma unroll
-      for (int v = 0; v < 8; v++) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 8; v++) {
-        if (!TAB && (v == 5 || v == 7)) { Lq[v] = (float)q[1][v]; Rq[v] = (float)q[2][v]; continue; }
-        float al, ar, al2, ar2;
-        para_recon_f32((float)q[0][v], (float)q[1][v], (float)q[2][v], gl, al, ar);
-        para_recon_f32((float)q[1][v], (float)q[2][v], (float)q[3][v], gr, al2, ar2);
-        Lq[v] = ar;
-        Rq[v] = al2;
-      }
-    }
-    if (polar) {
-      double vl[3] = {q[1][2], q[1][3], q[1][4]}, vr[3] = {q[2][2], q[2][3], q[2][4]};
-      polar_rotate(pb, rc, fk, vl, vr);
-      for (int d = 0; d < 3; d++) { Lq[2 + d] = (float)vl[d]; Rq[2 + d] = (float)vr[d]; }
-    }
-    PState<float, 3> L, Rt;
-    L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
-    Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
-    Rt.G = Rq[7];
-    FaceGeo<float> g32;
-    g32.area = 1.0f;
-    g32.scale = (float)fg.scale;
-    for (int d = 0; d < 3; d++) g32.gcov[d] = (float)fg.gcov[d];
-    float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, s32 = 0.0f;
-    if (fg.area >= 1.0e-16) {
-      if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
-      else hlle_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
-      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
-      sig = s32;
-    } else {
-      sig = 1.0e-30;
-    }
-  }
-  (void)NQ;
+  if (D > 0) hlle = tagged(a, ci, cj, ck3) || tagged(a, ci - (D == 0), cj - (D == 1), ck3 - (D == 2));
+  const bool polar = (D == 2) && (cj == nj_of(pb, ci) - 1);
+  double sig, F[6];
+  const int isshock = face_flux<R, D, TAB, 0>(pb, lev, fd, q, polar, ck3, dk, rc, hlle, F, sig);
   if (live && active) {
     for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
     if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
     if (D == 0 && isshock) {
-      for (int s = 0; s < 3; s++) {
-        const int ii = fi - s;
-        if (ii >= 0 && ii < pb.n[0] && fj < pb.n[1] && fk < pb.n[2]) a.flags[zid3(pb, ii, fj, fk)] = 0;
-      }
+      if (DEND) tag_radial(a, fi, fj, fk);
+      else
+        for (int s = 0; s < 3; s++) {
+          const int ii = fi - s;
+          if (ii >= 0 && ii < pb.n[0] && fj < pb.n[1] && fk < pb.n[2]) a.flags[zid3(pb, ii, fj, fk)] = 0;
+        }
     }
+  }
+  if (DEND) {
+    // DJS(i) dx1 / |s| and DKS(i,j) dx2 / |s| (step.c:1263, 1419)
+    const double width = (double)(D == 0 ? 1 : (D == 1 ? 1 << lev : dk)) * pb.dx[D];
+    face_dt(a, live && active, width, sig, isshock, hlle, D);
+    return;
   }
   // CFL: min dx/|s| over faces = dx / max |s| (one division per wave)
   double smax = (live && active) ? fabs(sig) : 0.0;
@@ -377,17 +639,63 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
 }
 
 // ---------------------------------------------------------------------------
+// dendritic faces whose pencil crosses a level change: radial (D = 0) and
+// theta (D = 1) faces from a list of (i, j, k, 0), k the face's phi index
+// ---------------------------------------------------------------------------
+template <class R, int D, int TAB>
+__global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> in, const int4 *list, long long n)
+{
+  const DevProb &pb = a.pb;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < n;
+  const int4 e = list[live ? t : 0];
+  const int fi = e.x, fj = e.y, fk = e.z;
+  const bool active = a.clock->active;
+  double q[4][8];
+  int lev = 0;
+  bool hlle = false;
+  if (D == 0) {
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) radial_entry<R, TAB>(a, in, fi, fj, fk, fi - 2 + s, q[s]);
+  } else {
+    const int lj = dend_lk(pb, fi, fj), lm = dend_lk(pb, fi, fj - 1);
+    const int lf = lj < lm ? lj : lm;
+    lev = dend_lj(pb, fi);
+#pragma unroll 1
+    for (int s = 0; s < 4; s++) theta_entry<R, TAB>(a, in, fi, lf, fk, fj - 2 + s, q[s]);
+    // the zone of row j owning this sub-face, and transverse_shock (fluid.c:189-201)
+    const int kz = fk >> (lj - lf);
+    hlle = tagged(a, fi, fj, kz);
+    if (lj >= lm)
+      for (int kk = kz << (lj - lm); kk < (kz + 1) << (lj - lm); kk++) hlle = hlle || tagged(a, fi, fj - 1, kk);
+  }
+  const double *rc = rec3(pb, fi, fj);
+  double sig, F[6];
+  const int isshock = face_flux<R, D, TAB, 1>(pb, lev, D == 0 ? fi : fj, q, false, fk, 1, rc, hlle, F, sig);
+  if (live && active) {
+    const long long f = fid3(pb, D, fi, fj, fk);
+    for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
+    if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
+    if (D == 0 && isshock) tag_radial(a, fi, fj, fk);
+  }
+  face_dt(a, live && active, (double)(1 << lev) * pb.dx[D], sig, isshock, hlle, D);
+}
+
+// ---------------------------------------------------------------------------
 // zone update: sources, flux divergence, RK combination, u -> p, dt limits
 // ---------------------------------------------------------------------------
-template <class R, int TAB, int MONO>
+template <class R, int TAB, int MONO, int DEND>
 __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp, StateArrays<R> sph, int stage)
 {
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
   const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = z < (long long)n1 * n2 * n3;
-  const long long zl = live ? z : 0;
-  const int i = (int)(zl % n1), j = (int)((zl / n1) % n2), k = (int)(zl / ((long long)n1 * n2));
+  bool live = z < (long long)n1 * n2 * n3;
+  const long long zl0 = live ? z : 0;
+  const int i0 = (int)(zl0 % n1), j0 = (int)((zl0 / n1) % n2), k0 = (int)(zl0 / ((long long)n1 * n2));
+  if (DEND) live = live && dend_valid(pb, i0, j0, k0);
+  const int i = live ? i0 : 0, j = live ? j0 : 0, k = live ? k0 : 0;
+  const long long zl = zid3(pb, i, j, k);
   DevClock *ck = a.clock;
   const bool active = ck->active;
   const double dt = ck->dt;
@@ -423,16 +731,28 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
     src[4] += T[2][0] * rc[MDS_ZS_C220];
     src[4] += T[2][1] * rc[MDS_ZS_C221];
   }
-  // fluxes through the six faces
-  const long long f0 = fid3(pb, 0, i, j, k), f0p = fid3(pb, 0, i + 1, j, k);
-  const long long f1 = fid3(pb, 1, i, j, k), f1p = fid3(pb, 1, i, j + 1, k);
-  const long long f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
+  // level relations with the neighbours (dendritic grids; all 0 otherwise)
+  const int li = DEND ? dend_lj(pb, i) : 0, lo = DEND ? dend_lj(pb, i + 1) : 0;
+  const int lk0 = DEND ? dend_lk(pb, i, j) : 0;
+  const int lkp = DEND ? dend_lk(pb, i, j + 1) : 0, lkm = DEND ? dend_lk(pb, i, j - 1) : 0;
+  const int njp = li > lo ? 1 << (li - lo) : 1;
+  const int jp0 = rescale(j, li, lo);
+  const long long f0 = fid3(pb, 0, i, j, k);
   if (MONO) {
-    // gravity.c:475-486
+    // gravity.c:415-437: mass flux per area through the finer outer faces
     const double gz = a.grav[i];
     const double xl = pb.startx[0] + i * pb.dx[0];
-    double vr = a.F[0][f0p];
-    vr /= (rec3(pb, i + 1, j)[MDS_ZS_AREA0] + 1.0e-16);
+    double vr = 0.0, area = 0.0;
+    for (int jj = jp0; jj < jp0 + njp; jj++) {
+      const int l = DEND ? dend_lk(pb, i + 1, jj) : 0;
+      const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
+      const double ar = rec3(pb, i + 1, jj)[MDS_ZS_AREA0];
+      for (int kk = k * nkp; kk < k * nkp + nkp; kk++) {
+        vr += a.F[0][fid3(pb, 0, i + 1, jj, kk)];
+        area += ar + 1.0e-16;
+      }
+    }
+    vr /= area;
     src[2] += q[0] * gz;
     const double vl = a.F[0][f0] / (rc[MDS_ZS_AREA0] + 1.0e-16);
     const double A = (vr - vl) / pb.dx[0];
@@ -449,13 +769,26 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const double vol = rc[MDS_ZS_VOLUME];
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
   const int nslot = TAB ? 6 : 5;
+  const long long f1 = fid3(pb, 1, i, j, k), f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
   for (int v = 0; v < nslot; v++) {
     const size_t o0 = (size_t)v * a.nf[0], o1 = (size_t)v * a.nf[1], o2 = (size_t)v * a.nf[2];
     double divF = 0.0;
-    divF += a.F[0][o0 + f0p];
+    // outer radial faces (step.c:398-408)
+    for (int jj = jp0; jj < jp0 + njp; jj++) {
+      const int l = DEND ? dend_lk(pb, i + 1, jj) : 0;
+      const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
+      for (int kk = k * nkp; kk < k * nkp + nkp; kk++) divF += a.F[0][o0 + fid3(pb, 0, i + 1, jj, kk)];
+    }
     divF -= a.F[0][o0 + f0];
-    divF += a.F[1][o1 + f1p];
-    divF -= a.F[1][o1 + f1];
+    // outer theta face (step.c:411-423)
+    if (lkp > lk0) divF += 0.5 * a.F[1][o1 + fid3(pb, 1, i, j + 1, k)];
+    else if (lkp < lk0)
+      divF += a.F[1][o1 + fid3(pb, 1, i, j + 1, 2 * k)] + a.F[1][o1 + fid3(pb, 1, i, j + 1, 2 * k + 1)];
+    else divF += a.F[1][o1 + fid3(pb, 1, i, j + 1, k)];
+    // inner theta face (step.c:425-433)
+    if (lkm < lk0)
+      divF -= 0.5 * a.F[1][o1 + fid3(pb, 1, i, j, 2 * k)] + 0.5 * a.F[1][o1 + fid3(pb, 1, i, j, 2 * k + 1)];
+    else divF -= a.F[1][o1 + f1];
     divF += a.F[2][o2 + f2p] - a.F[2][o2 + f2];
     const double ddt = (divF / vol) - src[v];
     U[v] = ((pf * U[v]) + (phf * Uh[v])) - ((ff * dt) * ddt);
@@ -467,15 +800,20 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
     // estimate_dt zone bound (step.c:177-187) with the stage's sound speed
     double cs = TAB ? (double)((const R *)a.CS)[zl] : sqrt((gam * press) / q[0]);
     double loc = 0.0;
-    for (int d = 0; d < 3; d++) loc += (fabs(vo[d]) + (cs / rc[MDS_ZS_SCALE0 + d])) / pb.dx[d];
+    for (int d = 0; d < 3; d++) {
+      double ldx = pb.dx[d];
+      if (DEND && d == 1) ldx *= (double)(1 << li);
+      if (DEND && d == 2) ldx *= (double)(1 << lk0);
+      loc += (fabs(vo[d]) + (cs / rc[MDS_ZS_SCALE0 + d])) / ldx;
+    }
     locmax = loc;
   }
   if (live && active) {
     StateArrays<R> &out = stage == 0 ? sph : sp;
-    out.v[0][z] = (R)ro;
-    out.v[1][z] = (R)uo;
-    for (int d = 0; d < 3; d++) out.v[2 + d][z] = (R)vo[d];
-    if (TAB) out.v[5][z] = (R)(U[MDS_YI] / U[0]);
+    out.v[0][zl] = (R)ro;
+    out.v[1][zl] = (R)uo;
+    for (int d = 0; d < 3; d++) out.v[2 + d][zl] = (R)vo[d];
+    if (TAB) out.v[5][zl] = (R)(U[MDS_YI] / U[0]);
   }
   const double wl = wave_min((live && active) ? lim_src : 1.0e300);
   const double wm = wave_max((live && active) ? locmax : 0.0);
@@ -489,6 +827,40 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
 // ---------------------------------------------------------------------------
 // launcher: one RK stage
 // ---------------------------------------------------------------------------
+template <class R, int DEND>
+static void launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArrays<R> &sp,
+                         const StateArrays<R> &sph, int stage, hipStream_t st)
+{
+  const DevProb &pb = a.pb;
+  const StateArrays<R> &in = stage == 0 ? sp : sph;
+  const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
+  const dim3 blk(256);
+#define MDS_FLUX(D)                                                                                        \
+  {                                                                                                        \
+    const dim3 g((unsigned)((L.nf[D] + 255) / 256));                                                      \
+    if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND>), g, blk, 0, st, a, in);                      \
+    else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND>), g, blk, 0, st, a, in);                            \
+  }
+#define MDS_FLUX_IRR(D)                                                                                    \
+  if (DEND && L.nirr[D] > 0) {                                                                             \
+    const dim3 g((unsigned)((L.nirr[D] + 255) / 256));                                                    \
+    if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, st, a, in, L.irr[D], L.nirr[D]);   \
+    else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, st, a, in, L.irr[D], L.nirr[D]);         \
+  }
+  MDS_FLUX(0)
+  MDS_FLUX_IRR(0)
+  MDS_FLUX(1)
+  MDS_FLUX_IRR(1)
+  MDS_FLUX(2)
+#undef MDS_FLUX
+#undef MDS_FLUX_IRR
+  const dim3 grd((unsigned)((nz + 255) / 256));
+  if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  else hipLaunchKernelGGL((sph3_update<R, 0, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+}
+
 template <class R>
 static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *clock, unsigned long long *counters,
                              const DevEos &eos, const StateArrays<R> &sp, const StateArrays<R> &sph, int stage,
@@ -513,18 +885,8 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
     hipLaunchKernelGGL(sph3_shell_mass<R>, dim3(pb.n[0]), blk, 0, st, a, in, L.ms);
     hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(64), 0, st, a, L.ms, L.grav);
   }
-#define MDS_FLUX(D)                                                                              \
-  if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1>), dim3((unsigned)((L.nf[D] + 255) / 256)), blk, 0, st, a, in); \
-  else hipLaunchKernelGGL((sph3_flux<R, D, 0>), dim3((unsigned)((L.nf[D] + 255) / 256)), blk, 0, st, a, in);
-  MDS_FLUX(0)
-  MDS_FLUX(1)
-  MDS_FLUX(2)
-#undef MDS_FLUX
-  const dim3 grd((unsigned)((nz + 255) / 256));
-  if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1>), grd, blk, 0, st, a, sp, sph, stage);
-  else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0>), grd, blk, 0, st, a, sp, sph, stage);
-  else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1>), grd, blk, 0, st, a, sp, sph, stage);
-  else hipLaunchKernelGGL((sph3_update<R, 0, 0>), grd, blk, 0, st, a, sp, sph, stage);
+  if (L.dend) launch_stage<R, 1>(L, a, sp, sph, stage, st);
+  else launch_stage<R, 0>(L, a, sp, sph, stage, st);
   return hipGetLastError();
 }
 

@@ -8,7 +8,9 @@
  *              :708-820 (volume, area), :837-929 (connection), :941-1051
  *              (gcov, gcon, scale), 1D columns (x1 in [-1,1], dx2 = 2 pi),
  *              POLYTH theta map with the io.c:5014-5015 defaults,
- *              radial map r=|x| (:149-152) or r=exp(x) (metric_spherical_logr.c)
+ *              radial map r=|x| (:149-152), r=exp(x) (metric_spherical_logr.c)
+ *              or r=rtrans sinh(|x|/rtrans) (:155-161); 3D grids uniform or
+ *              dendritic (utils.c:181-590)
  *   Romberg    utils.c:14-62
  * and adds the FP32-bulk reconstruction coefficients (MDS_RC_*), evaluated in
  * long double from the same rounded FP64 volume moments.
@@ -57,11 +59,34 @@ static double romberg(double x0, double x1, integrand f, const void *ctx)
 }
 
 /* ---- spherical maps ---- */
-typedef struct { int rlog; double pnorm, pxt, palpha; } maps;
+typedef struct { int rcoord; double pnorm, pxt, palpha, rtrans; } maps;
 
-static double rx(const maps *m, double x) { return m->rlog ? exp(x) : fabs(x); }
-static double drx(const maps *m, double x) { return m->rlog ? exp(x) : 1.0; }
-static double d2rx(const maps *m, double x) { return m->rlog ? exp(x) : 0.0; }
+static double rx(const maps *m, double x)
+{
+  if (m->rcoord == MDS_RCOORD_LOG) return exp(x);
+  if (m->rcoord == MDS_RCOORD_SINH) return m->rtrans * sinh(fabs(x) / m->rtrans);
+  return fabs(x);
+}
+static double drx(const maps *m, double x)
+{
+  if (m->rcoord == MDS_RCOORD_LOG) return exp(x);
+  if (m->rcoord == MDS_RCOORD_SINH) return cosh(fabs(x) / m->rtrans);
+  return 1.0;
+}
+static double d2rx(const maps *m, double x)
+{
+  if (m->rcoord == MDS_RCOORD_LOG) return exp(x);
+  if (m->rcoord == MDS_RCOORD_SINH) return sinh(fabs(x) / m->rtrans) / m->rtrans;
+  return 0.0;
+}
+static void maps_init(maps *m, const mds_grid *g)
+{
+  m->rcoord = g->rcoord;
+  m->rtrans = g->rtrans;
+  m->pxt = 1.2;
+  m->palpha = 4;
+  m->pnorm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m->palpha + 1.0)) * (1.0 / pow(m->pxt, m->palpha))));
+}
 static double fold(double x) { if (x < -1.0) x = -2.0 - x; if (x > 1.0) x = 2.0 - x; return x; }
 static double thx(const maps *m, double x)
 {
@@ -165,10 +190,7 @@ static int build_spherical1d(mds_grid *g)
 {
   if (g->ndim != 1) return MDS_ENOTIMPL;
   maps m;
-  m.rlog = (g->rcoord == MDS_RCOORD_LOG);
-  m.pxt = 1.2;
-  m.palpha = 4;
-  m.pnorm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m.palpha + 1.0)) * (1.0 / pow(m.pxt, m.palpha))));
+  maps_init(&m, g);
   const double dphi = 2.0 * M_PI, x1lo = -1.0, x1hi = 1.0;
   int n = g->n[0], nz = n + 2 * NG;
   for (int d = 0; d < 3; d++)
@@ -241,19 +263,132 @@ static int build_spherical1d(mds_grid *g)
 static double i_xth1(double x, const void *c) { const maps *m = c; return (x * sin(thx(m, x))) * dthx(m, x); }
 static double i_xth2(double x, const void *c) { const maps *m = c; return ((x * x) * sin(thx(m, x))) * dthx(m, x); }
 
-/* 3D spherical grid on a uniform angular mesh (DJS = DKS = 1):
- * metric_spherical_general.c:592-669 (moments), :708-740 (volume),
- * :745-833 (areas, zero on the axis), :837-929 (connection with the
- * pressure-balanced Gamma^theta_thetatheta), :941-1051 (gcov, gcon, scale). */
+static int lev_of(int f)
+{
+  int l = 0;
+  while (f >>= 1) l++;
+  return l;
+}
+
+/* Dendritic levels (utils.c:181-368, checks :540-579): nj doubles with
+ * radius while the zone aspect r dtheta / dr exceeds 2^0.5 times it, from 2
+ * at the innermost shell; nk starts at 4 in the polar row of every shell and
+ * doubles row by row towards the equator on the same rule, never below the
+ * shell inside; both are mirrored into the ghost zones.  A 4:1 jump between
+ * shells (the reference exits) or a 1/2:1 jump (the reference's fix-up
+ * leaves its ghost rows unchanged) is rejected with MDS_EINVAL. */
+static int build_levels(mds_grid *g, const maps *m)
+{
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
+  const int S1 = n2 + 2 * NG;
+  const double max_log2_aspect = 0.5;
+  if (n2 < 2 || (n2 & (n2 - 1)) || n3 < 4 || (n3 & (n3 - 1))) return MDS_EINVAL;
+  if (g->startx[1] != -1.0 || fabs(g->dx[1] * n2 - 2.0) > 1.0e-14) return MDS_EINVAL;
+  int *nj = calloc(n1 + 2 * NG, sizeof(int));
+  int *nk = calloc((size_t)(n1 + 2 * NG) * S1, sizeof(int));
+  g->dj = calloc(n1 + 2 * NG, sizeof(int));
+  g->dk = calloc((size_t)(n1 + 2 * NG) * S1, sizeof(int));
+  if (!nj || !nk || !g->dj || !g->dk) {
+    free(nj);
+    free(nk);
+    return MDS_ENOMEM;
+  }
+#define NJ(i) nj[(i) + NG]
+#define NK(i, j) nk[(size_t)((i) + NG) * S1 + (j) + NG]
+#define DK(i, j) g->dk[(size_t)((i) + NG) * S1 + (j) + NG]
+  const double dth = thx(m, g->startx[1] + n2 * g->dx[1]) - thx(m, g->startx[1]);
+  NJ(0) = 2;
+  for (int i = 1; i < n1; i++) {
+    const double r0 = rx(m, g->startx[0] + (i + 0.0) * g->dx[0]);
+    const double rc = rx(m, g->startx[0] + (i + 0.5) * g->dx[0]);
+    const double r1 = rx(m, g->startx[0] + (i + 1.0) * g->dx[0]);
+    const double aspect = rc * dth / (r1 - r0);
+    int v = NJ(i - 1);
+    if (log2(aspect / v) > max_log2_aspect) v *= 2;
+    NJ(i) = v < n2 ? v : n2;
+  }
+  for (int i = -NG; i < 0; i++) NJ(i) = NJ(-i - 1);
+  for (int i = n1; i < n1 + NG; i++) NJ(i) = NJ(n1 - 1);
+  for (int i = -NG; i < n1 + NG; i++) g->dj[i + NG] = n2 / NJ(i);
+  const double dph = n3 * g->dx[2];
+  for (int i = 0; i < n1; i++) {
+    const double r0 = rx(m, g->startx[0] + (i + 0.0) * g->dx[0]);
+    const double rc = rx(m, g->startx[0] + (i + 0.5) * g->dx[0]);
+    const double r1 = rx(m, g->startx[0] + (i + 1.0) * g->dx[0]);
+    const double dr = r1 - r0;
+    NK(i, 0) = 4;
+    for (int j = 1; j < NJ(i); j++) {
+      if (j >= NJ(i) / 2) {
+        NK(i, j) = NK(i, NJ(i) - 1 - j);
+        continue;
+      }
+      const double thc = thx(m, g->startx[1] + (j + 0.5) * g->dj[i + NG] * g->dx[1]);
+      const double aspect = rc * sin(thc) * dph / dr;
+      int v = NK(i, j - 1);
+      if (log2(aspect / v) > max_log2_aspect) v *= 2;
+      if (i > 0) {
+        const int inner = NJ(i - 1) == NJ(i) ? NK(i - 1, j) : NK(i - 1, j / 2);
+        if (inner > v) v = inner;
+      }
+      NK(i, j) = v < n3 ? v : n3;
+    }
+  }
+  for (int i = -NG; i < 0; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = NK(-i - 1, j);
+  for (int i = n1; i < n1 + NG; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = NK(n1 - 1, j);
+  for (int i = -NG; i < n1 + NG; i++) {
+    for (int j = -NG; j < 0; j++) NK(i, j) = NK(i, -j - 1);
+    for (int j = NJ(i); j < NJ(i) + NG; j++) NK(i, j) = NK(i, 2 * NJ(i) - 1 - j);
+    for (int j = -NG; j < NJ(i) + NG; j++) DK(i, j) = n3 / NK(i, j);
+  }
+  int rc = 0;
+  for (int i = 0; i < n1 && !rc; i++) {
+    if (g->dj[i + NG] != 2 * g->dj[i + 1 + NG]) continue;
+    for (int j = 0; j < NJ(i); j++) {
+      const int a = DK(i, j) / DK(i + 1, 2 * j), b = DK(i, j) / DK(i + 1, 2 * j + 1);
+      if (a == 4 || b == 4 || a == 0 || b == 0) rc = MDS_EINVAL;
+    }
+  }
+  g->maxjlev = lev_of(g->dj[NG]);
+  g->maxklev = 0;
+  for (int i = -NG; i < n1 + NG; i++)
+    for (int j = -NG; j < NJ(i) + NG; j++) {
+      const int l = lev_of(DK(i, j));
+      if (l > g->maxklev) g->maxklev = l;
+    }
+#undef NJ
+#undef NK
+#undef DK
+  free(nj);
+  free(nk);
+  return rc;
+}
+
+static int gdj(const mds_grid *g, int i) { return g->dj ? g->dj[i + NG] : 1; }
+static int gdk(const mds_grid *g, int i, int j)
+{
+  return g->dk ? g->dk[(size_t)(i + NG) * (g->n[1] + 2 * NG) + j + NG] : 1;
+}
+
+/* 3D spherical grid, uniform or dendritic: metric_spherical_general.c:592-669
+ * (moments per level), :708-740 (volume), :745-833 (areas, zero on the
+ * axis), :837-929 (connection with the pressure-balanced
+ * Gamma^theta_thetatheta; across a k-level change dA is the sum of the two
+ * finer faces or half the coarser one), :941-1051 (gcov, gcon, scale).
+ * Records are per (i, level-local j): nothing depends on phi, and a row's
+ * phi width enters through its dk. */
 static int build_spherical3d(mds_grid *g)
 {
   maps m;
-  m.rlog = (g->rcoord == MDS_RCOORD_LOG);
-  m.pxt = 1.2;
-  m.palpha = 4;
-  m.pnorm = ((0.5 * M_PI) * 1.0) / (1.0 + ((1.0 / (m.palpha + 1.0)) * (1.0 / pow(m.pxt, m.palpha))));
+  maps_init(&m, g);
   const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
-  const int N1 = n1 + 2 * NG, N2 = n2 + 2 * NG;
+  const int N1 = n1 + 2 * NG, N2 = n2 + 2 * NG, N3 = n3 + 2 * NG;
+  g->maxjlev = g->maxklev = 0;
+  if (g->dendritic) {
+    int rc = build_levels(g, &m);
+    if (rc) return rc;
+  }
   for (int d = 0; d < 3; d++)
     if (alloc_dir(g, d)) return MDS_ENOMEM;
   for (int i = -NG; i < n1 + NG; i++) {
@@ -262,20 +397,47 @@ static int build_spherical3d(mds_grid *g)
     g->beta[0][i + NG] = romberg(x0, x1, i_xr1, &m);
     g->alpha[0][i + NG] = romberg(x0, x1, i_xr2, &m);
   }
-  for (int j = -NG; j < n2 + NG; j++) {
-    double x0 = g->startx[1] + j * 1 * g->dx[1], x1 = g->startx[1] + (j + 1) * 1 * g->dx[1];
-    g->Gamma[1][j + NG] = romberg(x0, x1, i_sdth, &m);
-    g->beta[1][j + NG] = romberg(x0, x1, i_xth1, &m);
-    g->alpha[1][j + NG] = romberg(x0, x1, i_xth2, &m);
+  recon_coeffs(g->alpha[0], g->beta[0], g->Gamma[0], g->startx[0], g->dx[0], N1, g->rcoef[0]);
+  const int nlev[2] = {g->maxjlev + 1, g->maxklev + 1}, NN[2] = {N2, N3};
+  for (int t = 0; t < 2; t++) {
+    const size_t sz = (size_t)nlev[t] * NN[t];
+    g->lalpha[t] = calloc(sz, sizeof(double));
+    g->lbeta[t] = calloc(sz, sizeof(double));
+    g->lGamma[t] = calloc(sz, sizeof(double));
+    g->lrcoef[t] = calloc(sz * MDS_RC_COUNT, sizeof(double));
+    if (!g->lalpha[t] || !g->lbeta[t] || !g->lGamma[t] || !g->lrcoef[t]) return MDS_ENOMEM;
   }
-  for (int k = -NG; k < n3 + NG; k++) {
-    double x0 = g->startx[2] + k * 1 * g->dx[2], x1 = g->startx[2] + (k + 1) * 1 * g->dx[2];
-    g->Gamma[2][k + NG] = 1 * g->dx[2];
-    g->beta[2][k + NG] = ((x1 * x1) - (x0 * x0)) / 2.0;
-    g->alpha[2][k + NG] = ((x1 * x1 * x1) - (x0 * x0 * x0)) / 3.0;
+  for (int lev = 0; lev < nlev[0]; lev++) {
+    const int sc = 1 << lev;
+    double *A = g->lalpha[0] + (size_t)lev * N2, *B = g->lbeta[0] + (size_t)lev * N2, *G = g->lGamma[0] + (size_t)lev * N2;
+    for (int j = -NG; j < n2 / sc + NG; j++) {
+      double x0 = g->startx[1] + j * sc * g->dx[1], x1 = g->startx[1] + (j + 1) * sc * g->dx[1];
+      G[j + NG] = romberg(x0, x1, i_sdth, &m);
+      B[j + NG] = romberg(x0, x1, i_xth1, &m);
+      A[j + NG] = romberg(x0, x1, i_xth2, &m);
+    }
+    for (int j = n2 / sc + NG; j < n2 + NG; j++) G[j + NG] = 1.0;
+    recon_coeffs(A, B, G, g->startx[1], sc * g->dx[1], n2 / sc + 2 * NG, g->lrcoef[0] + (size_t)lev * N2 * MDS_RC_COUNT);
   }
-  for (int d = 0; d < 3; d++)
-    recon_coeffs(g->alpha[d], g->beta[d], g->Gamma[d], g->startx[d], g->dx[d], g->n[d] + 2 * NG, g->rcoef[d]);
+  for (int lev = 0; lev < nlev[1]; lev++) {
+    const int sc = 1 << lev;
+    double *A = g->lalpha[1] + (size_t)lev * N3, *B = g->lbeta[1] + (size_t)lev * N3, *G = g->lGamma[1] + (size_t)lev * N3;
+    for (int k = -NG; k < n3 / sc + NG; k++) {
+      double x0 = g->startx[2] + k * sc * g->dx[2], x1 = g->startx[2] + (k + 1) * sc * g->dx[2];
+      G[k + NG] = sc * g->dx[2];
+      B[k + NG] = ((x1 * x1) - (x0 * x0)) / 2.0;
+      A[k + NG] = ((x1 * x1 * x1) - (x0 * x0 * x0)) / 3.0;
+    }
+    for (int k = n3 / sc + NG; k < n3 + NG; k++) G[k + NG] = 1.0;
+    recon_coeffs(A, B, G, g->startx[2], sc * g->dx[2], n3 / sc + 2 * NG, g->lrcoef[1] + (size_t)lev * N3 * MDS_RC_COUNT);
+  }
+  for (int t = 0; t < 2; t++) {
+    const int d = t + 1;
+    memcpy(g->alpha[d], g->lalpha[t], NN[t] * sizeof(double));
+    memcpy(g->beta[d], g->lbeta[t], NN[t] * sizeof(double));
+    memcpy(g->Gamma[d], g->lGamma[t], NN[t] * sizeof(double));
+    memcpy(g->rcoef[d], g->lrcoef[t], (size_t)NN[t] * MDS_RC_COUNT * sizeof(double));
+  }
 
   g->zone = calloc((size_t)N1 * N2 * MDS_ZS_COUNT, sizeof(double));
   if (!g->zone) return MDS_ENOMEM;
@@ -287,20 +449,23 @@ static int build_spherical3d(mds_grid *g)
     double vrf = r2dr;
     double r2dr3 = romberg(xl, xh, i_r2dr3, &m);
     double r5 = 0.2 * (pow(rx(&m, xh), 5) - pow(rx(&m, xl), 5));
-    for (int j = -NG; j < n2 + NG; j++) {
-      double tl = g->startx[1] + j * 1 * g->dx[1], th = g->startx[1] + (j + 1) * 1 * g->dx[1];
+    const int dj = gdj(g, i), nj = n2 / dj;
+    const double *lB = g->lbeta[0] + (size_t)lev_of(dj) * N2, *lG = g->lGamma[0] + (size_t)lev_of(dj) * N2;
+    for (int j = -NG; j < nj + NG; j++) {
+      double tl = g->startx[1] + j * dj * g->dx[1], th = g->startx[1] + (j + 1) * dj * g->dx[1];
       double sth = romberg(tl, th, i_sdth, &m);
       double vthf = sth;
       double th_s3 = romberg(tl, th, i_sdth3, &m);
       double th_s3c = romberg(tl, th, i_s3dth, &m);
+      const int dk = gdk(g, i, j);
       double *z = &g->zone[((size_t)(j + NG) * N1 + (i + NG)) * MDS_ZS_COUNT];
-      z[MDS_ZS_VOLUME] = ((r2dr * sth) * 1) * dph;
-      if (i >= 0 && i <= n1 && j >= 0 && j <= n2) {
-        double dph_int = 1 * dph;
+      z[MDS_ZS_VOLUME] = ((r2dr * sth) * dk) * dph;
+      if (i >= 0 && i <= n1 && j >= 0 && j <= nj) {
+        double dph_int = dk * dph;
         z[MDS_ZS_AREA0] = (i_r2dr(xl, &m) * sth) * dph_int;
         z[MDS_ZS_AREA1] = (r2dr * i_sdth(tl, &m)) * dph_int;
         z[MDS_ZS_AREA2] = r2dr * sth;
-        if ((j == 0 && g->startx[1] == -1) || (j == n2 && top_axis)) z[MDS_ZS_AREA1] = 0.0;
+        if ((j == 0 && g->startx[1] == -1) || (j == nj && top_axis)) z[MDS_ZS_AREA1] = 0.0;
       }
       double gc[3], gf0[3], gf1[3];
       gc[0] = r2dr3 / vrf;
@@ -321,7 +486,7 @@ static int build_spherical3d(mds_grid *g)
       }
       z[MDS_ZS_SCALEF0] = sqrt(gf0[0]);
       z[MDS_ZS_SCALEF1] = sqrt(gf1[1]);
-      const double thc = thx(&m, g->beta[1][j + NG] / g->Gamma[1][j + NG]);
+      const double thc = thx(&m, lB[j + NG] / lG[j + NG]);
       z[MDS_ZS_STH] = sin(thc);
       z[MDS_ZS_CTH] = cos(thc);
     }
@@ -333,8 +498,9 @@ static int build_spherical3d(mds_grid *g)
     double r3i = romberg(xl, xh, i_r3, &m);
     double rdr2 = romberg(xl, xh, i_rdr2, &m);
     double vr = romberg(xl, xl + g->dx[0], i_r2dr, &m);
-    for (int j = 0; j < n2; j++) {
-      double tl = g->startx[1] + j * 1 * g->dx[1], th = g->startx[1] + (j + 1) * 1 * g->dx[1];
+    const int dj = gdj(g, i);
+    for (int j = 0; j < n2 / dj; j++) {
+      double tl = g->startx[1] + j * dj * g->dx[1], th = g->startx[1] + (j + 1) * dj * g->dx[1];
       double vthf = romberg(tl, th, i_sdth, &m);
       double th_s3 = romberg(tl, th, i_sdth3, &m);
       double th_cd2 = romberg(tl, th, i_cdth2, &m);
@@ -352,7 +518,11 @@ static int build_spherical3d(mds_grid *g)
       z[MDS_ZS_C221] = c212;
       z[MDS_ZS_C022] = ((-c101) * z[MDS_ZS_GCOV0 + 2]) / z[MDS_ZS_GCOV0 + 0];
       z[MDS_ZS_C122] = ((-c212) * z[MDS_ZS_GCOV0 + 2]) / z[MDS_ZS_GCOV0 + 1];
-      double dA = zu[MDS_ZS_AREA1];
+      const int dk0 = gdk(g, i, j), dk1 = gdk(g, i, j + 1);
+      double dA;
+      if (dk1 < dk0) dA = zu[MDS_ZS_AREA1] + zu[MDS_ZS_AREA1];
+      else if (dk1 > dk0) dA = 0.5 * zu[MDS_ZS_AREA1];
+      else dA = zu[MDS_ZS_AREA1];
       dA -= z[MDS_ZS_AREA1];
       z[MDS_ZS_C111] = dA / z[MDS_ZS_VOLUME] - c212;
     }
@@ -371,6 +541,11 @@ int mds_grid_build(mds_grid *g)
     g->alpha[d] = g->beta[d] = g->Gamma[d] = g->rcoef[d] = NULL;
   }
   g->zone = NULL;
+  g->dj = g->dk = NULL;
+  g->maxjlev = g->maxklev = 0;
+  for (int t = 0; t < 2; t++) g->lalpha[t] = g->lbeta[t] = g->lGamma[t] = g->lrcoef[t] = NULL;
+  if (g->geom == MDS_GEOM_SPHERICAL && g->rcoord == MDS_RCOORD_SINH && !(g->rtrans > 0.0)) return MDS_EINVAL;
+  if (g->dendritic && !(g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3)) return MDS_EINVAL;
   int rc;
   if (g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3) rc = build_spherical3d(g);
   else if (g->geom == MDS_GEOM_SPHERICAL) rc = build_spherical1d(g);
@@ -400,4 +575,11 @@ void mds_grid_free(mds_grid *g)
   }
   free(g->zone);
   g->zone = NULL;
+  free(g->dj);
+  free(g->dk);
+  g->dj = g->dk = NULL;
+  for (int t = 0; t < 2; t++) {
+    free(g->lalpha[t]); free(g->lbeta[t]); free(g->lGamma[t]); free(g->lrcoef[t]);
+    g->lalpha[t] = g->lbeta[t] = g->lGamma[t] = g->lrcoef[t] = NULL;
+  }
 }

@@ -108,6 +108,14 @@ int mds_init_rt(const mds_grid *g, double gam, double p0, double gz, double rho_
   return MDS_OK;
 }
 
+/* r(x) of the grid's radial map (metric_spherical_general.c:149-161, metric_spherical_logr.c) */
+static double radius_of(const mds_grid *g, double x)
+{
+  if (g->rcoord == MDS_RCOORD_LOG) return exp(x);
+  if (g->rcoord == MDS_RCOORD_SINH) return g->rtrans * sinh(fabs(x) / g->rtrans);
+  return fabs(x);
+}
+
 int mds_init_sph_riemann(const mds_grid *g, double gam, double rho_in, double p_in, double r0,
                          double frho, double fp, double *prims)
 {
@@ -115,7 +123,7 @@ int mds_init_sph_riemann(const mds_grid *g, double gam, double rho_in, double p_
   size_t nz = nzones(g);
   for (int i = 0; i < g->n[0]; i++) {
     double xc = g->startx[0] + (i + 0.5) * g->dx[0];
-    double r = (g->rcoord == MDS_RCOORD_LOG) ? exp(xc) : fabs(xc);
+    double r = radius_of(g, xc);
     int inner = r < r0;
     double rho = inner ? rho_in : rho_in / frho;
     double p = inner ? p_in : p_in / fp;
@@ -228,7 +236,7 @@ int mds_init_polytrope(const mds_grid *g, double n_index, double rho_c, double K
 #pragma omp parallel for schedule(dynamic, 8)
   for (int i = 0; i < n1; i++) {
     const double xc = g->beta[0][i + MDS_NG] / g->Gamma[0][i + MDS_NG];
-    const double r = g->rcoord == MDS_RCOORD_LOG ? exp(xc) : fabs(xc);
+    const double r = radius_of(g, xc);
     const double xi = r / alpha;
     double rho;
     if (xi < xi1) {

@@ -16,7 +16,7 @@ NG = 2
 FP64, FP32_BULK = 0, 1
 BC_OUTFLOW, BC_REFLECT, BC_PERIODIC = 0, 1, 3
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
-RCOORD_UNIFORM, RCOORD_LOG = 0, 1
+RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
 RC_COUNT = 8
 TILE_ROWS2D = 0
 TILE_TY3D = 1
@@ -47,7 +47,17 @@ class Grid(ctypes.Structure):
                 ("rcoef", _dp * 3),
                 ("volume", ctypes.c_double),
                 ("area", ctypes.c_double * 3),
-                ("zone", _dp)]
+                ("zone", _dp),
+                ("rtrans", ctypes.c_double),
+                ("dendritic", ctypes.c_int),
+                ("dj", ctypes.POINTER(ctypes.c_int)),
+                ("dk", ctypes.POINTER(ctypes.c_int)),
+                ("maxjlev", ctypes.c_int),
+                ("maxklev", ctypes.c_int),
+                ("lalpha", _dp * 2),
+                ("lbeta", _dp * 2),
+                ("lGamma", _dp * 2),
+                ("lrcoef", _dp * 2)]
 
 
 class EosTableDesc(ctypes.Structure):
@@ -152,8 +162,10 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def _grid_desc(ndim, n, startx, dx, geom, rcoord, g=None):
+def _grid_desc(ndim, n, startx, dx, geom, rcoord, g=None, rtrans=0.0, dendritic=False):
     g = g if g is not None else Grid()
+    g.rtrans = float(rtrans)
+    g.dendritic = int(bool(dendritic))
     g.ndim = ndim
     n = list(n) + [1] * (3 - len(n))
     for d in range(3):
@@ -168,11 +180,12 @@ def _grid_desc(ndim, n, startx, dx, geom, rcoord, g=None):
 class GridBuilder:
     """Owns an mds_grid built by mds_grid_build (the reference geometry)."""
 
-    def __init__(self, ndim, n, startx, dx, geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, _built=None):
+    def __init__(self, ndim, n, startx, dx, geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, _built=None,
+                 rtrans=0.0, dendritic=False):
         if _built is not None:
             self.g = _built
         else:
-            self.g = _grid_desc(ndim, n, startx, dx, geom, rcoord)
+            self.g = _grid_desc(ndim, n, startx, dx, geom, rcoord, rtrans=rtrans, dendritic=dendritic)
             rc = lib().mds_grid_build(ctypes.byref(self.g))
             if rc:
                 raise RuntimeError(f"mds_grid_build failed ({rc})")
@@ -217,6 +230,42 @@ class GridBuilder:
             nj = self.n[1] + 2 * NG
             return np.ctypeslib.as_array(self.g.zone, shape=(nj * nz * ZS_COUNT,)).reshape(nj, nz, ZS_COUNT).copy()
         return np.ctypeslib.as_array(self.g.zone, shape=(nz * ZG_COUNT,)).reshape(nz, ZG_COUNT).copy()
+
+    def levels(self):
+        """Dendritic grids: (nj[n1], nk[n1, n2] with 0 beyond nj[i]); None on uniform grids."""
+        if not self.g.dj:
+            return None
+        n1, n2, n3 = self.n
+        dj = np.ctypeslib.as_array(self.g.dj, shape=(n1 + 2 * NG,))[NG:NG + n1]
+        dk = np.ctypeslib.as_array(self.g.dk, shape=((n1 + 2 * NG) * (n2 + 2 * NG),)).reshape(n1 + 2 * NG, n2 + 2 * NG)
+        nj = (n2 // dj).astype(np.int32)
+        nk = np.zeros((n1, n2), dtype=np.int32)
+        for i in range(n1):
+            nk[i, :nj[i]] = n3 // dk[i + NG, NG:NG + nj[i]]
+        return nj, nk
+
+    def valid_mask(self):
+        """Boolean [n3, n2, n1]: which (i, j, k) positions are zones (all on uniform grids)."""
+        n1, n2, n3 = self.n
+        lv = self.levels()
+        if lv is None:
+            return np.ones((n3, n2, n1), bool)
+        nj, nk = lv
+        m = np.zeros((n3, n2, n1), bool)
+        for i in range(n1):
+            for j in range(nj[i]):
+                m[:nk[i, j], j, i] = True
+        return m
+
+    def level_moments(self, d, lev):
+        """theta (d=1) or phi (d=2) [alpha, beta, Gamma] of refinement level lev (length n + 2 NG)."""
+        t = d - 1
+        maxl = self.g.maxjlev if t == 0 else self.g.maxklev
+        if not self.g.lalpha[t] or lev > maxl:
+            return None
+        nn = self.n[d] + 2 * NG
+        return [np.ctypeslib.as_array(getattr(self.g, f)[t], shape=((maxl + 1) * nn,))[lev * nn:(lev + 1) * nn].copy()
+                for f in ("lalpha", "lbeta", "lGamma")]
 
 
 def run_params(cfl=0.5, tmax=1.0e300, **kw):

@@ -77,10 +77,38 @@ SPH3D_BC_LO = [mds.BC_REFLECT, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC]
 SPH3D_BC_HI = [mds.BC_OUTFLOW, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC]
 
 
-def star3d(n1=640, n2=128, n3=256, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, monopole=True, **kw):
+def rtrans_for(dr0, n1, rout):
+    """Scale of the sinh radial map r = rtrans sinh(x / rtrans) with central
+    spacing dr/dx = 1 over x in [0, n1 dr0] and r(n1 dr0) = rout (bisection)."""
+    lo, hi = dr0 * 1.0e-2, rout * 1.0e2
+    for _ in range(300):
+        rt = math.sqrt(lo * hi)
+        if rt * math.sinh(n1 * dr0 / rt) > rout:
+            lo = rt
+        else:
+            hi = rt
+    return rt
+
+
+def dend3d_grid(n1, n2, n3, dr0=5.0e4, rout=2.0e9, dendritic=True):
+    """3D spherical dendritic grid from the origin: sinh radial map with
+    central spacing dr0 reaching rout, theta over x1 in [-1, 1] (POLYTH map),
+    phi over [0, 2 pi); theta zones coarsened near the origin, phi zones near
+    the poles (utils.c:181-590)."""
+    return mds.GridBuilder(3, [int(n1), int(n2), int(n3)], [0.0, -1.0, 0.0],
+                           [float(dr0), 2.0 / int(n2), 2.0 * math.pi / int(n3)],
+                           geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_SINH,
+                           rtrans=rtrans_for(dr0, n1, rout), dendritic=dendritic)
+
+
+def star3d(n1=640, n2=128, n3=256, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, monopole=True,
+           dendritic=False, **kw):
     """The polytropic star (n=3, rho_c = 1e10) on a 3D spherical grid with
     monopole gravity; reflecting inner radius, outflow outer radius, theta
-    across the poles, phi periodic.  Uniform angular mesh (not dendritic)."""
-    g = sph3d_grid(n1, n2, n3)
+    across the poles, phi periodic.  dendritic=False: log r over [1e5, 2e9]
+    on a uniform angular mesh; True: the dendritic grid of dend3d_grid (the
+    log r map cannot carry one: its zone aspect is constant, so the level
+    counts double in every inner shell and the 2:1 rule fails)."""
+    g = dend3d_grid(n1, n2, n3) if dendritic else sph3d_grid(n1, n2, n3)
     prob = dict(grid=g, bc_lo=list(SPH3D_BC_LO), bc_hi=list(SPH3D_BC_HI), gam=gam, monopole=monopole, eos=table)
     return g, prob, mds.init_polytrope(g, gam=gam, table=table, sigma=sigma, seed=seed, **kw)

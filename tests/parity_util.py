@@ -15,7 +15,8 @@ def oracle_config(prob, rp_cfl=0.5, **kw):
         nd, g.n[:nd], [g.g.startx[d] for d in range(nd)], [g.g.dx[d] for d in range(nd)],
         list(prob["bc_lo"]), list(prob["bc_hi"]), prob["gam"], cfl=rp_cfl,
         geom=po.GEOM_SPHERICAL if g.g.geom == mds.GEOM_SPHERICAL else po.GEOM_CARTESIAN,
-        rcoord=g.g.rcoord, g0=tuple(prob.get("g0", (0.0, 0.0, 0.0))), **kw)
+        rcoord=g.g.rcoord, g0=tuple(prob.get("g0", (0.0, 0.0, 0.0))), rtrans=g.g.rtrans,
+        dendritic=bool(g.g.dendritic), **kw)
 
 
 def oracle_for(prob, rp_cfl=0.5, fast=False, **kw):

@@ -157,3 +157,191 @@ def test_blast_conserves_mass_and_energy():
     out = sim.get_prims()
     m = valid_mask(nj, nk, n)
     assert np.all(np.isfinite(out[:, m])) and np.all(out[0][m] > 0)
+
+
+def _host_grid(cfg):
+    from medusa_amd import mds
+    return mds.GridBuilder(3, [cfg.n[d] for d in range(3)], [cfg.startx[d] for d in range(3)],
+                           [cfg.dx[d] for d in range(3)], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_SINH,
+                           rtrans=cfg.rtrans, dendritic=True)
+
+
+@pytest.mark.parametrize("n", GRIDS)
+def test_host_levels_and_geometry_match_oracle_bitwise(n):
+    from medusa_amd import mds
+    cfg = dend_config(n)
+    sim = po.Sim(cfg)
+    g = _host_grid(cfg)
+    onj, onk, omj, omk = sim.levels()
+    hnj, hnk = g.levels()
+    assert np.array_equal(onj, hnj) and np.array_equal(onk, hnk)
+    assert (g.g.maxjlev, g.g.maxklev) == (omj, omk)
+    for d, maxl in ((1, omj), (2, omk)):
+        for lev in range(maxl + 1):
+            om, hm = sim.level_moments(d, lev), g.level_moments(d, lev)
+            nn = n[d] // (1 << lev) + 4
+            for q in range(3):
+                assert np.array_equal(om[q][:nn], hm[q][:nn]), (d, lev, q)
+    for q, (a, b) in enumerate(zip(sim.moments(0), g.moments(0))):
+        assert np.array_equal(a, b)
+    z = g.zone()
+    arr = sim.geometry()
+    N0, N1 = n[0] + 4, n[1] + 4
+    names = {"C000": (0, 0, 0), "C011": (0, 1, 1), "C101": (1, 0, 1), "C110": (1, 1, 0),
+             "C202": (2, 0, 2), "C220": (2, 2, 0), "C212": (2, 1, 2), "C221": (2, 2, 1),
+             "C022": (0, 2, 2), "C122": (1, 2, 2), "C111": (1, 1, 1)}
+    for i in range(-2, n[0] + 2):
+        nj = onj[min(max(i, -1 - i if i < 0 else i), n[0] - 1)] if i >= 0 else onj[-1 - i]
+        for j in range(-2, nj + 2):
+            rec = arr[(i + 2) + N0 * ((j + 2) + N1 * 2)]
+            h = z[j + 2, i + 2]
+            assert h[mds.ZS_VOLUME] == rec[po.GEOM_VOLUME], (i, j)
+            for a in range(3):
+                assert h[mds.ZS_AREA0 + a] == rec[po.GEOM_AREA + a], (i, j, a)
+                assert h[mds.ZS_GCOV0 + a] == rec[po.GEOM_GCOV + a]
+                assert h[mds.ZS_GCOVF0 + a] == rec[po.GEOM_GCOV + 3 + a]
+                assert h[mds.ZS_GCOVF1 + a] == rec[po.GEOM_GCOV + 6 + a]
+                assert h[mds.ZS_GCON + a] == rec[po.GEOM_GCON + a]
+                assert h[mds.ZS_SCALE0 + a] == rec[po.GEOM_SCALE + a]
+            assert h[mds.ZS_SCALEF0] == rec[po.GEOM_SCALE + 3]
+            assert h[mds.ZS_SCALEF1] == rec[po.GEOM_SCALE + 7]
+            if 0 <= i < n[0] and 0 <= j < nj:
+                conn = rec[po.GEOM_CONN:po.GEOM_CONN + 27].reshape(3, 3, 3)
+                for nm, idx in names.items():
+                    assert h[getattr(mds, "ZS_" + nm)] == conn[idx], (i, j, nm)
+
+
+# ---------------------------------------------------------------------------
+# device path against the oracle
+# ---------------------------------------------------------------------------
+def _dend_problem(n, gam=5.0 / 3.0, rout=400.0):
+    from medusa_amd import mds, problems
+    g = problems.dend3d_grid(*n, dr0=1.0, rout=rout)
+    prob = dict(grid=g, bc_lo=list(problems.SPH3D_BC_LO), bc_hi=list(problems.SPH3D_BC_HI), gam=gam)
+    return g, prob
+
+
+def _oracle(prob, **kw):
+    from parity_util import oracle_config
+    return po.Sim(oracle_config(prob, 0.5, **kw))
+
+
+def _weights(g, shape):
+    from medusa_amd import mds
+    vol = g.zone()[2:-2, 2:-2, mds.ZS_VOLUME]
+    return np.broadcast_to(vol[None], shape) * g.valid_mask()
+
+
+def _rel(a, b, w):
+    return float(np.sum(np.abs(a - b) * w) / np.sum(np.abs(b) * w))
+
+
+def _blast_prims(n, g):
+    prims = np.zeros((6, n[2], n[1], n[0]))
+    rng = np.random.default_rng(7)
+    prims[0] = 1.0 + 0.02 * rng.uniform(-1, 1, prims[0].shape)
+    prims[1] = 1.5
+    # over-pressured shells crossing the theta and phi level changes, and a
+    # blob in the last theta row (polar-axis phi faces)
+    prims[1][:, :, 2:5] *= 10.0
+    nj, nk = g.levels()
+    i0 = n[0] // 2
+    prims[1][:2, nj[i0] - 2:nj[i0], i0:i0 + 2] *= 20.0
+    prims[5] = 0.5
+    return prims * g.valid_mask()[None]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dendritic_blast_matches_oracle(precision):
+    from medusa_amd import mds
+    n = (24, 16, 32)
+    g, prob = _dend_problem(n)
+    prims = _blast_prims(n, g)
+    print(f"[dend] blast {n} precision={precision}, 12 steps", flush=True)
+    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    m0 = ctx.totals()
+    ctx.run(12)
+    dev = ctx.download()
+    m1 = ctx.totals()
+    sim = _oracle(prob, nvars=6)
+    sim.set_prims(prims)
+    sim.run(max_steps=12)
+    ref = sim.get_prims()
+    w = _weights(g, ref[0].shape)
+    errs = {"rho": _rel(dev[0], ref[0], w), "u": _rel(dev[1], ref[1], w)}
+    cs = np.sqrt(5.0 / 3.0 * (2.0 / 3.0) * np.where(w > 0, ref[1] / np.where(ref[0] > 0, ref[0], 1.0), 0.0))
+    sc = g.zone()[2:-2, 2:-2, mds.ZS_SCALE0:mds.ZS_SCALE0 + 3]
+    for d in range(3):
+        errs[f"v{d + 1}/cs"] = float(np.sum(np.abs(dev[2 + d] - ref[2 + d]) * sc[None, :, :, d] * w)
+                                     / np.sum(cs * w))
+    t, dt, ist = ctx.clock()
+    hl = ctx.counters()
+    print(f"[dend]   L1 {errs}; t {t!r} vs {sim.t!r}; shocks/HLLE {hl} vs {sim.shock_faces()}/{sim.hlle_faces()};"
+          f" mass drift {m1[0] / m0[0] - 1:.2e}", flush=True)
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+    assert abs(m1[0] / m0[0] - 1) < (1e-13 if precision == 0 else 1e-6)
+    if precision == 0:
+        assert ist == sim.istep and abs(t / sim.t - 1) < 1e-13
+        assert hl[0] == sim.shock_faces() and hl[1] == sim.hlle_faces()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_table", [False, True])
+def test_dendritic_star_fp64_matches_oracle(use_table):
+    from medusa_amd import mds, problems
+    T = mds.EosTable() if use_table else None
+    n = (32, 16, 32)
+    g = problems.dend3d_grid(*n, dr0=2.0e6, rout=2.0e9)
+    prob = dict(grid=g, bc_lo=list(problems.SPH3D_BC_LO), bc_hi=list(problems.SPH3D_BC_HI), gam=4.0 / 3.0,
+                monopole=True, eos=T)
+    prims = mds.init_polytrope(g, table=T, sigma=0.01) * g.valid_mask()[None]
+    print(f"[dend] star {n} table={use_table} FP64, 10 steps", flush=True)
+    ctx = mds.Context([prob], precision=mds.FP64, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    ctx.run(10, graph=True)
+    dev = ctx.download()
+    kw = dict(nvars=6, monopole=True)
+    if T is not None:
+        t = T.t
+        kw["eos_tab"] = po.eos_tab(dict(nr=t.nr, nt=t.nt, ny=t.ny, lr0=t.lr0, lr1=t.lr1, lt0=t.lt0, lt1=t.lt1,
+                                        ye0=t.ye0, ye1=t.ye1), T.data)
+    sim = _oracle(prob, **kw)
+    sim.set_prims(prims)
+    sim.run(max_steps=10)
+    ref = sim.get_prims()
+    w = _weights(g, ref[0].shape)
+    errs = {"rho": _rel(dev[0], ref[0], w), "u": _rel(dev[1], ref[1], w)}
+    if use_table:
+        errs["ye"] = _rel(dev[5], ref[5], w)
+    print(f"[dend]   L1 {errs}", flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-12, (k, v)
+
+
+@pytest.mark.gpu
+def test_dendritic_uniform_gas_stays_at_rest_on_device():
+    from medusa_amd import mds
+    n = (24, 16, 32)
+    g, prob = _dend_problem(n)
+    prims = np.zeros((5, n[2], n[1], n[0]))
+    prims[0] = 1.0
+    prims[1] = 1.5
+    prims *= g.valid_mask()[None]
+    for precision in (0, 1):
+        ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+        ctx.upload(prims)
+        ctx.update_eos()
+        ctx.run(10)
+        out = ctx.download()
+        cs = math.sqrt(5.0 / 3.0 * (2.0 / 3.0) * 1.5)
+        sc = g.zone()[2:-2, 2:-2, mds.ZS_SCALE0:mds.ZS_SCALE0 + 3]
+        m = g.valid_mask()
+        vmax = max(float(np.max((np.abs(out[2 + d]) * sc[None, :, :, d])[m])) for d in range(3))
+        print(f"[dend] uniform gas precision={precision}: max |v|/cs {vmax / cs:.2e}", flush=True)
+        assert vmax / cs < (1e-12 if precision == 0 else 1e-5)
