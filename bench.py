@@ -22,6 +22,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kh2d]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -77,14 +78,19 @@ def workload(name, n):
                     kernel={"fp32": "sweep1d_f32<1,true>", "fp64": "sweep1d_x64<3,true>"}, n=nprob)
     if name == "star3d":
         n1 = n or 640
-        n2, n3 = max(8, n1 // 5), max(8, (2 * n1) // 5)
+        n2 = 1 << max(3, round(math.log2(max(8, n1 // 5))))
+        n3 = 2 * n2
         T = mds_table()
-        g, prob, prims = problems.star3d(n1, n2, n3, table=T)
-        return dict(probs=[(prob, prims)], nvar=6, steps=200, zones=n1 * n2 * n3, gam=[4.0 / 3.0], table=T,
-                    label=f"star3d {n1}x{n2}x{n3} spherical (log r 1e5-2e9 cm, uniform angular mesh), "
-                          f"n=3 polytrope rho_c=1e10, synthetic FP32 EOS table 180x180x50, monopole gravity",
-                    kernel={"fp32": "sph3d stage: eos + 3 flux + update kernels (float state)",
-                            "fp64": "sph3d stage: eos + 3 flux + update kernels (double state)"}, n=n1)
+        g, prob, prims = problems.star3d(n1, n2, n3, table=T, dendritic=True)
+        zones = int(g.valid_mask().sum())
+        return dict(probs=[(prob, prims)], nvar=6, steps=200, zones=zones, gam=[4.0 / 3.0], table=T,
+                    label=f"star3d {n1}x{n2}x{n3} dendritic spherical grid ({zones} zones; sinh r from the "
+                          f"origin, 0.5 km central spacing, to 2e9 cm), n=3 polytrope rho_c=1e10, synthetic FP32 "
+                          f"EOS table 180x180x50, monopole gravity",
+                    kernel={"fp32": "sph3d stage: eos + 3 flux (+2 listed level-change) + update kernels (float "
+                                    "state)",
+                            "fp64": "sph3d stage: eos + 3 flux (+2 listed level-change) + update kernels (double "
+                                    "state)"}, n=n1)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -153,9 +159,9 @@ def cpu_baseline(name, W, budget_s):
         what = "first 64 columns of the ensemble"
     elif name == "star3d":
         n1 = W["n"]
-        g, prob, prims = problems.star3d(n1, 8, 8, table=W["table"])
+        g, prob, prims = problems.star3d(n1, 16, 32, table=W["table"], dendritic=True)
         sample = [(prob, prims)]
-        what = f"the same star on a {n1}x8x8 spherical grid"
+        what = f"the same star on a {n1}x16x32 dendritic grid ({int(g.valid_mask().sum())} zones)"
     else:
         sample = W["probs"]
         what = "the full grid"
@@ -172,7 +178,8 @@ def cpu_baseline(name, W, budget_s):
             sim = oracle_for(prob, fast=True)
         sim.set_prims(prims)
         sims.append(sim)
-        zones += int(np.prod(prims.shape[1:]))
+        zones += int(prob["grid"].valid_mask().sum()) if getattr(prob["grid"], "levels", None) and \
+            prob["grid"].levels() is not None else int(np.prod(prims.shape[1:]))
     t0 = time.perf_counter()
     for sim in sims:
         sim.run(max_steps=1)
