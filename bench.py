@@ -111,7 +111,7 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
                 ctx.set_tile(t, v)
         tuned_ms = None
     else:
-        tuned_ms = ctx.autotune(3)      # launch-shape autotuner (state is restored afterwards)
+        tuned_ms = ctx.autotune(8)      # launch-shape autotuner (state is restored afterwards)
     m0 = np.array([ctx.totals(k) for k in range(len(probs))])
     if warmup:
         ctx.run(warmup, graph=True)

@@ -176,7 +176,7 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
   int p0 = __shfl(lo, 0, 64);
   if (__all(lo == p0)) {
     double m = wave_min(b);
-    if ((threadIdx.x & 63) == 0) atomicMin(&ck[lo].mindt, dt_key(m));
+    if ((threadIdx.x & 63) == 0) fold_min_key(&ck[lo].mindt, min_key_seen(&ck[lo].mindt), m);
   } else {
     atomicMin(&ck[lo].mindt, dt_key(b));
   }
@@ -1065,8 +1065,15 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     else { c->ty3d = v.first; c->kc3d = v.second; }
     double ms[3];
     rc = mds_run_timed(c, 1, ms);      // warm the shape
-    if (rc == MDS_OK) rc = mds_run_timed(c, nsteps, ms);
-    if (rc == MDS_OK && ms[0] / nsteps < bms) { bms = ms[0] / nsteps; best = v; }
+    // best of two timed runs from the same state: one run of a few steps is
+    // at the mercy of clock ramps and other work on the device
+    double t = 1e300;
+    for (int rep = 0; rep < 2 && rc == MDS_OK; rep++) {
+      rc = snap_copy(c, s, false);
+      if (rc == MDS_OK) rc = mds_run_timed(c, nsteps, ms);
+      if (rc == MDS_OK && ms[0] < t) t = ms[0];
+    }
+    if (rc == MDS_OK && t / nsteps < bms) { bms = t / nsteps; best = v; }
     if (rc == MDS_OK) rc = snap_copy(c, s, false);
   }
   snap_free(s);

@@ -515,6 +515,21 @@ MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
 // positive double -> order-preserving 64-bit key for atomicMin
 MDS_DEV unsigned long long dt_key(double v) { return (unsigned long long)__double_as_longlong(v); }
 
+// Running minimum keys shared by a whole launch.  One same-address atomic per
+// wavefront serializes at the L2 and can cost more than the kernel's own
+// work, so a wavefront reads the current minimum (device-coherent load, the
+// L2 copy) when it starts and issues its atomic only if its value beats it.
+// A stale read is larger than the true minimum, so no needed update is lost.
+MDS_DEV unsigned long long min_key_seen(unsigned long long *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MDS_DEV void fold_min_key(unsigned long long *p, unsigned long long seen, double v)
+{
+  const unsigned long long k = dt_key(v);
+  if (k < seen) atomicMin(p, k);
+}
+
 // Cross-lane neighbours with whole-wave DPP shifts (GFX9 wave_shr:1 /
 // wave_shl:1): lane l reads lane l-1 (from_below) or l+1 (from_above); the
 // edge lane keeps its own value.  Must be called with every lane active.

@@ -312,6 +312,94 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
   return isshock;
 }
 
+// PPM of one zone along D from its 3-zone stencil q3 (zones zd-1, zd, zd+1,
+// level lev): the left and right face values of every pencil slot.  Equal,
+// slot by slot, to the values face_flux forms for the two faces of the zone.
+template <class R, int D, int TAB>
+MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8], R al[8], R ar[8])
+{
+  const int s = 1 << lev;
+  if (sizeof(R) == 8) {
+    const double *mom = mom_of(pb, D, lev);
+    const int b0 = zd - 1 + MDS_NG;
+    ReconGeo64 g;
+    g.am = mom[3 * b0]; g.bm = mom[3 * b0 + 1]; g.gm = mom[3 * b0 + 2];
+    g.a0 = mom[3 * (b0 + 1)]; g.b0 = mom[3 * (b0 + 1) + 1]; g.g0 = mom[3 * (b0 + 1) + 2];
+    g.ap = mom[3 * (b0 + 2)]; g.bp = mom[3 * (b0 + 2) + 1]; g.gp = mom[3 * (b0 + 2) + 2];
+    g.xlo = pb.startx[D] + (zd * s) * pb.dx[D];
+    g.xhi = pb.startx[D] + ((zd + 1) * s) * pb.dx[D];
+    recon_geo_x64(g);
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      if (!TAB && (v == 5 || v == 7)) { al[v] = (R)q3[1][v]; ar[v] = (R)q3[1][v]; continue; }
+      double l, r;
+      para_recon_x64(q3[0][v], q3[1][v], q3[2][v], g, l, r);
+      al[v] = (R)l;
+      ar[v] = (R)r;
+    }
+  } else {
+    const float *c = rc_of(pb, D, lev) + (size_t)(zd + MDS_NG) * MDS_RC_COUNT;
+    ReconGeo32 g;
+    g.clm = c[MDS_RC_CLM]; g.clp = c[MDS_RC_CLP]; g.crm = c[MDS_RC_CRM]; g.crp = c[MDS_RC_CRP];
+    g.mu1 = c[MDS_RC_MU1]; g.kinv = c[MDS_RC_KINV]; g.inur = c[MDS_RC_INUR]; g.imu2 = c[MDS_RC_IMU2];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+      if (!TAB && (v == 5 || v == 7)) { al[v] = (R)q3[1][v]; ar[v] = (R)q3[1][v]; continue; }
+      float l, r;
+      para_recon_f32((float)q3[0][v], (float)q3[1][v], (float)q3[2][v], g, l, r);
+      al[v] = (R)l;
+      ar[v] = (R)r;
+    }
+  }
+}
+
+// Riemann solve of one face from its reconstructed states (rho, u, v1..v3,
+// Ye, P, Gamma_1); face geometry from the record of the face's zone.
+template <class R, int D, int TAB>
+MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const double *rc, bool hlle, double F[6], double &sig)
+{
+  FaceGeo<double> fg;
+  if (D == 0) {
+    fg.area = rc[MDS_ZS_AREA0];
+    fg.scale = rc[MDS_ZS_SCALEF0];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF0 + d];
+  } else if (D == 1) {
+    fg.area = rc[MDS_ZS_AREA1];
+    fg.scale = rc[MDS_ZS_SCALEF1];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF1 + d];
+  } else {
+    fg.area = rc[MDS_ZS_AREA2];
+    fg.scale = rc[MDS_ZS_SCALE0 + 2];
+    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
+  }
+  int isshock = 0;
+  sig = 0.0;
+  for (int v = 0; v < 6; v++) F[v] = 0.0;
+  PState<R, 3> L, Rt;
+  L.rho = Lq[0]; L.uu = Lq[1]; L.v[0] = Lq[2]; L.v[1] = Lq[3]; L.v[2] = Lq[4]; L.ye = Lq[5]; L.P = Lq[6]; L.G = Lq[7];
+  Rt.rho = Rq[0]; Rt.uu = Rq[1]; Rt.v[0] = Rq[2]; Rt.v[1] = Rq[3]; Rt.v[2] = Rq[4]; Rt.ye = Rq[5]; Rt.P = Rq[6];
+  Rt.G = Rq[7];
+  if (sizeof(R) == 8) {
+    if (D == 0 || !hlle) isshock = hllc_x64<3, false, TAB>(*(PState<double, 3> *)&L, *(PState<double, 3> *)&Rt, fg, D, F, sig);
+    else hlle_x64<3, false, TAB>(*(PState<double, 3> *)&L, *(PState<double, 3> *)&Rt, fg, D, F, sig);
+  } else {
+    FaceGeo<float> g32;
+    g32.area = 1.0f;
+    g32.scale = (float)fg.scale;
+    for (int d = 0; d < 3; d++) g32.gcov[d] = (float)fg.gcov[d];
+    float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, s32 = 0.0f;
+    if (fg.area >= 1.0e-16) {
+      if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(*(PState<float, 3> *)&L, *(PState<float, 3> *)&Rt, g32, D, Ff, s32);
+      else hlle_f32<3, false, TAB>(*(PState<float, 3> *)&L, *(PState<float, 3> *)&Rt, g32, D, Ff, s32);
+      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
+      sig = s32;
+    } else {
+      sig = 1.0e-30;
+    }
+  }
+  return isshock;
+}
+
 // shock-tag test of an interior zone (ghosts and non-zones count as tagged)
 MDS_DEV bool tagged(const Sph3Args &a, int i, int j, int k)
 {
@@ -334,7 +422,8 @@ MDS_DEV void tag_radial(const Sph3Args &a, int i, int j, int k)
 }
 
 // per-wave CFL (min width/|s| over the wave's faces) and counters
-MDS_DEV void face_dt(const Sph3Args &a, bool ok, double width, double sig, bool shock, bool hl, int D)
+MDS_DEV void face_dt(const Sph3Args &a, unsigned long long seen, bool ok, double width, double sig, bool shock,
+                     bool hl, int D)
 {
   DevClock *ck = a.clock;
   const double w = (ok && sig != 0.0) ? width / fabs(sig) : 1.0e300;
@@ -342,7 +431,7 @@ MDS_DEV void face_dt(const Sph3Args &a, bool ok, double width, double sig, bool 
   const unsigned long long nsh = __popcll(__ballot(ok && D == 0 && shock));
   const unsigned long long nhl = __popcll(__ballot(ok && D > 0 && hl));
   if ((threadIdx.x & 63) == 0) {
-    if (wmin < 1.0e300) atomicMin(&ck->mindt, dt_key(wmin));
+    if (wmin < 1.0e300) fold_min_key(&ck->mindt, seen, wmin);
     if (nsh) atomicAdd(&a.counters[0], nsh);
     if (nhl) atomicAdd(&a.counters[1], nhl);
   }
@@ -549,66 +638,154 @@ __global__ void sph3_shell_field(Sph3Args a, const double *ms, double *grav)
 }
 
 // ---------------------------------------------------------------------------
-// face fluxes in direction D (on dendritic grids, the faces whose pencil
-// stays on one level)
+// face fluxes in direction D (on dendritic grids, the faces whose 4-zone
+// pencil stays on one level).  Each thread reconstructs one zone (the PPM of
+// a zone serves its two faces); the face between zones z-1 and z takes the
+// right value of z-1 from the neighbouring thread: the next lane for radial
+// faces (a wavefront runs along i, 63 faces per 64 zones), the LDS row of the
+// previous wavefront for theta and phi faces (a 64 x 8 block of zones in
+// (i, j) or (i, k), 7 rows of faces).
 // ---------------------------------------------------------------------------
+template <class R, int D, int DEND>
+MDS_DEV bool face_is_regular(const DevProb &pb, int fi, int fj, int fk, int &lev, int &dk)
+{
+  lev = 0;
+  dk = 1;
+  if (!DEND) return fi <= pb.n[0] && fj <= pb.n[1] && fk <= pb.n[2];
+  const int lj = dend_lj(pb, fi), nj = pb.n[1] >> lj;
+  if (D == 0) {
+    if (fj >= nj) return false;
+    const int lk = dend_lk(pb, fi, fj);
+    if (fk >= (pb.n[2] >> lk)) return false;
+    for (int s = -2; s < 2; s++)
+      if (dend_lj(pb, fi + s) != lj || dend_lk(pb, fi + s, fj) != lk) return false;
+    return true;
+  } else if (D == 1) {
+    if (fj > nj) return false;
+    const int lk = dend_lk(pb, fi, fj);
+    if (fk >= (pb.n[2] >> lk)) return false;
+    for (int s = -2; s < 2; s++)
+      if (dend_lk(pb, fi, fj + s) != lk) return false;
+    lev = lj;
+    return true;
+  }
+  if (fj >= nj) return false;
+  const int lk = dend_lk(pb, fi, fj);
+  if (fk > (pb.n[2] >> lk)) return false;
+  lev = lk;
+  dk = 1 << lk;
+  return true;
+}
+
 template <class R, int D, int TAB, int DEND>
-__global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
+__global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
 {
   const DevProb &pb = a.pb;
-  const int n1 = pb.n[0] + (D == 0), n2 = pb.n[1] + (D == 1), n3 = pb.n[2] + (D == 2);
-  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = f < (long long)n1 * n2 * n3;
-  const long long fl = live ? f : 0;
-  const int fi = (int)(fl % n1), fj = (int)((fl / n1) % n2), fk = (int)(fl / ((long long)n1 * n2));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   DevClock *ck = a.clock;
   const bool active = ck->active;
+  const unsigned long long seen = min_key_seen(&ck->mindt);
   const double gam = pb.gam;
+  // this thread's zone (zi, zj, zk) and the face (fi, fj, fk) = zone, whose
+  // left neighbour is the zone of the previous lane (D = 0) or row (D > 0)
+  int zi, zj, zk;
+  bool has_face;
+  if (D == 0) {
+    const int nseg = (pb.n[0] + 1 + 62) / 63;
+    const long long wv = (long long)blockIdx.x * (blockDim.x >> 6) + w;
+    const int seg = (int)(wv % nseg);
+    const long long rest = wv / nseg;
+    zj = (int)(rest % pb.n[1]);
+    zk = (int)(rest / pb.n[1]);
+    zi = seg * 63 + lane - 1;
+    has_face = lane > 0 && zi <= pb.n[0] && zk < pb.n[2];
+  } else if (D == 1) {
+    zi = blockIdx.x * 64 + lane;
+    zj = blockIdx.y * 7 + w - 1;
+    zk = blockIdx.z;
+    has_face = w > 0 && zi < pb.n[0] && zj <= pb.n[1];
+  } else {
+    zi = blockIdx.x * 64 + lane;
+    zj = blockIdx.y;
+    zk = blockIdx.z * 7 + w - 1;
+    has_face = w > 0 && zi < pb.n[0] && zk <= pb.n[2];
+  }
   int lev = 0, dk = 1;
+  bool live = has_face && face_is_regular<R, D, DEND>(pb, zi, zj, zk, lev, dk);
+  // levels of the zone's own stencil (for the reconstruction)
+  int zlev = 0;
   if (DEND) {
-    const int lj = dend_lj(pb, fi), nj = pb.n[1] >> lj;
-    if (D == 0) {
-      const int lk = fj < nj ? dend_lk(pb, fi, fj) : 0;
-      live = live && fj < nj && fk < (pb.n[2] >> lk);
-      for (int s = -2; s < 2 && live; s++)
-        if (dend_lj(pb, fi + s) != lj || dend_lk(pb, fi + s, fj) != lk) live = false;
-    } else if (D == 1) {
-      live = live && fj <= nj;
-      const int lk = live ? dend_lk(pb, fi, fj) : 0;
-      live = live && fk < (pb.n[2] >> lk);
-      for (int s = -2; s < 2 && live; s++)
-        if (dend_lk(pb, fi, fj + s) != lk) live = false;
-      lev = lj;
-    } else {
-      live = live && fj < nj;
-      const int lk = live ? dend_lk(pb, fi, fj) : 0;
-      live = live && fk <= (pb.n[2] >> lk);
-      lev = lk;
-      dk = 1 << lk;
+    const int ci = zi < 0 ? 0 : (zi > pb.n[0] ? pb.n[0] : zi);
+    if (D == 1) zlev = dend_lj(pb, ci);
+    else if (D == 2) {
+      const int cj = zj < (pb.n[1] >> dend_lj(pb, ci)) ? zj : 0;
+      zlev = dend_lk(pb, ci, cj);
     }
   }
-  const int ci = live ? fi : 0, cj = live ? fj : 0, ck3 = live ? fk : 0;
-  // the 4-zone pencil along D
-  double q[4][8];
-  for (int s = 0; s < 4; s++) {
-    int i = ci, j = cj, k = ck3;
-    if (D == 0) i += s - 2;
-    else if (D == 1) j += s - 2;
-    else k += s - 2;
+  // this zone exists (interior, or the one ghost zone at either end along D
+  // that a face needs); other threads read a safe zone and own no face
+  bool zvalid;
+  if (D == 0) {
+    zvalid = zi >= -1 && zi <= pb.n[0] && zj < pb.n[1] && zk < pb.n[2];
+  } else {
+    zvalid = zi < pb.n[0] && zj < pb.n[1] + (D == 1) && zk < pb.n[2] + (D == 2) && zj >= -1 && zk >= -1;
+  }
+  if (DEND && zvalid) {
+    const int ci = zi < 0 ? 0 : (zi >= pb.n[0] ? pb.n[0] - 1 : zi);
+    const int nj = pb.n[1] >> dend_lj(pb, ci);
+    if (D == 0) {
+      zvalid = zj < nj && zk < (pb.n[2] >> dend_lk(pb, ci, zj));
+    } else if (D == 1) {
+      zvalid = zj <= nj && zk < (pb.n[2] >> dend_lk(pb, ci, zj < 0 ? 0 : (zj >= nj ? nj - 1 : zj)));
+    } else {
+      zvalid = zj < nj && zk <= (pb.n[2] >> dend_lk(pb, ci, zj));
+    }
+  }
+  const int bi = zvalid ? zi : 0, bj = zvalid ? zj : 0, bk = zvalid ? zk : 0;
+  double q3[3][8];
+  for (int s = 0; s < 3; s++) {
+    int i = bi, j = bj, k = bk;
+    if (D == 0) i += s - 1;
+    else if (D == 1) j += s - 1;
+    else k += s - 1;
     double sgn[3];
     map_zone(pb, D, i, j, k, sgn);
-    zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q[s]);
-    for (int d = 0; d < 3; d++) q[s][2 + d] *= sgn[d];
+    zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
+    for (int d = 0; d < 3; d++) q3[s][2 + d] *= sgn[d];
   }
-  const int fd = D == 0 ? ci : (D == 1 ? cj : ck3);
-  const double *rc = rec3(pb, ci, cj);
-  // transverse faces: HLLE next to a tagged zone (ghosts count as tagged)
+  const int zd = D == 0 ? bi : (D == 1 ? bj : bk);
+  // polar-axis rows (last theta row): first-order states, exchanged as zone values
+  const bool polar = (D == 2) && zvalid && (bj == (pb.n[1] >> dend_lj(pb, bi)) - 1);
+  R al[8], ar[8];
+  if (polar) {
+    for (int v = 0; v < 8; v++) al[v] = ar[v] = (R)q3[1][v];
+  } else {
+    zone_recon<R, D, TAB>(pb, zlev, zd, q3, al, ar);
+  }
+  // the left state of this thread's face
+  R Lq[8];
+  if constexpr (D == 0) {
+    for (int v = 0; v < 8; v++) Lq[v] = __shfl_up(ar[v], 1, 64);
+  } else {
+    __shared__ R sh[8][8][64];
+    for (int v = 0; v < 8; v++) sh[v][w][lane] = ar[v];
+    __syncthreads();
+    for (int v = 0; v < 8; v++) Lq[v] = w > 0 ? sh[v][w - 1][lane] : ar[v];
+  }
+  const int fi = live ? zi : 0, fj = live ? zj : 0, fk = live ? zk : 0;
+  const double *rc = rec3(pb, fi, fj);
   bool hlle = false;
-  if (D > 0) hlle = tagged(a, ci, cj, ck3) || tagged(a, ci - (D == 0), cj - (D == 1), ck3 - (D == 2));
-  const bool polar = (D == 2) && (cj == nj_of(pb, ci) - 1);
+  if (D > 0) hlle = tagged(a, fi, fj, fk) || tagged(a, fi, fj - (D == 1), fk - (D == 2));
+  if (polar && live) {
+    // step.c:1299-1397: velocities of zones k-1 and k through Cartesian components
+    double vl[3] = {(double)Lq[2], (double)Lq[3], (double)Lq[4]}, vr[3] = {(double)al[2], (double)al[3], (double)al[4]};
+    polar_rotate(pb, rc, fk, dk, vl, vr);
+    for (int d = 0; d < 3; d++) { Lq[2 + d] = (R)vl[d]; al[2 + d] = (R)vr[d]; }
+  }
   double sig, F[6];
-  const int isshock = face_flux<R, D, TAB, 0>(pb, lev, fd, q, polar, ck3, dk, rc, hlle, F, sig);
+  const int isshock = face_riemann<R, D, TAB>(Lq, al, rc, hlle, F, sig);
   if (live && active) {
+    const long long f = fid3(pb, D, fi, fj, fk);
     for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
     if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
     if (D == 0 && isshock) {
@@ -623,7 +800,7 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
   if (DEND) {
     // DJS(i) dx1 / |s| and DKS(i,j) dx2 / |s| (step.c:1263, 1419)
     const double width = (double)(D == 0 ? 1 : (D == 1 ? 1 << lev : dk)) * pb.dx[D];
-    face_dt(a, live && active, width, sig, isshock, hlle, D);
+    face_dt(a, seen, live && active, width, sig, isshock, hlle, D);
     return;
   }
   // CFL: min dx/|s| over faces = dx / max |s| (one division per wave)
@@ -631,8 +808,8 @@ __global__ __launch_bounds__(256) void sph3_flux(Sph3Args a, StateArrays<R> in)
   const double wmax = wave_max(smax);
   const unsigned long long nsh = __popcll(__ballot(live && active && D == 0 && isshock));
   const unsigned long long nhl = __popcll(__ballot(live && active && D > 0 && hlle));
-  if ((threadIdx.x & 63) == 0) {
-    if (wmax > 0.0) atomicMin(&ck->mindt, dt_key(pb.dx[D] / wmax));
+  if (lane == 0) {
+    if (wmax > 0.0) fold_min_key(&ck->mindt, seen, pb.dx[D] / wmax);
     if (nsh) atomicAdd(&a.counters[0], nsh);
     if (nhl) atomicAdd(&a.counters[1], nhl);
   }
@@ -651,6 +828,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
   const int4 e = list[live ? t : 0];
   const int fi = e.x, fj = e.y, fk = e.z;
   const bool active = a.clock->active;
+  const unsigned long long seen = min_key_seen(&a.clock->mindt);
   double q[4][8];
   int lev = 0;
   bool hlle = false;
@@ -678,7 +856,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
     if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
     if (D == 0 && isshock) tag_radial(a, fi, fj, fk);
   }
-  face_dt(a, live && active, (double)(1 << lev) * pb.dx[D], sig, isshock, hlle, D);
+  face_dt(a, seen, live && active, (double)(1 << lev) * pb.dx[D], sig, isshock, hlle, D);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,6 +876,7 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const long long zl = zid3(pb, i, j, k);
   DevClock *ck = a.clock;
   const bool active = ck->active;
+  const unsigned long long seen = min_key_seen(&ck->mindt);
   const double dt = ck->dt;
   const double gam = pb.gam;
   const StateArrays<R> &in = stage == 0 ? sp : sph;
@@ -820,7 +999,7 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   if ((threadIdx.x & 63) == 0) {
     double l = wl;
     if (wm > 0.0) l = fmin(l, 1.0 / wm);
-    if (l < 1.0e300) atomicMin(&ck->mindt, dt_key(l));
+    if (l < 1.0e300) fold_min_key(&ck->mindt, seen, l);
   }
 }
 
@@ -835,11 +1014,15 @@ static void launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArra
   const StateArrays<R> &in = stage == 0 ? sp : sph;
   const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
   const dim3 blk(256);
+  const int nseg = (pb.n[0] + 1 + 62) / 63;
+  const dim3 fgrid[3] = {dim3((unsigned)(((long long)nseg * pb.n[1] * pb.n[2] + 7) / 8)),
+                         dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)((pb.n[1] + 1 + 6) / 7), (unsigned)pb.n[2]),
+                         dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)pb.n[1], (unsigned)((pb.n[2] + 1 + 6) / 7))};
+  const dim3 fblk(512);
 #define MDS_FLUX(D)                                                                                        \
   {                                                                                                        \
-    const dim3 g((unsigned)((L.nf[D] + 255) / 256));                                                      \
-    if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND>), g, blk, 0, st, a, in);                      \
-    else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND>), g, blk, 0, st, a, in);                            \
+    if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND>), fgrid[D], fblk, 0, st, a, in);              \
+    else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND>), fgrid[D], fblk, 0, st, a, in);                    \
   }
 #define MDS_FLUX_IRR(D)                                                                                    \
   if (DEND && L.nirr[D] > 0) {                                                                             \

@@ -35,6 +35,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   const DevProb &pb = *a.prob;
   DevClock *ck = a.clock;
   if (!ck->active) return;
+  const unsigned long long seen = min_key_seen(&ck->mindt);
   const int strip = wv % a.nstrip, chunk = wv / a.nstrip;
   const int n1 = pb.n[0], n2 = pb.n[1];
   const int c0 = strip * MDS_SEG1D;
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
     if (wloc > 0.0) lim = fmin(lim, 1.0 / wloc);
     if (wsrc < 1.0e300) lim = fmin(lim, wsrc / (0.0 + 1.0e-16));
     lim = fmin(lim, wlim);
-    if (lim < 1.0e300) atomicMin(&ck->mindt, dt_key(lim));
+    if (lim < 1.0e300) fold_min_key(&ck->mindt, seen, lim);
   }
   for (int off = 32; off > 0; off >>= 1) {
     nshock += __shfl_xor(nshock, off, 64);

@@ -39,6 +39,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   const DevProb &pb = *a.prob;
   DevClock *ck = a.clock;
   if (!ck->active) return;                       // uniform over the launch
+  const unsigned long long seen = min_key_seen(&ck->mindt);
   const int strip = blockIdx.x % a.nstrip;
   const int tile = (blockIdx.x / a.nstrip) % a.ntile;
   const int chunk = blockIdx.x / (a.nstrip * a.ntile);
@@ -281,7 +282,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     if (wloc > 0.0) lim = fmin(lim, 1.0 / wloc);
     if (wsrc < 1.0e300) lim = fmin(lim, wsrc / (0.0 + 1.0e-16));
     lim = fmin(lim, wlim);
-    if (lim < 1.0e300) atomicMin(&ck->mindt, dt_key(lim));
+    if (lim < 1.0e300) fold_min_key(&ck->mindt, seen, lim);
   }
   for (int off = 32; off > 0; off >>= 1) {
     nshock += __shfl_xor(nshock, off, 64);
