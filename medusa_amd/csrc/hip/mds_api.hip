@@ -162,7 +162,8 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
   for (int d = 0; d < pb.ndim; d++) {
     double sc = 1.0;
     if (pb.sph && pb.ndim == 3)
-      sc = pb.zone[((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT + MDS_ZS_SCALE0 + d];
+      sc = pb.zone[(size_t)(MDS_ZS_SCALE0 + d) * (pb.n[0] + 2 * MDS_NG) * (pb.n[1] + 2 * MDS_NG)
+                   + (size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)];
     else if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
     double v = sp.v[2 + d][pb.off + q];
     double ldx = pb.dx[d];                                  // DJS / DKS widths (step.c:179-181)
@@ -199,9 +200,10 @@ __global__ void totals_kernel(const DevProb *prob, int p, long long nz, int nv, 
       continue;
     if (pb.sph && pb.ndim == 3) {
       const int i = (int)(q % pb.n[0]), j = (int)((q / pb.n[0]) % pb.n[1]);
-      const double *zs = pb.zone + ((size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)) * MDS_ZS_COUNT;
-      vol = zs[MDS_ZS_VOLUME];
-      g[0] = zs[MDS_ZS_GCOV0]; g[1] = zs[MDS_ZS_GCOV0 + 1]; g[2] = zs[MDS_ZS_GCOV0 + 2];
+      const size_t fs = (size_t)(pb.n[0] + 2 * MDS_NG) * (pb.n[1] + 2 * MDS_NG);
+      const double *zs = pb.zone + (size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG);
+      vol = zs[MDS_ZS_VOLUME * fs];
+      g[0] = zs[MDS_ZS_GCOV0 * fs]; g[1] = zs[(MDS_ZS_GCOV0 + 1) * fs]; g[2] = zs[(MDS_ZS_GCOV0 + 2) * fs];
     } else if (pb.sph) {
       const double *zg = pb.zone + (size_t)(q + MDS_NG) * MDS_ZG_COUNT;
       vol = zg[MDS_ZG_VOLUME];
@@ -490,7 +492,17 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       const int nzq = g->n[0] + 2 * MDS_NG;
       const size_t zbytes = c->sph3 ? (size_t)nzq * (g->n[1] + 2 * MDS_NG) * MDS_ZS_COUNT * sizeof(double)
                                     : (size_t)nzq * MDS_ZG_COUNT * sizeof(double);
-      int rc1 = upload_geo(c, g->zone, zbytes, (const void **)&d.zone);
+      int rc1;
+      if (c->sph3) {
+        // field-major copy of the per-(i, j) records (the kernels read one field along i at a time)
+        const size_t nrec = (size_t)nzq * (g->n[1] + 2 * MDS_NG);
+        std::vector<double> soa(nrec * MDS_ZS_COUNT);
+        for (size_t q = 0; q < nrec; q++)
+          for (int f = 0; f < MDS_ZS_COUNT; f++) soa[(size_t)f * nrec + q] = g->zone[q * MDS_ZS_COUNT + f];
+        rc1 = upload_geo(c, soa.data(), zbytes, (const void **)&d.zone);
+      } else {
+        rc1 = upload_geo(c, g->zone, zbytes, (const void **)&d.zone);
+      }
       if (rc1) return rc1;
       if (c->precision == MDS_FP32_BULK && !c->sph3) {
         std::vector<ZoneF32> z32((size_t)nzq);

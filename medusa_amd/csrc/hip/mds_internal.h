@@ -46,7 +46,8 @@ struct DevProb {
   double volume, area[3];  // Cartesian uniform geometry
   const double *mom[3];    // per direction (n[d]+2NG)*3: alpha, beta, Gamma per zone
   const float *rc[3];      // per direction (n[d]+2NG)*MDS_RC_COUNT
-  const double *zone;      // spherical: (n[0]+2NG)*MDS_ZG_COUNT
+  const double *zone;      // spherical columns: (n[0]+2NG) x MDS_ZG_COUNT records; 3D spherical:
+                           // field-major, zone[field][j + NG][i + NG] (MDS_ZS_*)
   const ZoneF32 *z32;      // spherical, FP32-bulk: n[0]+2NG records
   // dendritic 3D spherical grids (null otherwise): log2 of the theta and phi
   // coarsening, ghost-extended: lj[i + NG], lk[(i + NG) * (n[1] + 2NG) + j + NG]

@@ -56,10 +56,22 @@ MDS_DEV long long fid3(const DevProb &pb, int D, int i, int j, int k)
   return (long long)i + a * ((long long)j + b * k);
 }
 
-MDS_DEV const double *rec3(const DevProb &pb, int i, int j)
+// Per-(i, j) geometry record.  On the device the records are stored field
+// by field (zone[field][j + NG][i + NG]): a wavefront runs along i, so every
+// field load is one contiguous 512-byte read instead of 64 cache lines.
+struct ZRec {
+  const double *base;
+  size_t stride;
+  MDS_DEV double operator[](int f) const { return base[(size_t)f * stride]; }
+};
+
+MDS_DEV ZRec rec3(const DevProb &pb, int i, int j)
 {
-  const int N1 = pb.n[0] + 2 * MDS_NG;
-  return pb.zone + ((size_t)(j + MDS_NG) * N1 + (i + MDS_NG)) * MDS_ZS_COUNT;
+  const int N1 = pb.n[0] + 2 * MDS_NG, N2 = pb.n[1] + 2 * MDS_NG;
+  ZRec r;
+  r.stride = (size_t)N1 * N2;
+  r.base = pb.zone + (size_t)(j + MDS_NG) * N1 + (i + MDS_NG);
+  return r;
 }
 
 // zone counts of radial zone i and of row (i, j) at their levels
@@ -159,7 +171,7 @@ MDS_DEV void any_state(const Sph3Args &a, const StateArrays<R> &in, int i, int j
 // Polar-axis states of the phi face at phi_k between zones k-1 (L) and k (R):
 // velocities rotated through Cartesian components (step.c:1299-1397); dk is
 // the row's phi coarsening
-MDS_DEV void polar_rotate(const DevProb &pb, const double *rc, int k, int dk, double vl[3], double vr[3])
+MDS_DEV void polar_rotate(const DevProb &pb, const ZRec &rc, int k, int dk, double vl[3], double vr[3])
 {
   const double sth = rc[MDS_ZS_STH], cth = rc[MDS_ZS_CTH];
   double *vv[2] = {vl, vr};
@@ -198,7 +210,7 @@ MDS_DEV double minmod(double a, double b)
 // hlle: transverse face next to a tagged zone.  Returns isshock.
 template <class R, int D, int TAB, int GREC>
 MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
-                      const double *rc, bool hlle, double F[6], double &sig)
+                      const ZRec &rc, bool hlle, double F[6], double &sig)
 {
   FaceGeo<double> fg;
   if (D == 0) {
@@ -356,7 +368,7 @@ MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8
 // Riemann solve of one face from its reconstructed states (rho, u, v1..v3,
 // Ye, P, Gamma_1); face geometry from the record of the face's zone.
 template <class R, int D, int TAB>
-MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const double *rc, bool hlle, double F[6], double &sig)
+MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const ZRec &rc, bool hlle, double F[6], double &sig)
 {
   FaceGeo<double> fg;
   if (D == 0) {
@@ -773,7 +785,7 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
     for (int v = 0; v < 8; v++) Lq[v] = w > 0 ? sh[v][w - 1][lane] : ar[v];
   }
   const int fi = live ? zi : 0, fj = live ? zj : 0, fk = live ? zk : 0;
-  const double *rc = rec3(pb, fi, fj);
+  const ZRec rc = rec3(pb, fi, fj);
   bool hlle = false;
   if (D > 0) hlle = tagged(a, fi, fj, fk) || tagged(a, fi, fj - (D == 1), fk - (D == 2));
   if (polar && live) {
@@ -847,7 +859,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
     if (lj >= lm)
       for (int kk = kz << (lj - lm); kk < (kz + 1) << (lj - lm); kk++) hlle = hlle || tagged(a, fi, fj - 1, kk);
   }
-  const double *rc = rec3(pb, fi, fj);
+  const ZRec rc = rec3(pb, fi, fj);
   double sig, F[6];
   const int isshock = face_flux<R, D, TAB, 1>(pb, lev, D == 0 ? fi : fj, q, false, fk, 1, rc, hlle, F, sig);
   if (live && active) {
@@ -884,7 +896,7 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   zone_state<R>(a, in, zl, TAB, gam, q);
   if (stage == 1) zone_state<R>(a, sp, zl, TAB, gam, qn);
   else for (int t = 0; t < 8; t++) qn[t] = q[t];
-  const double *rc = rec3(pb, i, j);
+  const ZRec rc = rec3(pb, i, j);
   double gcov[3], gcon[3];
   for (int d = 0; d < 3; d++) { gcov[d] = rc[MDS_ZS_GCOV0 + d]; gcon[d] = rc[MDS_ZS_GCON + d]; }
   // geometric sources (rad_fluid.c:195-214): the 11 non-zero conn[l][m][n]
