@@ -618,34 +618,47 @@ __global__ __launch_bounds__(256) void sph3_shell_mass(Sph3Args a, StateArrays<R
   if (threadIdx.x == 0) ms[i] = sh[0];
 }
 
-__global__ void sph3_shell_field(Sph3Args a, const double *ms, double *grav)
+// g at radial face i from the mass inside it: -G M / r^2 dr/dx (gravity.c:420-425)
+MDS_DEV double face_gravity(const DevProb &pb, int i, double mr)
 {
-  // one thread: n1 is a few thousand at most
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const DevProb &pb = a.pb;
   const double GN = 6.67384e-8;
-  double mr = 0.0, ge = 0.0;
-  for (int i = 0; i < pb.n[0]; i++) {
-    mr = mr + ms[i];
-    const double xe = pb.startx[0] + (i + 1) * pb.dx[0];
-    double r, drdx;
-    if (pb.rcoord == MDS_RCOORD_LOG) {
-      r = exp(xe);
-      drdx = exp(xe);
-    } else if (pb.rcoord == MDS_RCOORD_SINH) {
-      r = pb.rtrans * sinh(fabs(xe) / pb.rtrans);
-      drdx = cosh(fabs(xe) / pb.rtrans);
-    } else {
-      r = fabs(xe);
-      drdx = 1.0;
+  const double xe = pb.startx[0] + i * pb.dx[0];
+  double r, drdx;
+  if (pb.rcoord == MDS_RCOORD_LOG) {
+    r = exp(xe);
+    drdx = exp(xe);
+  } else if (pb.rcoord == MDS_RCOORD_SINH) {
+    r = pb.rtrans * sinh(fabs(xe) / pb.rtrans);
+    drdx = cosh(fabs(xe) / pb.rtrans);
+  } else {
+    r = fabs(xe);
+    drdx = 1.0;
+  }
+  return ((-GN * mr) / (r * r)) * drdx;
+}
+
+// One block: the enclosed masses by a sequential running sum (thread 0, in
+// place), then every shell's field at its volume centroid in parallel.
+__global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, double *ms, double *grav)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0];
+  if (threadIdx.x == 0) {
+    double mr = 0.0;
+    for (int i = 0; i < n1; i++) {
+      mr = mr + ms[i];
+      ms[i] = mr;
     }
-    const double gn = ((-GN * mr) / (r * r)) * drdx;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+    const double ge = i == 0 ? 0.0 : face_gravity(pb, i, ms[i - 1]);
+    const double gn = face_gravity(pb, i + 1, ms[i]);
     const double xl = pb.startx[0] + i * pb.dx[0];
     const double A = (gn - ge) / pb.dx[0];
     const double B = ge - A * xl;
     const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
     grav[i] = (A * be) / ga + B;
-    ge = gn;
   }
 }
 
@@ -845,13 +858,13 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
   int lev = 0;
   bool hlle = false;
   if (D == 0) {
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < 4; s++) radial_entry<R, TAB>(a, in, fi, fj, fk, fi - 2 + s, q[s]);
   } else {
     const int lj = dend_lk(pb, fi, fj), lm = dend_lk(pb, fi, fj - 1);
     const int lf = lj < lm ? lj : lm;
     lev = dend_lj(pb, fi);
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < 4; s++) theta_entry<R, TAB>(a, in, fi, lf, fk, fj - 2 + s, q[s]);
     // the zone of row j owning this sub-face, and transverse_shock (fluid.c:189-201)
     const int kz = fk >> (lj - lf);
@@ -1078,7 +1091,7 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
   if (L.tab) hipLaunchKernelGGL(sph3_eos_kernel<R>, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, a, in);
   if (L.mono) {
     hipLaunchKernelGGL(sph3_shell_mass<R>, dim3(pb.n[0]), blk, 0, st, a, in, L.ms);
-    hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(64), 0, st, a, L.ms, L.grav);
+    hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(1024), 0, st, a, L.ms, L.grav);
   }
   if (L.dend) launch_stage<R, 1>(L, a, sp, sph, stage, st);
   else launch_stage<R, 0>(L, a, sp, sph, stage, st);
