@@ -551,8 +551,13 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     const int nslot = c->tab ? 6 : 5;
     for (int d = 0; d < 3; d++) {
       L.nf[d] = (long long)(h.n[0] + (d == 0)) * (h.n[1] + (d == 1)) * (h.n[2] + (d == 2));
-      HIPCHK(c, hipMalloc(&L.F[d], sizeof(double) * nslot * (size_t)L.nf[d]));
-      HIPCHK(c, hipMemset(L.F[d], 0, sizeof(double) * nslot * (size_t)L.nf[d]));
+      const size_t fs = L.fp32 ? sizeof(float) : sizeof(double);
+      // the update addresses one slot's faces with 32-bit byte offsets
+      if (fs * (size_t)L.nf[d] >= ((size_t)1 << 32))
+        return set_err(c, MDS_EINVAL, "3D spherical grid too large: %lld faces in direction %d (at most %zu)",
+                       L.nf[d], d, (((size_t)1 << 32) - 1) / fs);
+      HIPCHK(c, hipMalloc(&L.F[d], fs * nslot * (size_t)L.nf[d]));
+      HIPCHK(c, hipMemset(L.F[d], 0, fs * nslot * (size_t)L.nf[d]));
     }
     if (c->tab) {
       HIPCHK(c, hipMalloc(&L.P, rs * (size_t)c->nzone));

@@ -120,7 +120,8 @@ struct Sweep3DArgs {
 // buffers of the direction-split 3D spherical stage
 struct Sph3Launch {
   int fp32, tab, mono;
-  double *F[3];            // face fluxes per direction, [slot][face]
+  void *F[3];              // face fluxes per direction, [slot][face]: FP64 area weighted,
+                           // or FP32 per unit area on the FP32-bulk path
   long long nf[3];         // faces per direction
   void *P, *G, *CS;        // table EOS arrays (state precision)
   double *ms, *grav;       // shell masses and monopole field (n1 each)

@@ -16,8 +16,11 @@
 // stencil index outside the grid is mapped onto the zone the reference's
 // ghost fill copies there (bc.c:705-1421): radial REFLECT/OUTFLOW, theta
 // across the pole (SPHERICAL_ORIGIN: mirror in j, phi + pi, v2 and v3
-// flipped) or REFLECT/OUTFLOW, phi PERIODIC.  Face fluxes are kept in FP64
-// (area weighted) for both precisions.
+// flipped) or REFLECT/OUTFLOW, phi PERIODIC.  Face fluxes are stored in the
+// state precision: FP64 area weighted, or (FP32-bulk) the FP32 flux per unit
+// area, which the update multiplies by the FP64 face area -- the same product
+// the flux kernel would form, so storing it halves the flux traffic and
+// changes no bit.
 //
 // Dendritic grids (utils.c:181-590) keep the (i, j, k) storage with
 // level-local j < nj(i), k < nk(i, j).  The flux kernels above handle the
@@ -39,7 +42,7 @@ struct Sph3Args {
   unsigned long long *counters;
   unsigned char *flags;       // 1: no shock tag, 0: tagged (interior zones)
   DevEos eos;
-  double *F[3];               // face fluxes [slot][face] per direction, slot 5 = Ye
+  void *F[3];                 // face fluxes [slot][face] per direction, slot 5 = Ye
   long long nf[3];            // faces per direction
   const void *P, *G, *CS;     // table EOS arrays (state precision)
   const double *grav;         // monopole field per radial shell, or null
@@ -54,6 +57,14 @@ MDS_DEV long long fid3(const DevProb &pb, int D, int i, int j, int k)
 {
   const long long a = pb.n[0] + (D == 0), b = pb.n[1] + (D == 1);
   return (long long)i + a * ((long long)j + b * k);
+}
+
+// store the fluxes of face f in direction D (slots 0..nslot-1)
+template <class R>
+MDS_DEV void store_flux(const Sph3Args &a, int D, long long f, const double F[6], int nslot)
+{
+  R *p = (R *)a.F[D];
+  for (int v = 0; v < nslot; v++) p[(size_t)v * a.nf[D] + f] = (R)F[v];
 }
 
 // Per-(i, j) geometry record.  On the device the records are stored field
@@ -207,7 +218,8 @@ MDS_DEV double minmod(double a, double b)
 // Reconstruction and Riemann solve of one face from its 4-zone pencil q
 // (zones fd-2 .. fd+1 along D, level lev).  GREC: reconstruct Gamma_1 of a
 // gamma law too (a restricted radial pencil carries v^3 in that slot).
-// hlle: transverse face next to a tagged zone.  Returns isshock.
+// hlle: transverse face next to a tagged zone.  Returns isshock.  F is area
+// weighted in FP64 and per unit area in FP32 (zero on faces of area < 1e-16).
 template <class R, int D, int TAB, int GREC>
 MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
                       const ZRec &rc, bool hlle, double F[6], double &sig)
@@ -315,7 +327,7 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
     if (fg.area >= 1.0e-16) {
       if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
       else hlle_f32<3, false, TAB>(L, Rt, g32, D, Ff, s32);
-      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
+      for (int v = 0; v < 6; v++) F[v] = (double)Ff[v];   // per unit area
       sig = s32;
     } else {
       sig = 1.0e-30;
@@ -366,7 +378,8 @@ MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8
 }
 
 // Riemann solve of one face from its reconstructed states (rho, u, v1..v3,
-// Ye, P, Gamma_1); face geometry from the record of the face's zone.
+// Ye, P, Gamma_1); face geometry from the record of the face's zone.  F as
+// in face_flux.
 template <class R, int D, int TAB>
 MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const ZRec &rc, bool hlle, double F[6], double &sig)
 {
@@ -403,7 +416,7 @@ MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const ZRec &rc, bool hlle
     if (fg.area >= 1.0e-16) {
       if (D == 0 || !hlle) isshock = hllc_f32<3, false, TAB>(*(PState<float, 3> *)&L, *(PState<float, 3> *)&Rt, g32, D, Ff, s32);
       else hlle_f32<3, false, TAB>(*(PState<float, 3> *)&L, *(PState<float, 3> *)&Rt, g32, D, Ff, s32);
-      for (int v = 0; v < 6; v++) F[v] = fg.area * (double)Ff[v];
+      for (int v = 0; v < 6; v++) F[v] = (double)Ff[v];   // per unit area
       sig = s32;
     } else {
       sig = 1.0e-30;
@@ -810,9 +823,7 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
   double sig, F[6];
   const int isshock = face_riemann<R, D, TAB>(Lq, al, rc, hlle, F, sig);
   if (live && active) {
-    const long long f = fid3(pb, D, fi, fj, fk);
-    for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
-    if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
+    store_flux<R>(a, D, fid3(pb, D, fi, fj, fk), F, TAB ? 6 : 5);
     if (D == 0 && isshock) {
       if (DEND) tag_radial(a, fi, fj, fk);
       else
@@ -876,9 +887,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
   double sig, F[6];
   const int isshock = face_flux<R, D, TAB, 1>(pb, lev, D == 0 ? fi : fj, q, false, fk, 1, rc, hlle, F, sig);
   if (live && active) {
-    const long long f = fid3(pb, D, fi, fj, fk);
-    for (int v = 0; v < 5; v++) a.F[D][(size_t)v * a.nf[D] + f] = F[v];
-    if (TAB) a.F[D][(size_t)5 * a.nf[D] + f] = F[5];
+    store_flux<R>(a, D, fid3(pb, D, fi, fj, fk), F, TAB ? 6 : 5);
     if (D == 0 && isshock) tag_radial(a, fi, fj, fk);
   }
   face_dt(a, seen, live && active, (double)(1 << lev) * pb.dx[D], sig, isshock, hlle, D);
@@ -893,12 +902,14 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
   const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = z < (long long)n1 * n2 * n3;
-  const long long zl0 = live ? z : 0;
-  const int i0 = (int)(zl0 % n1), j0 = (int)((zl0 / n1) % n2), k0 = (int)(zl0 / ((long long)n1 * n2));
-  if (DEND) live = live && dend_valid(pb, i0, j0, k0);
+  const bool inbox = z < (long long)n1 * n2 * n3;
+  const long long zl = inbox ? z : 0;
+  const int i0 = (int)(zl % n1), j0 = (int)((zl / n1) % n2), k0 = (int)(zl / ((long long)n1 * n2));
+  const bool live = inbox && (!DEND || dend_valid(pb, i0, j0, k0));
+  // The state and the (i, j) record of a box position are read whether or not
+  // it is a zone of its level (storage is the whole box), so these loads need
+  // not wait for the level tables; indices that depend on levels use a zone.
   const int i = live ? i0 : 0, j = live ? j0 : 0, k = live ? k0 : 0;
-  const long long zl = zid3(pb, i, j, k);
   DevClock *ck = a.clock;
   const bool active = ck->active;
   const unsigned long long seen = min_key_seen(&ck->mindt);
@@ -909,7 +920,7 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   zone_state<R>(a, in, zl, TAB, gam, q);
   if (stage == 1) zone_state<R>(a, sp, zl, TAB, gam, qn);
   else for (int t = 0; t < 8; t++) qn[t] = q[t];
-  const ZRec rc = rec3(pb, i, j);
+  const ZRec rc = rec3(pb, i0, j0);
   double gcov[3], gcon[3];
   for (int d = 0; d < 3; d++) { gcov[d] = rc[MDS_ZS_GCOV0 + d]; gcon[d] = rc[MDS_ZS_GCON + d]; }
   // geometric sources (rad_fluid.c:195-214): the 11 non-zero conn[l][m][n]
@@ -941,24 +952,113 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const int lkp = DEND ? dend_lk(pb, i, j + 1) : 0, lkm = DEND ? dend_lk(pb, i, j - 1) : 0;
   const int njp = li > lo ? 1 << (li - lo) : 1;
   const int jp0 = rescale(j, li, lo);
+  const int lq0 = DEND ? dend_lk(pb, i + 1, jp0) : 0;
   const long long f0 = fid3(pb, 0, i, j, k);
-  if (MONO) {
-    // gravity.c:415-437: mass flux per area through the finer outer faces
-    const double gz = a.grav[i];
-    const double xl = pb.startx[0] + i * pb.dx[0];
-    double vr = 0.0, area = 0.0;
+  const long long f1 = fid3(pb, 1, i, j, k), f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
+  // Face fluxes: FP64 ones carry their area; FP32 ones are per unit area and
+  // take the FP64 area of the record the flux kernel used, (fi, fj) of the
+  // face, here forming the same product that kernel would have stored.
+  // The slot base is uniform and the face's byte offset is 32-bit (one slot's
+  // array stays below 4 GiB, checked at set-up), so a load needs one VGPR of
+  // address, shared by the slots of a face.
+  auto load = [&](int D, int v, long long f) -> R {
+    const char *base = (const char *)a.F[D] + (size_t)v * (size_t)a.nf[D] * sizeof(R);
+    return *(const R *)(base + (unsigned)f * (unsigned)sizeof(R));
+  };
+  auto scaled = [&](R x, double area) -> double {
+    if constexpr (sizeof(R) == 8) return x;
+    else return area * (double)x;
+  };
+  auto flux = [&](int D, int v, long long f, double area) -> double { return scaled(load(D, v, f), area); };
+  const double ain = rc[MDS_ZS_AREA0], a1 = rc[MDS_ZS_AREA1], a2 = rc[MDS_ZS_AREA2];
+  const double a1p = sizeof(R) == 8 ? 0.0 : rec3(pb, i, j + 1)[MDS_ZS_AREA1];
+  // flux divergence per slot, in the reference's order (step.c:398-438):
+  // outer radial faces, inner radial, outer theta, inner theta, phi.  Each
+  // face's slots are read together; a zone whose neighbours all share its
+  // levels (every zone of a uniform grid) reads all of its faces at once.
+  constexpr int NSLOT = TAB ? 6 : 5;
+  double divF[NSLOT];
+  double vr = 0.0, varea = 0.0;                 // outer mass flux and area (monopole)
+  const bool simple = !DEND || (njp == 1 && lq0 == lk0 && lkp == lk0 && lkm == lk0);
+  if (simple) {
+    const long long fo = fid3(pb, 0, i + 1, jp0, k), ft = fid3(pb, 1, i, j + 1, k);
+    const double ao = rec3(pb, i + 1, jp0)[MDS_ZS_AREA0];
+    R x[NSLOT][6];
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) {
+      x[v][0] = load(0, v, fo);
+      x[v][1] = load(0, v, f0);
+      x[v][2] = load(1, v, ft);
+      x[v][3] = load(1, v, f1);
+      x[v][4] = load(2, v, f2p);
+      x[v][5] = load(2, v, f2);
+    }
+    // issue the zone's face loads together, ahead of their use
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) {
+      const double fro = scaled(x[v][0], ao);
+      double d = 0.0;
+      d += fro;
+      d -= scaled(x[v][1], ain);
+      d += scaled(x[v][2], a1p);
+      d -= scaled(x[v][3], a1);
+      d += scaled(x[v][4], a2) - scaled(x[v][5], a2);
+      divF[v] = d;
+      if (v == 0) { vr += fro; varea += ao + 1.0e-16; }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) divF[v] = 0.0;
     for (int jj = jp0; jj < jp0 + njp; jj++) {
       const int l = DEND ? dend_lk(pb, i + 1, jj) : 0;
       const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
       const double ar = rec3(pb, i + 1, jj)[MDS_ZS_AREA0];
       for (int kk = k * nkp; kk < k * nkp + nkp; kk++) {
-        vr += a.F[0][fid3(pb, 0, i + 1, jj, kk)];
-        area += ar + 1.0e-16;
+        const long long f = fid3(pb, 0, i + 1, jj, kk);
+#pragma unroll
+        for (int v = 0; v < NSLOT; v++) {
+          const double fro = flux(0, v, f, ar);
+          divF[v] += fro;
+          if (v == 0) { vr += fro; varea += ar + 1.0e-16; }
+        }
       }
     }
-    vr /= area;
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) divF[v] -= flux(0, v, f0, ain);
+    // outer theta face (step.c:411-423)
+    if (lkp > lk0) {
+      const long long f = fid3(pb, 1, i, j + 1, k);
+#pragma unroll
+      for (int v = 0; v < NSLOT; v++) divF[v] += 0.5 * flux(1, v, f, a1p);
+    } else if (lkp < lk0) {
+      const long long fa = fid3(pb, 1, i, j + 1, 2 * k), fb = fid3(pb, 1, i, j + 1, 2 * k + 1);
+#pragma unroll
+      for (int v = 0; v < NSLOT; v++) divF[v] += flux(1, v, fa, a1p) + flux(1, v, fb, a1p);
+    } else {
+      const long long f = fid3(pb, 1, i, j + 1, k);
+#pragma unroll
+      for (int v = 0; v < NSLOT; v++) divF[v] += flux(1, v, f, a1p);
+    }
+    // inner theta face (step.c:425-433)
+    if (lkm < lk0) {
+      const long long fa = fid3(pb, 1, i, j, 2 * k), fb = fid3(pb, 1, i, j, 2 * k + 1);
+#pragma unroll
+      for (int v = 0; v < NSLOT; v++) divF[v] -= 0.5 * flux(1, v, fa, a1) + 0.5 * flux(1, v, fb, a1);
+    } else {
+#pragma unroll
+      for (int v = 0; v < NSLOT; v++) divF[v] -= flux(1, v, f1, a1);
+    }
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) divF[v] += flux(2, v, f2p, a2) - flux(2, v, f2, a2);
+  }
+  if (MONO) {
+    // gravity.c:415-437: mass flux per area through the finer outer faces
+    const double gz = a.grav[i];
+    const double xl = pb.startx[0] + i * pb.dx[0];
+    vr /= varea;
     src[2] += q[0] * gz;
-    const double vl = a.F[0][f0] / (rc[MDS_ZS_AREA0] + 1.0e-16);
+    const double vl = flux(0, 0, f0, ain) / (ain + 1.0e-16);
     const double A = (vr - vl) / pb.dx[0];
     const double B = vl - A * xl;
     const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
@@ -972,29 +1072,9 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const double lim_src = dt_source_control_x64<3>(qn[1], &qn[2], U, src);
   const double vol = rc[MDS_ZS_VOLUME];
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
-  const int nslot = TAB ? 6 : 5;
-  const long long f1 = fid3(pb, 1, i, j, k), f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
-  for (int v = 0; v < nslot; v++) {
-    const size_t o0 = (size_t)v * a.nf[0], o1 = (size_t)v * a.nf[1], o2 = (size_t)v * a.nf[2];
-    double divF = 0.0;
-    // outer radial faces (step.c:398-408)
-    for (int jj = jp0; jj < jp0 + njp; jj++) {
-      const int l = DEND ? dend_lk(pb, i + 1, jj) : 0;
-      const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
-      for (int kk = k * nkp; kk < k * nkp + nkp; kk++) divF += a.F[0][o0 + fid3(pb, 0, i + 1, jj, kk)];
-    }
-    divF -= a.F[0][o0 + f0];
-    // outer theta face (step.c:411-423)
-    if (lkp > lk0) divF += 0.5 * a.F[1][o1 + fid3(pb, 1, i, j + 1, k)];
-    else if (lkp < lk0)
-      divF += a.F[1][o1 + fid3(pb, 1, i, j + 1, 2 * k)] + a.F[1][o1 + fid3(pb, 1, i, j + 1, 2 * k + 1)];
-    else divF += a.F[1][o1 + fid3(pb, 1, i, j + 1, k)];
-    // inner theta face (step.c:425-433)
-    if (lkm < lk0)
-      divF -= 0.5 * a.F[1][o1 + fid3(pb, 1, i, j, 2 * k)] + 0.5 * a.F[1][o1 + fid3(pb, 1, i, j, 2 * k + 1)];
-    else divF -= a.F[1][o1 + f1];
-    divF += a.F[2][o2 + f2p] - a.F[2][o2 + f2];
-    const double ddt = (divF / vol) - src[v];
+#pragma unroll
+  for (int v = 0; v < NSLOT; v++) {
+    const double ddt = (divF[v] / vol) - src[v];
     U[v] = ((pf * U[v]) + (phf * Uh[v])) - ((ff * dt) * ddt);
   }
   double ro, uo, vo[3];
