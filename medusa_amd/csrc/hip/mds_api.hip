@@ -283,6 +283,8 @@ extern "C" void *mds_stream(mds_ctx *ctx) { return ctx ? (void *)ctx->st : nullp
 extern "C" int mds_destroy(mds_ctx *c)
 {
   if (!c) return MDS_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  if (c->s3.side) hipStreamSynchronize(c->s3.side);
   drop_graphs(c);
   for (int k = 0; k < 6; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
   if (c->eos_alloc) hipFree(c->eos_alloc);
@@ -292,6 +294,9 @@ extern "C" int mds_destroy(mds_ctx *c)
   if (c->s3.G) hipFree(c->s3.G);
   if (c->s3.CS) hipFree(c->s3.CS);
   if (c->s3.ms) hipFree(c->s3.ms);
+  if (c->s3.part) hipFree(c->s3.part);
+  for (int k = 0; k < 5; k++) if (c->s3.ev[k]) hipEventDestroy(c->s3.ev[k]);
+  if (c->s3.side) hipStreamDestroy(c->s3.side);
   if (c->s3.grav) hipFree(c->s3.grav);
   for (void *g : c->geo_allocs) hipFree(g);
   if (c->dprob) hipFree(c->dprob);
@@ -465,7 +470,12 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       std::vector<unsigned char> lj(N1), lk((size_t)N1 * N2, 0);
       auto lg = [](int f) { int l = 0; while (f >>= 1) l++; return l; };
       for (int q = 0; q < N1; q++) lj[q] = (unsigned char)lg(g->dj[q]);
-      for (size_t q = 0; q < lk.size(); q++) lk[q] = g->dk[q] > 0 ? (unsigned char)lg(g->dk[q]) : 0;
+      // the device table is i fastest (a wavefront runs along i)
+      for (int i = 0; i < N1; i++)
+        for (int j = 0; j < N2; j++) {
+          const int f = g->dk[(size_t)i * N2 + j];
+          lk[(size_t)j * N1 + i] = f > 0 ? (unsigned char)lg(f) : 0;
+        }
       int rc1 = upload_geo(c, lj.data(), lj.size(), (const void **)&d.lj);
       if (rc1) return rc1;
       rc1 = upload_geo(c, lk.data(), lk.size(), (const void **)&d.lk);
@@ -545,6 +555,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->sph3) {
     const DevProb &h = c->hprob[0];
     Sph3Launch &L = c->s3;
+    HIPCHK(c, hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+    for (int k = 0; k < 5; k++) HIPCHK(c, hipEventCreateWithFlags(&L.ev[k], hipEventDisableTiming));
     L.fp32 = c->precision == MDS_FP32_BULK;
     L.tab = c->tab;
     L.mono = c->mono;
@@ -566,6 +578,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     }
     if (c->mono) {
       HIPCHK(c, hipMalloc(&L.ms, sizeof(double) * h.n[0]));
+      L.npart = (int)(((long long)h.n[1] * h.n[2] + 4 * MDS_SHELL_ROWS - 1) / (4 * MDS_SHELL_ROWS));
+      HIPCHK(c, hipMalloc(&L.part, sizeof(double) * (size_t)L.npart * h.n[0]));
       HIPCHK(c, hipMalloc(&L.grav, sizeof(double) * h.n[0]));
     }
     const mds_grid *g = probs[0].grid;

@@ -50,7 +50,7 @@ struct DevProb {
                            // field-major, zone[field][j + NG][i + NG] (MDS_ZS_*)
   const ZoneF32 *z32;      // spherical, FP32-bulk: n[0]+2NG records
   // dendritic 3D spherical grids (null otherwise): log2 of the theta and phi
-  // coarsening, ghost-extended: lj[i + NG], lk[(i + NG) * (n[1] + 2NG) + j + NG]
+  // coarsening, ghost-extended: lj[i + NG], lk[(j + NG) * (n[0] + 2NG) + i + NG]
   const unsigned char *lj, *lk;
   // theta (0) and phi (1) moments per level l at l * (n + 2NG) * 3 (alpha,
   // beta, Gamma per zone) and FP32 coefficients at l * (n + 2NG) * MDS_RC_COUNT
@@ -62,7 +62,7 @@ struct DevProb {
 __device__ __forceinline__ int dend_lj(const DevProb &pb, int i) { return pb.lj ? pb.lj[i + MDS_NG] : 0; }
 __device__ __forceinline__ int dend_lk(const DevProb &pb, int i, int j)
 {
-  return pb.lk ? pb.lk[(size_t)(i + MDS_NG) * (pb.n[1] + 2 * MDS_NG) + (j + MDS_NG)] : 0;
+  return pb.lk ? pb.lk[(size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)] : 0;
 }
 __device__ __forceinline__ bool dend_valid(const DevProb &pb, int i, int j, int k)
 {
@@ -125,11 +125,16 @@ struct Sph3Launch {
   long long nf[3];         // faces per direction
   void *P, *G, *CS;        // table EOS arrays (state precision)
   double *ms, *grav;       // shell masses and monopole field (n1 each)
+  double *part;            // shell-mass parts [npart][n1]
+  int npart;
+  hipStream_t side;        // second stream of a stage (default priority)
+  hipEvent_t ev[5];        // fork/join events of a stage (no timing)
   unsigned char *flags;    // shock tags
   int dend;                // dendritic grid
   const int4 *irr[2];      // dendritic: radial and theta faces whose pencils cross a level change
   long long nirr[2];       // (i, j, k, 0) per face; k is the face's own phi index
 };
 
+#define MDS_SHELL_ROWS 32  // (j, k) rows per wavefront in the 3D shell-mass pass
 #define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D stage sweep
 #define MDS_SEG1D2 56      // output zones per 64-lane wave in the 1D two-stage sweep

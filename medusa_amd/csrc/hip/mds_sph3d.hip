@@ -104,13 +104,13 @@ MDS_DEV const float *rc_of(const DevProb &pb, int D, int lev)
 }
 
 // Map a stencil zone that leaves the grid in direction D onto the zone the
-// ghost fill copies there; sgn gets the velocity sign flips.  Counts are those
-// of the zone's level.
-MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3])
+// ghost fill copies there; sgn gets the velocity sign flips.  n is the zone
+// count along D at the zone's level; nkm (theta across the pole only) the phi
+// count of the mirrored row.
+MDS_DEV void map_zone_n(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3], int n, int nkm)
 {
   sgn[0] = sgn[1] = sgn[2] = 1.0;
   int *c = D == 0 ? &i : (D == 1 ? &j : &k);
-  const int n = D == 0 ? pb.n[0] : (D == 1 ? nj_of(pb, i) : nk_of(pb, i, j));
   if (*c >= 0 && *c < n) return;
   const int side = *c >= n;
   const int code = side ? pb.bc_hi[D] : pb.bc_lo[D];
@@ -123,12 +123,20 @@ MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double s
     sgn[D] = -1.0;
   } else if (code == MDS_BC_SPHERICAL_ORIGIN && D == 1) {
     *c = side ? 2 * n - 1 - ig : -1 - ig;
-    const int nk = nk_of(pb, i, j);
-    k = (k + nk / 2) % nk;
+    k = (k + nkm / 2) % nkm;
     sgn[1] = sgn[2] = -1.0;
   } else {
     *c = side ? n - 1 : 0;
   }
+}
+
+// map_zone_n with the counts of the zone's level read from the level tables
+MDS_DEV void map_zone(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3])
+{
+  const int n = D == 0 ? pb.n[0] : (D == 1 ? nj_of(pb, i) : nk_of(pb, i, j));
+  int nkm = 1;
+  if (D == 1 && (j < 0 || j >= n)) nkm = nk_of(pb, i, j < 0 ? -1 - j : 2 * n - 1 - j);
+  map_zone_n(pb, D, i, j, k, sgn, n, nkm);
 }
 
 // Any stencil zone of a dendritic grid, corners included: phi wrap within its
@@ -220,9 +228,8 @@ MDS_DEV double minmod(double a, double b)
 // gamma law too (a restricted radial pencil carries v^3 in that slot).
 // hlle: transverse face next to a tagged zone.  Returns isshock.  F is area
 // weighted in FP64 and per unit area in FP32 (zero on faces of area < 1e-16).
-template <class R, int D, int TAB, int GREC>
-MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
-                      const ZRec &rc, bool hlle, double F[6], double &sig)
+template <int D>
+MDS_DEV FaceGeo<double> face_geo(const ZRec &rc)
 {
   FaceGeo<double> fg;
   if (D == 0) {
@@ -238,6 +245,14 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
     fg.scale = rc[MDS_ZS_SCALE0 + 2];
     for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
   }
+  return fg;
+}
+
+template <class R, int D, int TAB, int GREC>
+MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
+                      const ZRec &rc, bool hlle, double F[6], double &sig)
+{
+  const FaceGeo<double> fg = face_geo<D>(rc);
   int isshock = 0;
   sig = 0.0;
   for (int v = 0; v < 6; v++) F[v] = 0.0;
@@ -378,25 +393,12 @@ MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8
 }
 
 // Riemann solve of one face from its reconstructed states (rho, u, v1..v3,
-// Ye, P, Gamma_1); face geometry from the record of the face's zone.  F as
-// in face_flux.
+// Ye, P, Gamma_1) and its geometry (face_geo of the face's zone record).  F
+// as in face_flux.
 template <class R, int D, int TAB>
-MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const ZRec &rc, bool hlle, double F[6], double &sig)
+MDS_DEV int face_riemann(const R Lq[8], const R Rq[8], const FaceGeo<double> &fg, bool hlle, double F[6],
+                         double &sig)
 {
-  FaceGeo<double> fg;
-  if (D == 0) {
-    fg.area = rc[MDS_ZS_AREA0];
-    fg.scale = rc[MDS_ZS_SCALEF0];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF0 + d];
-  } else if (D == 1) {
-    fg.area = rc[MDS_ZS_AREA1];
-    fg.scale = rc[MDS_ZS_SCALEF1];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOVF1 + d];
-  } else {
-    fg.area = rc[MDS_ZS_AREA2];
-    fg.scale = rc[MDS_ZS_SCALE0 + 2];
-    for (int d = 0; d < 3; d++) fg.gcov[d] = rc[MDS_ZS_GCOV0 + d];
-  }
   int isshock = 0;
   sig = 0.0;
   for (int v = 0; v < 6; v++) F[v] = 0.0;
@@ -431,6 +433,16 @@ MDS_DEV bool tagged(const Sph3Args &a, int i, int j, int k)
   const DevProb &pb = a.pb;
   if (i < 0 || j < 0 || k < 0 || i >= pb.n[0] || j >= nj_of(pb, i) || k >= nk_of(pb, i, j)) return true;
   return a.flags[zid3(pb, i, j, k)] == 0;
+}
+
+// tagged() for an i inside the grid, with the row counts supplied; the tag
+// byte is read unconditionally (zone (i, 0, 0) for a ghost) so that both
+// zones' reads of a face go out together
+MDS_DEV bool tagged_n(const Sph3Args &a, int i, int j, int k, int nj, int nk)
+{
+  const bool in = j >= 0 && k >= 0 && j < nj && k < nk;
+  const unsigned char f = a.flags[zid3(a.pb, i, in ? j : 0, in ? k : 0)];
+  return !in || f == 0;
 }
 
 // radial shock tags on zones i, i-1, i-2 at their levels (step.c:1190-1198)
@@ -608,27 +620,37 @@ __global__ __launch_bounds__(256) void sph3_eos_kernel(Sph3Args a, StateArrays<R
 // monopole gravity on radial shells: masses, then enclosed mass and the
 // zone-centred field (gravity.c:229-233, 420-450)
 // ---------------------------------------------------------------------------
+// Shell masses in two passes, both in a fixed order: thread i of wave w of
+// block (x, y) sums V rho over rows r = (4 y + w) MDS_SHELL_ROWS + t of shell
+// i (row r = (j, k) = (r % n2, r / n2): a wavefront reads 64 consecutive
+// shells of one row, coalesced), the block adds its four waves in order into
+// part[y][i]; the field kernel adds the parts of a shell in order.
 template <class R>
-__global__ __launch_bounds__(256) void sph3_shell_mass(Sph3Args a, StateArrays<R> in, double *ms)
+__global__ __launch_bounds__(256) void sph3_shell_part(Sph3Args a, StateArrays<R> in, double *part)
 {
   const DevProb &pb = a.pb;
-  const int i = blockIdx.x;
-  const int n2 = pb.n[1], n3 = pb.n[2];
-  const int nj = nj_of(pb, i);
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int r0 = (blockIdx.y * 4 + w) * MDS_SHELL_ROWS;
+  const int r1 = min(r0 + MDS_SHELL_ROWS, n2 * n3);
   double s = 0.0;
-  for (int t = threadIdx.x; t < n2 * n3; t += blockDim.x) {
-    const int j = t % n2, k = t / n2;
-    if (j >= nj || k >= nk_of(pb, i, j)) continue;
-    s += rec3(pb, i, j)[MDS_ZS_VOLUME] * (double)in.v[0][zid3(pb, i, j, k)];
+  if (i < n1) {
+    const int nj = nj_of(pb, i);
+#pragma unroll 8
+    for (int r = r0; r < r1; r++) {
+      // every (i, j < n2, k < n3) is inside the stored box: read, then select
+      const int j = r % n2, k = r / n2;
+      const double v = rec3(pb, i, j)[MDS_ZS_VOLUME];
+      const double rho = (double)in.v[0][zid3(pb, i, j, k)];
+      const bool zone = j < nj && k < nk_of(pb, i, j);
+      s += zone ? v * rho : 0.0;
+    }
   }
-  __shared__ double sh[256];
-  sh[threadIdx.x] = s;
+  __shared__ double sh[4][64];
+  sh[w][lane] = s;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) ms[i] = sh[0];
+  if (w == 0 && i < n1) part[(size_t)blockIdx.y * n1 + i] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
 }
 
 // g at radial face i from the mass inside it: -G M / r^2 dr/dx (gravity.c:420-425)
@@ -650,20 +672,36 @@ MDS_DEV double face_gravity(const DevProb &pb, int i, double mr)
   return ((-GN * mr) / (r * r)) * drdx;
 }
 
-// One block: the enclosed masses by a sequential running sum (thread 0, in
-// place), then every shell's field at its volume centroid in parallel.
-__global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, double *ms, double *grav)
+// One block: each shell's mass from its parts, the enclosed masses by a
+// sequential running sum (thread 0, over 1024 shells at a time in LDS), then
+// every shell's field at its volume centroid in parallel.
+__global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, const double *part, int npart, double *ms,
+                                                         double *grav)
 {
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0];
-  if (threadIdx.x == 0) {
-    double mr = 0.0;
-    for (int i = 0; i < n1; i++) {
-      mr = mr + ms[i];
-      ms[i] = mr;
+  __shared__ double sm[1024];
+  double mr = 0.0;                                   // thread 0: running enclosed mass
+  for (int base = 0; base < n1; base += 1024) {
+    const int i = base + (int)threadIdx.x;
+    double m = 0.0;
+    if (i < n1) {
+#pragma unroll 16
+      for (int c = 0; c < npart; c++) m += part[(size_t)c * n1 + i];
     }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int cnt = min(1024, n1 - base);
+      for (int t = 0; t < cnt; t++) {
+        mr = mr + sm[t];
+        sm[t] = mr;
+      }
+    }
+    __syncthreads();
+    if (i < n1) ms[i] = sm[threadIdx.x];
+    __syncthreads();
   }
-  __syncthreads();
   for (int i = threadIdx.x; i < n1; i += blockDim.x) {
     const double ge = i == 0 ? 0.0 : face_gravity(pb, i, ms[i - 1]);
     const double gn = face_gravity(pb, i + 1, ms[i]);
@@ -684,39 +722,8 @@ __global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, double *ms,
 // previous wavefront for theta and phi faces (a 64 x 8 block of zones in
 // (i, j) or (i, k), 7 rows of faces).
 // ---------------------------------------------------------------------------
-template <class R, int D, int DEND>
-MDS_DEV bool face_is_regular(const DevProb &pb, int fi, int fj, int fk, int &lev, int &dk)
-{
-  lev = 0;
-  dk = 1;
-  if (!DEND) return fi <= pb.n[0] && fj <= pb.n[1] && fk <= pb.n[2];
-  const int lj = dend_lj(pb, fi), nj = pb.n[1] >> lj;
-  if (D == 0) {
-    if (fj >= nj) return false;
-    const int lk = dend_lk(pb, fi, fj);
-    if (fk >= (pb.n[2] >> lk)) return false;
-    for (int s = -2; s < 2; s++)
-      if (dend_lj(pb, fi + s) != lj || dend_lk(pb, fi + s, fj) != lk) return false;
-    return true;
-  } else if (D == 1) {
-    if (fj > nj) return false;
-    const int lk = dend_lk(pb, fi, fj);
-    if (fk >= (pb.n[2] >> lk)) return false;
-    for (int s = -2; s < 2; s++)
-      if (dend_lk(pb, fi, fj + s) != lk) return false;
-    lev = lj;
-    return true;
-  }
-  if (fj >= nj) return false;
-  const int lk = dend_lk(pb, fi, fj);
-  if (fk > (pb.n[2] >> lk)) return false;
-  lev = lk;
-  dk = 1 << lk;
-  return true;
-}
-
 template <class R, int D, int TAB, int DEND>
-__global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
+__global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> in)
 {
   const DevProb &pb = a.pb;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -748,20 +755,57 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
     zk = blockIdx.z * 7 + w - 1;
     has_face = w > 0 && zi < pb.n[0] && zk <= pb.n[2];
   }
-  int lev = 0, dk = 1;
-  bool live = has_face && face_is_regular<R, D, DEND>(pb, zi, zj, zk, lev, dk);
-  // levels of the zone's own stencil (for the reconstruction)
-  int zlev = 0;
+  // Level values of this thread, read together up front (dendritic grids):
+  // D = 0: lj and lk of shells zi-2 .. zi+1 in row zj; D = 1: lj of shell zi
+  // and lk of its rows zj-2 .. zj+1; D = 2: lj of shell zi and lk of row zj.
+  // Indices are clamped to the ghost-extended tables.  These values stand in
+  // for every level lookup of the face and of the zone stencil below (the
+  // tables reflect in their ghost rows, so a mirrored row has the level of
+  // the row it mirrors).
+  int Lj[4] = {0, 0, 0, 0}, Lk[4] = {0, 0, 0, 0};
   if (DEND) {
-    const int ci = zi < 0 ? 0 : (zi > pb.n[0] ? pb.n[0] : zi);
-    if (D == 1) zlev = dend_lj(pb, ci);
-    else if (D == 2) {
-      const int cj = zj < (pb.n[1] >> dend_lj(pb, ci)) ? zj : 0;
-      zlev = dend_lk(pb, ci, cj);
+    auto ci_of = [&](int i) { return i < -MDS_NG ? -MDS_NG : (i > pb.n[0] + MDS_NG - 1 ? pb.n[0] + MDS_NG - 1 : i); };
+    auto cj_of = [&](int j) { return j < -MDS_NG ? -MDS_NG : (j > pb.n[1] + MDS_NG - 1 ? pb.n[1] + MDS_NG - 1 : j); };
+    if (D == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int ci = ci_of(zi - 2 + t);
+        Lj[t] = dend_lj(pb, ci);
+        Lk[t] = dend_lk(pb, ci, cj_of(zj));
+      }
+    } else {
+      const int ci = ci_of(zi);
+      Lj[0] = dend_lj(pb, ci);
+      if (D == 1) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) Lk[t] = dend_lk(pb, ci, cj_of(zj - 2 + t));
+      } else {
+        Lk[0] = dend_lk(pb, ci, cj_of(zj));
+      }
     }
   }
+  // the face (zi, zj, zk) is done here if its 4-zone pencil stays on one level
+  int lev = 0, dk = 1;
+  bool live = has_face;
+  if (!DEND) {
+    live = live && zi <= pb.n[0] && zj <= pb.n[1] && zk <= pb.n[2];
+  } else if (D == 0) {
+    const int lj = Lj[2], lk = Lk[2];
+    live = live && zj < (pb.n[1] >> lj) && zk < (pb.n[2] >> lk);
+#pragma unroll
+    for (int t = 0; t < 4; t++) live = live && Lj[t] == lj && Lk[t] == lk;
+  } else if (D == 1) {
+    const int lk = Lk[2];
+    live = live && zj <= (pb.n[1] >> Lj[0]) && zk < (pb.n[2] >> lk);
+#pragma unroll
+    for (int t = 0; t < 4; t++) live = live && Lk[t] == lk;
+    if (live) lev = Lj[0];
+  } else {
+    live = live && zj < (pb.n[1] >> Lj[0]) && zk <= (pb.n[2] >> Lk[0]);
+    if (live) { lev = Lk[0]; dk = 1 << Lk[0]; }
+  }
   // this zone exists (interior, or the one ghost zone at either end along D
-  // that a face needs); other threads read a safe zone and own no face
+  // that a face needs); other threads read nothing and own no face
   bool zvalid;
   if (D == 0) {
     zvalid = zi >= -1 && zi <= pb.n[0] && zj < pb.n[1] && zk < pb.n[2];
@@ -769,31 +813,63 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
     zvalid = zi < pb.n[0] && zj < pb.n[1] + (D == 1) && zk < pb.n[2] + (D == 2) && zj >= -1 && zk >= -1;
   }
   if (DEND && zvalid) {
-    const int ci = zi < 0 ? 0 : (zi >= pb.n[0] ? pb.n[0] - 1 : zi);
-    const int nj = pb.n[1] >> dend_lj(pb, ci);
     if (D == 0) {
-      zvalid = zj < nj && zk < (pb.n[2] >> dend_lk(pb, ci, zj));
+      const int t = zi < 0 ? 3 : (zi >= pb.n[0] ? 1 : 2);       // shell clamped into the grid
+      zvalid = zj < (pb.n[1] >> Lj[t]) && zk < (pb.n[2] >> Lk[t]);
     } else if (D == 1) {
-      zvalid = zj <= nj && zk < (pb.n[2] >> dend_lk(pb, ci, zj < 0 ? 0 : (zj >= nj ? nj - 1 : zj)));
+      const int nj = pb.n[1] >> Lj[0];
+      const int t = zj < 0 ? 3 : (zj >= nj ? 1 : 2);             // row clamped into the level
+      zvalid = zj <= nj && zk < (pb.n[2] >> Lk[t]);
     } else {
-      zvalid = zj < nj && zk <= (pb.n[2] >> dend_lk(pb, ci, zj));
+      zvalid = zj < (pb.n[1] >> Lj[0]) && zk <= (pb.n[2] >> Lk[0]);
     }
   }
+  // level of the zone's own stencil (for the reconstruction)
+  int zlev = 0;
+  if (DEND && D == 1) zlev = Lj[0];
+  if (DEND && D == 2 && zvalid) zlev = Lk[0];
   const int bi = zvalid ? zi : 0, bj = zvalid ? zj : 0, bk = zvalid ? zk : 0;
   double q3[3][8];
-  for (int s = 0; s < 3; s++) {
-    int i = bi, j = bj, k = bk;
-    if (D == 0) i += s - 1;
-    else if (D == 1) j += s - 1;
-    else k += s - 1;
-    double sgn[3];
-    map_zone(pb, D, i, j, k, sgn);
-    zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
-    for (int d = 0; d < 3; d++) q3[s][2 + d] *= sgn[d];
+  if (zvalid) {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      int i = bi, j = bj, k = bk;
+      int n = pb.n[0], nkm = 1;
+      if (D == 0) {
+        i += s - 1;
+      } else if (D == 1) {
+        j += s - 1;
+        n = pb.n[1] >> Lj[0];
+        nkm = pb.n[2] >> Lk[s + 1];
+      } else {
+        k += s - 1;
+        n = pb.n[2] >> Lk[0];
+      }
+      double sgn[3];
+      map_zone_n(pb, D, i, j, k, sgn, n, nkm);
+      zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
+      for (int d = 0; d < 3; d++) q3[s][2 + d] *= sgn[d];
+    }
+  } else {
+    for (int s = 0; s < 3; s++)
+      for (int v = 0; v < 8; v++) q3[s][v] = 1.0;
+  }
+  // the face's geometry and transverse-shock switch, read with the stencil
+  const int fi = live ? zi : 0, fj = live ? zj : 0, fk = live ? zk : 0;
+  const ZRec rc = rec3(pb, fi, fj);
+  const FaceGeo<double> fg = face_geo<D>(rc);
+  bool hlle = false;
+  if (D > 0) {
+    // transverse_shock (fluid.c:189-210) on the zones either side of the face
+    const int nj = pb.n[1] >> Lj[0];
+    if (D == 1)
+      hlle = tagged_n(a, fi, fj, fk, nj, pb.n[2] >> Lk[2]) | tagged_n(a, fi, fj - 1, fk, nj, pb.n[2] >> Lk[1]);
+    else
+      hlle = tagged_n(a, fi, fj, fk, nj, pb.n[2] >> Lk[0]) | tagged_n(a, fi, fj, fk - 1, nj, pb.n[2] >> Lk[0]);
   }
   const int zd = D == 0 ? bi : (D == 1 ? bj : bk);
   // polar-axis rows (last theta row): first-order states, exchanged as zone values
-  const bool polar = (D == 2) && zvalid && (bj == (pb.n[1] >> dend_lj(pb, bi)) - 1);
+  const bool polar = (D == 2) && zvalid && (bj == (pb.n[1] >> Lj[0]) - 1);
   R al[8], ar[8];
   if (polar) {
     for (int v = 0; v < 8; v++) al[v] = ar[v] = (R)q3[1][v];
@@ -810,10 +886,6 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
     __syncthreads();
     for (int v = 0; v < 8; v++) Lq[v] = w > 0 ? sh[v][w - 1][lane] : ar[v];
   }
-  const int fi = live ? zi : 0, fj = live ? zj : 0, fk = live ? zk : 0;
-  const ZRec rc = rec3(pb, fi, fj);
-  bool hlle = false;
-  if (D > 0) hlle = tagged(a, fi, fj, fk) || tagged(a, fi, fj - (D == 1), fk - (D == 2));
   if (polar && live) {
     // step.c:1299-1397: velocities of zones k-1 and k through Cartesian components
     double vl[3] = {(double)Lq[2], (double)Lq[3], (double)Lq[4]}, vr[3] = {(double)al[2], (double)al[3], (double)al[4]};
@@ -821,7 +893,7 @@ __global__ __launch_bounds__(512) void sph3_flux(Sph3Args a, StateArrays<R> in)
     for (int d = 0; d < 3; d++) { Lq[2 + d] = (R)vl[d]; al[2 + d] = (R)vr[d]; }
   }
   double sig, F[6];
-  const int isshock = face_riemann<R, D, TAB>(Lq, al, rc, hlle, F, sig);
+  const int isshock = face_riemann<R, D, TAB>(Lq, al, fg, hlle, F, sig);
   if (live && active) {
     store_flux<R>(a, D, fid3(pb, D, fi, fj, fk), F, TAB ? 6 : 5);
     if (D == 0 && isshock) {
@@ -1111,14 +1183,40 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
 // ---------------------------------------------------------------------------
 // launcher: one RK stage
 // ---------------------------------------------------------------------------
+// One RK stage.  Work that does not depend on the kernels beside it runs on
+// a second stream (L.side), joined through events:
+//   main  [e0] reset_flags, eos [e1] flux0 [e3] {e2} flux1, flux2 {e4} update
+//   side  {e0} shell_part, shell_field {e1} flux_irr0 [e2] {e3} flux_irr1 [e4]
+// ([e] records, {e} waits).  The shell masses read only rho; the listed
+// level-change faces need the EOS arrays, and the theta ones the radial shock
+// tags of both radial kernels; the update needs everything.  Both streams
+// have the default priority.
 template <class R, int DEND>
-static void launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArrays<R> &sp,
-                         const StateArrays<R> &sph, int stage, hipStream_t st)
+static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArrays<R> &sp,
+                               const StateArrays<R> &sph, int stage, hipStream_t st)
 {
   const DevProb &pb = a.pb;
   const StateArrays<R> &in = stage == 0 ? sp : sph;
   const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
   const dim3 blk(256);
+  const bool irr = DEND && (L.nirr[0] > 0 || L.nirr[1] > 0);
+  const bool fork = L.mono || irr;
+  hipStream_t s2 = fork ? L.side : st;
+  hipError_t e = hipSuccess;
+  auto rec = [&](int k, hipStream_t s) { if (fork && e == hipSuccess) e = hipEventRecord(L.ev[k], s); };
+  auto wait = [&](hipStream_t s, int k) { if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, L.ev[k], 0); };
+  rec(0, st);
+  wait(s2, 0);
+  if (stage == 0)
+    hipLaunchKernelGGL(sph3_reset_flags, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, L.flags, nz);
+  if (L.tab) hipLaunchKernelGGL(sph3_eos_kernel<R>, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, a, in);
+  if (L.mono) {
+    hipLaunchKernelGGL(sph3_shell_part<R>, dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)L.npart), blk, 0, s2, a,
+                       in, L.part);
+    hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(1024), 0, s2, a, L.part, L.npart, L.ms, L.grav);
+  }
+  rec(1, st);
+  wait(s2, 1);
   const int nseg = (pb.n[0] + 1 + 62) / 63;
   const dim3 fgrid[3] = {dim3((unsigned)(((long long)nseg * pb.n[1] * pb.n[2] + 7) / 8)),
                          dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)((pb.n[1] + 1 + 6) / 7), (unsigned)pb.n[2]),
@@ -1132,14 +1230,20 @@ static void launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArra
 #define MDS_FLUX_IRR(D)                                                                                    \
   if (DEND && L.nirr[D] > 0) {                                                                             \
     const dim3 g((unsigned)((L.nirr[D] + 255) / 256));                                                    \
-    if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, st, a, in, L.irr[D], L.nirr[D]);   \
-    else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, st, a, in, L.irr[D], L.nirr[D]);         \
+    if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);   \
+    else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);         \
   }
-  MDS_FLUX(0)
   MDS_FLUX_IRR(0)
-  MDS_FLUX(1)
+  rec(2, s2);
+  MDS_FLUX(0)
+  rec(3, st);
+  wait(st, 2);
+  wait(s2, 3);
   MDS_FLUX_IRR(1)
+  rec(4, s2);
+  MDS_FLUX(1)
   MDS_FLUX(2)
+  wait(st, 4);
 #undef MDS_FLUX
 #undef MDS_FLUX_IRR
   const dim3 grd((unsigned)((nz + 255) / 256));
@@ -1147,6 +1251,7 @@ static void launch_stage(const Sph3Launch &L, const Sph3Args &a, const StateArra
   else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
   else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
   else hipLaunchKernelGGL((sph3_update<R, 0, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  return e;
 }
 
 template <class R>
@@ -1163,18 +1268,8 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
   for (int d = 0; d < 3; d++) { a.F[d] = L.F[d]; a.nf[d] = L.nf[d]; }
   a.P = L.P; a.G = L.G; a.CS = L.CS;
   a.grav = L.grav;
-  const StateArrays<R> &in = stage == 0 ? sp : sph;
-  const long long nz = (long long)pb.n[0] * pb.n[1] * pb.n[2];
-  const dim3 blk(256);
-  if (stage == 0)
-    hipLaunchKernelGGL(sph3_reset_flags, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, L.flags, nz);
-  if (L.tab) hipLaunchKernelGGL(sph3_eos_kernel<R>, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, a, in);
-  if (L.mono) {
-    hipLaunchKernelGGL(sph3_shell_mass<R>, dim3(pb.n[0]), blk, 0, st, a, in, L.ms);
-    hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(1024), 0, st, a, L.ms, L.grav);
-  }
-  if (L.dend) launch_stage<R, 1>(L, a, sp, sph, stage, st);
-  else launch_stage<R, 0>(L, a, sp, sph, stage, st);
+  const hipError_t e = L.dend ? launch_stage<R, 1>(L, a, sp, sph, stage, st) : launch_stage<R, 0>(L, a, sp, sph, stage, st);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 
