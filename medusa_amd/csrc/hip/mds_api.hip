@@ -293,6 +293,7 @@ extern "C" int mds_destroy(mds_ctx *c)
   if (c->s3.P) hipFree(c->s3.P);
   if (c->s3.G) hipFree(c->s3.G);
   if (c->s3.CS) hipFree(c->s3.CS);
+  if (c->s3.tg) hipFree(c->s3.tg);
   if (c->s3.ms) hipFree(c->s3.ms);
   if (c->s3.part) hipFree(c->s3.part);
   for (int k = 0; k < 5; k++) if (c->s3.ev[k]) hipEventDestroy(c->s3.ev[k]);
@@ -575,6 +576,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       HIPCHK(c, hipMalloc(&L.P, rs * (size_t)c->nzone));
       HIPCHK(c, hipMalloc(&L.G, rs * (size_t)c->nzone));
       HIPCHK(c, hipMalloc(&L.CS, rs * (size_t)c->nzone));
+      HIPCHK(c, hipMalloc(&L.tg, sizeof(unsigned short) * (size_t)c->nzone));
+      HIPCHK(c, hipMemset(L.tg, 0, sizeof(unsigned short) * (size_t)c->nzone));
     }
     if (c->mono) {
       HIPCHK(c, hipMalloc(&L.ms, sizeof(double) * h.n[0]));

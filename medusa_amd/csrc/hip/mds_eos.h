@@ -5,7 +5,9 @@
 // internal energy, the linear inverse in log T and the trilinear ln P and
 // Gamma_1.  The bracket is found by bisection over the T nodes instead of the
 // reference's walk from the previous temperature: e is strictly increasing in
-// T, so both give the unique node interval with E(j) < e <= E(j+1).  All of
+// T, so both give the unique node interval with E(j) < e <= E(j+1).  Where a
+// zone's previous bracket is kept (the 3D spherical stage), the search starts
+// there and bisects only when the bracket moved by more than one node.  All of
 // the inversion is FP64; the FP32-bulk variant interpolates ln P increments
 // and Gamma_1 in FP32.
 #pragma once
@@ -78,6 +80,42 @@ MDS_DEV void eos_invert(const DevEos &t, const EosW &w, double e, int &jt, doubl
   lt = tlo + (thi - tlo) * dtemp;
 }
 
+// eos_invert from a guessed bracket g (the zone's bracket of the previous
+// call, as the reference walks from the previous temperature): E(g) and
+// E(g+1) settle it, one node either side is tried next, and the bisection
+// settles anything else.  E is strictly increasing in T, so every route ends
+// on the same interval, with the same elo and ehi.  g returns the bracket.
+MDS_DEV void eos_invert_guess(const DevEos &t, const EosW &w, double e, int &g, double &dtemp, double &lt)
+{
+  const int top = t.nt - 2;
+  int c = g < 0 ? 0 : (g > top ? top : g);
+  const double eg = eos_e_at(t, w, c), eg1 = eos_e_at(t, w, c + 1);
+  int jt = -1;
+  double elo = 0.0, ehi = 0.0;
+  if (eg < e) {
+    if (c == top || eg1 >= e) {
+      jt = c; elo = eg; ehi = eg1;
+    } else {                                   // E(c+1) < e, c+1 <= top
+      const double eg2 = eos_e_at(t, w, c + 2);
+      if (c + 1 == top || eg2 >= e) { jt = c + 1; elo = eg1; ehi = eg2; }
+    }
+  } else if (c == 0) {                         // E(0) >= e
+    jt = 0; elo = eg; ehi = eg1;
+  } else {                                     // E(c) >= e
+    const double em = eos_e_at(t, w, c - 1);
+    if (em < e || c == 1) { jt = c - 1; elo = em; ehi = eg; }
+  }
+  if (jt < 0) {
+    eos_invert(t, w, e, g, dtemp, lt);
+    return;
+  }
+  g = jt;
+  const double tlo = t.lt0 + jt * t.dlt;
+  const double thi = tlo + t.dlt;
+  dtemp = (e - elo) / (ehi - elo);
+  lt = tlo + (thi - tlo) * dtemp;
+}
+
 // trilinear record at (jt, dtemp), reference summation order (eos_stuff.c:616-643)
 MDS_DEV double eos_tri_x64(const float *rec, const EosW &w, int jt, double dtemp)
 {
@@ -93,14 +131,20 @@ MDS_DEV double eos_tri_x64(const float *rec, const EosW &w, int jt, double dtemp
   return p;
 }
 
-// FP64 EOS fill of one zone: (rho, u, Ye) -> P, Gamma_1, cs, log10 T
+// FP64 EOS fill of one zone: (rho, u, Ye) -> P, Gamma_1, cs, log10 T.  With
+// guess non-null the bracket search starts from *guess and returns it there.
 MDS_DEV void eos_fill_x64(const DevEos &t, double rho, double u, double ye, double &press, double &gam1,
-                          double &cs, double &lt)
+                          double &cs, double &lt, int *guess = nullptr)
 {
   const EosW w = eos_weights(t, rho, ye);
   int jt;
   double dtemp;
-  eos_invert(t, w, u / rho, jt, dtemp, lt);
+  if (guess) {
+    eos_invert_guess(t, w, u / rho, *guess, dtemp, lt);
+    jt = *guess;
+  } else {
+    eos_invert(t, w, u / rho, jt, dtemp, lt);
+  }
   press = exp(eos_tri_x64(t.lnp, w, jt, dtemp));
   gam1 = eos_tri_x64(t.g1, w, jt, dtemp);
   cs = sqrt((gam1 * press) / rho);
@@ -125,14 +169,19 @@ MDS_DEV float eos_tri_f32(const float *rec, const EosW &w, int jt, double dtemp,
 }
 
 // FP32-bulk EOS fill: FP64 inversion, FP32 interpolation; P in FP64 from the
-// FP32 increment of ln P over an exact table value
+// FP32 increment of ln P over an exact table value.  guess as in eos_fill_x64.
 MDS_DEV void eos_fill_f32(const DevEos &t, double rho, double u, double ye, double &press, float &gam1, float &cs,
-                          double &lt)
+                          double &lt, int *guess = nullptr)
 {
   const EosW w = eos_weights(t, rho, ye);
   int jt;
   double dtemp;
-  eos_invert(t, w, u / rho, jt, dtemp, lt);
+  if (guess) {
+    eos_invert_guess(t, w, u / rho, *guess, dtemp, lt);
+    jt = *guess;
+  } else {
+    eos_invert(t, w, u / rho, jt, dtemp, lt);
+  }
   float b0, b1;
   const float dl = eos_tri_f32(t.lnp, w, jt, dtemp, b0);
   press = exp((double)b0 + (double)dl);

@@ -124,6 +124,7 @@ struct Sph3Launch {
                            // or FP32 per unit area on the FP32-bulk path
   long long nf[3];         // faces per direction
   void *P, *G, *CS;        // table EOS arrays (state precision)
+  unsigned short *tg;      // temperature bracket of each zone's last EOS call
   double *ms, *grav;       // shell masses and monopole field (n1 each)
   double *part;            // shell-mass parts [npart][n1]
   int npart;

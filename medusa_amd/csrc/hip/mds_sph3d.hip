@@ -45,6 +45,7 @@ struct Sph3Args {
   void *F[3];                 // face fluxes [slot][face] per direction, slot 5 = Ye
   long long nf[3];            // faces per direction
   const void *P, *G, *CS;     // table EOS arrays (state precision)
+  unsigned short *tg;         // each zone's temperature bracket of the last EOS call
   const double *grav;         // monopole field per radial shell, or null
 };
 
@@ -601,19 +602,21 @@ __global__ __launch_bounds__(256) void sph3_eos_kernel(Sph3Args a, StateArrays<R
   if (pb.lj && !dend_valid(pb, (int)(z % pb.n[0]), (int)((z / pb.n[0]) % pb.n[1]), (int)(z / ((long long)pb.n[0] * pb.n[1]))))
     return;
   double press, lt;
+  int jt = a.tg[z];                    // the bracket search starts from the last one
   if (sizeof(R) == 8) {
     double g1, cs;
-    eos_fill_x64(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt);
+    eos_fill_x64(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt, &jt);
     ((R *)a.P)[z] = (R)press;
     ((R *)a.G)[z] = (R)g1;
     ((R *)a.CS)[z] = (R)cs;
   } else {
     float g1, cs;
-    eos_fill_f32(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt);
+    eos_fill_f32(a.eos, (double)in.v[0][z], (double)in.v[1][z], (double)in.v[5][z], press, g1, cs, lt, &jt);
     ((R *)a.P)[z] = (R)press;
     ((R *)a.G)[z] = (R)g1;
     ((R *)a.CS)[z] = (R)cs;
   }
+  a.tg[z] = (unsigned short)jt;
 }
 
 // ---------------------------------------------------------------------------
@@ -1267,6 +1270,7 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
   a.eos = eos;
   for (int d = 0; d < 3; d++) { a.F[d] = L.F[d]; a.nf[d] = L.nf[d]; }
   a.P = L.P; a.G = L.G; a.CS = L.CS;
+  a.tg = L.tg;
   a.grav = L.grav;
   const hipError_t e = L.dend ? launch_stage<R, 1>(L, a, sp, sph, stage, st) : launch_stage<R, 0>(L, a, sp, sph, stage, st);
   if (e != hipSuccess) return e;
