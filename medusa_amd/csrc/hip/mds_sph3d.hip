@@ -972,7 +972,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
 // zone update: sources, flux divergence, RK combination, u -> p, dt limits
 // ---------------------------------------------------------------------------
 template <class R, int TAB, int MONO, int DEND>
-__global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp, StateArrays<R> sph, int stage)
+__global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R> sp, StateArrays<R> sph, int stage)
 {
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
@@ -980,56 +980,40 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   const bool inbox = z < (long long)n1 * n2 * n3;
   const long long zl = inbox ? z : 0;
   const int i0 = (int)(zl % n1), j0 = (int)((zl / n1) % n2), k0 = (int)(zl / ((long long)n1 * n2));
-  const bool live = inbox && (!DEND || dend_valid(pb, i0, j0, k0));
-  // The state and the (i, j) record of a box position are read whether or not
-  // it is a zone of its level (storage is the whole box), so these loads need
-  // not wait for the level tables; indices that depend on levels use a zone.
-  const int i = live ? i0 : 0, j = live ? j0 : 0, k = live ? k0 : 0;
+  // Every read of a zone whose neighbours share its levels (every zone of a
+  // uniform grid) goes out in one batch, with no branch between the reads:
+  // the state of both stages, the record, the six faces at the box position
+  // and the level values, all in range whether or not the position is a zone
+  // of its level (storage is the whole box).  A zone at a level change then
+  // redoes its flux sum with the reference's fan-in.
+  const int i = i0, j = j0, k = k0;
   DevClock *ck = a.clock;
   const bool active = ck->active;
   const unsigned long long seen = min_key_seen(&ck->mindt);
   const double dt = ck->dt;
   const double gam = pb.gam;
   const StateArrays<R> &in = stage == 0 ? sp : sph;
-  double q[8], qn[8];
+  double q[8], qs[8];
   zone_state<R>(a, in, zl, TAB, gam, q);
-  if (stage == 1) zone_state<R>(a, sp, zl, TAB, gam, qn);
-  else for (int t = 0; t < 8; t++) qn[t] = q[t];
-  const ZRec rc = rec3(pb, i0, j0);
-  double gcov[3], gcon[3];
-  for (int d = 0; d < 3; d++) { gcov[d] = rc[MDS_ZS_GCOV0 + d]; gcon[d] = rc[MDS_ZS_GCON + d]; }
-  // geometric sources (rad_fluid.c:195-214): the 11 non-zero conn[l][m][n]
-  // in the reference's (l, n) order per m
-  double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const double press = q[6];
-  {
-    double vc[3], T[3][3];
-    for (int d = 0; d < 3; d++) vc[d] = gcov[d] * q[2 + d];
-    for (int l = 0; l < 3; l++) {
-      for (int m = 0; m < 3; m++) T[l][m] = (q[0] * q[2 + m]) * vc[l];
-      T[l][l] += press;
-    }
-    src[2] += T[0][0] * rc[MDS_ZS_C000];
-    src[2] += T[1][1] * rc[MDS_ZS_C101];
-    src[2] += T[2][2] * rc[MDS_ZS_C202];
-    src[3] += T[0][1] * rc[MDS_ZS_C011];
-    src[3] += T[1][0] * rc[MDS_ZS_C110];
-    src[3] += T[1][1] * rc[MDS_ZS_C111];
-    src[3] += T[2][2] * rc[MDS_ZS_C212];
-    src[4] += T[0][2] * rc[MDS_ZS_C022];
-    src[4] += T[1][2] * rc[MDS_ZS_C122];
-    src[4] += T[2][0] * rc[MDS_ZS_C220];
-    src[4] += T[2][1] * rc[MDS_ZS_C221];
+  zone_state<R>(a, sp, zl, TAB, gam, qs);          // stage 1 needs it (the same data at stage 0)
+  const double cs_tab = TAB ? (double)((const R *)a.CS)[zl] : 0.0;
+  const ZRec rc = rec3(pb, i, j);
+  double gcov[3], gcon[3], scl[3], conn[11];
+  for (int d = 0; d < 3; d++) {
+    gcov[d] = rc[MDS_ZS_GCOV0 + d];
+    gcon[d] = rc[MDS_ZS_GCON + d];
+    scl[d] = rc[MDS_ZS_SCALE0 + d];
   }
-  // level relations with the neighbours (dendritic grids; all 0 otherwise)
-  const int li = DEND ? dend_lj(pb, i) : 0, lo = DEND ? dend_lj(pb, i + 1) : 0;
-  const int lk0 = DEND ? dend_lk(pb, i, j) : 0;
-  const int lkp = DEND ? dend_lk(pb, i, j + 1) : 0, lkm = DEND ? dend_lk(pb, i, j - 1) : 0;
-  const int njp = li > lo ? 1 << (li - lo) : 1;
-  const int jp0 = rescale(j, li, lo);
-  const int lq0 = DEND ? dend_lk(pb, i + 1, jp0) : 0;
+  {
+    const int cf[11] = {MDS_ZS_C000, MDS_ZS_C101, MDS_ZS_C202, MDS_ZS_C011, MDS_ZS_C110, MDS_ZS_C111,
+                        MDS_ZS_C212, MDS_ZS_C022, MDS_ZS_C122, MDS_ZS_C220, MDS_ZS_C221};
+#pragma unroll
+    for (int t = 0; t < 11; t++) conn[t] = rc[cf[t]];
+  }
+  const double vol = rc[MDS_ZS_VOLUME];
   const long long f0 = fid3(pb, 0, i, j, k);
   const long long f1 = fid3(pb, 1, i, j, k), f2 = fid3(pb, 2, i, j, k), f2p = fid3(pb, 2, i, j, k + 1);
+  const long long fo = fid3(pb, 0, i + 1, j, k), ft = fid3(pb, 1, i, j + 1, k);
   // Face fluxes: FP64 ones carry their area; FP32 ones are per unit area and
   // take the FP64 area of the record the flux kernel used, (fi, fj) of the
   // face, here forming the same product that kernel would have stored.
@@ -1047,46 +1031,87 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   auto flux = [&](int D, int v, long long f, double area) -> double { return scaled(load(D, v, f), area); };
   const double ain = rc[MDS_ZS_AREA0], a1 = rc[MDS_ZS_AREA1], a2 = rc[MDS_ZS_AREA2];
   const double a1p = sizeof(R) == 8 ? 0.0 : rec3(pb, i, j + 1)[MDS_ZS_AREA1];
-  // flux divergence per slot, in the reference's order (step.c:398-438):
-  // outer radial faces, inner radial, outer theta, inner theta, phi.  Each
-  // face's slots are read together; a zone whose neighbours all share its
-  // levels (every zone of a uniform grid) reads all of its faces at once.
+  const double ao = (sizeof(R) == 4 || MONO) ? rec3(pb, i + 1, j)[MDS_ZS_AREA0] : 0.0;
   constexpr int NSLOT = TAB ? 6 : 5;
+  R x[NSLOT][6];
+#pragma unroll
+  for (int v = 0; v < NSLOT; v++) {
+    x[v][0] = load(0, v, fo);
+    x[v][1] = load(0, v, f0);
+    x[v][2] = load(1, v, ft);
+    x[v][3] = load(1, v, f1);
+    x[v][4] = load(2, v, f2p);
+    x[v][5] = load(2, v, f2);
+  }
+  // level relations with the neighbours (dendritic grids; all 0 otherwise);
+  // lq is the outer shell's level of row j, its row when li == lo
+  int li = 0, lo = 0, lk0 = 0, lkp = 0, lkm = 0, lq = 0;
+  if (DEND) {
+    const int N1 = n1 + 2 * MDS_NG;
+    const unsigned char *lkr = pb.lk + (size_t)(j + MDS_NG) * N1 + (i + MDS_NG);
+    li = pb.lj[i + MDS_NG];
+    lo = pb.lj[i + 1 + MDS_NG];
+    lk0 = lkr[0];
+    lkp = lkr[N1];
+    lkm = lkr[-N1];
+    lq = lkr[1];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const bool live = inbox && (!DEND || (j < (n2 >> li) && k < (n3 >> lk0)));
+  if (stage == 0)
+    for (int t = 0; t < 8; t++) qs[t] = q[t];
+  const double *qn = qs;
+  // geometric sources (rad_fluid.c:195-214): the 11 non-zero conn[l][m][n]
+  // in the reference's (l, n) order per m
+  double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double press = q[6];
+  {
+    double vc[3], T[3][3];
+    for (int d = 0; d < 3; d++) vc[d] = gcov[d] * q[2 + d];
+    for (int l = 0; l < 3; l++) {
+      for (int m = 0; m < 3; m++) T[l][m] = (q[0] * q[2 + m]) * vc[l];
+      T[l][l] += press;
+    }
+    src[2] += T[0][0] * conn[0];
+    src[2] += T[1][1] * conn[1];
+    src[2] += T[2][2] * conn[2];
+    src[3] += T[0][1] * conn[3];
+    src[3] += T[1][0] * conn[4];
+    src[3] += T[1][1] * conn[5];
+    src[3] += T[2][2] * conn[6];
+    src[4] += T[0][2] * conn[7];
+    src[4] += T[1][2] * conn[8];
+    src[4] += T[2][0] * conn[9];
+    src[4] += T[2][1] * conn[10];
+  }
+  // flux divergence per slot, in the reference's order (step.c:398-438):
+  // outer radial faces, inner radial, outer theta, inner theta, phi
   double divF[NSLOT];
   double vr = 0.0, varea = 0.0;                 // outer mass flux and area (monopole)
-  const bool simple = !DEND || (njp == 1 && lq0 == lk0 && lkp == lk0 && lkm == lk0);
-  if (simple) {
-    const long long fo = fid3(pb, 0, i + 1, jp0, k), ft = fid3(pb, 1, i, j + 1, k);
-    const double ao = rec3(pb, i + 1, jp0)[MDS_ZS_AREA0];
-    R x[NSLOT][6];
 #pragma unroll
-    for (int v = 0; v < NSLOT; v++) {
-      x[v][0] = load(0, v, fo);
-      x[v][1] = load(0, v, f0);
-      x[v][2] = load(1, v, ft);
-      x[v][3] = load(1, v, f1);
-      x[v][4] = load(2, v, f2p);
-      x[v][5] = load(2, v, f2);
-    }
-    // issue the zone's face loads together, ahead of their use
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int v = 0; v < NSLOT; v++) {
-      const double fro = scaled(x[v][0], ao);
-      double d = 0.0;
-      d += fro;
-      d -= scaled(x[v][1], ain);
-      d += scaled(x[v][2], a1p);
-      d -= scaled(x[v][3], a1);
-      d += scaled(x[v][4], a2) - scaled(x[v][5], a2);
-      divF[v] = d;
-      if (v == 0) { vr += fro; varea += ao + 1.0e-16; }
-    }
-  } else {
+  for (int v = 0; v < NSLOT; v++) {
+    const double fro = scaled(x[v][0], ao);
+    double d = 0.0;
+    d += fro;
+    d -= scaled(x[v][1], ain);
+    d += scaled(x[v][2], a1p);
+    d -= scaled(x[v][3], a1);
+    d += scaled(x[v][4], a2) - scaled(x[v][5], a2);
+    divF[v] = d;
+    if (v == 0) { vr += fro; varea += ao + 1.0e-16; }
+  }
+  const bool simple = !DEND || (li == lo && lq == lk0 && lkp == lk0 && lkm == lk0);
+  if (DEND && live && !simple) {
+    // a zone at a level change: the finer outer radial faces (step.c:398-408)
+    // and the half-area theta rules (:411-433)
+    const int njp = li > lo ? 1 << (li - lo) : 1;
+    const int jp0 = rescale(j, li, lo);
+    vr = 0.0;
+    varea = 0.0;
 #pragma unroll
     for (int v = 0; v < NSLOT; v++) divF[v] = 0.0;
     for (int jj = jp0; jj < jp0 + njp; jj++) {
-      const int l = DEND ? dend_lk(pb, i + 1, jj) : 0;
+      const int l = dend_lk(pb, i + 1, jj);
       const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
       const double ar = rec3(pb, i + 1, jj)[MDS_ZS_AREA0];
       for (int kk = k * nkp; kk < k * nkp + nkp; kk++) {
@@ -1100,20 +1125,18 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
       }
     }
 #pragma unroll
-    for (int v = 0; v < NSLOT; v++) divF[v] -= flux(0, v, f0, ain);
+    for (int v = 0; v < NSLOT; v++) divF[v] -= scaled(x[v][1], ain);
     // outer theta face (step.c:411-423)
     if (lkp > lk0) {
-      const long long f = fid3(pb, 1, i, j + 1, k);
 #pragma unroll
-      for (int v = 0; v < NSLOT; v++) divF[v] += 0.5 * flux(1, v, f, a1p);
+      for (int v = 0; v < NSLOT; v++) divF[v] += 0.5 * scaled(x[v][2], a1p);
     } else if (lkp < lk0) {
       const long long fa = fid3(pb, 1, i, j + 1, 2 * k), fb = fid3(pb, 1, i, j + 1, 2 * k + 1);
 #pragma unroll
       for (int v = 0; v < NSLOT; v++) divF[v] += flux(1, v, fa, a1p) + flux(1, v, fb, a1p);
     } else {
-      const long long f = fid3(pb, 1, i, j + 1, k);
 #pragma unroll
-      for (int v = 0; v < NSLOT; v++) divF[v] += flux(1, v, f, a1p);
+      for (int v = 0; v < NSLOT; v++) divF[v] += scaled(x[v][2], a1p);
     }
     // inner theta face (step.c:425-433)
     if (lkm < lk0) {
@@ -1122,10 +1145,10 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
       for (int v = 0; v < NSLOT; v++) divF[v] -= 0.5 * flux(1, v, fa, a1) + 0.5 * flux(1, v, fb, a1);
     } else {
 #pragma unroll
-      for (int v = 0; v < NSLOT; v++) divF[v] -= flux(1, v, f1, a1);
+      for (int v = 0; v < NSLOT; v++) divF[v] -= scaled(x[v][3], a1);
     }
 #pragma unroll
-    for (int v = 0; v < NSLOT; v++) divF[v] += flux(2, v, f2p, a2) - flux(2, v, f2, a2);
+    for (int v = 0; v < NSLOT; v++) divF[v] += scaled(x[v][4], a2) - scaled(x[v][5], a2);
   }
   if (MONO) {
     // gravity.c:415-437: mass flux per area through the finer outer faces
@@ -1133,7 +1156,7 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
     const double xl = pb.startx[0] + i * pb.dx[0];
     vr /= varea;
     src[2] += q[0] * gz;
-    const double vl = flux(0, 0, f0, ain) / (ain + 1.0e-16);
+    const double vl = scaled(x[0][1], ain) / (ain + 1.0e-16);
     const double A = (vr - vl) / pb.dx[0];
     const double B = vl - A * xl;
     const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
@@ -1145,7 +1168,6 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   U[MDS_YI] = TAB ? U[0] * qn[5] : 0.0;
   Uh[MDS_YI] = TAB ? Uh[0] * q[5] : 0.0;
   const double lim_src = dt_source_control_x64<3>(qn[1], &qn[2], U, src);
-  const double vol = rc[MDS_ZS_VOLUME];
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
 #pragma unroll
   for (int v = 0; v < NSLOT; v++) {
@@ -1157,13 +1179,13 @@ __global__ __launch_bounds__(256) void sph3_update(Sph3Args a, StateArrays<R> sp
   double locmax = 0.0;
   if (stage == 1) {
     // estimate_dt zone bound (step.c:177-187) with the stage's sound speed
-    double cs = TAB ? (double)((const R *)a.CS)[zl] : sqrt((gam * press) / q[0]);
+    double cs = TAB ? cs_tab : sqrt((gam * press) / q[0]);
     double loc = 0.0;
     for (int d = 0; d < 3; d++) {
       double ldx = pb.dx[d];
       if (DEND && d == 1) ldx *= (double)(1 << li);
       if (DEND && d == 2) ldx *= (double)(1 << lk0);
-      loc += (fabs(vo[d]) + (cs / rc[MDS_ZS_SCALE0 + d])) / ldx;
+      loc += (fabs(vo[d]) + (cs / scl[d])) / ldx;
     }
     locmax = loc;
   }
