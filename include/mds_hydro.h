@@ -230,7 +230,7 @@ int mds_sync(mds_ctx *ctx);
  * reference, whose OpenMP runtime picks launch shapes).  Chosen by
  * mds_autotune() or set explicitly; every value gives identical results. */
 #define MDS_TILE_ROWS2D  0       /* rows marched by one wavefront in the 2D sweep */
-#define MDS_TILE_TY3D    1       /* output rows per workgroup in the 3D sweep (4 or 12) */
+#define MDS_TILE_TY3D    1       /* output rows per workgroup in the 3D sweep (4, 8 or 12) */
 #define MDS_TILE_KC3D    2       /* planes marched by one workgroup in the 3D sweep */
 #define MDS_TILE_FUSE1D  3       /* 1D: 1 = both RK stages of a step in one launch (default
                                     when no problem is periodic in x), 0 = one launch per stage */

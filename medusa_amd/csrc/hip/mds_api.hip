@@ -986,7 +986,8 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "rows per wavefront out of range: %d", value);
     slot = &c->chunk2d;
   } else if (param == MDS_TILE_TY3D) {
-    if (value != 4 && value != 12) return set_err(c, MDS_EINVAL, "3D tile rows must be 4 or 12: %d", value);
+    if (value != 4 && value != 8 && value != 12)
+      return set_err(c, MDS_EINVAL, "3D tile rows must be 4, 8 or 12: %d", value);
     slot = &c->ty3d;
   } else if (param == MDS_TILE_KC3D) {
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "planes per workgroup out of range: %d", value);
@@ -1082,7 +1083,7 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
       if (v <= c->hprob[0].n[1]) cand.push_back({v, 0});
   } else {
-    for (int ty : {4, 12})
+    for (int ty : {4, 8, 12})
       for (int kc : {8, 16, 32, 64})
         if (kc <= c->hprob[0].n[2]) cand.push_back({ty, kc});
   }

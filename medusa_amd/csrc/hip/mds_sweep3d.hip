@@ -304,9 +304,11 @@ extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, const Sweep3DArgs *a,
                      *(const StateArrays<R> *)sp, *(const StateArrays<R> *)sph, stage)
   if (!fp32) {
     if (ty == 4) { if (src) MDS_L3(double, true, 4); else MDS_L3(double, false, 4); }
+    else if (ty == 8) { if (src) MDS_L3(double, true, 8); else MDS_L3(double, false, 8); }
     else { if (src) MDS_L3(double, true, 12); else MDS_L3(double, false, 12); }
   } else {
     if (ty == 4) { if (src) MDS_L3(float, true, 4); else MDS_L3(float, false, 4); }
+    else if (ty == 8) { if (src) MDS_L3(float, true, 8); else MDS_L3(float, false, 8); }
     else { if (src) MDS_L3(float, true, 12); else MDS_L3(float, false, 12); }
   }
 #undef MDS_L3

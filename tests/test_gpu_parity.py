@@ -329,7 +329,7 @@ def test_results_independent_of_launch_shape(precision):
         outs.append(ctx.download())
     g3, prob3, prims3 = problems.rt(20)
     outs3 = []
-    for ty, kc in ((4, 3), (12, 8), (4, 20)):
+    for ty, kc in ((4, 3), (12, 8), (4, 20), (8, 5)):
         ctx = mds.Context([prob3], precision=precision, rp=mds.run_params(cfl=0.5))
         ctx.set_tile(mds.TILE_TY3D, ty)
         ctx.set_tile(mds.TILE_KC3D, kc)
