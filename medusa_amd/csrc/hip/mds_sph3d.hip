@@ -930,28 +930,47 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
 // dendritic faces whose pencil crosses a level change: radial (D = 0) and
 // theta (D = 1) faces from a list of (i, j, k, 0), k the face's phi index
 // ---------------------------------------------------------------------------
+// A workgroup takes 64 listed faces.  Its four wavefronts build one pencil
+// entry each (entry s of face l in lane l of wavefront s): the entries are
+// independent, and each is a chain of dependent reads (levels, then the zones
+// of a restriction or a limited-slope prolongation).  The entries meet in LDS
+// and the first wavefront solves the faces.
 template <class R, int D, int TAB>
 __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> in, const int4 *list, long long n)
 {
   const DevProb &pb = a.pb;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+  const long long t = (long long)blockIdx.x * 64 + lane;
   const bool live = t < n;
   const int4 e = list[live ? t : 0];
   const int fi = e.x, fj = e.y, fk = e.z;
+  __shared__ double sq[4][8][64];
+  {
+    double qs[8];
+    if (D == 0) {
+      radial_entry<R, TAB>(a, in, fi, fj, fk, fi - 2 + s, qs);
+    } else {
+      const int lj = dend_lk(pb, fi, fj), lm = dend_lk(pb, fi, fj - 1);
+      theta_entry<R, TAB>(a, in, fi, lj < lm ? lj : lm, fk, fj - 2 + s, qs);
+    }
+#pragma unroll
+    for (int v = 0; v < 8; v++) sq[s][v][lane] = qs[v];
+  }
+  __syncthreads();
+  if (s != 0) return;                                  // uniform per wavefront
   const bool active = a.clock->active;
   const unsigned long long seen = min_key_seen(&a.clock->mindt);
   double q[4][8];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) q[u][v] = sq[u][v][lane];
   int lev = 0;
   bool hlle = false;
-  if (D == 0) {
-#pragma unroll
-    for (int s = 0; s < 4; s++) radial_entry<R, TAB>(a, in, fi, fj, fk, fi - 2 + s, q[s]);
-  } else {
+  if (D == 1) {
     const int lj = dend_lk(pb, fi, fj), lm = dend_lk(pb, fi, fj - 1);
     const int lf = lj < lm ? lj : lm;
     lev = dend_lj(pb, fi);
-#pragma unroll
-    for (int s = 0; s < 4; s++) theta_entry<R, TAB>(a, in, fi, lf, fk, fj - 2 + s, q[s]);
     // the zone of row j owning this sub-face, and transverse_shock (fluid.c:189-201)
     const int kz = fk >> (lj - lf);
     hlle = tagged(a, fi, fj, kz);
@@ -1254,7 +1273,7 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
   }
 #define MDS_FLUX_IRR(D)                                                                                    \
   if (DEND && L.nirr[D] > 0) {                                                                             \
-    const dim3 g((unsigned)((L.nirr[D] + 255) / 256));                                                    \
+    const dim3 g((unsigned)((L.nirr[D] + 63) / 64));                                                      \
     if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);   \
     else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);         \
   }
