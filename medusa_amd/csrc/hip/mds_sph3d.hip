@@ -999,13 +999,29 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
   const bool inbox = z < (long long)n1 * n2 * n3;
   const long long zl = inbox ? z : 0;
   const int i0 = (int)(zl % n1), j0 = (int)((zl / n1) % n2), k0 = (int)(zl / ((long long)n1 * n2));
-  // Every read of a zone whose neighbours share its levels (every zone of a
-  // uniform grid) goes out in one batch, with no branch between the reads:
-  // the state of both stages, the record, the six faces at the box position
-  // and the level values, all in range whether or not the position is a zone
-  // of its level (storage is the whole box).  A zone at a level change then
-  // redoes its flux sum with the reference's fan-in.
-  const int i = i0, j = j0, k = k0;
+  // level relations with the neighbours (dendritic grids; all 0 otherwise),
+  // read first from the small level tables; lq is the outer shell's level of
+  // row j, its row when li == lo
+  int li = 0, lo = 0, lk0 = 0, lkp = 0, lkm = 0, lq = 0;
+  if (DEND) {
+    const int N1 = n1 + 2 * MDS_NG;
+    const unsigned char *lkr = pb.lk + (size_t)(j0 + MDS_NG) * N1 + (i0 + MDS_NG);
+    li = pb.lj[i0 + MDS_NG];
+    lo = pb.lj[i0 + 1 + MDS_NG];
+    lk0 = lkr[0];
+    lkp = lkr[N1];
+    lkm = lkr[-N1];
+    lq = lkr[1];
+  }
+  const bool live = inbox && (!DEND || (j0 < (n2 >> li) && k0 < (n3 >> lk0)));
+  // Every other read of a zone whose neighbours share its levels (every zone
+  // of a uniform grid) goes out in one batch, with no branch between the
+  // reads: the state of both stages, the record and the six faces.  A box
+  // position that is no zone of its level reads zone (0, 0, 0) instead, whose
+  // lines all such positions share.  A zone at a level change then redoes its
+  // flux sum with the reference's fan-in.
+  const int i = live ? i0 : 0, j = live ? j0 : 0, k = live ? k0 : 0;
+  const long long zr = live ? zl : 0;
   DevClock *ck = a.clock;
   const bool active = ck->active;
   const unsigned long long seen = min_key_seen(&ck->mindt);
@@ -1013,9 +1029,9 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
   const double gam = pb.gam;
   const StateArrays<R> &in = stage == 0 ? sp : sph;
   double q[8], qs[8];
-  zone_state<R>(a, in, zl, TAB, gam, q);
-  zone_state<R>(a, sp, zl, TAB, gam, qs);          // stage 1 needs it (the same data at stage 0)
-  const double cs_tab = TAB ? (double)((const R *)a.CS)[zl] : 0.0;
+  zone_state<R>(a, in, zr, TAB, gam, q);
+  zone_state<R>(a, sp, zr, TAB, gam, qs);          // stage 1 needs it (the same data at stage 0)
+  const double cs_tab = TAB ? (double)((const R *)a.CS)[zr] : 0.0;
   const ZRec rc = rec3(pb, i, j);
   double gcov[3], gcon[3], scl[3], conn[11];
   for (int d = 0; d < 3; d++) {
@@ -1062,21 +1078,7 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
     x[v][4] = load(2, v, f2p);
     x[v][5] = load(2, v, f2);
   }
-  // level relations with the neighbours (dendritic grids; all 0 otherwise);
-  // lq is the outer shell's level of row j, its row when li == lo
-  int li = 0, lo = 0, lk0 = 0, lkp = 0, lkm = 0, lq = 0;
-  if (DEND) {
-    const int N1 = n1 + 2 * MDS_NG;
-    const unsigned char *lkr = pb.lk + (size_t)(j + MDS_NG) * N1 + (i + MDS_NG);
-    li = pb.lj[i + MDS_NG];
-    lo = pb.lj[i + 1 + MDS_NG];
-    lk0 = lkr[0];
-    lkp = lkr[N1];
-    lkm = lkr[-N1];
-    lq = lkr[1];
-  }
   __builtin_amdgcn_sched_barrier(0);
-  const bool live = inbox && (!DEND || (j < (n2 >> li) && k < (n3 >> lk0)));
   if (stage == 0)
     for (int t = 0; t < 8; t++) qs[t] = q[t];
   const double *qn = qs;
