@@ -827,6 +827,16 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
       zvalid = zj < (pb.n[1] >> Lj[0]) && zk <= (pb.n[2] >> Lk[0]);
     }
   }
+  // Positions that are no zone of their level fill most of the rows nearest
+  // the poles: a wavefront (radial) or workgroup (theta, phi: they meet at a
+  // barrier below) without a zone has no face and leaves here.
+  if (DEND) {
+    if (D == 0) {
+      if (__ballot(zvalid) == 0) return;
+    } else {
+      if (!__syncthreads_or(zvalid)) return;
+    }
+  }
   // level of the zone's own stencil (for the reconstruction)
   int zlev = 0;
   if (DEND && D == 1) zlev = Lj[0];
@@ -1014,6 +1024,7 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
     lq = lkr[1];
   }
   const bool live = inbox && (!DEND || (j0 < (n2 >> li) && k0 < (n3 >> lk0)));
+  if (DEND && __ballot(live) == 0) return;          // a wavefront without a zone
   // Every other read of a zone whose neighbours share its levels (every zone
   // of a uniform grid) goes out in one batch, with no branch between the
   // reads: the state of both stages, the record and the six faces.  A box
