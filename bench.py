@@ -106,8 +106,8 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
         ctx.upload(prims, k)
     ctx.update_eos()
     if tiles:
-        for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D), tiles):
-            if t != mds.TILE_FUSE1D or ctx.ndim == 1:
+        for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D, mds.TILE_GRID1D), tiles):
+            if t not in (mds.TILE_FUSE1D, mds.TILE_GRID1D) or ctx.ndim == 1:
                 ctx.set_tile(t, v)
         tuned_ms = None
     else:
@@ -133,7 +133,8 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
                 launches=launches,
                 stage_ms=(st1_ms / steps, st2_ms / steps), mass_drift=float(drift), energy_drift=float(edrift),
                 counters=ctx.counters(), tuned_ms=tuned_ms,
-                tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D)])
+                tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D,
+                                                 mds.TILE_GRID1D)])
 
 
 def cpu_baseline(name, W, budget_s):
@@ -213,7 +214,7 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="grid size (problem count for ensemble)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time budget")
     ap.add_argument("--tiles", default=None,
-                    help="ROWS2D,TY3D,KC3D[,FUSE1D] launch shape to use instead of autotuning (profiling runs)")
+                    help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D]] launch shape to use instead of autotuning (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=None, metavar="BYTES",
                     help="only run the device copy kernel over BYTES (counter calibration) and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")

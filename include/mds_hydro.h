@@ -234,6 +234,9 @@ int mds_sync(mds_ctx *ctx);
 #define MDS_TILE_KC3D    2       /* planes marched by one workgroup in the 3D sweep */
 #define MDS_TILE_FUSE1D  3       /* 1D: 1 = both RK stages of a step in one launch (default
                                     when no problem is periodic in x), 0 = one launch per stage */
+#define MDS_TILE_GRID1D  4       /* 1D grid: 0 = one wavefront per segment of the segment table,
+                                    1 = persistent (the workgroups the device holds at once walk
+                                    the whole table) */
 int mds_set_tile(mds_ctx *ctx, int param, int value);
 int mds_get_tile(mds_ctx *ctx, int param, int *value);
 

@@ -67,6 +67,8 @@ struct mds_ctx {
   std::vector<void *> geo_allocs;
   int chunk2d = 24;
   int ty3d = 4;
+  int ncu = 0;                          // compute units of the device
+  int grid1d = 0;                       // 1D grid: 0 one wavefront per segment, 1 persistent
   int kc3d = 16;
   hipGraph_t graph[2] = {nullptr, nullptr};        // one step, per buffer parity
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -400,6 +402,11 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     c->nstate = 6;
   }
   HIPCHK(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   c->zstart.assign(nprob + 1, 0);
   c->hprob.resize(nprob);
   for (int p = 0; p < nprob; p++) {
@@ -743,6 +750,8 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.rp = c->rp;
     a.eos = c->deos;
     a.grav = c->dgrav;
+    a.ncu = c->ncu;
+    a.persist = c->grid1d;
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       // monopole field of the stage input state (gravity.c:145-251 at p_to_phys_src_start)
@@ -814,6 +823,8 @@ static int enqueue_fused1d(mds_ctx *c)
   a.rp = c->rp;
   a.eos = c->deos;
   a.grav = nullptr;
+  a.ncu = c->ncu;
+  a.persist = c->grid1d;
   int fp32 = c->precision == MDS_FP32_BULK;
   if (fp32) {
     StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
@@ -997,6 +1008,9 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
     if (value && !c->can_fuse1d)
       return set_err(c, MDS_EINVAL, "the two-stage 1D kernel needs 1D problems without periodic x boundaries");
     slot = &c->fuse1d;
+  } else if (param == MDS_TILE_GRID1D) {
+    if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "1D grid mode must be 0 or 1: %d", value);
+    slot = &c->grid1d;
   } else {
     return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
   }
@@ -1014,6 +1028,7 @@ extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
   if (param == MDS_TILE_TY3D) { *value = c->ty3d; return MDS_OK; }
   if (param == MDS_TILE_KC3D) { *value = c->kc3d; return MDS_OK; }
   if (param == MDS_TILE_FUSE1D) { *value = c->fuse1d; return MDS_OK; }
+  if (param == MDS_TILE_GRID1D) { *value = c->grid1d; return MDS_OK; }
   return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
 }
 
@@ -1077,8 +1092,11 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
   // 3D (rows per workgroup, planes per workgroup)
   std::vector<std::pair<int, int>> cand;
   if (c->ndim == 1) {
-    cand.push_back({0, 0});
-    if (c->can_fuse1d) cand.push_back({1, 0});
+    // (two-stage launch, persistent grid)
+    for (int g : {0, 1}) {
+      cand.push_back({0, g});
+      if (c->can_fuse1d) cand.push_back({1, g});
+    }
   } else if (c->ndim == 2) {
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
       if (v <= c->hprob[0].n[1]) cand.push_back({v, 0});
@@ -1090,12 +1108,12 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
   Snapshot s;
   int rc = snap_alloc(c, s);
   if (rc == MDS_OK) rc = snap_copy(c, s, true);
-  std::pair<int, int> best = c->ndim == 1 ? std::make_pair(c->fuse1d, 0)
+  std::pair<int, int> best = c->ndim == 1 ? std::make_pair(c->fuse1d, c->grid1d)
                              : c->ndim == 2 ? std::make_pair(c->chunk2d, 0) : std::make_pair(c->ty3d, c->kc3d);
   double bms = 1e300;
   for (auto v : cand) {
     if (rc) break;
-    if (c->ndim == 1) c->fuse1d = v.first;
+    if (c->ndim == 1) { c->fuse1d = v.first; c->grid1d = v.second; }
     else if (c->ndim == 2) c->chunk2d = v.first;
     else { c->ty3d = v.first; c->kc3d = v.second; }
     double ms[3];
@@ -1112,7 +1130,7 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     if (rc == MDS_OK) rc = snap_copy(c, s, false);
   }
   snap_free(s);
-  if (c->ndim == 1) c->fuse1d = best.first;
+  if (c->ndim == 1) { c->fuse1d = best.first; c->grid1d = best.second; }
   else if (c->ndim == 2) c->chunk2d = best.first;
   else { c->ty3d = best.first; c->kc3d = best.second; }
   drop_graphs(c);

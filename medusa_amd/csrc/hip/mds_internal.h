@@ -97,6 +97,8 @@ struct Sweep1DArgs {
   RunParams rp;
   DevEos eos;              // tabulated EOS (when the batch has one)
   const double *grav;      // monopole gravity at zone centres, packed like the state
+  int ncu;                 // compute units of the device
+  int persist;             // 1: persistent grid (resident workgroups walk the segment table)
 };
 
 struct Sweep2DArgs {

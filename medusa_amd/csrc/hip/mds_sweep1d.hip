@@ -353,41 +353,46 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
 {
   const int lane = threadIdx.x & 63;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (wv >= a.nseg) return;
-  const int2 sg = a.seg[wv];
-  const DevProb &pb = a.prob[sg.x];
-  DevClock *ck = &a.clock[sg.x];
-  if (!ck->active) return;
-  const double dt = ck->dt;
-  const int n = pb.n[0];
-  const int c = sg.y - MDS_NG + lane;
-  const long long off = pb.off;
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // each wavefront walks the segment table (longest problems first) with the
+  // grid's stride: one segment per wavefront, or, in a persistent grid, a
+  // share of the table per resident wavefront
+  for (int w = wv; w < a.nseg; w += nw) {
+    const int2 sg = a.seg[w];
+    const DevProb &pb = a.prob[sg.x];
+    DevClock *ck = &a.clock[sg.x];
+    if (!ck->active) continue;
+    const double dt = ck->dt;
+    const int n = pb.n[0];
+    const int c = sg.y - MDS_NG + lane;
+    const long long off = pb.off;
 
-  const StateArrays<double> &in = (stage == 0) ? sp : sph;
-  bool flip;
-  const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
-  Geo64 g;
-  load_geo64<SPH>(pb, c, n, g);
+    const StateArrays<double> &in = (stage == 0) ? sp : sph;
+    bool flip;
+    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
+    Geo64 g;
+    load_geo64<SPH>(pb, c, n, g);
 
-  Roles r;
-  r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
-  r.count = r.face && (lane <= 61 || c == n);
-  r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
-  // U^n of this zone: the stage input itself in stage 0, sim_p in stage 1
-  Prim<double> qn = q;
-  if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, false);
-  double U0[6];
-  conserved_x64<NV, SPH, TAB>(qn, g.gcov0, U0);
-  const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
+    Roles r;
+    r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
+    r.count = r.face && (lane <= 61 || c == n);
+    r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
+    // U^n of this zone: the stage input itself in stage 0, sim_p in stage 1
+    Prim<double> qn = q;
+    if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, false);
+    double U0[6];
+    conserved_x64<NV, SPH, TAB>(qn, g.gcov0, U0);
+    const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
-  Lim1D lim;
-  const Prim<double> o = stage_x64<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
-  if (r.out) {
-    StateArrays<double> &out = (stage == 0) ? sph : sp;
-    store_prims<NV, TAB>(out, off, c, o);
+    Lim1D lim;
+    const Prim<double> o = stage_x64<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
+    if (r.out) {
+      StateArrays<double> &out = (stage == 0) ? sph : sp;
+      store_prims<NV, TAB>(out, off, c, o);
+    }
+    commit_limits(pb, ck, a.counters, lane, lim);
   }
-  commit_limits(pb, ck, a.counters, lane, lim);
 }
 
 // FP64 two-stage kernel: reads in (U^n), writes out (U^{n+1}).
@@ -397,41 +402,46 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
 {
   const int lane = threadIdx.x & 63;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (wv >= a.nseg) return;
-  const int2 sg = a.seg[wv];
-  const DevProb &pb = a.prob[sg.x];
-  DevClock *ck = &a.clock[sg.x];
-  const int n = pb.n[0];
-  const int c0 = sg.y;
-  const int c = c0 - 2 * MDS_NG + lane;
-  const long long off = pb.off;
-  const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
-  if (!ck->active) {
-    // frozen problem: carry its state over to the output buffer
-    if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
-    return;
-  }
-  const double dt = ck->dt;
-  bool flip;
-  const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
-  Geo64 g;
-  load_geo64<SPH>(pb, c, n, g);
-  double U0[6];
-  conserved_x64<NV, SPH, TAB>(q, g.gcov0, U0);
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // each wavefront walks the segment table (longest problems first) with the
+  // grid's stride: one segment per wavefront, or, in a persistent grid, a
+  // share of the table per resident wavefront
+  for (int w = wv; w < a.nseg; w += nw) {
+    const int2 sg = a.seg[w];
+    const DevProb &pb = a.prob[sg.x];
+    DevClock *ck = &a.clock[sg.x];
+    const int n = pb.n[0];
+    const int c0 = sg.y;
+    const int c = c0 - 2 * MDS_NG + lane;
+    const long long off = pb.off;
+    const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
+    if (!ck->active) {
+      // frozen problem: carry its state over to the output buffer
+      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
+      continue;
+    }
+    const double dt = ck->dt;
+    bool flip;
+    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
+    Geo64 g;
+    load_geo64<SPH>(pb, c, n, g);
+    double U0[6];
+    conserved_x64<NV, SPH, TAB>(q, g.gcov0, U0);
 
-  const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
-  const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
-  Lim1D lim;
-  // stage 0: U^n -> U* on lanes 2..61
-  const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
-  Prim<double> h = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, q, 0.0, r0, lim);
-  ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
-  // stage 1: U^n, U* -> U^{n+1} on lanes 4..59
-  const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
-  const Prim<double> o = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, q, 0.0, r1, lim);
-  if (own) store_prims<NV, TAB>(out, off, c, o);
-  commit_limits(pb, ck, a.counters, lane, lim);
+    const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
+    const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
+    Lim1D lim;
+    // stage 0: U^n -> U* on lanes 2..61
+    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
+    Prim<double> h = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, q, 0.0, r0, lim);
+    ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
+    // stage 1: U^n, U* -> U^{n+1} on lanes 4..59
+    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
+    const Prim<double> o = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, q, 0.0, r1, lim);
+    if (own) store_prims<NV, TAB>(out, off, c, o);
+    commit_limits(pb, ck, a.counters, lane, lim);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -622,40 +632,45 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
 {
   const int lane = threadIdx.x & 63;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (wv >= a.nseg) return;
-  const int2 sg = a.seg[wv];
-  const DevProb &pb = a.prob[sg.x];
-  DevClock *ck = &a.clock[sg.x];
-  if (!ck->active) return;
-  const double dt = ck->dt;
-  const int n = pb.n[0];
-  const int c = sg.y - MDS_NG + lane;
-  const long long off = pb.off;
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // each wavefront walks the segment table (longest problems first) with the
+  // grid's stride: one segment per wavefront, or, in a persistent grid, a
+  // share of the table per resident wavefront
+  for (int w = wv; w < a.nseg; w += nw) {
+    const int2 sg = a.seg[w];
+    const DevProb &pb = a.prob[sg.x];
+    DevClock *ck = &a.clock[sg.x];
+    if (!ck->active) continue;
+    const double dt = ck->dt;
+    const int n = pb.n[0];
+    const int c = sg.y - MDS_NG + lane;
+    const long long off = pb.off;
 
-  const StateArrays<float> &in = (stage == 0) ? sp : sph;
-  bool flip;
-  const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
-  Geo32 g;
-  load_geo32<SPH>(pb, c, n, g);
-  Roles r;
-  r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
-  r.count = r.face && (lane <= 61 || c == n);
-  r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
-  Prim<float> qnf = q;
-  if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, false);
-  double U0[6];
-  Prim<double> qn;
-  conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
-  const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
+    const StateArrays<float> &in = (stage == 0) ? sp : sph;
+    bool flip;
+    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
+    Geo32 g;
+    load_geo32<SPH>(pb, c, n, g);
+    Roles r;
+    r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
+    r.count = r.face && (lane <= 61 || c == n);
+    r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
+    Prim<float> qnf = q;
+    if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, false);
+    double U0[6];
+    Prim<double> qn;
+    conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
+    const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
-  Lim1D lim;
-  const Prim<float> o = stage_f32<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
-  if (r.out) {
-    StateArrays<float> &out = (stage == 0) ? sph : sp;
-    store_prims<NV, TAB>(out, off, c, o);
+    Lim1D lim;
+    const Prim<float> o = stage_f32<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
+    if (r.out) {
+      StateArrays<float> &out = (stage == 0) ? sph : sp;
+      store_prims<NV, TAB>(out, off, c, o);
+    }
+    commit_limits(pb, ck, a.counters, lane, lim);
   }
-  commit_limits(pb, ck, a.counters, lane, lim);
 }
 
 template <int NV, bool SPH, int TAB>
@@ -664,39 +679,44 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
 {
   const int lane = threadIdx.x & 63;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (wv >= a.nseg) return;
-  const int2 sg = a.seg[wv];
-  const DevProb &pb = a.prob[sg.x];
-  DevClock *ck = &a.clock[sg.x];
-  const int n = pb.n[0];
-  const int c0 = sg.y;
-  const int c = c0 - 2 * MDS_NG + lane;
-  const long long off = pb.off;
-  const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
-  if (!ck->active) {
-    if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
-    return;
-  }
-  const double dt = ck->dt;
-  bool flip;
-  const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-  const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
-  Geo32 g;
-  load_geo32<SPH>(pb, c, n, g);
-  double U0[6];
-  Prim<double> qn;
-  conserved_f32<NV, SPH, TAB>(g, q, U0, qn);
+  const int nw = (int)((gridDim.x * blockDim.x) >> 6);
+  // each wavefront walks the segment table (longest problems first) with the
+  // grid's stride: one segment per wavefront, or, in a persistent grid, a
+  // share of the table per resident wavefront
+  for (int w = wv; w < a.nseg; w += nw) {
+    const int2 sg = a.seg[w];
+    const DevProb &pb = a.prob[sg.x];
+    DevClock *ck = &a.clock[sg.x];
+    const int n = pb.n[0];
+    const int c0 = sg.y;
+    const int c = c0 - 2 * MDS_NG + lane;
+    const long long off = pb.off;
+    const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
+    if (!ck->active) {
+      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
+      continue;
+    }
+    const double dt = ck->dt;
+    bool flip;
+    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
+    Geo32 g;
+    load_geo32<SPH>(pb, c, n, g);
+    double U0[6];
+    Prim<double> qn;
+    conserved_f32<NV, SPH, TAB>(g, q, U0, qn);
 
-  const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
-  const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
-  Lim1D lim;
-  const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
-  Prim<float> h = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, qn, 0.0, r0, lim);
-  ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
-  const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
-  const Prim<float> o = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, qn, 0.0, r1, lim);
-  if (own) store_prims<NV, TAB>(out, off, c, o);
-  commit_limits(pb, ck, a.counters, lane, lim);
+    const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
+    const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
+    Lim1D lim;
+    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
+    Prim<float> h = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, qn, 0.0, r0, lim);
+    ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
+    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
+    const Prim<float> o = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, qn, 0.0, r1, lim);
+    if (own) store_prims<NV, TAB>(out, off, c, o);
+    commit_limits(pb, ck, a.counters, lane, lim);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -768,6 +788,15 @@ extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *pr
   return hipGetLastError();
 }
 
+// workgroups of `block` threads of kernel k that one compute unit holds at once
+template <class K>
+static int resident_blocks(K k, int block)
+{
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, block, 0) != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
 // stage 0/1: single-stage kernels on (p, ph); stage -1: the two-stage kernel
 // with sp = input buffer and sph_ = output buffer (a->seg must then be the
 // MDS_SEG1D2 segment table).  tab: tabulated EOS with Ye; mono: monopole
@@ -777,11 +806,16 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
 {
   const int waves_per_block = 4;
   dim3 block(64 * waves_per_block);
-  dim3 grid((a->nseg + waves_per_block - 1) / waves_per_block);
+  const int need = (a->nseg + waves_per_block - 1) / waves_per_block;
   if ((tab || mono) && !sph) return hipErrorInvalidValue;
   if (mono && stage < 0) return hipErrorInvalidValue;
-#define MDS_L1(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h, stage)
-#define MDS_L2(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h)
+  // one wavefront per segment, or a persistent grid: the workgroups the
+  // device holds at once (occupancy of the kernel, once per instantiation)
+#define MDS_GRID(K)                                                                                    \
+  static const int nres = resident_blocks(K, 64 * waves_per_block);                                   \
+  dim3 grid((unsigned)std::max(1, a->persist ? std::min(need, nres * std::max(1, a->ncu)) : need));
+#define MDS_L1(K, ...) do { MDS_GRID((K<__VA_ARGS__>)) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h, stage); } while (0)
+#define MDS_L2(K, ...) do { MDS_GRID((K<__VA_ARGS__>)) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h); } while (0)
   if (!fp32) {
     const StateArrays<double> &p = *(const StateArrays<double> *)sp;
     const StateArrays<double> &h = *(const StateArrays<double> *)sph_;
@@ -817,5 +851,6 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
   }
 #undef MDS_L1
 #undef MDS_L2
+#undef MDS_GRID
   return hipGetLastError();
 }

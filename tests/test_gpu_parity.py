@@ -344,14 +344,41 @@ def test_results_independent_of_launch_shape(precision):
         assert np.array_equal(o, outs3[0])
 
 
-def _batch_ctx(ens, precision, fuse, tmax=None):
+def _batch_ctx(ens, precision, fuse, tmax=None, grid=0):
     rp = mds.run_params(cfl=0.5) if tmax is None else mds.run_params(cfl=0.5, tmax=tmax)
     ctx = mds.Context([e[1] for e in ens], precision=precision, rp=rp)
     ctx.set_tile(mds.TILE_FUSE1D, fuse)
+    ctx.set_tile(mds.TILE_GRID1D, grid)
     for k, e in enumerate(ens):
         ctx.upload(e[2], k)
     ctx.update_eos()
     return ctx
+
+
+@pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
+def test_persistent_1d_grid_matches_per_segment_grid(precision):
+    """The persistent 1D grid (the workgroups the device holds at once walk
+    the segment table) gives the same bits as one wavefront per segment.  The
+    batch has about 14k segments, more than the device holds resident, so the
+    persistent wavefronts each take several segments."""
+    par, ens = problems.ensemble(720, 900, 1300, seed=91)
+    ens = list(ens)
+    nseg = sum(int(r[0]) for r in par) // 56
+    say(f"persistent grid precision={precision} problems={len(ens)} two-stage segments~{nseg}")
+    outs = []
+    for fuse in (0, 1):
+        for grid in (0, 1):
+            ctx = _batch_ctx(ens, precision, fuse, grid=grid)
+            ctx.run(3)
+            ctx.run(2, graph=True)
+            outs.append(([ctx.download(k) for k in range(0, len(ens), 37)],
+                         [ctx.clock(k) for k in range(len(ens))], ctx.counters()))
+            del ctx
+    for o in outs[1:]:
+        for a, b in zip(outs[0][0], o[0]):
+            assert np.array_equal(a, b)
+        assert outs[0][1] == o[1]
+        assert outs[0][2] == o[2]
 
 
 @pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
