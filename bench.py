@@ -75,7 +75,9 @@ def workload(name, n):
         return dict(probs=[(e[1], e[2]) for e in ens], nvar=3, steps=500, zones=zones, gam=list(par[:, 1]),
                     label=f"ensemble of {nprob} spherical log-r Riemann columns, 256-8192 zones (seeded), "
                           f"{zones} zones, per-problem dt, 500 steps",
-                    kernel={"fp32": "sweep1d_f32<1,true>", "fp64": "sweep1d_x64<3,true>"}, n=nprob)
+                    kernel={"fp32": "sweep1d_f32<1,true> per stage, or sweep1d2_f32<1,true> for both stages",
+                            "fp64": "sweep1d_x64<3,true> per stage, or sweep1d2_x64<3,true> for both stages"},
+                    n=nprob)
     if name == "star3d":
         n1 = n or 640
         n2 = 1 << max(3, round(math.log2(max(8, n1 // 5))))
@@ -213,6 +215,8 @@ def main():
     ap.add_argument("--workload", default="kh2d", choices=["kh2d", "rt3d", "sod1d", "ensemble", "star3d"])
     ap.add_argument("--n", type=int, default=None, help="grid size (problem count for ensemble)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time budget")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="print the device's FMA issue rate (wavefront instructions/s) and exit")
     ap.add_argument("--tiles", default=None,
                     help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D]] launch shape to use instead of autotuning (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=None, metavar="BYTES",
@@ -234,6 +238,10 @@ def main():
         ms = mds.copy_probe_ms(args.copy_probe, 5)
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "ms": ms,
                           "GB/s": 2.0 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if args.valu_probe:
+        print(json.dumps({"valu_wave_instr_per_s": {"fp32": mds.valu_probe_rate(False),
+                                                    "fp64": mds.valu_probe_rate(True)}}), flush=True)
         return
     tiles = [int(t) for t in args.tiles.split(",")] if args.tiles else None
 
@@ -271,9 +279,20 @@ def main():
     bpz = 5 * W["nvar"] * (4 if head_name == "fp32" else 8)      # algorithmic bytes per zone-update
     bpl = bpz / head["launches"] * zones                           # per sweep launch
     achieved = bpl / (head["avg_launch_ms"] * 1e-3) / 1e9
+    kname = W["kernel"][head_name]
+    if args.workload in ("sod1d", "ensemble"):
+        # the launch the autotuner (or --tiles) chose: both RK stages in one, or one per stage
+        two = head["launches"] == 1
+        base = {"sod1d": ("sweep1d_f32<1,false>", "sweep1d_x64<1,false>"),
+                "ensemble": ("sweep1d_f32<1,true>", "sweep1d_x64<3,true>")}[args.workload]
+        kname = base[0 if head_name == "fp32" else 1]
+        if two:
+            kname = kname.replace("sweep1d_", "sweep1d2_") + " (both RK stages per launch)"
+    elif args.workload == "rt3d":
+        kname = kname.replace("TY", str(head["tiles"][1]))
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.workload, head_name),
-                    kernel=W["kernel"][head_name], bytes_per_launch=bpl,
+                    kernel=kname, bytes_per_launch=bpl,
                     avg_launch_us=round(head["avg_launch_ms"] * 1e3, 2))
 
     cpu = None

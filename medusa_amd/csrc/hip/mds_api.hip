@@ -989,6 +989,55 @@ extern "C" int mds_copy_probe(long long nbytes, int reps, double *ms)
   return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
 }
 
+// eight independent FMA chains per lane: the VALU issue rate without memory
+template <class T>
+__global__ __launch_bounds__(256) void valu_probe_kernel(T *out, int iters, T a, T b)
+{
+  T x[8];
+  for (int k = 0; k < 8; k++) x[k] = (T)(threadIdx.x * 1.0e-3 + k);
+  for (int i = 0; i < iters; i++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = x[k] * a + b;
+  }
+  T s = 0;
+  for (int k = 0; k < 8; k++) s += x[k];
+  out[(long long)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+template <class T>
+static hipError_t valu_probe(int reps, double *rate)
+{
+  int dev = 0, ncu = 256;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int blocks = ncu * 16, iters = 4000;
+  T *d = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(T) * (size_t)blocks * 256);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) hipLaunchKernelGGL(valu_probe_kernel<T>, dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++)
+    hipLaunchKernelGGL(valu_probe_kernel<T>, dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float f = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+  if (e == hipSuccess) *rate = (double)blocks * 4.0 * iters * 8.0 * reps / (f * 1.0e-3);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (d) hipFree(d);
+  return e;
+}
+
+extern "C" int mds_valu_probe(int fp64, int reps, double *wave_instr_per_s)
+{
+  if (reps < 1 || !wave_instr_per_s) return MDS_EINVAL;
+  const hipError_t e = fp64 ? valu_probe<double>(reps, wave_instr_per_s) : valu_probe<float>(reps, wave_instr_per_s);
+  return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}
+
 extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
 {
   if (!c) return MDS_EINVAL;

@@ -96,7 +96,7 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
-            "mds_set_tile", "mds_get_tile", "mds_autotune",
+            "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
@@ -137,6 +137,7 @@ def lib():
     L.mds_stream.restype = vp
     L.mds_run_timed.argtypes = [vp, ctypes.c_int, _dp]
     L.mds_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, _dp]
+    L.mds_valu_probe.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
     L.mds_set_tile.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.mds_get_tile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mds_autotune.argtypes = [vp, ctypes.c_int, _dp]
@@ -394,6 +395,14 @@ def copy_probe_ms(nbytes, reps=20):
     out = np.zeros(1)
     if lib().mds_copy_probe(int(nbytes), reps, _ptr(out)):
         raise RuntimeError("mds_copy_probe failed")
+    return float(out[0])
+
+
+def valu_probe_rate(fp64=False, reps=5):
+    """Wavefront FMA instructions per second of the device (FP32 or FP64)."""
+    out = np.zeros(1)
+    if lib().mds_valu_probe(int(bool(fp64)), reps, _ptr(out)):
+        raise RuntimeError("mds_valu_probe failed")
     return float(out[0])
 
 
