@@ -997,7 +997,7 @@ __global__ __launch_bounds__(256) void valu_probe_kernel(T *out, int iters, T a,
   for (int k = 0; k < 8; k++) x[k] = (T)(threadIdx.x * 1.0e-3 + k);
   for (int i = 0; i < iters; i++) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) x[k] = x[k] * a + b;
+    for (int k = 0; k < 8; k++) x[k] = fma(x[k], a, b);   // one v_fma per chain (the library builds with contraction off)
   }
   T s = 0;
   for (int k = 0; k < 8; k++) s += x[k];
