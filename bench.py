@@ -108,7 +108,8 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
         ctx.upload(prims, k)
     ctx.update_eos()
     if tiles:
-        for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D, mds.TILE_GRID1D), tiles):
+        for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D, mds.TILE_GRID1D,
+                         mds.TILE_HALO3D), tiles):
             if t not in (mds.TILE_FUSE1D, mds.TILE_GRID1D) or ctx.ndim == 1:
                 ctx.set_tile(t, v)
         tuned_ms = None
@@ -136,7 +137,7 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
                 stage_ms=(st1_ms / steps, st2_ms / steps), mass_drift=float(drift), energy_drift=float(edrift),
                 counters=ctx.counters(), tuned_ms=tuned_ms,
                 tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D,
-                                                 mds.TILE_GRID1D)])
+                                                 mds.TILE_GRID1D, mds.TILE_HALO3D)])
 
 
 def cpu_baseline(name, W, budget_s):
@@ -218,7 +219,8 @@ def main():
     ap.add_argument("--valu-probe", action="store_true",
                     help="print the device's FMA issue rate (wavefront instructions/s) and exit")
     ap.add_argument("--tiles", default=None,
-                    help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D]] launch shape to use instead of autotuning (profiling runs)")
+                    help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D[,HALO3D]]] launch shape to use instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=None, metavar="BYTES",
                     help="only run the device copy kernel over BYTES (counter calibration) and exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -290,6 +292,8 @@ def main():
             kname = kname.replace("sweep1d_", "sweep1d2_") + " (both RK stages per launch)"
     elif args.workload == "rt3d":
         kname = kname.replace("TY", str(head["tiles"][1]))
+        if head["tiles"][5]:
+            kname = kname.replace("sweep3d<", "sweep3dh<")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.workload, head_name),
                     kernel=kname, bytes_per_launch=bpl,

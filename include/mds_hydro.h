@@ -237,6 +237,8 @@ int mds_sync(mds_ctx *ctx);
 #define MDS_TILE_GRID1D  4       /* 1D grid: 0 = one wavefront per segment of the segment table,
                                     1 = persistent (the workgroups the device holds at once walk
                                     the whole table) */
+#define MDS_TILE_HALO3D  5       /* 3D: 0 = the four halo rows of a tile on wavefronts of their own,
+                                    1 = loaded by the output wavefronts beside their rows */
 int mds_set_tile(mds_ctx *ctx, int param, int value);
 int mds_get_tile(mds_ctx *ctx, int param, int *value);
 

@@ -25,7 +25,7 @@ extern "C" hipError_t mds_launch_sph3d(const Sph3Launch *L, const DevProb *pb, D
                                        const void *sph, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a, const void *sp,
                                          const void *sph, int stage, hipStream_t st);
-extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, const Sweep3DArgs *a, const void *sp,
+extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, int hshare, const Sweep3DArgs *a, const void *sp,
                                          const void *sph, int stage, hipStream_t st);
 
 struct mds_ctx {
@@ -69,6 +69,7 @@ struct mds_ctx {
   int ty3d = 4;
   int ncu = 0;                          // compute units of the device
   int grid1d = 0;                       // 1D grid: 0 one wavefront per segment, 1 persistent
+  int halo3d = 0;                       // 3D: 1 = the output wavefronts also load the halo rows
   int kc3d = 16;
   hipGraph_t graph[2] = {nullptr, nullptr};        // one step, per buffer parity
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -785,10 +786,10 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.src = (h.g0[0] != 0.0 || h.g0[1] != 0.0 || h.g0[2] != 0.0);
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
-      HIPCHK(c, mds_launch_sweep3d(1, c->ty3d, &a, &sp, &sh, stage, c->st));
+      HIPCHK(c, mds_launch_sweep3d(1, c->ty3d, c->halo3d, &a, &sp, &sh, stage, c->st));
     } else {
       StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-      HIPCHK(c, mds_launch_sweep3d(0, c->ty3d, &a, &sp, &sh, stage, c->st));
+      HIPCHK(c, mds_launch_sweep3d(0, c->ty3d, c->halo3d, &a, &sp, &sh, stage, c->st));
     }
   } else {
     Sweep2DArgs a;
@@ -1057,6 +1058,9 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
     if (value && !c->can_fuse1d)
       return set_err(c, MDS_EINVAL, "the two-stage 1D kernel needs 1D problems without periodic x boundaries");
     slot = &c->fuse1d;
+  } else if (param == MDS_TILE_HALO3D) {
+    if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "3D halo mode must be 0 or 1: %d", value);
+    slot = &c->halo3d;
   } else if (param == MDS_TILE_GRID1D) {
     if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "1D grid mode must be 0 or 1: %d", value);
     slot = &c->grid1d;
@@ -1078,6 +1082,7 @@ extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
   if (param == MDS_TILE_KC3D) { *value = c->kc3d; return MDS_OK; }
   if (param == MDS_TILE_FUSE1D) { *value = c->fuse1d; return MDS_OK; }
   if (param == MDS_TILE_GRID1D) { *value = c->grid1d; return MDS_OK; }
+  if (param == MDS_TILE_HALO3D) { *value = c->halo3d; return MDS_OK; }
   return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
 }
 
@@ -1137,34 +1142,38 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
   if (c->sph3) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // fixed shape: one thread per face/zone
-  // candidate shapes: 1D single/two-stage kernels; 2D rows per wavefront;
-  // 3D (rows per workgroup, planes per workgroup)
-  std::vector<std::pair<int, int>> cand;
+  // candidates: (a, b, c) = 1D (two-stage launch, persistent grid, -);
+  // 2D (rows per wavefront, -, -); 3D (rows per workgroup, planes, halo mode)
+  struct Shape { int a, b, c; };
+  std::vector<Shape> cand;
   if (c->ndim == 1) {
-    // (two-stage launch, persistent grid)
     for (int g : {0, 1}) {
-      cand.push_back({0, g});
-      if (c->can_fuse1d) cand.push_back({1, g});
+      cand.push_back({0, g, 0});
+      if (c->can_fuse1d) cand.push_back({1, g, 0});
     }
   } else if (c->ndim == 2) {
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
-      if (v <= c->hprob[0].n[1]) cand.push_back({v, 0});
+      if (v <= c->hprob[0].n[1]) cand.push_back({v, 0, 0});
   } else {
-    for (int ty : {4, 8, 12})
-      for (int kc : {8, 16, 32, 64})
-        if (kc <= c->hprob[0].n[2]) cand.push_back({ty, kc});
+    for (int hs : {0, 1})
+      for (int ty : {4, 8, 12})
+        for (int kc : {8, 16, 32, 64})
+          if (kc <= c->hprob[0].n[2]) cand.push_back({ty, kc, hs});
   }
+  auto apply = [&](const Shape &v) {
+    if (c->ndim == 1) { c->fuse1d = v.a; c->grid1d = v.b; }
+    else if (c->ndim == 2) c->chunk2d = v.a;
+    else { c->ty3d = v.a; c->kc3d = v.b; c->halo3d = v.c; }
+  };
   Snapshot s;
   int rc = snap_alloc(c, s);
   if (rc == MDS_OK) rc = snap_copy(c, s, true);
-  std::pair<int, int> best = c->ndim == 1 ? std::make_pair(c->fuse1d, c->grid1d)
-                             : c->ndim == 2 ? std::make_pair(c->chunk2d, 0) : std::make_pair(c->ty3d, c->kc3d);
+  Shape best = c->ndim == 1 ? Shape{c->fuse1d, c->grid1d, 0}
+               : c->ndim == 2 ? Shape{c->chunk2d, 0, 0} : Shape{c->ty3d, c->kc3d, c->halo3d};
   double bms = 1e300;
-  for (auto v : cand) {
+  for (const Shape &v : cand) {
     if (rc) break;
-    if (c->ndim == 1) { c->fuse1d = v.first; c->grid1d = v.second; }
-    else if (c->ndim == 2) c->chunk2d = v.first;
-    else { c->ty3d = v.first; c->kc3d = v.second; }
+    apply(v);
     double ms[3];
     rc = mds_run_timed(c, 1, ms);      // warm the shape
     // best of two timed runs from the same state: one run of a few steps is
@@ -1179,9 +1188,7 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     if (rc == MDS_OK) rc = snap_copy(c, s, false);
   }
   snap_free(s);
-  if (c->ndim == 1) { c->fuse1d = best.first; c->grid1d = best.second; }
-  else if (c->ndim == 2) c->chunk2d = best.first;
-  else { c->ty3d = best.first; c->kc3d = best.second; }
+  apply(best);
   drop_graphs(c);
   if (best_ms) *best_ms = bms;
   return rc;

@@ -23,6 +23,7 @@ TILE_TY3D = 1
 TILE_KC3D = 2
 TILE_FUSE1D = 3
 TILE_GRID1D = 4
+TILE_HALO3D = 5
 ZG_COUNT = 24
 # 3D spherical per-(i,j) record (MDS_ZS_*)
 (ZS_VOLUME, ZS_AREA0, ZS_AREA1, ZS_AREA2, ZS_GCOV0) = range(5)

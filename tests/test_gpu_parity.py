@@ -329,10 +329,11 @@ def test_results_independent_of_launch_shape(precision):
         outs.append(ctx.download())
     g3, prob3, prims3 = problems.rt(20)
     outs3 = []
-    for ty, kc in ((4, 3), (12, 8), (4, 20), (8, 5)):
+    for ty, kc, hs in ((4, 3, 0), (12, 8, 0), (4, 20, 0), (8, 5, 0), (4, 3, 1), (8, 5, 1), (12, 7, 1)):
         ctx = mds.Context([prob3], precision=precision, rp=mds.run_params(cfl=0.5))
         ctx.set_tile(mds.TILE_TY3D, ty)
         ctx.set_tile(mds.TILE_KC3D, kc)
+        ctx.set_tile(mds.TILE_HALO3D, hs)
         ctx.upload(prims3)
         ctx.update_eos()
         ctx.run(4)
