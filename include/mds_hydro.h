@@ -38,7 +38,9 @@ extern "C" {
 #define MDS_BC_OUTFLOW   0
 #define MDS_BC_REFLECT   1
 #define MDS_BC_PERIODIC  3
-#define MDS_BC_SPHERICAL_ORIGIN 5 /* theta boundary of 3D spherical grids: across the pole */
+#define MDS_BC_SPHERICAL_ORIGIN 5 /* spherical grids: theta boundary across the pole (3D), or inner
+                                     radius through the origin (1D, 3D; theta -> pi - theta,
+                                     phi + pi, v^1 and v^3 flipped) */
 
 #define MDS_GEOM_CARTESIAN 0
 #define MDS_GEOM_SPHERICAL 1     /* 1D radial columns or 3D (r, theta, phi) grids */

@@ -424,18 +424,24 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       if (g->n[d] < 4) return set_err(c, MDS_EINVAL, "problem %d: need at least 4 zones per active direction", p);
       for (int side = 0; side < 2; side++) {
         int b = side ? P.bc_hi[d] : P.bc_lo[d];
+        // SPHERICAL_ORIGIN: theta across the pole of 3D spherical grids, or
+        // the inner radius through the origin of 1D and 3D spherical grids
         const bool pole = c->sph3 && d == 1 && b == MDS_BC_SPHERICAL_ORIGIN && (g->n[2] % 2) == 0;
-        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole)
+        const bool origin = c->sph && d == 0 && side == 0 && b == MDS_BC_SPHERICAL_ORIGIN &&
+                            (!c->sph3 || (g->n[2] % 2) == 0);
+        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole && !origin)
           return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
       }
     }
     if (g->dendritic) {
       if (!c->sph3 || !g->dj || !g->dk) return set_err(c, MDS_EINVAL, "problem %d: dendritic grids are built 3D spherical grids", p);
-      if (P.bc_lo[0] != MDS_BC_REFLECT || (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT) ||
+      if ((P.bc_lo[0] != MDS_BC_REFLECT && P.bc_lo[0] != MDS_BC_SPHERICAL_ORIGIN) ||
+          (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT) ||
           P.bc_lo[1] != MDS_BC_SPHERICAL_ORIGIN || P.bc_hi[1] != MDS_BC_SPHERICAL_ORIGIN ||
           P.bc_lo[2] != MDS_BC_PERIODIC || P.bc_hi[2] != MDS_BC_PERIODIC)
         return set_err(c, MDS_ENOTIMPL,
-                       "problem %d: dendritic grids need REFLECT inner / OUTFLOW or REFLECT outer radius, the pole in "
+                       "problem %d: dendritic grids need REFLECT or SPHERICAL_ORIGIN inner / OUTFLOW or REFLECT outer "
+                       "radius, the pole in "
                        "theta and periodic phi", p);
     }
     long long nz = (long long)g->n[0] * (g->ndim > 1 ? g->n[1] : 1) * (g->ndim > 2 ? g->n[2] : 1);

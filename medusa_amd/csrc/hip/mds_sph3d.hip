@@ -107,7 +107,9 @@ MDS_DEV const float *rc_of(const DevProb &pb, int D, int lev)
 // Map a stencil zone that leaves the grid in direction D onto the zone the
 // ghost fill copies there; sgn gets the velocity sign flips.  n is the zone
 // count along D at the zone's level; nkm (theta across the pole only) the phi
-// count of the mirrored row.
+// count of the mirrored row.  The inner radius through the origin
+// (SPHERICAL_ORIGIN, bc.c:774-801) mirrors i, takes theta to pi - theta at
+// the level of the mirrored shell and phi to phi + pi, flipping v^1 and v^3.
 MDS_DEV void map_zone_n(const DevProb &pb, int D, int &i, int &j, int &k, double sgn[3], int n, int nkm)
 {
   sgn[0] = sgn[1] = sgn[2] = 1.0;
@@ -126,6 +128,12 @@ MDS_DEV void map_zone_n(const DevProb &pb, int D, int &i, int &j, int &k, double
     *c = side ? 2 * n - 1 - ig : -1 - ig;
     k = (k + nkm / 2) % nkm;
     sgn[1] = sgn[2] = -1.0;
+  } else if (code == MDS_BC_SPHERICAL_ORIGIN && D == 0 && !side) {
+    i = -1 - ig;
+    j = nj_of(pb, i) - 1 - j;
+    const int nk = nk_of(pb, i, j);
+    k = (k + nk / 2) % nk;
+    sgn[0] = sgn[2] = -1.0;
   } else {
     *c = side ? n - 1 : 0;
   }
@@ -152,7 +160,7 @@ MDS_DEV void map_any(const DevProb &pb, int &i, int &j, int &k, double sgn[3])
   double s[3];
   if (i < 0 || i >= pb.n[0]) {
     map_zone(pb, 0, i, j, k, s);
-    sgn[0] *= s[0];
+    for (int d = 0; d < 3; d++) sgn[d] *= s[d];
   }
   if (j < 0 || j >= nj_of(pb, i)) {
     map_zone(pb, 1, i, j, k, s);

@@ -38,19 +38,23 @@
 namespace {
 
 // Map a stencil zone index onto the interior zone that the reference's ghost
-// fill would copy there (bc.c:727-1030 for direction 0).
-MDS_DEV int bc_src(int c, int n, int lo, int hi, bool &flip)
+// fill would copy there (bc.c:727-1030 for direction 0).  flip gets the
+// velocity components whose sign the fill changes (bit d: v^(d+1)): REFLECT
+// v^1, the inner radius through the origin (SPHERICAL_ORIGIN, bc.c:774-801)
+// v^1 and v^3.
+MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
 {
-  flip = false;
+  flip = 0;
   if (c < 0) {
     if (c < -MDS_NG) return 0;
-    if (lo == MDS_BC_REFLECT) { flip = true; return -1 - c; }
+    if (lo == MDS_BC_REFLECT) { flip = 1; return -1 - c; }
+    if (lo == MDS_BC_SPHERICAL_ORIGIN) { flip = 5; return -1 - c; }
     if (lo == MDS_BC_PERIODIC) return c + n;
     return 0;
   }
   if (c >= n) {
     if (c >= n + MDS_NG) return n - 1;
-    if (hi == MDS_BC_REFLECT) { flip = true; return 2 * n - 1 - c; }
+    if (hi == MDS_BC_REFLECT) { flip = 1; return 2 * n - 1 - c; }
     if (hi == MDS_BC_PERIODIC) return c - n;
     return n - 1;
   }
@@ -109,7 +113,7 @@ MDS_DEV void commit_limits(const DevProb &pb, DevClock *ck, unsigned long long *
 template <int NV, int TAB, class R>
 MDS_DEV void ghost_permute(int c, int n, int c0, int lo, int hi, Prim<R> &q)
 {
-  bool flip;
+  int flip;
   const int s = bc_src(c, n, lo, hi, flip);
   int src_lane = s - c0 + 2 * MDS_NG;
   src_lane = src_lane < 0 ? 0 : (src_lane > 63 ? 63 : src_lane);
@@ -117,11 +121,12 @@ MDS_DEV void ghost_permute(int c, int n, int c0, int lo, int hi, Prim<R> &q)
   q.uu = __shfl(q.uu, src_lane, 64);
   for (int d = 0; d < NV; d++) q.v[d] = __shfl(q.v[d], src_lane, 64);
   if (TAB) q.ye = __shfl(q.ye, src_lane, 64);
-  if (flip) q.v[0] = -q.v[0];
+  if (flip & 1) q.v[0] = -q.v[0];
+  if (NV > 2 && (flip & 4)) q.v[2] = -q.v[2];
 }
 
 template <int NV, int TAB, class R>
-MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, bool flip)
+MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, int flip)
 {
   Prim<R> q;
   q.rho = in.v[0][off + s];
@@ -129,7 +134,8 @@ MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, bool 
   for (int d = 0; d < 3; d++) q.v[d] = R(0);
   for (int d = 0; d < NV; d++) q.v[d] = in.v[2 + d][off + s];
   q.ye = TAB ? in.v[5][off + s] : R(0);
-  if (flip) q.v[0] = -q.v[0];
+  if (flip & 1) q.v[0] = -q.v[0];
+  if (NV > 2 && (flip & 4)) q.v[2] = -q.v[2];
   return q;
 }
 
@@ -368,7 +374,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     const long long off = pb.off;
 
     const StateArrays<double> &in = (stage == 0) ? sp : sph;
-    bool flip;
+    int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
     Geo64 g;
@@ -380,7 +386,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
     // U^n of this zone: the stage input itself in stage 0, sim_p in stage 1
     Prim<double> qn = q;
-    if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, false);
+    if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, 0);
     double U0[6];
     conserved_x64<NV, SPH, TAB>(qn, g.gcov0, U0);
     const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
@@ -417,11 +423,11 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
     const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
     if (!ck->active) {
       // frozen problem: carry its state over to the output buffer
-      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
+      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, 0));
       continue;
     }
     const double dt = ck->dt;
-    bool flip;
+    int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
     Geo64 g;
@@ -647,7 +653,7 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     const long long off = pb.off;
 
     const StateArrays<float> &in = (stage == 0) ? sp : sph;
-    bool flip;
+    int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
     Geo32 g;
@@ -657,7 +663,7 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     r.count = r.face && (lane <= 61 || c == n);
     r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
     Prim<float> qnf = q;
-    if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, false);
+    if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, 0);
     double U0[6];
     Prim<double> qn;
     conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
@@ -693,11 +699,11 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
     const long long off = pb.off;
     const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
     if (!ck->active) {
-      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, false));
+      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, 0));
       continue;
     }
     const double dt = ck->dt;
-    bool flip;
+    int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
     Geo32 g;

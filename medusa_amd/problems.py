@@ -104,11 +104,16 @@ def dend3d_grid(n1, n2, n3, dr0=5.0e4, rout=2.0e9, dendritic=True):
 def star3d(n1=640, n2=128, n3=256, table=None, gam=4.0 / 3.0, sigma=0.01, seed=0x5EED0006, monopole=True,
            dendritic=False, **kw):
     """The polytropic star (n=3, rho_c = 1e10) on a 3D spherical grid with
-    monopole gravity; reflecting inner radius, outflow outer radius, theta
-    across the poles, phi periodic.  dendritic=False: log r over [1e5, 2e9]
-    on a uniform angular mesh; True: the dendritic grid of dend3d_grid (the
-    log r map cannot carry one: its zone aspect is constant, so the level
-    counts double in every inner shell and the 2:1 rule fails)."""
+    monopole gravity; outflow outer radius, theta across the poles, phi
+    periodic.  dendritic=False: log r over [1e5, 2e9] on a uniform angular
+    mesh with a reflecting inner radius; True: the dendritic grid of
+    dend3d_grid (the log r map cannot carry one: its zone aspect is constant,
+    so the level counts double in every inner shell and the 2:1 rule fails),
+    which reaches the centre, with the inner radius through the origin as in
+    the reference's core-collapse setup (problems/init_ccsn.c:112)."""
     g = dend3d_grid(n1, n2, n3) if dendritic else sph3d_grid(n1, n2, n3)
-    prob = dict(grid=g, bc_lo=list(SPH3D_BC_LO), bc_hi=list(SPH3D_BC_HI), gam=gam, monopole=monopole, eos=table)
+    bc_lo = list(SPH3D_BC_LO)
+    if dendritic:
+        bc_lo[0] = mds.BC_SPHERICAL_ORIGIN
+    prob = dict(grid=g, bc_lo=bc_lo, bc_hi=list(SPH3D_BC_HI), gam=gam, monopole=monopole, eos=table)
     return g, prob, mds.init_polytrope(g, gam=gam, table=table, sigma=sigma, seed=seed, **kw)

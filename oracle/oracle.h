@@ -41,7 +41,8 @@ extern "C" {
 #define ORC_BC_REFLECT  1
 #define ORC_BC_EXTRAP   2
 #define ORC_BC_PERIODIC 3
-#define ORC_BC_SPHERICAL_ORIGIN 5   /* theta: across the pole (phi + pi); decs.h:69 */
+#define ORC_BC_SPHERICAL_ORIGIN 5   /* theta: across the pole (phi + pi); inner radius: through the
+                                       origin (theta -> pi - theta, phi + pi); decs.h:69 */
 
 #define ORC_GEOM_CARTESIAN 0
 #define ORC_GEOM_SPHERICAL 1
@@ -131,6 +132,8 @@ void orc_set_prims(orc_sim *s, const double *prims);
 void orc_get_prims(const orc_sim *s, double *prims);
 /* geometry export for cross-checks: per zone incl. ghosts, natural order */
 int orc_zone_count_all(const orc_sim *s);
+void orc_fill_ghosts(orc_sim *s);                /* bc.c:705-1421 on the current state */
+int orc_zone_prims(const orc_sim *s, int i, int j, int k, double *out);   /* any zone, ghosts included */
 const orc_zone_geom *orc_geometry(const orc_sim *s);
 /* volume moments per direction, length n[d]+2*NG (index 0 = zone -NG) */
 const double *orc_moment(const orc_sim *s, int which /*0 alpha,1 beta,2 Gamma*/, int dir);

@@ -664,12 +664,16 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
       for (int gidx = 0; gidx < ORC_NG; gidx++) {
         int ig = side ? nd + gidx : -ORC_NG + gidx;   /* ghost index along d */
         int src = 0, flip = 0, extrap = 0, pole = 0;
+        int origin = 0;
         switch (code) {
           case ORC_BC_SPHERICAL_ORIGIN:
             /* theta boundary across the pole: mirror in j, phi + pi, flip v2 and v3
-             * (bc.c:1094-1121, 1235-1262); other directions are not supported */
+             * (bc.c:1094-1121, 1235-1262); inner radial boundary through the
+             * origin: mirror in i, theta -> pi - theta, phi + pi, flip v1 and v3
+             * (bc.c:774-801) */
             src = side ? 2 * nd - ig - 1 : -1 - ig;
             pole = (d == 1);
+            origin = (d == 0 && !side);
             break;
           case ORC_BC_OUTFLOW:
             src = side ? nd - 1 : 0;
@@ -692,6 +696,10 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
         idx[d] = ig; idx[o1] = a; idx[o2] = b;
         sidx[d] = src; sidx[o1] = a; sidx[o2] = b;
         if (pole) sidx[2] = (idx[2] + c->n[2] / 2) % c->n[2];
+        if (origin) {
+          sidx[1] = c->n[1] - 1 - idx[1];
+          sidx[2] = (idx[2] + c->n[2] / 2) % c->n[2];
+        }
         double *dst = &P[zid(s, idx[0], idx[1], idx[2]) * nv];
         if (extrap) {
           int i0, i1;
@@ -710,6 +718,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           for (int v = 0; v < nv; v++) dst[v] = sp[v];
           if (flip) dst[ORC_U1 + d] *= -1.0;
           if (pole) { dst[ORC_U2] *= -1.0; dst[ORC_U3] *= -1.0; }
+          if (origin) { dst[ORC_U1] *= -1.0; dst[ORC_U3] *= -1.0; }
         }
       }
 }
@@ -832,7 +841,8 @@ static void polar_states(orc_sim *s, const double *P, int i, int j, int k, doubl
 /* ------------------------------------------------------------------ */
 /* Dendritic 3D spherical grids */
 
-/* bc.c:705-1421 on a dendritic grid: radial REFLECT/OUTFLOW, theta across
+/* bc.c:705-1421 on a dendritic grid: radial REFLECT/OUTFLOW or through the
+ * origin (inner radius, SPHERICAL_ORIGIN), theta across
  * the pole (SPHERICAL_ORIGIN), phi PERIODIC, each at the level of the ghost
  * zone's own row.  Filled in the order radial, theta, phi; the theta fill
  * also covers the radial ghost zones and the phi fill every row including
@@ -850,10 +860,16 @@ static void fill_ghost_dend(orc_sim *s, double *P)
       for (int j = 0; j < njs(s, i); j++)
         for (int k = 0; k < nks(s, i, j); k++) {
           int ip, jp = j, kp = k;
+          const int origin = code == ORC_BC_SPHERICAL_ORIGIN && !side;
           if (code == ORC_BC_REFLECT) {
             ip = side ? 2 * n1 - 1 - i : -1 - i;
             jp = j * djs(s, i) / djs(s, ip);
             kp = k * dks(s, i, j) / dks(s, ip, jp);
+          } else if (origin) {
+            /* bc.c:774-801: i and j reflected across the origin, k + nk/2 */
+            ip = -1 - i;
+            jp = njs(s, ip) - 1 - j;
+            kp = (k + nks(s, ip, jp) / 2) % nks(s, ip, jp);
           } else {
             ip = side ? n1 - 1 : 0;
           }
@@ -861,6 +877,7 @@ static void fill_ghost_dend(orc_sim *s, double *P)
           const double *src = &P[zid(s, ip, jp, kp) * nv];
           for (int v = 0; v < nv; v++) dst[v] = src[v];
           if (code == ORC_BC_REFLECT) dst[ORC_U1] *= -1.0;
+          if (origin) { dst[ORC_U1] *= -1.0; dst[ORC_U3] *= -1.0; }
         }
     }
   }
@@ -1386,6 +1403,24 @@ static void monopole(orc_sim *s, const double *P)
       }
   }
   free(grav);
+}
+
+/* The ghost fill that starts each stage, applied to the current state; with
+ * orc_zone_prims it lets a test read any ghost zone. */
+void orc_fill_ghosts(orc_sim *s)
+{
+  if (s->dj) fill_ghost_dend(s, s->p);
+  else reset_boundaries(s, s->p);
+}
+
+int orc_zone_prims(const orc_sim *s, int i, int j, int k, double *out)
+{
+  const int idx[3] = {i, j, k};
+  for (int d = 0; d < 3; d++)
+    if (idx[d] < -s->g[d] || idx[d] >= s->c.n[d] + s->g[d]) return -1;
+  const double *src = &s->p[zid(s, i, j, k) * s->nvars];
+  for (int v = 0; v < s->nvars; v++) out[v] = src[v];
+  return 0;
 }
 
 static void stage(orc_sim *s, int st)

@@ -85,6 +85,9 @@ def _load(fast=False):
     for fn in ("orc_shock_face_count", "orc_hlle_face_count"):
         getattr(lib, fn).argtypes = [ctypes.c_void_p]
         getattr(lib, fn).restype = ctypes.c_long
+    lib.orc_fill_ghosts.argtypes = [ctypes.c_void_p]
+    lib.orc_zone_prims.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    lib.orc_zone_prims.restype = ctypes.c_int
     lib.orc_zone_count_all.argtypes = [ctypes.c_void_p]
     lib.orc_zone_count_all.restype = ctypes.c_int
     lib.orc_geometry.argtypes = [ctypes.c_void_p]
@@ -228,6 +231,17 @@ class Sim:
         a = np.zeros(self.shape, dtype=np.float64)
         self.lib.orc_get_prims(self.h, ptr(a))
         return a
+
+    def fill_ghosts(self):
+        """Apply the stage-start ghost fill (bc.c:705-1421) to the current state."""
+        self.lib.orc_fill_ghosts(self.h)
+
+    def zone(self, i, j=0, k=0):
+        """Primitives of zone (i, j, k), ghost zones included (indices from -2)."""
+        out = np.zeros(self.nvars)
+        if self.lib.orc_zone_prims(self.h, i, j, k, ptr(out)) != 0:
+            raise IndexError((i, j, k))
+        return out
 
     def monopole_edges(self):
         out = np.zeros(self.n[0] + 1)

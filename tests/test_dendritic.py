@@ -291,16 +291,19 @@ def test_dendritic_blast_matches_oracle(precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_table", [False, True])
-def test_dendritic_star_fp64_matches_oracle(use_table):
+@pytest.mark.parametrize("use_table,inner", [(False, "reflect"), (True, "reflect"), (False, "origin"),
+                                             (True, "origin")])
+def test_dendritic_star_fp64_matches_oracle(use_table, inner):
     from medusa_amd import mds, problems
     T = mds.EosTable() if use_table else None
     n = (32, 16, 32)
     g = problems.dend3d_grid(*n, dr0=2.0e6, rout=2.0e9)
-    prob = dict(grid=g, bc_lo=list(problems.SPH3D_BC_LO), bc_hi=list(problems.SPH3D_BC_HI), gam=4.0 / 3.0,
-                monopole=True, eos=T)
+    bc_lo = list(problems.SPH3D_BC_LO)
+    if inner == "origin":
+        bc_lo[0] = mds.BC_SPHERICAL_ORIGIN
+    prob = dict(grid=g, bc_lo=bc_lo, bc_hi=list(problems.SPH3D_BC_HI), gam=4.0 / 3.0, monopole=True, eos=T)
     prims = mds.init_polytrope(g, table=T, sigma=0.01) * g.valid_mask()[None]
-    print(f"[dend] star {n} table={use_table} FP64, 10 steps", flush=True)
+    print(f"[dend] star {n} table={use_table} inner={inner} FP64, 10 steps", flush=True)
     ctx = mds.Context([prob], precision=mds.FP64, rp=mds.run_params(cfl=0.5))
     ctx.upload(prims)
     ctx.update_eos()
