@@ -37,7 +37,10 @@ extern "C" {
 /* boundary codes: same numbers and meaning as decs.h:64-72 */
 #define MDS_BC_OUTFLOW   0
 #define MDS_BC_REFLECT   1
+#define MDS_BC_EXTRAP    2 /* direction 0 only: linear extrapolation from the two edge zones
+                              (bc.c:806-821, 937-954; the reference fills no other direction) */
 #define MDS_BC_PERIODIC  3
+#define MDS_BC_PROB      4 /* ghost values of the problem, set with mds_set_prob_bounds (bc.c:125-140) */
 #define MDS_BC_SPHERICAL_ORIGIN 5 /* spherical grids: theta boundary across the pole (3D), or inner
                                      radius through the origin (1D, 3D; theta -> pi - theta,
                                      phi + pi, v^1 and v^3 flipped) */
@@ -215,6 +218,23 @@ const char *mds_last_error(const mds_ctx *ctx);
  * update to/from, main.c:180, step.c:16-24).  prims: nvars*n3*n2*n1 doubles. */
 int mds_upload_prims(mds_ctx *ctx, int prob, const double *prims, int nvars);
 int mds_download_prims(mds_ctx *ctx, int prob, double *prims, int nvars);
+
+/* The ghost values of the PROB sides of problem prob: fn(i, j, k, p, user) is
+ * called once for every ghost zone (i, j, k) of every PROB side -- the
+ * reference's per-problem hook prob_bounds(i, j, k, p) (problems/init_*.c,
+ * e.g. problems/init_rt_3d.c:223-236), which its fill calls per ghost zone
+ * (bc.c:125-140 and the like) -- and p receives rho, u, v1, v2, v3 and, with a
+ * table EOS, Ye.  (i, j, k): the ghost index along the side's direction
+ * (-2, -1 below the grid, n, n + 1 above), interior indices along the others
+ * (on a dendritic grid, positions of the ghost shell's own level; 0 for
+ * absent directions).  The values stay on the device for every later stage
+ * (time-independent; call again to change them); mds_run refuses a problem
+ * with a PROB side that was never set.  PROB is accepted on every side of
+ * Cartesian grids and 1D spherical columns and at the outer radius of 3D
+ * spherical grids; EXTRAP on both sides of direction 0, except the inner
+ * radius of 3D spherical grids. */
+typedef void (*mds_prob_bounds_fn)(int i, int j, int k, double *p, void *user);
+int mds_set_prob_bounds(mds_ctx *ctx, int prob, mds_prob_bounds_fn fn, void *user);
 
 /* Initial EOS fill and first dt estimate input (main.c:181, eos.c:4-18);
  * also resets t, the step counter and the dt history of every problem. */

@@ -17,6 +17,8 @@
 
 extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int mono, const Sweep1DArgs *a,
                                          const void *sp, const void *sph_, int stage, hipStream_t st);
+extern "C" hipError_t mds_launch_slab_fill(int fp32, const DevProb *probs, int nprob, long long items,
+                                           void *const *state, const DevEos *eos, int tab, int nv, hipStream_t st);
 extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
                                           hipStream_t s);
 hipError_t mds_eos_to_device(const mds_eos_table *t, DevEos *d, void **alloc);
@@ -71,6 +73,10 @@ struct mds_ctx {
   int grid1d = 0;                       // 1D grid: 0 one wavefront per segment, 1 persistent
   int halo3d = 0;                       // 3D: 1 = the output wavefronts also load the halo rows
   int kc3d = 16;
+  int has_slab = 0;                     // some problem has a PROB or EXTRAP side
+  long long slab_items = 0;             // most ghost zones of PROB/EXTRAP sides in one problem
+  std::vector<int> prob_pending;        // per problem: PROB sides whose values were never set
+  std::vector<int> hdj, hdk;            // dendritic grid: dj and dk with ghosts (the grid's layout)
   hipGraph_t graph[2] = {nullptr, nullptr};        // one step, per buffer parity
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   char err[512] = {0};
@@ -429,20 +435,37 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
         const bool pole = c->sph3 && d == 1 && b == MDS_BC_SPHERICAL_ORIGIN && (g->n[2] % 2) == 0;
         const bool origin = c->sph && d == 0 && side == 0 && b == MDS_BC_SPHERICAL_ORIGIN &&
                             (!c->sph3 || (g->n[2] % 2) == 0);
-        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole && !origin)
+        // ghost layers held on the device: PROB anywhere but the inner radius
+        // and the angles of 3D spherical grids; EXTRAP in direction 0 (the
+        // only direction the reference extrapolates), not at that inner radius
+        const bool slab = (b == MDS_BC_PROB && (!c->sph3 || (d == 0 && side == 1))) ||
+                          (b == MDS_BC_EXTRAP && d == 0 && (!c->sph3 || side == 1));
+        if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole && !origin && !slab)
           return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
       }
     }
     if (g->dendritic) {
       if (!c->sph3 || !g->dj || !g->dk) return set_err(c, MDS_EINVAL, "problem %d: dendritic grids are built 3D spherical grids", p);
       if ((P.bc_lo[0] != MDS_BC_REFLECT && P.bc_lo[0] != MDS_BC_SPHERICAL_ORIGIN) ||
-          (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT) ||
+          (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT && P.bc_hi[0] != MDS_BC_PROB &&
+           P.bc_hi[0] != MDS_BC_EXTRAP) ||
           P.bc_lo[1] != MDS_BC_SPHERICAL_ORIGIN || P.bc_hi[1] != MDS_BC_SPHERICAL_ORIGIN ||
           P.bc_lo[2] != MDS_BC_PERIODIC || P.bc_hi[2] != MDS_BC_PERIODIC)
         return set_err(c, MDS_ENOTIMPL,
-                       "problem %d: dendritic grids need REFLECT or SPHERICAL_ORIGIN inner / OUTFLOW or REFLECT outer "
-                       "radius, the pole in "
-                       "theta and periodic phi", p);
+                       "problem %d: dendritic grids need REFLECT or SPHERICAL_ORIGIN inner / OUTFLOW, REFLECT, PROB "
+                       "or EXTRAP outer radius, the pole in theta and periodic phi", p);
+      const int n1 = g->n[0], S1 = g->n[1] + 2 * MDS_NG;
+      c->hdj.assign(g->dj, g->dj + (n1 + 2 * MDS_NG));
+      c->hdk.assign(g->dk, g->dk + (size_t)(n1 + 2 * MDS_NG) * S1);
+      if (P.bc_hi[0] == MDS_BC_EXTRAP) {
+        // the reference extrapolates at the ghost's own (j, k): the two outer
+        // shells must share their levels
+        bool same = g->dj[n1 - 2 + MDS_NG] == g->dj[n1 - 1 + MDS_NG];
+        for (int j = 0; same && j < g->n[1] / g->dj[n1 - 1 + MDS_NG]; j++)
+          same = g->dk[(size_t)(n1 - 2 + MDS_NG) * S1 + j + MDS_NG] == g->dk[(size_t)(n1 - 1 + MDS_NG) * S1 + j + MDS_NG];
+        if (!same)
+          return set_err(c, MDS_ENOTIMPL, "problem %d: EXTRAP needs the two outer shells on one level", p);
+      }
     }
     long long nz = (long long)g->n[0] * (g->ndim > 1 ? g->n[1] : 1) * (g->ndim > 2 ? g->n[2] : 1);
     c->zstart[p + 1] = c->zstart[p] + nz;
@@ -464,6 +487,29 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       d.area[k] = g->area[k];
     }
     d.volume = g->volume;
+    // ghost layers of the PROB and EXTRAP sides
+    d.has_slab = 0;
+    {
+      long long items = 0;
+      int pending = 0;
+      for (int dd = 0; dd < g->ndim; dd++)
+        for (int side = 0; side < 2; side++) {
+          const int b = side ? P.bc_hi[dd] : P.bc_lo[dd];
+          if (b != MDS_BC_PROB && b != MDS_BC_EXTRAP) continue;
+          const size_t bytes = sizeof(double) * MDS_NG * MDS_SLAB_NV * (size_t)slab_T(d, dd);
+          void *m = nullptr;
+          HIPCHK(c, hipMalloc(&m, bytes));
+          c->geo_allocs.push_back(m);
+          HIPCHK(c, hipMemset(m, 0, bytes));
+          d.gs[dd][side] = (double *)m;
+          d.has_slab = 1;
+          items += MDS_NG * slab_T(d, dd);
+          pending += b == MDS_BC_PROB;
+        }
+      c->has_slab |= d.has_slab;
+      c->slab_items = std::max(c->slab_items, items);
+      c->prob_pending.push_back(pending);
+    }
     for (int k = 0; k < g->ndim; k++) {
       int nzk = g->n[k] + 2 * MDS_NG;
       std::vector<double> mom(3 * (size_t)nzk);
@@ -633,7 +679,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     HIPCHK(c, hipMemcpy(c->dseg2, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
     // the two-stage kernel needs every ghost source zone inside the wave, and
     // no field that depends on the whole column between the stages
-    c->can_fuse1d = !c->mono;
+    c->can_fuse1d = !c->mono && !c->has_slab;
     for (int p = 0; p < nprob; p++)
       if (c->hprob[p].bc_lo[0] == MDS_BC_PERIODIC || c->hprob[p].bc_hi[0] == MDS_BC_PERIODIC) c->can_fuse1d = 0;
     c->fuse1d = c->can_fuse1d;
@@ -662,6 +708,49 @@ extern "C" int mds_create(const mds_problem *probs, int nprob, const mds_run_par
     return rc;
   }
   *out = c;
+  return MDS_OK;
+}
+
+extern "C" int mds_set_prob_bounds(mds_ctx *c, int prob, mds_prob_bounds_fn fn, void *user)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !fn) return MDS_EINVAL;
+  const DevProb &pb = c->hprob[prob];
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int d = 0; d < pb.ndim; d++)
+    for (int side = 0; side < 2; side++) {
+      const int code = side ? pb.bc_hi[d] : pb.bc_lo[d];
+      if (code != MDS_BC_PROB) continue;
+      const long long T = slab_T(pb, d);
+      std::vector<double> h((size_t)MDS_NG * MDS_SLAB_NV * T, 0.0);
+      const int lo1 = d == 0 ? 1 : 0, lo2 = d == 2 ? 1 : 2;   // transverse directions in index order
+      for (int g = 0; g < MDS_NG; g++) {
+        const int c0 = side ? pb.n[d] + g : g - MDS_NG;
+        for (long long t = 0; t < T; t++) {
+          const int a = (int)(t % pb.n[lo1]), b = (int)(t / pb.n[lo1]);
+          int idx[3];
+          idx[d] = c0;
+          idx[lo1] = a;
+          idx[lo2] = b;
+          if (!c->hdj.empty()) {
+            // dendritic outer radius: the positions of the ghost shell's own level
+            const int S1 = pb.n[1] + 2 * MDS_NG;
+            const int dj = c->hdj[c0 + MDS_NG];
+            if (idx[1] >= pb.n[1] / dj) continue;
+            const int dk = c->hdk[(size_t)(c0 + MDS_NG) * S1 + idx[1] + MDS_NG];
+            if (idx[2] >= pb.n[2] / dk) continue;
+          }
+          double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          fn(pb.ndim > 0 ? idx[0] : 0, pb.ndim > 1 ? idx[1] : 0, pb.ndim > 2 ? idx[2] : 0, q, user);
+          for (int v = 0; v < 6; v++) {
+            if (v >= 2 + c->nv && v < 5) continue;   // velocity components the state does not carry
+            if (v == 5 && !c->tab) continue;
+            h[((size_t)g * MDS_SLAB_NV + v) * T + t] = q[v];
+          }
+        }
+      }
+      HIPCHK(c, hipMemcpy(pb.gs[d][side], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  c->prob_pending[prob] = 0;
   return MDS_OK;
 }
 
@@ -747,6 +836,13 @@ extern "C" int mds_update_eos(mds_ctx *c)
 // one RK stage of every problem: the fused sweep kernel
 static int enqueue_stage(mds_ctx *c, int stage, int fp32)
 {
+  if (c->has_slab) {
+    for (int p = 0; p < c->nprob; p++)
+      if (c->prob_pending[p]) return set_err(c, MDS_EINVAL, "problem %d: PROB boundary values not set", p);
+    // ghost layers of the PROB/EXTRAP sides from the stage's input state
+    void *const *in = stage == 0 ? c->p : c->ph;
+    HIPCHK(c, mds_launch_slab_fill(fp32, c->dprob, c->nprob, c->slab_items, in, &c->deos, c->tab, c->nv, c->st));
+  }
   if (c->ndim == 1) {
     Sweep1DArgs a;
     a.seg = c->dseg;

@@ -56,7 +56,28 @@ struct DevProb {
   // beta, Gamma per zone) and FP32 coefficients at l * (n + 2NG) * MDS_RC_COUNT
   const double *lmom[2];
   const float *lrc[2];
+  // Ghost layers of the PROB and EXTRAP sides (null on other sides): FP64,
+  // gs[d][side][(g * MDS_SLAB_NV + v) * T + t] for ghost layer g (index
+  // g - NG below the grid, n[d] + g above), entry v (rho, u, v1, v2, v3, Ye,
+  // P, Gamma_1) and transverse position t (slab_t); T = slab_T(d).  PROB
+  // layers hold the caller's values, EXTRAP layers are refilled from the
+  // state before every stage; P and Gamma_1 are refilled for both.
+  int has_slab;
+  double *gs[3][2];
 };
+
+#define MDS_SLAB_NV 8
+
+// transverse positions of a direction-d ghost layer, and the index of the
+// zone (i, j, k) among them (box positions, natural order)
+__host__ __device__ __forceinline__ long long slab_T(const DevProb &pb, int d)
+{
+  return d == 0 ? (long long)pb.n[1] * pb.n[2] : (d == 1 ? (long long)pb.n[0] * pb.n[2] : (long long)pb.n[0] * pb.n[1]);
+}
+__host__ __device__ __forceinline__ long long slab_t(const DevProb &pb, int d, int i, int j, int k)
+{
+  return d == 0 ? j + (long long)pb.n[1] * k : (d == 1 ? i + (long long)pb.n[0] * k : i + (long long)pb.n[0] * j);
+}
 
 // level-local counts of a dendritic grid (uniform grids: level 0 everywhere)
 __device__ __forceinline__ int dend_lj(const DevProb &pb, int i) { return pb.lj ? pb.lj[i + MDS_NG] : 0; }

@@ -128,6 +128,9 @@ MDS_DEV void map_zone_n(const DevProb &pb, int D, int &i, int &j, int &k, double
     *c = side ? 2 * n - 1 - ig : -1 - ig;
     k = (k + nkm / 2) % nkm;
     sgn[1] = sgn[2] = -1.0;
+  } else if (code == MDS_BC_PROB || code == MDS_BC_EXTRAP) {
+    // outer radius with ghost layers of its own: the index stays in the
+    // ghost shell and the caller reads the layer (slab_state)
   } else if (code == MDS_BC_SPHERICAL_ORIGIN && D == 0 && !side) {
     i = -1 - ig;
     j = nj_of(pb, i) - 1 - j;
@@ -186,13 +189,33 @@ MDS_DEV void zone_state(const Sph3Args &a, const StateArrays<R> &in, long long z
   }
 }
 
+// the pencil values of a zone of the outer ghost shells of a PROB or EXTRAP
+// radius (i >= n[0]): its ghost layer, at the state's precision
+template <class R>
+MDS_DEV void slab_state(const DevProb &pb, int i, int j, int k, int tab, double q[8])
+{
+  const long long T = slab_T(pb, 0);
+  const double *e = pb.gs[0][1] + (size_t)(i - pb.n[0]) * MDS_SLAB_NV * T + slab_t(pb, 0, i, j, k);
+  for (int v = 0; v < 5; v++) q[v] = (double)(R)e[v * T];
+  if (tab) {
+    q[5] = (double)(R)e[5 * T];
+    q[6] = (double)(R)e[6 * T];
+    q[7] = (double)(R)e[7 * T];
+  } else {
+    q[5] = 0.0;
+    q[6] = (pb.gam - 1.0) * q[1];
+    q[7] = pb.gam;
+  }
+}
+
 // the pencil values of any (possibly ghost or corner) zone
 template <class R>
 MDS_DEV void any_state(const Sph3Args &a, const StateArrays<R> &in, int i, int j, int k, int tab, double q[8])
 {
   double sgn[3];
   map_any(a.pb, i, j, k, sgn);
-  zone_state<R>(a, in, zid3(a.pb, i, j, k), tab, a.pb.gam, q);
+  if (i >= a.pb.n[0]) slab_state<R>(a.pb, i, j, k, tab, q);
+  else zone_state<R>(a, in, zid3(a.pb, i, j, k), tab, a.pb.gam, q);
   for (int d = 0; d < 3; d++) q[2 + d] *= sgn[d];
 }
 
@@ -868,7 +891,8 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
       }
       double sgn[3];
       map_zone_n(pb, D, i, j, k, sgn, n, nkm);
-      zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
+      if (D == 0 && i >= pb.n[0]) slab_state<R>(pb, i, j, k, TAB, q3[s]);
+      else zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
       for (int d = 0; d < 3; d++) q3[s][2 + d] *= sgn[d];
     }
   } else {

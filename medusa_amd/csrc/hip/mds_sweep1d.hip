@@ -41,12 +41,14 @@ namespace {
 // fill would copy there (bc.c:727-1030 for direction 0).  flip gets the
 // velocity components whose sign the fill changes (bit d: v^(d+1)): REFLECT
 // v^1, the inner radius through the origin (SPHERICAL_ORIGIN, bc.c:774-801)
-// v^1 and v^3.
+// v^1 and v^3.  A PROB or EXTRAP side has its own ghost layers: flip gets bit
+// 8 (and bit 16 above the grid) and the layer index is returned.
 MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
 {
   flip = 0;
   if (c < 0) {
     if (c < -MDS_NG) return 0;
+    if (lo == MDS_BC_PROB || lo == MDS_BC_EXTRAP) { flip = 8; return c + MDS_NG; }
     if (lo == MDS_BC_REFLECT) { flip = 1; return -1 - c; }
     if (lo == MDS_BC_SPHERICAL_ORIGIN) { flip = 5; return -1 - c; }
     if (lo == MDS_BC_PERIODIC) return c + n;
@@ -54,6 +56,7 @@ MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
   }
   if (c >= n) {
     if (c >= n + MDS_NG) return n - 1;
+    if (hi == MDS_BC_PROB || hi == MDS_BC_EXTRAP) { flip = 8 | 16; return c - n; }
     if (hi == MDS_BC_REFLECT) { flip = 1; return 2 * n - 1 - c; }
     if (hi == MDS_BC_PERIODIC) return c - n;
     return n - 1;
@@ -137,6 +140,22 @@ MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, int f
   if (flip & 1) q.v[0] = -q.v[0];
   if (NV > 2 && (flip & 4)) q.v[2] = -q.v[2];
   return q;
+}
+
+// the zone bc_src names: a state zone, or a ghost layer of a PROB/EXTRAP side
+template <int NV, int TAB, class R>
+MDS_DEV Prim<R> load_zone(const DevProb &pb, const StateArrays<R> &in, int s, int flip)
+{
+  if (flip & 8) {
+    const double *e = pb.gs[0][(flip >> 4) & 1] + (size_t)s * MDS_SLAB_NV;   // one transverse position
+    Prim<R> q;
+    q.rho = (R)e[0];
+    q.uu = (R)e[1];
+    for (int d = 0; d < 3; d++) q.v[d] = d < NV ? (R)e[2 + d] : R(0);
+    q.ye = TAB ? (R)e[5] : R(0);
+    return q;
+  }
+  return load_prims<NV, TAB>(in, pb.off, s, flip);
 }
 
 template <int NV, int TAB, class R>
@@ -376,7 +395,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     const StateArrays<double> &in = (stage == 0) ? sp : sph;
     int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
+    const Prim<double> q = load_zone<NV, TAB>(pb, in, s, flip);
     Geo64 g;
     load_geo64<SPH>(pb, c, n, g);
 
@@ -429,7 +448,7 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
     const double dt = ck->dt;
     int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<double> q = load_prims<NV, TAB>(in, off, s, flip);
+    const Prim<double> q = load_zone<NV, TAB>(pb, in, s, flip);
     Geo64 g;
     load_geo64<SPH>(pb, c, n, g);
     double U0[6];
@@ -655,7 +674,7 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     const StateArrays<float> &in = (stage == 0) ? sp : sph;
     int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
+    const Prim<float> q = load_zone<NV, TAB>(pb, in, s, flip);
     Geo32 g;
     load_geo32<SPH>(pb, c, n, g);
     Roles r;
@@ -705,7 +724,7 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
     const double dt = ck->dt;
     int flip;
     const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<float> q = load_prims<NV, TAB>(in, off, s, flip);
+    const Prim<float> q = load_zone<NV, TAB>(pb, in, s, flip);
     Geo32 g;
     load_geo32<SPH>(pb, c, n, g);
     double U0[6];

@@ -71,9 +71,43 @@ MDS_DEV void load_rgeo(const DevProb &pb, int d, int z, typename ReconGeoT<R>::t
   }
 }
 
+// A stencil zone in a ghost layer of a PROB or EXTRAP side: its entries
+// (entry v at e[v * T]).  The first such direction decides; the transverse
+// position is clamped into the grid, so corner zones (which no output reads)
+// stay in bounds.
+MDS_DEV bool slab_entry(const DevProb &pb, int i, int j, int k, const double *&e, long long &T)
+{
+  const int c[3] = {i, j, k};
+  for (int d = 0; d < pb.ndim; d++) {
+    const int n = pb.n[d];
+    int side, g;
+    if (c[d] < 0 && c[d] >= -MDS_NG) { side = 0; g = c[d] + MDS_NG; }
+    else if (c[d] >= n && c[d] < n + MDS_NG) { side = 1; g = c[d] - n; }
+    else continue;
+    const double *sl = pb.gs[d][side];
+    if (!sl) continue;
+    const int q0 = min(max(i, 0), pb.n[0] - 1), q1 = min(max(j, 0), pb.n[1] - 1), q2 = min(max(k, 0), pb.n[2] - 1);
+    T = slab_T(pb, d);
+    e = sl + (size_t)g * MDS_SLAB_NV * T + slab_t(pb, d, q0, q1, q2);
+    return true;
+  }
+  return false;
+}
+
 template <class R, int NV>
 MDS_DEV void load_cell(const StateArrays<R> &in, const DevProb &pb, int i, int j, int k, R gm1, Cell<R, NV> &c)
 {
+  if (pb.has_slab) {
+    const double *e;
+    long long T;
+    if (slab_entry(pb, i, j, k, e, T)) {
+      c.rho = (R)e[0];
+      c.uu = (R)e[T];
+      for (int d = 0; d < NV; d++) c.v[d] = (R)e[(2 + d) * T];
+      c.P = gm1 * c.uu;
+      return;
+    }
+  }
   bool fx, fy, fz = false;
   int si = bc_src2(i, pb.n[0], pb.bc_lo[0], pb.bc_hi[0], fx);
   int sj = bc_src2(j, pb.n[1], pb.bc_lo[1], pb.bc_hi[1], fy);
@@ -93,6 +127,11 @@ MDS_DEV void load_cell(const StateArrays<R> &in, const DevProb &pb, int i, int j
 template <class R>
 MDS_DEV R load_press(const StateArrays<R> &in, const DevProb &pb, int i, int j, int k, R gm1)
 {
+  if (pb.has_slab) {
+    const double *e;
+    long long T;
+    if (slab_entry(pb, i, j, k, e, T)) return gm1 * (R)e[T];
+  }
   bool f;
   int si = bc_src2(i, pb.n[0], pb.bc_lo[0], pb.bc_hi[0], f);
   int sj = bc_src2(j, pb.n[1], pb.bc_lo[1], pb.bc_hi[1], f);

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmds.so")
 
 NG = 2
 FP64, FP32_BULK = 0, 1
-BC_OUTFLOW, BC_REFLECT, BC_PERIODIC = 0, 1, 3
+BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC, BC_PROB = 0, 1, 2, 3, 4
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
 RC_COUNT = 8
@@ -97,7 +97,7 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
-            "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune",
+            "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
@@ -142,6 +142,7 @@ def lib():
     L.mds_set_tile.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.mds_get_tile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mds_autotune.argtypes = [vp, ctypes.c_int, _dp]
+    L.mds_set_prob_bounds.argtypes = [vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.mds_recon_points.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mds_riemann_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_int)]
@@ -281,6 +282,11 @@ def run_params(cfl=0.5, tmax=1.0e300, **kw):
     return rp
 
 
+# mds_prob_bounds_fn: (i, j, k, p, user)
+PROB_BOUNDS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.c_void_p)
+
+
 class Context:
     """A device context holding one or more problems (mds_ctx)."""
 
@@ -320,6 +326,17 @@ class Context:
     def _check(self, rc, what):
         if rc:
             raise RuntimeError(f"{what} failed ({rc}): {lib().mds_last_error(self.h).decode()}")
+
+    def set_prob_bounds(self, fn, prob=0):
+        """PROB ghost values: fn(i, j, k) -> (rho, u, v1, v2, v3[, Ye]) for every
+        ghost zone of the problem's PROB sides (the reference's prob_bounds)."""
+        def _cb(i, j, k, p, user):
+            vals = fn(i, j, k)
+            for v in range(min(len(vals), 6)):
+                p[v] = float(vals[v])
+        cb = PROB_BOUNDS_FN(_cb)
+        self._check(lib().mds_set_prob_bounds(self.h, prob, ctypes.cast(cb, ctypes.c_void_p), None),
+                    "set_prob_bounds")
 
     def _shape(self, prob):
         g = self.grids[prob]

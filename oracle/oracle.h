@@ -41,6 +41,7 @@ extern "C" {
 #define ORC_BC_REFLECT  1
 #define ORC_BC_EXTRAP   2
 #define ORC_BC_PERIODIC 3
+#define ORC_BC_PROB     4   /* ghost values from the problem's prob_bounds (bc.c:125-140) */
 #define ORC_BC_SPHERICAL_ORIGIN 5   /* theta: across the pole (phi + pi); inner radius: through the
                                        origin (theta -> pi - theta, phi + pi); decs.h:69 */
 
@@ -82,6 +83,11 @@ typedef struct {
   const struct orc_eos_tab *eos_tab;   /* tabulated EOS (NULL: gamma law); nvars 6 carries Ye */
   double rtrans;       /* ORC_RCOORD_SINH scale */
   int dendritic;       /* 3D spherical: dendritic angular coarsening (utils.c:181-590) */
+  /* PROB boundaries: the problem's prob_bounds(i, j, k, p) (e.g.
+   * problems/init_rt_3d.c:223-236), called for every ghost zone of a PROB side
+   * with p holding nvars entries; time-independent here */
+  void (*prob_bounds)(int i, int j, int k, double *p, void *user);
+  void *prob_user;
 } orc_config;
 
 typedef struct orc_sim orc_sim;

@@ -539,6 +539,8 @@ static void build_geometry_spherical1d(orc_sim *s)
 
 orc_sim *orc_create(const orc_config *cfg)
 {
+  for (int d = 0; d < cfg->ndim; d++)
+    if ((cfg->bc_lo[d] == ORC_BC_PROB || cfg->bc_hi[d] == ORC_BC_PROB) && !cfg->prob_bounds) return NULL;
   orc_sim *s = calloc(1, sizeof(*s));
   s->c = *cfg;
   for (int d = 0; d < 3; d++) {
@@ -664,7 +666,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
       for (int gidx = 0; gidx < ORC_NG; gidx++) {
         int ig = side ? nd + gidx : -ORC_NG + gidx;   /* ghost index along d */
         int src = 0, flip = 0, extrap = 0, pole = 0;
-        int origin = 0;
+        int origin = 0, prob = 0;
         switch (code) {
           case ORC_BC_SPHERICAL_ORIGIN:
             /* theta boundary across the pole: mirror in j, phi + pi, flip v2 and v3
@@ -689,6 +691,9 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           case ORC_BC_EXTRAP:
             extrap = 1;
             break;
+          case ORC_BC_PROB:
+            prob = 1;   /* bc.c:125-140: prob_bounds(i, j, k, p) per ghost zone */
+            break;
           default:
             src = side ? nd - 1 : 0;
         }
@@ -701,7 +706,9 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           sidx[2] = (idx[2] + c->n[2] / 2) % c->n[2];
         }
         double *dst = &P[zid(s, idx[0], idx[1], idx[2]) * nv];
-        if (extrap) {
+        if (prob) {
+          c->prob_bounds(idx[0], idx[1], idx[2], dst, c->prob_user);
+        } else if (extrap) {
           int i0, i1;
           double w;
           if (side) { i0 = nd - 1; i1 = nd - 2; w = (double)(ig - nd + 1); }
@@ -861,6 +868,20 @@ static void fill_ghost_dend(orc_sim *s, double *P)
         for (int k = 0; k < nks(s, i, j); k++) {
           int ip, jp = j, kp = k;
           const int origin = code == ORC_BC_SPHERICAL_ORIGIN && !side;
+          double *dst = &P[zid(s, i, j, k) * nv];
+          if (code == ORC_BC_PROB) {
+            c->prob_bounds(i, j, k, dst, c->prob_user);
+            continue;
+          }
+          if (code == ORC_BC_EXTRAP) {
+            /* bc.c EXTRAP: the edge zone plus (distance) x (edge - next) at
+             * the ghost's own (j, k), as the reference indexes it */
+            const int i0 = side ? n1 - 1 : 0, i1 = side ? n1 - 2 : 1;
+            const double w = side ? (double)(i - n1 + 1) : (double)i;
+            const double *q0 = &P[zid(s, i0, j, k) * nv], *q1 = &P[zid(s, i1, j, k) * nv];
+            for (int v = 0; v < nv; v++) dst[v] = side ? q0[v] + w * (q0[v] - q1[v]) : q0[v] + w * (q1[v] - q0[v]);
+            continue;
+          }
           if (code == ORC_BC_REFLECT) {
             ip = side ? 2 * n1 - 1 - i : -1 - i;
             jp = j * djs(s, i) / djs(s, ip);
@@ -873,7 +894,6 @@ static void fill_ghost_dend(orc_sim *s, double *P)
           } else {
             ip = side ? n1 - 1 : 0;
           }
-          double *dst = &P[zid(s, i, j, k) * nv];
           const double *src = &P[zid(s, ip, jp, kp) * nv];
           for (int v = 0; v < nv; v++) dst[v] = src[v];
           if (code == ORC_BC_REFLECT) dst[ORC_U1] *= -1.0;

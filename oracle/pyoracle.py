@@ -16,6 +16,7 @@ RHO, UU, U1, U2, U3, YE = 0, 1, 2, 3, 4, 5
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC = 0, 1, 2, 3
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
+BC_PROB = 4
 BC_SPHERICAL_ORIGIN = 5
 
 
@@ -51,7 +52,14 @@ class Config(ctypes.Structure):
                 ("use_monopole", ctypes.c_int),
                 ("eos_tab", ctypes.c_void_p),
                 ("rtrans", ctypes.c_double),
-                ("dendritic", ctypes.c_int)]
+                ("dendritic", ctypes.c_int),
+                ("prob_bounds", ctypes.c_void_p),
+                ("prob_user", ctypes.c_void_p)]
+
+
+# prob_bounds(i, j, k, p, user): the ghost values of a PROB boundary zone
+PROB_BOUNDS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 def _load(fast=False):
@@ -168,9 +176,10 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
                 geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, interp_order=3,
                 g0=(0.0, 0.0, 0.0), dt_init_safety=0.1, dt_grow_safety=1.1,
                 dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100, monopole=False, eos_tab=None,
-                rtrans=0.0, dendritic=False):
+                rtrans=0.0, dendritic=False, prob_bounds=None):
     """eos_tab: an EosTab (tabulated EOS; nvars must be 6 to carry Ye).
-    rtrans: scale of the sinh radial map; dendritic: 3D spherical dendritic grid."""
+    rtrans: scale of the sinh radial map; dendritic: 3D spherical dendritic grid.
+    prob_bounds: f(i, j, k) -> sequence of nvars ghost values for PROB sides."""
     c = Config()
     c.ndim = ndim
     n = list(n) + [1] * (3 - len(n))
@@ -199,6 +208,14 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
     if eos_tab is not None:
         c.eos_tab = ctypes.cast(ctypes.pointer(eos_tab), ctypes.c_void_p)
         c._eos_keep = eos_tab
+    if prob_bounds is not None:
+        def _cb(i, j, k, p, user, f=prob_bounds, nv=nvars):
+            vals = f(i, j, k)
+            for v in range(nv):
+                p[v] = float(vals[v])
+        cb = PROB_BOUNDS_FN(_cb)
+        c.prob_bounds = ctypes.cast(cb, ctypes.c_void_p)
+        c._prob_keep = cb
     return c
 
 

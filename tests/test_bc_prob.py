@@ -1,0 +1,272 @@
+"""PROB and EXTRAP boundaries: ghost values from the problem's prob_bounds
+(bc.c:125-140 and the like; e.g. the hydrostatic z ghosts of
+problems/init_rt_3d.c:223-236) and linear extrapolation in direction 0
+(bc.c:806-821, 937-954).
+
+Pins of the oracle: EXTRAP reproduces a linear profile exactly at the ghost
+centres (dyadic data, so exact in binary), on both sides and on the outer
+radius of a dendritic grid; PROB calls prob_bounds exactly for the ghost
+zones the reference loops over (ghost layers along the side's direction,
+interior positions along the others, the ghost shell's own level on a
+dendritic grid) and stores what it returns; a uniform flow whose PROB ghosts
+hold the same state stays uniform bit for bit (a wrong or missing layer
+breaks it).  The device path is held against the oracle in FP64 (1e-12
+relative L1, bit-identical in practice) and FP32-bulk on 1D columns, 2D and
+3D Cartesian grids and the outer radius of 3D spherical grids."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+NG = 2
+
+
+def _cfg(ndim, n, lo, hi, gam=1.4, **kw):
+    return po.make_config(ndim, n, [0.0] * ndim, [1.0 / n[0]] * ndim, lo, hi, gam, nvars=5, **kw)
+
+
+def test_extrap_is_exact_on_linear_profiles():
+    n = 16
+    sim = po.Sim(_cfg(1, [n], [po.BC_EXTRAP], [po.BC_EXTRAP]))
+    prims = np.zeros((5, 1, 1, n))
+    for v, (a, b) in enumerate([(3.0, 0.25), (5.0, -0.125), (0.5, 0.0625), (0.0, 0.0), (0.0, 0.0)]):
+        prims[v, 0, 0] = a + b * np.arange(n)
+    sim.set_prims(prims)
+    sim.fill_ghosts()
+    for c in (-2, -1, n, n + 1):
+        want = prims[:, 0, 0, 0] + c * (prims[:, 0, 0, 1] - prims[:, 0, 0, 0])
+        assert np.array_equal(sim.zone(c), want), c
+
+
+def test_prob_calls_cover_the_reference_ghost_zones():
+    n = (6, 5)
+    calls = []
+
+    def pb(i, j, k):
+        calls.append((i, j, k))
+        return (1.0 + i, 2.0 + 0.5 * j, 0.25 * i, -0.5 * j, 0.0)
+
+    cfg = _cfg(2, list(n), [po.BC_PERIODIC, po.BC_REFLECT], [po.BC_PERIODIC, po.BC_PROB], prob_bounds=pb)
+    sim = po.Sim(cfg)
+    sim.set_prims(np.ones((5, 1, n[1], n[0])))
+    sim.fill_ghosts()
+    want = {(i, n[1] + g, 0) for i in range(n[0]) for g in range(NG)}
+    assert set(calls) == want and len(calls) == len(want)
+    for (i, j, k) in want:
+        assert np.array_equal(sim.zone(i, j, k), np.array(pb(i, j, k)))
+
+
+def test_prob_without_bounds_is_rejected():
+    with pytest.raises(ValueError):
+        po.Sim(_cfg(1, [8], [po.BC_PROB], [po.BC_OUTFLOW]))
+
+
+def test_uniform_flow_through_prob_sides_stays_uniform():
+    n = 32
+    state = (1.0, 2.5 / 0.4, 0.3, 0.0, 0.0)
+    sim = po.Sim(_cfg(1, [n], [po.BC_PROB], [po.BC_PROB], prob_bounds=lambda i, j, k: state))
+    prims = np.zeros((5, 1, 1, n))
+    prims[:, 0, 0, :] = np.array(state)[:, None]
+    sim.set_prims(prims)
+    sim.run(max_steps=20)
+    assert np.array_equal(sim.get_prims(), prims)
+    # the same flow with zeros in the PROB ghosts does not
+    sim = po.Sim(_cfg(1, [n], [po.BC_PROB], [po.BC_PROB], prob_bounds=lambda i, j, k: (1e-3, 1e-3, 0, 0, 0)))
+    sim.set_prims(prims)
+    sim.run(max_steps=5)
+    assert not np.array_equal(sim.get_prims(), prims)
+
+
+def _dend(n, hi, **kw):
+    from medusa_amd import problems
+    from parity_util import oracle_config
+    g = problems.dend3d_grid(*n, dr0=1.0, rout=400.0)
+    prob = dict(grid=g, bc_lo=[po.BC_SPHERICAL_ORIGIN, po.BC_SPHERICAL_ORIGIN, po.BC_PERIODIC],
+                bc_hi=[hi, po.BC_SPHERICAL_ORIGIN, po.BC_PERIODIC], gam=5.0 / 3.0)
+    return g, prob, po.Sim(oracle_config(prob, 0.5, nvars=5, **kw))
+
+
+def test_dendritic_outer_radius_extrap_and_prob():
+    n = (24, 16, 32)
+    g, prob, sim = _dend(n, po.BC_EXTRAP)
+    NJ, NK, _, _ = sim.levels()
+    assert NJ[-1] == NJ[-2] and np.array_equal(NK[-1], NK[-2])
+    prims = np.zeros((5, n[2], n[1], n[0]))
+    ii = np.arange(n[0])[None, None, :]
+    kk = np.arange(n[2])[:, None, None]
+    prims[0] = 2.0 + 0.125 * ii + 0.0625 * kk
+    prims[1] = 1.0 + 0.25 * ii
+    prims *= g.valid_mask()[None]
+    sim.set_prims(prims)
+    sim.fill_ghosts()
+    n1 = n[0]
+    for gg in range(NG):
+        for j in range(NJ[-1]):
+            for k in range(NK[-1, j]):
+                z = sim.zone(n1 + gg, j, k)
+                assert z[0] == 2.0 + 0.125 * (n1 + gg) + 0.0625 * k and z[1] == 1.0 + 0.25 * (n1 + gg)
+    calls = []
+
+    def pb(i, j, k):
+        calls.append((i, j, k))
+        return (1.0, 1.5, 0.0, 0.0, 0.0)
+
+    g, prob, sim = _dend(n, po.BC_PROB, prob_bounds=pb)
+    sim.set_prims(np.ones((5, n[2], n[1], n[0])) * g.valid_mask()[None])
+    sim.fill_ghosts()
+    want = {(n1 + gg, j, k) for gg in range(NG) for j in range(NJ[-1]) for k in range(NK[-1, j])}
+    assert set(calls) == want and len(calls) == len(want)
+
+
+# ---------------------------------------------------------------------------
+# device path against the oracle
+# ---------------------------------------------------------------------------
+def _rel(a, b, w=None):
+    w = 1.0 if w is None else w
+    return float(np.sum(np.abs(a - b) * w) / np.sum(np.abs(b) * w))
+
+
+def _pair(prob, prims, precision, steps, bounds=None, nvars=5):
+    from medusa_amd import mds
+    from parity_util import oracle_config
+    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    if bounds is not None:
+        ctx.set_prob_bounds(bounds)
+    ctx.update_eos()
+    ctx.run(steps)
+    dev = ctx.download()
+    sim = po.Sim(oracle_config(prob, 0.5, nvars=nvars, prob_bounds=bounds))
+    sim.set_prims(prims)
+    sim.run(max_steps=steps)
+    return ctx, sim, dev, sim.get_prims()
+
+
+def _check(dev, ref, precision, rows, w=None, tag=""):
+    errs = {r: _rel(dev[r], ref[r], w) for r in rows}
+    zmax = {r: float(np.max(np.abs(dev[r] - ref[r])) / max(np.max(np.abs(ref[r])), 1e-300)) for r in rows}
+    print(f"[bc] {tag} L1 {errs}; max zone {zmax}", flush=True)
+    for r in rows:
+        assert errs[r] <= (1e-12 if precision == 0 else 1e-5), (r, errs[r])
+        assert zmax[r] <= (1e-11 if precision == 0 else 1e-3), (r, zmax[r])
+
+
+@pytest.mark.gpu
+def test_unset_prob_sides_are_refused():
+    from medusa_amd import mds
+    g = mds.GridBuilder(1, [64], [0.0], [1.0 / 64])
+    ctx = mds.Context([dict(grid=g, bc_lo=[mds.BC_PROB], bc_hi=[mds.BC_OUTFLOW], gam=1.4)], precision=0)
+    prims = np.zeros((5, 1, 1, 64))
+    prims[0] = prims[1] = 1.0
+    ctx.upload(prims)
+    ctx.update_eos()
+    with pytest.raises(RuntimeError, match="PROB"):
+        ctx.run(1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_1d_extrap_and_prob_match_oracle(precision):
+    """Sod-like tube: EXTRAP below, an inflow state held by PROB above"""
+    from medusa_amd import mds
+    n = 256
+    g = mds.GridBuilder(1, [n], [0.0], [1.0 / n])
+    prob = dict(grid=g, bc_lo=[mds.BC_EXTRAP], bc_hi=[mds.BC_PROB], gam=1.4)
+    x = (np.arange(n) + 0.5) / n
+    prims = np.zeros((5, 1, 1, n))
+    prims[0, 0, 0] = np.where(x < 0.5, 1.0, 0.125) + 0.05 * x
+    prims[1, 0, 0] = np.where(x < 0.5, 2.5, 0.25)
+    prims[2, 0, 0] = 0.1 * x
+    inflow = (0.2, 0.5, -0.4, 0.0, 0.0)
+    print(f"[bc] 1D EXTRAP/PROB n={n} precision={precision}, 80 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 80, bounds=lambda i, j, k: inflow)
+    _check(dev, ref, precision, (0, 1, 2), tag="1D")
+    t, dt, ist = ctx.clock()
+    if precision == 0:
+        assert ist == sim.istep and t == sim.t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_1d_spherical_column_prob_outer_radius(precision):
+    """a blast in a spherical column whose outer ghosts hold the ambient state"""
+    from medusa_amd import mds
+    n = 200
+    g = mds.GridBuilder(1, [n], [0.0], [1.0 / n], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_UNIFORM)
+    prob = dict(grid=g, bc_lo=[mds.BC_SPHERICAL_ORIGIN], bc_hi=[mds.BC_PROB], gam=5.0 / 3.0)
+    prims = np.zeros((5, 1, 1, n))
+    prims[0] = 1.0
+    prims[1, 0, 0, :6] = 50.0
+    prims[1, 0, 0, 6:] = 0.1
+    print(f"[bc] 1D spherical PROB n={n} precision={precision}, 60 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 60, bounds=lambda i, j, k: (1.0, 0.1, 0.0, 0.0, 0.0))
+    _check(dev, ref, precision, (0, 1, 2), w=g.zone()[2:-2, 0], tag="1D sph")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_2d_prob_sides_match_oracle(precision):
+    """KH with the y sides held at the initial outer rows by PROB"""
+    from medusa_amd import mds, problems
+    g, prob, prims = problems.kh(64)
+    prob = dict(prob, bc_lo=[mds.BC_PERIODIC, mds.BC_PROB], bc_hi=[mds.BC_PERIODIC, mds.BC_PROB])
+    n = 64
+
+    def pb(i, j, k):
+        row = 0 if j < 0 else n - 1
+        return tuple(prims[v, 0, row, i] for v in range(5))
+
+    print(f"[bc] 2D KH 64^2 PROB in y precision={precision}, 20 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 20, bounds=pb)
+    _check(dev, ref, precision, (0, 1, 2, 3), tag="2D")
+
+
+def _rt_bounds(g, gam=1.4, p0=2.5, gz=-0.1):
+    """problems/init_rt_3d.c:223-236: the hydrostatic profile at rest"""
+    def pb(i, j, k):
+        z = g.g.startx[2] + (k + 0.5) * g.g.dx[2]
+        rho = 1.0 if z < 0 else 2.0
+        return (rho, (p0 + gz * rho * z) / (gam - 1.0), 0.0, 0.0, 0.0)
+    return pb
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_rt3d_prob_z_matches_oracle(precision):
+    """RT 16^3 with the reference's own z boundary: PROB holding the hydrostatic profile"""
+    from medusa_amd import mds, problems
+    g, prob, prims = problems.rt(16)
+    prob = dict(prob, bc_lo=[mds.BC_PERIODIC, mds.BC_PERIODIC, mds.BC_PROB],
+                bc_hi=[mds.BC_PERIODIC, mds.BC_PERIODIC, mds.BC_PROB])
+    print(f"[bc] RT 16^3 PROB in z precision={precision}, 10 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 10, bounds=_rt_bounds(g))
+    _check(dev, ref, precision, (0, 1, 4), tag="RT3D")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dendritic", [False, True])
+@pytest.mark.parametrize("code", ["prob", "extrap"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sph3d_outer_radius_matches_oracle(dendritic, code, precision):
+    from medusa_amd import mds, problems
+    n = (24, 16, 32)
+    g = problems.dend3d_grid(*n, dr0=1.0, rout=400.0, dendritic=dendritic)
+    hi = mds.BC_PROB if code == "prob" else mds.BC_EXTRAP
+    prob = dict(grid=g, bc_lo=[mds.BC_SPHERICAL_ORIGIN, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC],
+                bc_hi=[hi, mds.BC_SPHERICAL_ORIGIN, mds.BC_PERIODIC], gam=5.0 / 3.0)
+    rng = np.random.default_rng(5)
+    prims = np.zeros((5, n[2], n[1], n[0]))
+    prims[0] = 1.0 + 0.05 * rng.uniform(-1, 1, prims[0].shape)
+    prims[1] = 1.5
+    prims[1][:, :, n[0] - 6:n[0] - 3] *= 6.0          # a shell about to hit the outer radius
+    prims[2] = 0.05 * rng.uniform(-1, 1, prims[0].shape)
+    if dendritic:
+        prims *= g.valid_mask()[None]
+    bounds = (lambda i, j, k: (0.8, 1.2, 0.05, 0.0, 0.0)) if code == "prob" else None
+    print(f"[bc] sph3d {n} dendritic={dendritic} outer {code} precision={precision}, 10 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 10, bounds=bounds)
+    vol = g.zone()[2:-2, 2:-2, mds.ZS_VOLUME]
+    w = np.broadcast_to(vol[None], ref[0].shape) * (g.valid_mask() if dendritic else 1.0)
+    _check(dev, ref, precision, (0, 1), w=w, tag="sph3d")
