@@ -199,6 +199,8 @@ typedef struct mds_run_params {
   double dtmax;            /* 1e200 */
   double initial_dt;       /* 1e100 */
   double tmax;             /* problems freeze once t >= tmax */
+  int hydro_interp_order;  /* 3: para_recon (the default, io.c:5054); 2: lin_recon; 1: piecewise
+                              constant (reconstruct.c:246-266).  FP32-bulk runs order 3 only */
 } mds_run_params;
 
 typedef struct mds_ctx mds_ctx;
@@ -252,7 +254,9 @@ int mds_sync(mds_ctx *ctx);
  * reference, whose OpenMP runtime picks launch shapes).  Chosen by
  * mds_autotune() or set explicitly; every value gives identical results. */
 #define MDS_TILE_ROWS2D  0       /* rows marched by one wavefront in the 2D sweep */
-#define MDS_TILE_TY3D    1       /* output rows per workgroup in the 3D sweep (4, 8 or 12) */
+#define MDS_TILE_TY3D    1       /* output rows per workgroup in the 3D sweep (4, 8 or 12; 4 and
+                                    halo mode 0 when a problem has PROB/EXTRAP sides or
+                                    hydro_interp_order < 3) */
 #define MDS_TILE_KC3D    2       /* planes marched by one workgroup in the 3D sweep */
 #define MDS_TILE_FUSE1D  3       /* 1D: 1 = both RK stages of a step in one launch (default
                                     when no problem is periodic in x), 0 = one launch per stage */

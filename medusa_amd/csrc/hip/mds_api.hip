@@ -283,6 +283,7 @@ extern "C" void mds_default_run_params(mds_run_params *rp)
   rp->dtmax = 1.0e200;
   rp->initial_dt = 1.0e100;
   rp->tmax = 0.0;
+  rp->hydro_interp_order = 3;
 }
 
 extern "C" const char *mds_last_error(const mds_ctx *ctx) { return ctx ? ctx->err : "null context"; }
@@ -372,6 +373,10 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   c->rp.dtmin = rp->dtmin;
   c->rp.dtmax = rp->dtmax;
   c->rp.initial_dt = rp->initial_dt;
+  if (rp->hydro_interp_order < 1 || rp->hydro_interp_order > 3)
+    return set_err(c, MDS_EINVAL, "hydro_interp_order must be 1, 2 or 3");
+  if (rp->hydro_interp_order != 3 && c->precision == MDS_FP32_BULK)
+    return set_err(c, MDS_ENOTIMPL, "FP32-bulk reconstructs with para_recon (hydro_interp_order 3) only");
   c->tmax = rp->tmax;
   c->ndim = probs[0].grid->ndim;
   c->sph = probs[0].grid->geom == MDS_GEOM_SPHERICAL;
@@ -487,6 +492,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       d.area[k] = g->area[k];
     }
     d.volume = g->volume;
+    d.order = rp->hydro_interp_order;
     // ghost layers of the PROB and EXTRAP sides
     d.has_slab = 0;
     {
@@ -679,7 +685,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     HIPCHK(c, hipMemcpy(c->dseg2, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
     // the two-stage kernel needs every ghost source zone inside the wave, and
     // no field that depends on the whole column between the stages
-    c->can_fuse1d = !c->mono && !c->has_slab;
+    c->can_fuse1d = !c->mono && !c->has_slab && rp->hydro_interp_order == 3;
     for (int p = 0; p < nprob; p++)
       if (c->hprob[p].bc_lo[0] == MDS_BC_PERIODIC || c->hprob[p].bc_hi[0] == MDS_BC_PERIODIC) c->can_fuse1d = 0;
     c->fuse1d = c->can_fuse1d;
@@ -855,6 +861,7 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.grav = c->dgrav;
     a.ncu = c->ncu;
     a.persist = c->grid1d;
+    a.gen = c->has_slab || c->hprob[0].order != 3;
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       // monopole field of the stage input state (gravity.c:145-251 at p_to_phys_src_start)
@@ -886,6 +893,7 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.chunk = c->kc3d;
     a.nchunk = (h.n[2] + a.chunk - 1) / a.chunk;
     a.src = (h.g0[0] != 0.0 || h.g0[1] != 0.0 || h.g0[2] != 0.0);
+    a.slab = c->has_slab || h.order != 3;     // the general instances: ghost layers, low orders
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       HIPCHK(c, mds_launch_sweep3d(1, c->ty3d, c->halo3d, &a, &sp, &sh, stage, c->st));
@@ -903,6 +911,7 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     a.nstrip = (c->hprob[0].n[0] + MDS_SEG1D - 1) / MDS_SEG1D;
     a.nchunk = (c->hprob[0].n[1] + a.chunk - 1) / a.chunk;
     a.src = (c->hprob[0].g0[0] != 0.0 || c->hprob[0].g0[1] != 0.0 || c->hprob[0].g0[2] != 0.0);
+    a.slab = c->has_slab || c->hprob[0].order != 3;     // the general instances: ghost layers, low orders
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       HIPCHK(c, mds_launch_sweep2d(1, c->nv, &a, &sp, &sh, stage, c->st));
@@ -928,6 +937,7 @@ static int enqueue_fused1d(mds_ctx *c)
   a.grav = nullptr;
   a.ncu = c->ncu;
   a.persist = c->grid1d;
+  a.gen = 0;                  // two-stage instances are plain (order 3, no ghost layers)
   int fp32 = c->precision == MDS_FP32_BULK;
   if (fp32) {
     StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
@@ -1141,6 +1151,12 @@ extern "C" int mds_valu_probe(int fp64, int reps, double *wave_instr_per_s)
   return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
 }
 
+// 3D Cartesian problems on the general sweep instances (one tile shape)
+static bool gen3d(const mds_ctx *c)
+{
+  return c->ndim == 3 && !c->sph3 && (c->has_slab || c->hprob[0].order != 3);
+}
+
 extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
 {
   if (!c) return MDS_EINVAL;
@@ -1151,6 +1167,8 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
   } else if (param == MDS_TILE_TY3D) {
     if (value != 4 && value != 8 && value != 12)
       return set_err(c, MDS_EINVAL, "3D tile rows must be 4, 8 or 12: %d", value);
+    if (value != 4 && gen3d(c))
+      return set_err(c, MDS_EINVAL, "3D problems with PROB/EXTRAP sides or hydro_interp_order < 3 run 4-row tiles");
     slot = &c->ty3d;
   } else if (param == MDS_TILE_KC3D) {
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "planes per workgroup out of range: %d", value);
@@ -1162,6 +1180,9 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
     slot = &c->fuse1d;
   } else if (param == MDS_TILE_HALO3D) {
     if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "3D halo mode must be 0 or 1: %d", value);
+    if (value && gen3d(c))
+      return set_err(c, MDS_EINVAL,
+                     "3D problems with PROB/EXTRAP sides or hydro_interp_order < 3 run the halo-wavefront tile");
     slot = &c->halo3d;
   } else if (param == MDS_TILE_GRID1D) {
     if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "1D grid mode must be 0 or 1: %d", value);
@@ -1257,10 +1278,12 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})
       if (v <= c->hprob[0].n[1]) cand.push_back({v, 0, 0});
   } else {
+    // problems with PROB/EXTRAP ghost layers have one tile shape (4 rows,
+    // halo wavefronts); only the planes per workgroup vary
     for (int hs : {0, 1})
       for (int ty : {4, 8, 12})
         for (int kc : {8, 16, 32, 64})
-          if (kc <= c->hprob[0].n[2]) cand.push_back({ty, kc, hs});
+          if (kc <= c->hprob[0].n[2] && (!gen3d(c) || (ty == 4 && hs == 0))) cand.push_back({ty, kc, hs});
   }
   auto apply = [&](const Shape &v) {
     if (c->ndim == 1) { c->fuse1d = v.a; c->grid1d = v.b; }

@@ -64,6 +64,7 @@ struct DevProb {
   // state before every stage; P and Gamma_1 are refilled for both.
   int has_slab;
   double *gs[3][2];
+  int order;               // hydro_interp_order (3 para_recon, 2 lin_recon, 1 constant)
 };
 
 #define MDS_SLAB_NV 8
@@ -120,6 +121,7 @@ struct Sweep1DArgs {
   const double *grav;      // monopole gravity at zone centres, packed like the state
   int ncu;                 // compute units of the device
   int persist;             // 1: persistent grid (resident workgroups walk the segment table)
+  int gen;                 // the general instances: PROB/EXTRAP ghost layers, hydro_interp_order < 3
 };
 
 struct Sweep2DArgs {
@@ -129,6 +131,7 @@ struct Sweep2DArgs {
   unsigned char *flags;    // stage-1 shock tags, one byte per zone
   int nstrip, nchunk, chunk;
   int src;                 // source terms present (fixed gravity)
+  int slab;                // the general instances: PROB/EXTRAP ghost layers, hydro_interp_order < 3
 };
 
 struct Sweep3DArgs {
@@ -138,6 +141,7 @@ struct Sweep3DArgs {
   unsigned char *flags;    // stage-1 shock tags, one byte per zone
   int nstrip, ntile, nchunk, chunk;
   int src;                 // source terms present (fixed gravity)
+  int slab;                // the general instances: PROB/EXTRAP ghost layers, hydro_interp_order < 3
 };
 
 // buffers of the direction-split 3D spherical stage

@@ -71,6 +71,7 @@ struct ReconGeo64 {
   double gm, g0, gp;     // Gamma
   double xlo, xhi;
   double den, den2;
+  int order = 3;         // hydro_interp_order: 3 para_recon, 2 lin_recon, 1 piecewise constant
 };
 
 MDS_DEV void recon_geo_x64(ReconGeo64 &g)
@@ -122,6 +123,36 @@ MDS_DEV void para_recon_x64(double qm, double q0, double qp, const ReconGeo64 &g
   }
   al = l;
   ar = r;
+}
+
+// lin_recon for one zone (reconstruct.c:89-124): a limited linear profile
+// through the zone's centroid x0 = beta/Gamma, the slope the smallest of the
+// one-sided slopes to the neighbours' centroid-face midpoints and the centred
+// slope; flat at a local extremum
+MDS_DEV void lin_recon_x64(double qm, double q0, double qp, const ReconGeo64 &g, double &al, double &ar)
+{
+  const double xm = g.bm / g.gm;
+  const double xp = g.bp / g.gp;
+  const double x0 = g.b0 / g.g0;
+  const double sm = (q0 - qm) / (x0 - (0.5 * (g.xlo + xm)));
+  const double sp = (qp - q0) / ((0.5 * (g.xhi + xp)) - x0);
+  if (sm * sp <= 0.0) { al = q0; ar = q0; return; }
+  const double sc = (qp - qm) / (xp - xm);
+  const double asm_ = fabs(sm), asp = fabs(sp), asc = fabs(sc);
+  const double m1 = asm_ < asp ? asm_ : asp;                // MIN of decs.h:627, NaN order included
+  const double A = (sc < 0.0 ? -1.0 : 1.0) * (m1 < asc ? m1 : asc);
+  const double B = q0 - ((A * g.b0) / g.g0);
+  al = (A * g.xlo) + B;
+  ar = (A * g.xhi) + B;
+}
+
+// the reconstruction of hydro_interp_order (reconstruct.c:246-266): every
+// zone of a launch shares the order, so the branch is uniform
+MDS_DEV void recon_x64(double qm, double q0, double qp, const ReconGeo64 &g, double &al, double &ar)
+{
+  if (g.order == 3) para_recon_x64(qm, q0, qp, g, al, ar);
+  else if (g.order == 2) lin_recon_x64(qm, q0, qp, g, al, ar);
+  else { al = q0; ar = q0; }
 }
 
 // wave-speed estimate shared by HLLC/HLLE (riemann.c:210-303, 368-429)

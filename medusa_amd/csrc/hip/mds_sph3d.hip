@@ -30,6 +30,7 @@
 // limited linear) and volume-weighted restriction (restrict0/1),
 // step.c:783-1085.  The update gathers the finer faces of a coarser zone
 // (the fan-in of step.c:374-433).
+#include <type_traits>
 #include "mds_internal.h"
 #include "mds_physics.h"
 #include "mds_eos.h"
@@ -280,7 +281,7 @@ MDS_DEV FaceGeo<double> face_geo(const ZRec &rc)
   return fg;
 }
 
-template <class R, int D, int TAB, int GREC>
+template <class R, int D, int TAB, int GREC, int LO = 0>
 MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool polar, int fk, int dk,
                       const ZRec &rc, bool hlle, double F[6], double &sig)
 {
@@ -306,6 +307,7 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
     gr.xhi = pb.startx[D] + ((fd + 1) * s) * pb.dx[D];
     recon_geo_x64(gl);
     recon_geo_x64(gr);
+    gl.order = gr.order = pb.order;
     double Lq[8], Rq[8];
     if (polar) {
 #pragma unroll
@@ -315,8 +317,13 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
       for (int v = 0; v < 8; v++) {
         if (!TAB && (v == 5 || (v == 7 && !GREC))) { Lq[v] = q[1][v]; Rq[v] = q[2][v]; continue; }
         double al, ar, al2, ar2;
-        para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
-        para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+        if (LO) {   // hydro_interp_order 2 or 1
+          recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
+          recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+        } else {
+          para_recon_x64(q[0][v], q[1][v], q[2][v], gl, al, ar);
+          para_recon_x64(q[1][v], q[2][v], q[3][v], gr, al2, ar2);
+        }
         Lq[v] = ar;
         Rq[v] = al2;
       }
@@ -386,7 +393,7 @@ MDS_DEV int face_flux(const DevProb &pb, int lev, int fd, double q[4][8], bool p
 // PPM of one zone along D from its 3-zone stencil q3 (zones zd-1, zd, zd+1,
 // level lev): the left and right face values of every pencil slot.  Equal,
 // slot by slot, to the values face_flux forms for the two faces of the zone.
-template <class R, int D, int TAB>
+template <class R, int D, int TAB, int LO = 0>
 MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8], R al[8], R ar[8])
 {
   const int s = 1 << lev;
@@ -400,11 +407,13 @@ MDS_DEV void zone_recon(const DevProb &pb, int lev, int zd, const double q3[3][8
     g.xlo = pb.startx[D] + (zd * s) * pb.dx[D];
     g.xhi = pb.startx[D] + ((zd + 1) * s) * pb.dx[D];
     recon_geo_x64(g);
+    g.order = pb.order;
 #pragma unroll
     for (int v = 0; v < 8; v++) {
       if (!TAB && (v == 5 || v == 7)) { al[v] = (R)q3[1][v]; ar[v] = (R)q3[1][v]; continue; }
       double l, r;
-      para_recon_x64(q3[0][v], q3[1][v], q3[2][v], g, l, r);
+      if (LO) recon_x64(q3[0][v], q3[1][v], q3[2][v], g, l, r);     // hydro_interp_order 2 or 1
+      else para_recon_x64(q3[0][v], q3[1][v], q3[2][v], g, l, r);
       al[v] = (R)l;
       ar[v] = (R)r;
     }
@@ -756,7 +765,7 @@ __global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, const doubl
 // previous wavefront for theta and phi faces (a 64 x 8 block of zones in
 // (i, j) or (i, k), 7 rows of faces).
 // ---------------------------------------------------------------------------
-template <class R, int D, int TAB, int DEND>
+template <class R, int D, int TAB, int DEND, int LO = 0>
 __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> in)
 {
   const DevProb &pb = a.pb;
@@ -874,7 +883,10 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
   if (DEND && D == 2 && zvalid) zlev = Lk[0];
   const int bi = zvalid ? zi : 0, bj = zvalid ? zj : 0, bk = zvalid ? zk : 0;
   double q3[3][8];
-  if (zvalid) {
+  // the zone's stencil; with PROB/EXTRAP ghost layers at the outer radius
+  // (radial faces) a second copy of the loop reads them, so that the plain
+  // loop stays one block
+  auto stencil = [&](auto slab) {
 #pragma unroll
     for (int s = 0; s < 3; s++) {
       int i = bi, j = bj, k = bk;
@@ -891,10 +903,14 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
       }
       double sgn[3];
       map_zone_n(pb, D, i, j, k, sgn, n, nkm);
-      if (D == 0 && i >= pb.n[0]) slab_state<R>(pb, i, j, k, TAB, q3[s]);
+      if (decltype(slab)::value && i >= pb.n[0]) slab_state<R>(pb, i, j, k, TAB, q3[s]);
       else zone_state<R>(a, in, zid3(pb, i, j, k), TAB, gam, q3[s]);
       for (int d = 0; d < 3; d++) q3[s][2 + d] *= sgn[d];
     }
+  };
+  if (zvalid) {
+    if (D == 0 && pb.has_slab) stencil(std::true_type{});
+    else stencil(std::false_type{});
   } else {
     for (int s = 0; s < 3; s++)
       for (int v = 0; v < 8; v++) q3[s][v] = 1.0;
@@ -919,7 +935,7 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
   if (polar) {
     for (int v = 0; v < 8; v++) al[v] = ar[v] = (R)q3[1][v];
   } else {
-    zone_recon<R, D, TAB>(pb, zlev, zd, q3, al, ar);
+    zone_recon<R, D, TAB, LO>(pb, zlev, zd, q3, al, ar);
   }
   // the left state of this thread's face
   R Lq[8];
@@ -977,7 +993,7 @@ __global__ __launch_bounds__(512, 4) void sph3_flux(Sph3Args a, StateArrays<R> i
 // independent, and each is a chain of dependent reads (levels, then the zones
 // of a restriction or a limited-slope prolongation).  The entries meet in LDS
 // and the first wavefront solves the faces.
-template <class R, int D, int TAB>
+template <class R, int D, int TAB, int LO = 0>
 __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> in, const int4 *list, long long n)
 {
   const DevProb &pb = a.pb;
@@ -1021,7 +1037,7 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
   }
   const ZRec rc = rec3(pb, fi, fj);
   double sig, F[6];
-  const int isshock = face_flux<R, D, TAB, 1>(pb, lev, D == 0 ? fi : fj, q, false, fk, 1, rc, hlle, F, sig);
+  const int isshock = face_flux<R, D, TAB, 1, LO>(pb, lev, D == 0 ? fi : fj, q, false, fk, 1, rc, hlle, F, sig);
   if (live && active) {
     store_flux<R>(a, D, fid3(pb, D, fi, fj, fk), F, TAB ? 6 : 5);
     if (D == 0 && isshock) tag_radial(a, fi, fj, fk);
@@ -1311,16 +1327,30 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
                          dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)((pb.n[1] + 1 + 6) / 7), (unsigned)pb.n[2]),
                          dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)pb.n[1], (unsigned)((pb.n[2] + 1 + 6) / 7))};
   const dim3 fblk(512);
+  // hydro_interp_order 2 or 1 (FP64 only) runs its own instances, so the
+  // para_recon kernels keep their register budget
+  const bool lo = sizeof(R) == 8 && pb.order != 3;
+  constexpr int LOW_R = sizeof(R) == 8 ? 1 : 0;     // FP32-bulk runs order 3 only
 #define MDS_FLUX(D)                                                                                        \
   {                                                                                                        \
-    if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND>), fgrid[D], fblk, 0, st, a, in);              \
-    else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND>), fgrid[D], fblk, 0, st, a, in);                    \
+    if (lo) {                                                                                              \
+      if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND, LOW_R>), fgrid[D], fblk, 0, st, a, in);     \
+      else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND, LOW_R>), fgrid[D], fblk, 0, st, a, in);           \
+    } else {                                                                                               \
+      if (L.tab) hipLaunchKernelGGL((sph3_flux<R, D, 1, DEND>), fgrid[D], fblk, 0, st, a, in);            \
+      else hipLaunchKernelGGL((sph3_flux<R, D, 0, DEND>), fgrid[D], fblk, 0, st, a, in);                  \
+    }                                                                                                      \
   }
 #define MDS_FLUX_IRR(D)                                                                                    \
   if (DEND && L.nirr[D] > 0) {                                                                             \
     const dim3 g((unsigned)((L.nirr[D] + 63) / 64));                                                      \
-    if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);   \
-    else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);         \
+    if (lo) {                                                                                              \
+      if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1, LOW_R>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]); \
+      else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0, LOW_R>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);       \
+    } else {                                                                                               \
+      if (L.tab) hipLaunchKernelGGL((sph3_flux_irr<R, D, 1>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);   \
+      else hipLaunchKernelGGL((sph3_flux_irr<R, D, 0>), g, blk, 0, s2, a, in, L.irr[D], L.nirr[D]);         \
+    }                                                                                                      \
   }
   MDS_FLUX_IRR(0)
   rec(2, s2);

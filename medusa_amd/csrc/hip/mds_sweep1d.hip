@@ -41,14 +41,17 @@ namespace {
 // fill would copy there (bc.c:727-1030 for direction 0).  flip gets the
 // velocity components whose sign the fill changes (bit d: v^(d+1)): REFLECT
 // v^1, the inner radius through the origin (SPHERICAL_ORIGIN, bc.c:774-801)
-// v^1 and v^3.  A PROB or EXTRAP side has its own ghost layers: flip gets bit
-// 8 (and bit 16 above the grid) and the layer index is returned.
+// v^1 and v^3.  In the general instances (GEN) a PROB or EXTRAP side has its
+// own ghost layers: flip gets bit 8 (and bit 16 above the grid) and the layer
+// index is returned.
+template <bool GEN = false>
 MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
 {
   flip = 0;
   if (c < 0) {
     if (c < -MDS_NG) return 0;
-    if (lo == MDS_BC_PROB || lo == MDS_BC_EXTRAP) { flip = 8; return c + MDS_NG; }
+    if constexpr (GEN)
+      if (lo == MDS_BC_PROB || lo == MDS_BC_EXTRAP) { flip = 8; return c + MDS_NG; }
     if (lo == MDS_BC_REFLECT) { flip = 1; return -1 - c; }
     if (lo == MDS_BC_SPHERICAL_ORIGIN) { flip = 5; return -1 - c; }
     if (lo == MDS_BC_PERIODIC) return c + n;
@@ -56,7 +59,8 @@ MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
   }
   if (c >= n) {
     if (c >= n + MDS_NG) return n - 1;
-    if (hi == MDS_BC_PROB || hi == MDS_BC_EXTRAP) { flip = 8 | 16; return c - n; }
+    if constexpr (GEN)
+      if (hi == MDS_BC_PROB || hi == MDS_BC_EXTRAP) { flip = 8 | 16; return c - n; }
     if (hi == MDS_BC_REFLECT) { flip = 1; return 2 * n - 1 - c; }
     if (hi == MDS_BC_PERIODIC) return c - n;
     return n - 1;
@@ -143,10 +147,10 @@ MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, int f
 }
 
 // the zone bc_src names: a state zone, or a ghost layer of a PROB/EXTRAP side
-template <int NV, int TAB, class R>
+template <int NV, int TAB, bool GEN, class R>
 MDS_DEV Prim<R> load_zone(const DevProb &pb, const StateArrays<R> &in, int s, int flip)
 {
-  if (flip & 8) {
+  if (GEN && (flip & 8)) {
     const double *e = pb.gs[0][(flip >> 4) & 1] + (size_t)s * MDS_SLAB_NV;   // one transverse position
     Prim<R> q;
     q.rho = (R)e[0];
@@ -193,6 +197,7 @@ MDS_DEV void load_geo64(const DevProb &pb, int c, int n, Geo64 &g)
   rg.xlo = pb.startx[0] + c * pb.dx[0];
   rg.xhi = pb.startx[0] + (c + 1) * pb.dx[0];
   recon_geo_x64(rg);
+  rg.order = pb.order;
   if (SPH) {
     const double *zg = pb.zone + (size_t)q * MDS_ZG_COUNT;
     g.zg = zg;
@@ -226,7 +231,7 @@ MDS_DEV void conserved_x64(const Prim<double> &q, const double gcov[3], double U
 // conserved form of U^n (sim_p) at this zone, qn its primitives; gz: monopole
 // gravity at the zone centre.  The output is the input where the update is
 // not valid.
-template <int NV, bool SPH, int TAB, int MONO>
+template <int NV, bool SPH, int TAB, int MONO, int GEN = 0>
 MDS_DEV Prim<double> stage_x64(const DevProb &pb, const DevEos &eos, const Geo64 &g, int stage, double dt,
                                const Prim<double> &q, const double U0[6], const Prim<double> &qn, double gz,
                                Roles r, Lim1D &lim)
@@ -250,11 +255,24 @@ MDS_DEV Prim<double> stage_x64(const DevProb &pb, const DevEos &eos, const Geo64
     for (int d = 0; d < NV; d++) qv[2 + d] = q.v[d];
     if (TAB) { qv[2 + NV] = q.ye; qv[3 + NV] = press; qv[4 + NV] = gam1; }
     else qv[2 + NV] = press;
+    // hydro_interp_order 2 or 1: general instances only, one uniform branch
+    // around the variable loop
+    bool low = false;
+    if constexpr (GEN != 0) low = g.rg.order != 3;
+    if (!low) {
 #pragma unroll
-    for (int k = 0; k < NQ; k++) {
-      double qm = lane_from_below(qv[k]);
-      double qp = lane_from_above(qv[k]);
-      para_recon_x64(qm, qv[k], qp, g.rg, al[k], ar[k]);
+      for (int k = 0; k < NQ; k++) {
+        double qm = lane_from_below(qv[k]);
+        double qp = lane_from_above(qv[k]);
+        para_recon_x64(qm, qv[k], qp, g.rg, al[k], ar[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
+        double qm = lane_from_below(qv[k]);
+        double qp = lane_from_above(qv[k]);
+        recon_x64(qm, qv[k], qp, g.rg, al[k], ar[k]);
+      }
     }
   }
   // face below this zone: left state = right edge of the zone below
@@ -372,7 +390,7 @@ MDS_DEV Prim<double> stage_x64(const DevProb &pb, const DevEos &eos, const Geo64
 }  // namespace
 
 // FP64 single-stage kernel: stage 0 reads p, writes ph; stage 1 reads p, ph, writes p.
-template <int NV, bool SPH, int TAB, int MONO>
+template <int NV, bool SPH, int TAB, int MONO, int GEN>
 __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<double> sp,
                                                    StateArrays<double> sph, int stage)
 {
@@ -394,8 +412,8 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
 
     const StateArrays<double> &in = (stage == 0) ? sp : sph;
     int flip;
-    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<double> q = load_zone<NV, TAB>(pb, in, s, flip);
+    const int s = bc_src<GEN != 0>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<double> q = load_zone<NV, TAB, GEN != 0>(pb, in, s, flip);
     Geo64 g;
     load_geo64<SPH>(pb, c, n, g);
 
@@ -411,7 +429,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
     Lim1D lim;
-    const Prim<double> o = stage_x64<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
+    const Prim<double> o = stage_x64<NV, SPH, TAB, MONO, GEN>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
     if (r.out) {
       StateArrays<double> &out = (stage == 0) ? sph : sp;
       store_prims<NV, TAB>(out, off, c, o);
@@ -447,8 +465,8 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
     }
     const double dt = ck->dt;
     int flip;
-    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<double> q = load_zone<NV, TAB>(pb, in, s, flip);
+    const int s = bc_src<false>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<double> q = load_zone<NV, TAB, false>(pb, in, s, flip);
     Geo64 g;
     load_geo64<SPH>(pb, c, n, g);
     double U0[6];
@@ -651,7 +669,7 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
 
 }  // namespace
 
-template <int NV, bool SPH, int TAB, int MONO>
+template <int NV, bool SPH, int TAB, int MONO, int GEN>
 __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<float> sp,
                                                    StateArrays<float> sph, int stage)
 {
@@ -673,8 +691,8 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
 
     const StateArrays<float> &in = (stage == 0) ? sp : sph;
     int flip;
-    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<float> q = load_zone<NV, TAB>(pb, in, s, flip);
+    const int s = bc_src<GEN != 0>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<float> q = load_zone<NV, TAB, GEN != 0>(pb, in, s, flip);
     Geo32 g;
     load_geo32<SPH>(pb, c, n, g);
     Roles r;
@@ -723,8 +741,8 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
     }
     const double dt = ck->dt;
     int flip;
-    const int s = bc_src(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<float> q = load_zone<NV, TAB>(pb, in, s, flip);
+    const int s = bc_src<false>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
+    const Prim<float> q = load_zone<NV, TAB, false>(pb, in, s, flip);
     Geo32 g;
     load_geo32<SPH>(pb, c, n, g);
     double U0[6];
@@ -839,7 +857,9 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
 #define MDS_GRID(K)                                                                                    \
   static const int nres = resident_blocks(K, 64 * waves_per_block);                                   \
   dim3 grid((unsigned)std::max(1, a->persist ? std::min(need, nres * std::max(1, a->ncu)) : need));
-#define MDS_L1(K, ...) do { MDS_GRID((K<__VA_ARGS__>)) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h, stage); } while (0)
+#define MDS_L1G(K, ...) do { MDS_GRID((K<__VA_ARGS__>)) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h, stage); } while (0)
+// the general instances (PROB/EXTRAP ghost layers, hydro_interp_order < 3)
+#define MDS_L1(K, ...) do { if (a->gen) MDS_L1G(K, __VA_ARGS__, 1); else MDS_L1G(K, __VA_ARGS__, 0); } while (0)
 #define MDS_L2(K, ...) do { MDS_GRID((K<__VA_ARGS__>)) hipLaunchKernelGGL((K<__VA_ARGS__>), grid, block, 0, st, *a, p, h); } while (0)
   if (!fp32) {
     const StateArrays<double> &p = *(const StateArrays<double> *)sp;
@@ -875,6 +895,7 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
     }
   }
 #undef MDS_L1
+#undef MDS_L1G
 #undef MDS_L2
 #undef MDS_GRID
   return hipGetLastError();

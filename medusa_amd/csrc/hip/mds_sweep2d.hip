@@ -25,7 +25,7 @@
 
 using namespace mds;
 
-template <class R, int NV, bool SRC>
+template <class R, int NV, bool SRC, bool SLAB>
 __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp, StateArrays<R> sph, int stage)
 {
   constexpr bool X64 = std::is_same<R, double>::value;
@@ -61,10 +61,10 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   unsigned long long nshock = 0, nhlle = 0;
 
   Cell<R, NV> W[5];
-  for (int k = 0; k < 4; k++) load_cell<R, NV>(in, pb, i, j0 - 2 + k, 0, gm1, W[k]);
+  for (int k = 0; k < 4; k++) load_cell<R, NV, SLAB>(in, pb, i, j0 - 2 + k, 0, gm1, W[k]);
   auto pextra_of = [&](int j) -> R {
     R pe = (R)1;
-    if (lane == 63) pe = load_press<R>(in, pb, i + 1, j, 0, gm1);
+    if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, j, 0, gm1);
     return pe;
   };
   auto stage1_tag = [&](int j) -> int {
@@ -74,11 +74,11 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
 
   int tag_prev;
   if (j0 - 1 < 0) tag_prev = 1;
-  else tag_prev = zone_tag(xshock_only<R>(W[1].P, pextra_of(j0 - 1), rgx)) | stage1_tag(j0 - 1);
+  else tag_prev = zone_tag(xshock_only<R, SLAB>(W[1].P, pextra_of(j0 - 1), rgx)) | stage1_tag(j0 - 1);
 
   R Fx[5], sx;
   int shx;
-  xsweep<R, NV>(W[2], pextra_of(j0), rgx, gam, igm1, area0, Fx, sx, shx);
+  xsweep<R, NV, SLAB>(W[2], pextra_of(j0), rgx, gam, igm1, area0, Fx, sx, shx);
   if (xface_ok) smax_x = sx > smax_x ? sx : smax_x;
   if (xface_ok && lane < 62) nshock += shx;
   int tag_cur = zone_tag(shx) | stage1_tag(j0);
@@ -91,9 +91,9 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
     typename ReconGeoT<R>::type rg;
     Cell<R, NV> yl_prev, yr_prev;
     load_rgeo<R>(pb, 1, j0 - 1, rg);
-    recon_cell<R, NV>(W[0], W[1], W[2], rg, igm1, yl_prev, yr_prev);
+    recon_cell<R, NV, SLAB>(W[0], W[1], W[2], rg, igm1, yl_prev, yr_prev);
     load_rgeo<R>(pb, 1, j0, rg);
-    recon_cell<R, NV>(W[1], W[2], W[3], rg, igm1, yl_cur, yr_cur);
+    recon_cell<R, NV, SLAB>(W[1], W[2], W[3], rg, igm1, yl_cur, yr_cur);
     bool hl = (tag_prev | tag_cur) != 0;
     R sy;
     solve<R, NV>(yr_prev, yl_cur, gam, area1, 1, hl, Fym, sy);
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   const double ffdt = ff * dt;
   const double idx0 = 1.0 / pb.dx[0], idx1 = 1.0 / pb.dx[1];
   for (int j = j0; j < j1; j++) {
-    load_cell<R, NV>(in, pb, i, j + 2, 0, gm1, W[4]);
+    load_cell<R, NV, SLAB>(in, pb, i, j + 2, 0, gm1, W[4]);
     int tag_next;
     R Fxp_next[5], Fxm_next[5];
     if (j + 1 >= n2) {
@@ -115,19 +115,19 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
     } else if (j + 1 < j1) {
       R F2[5], s2;
       int sh2;
-      xsweep<R, NV>(W[3], pextra_of(j + 1), rgx, gam, igm1, area0, F2, s2, sh2);
+      xsweep<R, NV, SLAB>(W[3], pextra_of(j + 1), rgx, gam, igm1, area0, F2, s2, sh2);
       if (xface_ok) smax_x = s2 > smax_x ? s2 : smax_x;
       if (xface_ok && lane < 62) nshock += sh2;
       tag_next = zone_tag(sh2) | stage1_tag(j + 1);
       for (int q = 0; q < 2 + NV; q++) { Fxp_next[q] = lane_from_above(F2[q]); Fxm_next[q] = F2[q]; }
     } else {
-      tag_next = zone_tag(xshock_only<R>(W[3].P, pextra_of(j + 1), rgx)) | stage1_tag(j + 1);
+      tag_next = zone_tag(xshock_only<R, SLAB>(W[3].P, pextra_of(j + 1), rgx)) | stage1_tag(j + 1);
     }
     Cell<R, NV> yl_next, yr_next;
     {
       typename ReconGeoT<R>::type rg;
       load_rgeo<R>(pb, 1, j + 1, rg);
-      recon_cell<R, NV>(W[2], W[3], W[4], rg, igm1, yl_next, yr_next);
+      recon_cell<R, NV, SLAB>(W[2], W[3], W[4], rg, igm1, yl_next, yr_next);
     }
     R Fyp[5], sy;
     bool hl = (tag_cur | tag_next) != 0;
@@ -243,9 +243,15 @@ extern "C" hipError_t mds_launch_sweep2d(int fp32, int nv, const Sweep2DArgs *a,
   int waves = a->nstrip * a->nchunk;
   dim3 block(64 * wpb), grid((waves + wpb - 1) / wpb);
   bool src = a->src;
-#define MDS_L2(R, NV, SRC)                                                                        \
-  hipLaunchKernelGGL((sweep2d<R, NV, SRC>), grid, block, 0, st, *a, *(const StateArrays<R> *)sp, \
-                     *(const StateArrays<R> *)sph, stage)
+#define MDS_L2(R, NV, SRC)                                                                                \
+  do {                                                                                                    \
+    if (a->slab)                                                                                          \
+      hipLaunchKernelGGL((sweep2d<R, NV, SRC, true>), grid, block, 0, st, *a, *(const StateArrays<R> *)sp, \
+                         *(const StateArrays<R> *)sph, stage);                                            \
+    else                                                                                                  \
+      hipLaunchKernelGGL((sweep2d<R, NV, SRC, false>), grid, block, 0, st, *a, *(const StateArrays<R> *)sp, \
+                         *(const StateArrays<R> *)sph, stage);                                            \
+  } while (0)
   if (!fp32) {
     if (nv == 2) { if (src) MDS_L2(double, 2, true); else MDS_L2(double, 2, false); }
     else { if (src) MDS_L2(double, 3, true); else MDS_L2(double, 3, false); }

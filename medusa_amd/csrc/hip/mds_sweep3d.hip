@@ -23,7 +23,7 @@
 
 using namespace mds;
 
-template <class R, int NV, bool SRC, int TY>
+template <class R, int NV, bool SRC, int TY, bool SLAB = false>
 __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArrays<R> sp, StateArrays<R> sph,
                                                           int stage)
 {
@@ -77,7 +77,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   };
   auto pextra = [&](int kk) -> R {
     R pe = (R)1;
-    if (lane == 63) pe = load_press<R>(in, pb, i + 1, j, kk, gm1);
+    if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, j, kk, gm1);
     return pe;
   };
 
@@ -86,13 +86,13 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   Cell<R, NV> zl_cur, zr_cur;
   int tag_cur = 1;
   if (zw) {
-    for (int q = 0; q < 4; q++) load_cell<R, NV>(in, pb, i, j, k0 - 2 + q, gm1, Z[q]);
+    for (int q = 0; q < 4; q++) load_cell<R, NV, SLAB>(in, pb, i, j, k0 - 2 + q, gm1, Z[q]);
     int tag_prev;
     if (!row_in || k0 - 1 < 0) tag_prev = 1;
-    else tag_prev = zone_tag(xshock_only<R>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
+    else tag_prev = zone_tag(xshock_only<R, SLAB>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
     R Fx[5], sx;
     int shx;
-    xsweep<R, NV>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
+    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
@@ -100,9 +100,9 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     typename ReconGeoT<R>::type rg;
     Cell<R, NV> zl_prev, zr_prev;
     load_rgeo<R>(pb, 2, k0 - 1, rg);
-    recon_cell<R, NV>(Z[0], Z[1], Z[2], rg, igm1, zl_prev, zr_prev);
+    recon_cell<R, NV, SLAB>(Z[0], Z[1], Z[2], rg, igm1, zl_prev, zr_prev);
     load_rgeo<R>(pb, 2, k0, rg);
-    recon_cell<R, NV>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
+    recon_cell<R, NV, SLAB>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
     bool hl = (tag_prev | tag_cur) != 0;
     R sz;
     solve<R, NV>(zr_prev, zl_cur, gam, pb.area[2], 2, hl, Fzm, sz);
@@ -122,23 +122,23 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     Cell<R, NV> zl_next, zr_next;
     int tag_next = 1, mytag = 1;
     if (zw) {
-      load_cell<R, NV>(in, pb, i, j, k + 2, gm1, Z[4]);
+      load_cell<R, NV, SLAB>(in, pb, i, j, k + 2, gm1, Z[4]);
       if (!row_in || k + 1 >= n3) {
         tag_next = 1;
       } else if (k + 1 < k1) {
         R F2[5], s2;
         int sh2;
-        xsweep<R, NV>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
+        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
         for (int q = 0; q < 2 + NV; q++) { Fxp_next[q] = lane_from_above(F2[q]); Fxm_next[q] = F2[q]; }
       } else {
-        tag_next = zone_tag(xshock_only<R>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
+        tag_next = zone_tag(xshock_only<R, SLAB>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
       }
       typename ReconGeoT<R>::type rg;
       load_rgeo<R>(pb, 2, k + 1, rg);
-      recon_cell<R, NV>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
+      recon_cell<R, NV, SLAB>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
       bool hl = (tag_cur | tag_next) != 0;
       R sz;
       solve<R, NV>(zr_cur, zl_next, gam, pb.area[2], 2, hl, Fzp, sz);
@@ -149,9 +149,9 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       me = Z[2];
       mytag = tag_cur;
     } else {
-      load_cell<R, NV>(in, pb, i, j, k, gm1, me);
+      load_cell<R, NV, SLAB>(in, pb, i, j, k, gm1, me);
       if ((w == 1 || w == TY + 2) && row_in)
-        mytag = zone_tag(xshock_only<R>(me.P, pextra(k), rgx)) | s1tag(k);
+        mytag = zone_tag(xshock_only<R, SLAB>(me.P, pextra(k), rgx)) | s1tag(k);
       else
         mytag = 1;
     }
@@ -170,7 +170,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       ym.uu = sCell[1][w - 1][lane];  yp.uu = sCell[1][w + 1][lane];
       for (int d = 0; d < NV; d++) { ym.v[d] = sCell[2 + d][w - 1][lane]; yp.v[d] = sCell[2 + d][w + 1][lane]; }
       ym.P = sCell[2 + NV][w - 1][lane]; yp.P = sCell[2 + NV][w + 1][lane];
-      recon_cell<R, NV>(ym, me, yp, rgy, igm1, yl, yr);
+      recon_cell<R, NV, SLAB>(ym, me, yp, rgy, igm1, yl, yr);
       sAr[0][w][lane] = yr.rho;
       sAr[1][w][lane] = yr.uu;
       for (int d = 0; d < NV; d++) sAr[2 + d][w][lane] = yr.v[d];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
 // rows, and the first and last wavefronts also reconstruct the two halo rows
 // that have faces (rows j0-1 and j0+TY) and solve the top face.  No
 // wavefront of the workgroup is idle in the x and z sweeps.
-template <class R, int NV, bool SRC, int TY>
+template <class R, int NV, bool SRC, int TY, bool SLAB = false>
 __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R> sp, StateArrays<R> sph,
                                                     int stage)
 {
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   };
   auto pextra = [&](int kk) -> R {
     R pe = (R)1;
-    if (lane == 63) pe = load_press<R>(in, pb, i + 1, j, kk, gm1);
+    if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, j, kk, gm1);
     return pe;
   };
   auto s1tag_h = [&](int kk) -> int {
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   };
   auto pextra_h = [&](int kk) -> R {
     R pe = (R)1;
-    if (lane == 63) pe = load_press<R>(in, pb, i + 1, jh, kk, gm1);
+    if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, jh, kk, gm1);
     return pe;
   };
 
@@ -382,13 +382,13 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   Cell<R, NV> zl_cur, zr_cur;
   int tag_cur = 1;
   if (zw) {
-    for (int q = 0; q < 4; q++) load_cell<R, NV>(in, pb, i, j, k0 - 2 + q, gm1, Z[q]);
+    for (int q = 0; q < 4; q++) load_cell<R, NV, SLAB>(in, pb, i, j, k0 - 2 + q, gm1, Z[q]);
     int tag_prev;
     if (!row_in || k0 - 1 < 0) tag_prev = 1;
-    else tag_prev = zone_tag(xshock_only<R>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
+    else tag_prev = zone_tag(xshock_only<R, SLAB>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
     R Fx[5], sx;
     int shx;
-    xsweep<R, NV>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
+    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
@@ -396,9 +396,9 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     typename ReconGeoT<R>::type rg;
     Cell<R, NV> zl_prev, zr_prev;
     load_rgeo<R>(pb, 2, k0 - 1, rg);
-    recon_cell<R, NV>(Z[0], Z[1], Z[2], rg, igm1, zl_prev, zr_prev);
+    recon_cell<R, NV, SLAB>(Z[0], Z[1], Z[2], rg, igm1, zl_prev, zr_prev);
     load_rgeo<R>(pb, 2, k0, rg);
-    recon_cell<R, NV>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
+    recon_cell<R, NV, SLAB>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
     bool hl = (tag_prev | tag_cur) != 0;
     R sz;
     solve<R, NV>(zr_prev, zl_cur, gam, pb.area[2], 2, hl, Fzm, sz);
@@ -418,23 +418,23 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     Cell<R, NV> zl_next, zr_next;
     int tag_next = 1, mytag = 1;
     if (zw) {
-      load_cell<R, NV>(in, pb, i, j, k + 2, gm1, Z[4]);
+      load_cell<R, NV, SLAB>(in, pb, i, j, k + 2, gm1, Z[4]);
       if (!row_in || k + 1 >= n3) {
         tag_next = 1;
       } else if (k + 1 < k1) {
         R F2[5], s2;
         int sh2;
-        xsweep<R, NV>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
+        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
         for (int q = 0; q < 2 + NV; q++) { Fxp_next[q] = lane_from_above(F2[q]); Fxm_next[q] = F2[q]; }
       } else {
-        tag_next = zone_tag(xshock_only<R>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
+        tag_next = zone_tag(xshock_only<R, SLAB>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
       }
       typename ReconGeoT<R>::type rg;
       load_rgeo<R>(pb, 2, k + 1, rg);
-      recon_cell<R, NV>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
+      recon_cell<R, NV, SLAB>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
       bool hl = (tag_cur | tag_next) != 0;
       R sz;
       solve<R, NV>(zr_cur, zl_next, gam, pb.area[2], 2, hl, Fzp, sz);
@@ -449,8 +449,8 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     Cell<R, NV> hc;
     int htag = 1;
     if (h >= 0) {
-      load_cell<R, NV>(in, pb, i, jh, k, gm1, hc);
-      if ((h == 1 || h == TY + 2) && halo_in) htag = zone_tag(xshock_only<R>(hc.P, pextra_h(k), rgx)) | s1tag_h(k);
+      load_cell<R, NV, SLAB>(in, pb, i, jh, k, gm1, hc);
+      if ((h == 1 || h == TY + 2) && halo_in) htag = zone_tag(xshock_only<R, SLAB>(hc.P, pextra_h(k), rgx)) | s1tag_h(k);
     }
 
     // ---- y direction through LDS ----
@@ -476,18 +476,18 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     }
     __syncthreads();
     Cell<R, NV> yl, yr, ylh;
-    recon_cell<R, NV>(get(sCell, r - 1), me, get(sCell, r + 1), rgy, igm1, yl, yr);
+    recon_cell<R, NV, SLAB>(get(sCell, r - 1), me, get(sCell, r + 1), rgy, igm1, yl, yr);
     put(sAr, r, yr);
     if (w == 0) {
       // row j0-1: its right state for face j0
       Cell<R, NV> a0, a1;
-      recon_cell<R, NV>(get(sCell, 0), get(sCell, 1), get(sCell, 2), rgh, igm1, a0, a1);
+      recon_cell<R, NV, SLAB>(get(sCell, 0), get(sCell, 1), get(sCell, 2), rgh, igm1, a0, a1);
       put(sAr, 1, a1);
     }
     if (top) {
       // row j0+TY: its left state for face j0+TY
       Cell<R, NV> a1;
-      recon_cell<R, NV>(get(sCell, TY + 1), get(sCell, TY + 2), get(sCell, TY + 3), rgh, igm1, ylh, a1);
+      recon_cell<R, NV, SLAB>(get(sCell, TY + 1), get(sCell, TY + 2), get(sCell, TY + 3), rgh, igm1, ylh, a1);
     }
     __syncthreads();
     {
@@ -615,9 +615,14 @@ extern "C" hipError_t mds_launch_sweep3d(int fp32, int ty, int hshare, const Swe
 {
   dim3 grid((unsigned)(a->nstrip * a->ntile * a->nchunk));
   bool src = a->src;
+  // the general instances (PROB/EXTRAP ghost layers, hydro_interp_order < 3) have one shape: 4 rows, halo wavefronts
+  if (a->slab && (ty != 4 || hshare)) return hipErrorInvalidValue;
 #define MDS_L3(R, SRC, TY)                                                                           \
   do {                                                                                               \
-    if (hshare)                                                                                      \
+    if (a->slab)                                                                                     \
+      hipLaunchKernelGGL((sweep3d<R, 3, SRC, 4, true>), grid, dim3(8 * 64), 0, st, *a,              \
+                         *(const StateArrays<R> *)sp, *(const StateArrays<R> *)sph, stage);          \
+    else if (hshare)                                                                                 \
       hipLaunchKernelGGL((sweep3dh<R, 3, SRC, TY>), grid, dim3(TY * 64), 0, st, *a,                \
                          *(const StateArrays<R> *)sp, *(const StateArrays<R> *)sph, stage);          \
     else                                                                                             \

@@ -64,6 +64,7 @@ MDS_DEV void load_rgeo(const DevProb &pb, int d, int z, typename ReconGeoT<R>::t
     g.xlo = pb.startx[d] + z * pb.dx[d];
     g.xhi = pb.startx[d] + (z + 1) * pb.dx[d];
     recon_geo_x64(g);
+    g.order = pb.order;
   } else {
     const float *r = pb.rc[d] + (size_t)q * MDS_RC_COUNT;
     g.clm = r[MDS_RC_CLM]; g.clp = r[MDS_RC_CLP]; g.crm = r[MDS_RC_CRM]; g.crp = r[MDS_RC_CRP];
@@ -94,10 +95,10 @@ MDS_DEV bool slab_entry(const DevProb &pb, int i, int j, int k, const double *&e
   return false;
 }
 
-template <class R, int NV>
+template <class R, int NV, bool SLAB = false>
 MDS_DEV void load_cell(const StateArrays<R> &in, const DevProb &pb, int i, int j, int k, R gm1, Cell<R, NV> &c)
 {
-  if (pb.has_slab) {
+  if constexpr (SLAB) {
     const double *e;
     long long T;
     if (slab_entry(pb, i, j, k, e, T)) {
@@ -124,10 +125,10 @@ MDS_DEV void load_cell(const StateArrays<R> &in, const DevProb &pb, int i, int j
 }
 
 // internal energy of a zone only (P of a neighbour for the lane-63 shock tag)
-template <class R>
+template <class R, bool SLAB = false>
 MDS_DEV R load_press(const StateArrays<R> &in, const DevProb &pb, int i, int j, int k, R gm1)
 {
-  if (pb.has_slab) {
+  if constexpr (SLAB) {
     const double *e;
     long long T;
     if (slab_entry(pb, i, j, k, e, T)) return gm1 * (R)e[T];
@@ -144,10 +145,21 @@ MDS_DEV R load_press(const StateArrays<R> &in, const DevProb &pb, int i, int j, 
 // path derives the internal-energy face values from the pressure ones: the
 // reconstruction is positively homogeneous, so recon(u) = recon(P)/(gam-1)
 // up to rounding.
-template <class R, int NV>
+// GEN instances (FP64) also serve hydro_interp_order 2 and 1, with one
+// uniform branch for the whole zone; the plain instances run para_recon only.
+template <class R, int NV, bool GEN = false>
 MDS_DEV void recon_cell(const Cell<R, NV> &m, const Cell<R, NV> &c, const Cell<R, NV> &p,
                         const typename ReconGeoT<R>::type &g, R igm1, Cell<R, NV> &al, Cell<R, NV> &ar)
 {
+  if constexpr (std::is_same<R, double>::value && GEN) {
+    if (g.order != 3) {
+      recon_x64(m.rho, c.rho, p.rho, g, al.rho, ar.rho);
+      for (int d = 0; d < NV; d++) recon_x64(m.v[d], c.v[d], p.v[d], g, al.v[d], ar.v[d]);
+      recon_x64(m.P, c.P, p.P, g, al.P, ar.P);
+      recon_x64(m.uu, c.uu, p.uu, g, al.uu, ar.uu);
+      return;
+    }
+  }
   recon<R>(m.rho, c.rho, p.rho, g, al.rho, ar.rho);
   for (int d = 0; d < NV; d++) recon<R>(m.v[d], c.v[d], p.v[d], g, al.v[d], ar.v[d]);
   recon<R>(m.P, c.P, p.P, g, al.P, ar.P);
@@ -192,7 +204,7 @@ MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, double area
 // x sweep of one row: per-lane flux through the lower x face, its shock tag
 // and |signal speed|.  Lane 63 additionally needs P of the zone above it so
 // that the tag of its own face is exact (pextra).
-template <class R, int NV>
+template <class R, int NV, bool SLAB = false>
 MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>::type &rgx, R gam, R igm1,
                     double area, R F[5], R &ssig, int &isshock)
 {
@@ -208,7 +220,7 @@ MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>
   } else {
     mm.uu = pp.uu = (R)0;
   }
-  recon_cell<R, NV>(mm, me, pp, rgx, igm1, al, ar);
+  recon_cell<R, NV, SLAB>(mm, me, pp, rgx, igm1, al, ar);
   Cell<R, NV> L;
   L.rho = lane_from_below(ar.rho);
   L.uu = lane_from_below(ar.uu);
@@ -218,14 +230,15 @@ MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>
 }
 
 // shock tags only (P reconstruction + the HLLC pressure-jump test, riemann.c:374-383)
-template <class R>
+template <class R, bool GEN = false>
 MDS_DEV int xshock_only(R P, R pextra, const typename ReconGeoT<R>::type &rgx)
 {
   const int lane = threadIdx.x & 63;
   R qm = lane_from_below(P), qp = lane_from_above(P);
   if (lane == 63) qp = pextra;
   R al, ar;
-  recon<R>(qm, P, qp, rgx, al, ar);
+  if constexpr (std::is_same<R, double>::value && GEN) recon_x64(qm, P, qp, rgx, al, ar);
+  else recon<R>(qm, P, qp, rgx, al, ar);
   R p_l = lane_from_below(ar), p_r = al;
   R p_max = p_l > p_r ? p_l : p_r, p_min = p_l > p_r ? p_r : p_l;
   if constexpr (std::is_same<R, double>::value) return ((p_max - p_min) / p_min > 0.5) ? 1 : 0;

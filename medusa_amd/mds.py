@@ -88,7 +88,8 @@ class RunParams(ctypes.Structure):
                 ("dtmin", ctypes.c_double),
                 ("dtmax", ctypes.c_double),
                 ("initial_dt", ctypes.c_double),
-                ("tmax", ctypes.c_double)]
+                ("tmax", ctypes.c_double),
+                ("hydro_interp_order", ctypes.c_int)]
 
 
 _LIB = None

@@ -76,6 +76,56 @@ def test_recon_exact_on_linear_data(geom):
         assert r == pytest.approx(c0 + c1 * xs[2], abs=1e-10)
 
 
+def lin(a3, beta, gam, xlo, xhi):
+    """lin_recon (reconstruct.c:89-124) of the middle zone"""
+    a = np.array([a3[0], a3[1], a3[2], 0.0])
+    be_ = np.array([beta[0], beta[1], beta[2], 1.0])
+    ga_ = np.array([gam[0], gam[1], gam[2], 1.0])
+    x = np.array([0.0, xlo, xhi, 0.0])
+    l = np.zeros(4)
+    r = np.zeros(4)
+    L.orc_lin_recon(po.ptr(a), po.ptr(l), po.ptr(r), po.ptr(be_), po.ptr(ga_), po.ptr(x), -1, -1)
+    return l[1], r[1]
+
+
+@pytest.mark.parametrize("geom", ["cart", "sph"])
+def test_lin_recon_exact_on_linear_data(geom):
+    """the centred slope of linear data is exact and never the largest of
+    the three, so the faces carry the line's values"""
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        h = rng.uniform(0.01, 0.1)
+        alpha, beta, gam, xs = cart_moments(rng.uniform(-1, 1), h) if geom == "cart" else sph_moments(
+            rng.uniform(-1, 1), h)
+        c0, c1 = rng.uniform(-1, 1), rng.uniform(-2, 2)
+        l, r = lin(c0 + c1 * beta / gam, beta, gam, xs[1], xs[2])
+        assert l == pytest.approx(c0 + c1 * xs[1], abs=1e-10)
+        assert r == pytest.approx(c0 + c1 * xs[2], abs=1e-10)
+
+
+def test_lin_recon_flat_at_extrema_and_bounded_on_monotone_data():
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        alpha, beta, gam, xs = sph_moments(rng.uniform(-1, 1), rng.uniform(0.01, 0.2))
+        q = np.sort(rng.uniform(-3, 3, 3)) * rng.choice([-1.0, 1.0])
+        l, r = lin(q, beta, gam, xs[1], xs[2])
+        # no new extrema: each face value lies between the zone and the neighbour on its side
+        assert min(q[0], q[1]) - 1e-12 <= l <= max(q[0], q[1]) + 1e-12
+        assert min(q[1], q[2]) - 1e-12 <= r <= max(q[1], q[2]) + 1e-12
+        peak = np.array([q[0], max(q) + 1.0, q[2]])
+        assert lin(peak, beta, gam, xs[1], xs[2]) == (peak[1], peak[1])
+
+
+def test_lin_recon_slope_saturates_at_the_one_sided_limit():
+    """uniform zones of width 1, means 0, 1, M: once M is large the slope is
+    the backward one-sided bound (the face midpoint at x_i - 1/4, 3/4 from the
+    centroid), so both faces stop depending on M: 1/3 and 5/3"""
+    alpha, beta, gam, xs = cart_moments(0.0, 1.0)
+    for M in (3.0, 10.0, 1.0e6):
+        l, r = lin([0.0, 1.0, M], beta, gam, xs[1], xs[2])
+        assert l == pytest.approx(1.0 / 3.0, rel=1e-14) and r == pytest.approx(5.0 / 3.0, rel=1e-14)
+
+
 @pytest.mark.parametrize("geom", ["cart", "sph"])
 def test_recon_exact_on_monotone_quadratic(geom):
     rng = np.random.default_rng(2)
