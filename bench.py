@@ -55,13 +55,13 @@ def workload(name, n):
         g, prob, prims = problems.kh(n)
         return dict(probs=[(prob, prims)], nvar=4, steps=200, zones=n * n, gam=[5.0 / 3.0],
                     label=f"kh2d {n}x{n} periodic, gamma=5/3, PPM+HLLC, SSP-RK2, CFL 0.5",
-                    kernel={"fp32": "sweep2d<float,2,false>", "fp64": "sweep2d<double,2,false>"}, n=n)
+                    kernel={"fp32": "sweep2d<float,2,false,false>", "fp64": "sweep2d<double,2,false,false>"}, n=n)
     if name == "rt3d":
         n = n or 512
         g, prob, prims = problems.rt(n)
         return dict(probs=[(prob, prims)], nvar=5, steps=100, zones=n ** 3, gam=[1.4],
                     label=f"rt3d {n}^3, g=-0.1 z, periodic x,y, reflecting z, gamma=1.4, CFL 0.5",
-                    kernel={"fp32": "sweep3d<float,3,true,TY>", "fp64": "sweep3d<double,3,true,TY>"}, n=n)
+                    kernel={"fp32": "sweep3d<float,3,true,TY,false>", "fp64": "sweep3d<double,3,true,TY,false>"}, n=n)
     if name == "sod1d":
         n = n or 1024
         g, prob, prims = problems.sod(n)
