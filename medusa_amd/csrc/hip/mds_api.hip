@@ -170,7 +170,7 @@ __global__ void init_dt_kernel(const DevProb *prob, DevClock *ck, const long lon
   double loc = 0.0;
   for (int d = 0; d < pb.ndim; d++) {
     double sc = 1.0;
-    if (pb.sph && pb.ndim == 3)
+    if (pb.sph && pb.ndim >= 2)
       sc = pb.zone[(size_t)(MDS_ZS_SCALE0 + d) * (pb.n[0] + 2 * MDS_NG) * (pb.n[1] + 2 * MDS_NG)
                    + (size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG)];
     else if (pb.sph && d == 0) sc = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_SCALE0];
@@ -207,7 +207,7 @@ __global__ void totals_kernel(const DevProb *prob, int p, long long nz, int nv, 
     if (pb.lj && !dend_valid(pb, (int)(q % pb.n[0]), (int)((q / pb.n[0]) % pb.n[1]),
                              (int)(q / ((long long)pb.n[0] * pb.n[1]))))
       continue;
-    if (pb.sph && pb.ndim == 3) {
+    if (pb.sph && pb.ndim >= 2) {
       const int i = (int)(q % pb.n[0]), j = (int)((q / pb.n[0]) % pb.n[1]);
       const size_t fs = (size_t)(pb.n[0] + 2 * MDS_NG) * (pb.n[1] + 2 * MDS_NG);
       const double *zs = pb.zone + (size_t)(j + MDS_NG) * (pb.n[0] + 2 * MDS_NG) + (i + MDS_NG);
@@ -393,7 +393,11 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
   if (c->ndim == 3) c->nv = 3;
-  if (c->sph && c->ndim == 2) return set_err(c, MDS_ENOTIMPL, "spherical geometry is 1D or 3D");
+  // 2D (r, theta) grids: the host builds their geometry, but the reference's
+  // 2D step differs from its 3D one on one phi zone (the CFL rules of
+  // prim_to_fluxes2), so the device does not run them yet
+  if (c->sph && c->ndim == 2)
+    return set_err(c, MDS_ENOTIMPL, "2D spherical grids are not run on the device (1D and 3D are)");
   c->sph3 = c->sph && c->ndim == 3;
   if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
@@ -429,7 +433,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     int nvp = auto_nv(P);
     if (c->ndim == 1 && nvp == 2) nvp = 3;
     if (c->ndim == 2 && nvp < 2) nvp = 2;
-    if (c->ndim == 3) nvp = 3;
+    if (c->ndim == 3 || c->sph3) nvp = 3;
     if (nvp != c->nv) return set_err(c, MDS_EINVAL, "problem %d: velocity component count differs in the batch", p);
     for (int d = 0; d < g->ndim; d++) {
       if (g->n[d] < 4) return set_err(c, MDS_EINVAL, "problem %d: need at least 4 zones per active direction", p);
@@ -437,9 +441,9 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
         int b = side ? P.bc_hi[d] : P.bc_lo[d];
         // SPHERICAL_ORIGIN: theta across the pole of 3D spherical grids, or
         // the inner radius through the origin of 1D and 3D spherical grids
-        const bool pole = c->sph3 && d == 1 && b == MDS_BC_SPHERICAL_ORIGIN && (g->n[2] % 2) == 0;
-        const bool origin = c->sph && d == 0 && side == 0 && b == MDS_BC_SPHERICAL_ORIGIN &&
-                            (!c->sph3 || (g->n[2] % 2) == 0);
+        const bool phi_ok = g->ndim == 2 || (g->n[2] % 2) == 0;     // the phi + pi map
+        const bool pole = c->sph3 && d == 1 && b == MDS_BC_SPHERICAL_ORIGIN && phi_ok;
+        const bool origin = c->sph && d == 0 && side == 0 && b == MDS_BC_SPHERICAL_ORIGIN && (!c->sph3 || phi_ok);
         // ghost layers held on the device: PROB anywhere but the inner radius
         // and the angles of 3D spherical grids; EXTRAP in direction 0 (the
         // only direction the reference extrapolates), not at that inner radius

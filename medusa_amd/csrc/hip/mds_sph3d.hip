@@ -1361,7 +1361,7 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
   MDS_FLUX_IRR(1)
   rec(4, s2);
   MDS_FLUX(1)
-  MDS_FLUX(2)
+  if (pb.ndim == 3) MDS_FLUX(2)     // 2D grids have no phi faces (their flux slots stay zero)
   wait(st, 4);
 #undef MDS_FLUX
 #undef MDS_FLUX_IRR

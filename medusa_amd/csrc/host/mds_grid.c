@@ -547,7 +547,21 @@ int mds_grid_build(mds_grid *g)
   if (g->geom == MDS_GEOM_SPHERICAL && g->rcoord == MDS_RCOORD_SINH && !(g->rtrans > 0.0)) return MDS_EINVAL;
   if (g->dendritic && !(g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3)) return MDS_EINVAL;
   int rc;
-  if (g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3) rc = build_spherical3d(g);
+  if (g->geom == MDS_GEOM_SPHERICAL && g->ndim == 2) {
+    /* 2D (r, theta): the 3D records of one phi zone spanning 2 pi, the
+     * width the reference gives the absent direction of a spherical grid
+     * (init_coords, dx[2] = 2 pi) */
+    g->startx[2] = 0.0;
+    g->dx[2] = 2.0 * M_PI;
+  }
+  if (g->geom == MDS_GEOM_SPHERICAL && g->ndim >= 2) {
+    rc = build_spherical3d(g);
+    if (!rc && g->ndim == 2) {
+      /* a 2D grid has no phi faces: their area is zero, as in the reference */
+      const size_t nf = (size_t)(g->n[0] + 2 * NG) * (g->n[1] + 2 * NG);
+      for (size_t q = 0; q < nf; q++) g->zone[q * MDS_ZS_COUNT + MDS_ZS_AREA0 + 2] = 0.0;
+    }
+  }
   else if (g->geom == MDS_GEOM_SPHERICAL) rc = build_spherical1d(g);
   else rc = build_cartesian(g);
   if (rc) mds_grid_free(g);

@@ -231,7 +231,7 @@ class GridBuilder:
         if not self.g.zone:
             return None
         nz = self.n[0] + 2 * NG
-        if self.ndim == 3:
+        if self.ndim >= 2 and self.g.geom == GEOM_SPHERICAL:
             nj = self.n[1] + 2 * NG
             return np.ctypeslib.as_array(self.g.zone, shape=(nj * nz * ZS_COUNT,)).reshape(nj, nz, ZS_COUNT).copy()
         return np.ctypeslib.as_array(self.g.zone, shape=(nz * ZG_COUNT,)).reshape(nz, ZG_COUNT).copy()
