@@ -393,12 +393,10 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
   if (c->ndim == 3) c->nv = 3;
-  // 2D (r, theta) grids: the host builds their geometry, but the reference's
-  // 2D step differs from its 3D one on one phi zone (the CFL rules of
-  // prim_to_fluxes2), so the device does not run them yet
-  if (c->sph && c->ndim == 2)
-    return set_err(c, MDS_ENOTIMPL, "2D spherical grids are not run on the device (1D and 3D are)");
-  c->sph3 = c->sph && c->ndim == 3;
+  // 2D (r, theta) grids run the 3D spherical kernels on one phi zone of 2 pi,
+  // without phi faces and without the phi term of the zone dt
+  c->sph3 = c->sph && c->ndim >= 2;
+  if (c->sph3) c->nv = 3;
   if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
   if (c->sph && !c->sph3 && c->precision == MDS_FP32_BULK && c->nv != 1)

@@ -1261,7 +1261,9 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
     // estimate_dt zone bound (step.c:177-187) with the stage's sound speed
     double cs = TAB ? cs_tab : sqrt((gam * press) / q[0]);
     double loc = 0.0;
+#pragma unroll
     for (int d = 0; d < 3; d++) {
+      if (d >= pb.ndim) break;                 // NDIM directions (2D grids have no phi term)
       double ldx = pb.dx[d];
       if (DEND && d == 1) ldx *= (double)(1 << li);
       if (DEND && d == 2) ldx *= (double)(1 << lk0);
