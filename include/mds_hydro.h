@@ -90,7 +90,9 @@ enum {
   MDS_ZG_COUNT
 };
 
-/* Geometry record of a 3D spherical (r, theta, phi) grid, one per (i, j)
+/* Geometry record of a 3D spherical (r, theta, phi) grid, one per (i, j);
+ * 2D (r, theta) grids use it as a 3D grid of one phi zone spanning 2 pi
+ * with zero phi-face areas
  * (the metric does not depend on phi): (n1 + 2*NG) * (n2 + 2*NG) records,
  * i fastest, zone (-NG, -NG) first; restating zone_geom (decs.h:144-153) of
  * metric_spherical_general.c:708-1051 on a uniform angular grid. */
