@@ -161,6 +161,13 @@ void orc_monopole_edges(const orc_sim *s, double *grav);
 int orc_dendritic_levels(const orc_sim *s, int *nj, int *nk);
 /* theta (dir 0) / phi (dir 1) moments of refinement level lev (length n + 2 NG), NULL if none */
 const double *orc_level_moment(const orc_sim *s, int which, int dir, int lev);
+/* tests only: pencil entries of the dendritic sweeps and the polar phi-face
+ * states on the current state (step.c:783-1085, 1299-1397); call
+ * orc_prepare_entries first (ghost fill + EOS update) */
+void orc_prepare_entries(orc_sim *s);
+int orc_dend_radial_entry(orc_sim *s, int i, int j, int k, int ii, double *out);
+int orc_dend_theta_entry(orc_sim *s, int i, int j, int kk, int jj, double *out);
+int orc_polar_face_states(orc_sim *s, int i, int j, int k, double *left, double *right);
 
 #ifdef __cplusplus
 }

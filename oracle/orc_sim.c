@@ -1568,3 +1568,75 @@ const double *orc_level_moment(const orc_sim *s, int which, int dir, int lev)
   if (lev < 0 || lev > maxl || !s->lmom[which][dir]) return 0;
   return lev_arr(s, which, dir, lev) - ORC_NG;
 }
+
+/* ------------------------------------------------------------------ */
+/* Entry points for pinning the pencil entries and the polar phi-face
+ * states (tests only; the stepping code never calls them).  They read the
+ * current state after orc_prepare_entries has applied the stage-start
+ * ghost fill and EOS update (step.c:76-81). */
+
+void orc_prepare_entries(orc_sim *s)
+{
+  orc_fill_ghosts(s);
+  update_eos(s, s->p);
+}
+
+/* Entry of row ii of the radial pencil of the face below zone (i,j,k) of a
+ * 3D spherical grid, as sweep_dend chooses it (step.c:1159-1175): restrict0
+ * when row ii is finer in theta than row i, prolongate0 otherwise.  out:
+ * nvars primitives, then P and Gamma_1.  Returns -1 outside the grid. */
+int orc_dend_radial_entry(orc_sim *s, int i, int j, int k, int ii, double *out)
+{
+  const orc_config *c = &s->c;
+  if (c->ndim != 3 || c->geom != ORC_GEOM_SPHERICAL) return -1;
+  if (i < 0 || i > c->n[0] || ii < i - ORC_NG || ii > i + ORC_NG - 1) return -1;
+  if (j < 0 || j >= njs(s, i) || k < 0 || k >= nks(s, i, j)) return -1;
+  if (djs(s, ii) < djs(s, i)) restrict0(s, s->p, i, j, k, ii, out);
+  else prolongate0(s, s->p, i, j, k, ii, out);
+  return 0;
+}
+
+/* Entry of row jj of the theta pencil of the face below row j of shell i,
+ * at phi index kk of the face's (finer) phi level (theta_row: prolongate1,
+ * restrict1 or the zone itself, step.c:1201-1240). */
+int orc_dend_theta_entry(orc_sim *s, int i, int j, int kk, int jj, double *out)
+{
+  const orc_config *c = &s->c;
+  if (c->ndim != 3 || c->geom != ORC_GEOM_SPHERICAL) return -1;
+  if (i < 0 || i >= c->n[0] || j < 0 || j > njs(s, i) || jj < j - ORC_NG || jj > j + ORC_NG - 1) return -1;
+  const int dkf = OMIN(dks(s, i, j), dks(s, i, j - 1));
+  if (kk < 0 || kk >= c->n[2] / dkf) return -1;
+  theta_row(s, s->p, i, j, kk, jj, out);
+  return 0;
+}
+
+/* The two states of the phi face below zone (i, j, k) in the last theta row
+ * (step.c:1299-1397), after reconstruction and the polar-axis treatment:
+ * left = the upper state of zone k-1, right = the lower state of zone k,
+ * nvars primitives then P and Gamma_1 each. */
+int orc_polar_face_states(orc_sim *s, int i, int j, int k, double *left, double *right)
+{
+  const orc_config *c = &s->c;
+  if (c->ndim != 3 || c->geom != ORC_GEOM_SPHERICAL) return -1;
+  if (i < 0 || i >= c->n[0] || j != njs(s, i) - 1 || k < 0 || k > nks(s, i, j)) return -1;
+  const int nv = s->nvars, dk = dks(s, i, j), lk = lev_of(dk);
+  const double *A = lev_arr(s, 0, 1, lk), *B = lev_arr(s, 1, 1, lk), *G = lev_arr(s, 2, 1, lk);
+  double pen[ORC_NSIZE][2 * ORC_NG], pleft[ORC_NSIZE][2 * ORC_NG], pright[ORC_NSIZE][2 * ORC_NG];
+  double al[2 * ORC_NG], be[2 * ORC_NG], ga[2 * ORC_NG], x[2 * ORC_NG], pp[ORC_NSIZE];
+  for (int q = 0; q < 2 * ORC_NG; q++) {
+    const int kt = k - ORC_NG + q;
+    x[q] = c->startx[2] + kt * dk * c->dx[2];
+    al[q] = A[kt];
+    be[q] = B[kt];
+    ga[q] = G[kt];
+    zone_row(s, s->p, i, j, kt, pp);
+    for (int vv = 0; vv < nv + 2; vv++) pen[vv][q] = pp[vv];
+  }
+  interp_pencil(s, pen, pleft, pright, al, be, ga, x);
+  polar_states(s, s->p, i, j, k, pen, pleft, pright);
+  for (int vv = 0; vv < nv + 2; vv++) {
+    left[vv] = pright[vv][ORC_NG - 1];
+    right[vv] = pleft[vv][ORC_NG];
+  }
+  return 0;
+}

@@ -122,6 +122,13 @@ def _load(fast=False):
     lib.orc_eos_root.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_int)]
     lib.orc_eos_root.restype = ctypes.c_int
+    lib.orc_prepare_entries.argtypes = [ctypes.c_void_p]
+    lib.orc_dend_radial_entry.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    lib.orc_dend_radial_entry.restype = ctypes.c_int
+    lib.orc_dend_theta_entry.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    lib.orc_dend_theta_entry.restype = ctypes.c_int
+    lib.orc_polar_face_states.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+    lib.orc_polar_face_states.restype = ctypes.c_int
     return lib
 
 
@@ -317,6 +324,32 @@ class Sim:
                 return None
             out.append(np.ctypeslib.as_array(p, shape=(ng,)).copy())
         return out
+
+    def prepare_entries(self):
+        """Stage-start ghost fill and EOS update on the current state (before the entry evaluators)."""
+        self.lib.orc_prepare_entries(self.h)
+
+    def radial_entry(self, i, j, k, ii):
+        """Row ii of the radial pencil of the face below zone (i, j, k) (restrict0 / prolongate0):
+        nvars primitives, then P and Gamma_1."""
+        out = np.zeros(self.nvars + 2)
+        if self.lib.orc_dend_radial_entry(self.h, i, j, k, ii, ptr(out)) != 0:
+            raise IndexError((i, j, k, ii))
+        return out
+
+    def theta_entry(self, i, j, kk, jj):
+        """Row jj of the theta pencil of the face below row j of shell i at phi index kk of the face level."""
+        out = np.zeros(self.nvars + 2)
+        if self.lib.orc_dend_theta_entry(self.h, i, j, kk, jj, ptr(out)) != 0:
+            raise IndexError((i, j, kk, jj))
+        return out
+
+    def polar_face_states(self, i, j, k):
+        """(left, right) states of the phi face below zone (i, j, k) of the last theta row."""
+        left, right = np.zeros(self.nvars + 2), np.zeros(self.nvars + 2)
+        if self.lib.orc_polar_face_states(self.h, i, j, k, ptr(left), ptr(right)) != 0:
+            raise IndexError((i, j, k))
+        return left, right
 
     def moments(self, d):
         ng = self.cfg.n[d] + 4 if d < self.cfg.ndim else 5
