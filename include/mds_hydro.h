@@ -58,6 +58,7 @@ extern "C" {
 #define MDS_ENOMEM     -2
 #define MDS_EDEVICE    -3
 #define MDS_ENOTIMPL   -4
+#define MDS_EIO        -5
 
 /* Per-zone geometry record of a 1D spherical column (one record per zone,
  * n1 + 2*MDS_NG records, zone -NG first), restating the fields of zone_geom
@@ -301,6 +302,29 @@ int mds_get_clock(mds_ctx *ctx, int prob, double *t, double *dt, int *istep);
 /* Compensated (Neumaier) FP64 totals of rho*V and E*V over interior zones
  * (reference calculate_total_mass/energy, utils.c:2337-2366, 2581-2613). */
 int mds_conserved_totals(mds_ctx *ctx, int prob, double out[2]);
+
+/* Device health of one problem, read with its clock.  The reference has no
+ * error channel on this path: it prints EOS off-table warnings and goes on
+ * (eos/burrows/eos_stuff.c:571-604) and exits when dt < dtmin (main.c:229-238).
+ * Here the stage kernels count, since mds_update_eos:
+ *   bad_state      zone updates (one per zone and RK stage) whose new state
+ *                  has a non-finite or non-positive density or a non-finite
+ *                  internal energy -- or, with the gamma law, a negative one
+ *                  (negative pressure).  A NaN state makes no CFL limit (its
+ *                  key loses every minimum), so this is where it shows;
+ *   eos_off_table  table EOS evaluations of the stage's zones with density,
+ *                  Ye or temperature outside the table (clamped to its edge
+ *                  in density and Ye, extrapolated in temperature);
+ *   dt_below_min   1 once dt fell below dtmin (the run froze there). */
+typedef struct {
+  double t, dt;
+  int istep;
+  int dt_below_min;
+  long long bad_state;
+  long long eos_off_table;
+} mds_health;
+
+int mds_get_health(mds_ctx *ctx, int prob, mds_health *out);
 
 /* Diagnostic counters since mds_update_eos: [0] radial faces tagged as shocks
  * by HLLC, [1] transverse faces solved with HLLE. */

@@ -70,6 +70,7 @@ struct mds_ctx {
   int chunk2d = 24;
   int ty3d = 4;
   int ncu = 0;                          // compute units of the device
+  int dev = 0;                          // the device the context was created on
   int grid1d = 0;                       // 1D grid: 0 one wavefront per segment, 1 persistent
   int halo3d = 0;                       // 3D: 1 = the output wavefronts also load the halo rows
   int kc3d = 16;
@@ -420,6 +421,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     int dev = 0;
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    c->dev = dev;
   }
   c->zstart.assign(nprob + 1, 0);
   c->hprob.resize(nprob);
@@ -494,7 +496,28 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       d.area[k] = g->area[k];
     }
     d.volume = g->volume;
+    d.ivolume = g->volume != 0.0 ? 1.0 / g->volume : 0.0;
+    for (int k = 0; k < 3; k++) d.idx[k] = 1.0 / d.dx[k];
     d.order = rp->hydro_interp_order;
+    // one FP32 coefficient set for the whole column when the zones' sets
+    // agree to 1e-5 (the moments of log-r and uniform grids differ only by
+    // scale and shift; the spread left is the rounding of the FP64 moments,
+    // which grows with n): the per-coefficient median of the reconstructed
+    // zones -1 .. n
+    {
+      const int nzk = g->n[0] + 2 * MDS_NG;
+      bool uni = true;
+      std::vector<double> col(nzk - 2);
+      for (int k = 0; k < MDS_RC_COUNT && uni; k++) {
+        for (int q = 1; q < nzk - 1; q++) col[q - 1] = g->rcoef[0][(size_t)q * MDS_RC_COUNT + k];
+        std::vector<double> srt(col);
+        std::nth_element(srt.begin(), srt.begin() + srt.size() / 2, srt.end());
+        const double med = srt[srt.size() / 2];
+        for (double v : col) uni = uni && fabs(v - med) <= 1.0e-5 * fabs(med) + 1.0e-12;
+        d.rcu_c[k] = (float)med;
+      }
+      d.rcu = uni ? 1 : 0;
+    }
     // ghost layers of the PROB and EXTRAP sides
     d.has_slab = 0;
     {
@@ -589,16 +612,13 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
           const double *zg = g->zone + (size_t)q * MDS_ZG_COUNT;
           ZoneF32 &z = z32[q];
           memset(&z, 0, sizeof(z));
-          z.vol = zg[MDS_ZG_VOLUME];
+          z.ivol = zg[MDS_ZG_VOLUME] != 0.0 ? 1.0 / zg[MDS_ZG_VOLUME] : 0.0;
           z.area = zg[MDS_ZG_AREA0];
           z.gcov0 = zg[MDS_ZG_GCOV00];
-          z.gcon0 = zg[MDS_ZG_GCON0];
           z.cp = (zg[MDS_ZG_C000] + zg[MDS_ZG_C101]) + zg[MDS_ZG_C202];
           z.c000 = (float)zg[MDS_ZG_C000];
-          z.isc0 = (float)(1.0 / zg[MDS_ZG_SCALE0]);
           z.scalef = (float)zg[MDS_ZG_SCALEF];
           z.gcovf0 = (float)zg[MDS_ZG_GCOVF0];
-          for (int k = 0; k < MDS_RC_COUNT; k++) z.rc[k] = (float)g->rcoef[0][(size_t)q * MDS_RC_COUNT + k];
           const double xlo = g->startx[0] + (q - MDS_NG) * g->dx[0];
           z.dcen = (float)(g->beta[0][q] / g->Gamma[0][q] - xlo);
         }
@@ -1341,6 +1361,23 @@ extern "C" int mds_get_clock(mds_ctx *c, int prob, double *t, double *dt, int *i
   return MDS_OK;
 }
 
+extern "C" int mds_get_health(mds_ctx *c, int prob, mds_health *out)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !out) return MDS_EINVAL;
+  DevClock ck;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(&ck, c->dclock + prob, sizeof(ck), hipMemcpyDeviceToHost));
+  out->t = ck.t;
+  out->dt = ck.dt;
+  out->istep = ck.istep;
+  out->dt_below_min = ck.status;
+  out->bad_state = (long long)ck.nbad;
+  out->eos_off_table = (long long)ck.noff;
+  // the running CFL minimum of the step being taken: a NaN limit never wins
+  // the bitwise minimum, so a NaN state shows here only through bad_state
+  return MDS_OK;
+}
+
 extern "C" int mds_counters(mds_ctx *c, long long out[2])
 {
   if (!c || !out) return MDS_EINVAL;
@@ -1352,24 +1389,34 @@ extern "C" int mds_counters(mds_ctx *c, long long out[2])
   return MDS_OK;
 }
 
-extern "C" int mds_conserved_totals(mds_ctx *c, int prob, double out[2])
+// Block partials of the totals of problem prob into dpart (device, 4 doubles
+// per block) and back into h (host), on the context stream, synchronised.
+hipError_t mds_totals_partials(mds_ctx *c, int prob, int nblk, double *dpart, double *h)
 {
-  if (!c || prob < 0 || prob >= c->nprob || !out) return MDS_EINVAL;
-  long long nz = c->zstart[prob + 1] - c->zstart[prob];
-  int nblk = 256;
-  double *dpart = nullptr;
-  HIPCHK(c, hipMalloc(&dpart, sizeof(double) * 4 * nblk));
+  const long long nz = c->zstart[prob + 1] - c->zstart[prob];
   if (c->precision == MDS_FP32_BULK)
     hipLaunchKernelGGL(totals_kernel<float>, dim3(nblk), dim3(256), 0, c->st, c->dprob, prob, nz, c->nv,
                        arrays<float>(c->p), dpart);
   else
     hipLaunchKernelGGL(totals_kernel<double>, dim3(nblk), dim3(256), 0, c->st, c->dprob, prob, nz, c->nv,
                        arrays<double>(c->p), dpart);
-  std::vector<double> h(4 * nblk);
-  hipError_t e = hipMemcpyAsync(h.data(), dpart, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, c->st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, dpart, sizeof(double) * 4 * nblk, hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  hipFree(dpart);
-  if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "totals: %s", hipGetErrorString(e));
+  return e;
+}
+
+// accessors for the other translation units of the library
+int mds_ctx_device(const mds_ctx *c) { return c->dev; }
+int mds_ctx_check_prob(mds_ctx *c, int prob)
+{
+  return (prob < 0 || prob >= c->nprob) ? set_err(c, MDS_EINVAL, "problem %d out of range", prob) : MDS_OK;
+}
+int mds_ctx_error(mds_ctx *c, int code, const char *msg) { return set_err(c, code, "%s", msg); }
+
+// Neumaier sum of the block partials, in block order
+void mds_totals_finish(const double *h, int nblk, double out[2])
+{
   for (int k = 0; k < 2; k++) {
     double S = 0.0, C = 0.0;
     for (int b = 0; b < nblk; b++)
@@ -1381,6 +1428,20 @@ extern "C" int mds_conserved_totals(mds_ctx *c, int prob, double out[2])
       }
     out[k] = S + C;
   }
+}
+
+extern "C" int mds_conserved_totals(mds_ctx *c, int prob, double out[2])
+{
+  if (!c || prob < 0 || prob >= c->nprob || !out) return MDS_EINVAL;
+  long long nz = c->zstart[prob + 1] - c->zstart[prob];
+  int nblk = 256;
+  double *dpart = nullptr;
+  HIPCHK(c, hipMalloc(&dpart, sizeof(double) * 4 * nblk));
+  std::vector<double> h(4 * nblk);
+  hipError_t e = mds_totals_partials(c, prob, nblk, dpart, h.data());
+  hipFree(dpart);
+  if (e != hipSuccess) return set_err(c, MDS_EDEVICE, "totals: %s", hipGetErrorString(e));
+  mds_totals_finish(h.data(), nblk, out);
   return MDS_OK;
 }
 

@@ -131,6 +131,17 @@ MDS_DEV double eos_tri_x64(const float *rec, const EosW &w, int jt, double dtemp
   return p;
 }
 
+// The state lies outside the table: density or Ye beyond its nodes (the
+// weights clamp them to the edge) or a temperature beyond the T nodes (the
+// linear inverse extrapolates).  NaN counts as outside.
+MDS_DEV bool eos_off_table(const DevEos &t, double rho, double ye, double lt)
+{
+  return !((rho >= t.rmin) && (rho <= t.rmax) && (ye >= t.ye0) && (ye <= t.ye1) && (lt >= t.lt0) && (lt <= t.lt1));
+}
+
+// specific internal energy in table units (eos_stuff.c:530): e / e_scale - e_shift
+MDS_DEV double eos_e_table(const DevEos &t, double rho, double u) { return (u / rho) / t.escale - t.eshift; }
+
 // FP64 EOS fill of one zone: (rho, u, Ye) -> P, Gamma_1, cs, log10 T.  With
 // guess non-null the bracket search starts from *guess and returns it there.
 MDS_DEV void eos_fill_x64(const DevEos &t, double rho, double u, double ye, double &press, double &gam1,
@@ -140,12 +151,12 @@ MDS_DEV void eos_fill_x64(const DevEos &t, double rho, double u, double ye, doub
   int jt;
   double dtemp;
   if (guess) {
-    eos_invert_guess(t, w, u / rho, *guess, dtemp, lt);
+    eos_invert_guess(t, w, eos_e_table(t, rho, u), *guess, dtemp, lt);
     jt = *guess;
   } else {
-    eos_invert(t, w, u / rho, jt, dtemp, lt);
+    eos_invert(t, w, eos_e_table(t, rho, u), jt, dtemp, lt);
   }
-  press = exp(eos_tri_x64(t.lnp, w, jt, dtemp));
+  press = exp(eos_tri_x64(t.lnp, w, jt, dtemp)) * t.pscale;       // eos_stuff.c:618
   gam1 = eos_tri_x64(t.g1, w, jt, dtemp);
   cs = sqrt((gam1 * press) / rho);
 }
@@ -177,16 +188,16 @@ MDS_DEV void eos_fill_f32(const DevEos &t, double rho, double u, double ye, doub
   int jt;
   double dtemp;
   if (guess) {
-    eos_invert_guess(t, w, u / rho, *guess, dtemp, lt);
+    eos_invert_guess(t, w, eos_e_table(t, rho, u), *guess, dtemp, lt);
     jt = *guess;
   } else {
-    eos_invert(t, w, u / rho, jt, dtemp, lt);
+    eos_invert(t, w, eos_e_table(t, rho, u), jt, dtemp, lt);
   }
   float b0, b1;
   const float dl = eos_tri_f32(t.lnp, w, jt, dtemp, b0);
-  press = exp((double)b0 + (double)dl);
+  press = exp((double)b0 + (double)dl) * t.pscale;
   gam1 = b1 = 0.0f;
   const float dg = eos_tri_f32(t.g1, w, jt, dtemp, b1);
   gam1 = b1 + dg;
-  cs = __builtin_sqrtf(gam1 * (float)(press / rho));
+  cs = __builtin_amdgcn_sqrtf(gam1 * (float)(press / rho));
 }

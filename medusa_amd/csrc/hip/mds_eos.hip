@@ -24,6 +24,11 @@ hipError_t mds_eos_to_device(const mds_eos_table *t, DevEos *d, void **alloc)
   d->dlr = (t->lr1 - t->lr0) / (t->nr - 1);
   d->dlt = (t->lt1 - t->lt0) / (t->nt - 1);
   d->dye = (t->ye1 - t->ye0) / (t->ny - 1);
+  d->escale = t->e_scale > 0.0 ? t->e_scale : 1.0;
+  d->eshift = t->e_shift;
+  d->pscale = t->p_scale > 0.0 ? t->p_scale : 1.0;
+  d->rmin = pow(10.0, t->lr0);
+  d->rmax = pow(10.0, t->lr1);
   d->e = (const float *)p + MDS_EOS_E * rec;
   d->lnp = (const float *)p + MDS_EOS_LNP * rec;
   d->g1 = (const float *)p + MDS_EOS_GAM1 * rec;

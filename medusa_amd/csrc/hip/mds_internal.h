@@ -5,29 +5,30 @@
 #include "../../../include/mds_hydro.h"
 
 // Compact per-zone geometry of a spherical column for the FP32-bulk path:
-// the FP64 factors the update needs (volume, lower-face area, zone metric,
-// the combined pressure coefficient of the connection terms) and the FP32
-// factors the Riemann solve and the CFL bound need, plus the reconstruction
-// coefficients.  96 bytes, read with six 16-byte loads.
+// the FP64 factors the update needs (inverse volume, lower-face area, zone
+// metric, the combined pressure coefficient of the connection terms) and the
+// FP32 factors of the Riemann solve and the gravity source.  gcon[0] =
+// 1 / gcov0 and scale[0][0] = sqrt(gcov0) are recomputed where needed.  The
+// reconstruction coefficients live beside it (DevProb::rc, or one set per
+// problem, DevProb::rcu).  48 bytes, read with three 16-byte loads.
 struct __align__(16) ZoneF32 {
-  double vol;              // zone_geom.volume
+  double ivol;             // 1 / zone_geom.volume
   double area;             // area[0] of the lower radial face
-  double gcov0, gcon0;     // zone-averaged gcov[0][0], gcon[0]
+  double gcov0;            // zone-averaged gcov[0][0]
   double cp;               // conn[0][0][0] + conn[1][0][1] + conn[2][0][2]
   float c000;              // conn[0][0][0]
-  float isc0;              // 1 / scale[0][0]
   float scalef, gcovf0;    // lower face: scale[1][0], gcov[1][0]
-  float rc[MDS_RC_COUNT];  // FP32 reconstruction coefficients
   float dcen;              // volume centroid beta/Gamma minus the lower face x
-  float pad;
 };
-static_assert(sizeof(ZoneF32) == 96, "ZoneF32 layout");
+static_assert(sizeof(ZoneF32) == 48, "ZoneF32 layout");
 
 // Tabulated EOS as the kernels see it (records of mds_eos_table in HBM).
 struct DevEos {
   int nr, nt, ny;
   double lr0, lr1, lt0, lt1, ye0, ye1;
   double dlr, dlt, dye;          // node spacings, (last - first) / (n - 1)
+  double rmin, rmax;             // 10^lr0, 10^lr1 (the table's density range)
+  double escale, eshift, pscale; // table units: e / escale - eshift, exp(ln P) * pscale
   const float *e, *lnp, *g1;     // record bases (MDS_EOS_E, _LNP, _GAM1)
 };
 
@@ -43,7 +44,14 @@ struct DevProb {
   double gam;
   double g0[3];            // fixed gravity (contravariant)
   double startx[3], dx[3];
+  double idx[3];           // 1 / dx
   double volume, area[3];  // Cartesian uniform geometry
+  double ivolume;          // 1 / volume
+  // direction-0 FP32 reconstruction coefficients shared by every zone (1:
+  // rcu_c holds them) -- the moments of a log-r column or of a uniform
+  // Cartesian grid are the same up to scale and shift in every zone
+  int rcu;
+  float rcu_c[MDS_RC_COUNT];
   const double *mom[3];    // per direction (n[d]+2NG)*3: alpha, beta, Gamma per zone
   const float *rc[3];      // per direction (n[d]+2NG)*MDS_RC_COUNT
   const double *zone;      // spherical columns: (n[0]+2NG) x MDS_ZG_COUNT records; 3D spherical:
@@ -99,6 +107,13 @@ struct DevClock {
   int istep, active;
   int status;                 // 1: dt fell below dtmin (main.c:229-238)
   int pad;
+  // health counters since the clock was set (mds_update_eos): zone updates
+  // that produced a non-finite or non-positive density, or a non-finite
+  // internal energy (negative too with the gamma law, where it is the
+  // pressure over gamma - 1); table EOS evaluations of a stage's own zones
+  // outside the table in density, Ye or temperature (the reference prints
+  // these and continues, eos_stuff.c:571-604)
+  unsigned long long nbad, noff;
 };
 
 struct RunParams {

@@ -374,7 +374,12 @@ struct ReconGeo32 {
 };
 
 MDS_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-MDS_DEV float fsqrt(float x) { return __builtin_sqrtf(x); }
+// hardware square root and reciprocal square root (v_sqrt_f32, v_rsq_f32,
+// about 1 ulp): the FP32-bulk path needs no correctly rounded result, and the
+// IEEE sequence costs some 16 VALU instructions per call.  Their arguments
+// must stay in the FP32 normal range: the hardware flushes denormal inputs.
+MDS_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+MDS_DEV float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // para_recon in local coordinates: identical branch structure, well-conditioned.
 MDS_DEV void para_recon_f32(float qm, float q0, float qp, const ReconGeo32 &g, float &al, float &ar)
@@ -427,8 +432,12 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
     float a_r = 2.0f * frcp((G_r + 1.0f) * rr);
     float b_l = (G_l - 1.0f) * frcp(G_l + 1.0f) * p_l;
     float b_r = (G_r - 1.0f) * frcp(G_r + 1.0f) * p_r;
-    float g_l = fsqrt(a_l * frcp(p_s + b_l));
-    float g_r = fsqrt(a_r * frcp(p_s + b_r));
+    // sqrt(a / (p + b)) as sqrt(a) / sqrt(p + b): the quotient itself falls
+    // below the FP32 normal range at stellar-core densities and pressures
+    // (2 / (2.3 rho) / (p + b) near 1e-38 at rho 1e10, p 1e28), where the
+    // hardware square root flushes it to zero
+    float g_l = fsqrt(a_l) * frsqrt(p_s + b_l);
+    float g_r = fsqrt(a_r) * frsqrt(p_s + b_r);
     p_s = (g_l * p_l + g_r * p_r - (u_r - u_l)) * frcp(g_l + g_r);
     p_s = fmaxf(p_s, 0.0f);
   }
@@ -561,6 +570,17 @@ MDS_DEV void fold_min_key(unsigned long long *p, unsigned long long seen, double
   if (k < seen) atomicMin(p, k);
 }
 
+// Device health: lanes of the wavefront whose new state is unphysical --
+// a non-finite or non-positive density, a non-finite internal energy, and
+// with the gamma law a negative one (negative pressure).  Call with every
+// lane of the wavefront active; returns the wave-uniform count.
+MDS_DEV unsigned bad_state_lanes(bool valid, double rho, double uu, bool gamma_law)
+{
+  const double inf = __builtin_huge_val();
+  const bool ok = (rho > 0.0) && (rho < inf) && (uu > -inf) && (uu < inf) && (!gamma_law || uu >= 0.0);
+  return (unsigned)__popcll(__ballot(valid && !ok));
+}
+
 // Cross-lane neighbours with whole-wave DPP shifts (GFX9 wave_shr:1 /
 // wave_shl:1): lane l reads lane l-1 (from_below) or l+1 (from_above); the
 // edge lane keeps its own value.  Must be called with every lane active.
@@ -611,14 +631,17 @@ MDS_DEV double wave_min(double v)
   return v;
 }
 
+// The FP32-bulk source limit: hardware reciprocal estimates (v_rcp_f64, no
+// Newton steps).  The limit bounds dt; a relative error near 1e-8 in it is
+// far below what FP32 state storage already changes.
 template <int NV>
 MDS_DEV double dt_source_control_fast(double uu, const double v[3], const double u[5], const double s[5])
 {
-  double mdt = (0.02 * u[0]) * drcp(fabs(s[0]) + 1.0e-16);
-  double t = (0.02 * u[1]) * drcp(fabs(s[1]) + 1.0e-16);
+  double mdt = (0.02 * u[0]) * __builtin_amdgcn_rcp(fabs(s[0]) + 1.0e-16);
+  double t = (0.02 * u[1]) * __builtin_amdgcn_rcp(fabs(s[1]) + 1.0e-16);
   mdt = fmin(mdt, t);
   double momterm = 0.0;
   for (int d = 0; d < NV; d++) momterm += v[d] * s[2 + d];
-  t = (2.0 * uu) * drcp(fabs(fabs(s[1]) - fabs(momterm)) + 1.0e-16);
+  t = (2.0 * uu) * __builtin_amdgcn_rcp(fabs(fabs(s[1]) - fabs(momterm)) + 1.0e-16);
   return fmin(mdt, t);
 }

@@ -657,6 +657,10 @@ __global__ __launch_bounds__(256) void sph3_eos_kernel(Sph3Args a, StateArrays<R
     ((R *)a.CS)[z] = (R)cs;
   }
   a.tg[z] = (unsigned short)jt;
+  // table lookups off the table: one atomic per wavefront from its first active lane
+  const unsigned long long m = __ballot(eos_off_table(a.eos, (double)in.v[0][z], (double)in.v[5][z], lt));
+  if (m && (threadIdx.x & 63) == (unsigned)__builtin_ctzll(__ballot(true)))
+    atomicAdd(&a.clock->noff, (unsigned long long)__popcll(m));
 }
 
 // ---------------------------------------------------------------------------
@@ -1280,10 +1284,12 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
   }
   const double wl = wave_min((live && active) ? lim_src : 1.0e300);
   const double wm = wave_max((live && active) ? locmax : 0.0);
+  const unsigned nbad = bad_state_lanes(live && active, ro, uo, !TAB);
   if ((threadIdx.x & 63) == 0) {
     double l = wl;
     if (wm > 0.0) l = fmin(l, 1.0 / wm);
     if (l < 1.0e300) fold_min_key(&ck->mindt, seen, l);
+    if (nbad) atomicAdd(&ck->nbad, (unsigned long long)nbad);
   }
 }
 

@@ -87,6 +87,8 @@ struct Lim1D {
   double srcnum = 1.0e300;  // Cartesian without sources: min numerator over 1e-16
   double lim_src = 1.0e300; // min dt_source_control
   unsigned nsh = 0;         // shock-tagged owned faces (wave-uniform)
+  unsigned nbad = 0;        // owned zone updates with an unphysical state (wave-uniform)
+  unsigned noff = 0;        // table EOS evaluations of owned zones off the table (wave-uniform)
 };
 
 // Lane roles of one stage.
@@ -94,6 +96,7 @@ struct Roles {
   bool face;    // the lower face is valid (its CFL limit counts)
   bool count;   // the lower face is owned by this wave (shock counter)
   bool out;     // the zone update is valid
+  bool own;     // the zone is owned by this wave (health counts)
 };
 
 // Fold the wave's limits into the problem clock (lane 0 of the wave).
@@ -110,6 +113,8 @@ MDS_DEV void commit_limits(const DevProb &pb, DevClock *ck, unsigned long long *
     l = fmin(l, wlim);
     if (l < 1.0e300) atomicMin(&ck->mindt, dt_key(l));
     if (lim.nsh) atomicAdd(&counters[0], (unsigned long long)lim.nsh);
+    if (lim.nbad) atomicAdd(&ck->nbad, (unsigned long long)lim.nbad);
+    if (lim.noff) atomicAdd(&ck->noff, (unsigned long long)lim.noff);
   }
 }
 
@@ -241,6 +246,7 @@ MDS_DEV Prim<double> stage_x64(const DevProb &pb, const DevEos &eos, const Geo64
   if (TAB) {
     double lt;
     eos_fill_x64(eos, q.rho, q.uu, q.ye, press, gam1, cs, lt);
+    lim.noff += (unsigned)__popcll(__ballot(r.own && eos_off_table(eos, q.rho, q.ye, lt)));
   } else {
     press = (gam - 1.0) * q.uu;
     gam1 = gam;
@@ -395,7 +401,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
                                                    StateArrays<double> sph, int stage)
 {
   const int lane = threadIdx.x & 63;
-  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
   // each wavefront walks the segment table (longest problems first) with the
   // grid's stride: one segment per wavefront, or, in a persistent grid, a
@@ -421,6 +427,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
     r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
     r.count = r.face && (lane <= 61 || c == n);
     r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
+    r.own = r.out;
     // U^n of this zone: the stage input itself in stage 0, sim_p in stage 1
     Prim<double> qn = q;
     if (stage == 1 && r.out) qn = load_prims<NV, TAB>(sp, off, c, 0);
@@ -430,6 +437,7 @@ __global__ __launch_bounds__(256) void sweep1d_x64(Sweep1DArgs a, StateArrays<do
 
     Lim1D lim;
     const Prim<double> o = stage_x64<NV, SPH, TAB, MONO, GEN>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
+    lim.nbad += bad_state_lanes(r.out, o.rho, o.uu, !TAB);
     if (r.out) {
       StateArrays<double> &out = (stage == 0) ? sph : sp;
       store_prims<NV, TAB>(out, off, c, o);
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
                                                     StateArrays<double> out)
 {
   const int lane = threadIdx.x & 63;
-  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
   // each wavefront walks the segment table (longest problems first) with the
   // grid's stride: one segment per wavefront, or, in a persistent grid, a
@@ -476,12 +484,14 @@ __global__ __launch_bounds__(256) void sweep1d2_x64(Sweep1DArgs a, StateArrays<d
     const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
     Lim1D lim;
     // stage 0: U^n -> U* on lanes 2..61
-    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
+    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61, own};
     Prim<double> h = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, q, 0.0, r0, lim);
+    lim.nbad += bad_state_lanes(own, h.rho, h.uu, !TAB);
     ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
     // stage 1: U^n, U* -> U^{n+1} on lanes 4..59
-    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
+    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59, own};
     const Prim<double> o = stage_x64<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, q, 0.0, r1, lim);
+    lim.nbad += bad_state_lanes(own, o.rho, o.uu, !TAB);
     if (own) store_prims<NV, TAB>(out, off, c, o);
     commit_limits(pb, ck, a.counters, lane, lim);
   }
@@ -496,44 +506,47 @@ struct Geo32 {
   ReconGeo32 rg;
   double area;              // lower face (FP64)
   FaceGeo<float> fg;        // FP32 face metric for the Riemann solve (unit area)
-  double vol, gcov0, gcon0, cp;
-  float c000, isc0, dcen;
+  double ivol, gcov0, cp;
+  float c000, dcen;
 };
 
 template <bool SPH>
 MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
 {
   const int q = clamp_q(c, n);
-  const float *rc;
   g.fg.area = 1.0f;
   g.fg.gcov[1] = g.fg.gcov[2] = 1.0f;
   if (SPH) {
     const ZoneF32 &z = pb.z32[q];
-    rc = z.rc;
     g.area = z.area;
     g.fg.scale = z.scalef;
     g.fg.gcov[0] = z.gcovf0;
-    g.vol = z.vol;
+    g.ivol = z.ivol;
     g.gcov0 = z.gcov0;
-    g.gcon0 = z.gcon0;
     g.cp = z.cp;
     g.c000 = z.c000;
-    g.isc0 = z.isc0;
     g.dcen = z.dcen;
   } else {
-    rc = pb.rc[0] + (size_t)q * MDS_RC_COUNT;
     g.area = pb.area[0];
     g.fg.scale = 1.0f;
     g.fg.gcov[0] = 1.0f;
-    g.vol = pb.volume;
-    g.gcov0 = g.gcon0 = 1.0;
+    g.ivol = pb.ivolume;
+    g.gcov0 = 1.0;
     g.cp = 0.0;
     g.c000 = 0.0f;
-    g.isc0 = 1.0f;
     g.dcen = 0.5f * (float)pb.dx[0];
   }
-  g.rg.clm = rc[MDS_RC_CLM]; g.rg.clp = rc[MDS_RC_CLP]; g.rg.crm = rc[MDS_RC_CRM]; g.rg.crp = rc[MDS_RC_CRP];
-  g.rg.mu1 = rc[MDS_RC_MU1]; g.rg.kinv = rc[MDS_RC_KINV]; g.rg.inur = rc[MDS_RC_INUR]; g.rg.imu2 = rc[MDS_RC_IMU2];
+  // one coefficient set per problem (wave-uniform: scalar loads) or per zone
+  if (pb.rcu) {
+    g.rg.clm = pb.rcu_c[MDS_RC_CLM]; g.rg.clp = pb.rcu_c[MDS_RC_CLP];
+    g.rg.crm = pb.rcu_c[MDS_RC_CRM]; g.rg.crp = pb.rcu_c[MDS_RC_CRP];
+    g.rg.mu1 = pb.rcu_c[MDS_RC_MU1]; g.rg.kinv = pb.rcu_c[MDS_RC_KINV];
+    g.rg.inur = pb.rcu_c[MDS_RC_INUR]; g.rg.imu2 = pb.rcu_c[MDS_RC_IMU2];
+  } else {
+    const float *rc = pb.rc[0] + (size_t)q * MDS_RC_COUNT;
+    g.rg.clm = rc[MDS_RC_CLM]; g.rg.clp = rc[MDS_RC_CLP]; g.rg.crm = rc[MDS_RC_CRM]; g.rg.crp = rc[MDS_RC_CRP];
+    g.rg.mu1 = rc[MDS_RC_MU1]; g.rg.kinv = rc[MDS_RC_KINV]; g.rg.inur = rc[MDS_RC_INUR]; g.rg.imu2 = rc[MDS_RC_IMU2];
+  }
 }
 
 // U^n in conserved form (FP64) and its primitives as doubles.
@@ -547,13 +560,43 @@ MDS_DEV void conserved_f32(const Geo32 &g, const Prim<float> &q, double U0[6], P
   U0[MDS_YI] = TAB ? U0[0] * qn.ye : 0.0;
 }
 
+// CFL and source limits of one FP32-bulk lane: the smallest of its faces'
+// dx / |signal speed|, its zone's estimate_dt bound and its source limit,
+// as one FP32 candidate, reduced once per wavefront.
+struct Lim32 {
+  float dtc = __builtin_huge_valf();
+  unsigned nsh = 0;         // shock-tagged owned faces (wave-uniform)
+  unsigned nbad = 0;        // owned zone updates with an unphysical state (wave-uniform)
+  unsigned noff = 0;        // table EOS evaluations of owned zones off the table (wave-uniform)
+};
+
+MDS_DEV float wave_min_f32(float v)
+{
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+MDS_DEV void commit_limits32(DevClock *ck, unsigned long long *counters, int lane, const Lim32 &lim)
+{
+  const float w = wave_min_f32(lim.dtc);
+  if (lane == 0) {
+    if (w < __builtin_huge_valf()) atomicMin(&ck->mindt, dt_key((double)w));
+    if (lim.nsh) atomicAdd(&counters[0], (unsigned long long)lim.nsh);
+    if (lim.nbad) atomicAdd(&ck->nbad, (unsigned long long)lim.nbad);
+    if (lim.noff) atomicAdd(&ck->noff, (unsigned long long)lim.noff);
+  }
+}
+
 // One FP32-bulk RK stage of one lane: EOS table interpolation,
 // reconstruction and Riemann fluxes in FP32 (per unit area), area-weighted
-// fluxes, sources, the update and u -> p in FP64.
+// fluxes, sources, the update and u -> p in FP64.  With the gamma law the
+// face pressures are (gamma - 1) times the face energies: para_recon is
+// homogeneous of degree one, so this is the reference's separate pressure
+// reconstruction (reconstruct.c:246-266 over P) up to rounding.
 template <int NV, bool SPH, int TAB, int MONO>
 MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 &g, int stage, double dt,
                               const Prim<float> &q, const double U0[6], const Prim<double> &qn, double gz,
-                              Roles r, Lim1D &lim)
+                              Roles r, Lim32 &lim)
 {
   const float gam = (float)pb.gam;
   float press, gam1, cs = 0.0f;
@@ -562,19 +605,19 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
     double lt;
     eos_fill_f32(eos, (double)q.rho, (double)q.uu, (double)q.ye, dpress, gam1, cs, lt);
     press = (float)dpress;
+    lim.noff += (unsigned)__popcll(__ballot(r.own && eos_off_table(eos, (double)q.rho, (double)q.ye, lt)));
   } else {
     press = (gam - 1.0f) * q.uu;
     dpress = press;
     gam1 = gam;
   }
-  constexpr int NQ = 3 + NV + 2 * TAB;
+  constexpr int NQ = TAB ? 5 + NV : 2 + NV;
   float al[NQ], ar[NQ];
   {
     float qv[NQ];
     qv[0] = q.rho; qv[1] = q.uu;
     for (int d = 0; d < NV; d++) qv[2 + d] = q.v[d];
     if (TAB) { qv[2 + NV] = q.ye; qv[3 + NV] = press; qv[4 + NV] = gam1; }
-    else qv[2 + NV] = press;
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
       float qm = lane_from_below(qv[k]);
@@ -593,7 +636,7 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
     L.G = lane_from_below(ar[4 + NV]);  Rt.G = al[4 + NV];
   } else {
     L.ye = Rt.ye = 0.0f;
-    L.P = lane_from_below(ar[2 + NV]); Rt.P = al[2 + NV];
+    L.P = (gam - 1.0f) * L.uu; Rt.P = (gam - 1.0f) * Rt.uu;
     L.G = gam; Rt.G = gam;
   }
 
@@ -601,7 +644,7 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
   int isshock = 0;
   if (g.area >= 1.0e-16) isshock = hllc_f32<NV, !SPH, TAB>(L, Rt, g.fg, 0, Ff, sig);
   else sig = 1.0e-30f;
-  if (r.face) lim.smax = fmax(lim.smax, (double)fabsf(sig));
+  if (r.face) lim.dtc = fminf(lim.dtc, (float)pb.dx[0] * frcp(fabsf(sig)));
   lim.nsh += __popcll(__ballot(r.count && isshock));
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 2 + NV; k++) F[k] = g.area * (double)Ff[k];
@@ -628,41 +671,43 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
     // centroid offset from the lower face
     const double vl = F[0] / (g.area + 1.0e-16), vr = Fp[0] / (area_hi + 1.0e-16);
     src[2] += drho * gz;
-    src[1] += (vl + ((vr - vl) / pb.dx[0]) * (double)g.dcen) * gz;
+    src[1] += (vl + ((vr - vl) * pb.idx[0]) * (double)g.dcen) * gz;
   }
   const double gcov[3] = {g.gcov0, 1.0, 1.0};
   double U[6], Uh[6];
   for (int k = 0; k < 6; k++) U[k] = U0[k];
   p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
   Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
-  lim.lim_src = fmin(lim.lim_src, dt_source_control_fast<NV>(qn.uu, qn.v, U, src));
-  const double ivol = drcp(g.vol);
+  lim.dtc = fminf(lim.dtc, (float)dt_source_control_fast<NV>(qn.uu, qn.v, U, src));
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
   for (int k = 0; k < 2 + NV; k++) {
-    double ddt = (Fp[k] - F[k]) * ivol - src[k];
+    double ddt = (Fp[k] - F[k]) * g.ivol - src[k];
     U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
   }
   if (TAB) {
     const int k = MDS_YI;
-    double ddt = (Fp[k] - F[k]) * ivol;
+    double ddt = (Fp[k] - F[k]) * g.ivol;
     U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
   }
-  double vout[3] = {0.0, 0.0, 0.0}, vsq = 0.0;
-  const double ri = drcp(U[0]);
+  // u -> p (fluid.c:27-53): v^d = gcon[d] m_d / rho with gcon[0] = 1 / gcov0,
+  // one reciprocal of rho gcov0 for the radial column
+  double vout[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+  const double rg = drcp(SPH ? U[0] * g.gcov0 : U[0]);
   for (int d = 0; d < NV; d++) {
-    double m = U[2 + d] * ri;           // covariant velocity
-    vout[d] = SPH ? g.gcon0 * m : m;
-    vsq += m * vout[d];
+    vout[d] = U[2 + d] * ((SPH && d == 0) ? rg : (SPH ? rg * g.gcov0 : rg));
+    ke += U[2 + d] * vout[d];
   }
-  const double uout = U[1] - 0.5 * U[0] * vsq;
+  const double uout = U[1] - 0.5 * ke;
   o.rho = (float)U[0];
   o.uu = (float)uout;
   for (int d = 0; d < NV; d++) o.v[d] = (float)vout[d];
-  if (TAB) o.ye = (float)(U[MDS_YI] * ri);
+  if (TAB) o.ye = (float)(U[MDS_YI] * (SPH ? rg * g.gcov0 : rg));
   if (stage == 1) {
+    // estimate_dt zone bound (step.c:177-187) with 1 / scale[0][0] = rsqrt(gcov0)
     if (!TAB) cs = fsqrt(gam * press * frcp(q.rho));
-    double loc = (fabs(vout[0]) + (double)(cs * g.isc0)) * (1.0 / pb.dx[0]);
-    lim.locmax = fmax(lim.locmax, loc);
+    const float isc0 = SPH ? frsqrt((float)g.gcov0) : 1.0f;
+    const float loc = (fabsf((float)vout[0]) + cs * isc0) * (float)pb.idx[0];
+    lim.dtc = fminf(lim.dtc, frcp(loc));
   }
   return o;
 }
@@ -674,7 +719,9 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
                                                    StateArrays<float> sph, int stage)
 {
   const int lane = threadIdx.x & 63;
-  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  // wave-uniform (threadIdx.x / 64 is the same on every lane of a wave), so
+  // the segment and problem records come in through scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
   // each wavefront walks the segment table (longest problems first) with the
   // grid's stride: one segment per wavefront, or, in a persistent grid, a
@@ -699,6 +746,7 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
     r.count = r.face && (lane <= 61 || c == n);
     r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
+    r.own = r.out;
     Prim<float> qnf = q;
     if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, 0);
     double U0[6];
@@ -706,13 +754,14 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
     const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
 
-    Lim1D lim;
+    Lim32 lim;
     const Prim<float> o = stage_f32<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
+    lim.nbad += bad_state_lanes(r.out, o.rho, o.uu, !TAB);
     if (r.out) {
       StateArrays<float> &out = (stage == 0) ? sph : sp;
       store_prims<NV, TAB>(out, off, c, o);
     }
-    commit_limits(pb, ck, a.counters, lane, lim);
+    commit_limits32(ck, a.counters, lane, lim);
   }
 }
 
@@ -721,7 +770,7 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
                                                     StateArrays<float> out)
 {
   const int lane = threadIdx.x & 63;
-  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
   // each wavefront walks the segment table (longest problems first) with the
   // grid's stride: one segment per wavefront, or, in a persistent grid, a
@@ -751,14 +800,16 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
 
     const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
     const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
-    Lim1D lim;
-    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61};
+    Lim32 lim;
+    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61, own};
     Prim<float> h = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, qn, 0.0, r0, lim);
+    lim.nbad += bad_state_lanes(own, h.rho, h.uu, !TAB);
     ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
-    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59};
+    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59, own};
     const Prim<float> o = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, qn, 0.0, r1, lim);
+    lim.nbad += bad_state_lanes(own, o.rho, o.uu, !TAB);
     if (own) store_prims<NV, TAB>(out, off, c, o);
-    commit_limits(pb, ck, a.counters, lane, lim);
+    commit_limits32(ck, a.counters, lane, lim);
   }
 }
 

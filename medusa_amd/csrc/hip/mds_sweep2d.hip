@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   // 1/dt_zone and (no sources) the minimum dt_source_control numerator
   R smax_x = (R)0, smax_y = (R)0;
   double locmax = 0.0, srcnum = 1.0e300, lim_src = 1.0e300;
-  unsigned long long nshock = 0, nhlle = 0;
+  unsigned long long nshock = 0, nhlle = 0, nbad = 0;
 
   Cell<R, NV> W[5];
   for (int k = 0; k < 4; k++) load_cell<R, NV, SLAB>(in, pb, i, j0 - 2 + k, 0, gm1, W[k]);
@@ -137,6 +137,8 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
       nhlle += hl;
     }
 
+    bool bad_live = false;
+    double bad_ro = 1.0, bad_uo = 0.0;
     if (col_out) {
       const long long idx = pb.off + (long long)j * n1 + i;
       const Cell<R, NV> &me = W[2];
@@ -188,6 +190,9 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
         for (int d = 0; d < NV; d++) { vo[d] = U[2 + d] * ri; vsq += U[2 + d] * vo[d]; }
         uo = U[1] - 0.5 * vsq;
       }
+      bad_ro = ro;
+      bad_uo = uo;
+      bad_live = true;
       StateArrays<R> &out = (stage == 0) ? sph : sp;
       out.v[0][idx] = (R)ro;
       out.v[1][idx] = (R)uo;
@@ -208,6 +213,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
       }
     }
 
+    nbad += bad_state_lanes(bad_live, bad_ro, bad_uo, true);
     for (int k = 0; k < 4; k++) W[k] = W[k + 1];
     yl_cur = yl_next; yr_cur = yr_next;
     tag_cur = tag_next;
@@ -233,6 +239,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   if (lane == 0) {
     if (nshock) atomicAdd(&a.counters[0], nshock);
     if (nhlle) atomicAdd(&a.counters[1], nhlle);
+    if (nbad) atomicAdd(&ck->nbad, nbad);
   }
 }
 

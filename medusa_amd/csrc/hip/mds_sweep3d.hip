@@ -68,7 +68,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
 
   R smax[3] = {(R)0, (R)0, (R)0};
   double locmax = 0.0, srcnum = 1.0e300, lim_src = 1.0e300;
-  unsigned long long nshock = 0, nhlle = 0;
+  unsigned long long nshock = 0, nhlle = 0, nbad = 0;
 
   auto s1tag = [&](int kk) -> int {
     if (stage == 1 && lane_out && row_in && kk >= 0 && kk < n3)
@@ -192,6 +192,8 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     }
     __syncthreads();
 
+    bool bad_live = false;
+    double bad_ro = 1.0, bad_uo = 0.0;
     if (col_out) {
       const long long idz = pb.off + ((long long)k * n2 + j) * n1 + i;
       double rho = me.rho, uu = me.uu, press = me.P, v[3] = {0.0, 0.0, 0.0};
@@ -244,6 +246,9 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
         for (int d = 0; d < NV; d++) { vo[d] = U[2 + d] * ri; vsq += U[2 + d] * vo[d]; }
         uo = U[1] - 0.5 * vsq;
       }
+      bad_ro = ro;
+      bad_uo = uo;
+      bad_live = true;
       StateArrays<R> &out = (stage == 0) ? sph : sp;
       out.v[0][idz] = (R)ro;
       out.v[1][idz] = (R)uo;
@@ -264,6 +269,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       }
     }
 
+    nbad += bad_state_lanes(bad_live, bad_ro, bad_uo, true);
     if (zw) {
       for (int q = 0; q < 4; q++) Z[q] = Z[q + 1];
       zl_cur = zl_next; zr_cur = zr_next;
@@ -291,6 +297,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   if (lane == 0) {
     if (nshock) atomicAdd(&a.counters[0], nshock);
     if (nhlle) atomicAdd(&a.counters[1], nhlle);
+    if (nbad) atomicAdd(&ck->nbad, nbad);
   }
 }
 
@@ -354,7 +361,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
 
   R smax[3] = {(R)0, (R)0, (R)0};
   double locmax = 0.0, srcnum = 1.0e300, lim_src = 1.0e300;
-  unsigned long long nshock = 0, nhlle = 0;
+  unsigned long long nshock = 0, nhlle = 0, nbad = 0;
 
   auto s1tag = [&](int kk) -> int {
     if (stage == 1 && lane_out && row_in && kk >= 0 && kk < n3)
@@ -508,6 +515,8 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     }
     __syncthreads();
 
+    bool bad_live = false;
+    double bad_ro = 1.0, bad_uo = 0.0;
     if (col_out) {
       const long long idz = pb.off + ((long long)k * n2 + j) * n1 + i;
       double rho = me.rho, uu = me.uu, press = me.P, v[3] = {0.0, 0.0, 0.0};
@@ -560,6 +569,9 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
         for (int d = 0; d < NV; d++) { vo[d] = U[2 + d] * ri; vsq += U[2 + d] * vo[d]; }
         uo = U[1] - 0.5 * vsq;
       }
+      bad_ro = ro;
+      bad_uo = uo;
+      bad_live = true;
       StateArrays<R> &out = (stage == 0) ? sph : sp;
       out.v[0][idz] = (R)ro;
       out.v[1][idz] = (R)uo;
@@ -580,6 +592,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
       }
     }
 
+    nbad += bad_state_lanes(bad_live, bad_ro, bad_uo, true);
     if (zw) {
       for (int q = 0; q < 4; q++) Z[q] = Z[q + 1];
       zl_cur = zl_next; zr_cur = zr_next;
@@ -607,6 +620,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   if (lane == 0) {
     if (nshock) atomicAdd(&a.counters[0], nshock);
     if (nhlle) atomicAdd(&a.counters[1], nhlle);
+    if (nbad) atomicAdd(&ck->nbad, nbad);
   }
 }
 

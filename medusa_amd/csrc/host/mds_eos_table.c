@@ -13,7 +13,11 @@
  * chi_rho + P chi_T^2 / (rho T c_v) from central differences of the model.
  * A, K53 and rho_x carry a seeded +-10%, +-5%, +-10% jitter.
  */
+#define _FILE_OFFSET_BITS 64
+#define _POSIX_C_SOURCE 200809L
+#include <sys/types.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "../../../include/mds_hydro.h"
@@ -26,6 +30,57 @@ void mds_eos_table_default(mds_eos_table *t)
   t->nr = 180;  t->lr0 = 3.0;  t->lr1 = 15.0;
   t->nt = 180;  t->lt0 = 9.0;  t->lt1 = 12.0;
   t->ny = 50;   t->ye0 = 0.05; t->ye1 = 0.55;
+  t->e_scale = 1.0;
+  t->e_shift = 0.0;
+  t->p_scale = 1.0;
+}
+
+void mds_eos_burrows_default(mds_eos_table *t)
+{
+  memset(t, 0, sizeof(*t));
+  t->nr = 300;  t->lr0 = 1.0;    t->lr1 = 15.5;
+  t->nt = 300;  t->lt0 = -2.0;   t->lt1 = 2.0;
+  t->ny = 50;   t->ye0 = 0.035;  t->ye1 = 0.56;
+  t->e_scale = 0.95655684e18;    /* erg/g per MeV/baryon (eos_stuff.c:530) */
+  t->e_shift = 0.0;              /* const_hack (eos_stuff.c:375) */
+  t->p_scale = 1.60217733e33;    /* dyn/cm^2 per MeV/fm^3 (eos_stuff.c:618) */
+}
+
+/* collapse_init, eos_stuff.c:76-133: the records are read one at a time at
+ * their byte offsets; record 2 is log-transformed after the read.  Only the
+ * three records the sweep uses are kept. */
+int mds_eos_table_load_burrows(const char *path, mds_eos_table *t)
+{
+  if (!path || !t || t->nr < 2 || t->nt < 3 || t->ny < 2) return MDS_EINVAL;
+  const size_t rec = (size_t)t->ny * t->nr * t->nt;
+  static const int want[MDS_EOS_NFIELD] = {1, 2, 12};     /* reference record numbers (1-based) */
+  FILE *f = fopen(path, "rb");
+  if (!f) return MDS_EIO;
+  /* the file must hold all MDS_BURROWS_NREC records */
+  if (fseeko(f, 0, SEEK_END) != 0 || ftello(f) < (off_t)MDS_BURROWS_NREC * 8 * (off_t)rec) {
+    fclose(f);
+    return MDS_EIO;
+  }
+  float *data = (float *)malloc(MDS_EOS_NFIELD * rec * sizeof(float));
+  double *buf = (double *)malloc(rec * sizeof(double));
+  int rc = (data && buf) ? MDS_OK : MDS_ENOMEM;
+  for (int k = 0; k < MDS_EOS_NFIELD && rc == MDS_OK; k++) {
+    const long long off = (long long)(want[k] - 1) * 8LL * (long long)rec;
+    if (fseeko(f, (off_t)off, SEEK_SET) != 0 || fread(buf, sizeof(double), rec, f) != rec) {
+      rc = MDS_EIO;
+      break;
+    }
+    float *dst = data + (size_t)k * rec;
+    for (size_t q = 0; q < rec; q++) dst[q] = (float)(k == MDS_EOS_LNP ? log(buf[q]) : buf[q]);
+  }
+  fclose(f);
+  free(buf);
+  if (rc != MDS_OK) {
+    free(data);
+    return rc;
+  }
+  t->data = data;
+  return MDS_OK;
 }
 
 void mds_eos_model_params(unsigned long long seed, double out[3])

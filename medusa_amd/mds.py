@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmds.so")
 
 NG = 2
 FP64, FP32_BULK = 0, 1
+EIO = -5                       # MDS_EIO: a table file missing or short
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC, BC_PROB = 0, 1, 2, 3, 4
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
@@ -67,7 +68,8 @@ class EosTableDesc(ctypes.Structure):
                 ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
                 ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
                 ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
-                ("data", ctypes.POINTER(ctypes.c_float))]
+                ("data", ctypes.POINTER(ctypes.c_float)),
+                ("e_scale", ctypes.c_double), ("e_shift", ctypes.c_double), ("p_scale", ctypes.c_double)]
 
 
 class Problem(ctypes.Structure):
@@ -92,17 +94,25 @@ class RunParams(ctypes.Structure):
                 ("hydro_interp_order", ctypes.c_int)]
 
 
+class Health(ctypes.Structure):
+    """mds_health: device health of one problem (include/mds_hydro.h)."""
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("istep", ctypes.c_int),
+                ("dt_below_min", ctypes.c_int), ("bad_state", ctypes.c_longlong),
+                ("eos_off_table", ctypes.c_longlong)]
+
+
 _LIB = None
 
 EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
-            "mds_run_graph", "mds_sync", "mds_get_clock", "mds_conserved_totals", "mds_counters",
+            "mds_run_graph", "mds_sync", "mds_get_clock", "mds_get_health", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
             "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
-            "mds_eos_table_free", "mds_eos_points", "mds_init_polytrope"]
+            "mds_eos_table_free", "mds_eos_points", "mds_init_polytrope", "mds_eos_burrows_default",
+            "mds_eos_table_load_burrows"]
 
 
 def lib():
@@ -133,6 +143,7 @@ def lib():
     L.mds_run_graph.argtypes = [vp, ctypes.c_int]
     L.mds_sync.argtypes = [vp]
     L.mds_get_clock.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.mds_get_health.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Health)]
     L.mds_conserved_totals.argtypes = [vp, ctypes.c_int, _dp]
     L.mds_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.mds_stream.argtypes = [vp]
@@ -154,6 +165,9 @@ def lib():
     L.mds_eos_model_eval.argtypes = [_dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
     L.mds_eos_model_eval.restype = None
     L.mds_eos_table_build.argtypes = [ctypes.POINTER(EosTableDesc), ctypes.c_ulonglong]
+    L.mds_eos_burrows_default.argtypes = [ctypes.POINTER(EosTableDesc)]
+    L.mds_eos_burrows_default.restype = None
+    L.mds_eos_table_load_burrows.argtypes = [ctypes.c_char_p, ctypes.POINTER(EosTableDesc)]
     L.mds_eos_table_free.argtypes = [ctypes.POINTER(EosTableDesc)]
     L.mds_eos_table_free.restype = None
     L.mds_eos_points.argtypes = [ctypes.c_int, ctypes.POINTER(EosTableDesc), ctypes.c_int, _dp, _dp, _dp,
@@ -388,6 +402,12 @@ class Context:
                     "get_clock")
         return t.value, dt.value, ist.value
 
+    def health(self, prob=0):
+        """Device health counters of a problem: dict of bad_state, eos_off_table, dt_below_min (and the clock)."""
+        h = Health()
+        self._check(lib().mds_get_health(self.h, prob, ctypes.byref(h)), "get_health")
+        return {f: getattr(h, f) for f, _ in Health._fields_}
+
     def totals(self, prob=0):
         out = np.zeros(2)
         self._check(lib().mds_conserved_totals(self.h, prob, _ptr(out)), "totals")
@@ -538,15 +558,24 @@ class EosTable:
 
     NFIELD = 3
 
-    def __init__(self, seed=EOS_SEED, **dims):
+    def __init__(self, seed=EOS_SEED, burrows_path=None, **dims):
+        """The synthetic table (seeded), or with burrows_path a table file in the
+        reference's Burrows binary layout (dimensions from mds_eos_burrows_default
+        unless given in dims)."""
         self.t = EosTableDesc()
-        lib().mds_eos_table_default(ctypes.byref(self.t))
+        if burrows_path is None:
+            lib().mds_eos_table_default(ctypes.byref(self.t))
+        else:
+            lib().mds_eos_burrows_default(ctypes.byref(self.t))
         for k, v in dims.items():
             setattr(self.t, k, v)
         self.seed = seed
-        rc = lib().mds_eos_table_build(ctypes.byref(self.t), seed)
+        if burrows_path is None:
+            rc = lib().mds_eos_table_build(ctypes.byref(self.t), seed)
+        else:
+            rc = lib().mds_eos_table_load_burrows(os.fsencode(burrows_path), ctypes.byref(self.t))
         if rc:
-            raise RuntimeError(f"mds_eos_table_build failed ({rc})")
+            raise RuntimeError(f"EOS table set-up failed ({rc})")
         n = self.NFIELD * self.t.ny * self.t.nr * self.t.nt
         self.data = np.ctypeslib.as_array(self.t.data, shape=(n,)).reshape(self.NFIELD, self.t.ny, self.t.nr,
                                                                            self.t.nt)

@@ -98,6 +98,9 @@ typedef struct orc_eos_tab {
   int nr, nt, ny;
   double lr0, lr1, lt0, lt1, ye0, ye1;
   const float *data;
+  /* table units (eos_stuff.c:530, 618): e / e_scale - e_shift is compared
+   * with the energy record, P = exp(ln P) * p_scale; 0 reads as 1 */
+  double e_scale, e_shift, p_scale;
 } orc_eos_tab;
 
 /* eos_stuff.c:507-650: out = {P, cs, T, Gamma_1}; returns off-table bits

@@ -5,10 +5,14 @@
  * eos/burrows/eos_stuff.c:507-650 (eos_given_rex_tianshu): bilinear weights in
  * (log10 rho, Ye), the walk along the T nodes from the previous temperature
  * until the energy is bracketed, the linear inverse in log T, and trilinear
- * ln P and Gamma_1 at the recovered temperature.  Differences, all stated in
- * the library header as well: 0-based nodes; the table's own units (e in
- * erg/g, T in K: no MeV conversions or energy shift); (rho, Ye) are clamped
- * to the table edges (the reference reads outside the table there).
+ * ln P and Gamma_1 at the recovered temperature, with the walk's unit
+ * handling: the specific energy divided by e_scale minus e_shift (MeV per
+ * baryon and const_hack for a Burrows table, eos_stuff.c:530) is what the
+ * energy record is compared with, and P = exp(ln P) * p_scale (MeV/fm^3 to
+ * dyn/cm^2, :618); T is returned in the table's unit (MeV there, :531-532,
+ * 645).  Differences, all stated in the library header as well: 0-based
+ * nodes; FP32 table values; (rho, Ye) are clamped to the table edges (the
+ * reference reads outside the table there).
  *
  * orc_eos_root restates the Newton iteration with central-difference
  * derivative and the bisection fallback of
@@ -23,6 +27,9 @@
 #include <math.h>
 #include <stddef.h>
 #include "oracle.h"
+
+static double escale(const orc_eos_tab *t) { return t->e_scale > 0.0 ? t->e_scale : 1.0; }
+static double pscale(const orc_eos_tab *t) { return t->p_scale > 0.0 ? t->p_scale : 1.0; }
 
 static double tab(const orc_eos_tab *t, int rec, int jy, int jr, int jt)
 {
@@ -90,7 +97,7 @@ int orc_eos_walk(const orc_eos_tab *t, double rho, double u, double ye, double t
 {
   const orc_w w = weights(t, rho, ye);
   const double dt = (t->lt1 - t->lt0) / (t->nt - 1);
-  const double e = u / rho;
+  const double e = (u / rho) / escale(t) - t->e_shift;          /* eos_stuff.c:527, 530 */
   double temp = temp_guess < 1.0e-10 ? 1.0 : temp_guess;       /* eos_stuff.c:528 */
   const double st = (log10(temp) - t->lt0) / dt;
   int jt = !(st >= 0.0) ? 0 : (st > t->nt - 2 ? t->nt - 2 : (int)st);
@@ -121,7 +128,7 @@ int orc_eos_walk(const orc_eos_tab *t, double rho, double u, double ye, double t
   const double lt = tlo + (thi - tlo) * dtemp;
   const double lnp = tri(t, &w, 1, jt, dtemp);
   const double g1 = tri(t, &w, 2, jt, dtemp);
-  const double p = exp(lnp);
+  const double p = exp(lnp) * pscale(t);                       /* eos_stuff.c:618 */
   out[0] = p;
   out[1] = sqrt((g1 * p) / rho);
   out[2] = pow(10.0, lt);
@@ -177,7 +184,7 @@ int orc_eos_root(const orc_eos_tab *t, double rho, double u, double ye, double l
                  double *lt_out, int *newton_iters)
 {
   const orc_w w = weights(t, rho, ye);
-  const double val = u / rho;
+  const double val = (u / rho) / escale(t) - t->e_shift;
   double lt = lt_guess, lt_last;
   int iter = 0;
   do {

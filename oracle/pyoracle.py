@@ -141,13 +141,16 @@ class EosTab(ctypes.Structure):
                 ("lr0", ctypes.c_double), ("lr1", ctypes.c_double),
                 ("lt0", ctypes.c_double), ("lt1", ctypes.c_double),
                 ("ye0", ctypes.c_double), ("ye1", ctypes.c_double),
-                ("data", ctypes.POINTER(ctypes.c_float))]
+                ("data", ctypes.POINTER(ctypes.c_float)),
+                ("e_scale", ctypes.c_double), ("e_shift", ctypes.c_double), ("p_scale", ctypes.c_double)]
 
 
 def eos_tab(dims, data):
-    """Describe a float32 table [3][ny][nr][nt]; dims: dict of nr, nt, ny and the ranges."""
+    """Describe a float32 table [3][ny][nr][nt]; dims: dict of nr, nt, ny, the ranges and
+    (optionally) the units e_scale, e_shift, p_scale (default cgs: 1, 0, 1)."""
     data = np.ascontiguousarray(data, dtype=np.float32)
-    t = EosTab(**{k: dims[k] for k in ("nr", "nt", "ny", "lr0", "lr1", "lt0", "lt1", "ye0", "ye1")})
+    t = EosTab(**{k: dims[k] for k in ("nr", "nt", "ny", "lr0", "lr1", "lt0", "lt1", "ye0", "ye1")},
+               e_scale=dims.get("e_scale", 1.0), e_shift=dims.get("e_shift", 0.0), p_scale=dims.get("p_scale", 1.0))
     t.data = data.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     t._keep = data
     return t

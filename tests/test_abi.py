@@ -26,7 +26,7 @@ def built_library():
 
 def declared_functions():
     names = set()
-    for h in ("mds_hydro.h", "mds_host.h"):
+    for h in ("mds_hydro.h", "mds_host.h", "mds_eos.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names |= set(re.findall(r"\b(mds_[a-z0-9_]+)\s*\(", txt))
