@@ -44,6 +44,10 @@ extern "C" {
 #define MDS_BC_SPHERICAL_ORIGIN 5 /* spherical grids: theta boundary across the pole (3D), or inner
                                      radius through the origin (1D, 3D; theta -> pi - theta,
                                      phi + pi, v^1 and v^3 flipped) */
+#define MDS_BC_DISK      8 /* inner radius (direction 0, lower side) only: linear extrapolation from
+                              zones 0 and 1 as EXTRAP, with v^1 clamped to 0 where it would point
+                              out of the boundary, inflow only (bc.c:863-878); 1D columns and
+                              2D/3D Cartesian grids */
 
 #define MDS_GEOM_CARTESIAN 0
 #define MDS_GEOM_SPHERICAL 1     /* 1D radial columns or 3D (r, theta, phi) grids */

@@ -448,7 +448,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
         // and the angles of 3D spherical grids; EXTRAP in direction 0 (the
         // only direction the reference extrapolates), not at that inner radius
         const bool slab = (b == MDS_BC_PROB && (!c->sph3 || (d == 0 && side == 1))) ||
-                          (b == MDS_BC_EXTRAP && d == 0 && (!c->sph3 || side == 1));
+                          (b == MDS_BC_EXTRAP && d == 0 && (!c->sph3 || side == 1)) ||
+                          (b == MDS_BC_DISK && d == 0 && side == 0 && !c->sph3);
         if (b != MDS_BC_OUTFLOW && b != MDS_BC_REFLECT && b != MDS_BC_PERIODIC && !pole && !origin && !slab)
           return set_err(c, MDS_ENOTIMPL, "problem %d: boundary code %d not supported on the device", p, b);
       }
@@ -526,7 +527,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       for (int dd = 0; dd < g->ndim; dd++)
         for (int side = 0; side < 2; side++) {
           const int b = side ? P.bc_hi[dd] : P.bc_lo[dd];
-          if (b != MDS_BC_PROB && b != MDS_BC_EXTRAP) continue;
+          if (b != MDS_BC_PROB && b != MDS_BC_EXTRAP && b != MDS_BC_DISK) continue;
           const size_t bytes = sizeof(double) * MDS_NG * MDS_SLAB_NV * (size_t)slab_T(d, dd);
           void *m = nullptr;
           HIPCHK(c, hipMalloc(&m, bytes));

@@ -2,8 +2,9 @@
 //
 // Every other boundary code is a map from a ghost index onto an interior zone
 // and is applied by the stage kernels at load time.  PROB (the problem's own
-// ghost values, bc.c:125-140) and EXTRAP (linear extrapolation, bc.c:806-821,
-// 937-954) produce values that are no interior zone, so they live in FP64
+// ghost values, bc.c:125-140), EXTRAP (linear extrapolation, bc.c:806-821,
+// 937-954) and DISK (the same at the inner radius with an inflow-only v^1,
+// bc.c:863-878) produce values that are no interior zone, so they live in FP64
 // ghost layers beside the state (DevProb::gs).  Before every stage this
 // kernel refills the EXTRAP layers from the stage's input state and the
 // pressure and Gamma_1 entries of every layer (the reference's EOS pass also
@@ -33,9 +34,10 @@ __global__ __launch_bounds__(256) void slab_fill(const DevProb *probs, StateArra
       const long long t = it % T;
       double *e = sl + (size_t)g * MDS_SLAB_NV * T + t;       // entry v at e[v * T]
       const int code = side ? pb.bc_hi[d] : pb.bc_lo[d];
-      if (code == MDS_BC_EXTRAP) {
+      if (code == MDS_BC_EXTRAP || code == MDS_BC_DISK) {
         // direction 0: edge zone + (distance) x (edge - next), at the ghost's
-        // own (j, k) as the reference indexes it
+        // own (j, k) as the reference indexes it; DISK (inner radius) then
+        // clamps an outward v^1 to 0 (bc.c:863-878)
         const int n = pb.n[0];
         const int c = side ? n + g : g - MDS_NG;
         const int i0 = side ? n - 1 : 0, i1 = side ? n - 2 : 1;
@@ -51,6 +53,7 @@ __global__ __launch_bounds__(256) void slab_fill(const DevProb *probs, StateArra
             const double q0 = (double)st.v[v][z0], q1 = (double)st.v[v][z1];
             x = side ? q0 + w * (q0 - q1) : q0 + w * (q1 - q0);
           }
+          if (code == MDS_BC_DISK && v == 2 && x > 0.0) x = 0.0;
           e[v * T] = x;
         }
       }

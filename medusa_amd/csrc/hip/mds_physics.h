@@ -380,6 +380,10 @@ MDS_DEV float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // must stay in the FP32 normal range: the hardware flushes denormal inputs.
 MDS_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 MDS_DEV float frsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+// x^y for x > 0 as 2^(y log2 x) (v_log_f32, v_exp_f32): three instructions
+// instead of the library powf's range reduction; the relative error grows to
+// a few 1e-7 for |y log2 x| near 10, within what FP32 storage already makes
+MDS_DEV float fpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 
 // para_recon in local coordinates: identical branch structure, well-conditioned.
 MDS_DEV void para_recon_f32(float qm, float q0, float qp, const ReconGeo32 &g, float &al, float &ar)
@@ -421,11 +425,11 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
   if (pratio < 2.0f && p_s >= p_min && p_s <= p_max) {
   } else if (p_s < p_min) {
     float G_avg = 0.5f * (G_l + G_r);
-    float plr = __builtin_powf(p_l * frcp(p_r), (G_avg - 1.0f) * frcp(2.0f * G_avg));
+    float plr = fpow(p_l * frcp(p_r), (G_avg - 1.0f) * frcp(2.0f * G_avg));
     float icl = frcp(cs_l), icr = frcp(cs_r);
     float s0 = (plr * u_l * icl + u_r * icr + 2.0f * (plr - 1.0f) * frcp(G_avg - 1.0f)) * frcp(plr * icl + icr);
-    p_s = 0.5f * (p_l * __builtin_powf(1.0f + (G_l - 1.0f) * 0.5f * icl * (u_l - s0), 2.0f * G_l * frcp(G_l - 1.0f))
-                  + p_r * __builtin_powf(1.0f + (G_r - 1.0f) * 0.5f * icr * (s0 - u_r), 2.0f * G_r * frcp(G_r - 1.0f)));
+    p_s = 0.5f * (p_l * fpow(1.0f + (G_l - 1.0f) * 0.5f * icl * (u_l - s0), 2.0f * G_l * frcp(G_l - 1.0f))
+                  + p_r * fpow(1.0f + (G_r - 1.0f) * 0.5f * icr * (s0 - u_r), 2.0f * G_r * frcp(G_r - 1.0f)));
     p_s = fmaxf(p_s, 0.0f);
   } else {
     float a_l = 2.0f * frcp((G_l + 1.0f) * rl);

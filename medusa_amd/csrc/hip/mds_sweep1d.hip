@@ -51,7 +51,7 @@ MDS_DEV int bc_src(int c, int n, int lo, int hi, int &flip)
   if (c < 0) {
     if (c < -MDS_NG) return 0;
     if constexpr (GEN)
-      if (lo == MDS_BC_PROB || lo == MDS_BC_EXTRAP) { flip = 8; return c + MDS_NG; }
+      if (lo == MDS_BC_PROB || lo == MDS_BC_EXTRAP || lo == MDS_BC_DISK) { flip = 8; return c + MDS_NG; }
     if (lo == MDS_BC_REFLECT) { flip = 1; return -1 - c; }
     if (lo == MDS_BC_SPHERICAL_ORIGIN) { flip = 5; return -1 - c; }
     if (lo == MDS_BC_PERIODIC) return c + n;

@@ -16,6 +16,7 @@ NG = 2
 FP64, FP32_BULK = 0, 1
 EIO = -5                       # MDS_EIO: a table file missing or short
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC, BC_PROB = 0, 1, 2, 3, 4
+BC_DISK = 8
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
 RC_COUNT = 8
@@ -26,6 +27,7 @@ TILE_FUSE1D = 3
 TILE_GRID1D = 4
 TILE_HALO3D = 5
 ZG_COUNT = 24
+ZG_VOLUME, ZG_SCALE0 = 0, 12            # 1D spherical record fields (MDS_ZG_*)
 # 3D spherical per-(i,j) record (MDS_ZS_*)
 (ZS_VOLUME, ZS_AREA0, ZS_AREA1, ZS_AREA2, ZS_GCOV0) = range(5)
 ZS_GCOVF0, ZS_GCOVF1, ZS_GCON, ZS_SCALE0 = 7, 10, 13, 16

@@ -44,6 +44,7 @@ extern "C" {
 #define ORC_BC_PROB     4   /* ghost values from the problem's prob_bounds (bc.c:125-140) */
 #define ORC_BC_SPHERICAL_ORIGIN 5   /* theta: across the pole (phi + pi); inner radius: through the
                                        origin (theta -> pi - theta, phi + pi); decs.h:69 */
+#define ORC_BC_DISK     8   /* inner radius only: EXTRAP with v1 clamped to <= 0 (bc.c:863-878) */
 
 #define ORC_GEOM_CARTESIAN 0
 #define ORC_GEOM_SPHERICAL 1

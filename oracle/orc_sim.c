@@ -666,7 +666,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
       for (int gidx = 0; gidx < ORC_NG; gidx++) {
         int ig = side ? nd + gidx : -ORC_NG + gidx;   /* ghost index along d */
         int src = 0, flip = 0, extrap = 0, pole = 0;
-        int origin = 0, prob = 0;
+        int origin = 0, prob = 0, disk = 0;
         switch (code) {
           case ORC_BC_SPHERICAL_ORIGIN:
             /* theta boundary across the pole: mirror in j, phi + pi, flip v2 and v3
@@ -694,6 +694,12 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           case ORC_BC_PROB:
             prob = 1;   /* bc.c:125-140: prob_bounds(i, j, k, p) per ghost zone */
             break;
+          case ORC_BC_DISK:
+            /* bc.c:863-878, inner radius only: p(0) + i (p(1) - p(0)), then an
+             * outward v1 set to 0 */
+            if (d == 0 && !side) { extrap = 1; disk = 1; }
+            else src = side ? nd - 1 : 0;
+            break;
           default:
             src = side ? nd - 1 : 0;
         }
@@ -720,6 +726,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
           const double *q1 = &P[zid(s, a1[0], a1[1], a1[2]) * nv];
           for (int v = 0; v < nv; v++)
             dst[v] = side ? q0[v] + w * (q0[v] - q1[v]) : q0[v] + w * (q1[v] - q0[v]);
+          if (disk && dst[ORC_U1] > 0.0) dst[ORC_U1] = 0.0;
         } else {
           const double *sp = &P[zid(s, sidx[0], sidx[1], sidx[2]) * nv];
           for (int v = 0; v < nv; v++) dst[v] = sp[v];

@@ -17,6 +17,7 @@ BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC = 0, 1, 2, 3
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
 BC_PROB = 4
+BC_DISK = 8
 BC_SPHERICAL_ORIGIN = 5
 
 

@@ -48,3 +48,18 @@ def compare(dev, ref, gam, w=None):
         "p": l1rel(pressure(dev, gam), pressure(ref, gam), w),
         "v": l1rel(sp_dev, sp_ref, w),
     }
+
+
+def v_over_cs(dev, ref, gam, w=None, scale=None):
+    """Velocity difference on the sound-speed scale: sum |dv_phys| w / sum cs w,
+    with the reference's gamma-law sound speed and (spherical) the physical
+    components v^d scale_d.  Meaningful where the flow is near rest, where a
+    relative L1 of |v| divides by almost nothing."""
+    dv = 0.0
+    for d in range(3):
+        s = 1.0 if scale is None else scale[d]
+        dv = dv + (np.abs(dev[2 + d] - ref[2 + d]) * s) ** 2
+    dv = np.sqrt(dv)
+    cs = np.sqrt(gam * (gam - 1.0) * np.abs(ref[1]) / ref[0])
+    w = np.ones_like(cs) if w is None else np.broadcast_to(w, cs.shape)
+    return float(np.sum(dv * w) / np.sum(cs * w))

@@ -40,6 +40,30 @@ def test_extrap_is_exact_on_linear_profiles():
         assert np.array_equal(sim.zone(c), want), c
 
 
+def test_disk_extrapolates_and_keeps_inflow_only():
+    """DISK (bc.c:863-878): the inner ghosts are the linear extrapolation of
+    zones 0 and 1 (exact on dyadic linear data), except an outward (positive)
+    v^1, which becomes 0; an inward v^1 stays extrapolated.  The outer side
+    of the same code falls back to the default copy, as in the reference."""
+    n = 16
+    for slope, start in ((0.125, -0.5), (-0.25, -0.125), (0.25, 0.125)):
+        sim = po.Sim(_cfg(1, [n], [po.BC_DISK], [po.BC_OUTFLOW]))
+        prims = np.zeros((5, 1, 1, n))
+        prims[0, 0, 0] = 3.0 + 0.25 * np.arange(n)
+        prims[1, 0, 0] = 5.0 - 0.125 * np.arange(n)
+        prims[2, 0, 0] = start + slope * np.arange(n)
+        sim.set_prims(prims)
+        sim.fill_ghosts()
+        for c in (-2, -1):
+            want = prims[:, 0, 0, 0] + c * (prims[:, 0, 0, 1] - prims[:, 0, 0, 0])
+            if want[2] > 0.0:
+                want[2] = 0.0
+            assert np.array_equal(sim.zone(c), want), (slope, start, c, sim.zone(c), want)
+        assert np.array_equal(sim.zone(n), prims[:, 0, 0, n - 1])
+    # inflow kept, outflow clamped: both branches were met above
+    assert (-0.5 - 2 * 0.125) < 0 and (0.125 - 2 * 0.25) < 0 and (-0.125 + 0.25) > 0
+
+
 def test_prob_calls_cover_the_reference_ghost_zones():
     n = (6, 5)
     calls = []
@@ -186,6 +210,34 @@ def test_1d_extrap_and_prob_match_oracle(precision):
     t, dt, ist = ctx.clock()
     if precision == 0:
         assert ist == sim.istep and t == sim.t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_disk_inner_boundary_matches_oracle(precision):
+    """DISK at the inner radius of a log-r spherical column (accretion onto
+    the inner boundary: inflow through it, outflow clamped), and at the lower
+    x side of a 2D box"""
+    from medusa_amd import mds
+    n = 300
+    x0, x1 = math.log(1.0e6), math.log(1.0e9)
+    g = mds.GridBuilder(1, [n], [x0], [(x1 - x0) / n], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
+    prob = dict(grid=g, bc_lo=[mds.BC_DISK], bc_hi=[mds.BC_OUTFLOW], gam=4.0 / 3.0)
+    r = np.exp(x0 + (np.arange(n) + 0.5) * (x1 - x0) / n)
+    prims = np.zeros((5, 1, 1, n))
+    prims[0, 0, 0] = 1.0e8 * (r / 1.0e6) ** -1.5
+    prims[1, 0, 0] = 3.0e26 * (r / 1.0e6) ** -2.5
+    prims[2, 0, 0] = -0.3 * np.sqrt(1.0e18 * 1.0e6 / r) / r          # infall, coordinate velocity d ln r / dt
+    prims[2, 0, 0, 150:170] *= -1.0                                    # an outward-moving shell
+    print(f"[bc] 1D spherical DISK n={n} precision={precision}, 60 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob, prims, precision, 60)
+    _check(dev, ref, precision, (0, 1, 2), w=g.zone()[2:-2, 0], tag="1D DISK")
+    from medusa_amd import problems
+    g2, prob2, prims2 = problems.kh(64)
+    prob2 = dict(prob2, bc_lo=[mds.BC_DISK, mds.BC_PERIODIC], bc_hi=[mds.BC_OUTFLOW, mds.BC_PERIODIC])
+    print(f"[bc] 2D KH 64^2 DISK at x lower precision={precision}, 20 steps", flush=True)
+    ctx, sim, dev, ref = _pair(prob2, prims2, precision, 20)
+    _check(dev, ref, precision, (0, 1, 2, 3), tag="2D DISK")
 
 
 @pytest.mark.gpu

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import pyoracle as po
 from medusa_amd import mds, problems
-from parity_util import oracle_for, compare, l1rel
+from parity_util import oracle_for, compare, l1rel, v_over_cs
 
 pytestmark = pytest.mark.gpu
 
@@ -223,8 +223,10 @@ def test_kh_fp32_small_within_tolerance():
     dev = ctx.download()
     ref = sim.get_prims()
     err = compare(dev, ref, 5.0 / 3.0)
+    err["v/cs"] = v_over_cs(dev, ref, 5.0 / 3.0)
     say(f"  L1 {err}")
-    assert err["rho"] <= FP32_RHO_TOL
+    for k, v in err.items():
+        assert v <= FP32_RHO_TOL, (k, v)
 
 
 def test_sph_riemann_fp64_parity():
@@ -311,9 +313,13 @@ def test_rt3d_fp32_within_tolerance():
     g, prob, prims = problems.rt(16)
     say("RT 3D 16^3 FP32-bulk 10 steps")
     ctx, sim = _run_pair(prob, prims, mds.FP32_BULK, nsteps=10)
-    err = compare(ctx.download(), sim.get_prims(), 1.4)
+    dev, ref = ctx.download(), sim.get_prims()
+    err = compare(dev, ref, 1.4)
+    err["v/cs"] = v_over_cs(dev, ref, 1.4)
     say(f"  L1 {err}")
-    assert err["rho"] <= FP32_RHO_TOL
+    # the gas starts near rest (|v| <= 0.01 against cs near 1.5): its speed is
+    # checked on the sound-speed scale
+    assert err["rho"] <= FP32_RHO_TOL and err["p"] <= FP32_RHO_TOL and err["v/cs"] <= FP32_RHO_TOL, err
 
 
 @pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
@@ -444,8 +450,11 @@ def test_batched_columns_fp32_within_tolerance():
         sim = oracle_for(prob)
         sim.set_prims(prims)
         sim.run(max_steps=120)
-        w = g.zone()[2:-2, 0]
-        err = compare(ctx.download(k), sim.get_prims(), prob["gam"], w=w)
+        w = g.zone()[2:-2, mds.ZG_VOLUME]
+        dev, ref = ctx.download(k), sim.get_prims()
+        err = compare(dev, ref, prob["gam"], w=w)
+        err["v/cs"] = v_over_cs(dev, ref, prob["gam"], w=w, scale=(g.zone()[2:-2, mds.ZG_SCALE0], 1.0, 1.0))
         say(f"  column {k} n={g.n[0]}: L1 {err}")
         assert err["rho"] <= FP32_RHO_TOL
-        assert err["p"] <= 10 * FP32_RHO_TOL
+        assert err["p"] <= FP32_RHO_TOL
+        assert err["v/cs"] <= FP32_RHO_TOL
