@@ -455,12 +455,14 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       }
     }
     if (g->dendritic) {
-      if (!c->sph3 || !g->dj || !g->dk) return set_err(c, MDS_EINVAL, "problem %d: dendritic grids are built 3D spherical grids", p);
+      if (!c->sph3 || !g->dj || !g->dk)
+        return set_err(c, MDS_EINVAL, "problem %d: dendritic grids are built 2D or 3D spherical grids", p);
+      // (2D grids have one phi zone and no phi boundary)
       if ((P.bc_lo[0] != MDS_BC_REFLECT && P.bc_lo[0] != MDS_BC_SPHERICAL_ORIGIN) ||
           (P.bc_hi[0] != MDS_BC_OUTFLOW && P.bc_hi[0] != MDS_BC_REFLECT && P.bc_hi[0] != MDS_BC_PROB &&
            P.bc_hi[0] != MDS_BC_EXTRAP) ||
           P.bc_lo[1] != MDS_BC_SPHERICAL_ORIGIN || P.bc_hi[1] != MDS_BC_SPHERICAL_ORIGIN ||
-          P.bc_lo[2] != MDS_BC_PERIODIC || P.bc_hi[2] != MDS_BC_PERIODIC)
+          (g->ndim == 3 && (P.bc_lo[2] != MDS_BC_PERIODIC || P.bc_hi[2] != MDS_BC_PERIODIC)))
         return set_err(c, MDS_ENOTIMPL,
                        "problem %d: dendritic grids need REFLECT or SPHERICAL_ORIGIN inner / OUTFLOW, REFLECT, PROB "
                        "or EXTRAP outer radius, the pole in theta and periodic phi", p);

@@ -282,7 +282,8 @@ static int build_levels(mds_grid *g, const maps *m)
   const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2];
   const int S1 = n2 + 2 * NG;
   const double max_log2_aspect = 0.5;
-  if (n2 < 2 || (n2 & (n2 - 1)) || n3 < 4 || (n3 & (n3 - 1))) return MDS_EINVAL;
+  if (n2 < 2 || (n2 & (n2 - 1))) return MDS_EINVAL;
+  if (g->ndim == 3 && (n3 < 4 || (n3 & (n3 - 1)))) return MDS_EINVAL;
   if (g->startx[1] != -1.0 || fabs(g->dx[1] * n2 - 2.0) > 1.0e-14) return MDS_EINVAL;
   int *nj = calloc(n1 + 2 * NG, sizeof(int));
   int *nk = calloc((size_t)(n1 + 2 * NG) * S1, sizeof(int));
@@ -311,7 +312,10 @@ static int build_levels(mds_grid *g, const maps *m)
   for (int i = n1; i < n1 + NG; i++) NJ(i) = NJ(n1 - 1);
   for (int i = -NG; i < n1 + NG; i++) g->dj[i + NG] = n2 / NJ(i);
   const double dph = n3 * g->dx[2];
-  for (int i = 0; i < n1; i++) {
+  /* compute_dk; a 2D grid keeps every row at n3 = 1 zone (utils.c:311-316) */
+  for (int i = 0; i < n1 && g->ndim == 2; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = n3;
+  for (int i = 0; i < n1 && g->ndim == 3; i++) {
     const double r0 = rx(m, g->startx[0] + (i + 0.0) * g->dx[0]);
     const double rc = rx(m, g->startx[0] + (i + 0.5) * g->dx[0]);
     const double r1 = rx(m, g->startx[0] + (i + 1.0) * g->dx[0]);
@@ -545,7 +549,7 @@ int mds_grid_build(mds_grid *g)
   g->maxjlev = g->maxklev = 0;
   for (int t = 0; t < 2; t++) g->lalpha[t] = g->lbeta[t] = g->lGamma[t] = g->lrcoef[t] = NULL;
   if (g->geom == MDS_GEOM_SPHERICAL && g->rcoord == MDS_RCOORD_SINH && !(g->rtrans > 0.0)) return MDS_EINVAL;
-  if (g->dendritic && !(g->geom == MDS_GEOM_SPHERICAL && g->ndim == 3)) return MDS_EINVAL;
+  if (g->dendritic && !(g->geom == MDS_GEOM_SPHERICAL && g->ndim >= 2)) return MDS_EINVAL;
   int rc;
   if (g->geom == MDS_GEOM_SPHERICAL && g->ndim == 2) {
     /* 2D (r, theta): the 3D records of one phi zone spanning 2 pi, the

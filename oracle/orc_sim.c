@@ -262,10 +262,12 @@ static int build_levels(orc_sim *s, const sphmap *m)
   for (int i = -NG; i < 0; i++) NJ(i) = NJ(-i - 1);
   for (int i = n1; i < n1 + NG; i++) NJ(i) = NJ(n1 - 1);
   for (int i = -NG; i < n1 + NG; i++) s->dj[i + NG] = n2 / NJ(i);
-  /* compute_dk */
+  /* compute_dk; a 2D grid keeps every row at n3 = 1 zone (utils.c:311-316) */
   const int nk_min = 4, nk_max = n3;
   const double dph = n3 * c->dx[2];
-  for (int i = 0; i < n1; i++) {
+  for (int i = 0; i < n1 && c->ndim == 2; i++)
+    for (int j = 0; j < NJ(i); j++) NK(i, j) = n3;
+  for (int i = 0; i < n1 && c->ndim == 3; i++) {
     double r0 = r_of_x(m, c->startx[0] + (i + 0.0) * c->dx[0]);
     double rc = r_of_x(m, c->startx[0] + (i + 0.5) * c->dx[0]);
     double r1 = r_of_x(m, c->startx[0] + (i + 1.0) * c->dx[0]);
@@ -545,7 +547,10 @@ orc_sim *orc_create(const orc_config *cfg)
   s->c = *cfg;
   for (int d = 0; d < 3; d++) {
     if (d >= cfg->ndim) s->c.n[d] = 1;
-    s->g[d] = (d < cfg->ndim) ? ORC_NG : 0;
+    /* a 2D dendritic grid keeps phi ghosts around its one phi zone: the
+     * dendritic pencils (kslopes0/1) read one zone either side in phi */
+    const int dend2 = d == 2 && cfg->dendritic && cfg->geom == ORC_GEOM_SPHERICAL;
+    s->g[d] = (d < cfg->ndim || dend2) ? ORC_NG : 0;
     s->N[d] = s->c.n[d] + 2 * s->g[d];
   }
   s->nall = (long)s->N[0] * s->N[1] * s->N[2];
@@ -559,7 +564,7 @@ orc_sim *orc_create(const orc_config *cfg)
   s->shock = calloc(s->nall, 1);
   s->geom = calloc(s->nall, sizeof(orc_zone_geom));
   for (int d = 0; d < 3; d++) s->flux[d] = calloc(s->nall * s->nvars, sizeof(double));
-  if (cfg->dendritic && !(cfg->geom == ORC_GEOM_SPHERICAL && cfg->ndim == 3)) {
+  if (cfg->dendritic && !(cfg->geom == ORC_GEOM_SPHERICAL && cfg->ndim >= 2)) {
     orc_destroy(s);
     return NULL;
   }
@@ -1112,7 +1117,10 @@ static void riemann_store(orc_sim *s, int d, int isT, double *pl, double *pr, lo
 
 /* prim_to_fluxes3 on a dendritic grid, step.c:1159-1199 (radial faces),
  * 1201-1265 (theta faces at the finer k-level of the two rows), 1266-1421
- * (phi faces, polar-axis states in the last theta row) */
+ * (phi faces, polar-axis states in the last theta row); on a 2D grid
+ * (prim_to_fluxes2, step.c:659-764) the radial and theta faces only, with
+ * one phi zone per row, where kslopes0 is flat and the pencils reduce to the
+ * 2D prolongate0 / restrict0 (step.c:579-656) */
 static void sweep_dend(orc_sim *s, const double *P)
 {
   const orc_config *c = &s->c;
@@ -1178,7 +1186,7 @@ static void sweep_dend(orc_sim *s, const double *P)
         }
       }
   }
-  for (int i = 0; i < n1; i++)
+  for (int i = 0; i < n1 && c->ndim == 3; i++)
     for (int j = 0; j < njs(s, i); j++) {
       const int dk = dks(s, i, j), lk = lev_of(dk);
       const double *A = lev_arr(s, 0, 1, lk), *B = lev_arr(s, 1, 1, lk), *G = lev_arr(s, 2, 1, lk);
@@ -1206,7 +1214,8 @@ static void sweep_dend(orc_sim *s, const double *P)
 }
 
 /* push on a dendritic grid (step.c:374-470): radial fan-in over the finer
- * outer zones, theta faces with the half-area and two-face rules, phi faces */
+ * outer zones, theta faces with the half-area and two-face rules, phi faces
+ * (3D only: #if NDIM>2, step.c:435-438) */
 static void push_dend(orc_sim *s, double dtpush, const double *p, const double *ph, double *pf, int stage)
 {
   const orc_config *c = &s->c;
@@ -1254,8 +1263,10 @@ static void push_dend(orc_sim *s, double dtpush, const double *p, const double *
         } else {
           for (int vv = 0; vv < nv; vv++) divF[vv] -= F1[z * nv + vv];
         }
-        const long z2 = zid(s, ii, jj, kk + 1);
-        for (int vv = 0; vv < nv; vv++) divF[vv] += F2[z2 * nv + vv] - F2[z * nv + vv];
+        if (c->ndim == 3) {
+          const long z2 = zid(s, ii, jj, kk + 1);
+          for (int vv = 0; vv < nv; vv++) divF[vv] += F2[z2 * nv + vv] - F2[z * nv + vv];
+        }
         for (int vv = 0; vv < nv; vv++) {
           double ddt = (divF[vv] / gm->volume) - s->src[z * nv + vv];
           u[vv] = ((pfact * u[vv]) + (phfact * uh[vv])) - ((ffact * dtpush) * ddt);
