@@ -174,7 +174,10 @@ typedef struct mds_grid {
 } mds_grid;
 
 /* Build the reference geometry for a grid.  geom/rcoord/ndim/n/startx/dx
- * must be set on input; allocates the arrays (release with mds_grid_free). */
+ * must be set on input; allocates the arrays (release with mds_grid_free).
+ * 2D spherical (r, theta) grids: startx[2] and dx[2] are set to 0 and 2 pi
+ * on return (one phi zone spanning the circle, as the reference's
+ * init_coords sets the absent direction); a phi wedge is not expressible. */
 int mds_grid_build(mds_grid *g);
 void mds_grid_free(mds_grid *g);
 /* Build count independent grids (host threads); returns the lowest error code. */
@@ -189,12 +192,15 @@ typedef struct mds_problem {
   int nvel;                /* velocity components carried; 0 = auto: ndim, except FP64
                               spherical columns, which carry 3 (the connection terms give
                               the transverse components round-off sources); FP32-bulk
-                              spherical columns carry the radial component only */
+                              spherical columns carry the radial component only.  3D grids
+                              and 2D/3D spherical grids always carry 3: any other explicit
+                              value is rejected with MDS_EINVAL */
   const struct mds_eos_table *eos;  /* tabulated EOS (mds_eos.h), NULL = gamma law; with a table
                               the state carries Ye as a sixth variable (decs.h:646).
-                              Spherical columns only; every problem of a batch shares it */
+                              Spherical grids (1D columns, 2D and 3D); every problem of a
+                              batch shares it */
   int monopole;            /* Newtonian monopole self-gravity (gravity.c:145-498);
-                              spherical columns only */
+                              spherical grids (1D columns, 2D and 3D) */
 } mds_problem;
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */

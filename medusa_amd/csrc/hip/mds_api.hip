@@ -391,6 +391,10 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   };
   c->nv = auto_nv(probs[0]);
   if (c->ndim > 1 && nprob != 1) return set_err(c, MDS_ENOTIMPL, "batches are supported for 1D columns only");
+  // 3D grids and 2D/3D spherical grids evolve all three components: an
+  // explicit other count is an error, not something to override
+  if ((c->ndim == 3 || (c->sph && c->ndim >= 2)) && probs[0].nvel != 0 && probs[0].nvel != 3)
+    return set_err(c, MDS_EINVAL, "3D and 2D/3D spherical grids carry three velocity components (nvel 0 or 3)");
   if (c->ndim == 1 && c->nv == 2) c->nv = 3;
   if (c->ndim == 2 && c->nv < 2) c->nv = 2;
   if (c->ndim == 3) c->nv = 3;

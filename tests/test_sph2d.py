@@ -132,3 +132,54 @@ def test_sph2d_blast_matches_oracle(precision):
     if precision == 0:
         assert ist == sim.istep and abs(t / sim.t - 1) < 1e-13
         assert ctx.counters()[1] == sim.hlle_faces()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_table,precision", [(False, 0), (True, 0), (True, 1)])
+def test_sph2d_star_matches_oracle(use_table, precision):
+    """A polytropic star on a 2D (r, theta) grid with monopole gravity and
+    (optionally) the table EOS: the shell-mass kernels over one phi zone and
+    the table path on 2D records.  FP64: relative L1 <= 1e-12 in rho, u (and
+    Ye); FP32-bulk: <= 1e-5, velocity on the sound-speed scale <= 1e-5."""
+    from medusa_amd import mds
+    from test_dendritic import star_sound_speed
+    T = mds.EosTable() if use_table else None
+    n = (40, 16)
+    g = _grid(n)
+    prob = dict(_prob(g, gam=4.0 / 3.0), monopole=True, eos=T)
+    prims = mds.init_polytrope(g, table=T, sigma=0.01)
+    nsteps = 10 if precision == 0 else 20
+    print(f"[sph2d] star {n} table={use_table} precision={precision}, {nsteps} steps", flush=True)
+    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    m0 = ctx.totals()
+    ctx.run(nsteps, graph=True)
+    dev = ctx.download()
+    m1 = ctx.totals()
+    kw = dict(nvars=6, monopole=True)
+    if T is not None:
+        t = T.t
+        kw["eos_tab"] = po.eos_tab(dict(nr=t.nr, nt=t.nt, ny=t.ny, lr0=t.lr0, lr1=t.lr1, lt0=t.lt0, lt1=t.lt1,
+                                        ye0=t.ye0, ye1=t.ye1), T.data)
+    from parity_util import oracle_config
+    sim = po.Sim(oracle_config(prob, 0.5, **kw))
+    sim.set_prims(prims)
+    sim.run(max_steps=nsteps)
+    ref = sim.get_prims()
+    w = np.broadcast_to(g.zone()[2:-2, 2:-2, mds.ZS_VOLUME][None], ref[0].shape)
+    errs = {"rho": float(np.sum(np.abs(dev[0] - ref[0]) * w) / np.sum(np.abs(ref[0]) * w)),
+            "u": float(np.sum(np.abs(dev[1] - ref[1]) * w) / np.sum(np.abs(ref[1]) * w))}
+    if use_table:
+        errs["ye"] = float(np.sum(np.abs(dev[5] - ref[5]) * w) / np.sum(np.abs(ref[5]) * w))
+    cs = star_sound_speed(ref, g, T, 4.0 / 3.0, np.ones(ref[0].shape, bool))
+    sc = g.zone()[2:-2, 2:-2, mds.ZS_SCALE0:mds.ZS_SCALE0 + 3]
+    for d in range(2):
+        errs[f"v{d + 1}/cs"] = float(np.sum(np.abs(dev[2 + d] - ref[2 + d]) * sc[None, :, :, d] * w)
+                                     / np.sum(cs * w))
+    drift = abs(m1[0] - sim.totals()[0]) / m0[0]
+    print(f"[sph2d]   L1 {errs}; mass vs oracle {drift:.2e}", flush=True)
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+    assert drift <= (1e-13 if precision == 0 else 1e-6)
