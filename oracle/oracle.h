@@ -80,7 +80,8 @@ typedef struct {
   double dt_init_safety, dt_grow_safety, dtmin, dtmax, initial_dt;
   double g0[3];        /* fixed gravity (contravariant), gravity.c:46-64; all zero = none */
   int use_fixed_grav;
-  int use_monopole;    /* Newtonian monopole gravity, gravity.c:145-251, 432-498 */
+  int use_monopole;    /* 1: Newtonian monopole gravity, gravity.c:145-251, 432-498;
+                          2: GR effective monopole (GR_MONOPOLE), gravity.c:142-251, 266-375 */
   const struct orc_eos_tab *eos_tab;   /* tabulated EOS (NULL: gamma law); nvars 6 carries Ye */
   double rtrans;       /* ORC_RCOORD_SINH scale */
   int dendritic;       /* 3D spherical: dendritic angular coarsening (utils.c:181-590) */
@@ -161,6 +162,14 @@ long orc_hlle_face_count(const orc_sim *s);     /* transverse faces solved with 
 void orc_conserved_totals(const orc_sim *s, double out[2]);
 /* monopole gravity at the n1+1 radial faces from the current state (gravity.c:420-425) */
 void orc_monopole_edges(const orc_sim *s, double *grav);
+/* GR effective monopole state after the last call (use_monopole == 2): the
+ * TOV factor and mass at the n1+1 faces, the lapses at the n1 zone centroids
+ * and the n1+1 faces, the iteration count.  Returns 1 if the last call did
+ * not converge in 20 passes, -1 without GR monopole.  Any output may be NULL. */
+int orc_gr_state(const orc_sim *s, double *gtov, double *mr, double *lapse, double *lapse_edge, int *iter);
+/* GR effective monopole face gravity (n1+1 values) of the current state, after
+ * an EOS fill of it; updates the kept TOV factor like a stage does. */
+int orc_gr_edges(orc_sim *s, double *grav);
 /* dendritic levels: nj[n1], nk[n1*n2] (0 beyond nj[i]); returns maxjlev*256 + maxklev */
 int orc_dendritic_levels(const orc_sim *s, int *nj, int *nk);
 /* theta (dir 0) / phi (dir 1) moments of refinement level lev (length n + 2 NG), NULL if none */

@@ -70,6 +70,12 @@ struct orc_sim {
    * (alpha1s/beta1s/Gamma1s, alpha2s/..., metric_spherical_general.c:592-669) */
   double *lmom[3][2];
   unsigned char *live; /* zones the dendritic sweeps read: interior and ghosts */
+  /* GR effective monopole (use_monopole == 2): the TOV factor Gamma at the
+   * radial faces, kept between calls (the reference's static gtov, set once
+   * from the first state); the enclosed TOV mass, the lapses of the last
+   * call and its iteration count */
+  double *gtov, *gr_mr, *gr_lapse, *gr_lapse_edge;
+  int gtov_set, gr_iter, gr_fail;
 };
 
 /* DJS/DKS and the level-local zone counts (decs.h:569-590) */
@@ -582,6 +588,12 @@ orc_sim *orc_create(const orc_config *cfg)
       for (int j = -ORC_NG; j < njs(s, i) + ORC_NG; j++)
         for (int k = -ORC_NG; k < nks(s, i, j) + ORC_NG; k++) s->live[zid(s, i, j, k)] = 1;
   }
+  if (cfg->use_monopole == 2) {
+    s->gtov = calloc(s->c.n[0] + 1, sizeof(double));
+    s->gr_mr = calloc(s->c.n[0] + 1, sizeof(double));
+    s->gr_lapse = calloc(s->c.n[0], sizeof(double));
+    s->gr_lapse_edge = calloc(s->c.n[0] + 1, sizeof(double));
+  }
   s->t = 0.0;
   s->istep = 0;
   s->firstc = 1;
@@ -605,6 +617,7 @@ void orc_destroy(orc_sim *s)
     }
   }
   free(s->dj); free(s->dk); free(s->live);
+  free(s->gtov); free(s->gr_mr); free(s->gr_lapse); free(s->gr_lapse_edge);
   free(s);
 }
 
@@ -1404,14 +1417,158 @@ static void monopole_edges(const orc_sim *s, const double *P, double *grav)
   }
 }
 
+/* GR effective monopole (gravity.c:142-251 start and 266-375 finish with
+ * GR_MONOPOLE, no radiation, so the radiation pressure, energy and v.F
+ * shell terms are zero): volume averages of rho, P, u and v^1 over each
+ * shell (divided by the exact shell volume 4 pi/3 d(r^3)); the TOV mass and
+ * factor Gamma = sqrt(1 + v^2/c^2 - 2 G m / (r c^2)) at the faces,
+ * iterated until the outer mass changes by less than 1e-10 (at most 20
+ * passes; Gamma starts from the first call's Newtonian estimate and is kept
+ * between calls); the face gravity
+ *   g = -G (m + 4 pi r^3 P / c^2) / (r Gamma)^2 (rho + (u + P)/c^2) / rho dr/dx
+ * with face averages of neighbouring shells (the outer face uses the last
+ * shell); and the lapses exp(phi / c^2) of the potential integrated inwards
+ * from -G M / r at the outer face.  Returns 0, or 1 when 20 passes do not
+ * converge (the reference exits there). */
+#define CLIGHT 2.99792458e10    /* constants.h:10 */
+static int monopole_edges_gr(orc_sim *s, const double *P, double *grav)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], nv = s->nvars;
+  sphmap m;
+  sphmap_init(&m, c->rcoord, c->rtrans);
+  double *rhoavg = calloc(n1, sizeof(double)), *pgasavg = calloc(n1, sizeof(double));
+  double *pradavg = calloc(n1, sizeof(double)), *uavg = calloc(n1, sizeof(double));
+  double *vavg = calloc(n1, sizeof(double)), *eradavg = calloc(n1, sizeof(double));
+  double *vdF = calloc(n1, sizeof(double)), *redge = calloc(n1 + 1, sizeof(double));
+  double *gtov = s->gtov, *mr = s->gr_mr;
+  for (int ii = 0; ii <= n1; ii++) redge[ii] = r_of_x(&m, fabs(ii * c->dx[0] + c->startx[0]));
+  for (int ii = 0; ii < n1; ii++)
+    for (int k = 0; k < c->n[2]; k++)
+      for (int j = 0; j < c->n[1]; j++) {
+        if (!zone_valid(s, ii, j, k)) continue;
+        const long z = zid(s, ii, j, k);
+        const double vol = s->geom[z].volume;
+        rhoavg[ii] += vol * P[z * nv + ORC_RHO];
+        pgasavg[ii] += vol * s->eos[z * NEOS + E_PRESS];
+        pradavg[ii] += vol * 0.0;
+        uavg[ii] += vol * P[z * nv + ORC_UU];
+        vavg[ii] += vol * P[z * nv + ORC_U1];
+        eradavg[ii] += vol * 0.0;
+        vdF[ii] += vol * 0.0;
+      }
+  for (int ii = 0; ii < n1; ii++) {
+    const double Vshell = 1.0 / (((4.0 * M_PI) / 3.0) * (pow(redge[ii + 1], 3) - pow(redge[ii], 3)));
+    rhoavg[ii] *= Vshell;
+    pgasavg[ii] *= Vshell;
+    pradavg[ii] *= Vshell;
+    uavg[ii] *= Vshell;
+    vavg[ii] *= Vshell;
+    eradavg[ii] *= Vshell;
+    vdF[ii] *= Vshell;
+  }
+  const double c2 = OSQR(CLIGHT);
+  mr[0] = 0.0;
+  grav[0] = 0.0;
+  double mstar = 0.0, mlast = 0.0;
+  if (!s->gtov_set) {
+    gtov[0] = 1.0;
+    for (int ii = 1; ii <= n1; ii++) {
+      const double Vshell = ((4.0 * M_PI) / 3.0) * (pow(redge[ii], 3) - pow(redge[ii - 1], 3));
+      mstar += Vshell * rhoavg[ii - 1];
+      gtov[ii] = sqrt(1.0 - ((2.0 * GNEWT) * mstar) / (c2 * redge[ii]));
+    }
+    s->gtov_set = 1;
+  }
+  int iter = 0, fail = 0;
+  const double *be = s->mom[1][0] + ORC_NG, *ga = s->mom[2][0] + ORC_NG;
+  while (1) {
+    mstar = 0.0;
+    for (int ii = 1; ii <= n1; ii++) {
+      const double xl = c->startx[0] + ii * c->dx[0];
+      double A = (gtov[ii] - gtov[ii - 1]) / c->dx[0];
+      double B = gtov[ii - 1] - A * xl;
+      double gtov_avg = (A * be[ii]) / ga[ii] + B;
+      const double Vshell = ((4.0 * M_PI) / 3.0) * (pow(redge[ii], 3) - pow(redge[ii - 1], 3));
+      const double vedge = (ii < n1) ? 0.5 * (vavg[ii - 1] + vavg[ii]) : vavg[ii - 1];
+      const double w = (rhoavg[ii - 1] + uavg[ii - 1] / c2) + eradavg[ii - 1];
+      gtov[ii] = sqrt((1.0 + pow(vedge / CLIGHT, 2)) -
+                      ((2.0 * GNEWT) * (mr[ii - 1] + Vshell * ((w * gtov_avg) + vdF[ii - 1]))) / (redge[ii] * c2));
+      A = (gtov[ii] - gtov[ii - 1]) / c->dx[0];
+      B = gtov[ii - 1] - A * xl;
+      gtov_avg = (A * be[ii]) / ga[ii] + B;
+      mr[ii] = mr[ii - 1] + Vshell * ((w * gtov_avg) + vdF[ii - 1]);
+      mstar += Vshell * rhoavg[ii - 1];
+    }
+    if (iter > 0 && fabs(mlast - mr[n1]) / mr[n1] < 1.0e-10) break;
+    mlast = mr[n1];
+    iter++;
+    if (iter > 20) { fail = 1; break; }
+  }
+  s->gr_iter = iter;
+  s->gr_fail = fail;
+  for (int ii = 1; ii < n1; ii++) {
+    const double pgasedge = 0.5 * (pgasavg[ii - 1] + pgasavg[ii]);
+    const double pradedge = 0.5 * (pradavg[ii - 1] + pradavg[ii]);
+    const double rhoedge = 0.5 * (rhoavg[ii - 1] + rhoavg[ii]);
+    const double uedge = 0.5 * (uavg[ii - 1] + uavg[ii]);
+    grav[ii] = ((((-GNEWT * (mr[ii] + (((4.0 * M_PI) * pow(redge[ii], 3)) * (pgasedge + pradedge)) / c2)) /
+                  pow(redge[ii] * gtov[ii], 2)) * (rhoedge + (uedge + pgasedge) / c2)) / rhoedge) *
+               dr_dx(&m, c->startx[0] + ii * c->dx[0]);
+  }
+  grav[n1] = ((((-GNEWT * (mr[n1] + (((4.0 * M_PI) * pow(redge[n1], 3)) * (pgasavg[n1 - 1] + pradavg[n1 - 1])) / c2)) /
+                pow(redge[n1] * gtov[n1], 2)) * (rhoavg[n1 - 1] + (uavg[n1 - 1] + pgasavg[n1 - 1]) / c2)) /
+              rhoavg[n1 - 1]) * dr_dx(&m, c->startx[0] + n1 * c->dx[0]);
+  /* the potential, integrated inwards from the outer face, and the lapses */
+  double *le = s->gr_lapse_edge, *lc = s->gr_lapse;
+  le[n1] = (-GNEWT * mstar) / redge[n1];
+  for (int ii = n1 - 1; ii >= 0; ii--) le[ii] = le[ii + 1] + (0.5 * (grav[ii] + grav[ii + 1])) * c->dx[0];
+  for (int ii = 0; ii < n1; ii++) {
+    const double xl = c->startx[0] + ii * c->dx[0];
+    const double A = (le[ii + 1] - le[ii]) / c->dx[0];
+    const double B = le[ii] - A * xl;
+    lc[ii] = (A * be[ii]) / ga[ii] + B;
+  }
+  for (int ii = 0; ii < n1; ii++) {
+    lc[ii] = exp(lc[ii] / c2);
+    le[ii] = exp(le[ii] / c2);
+  }
+  le[n1] = exp(le[n1] / c2);
+  free(rhoavg); free(pgasavg); free(pradavg); free(uavg); free(vavg); free(eradavg); free(vdF); free(redge);
+  return fail;
+}
+
 void orc_monopole_edges(const orc_sim *s, double *grav) { monopole_edges(s, s->p, grav); }
+
+static void update_eos(orc_sim *s, const double *P);
+int orc_gr_edges(orc_sim *s, double *grav)
+{
+  if (!s->gtov) return -1;
+  update_eos(s, s->p);
+  return monopole_edges_gr(s, s->p, grav);
+}
+
+int orc_gr_state(const orc_sim *s, double *gtov, double *mr, double *lapse, double *lapse_edge, int *iter)
+{
+  if (!s->gtov) return -1;
+  const int n1 = s->c.n[0];
+  for (int ii = 0; ii <= n1; ii++) {
+    if (gtov) gtov[ii] = s->gtov[ii];
+    if (mr) mr[ii] = s->gr_mr[ii];
+    if (lapse_edge) lapse_edge[ii] = s->gr_lapse_edge[ii];
+    if (lapse && ii < n1) lapse[ii] = s->gr_lapse[ii];
+  }
+  if (iter) *iter = s->gr_iter;
+  return s->gr_fail;
+}
 
 static void monopole(orc_sim *s, const double *P)
 {
   const orc_config *c = &s->c;
   const int n1 = c->n[0], nv = s->nvars;
   double *grav = calloc(n1 + 1, sizeof(double));
-  monopole_edges(s, P, grav);
+  if (c->use_monopole == 2) monopole_edges_gr(s, P, grav);
+  else monopole_edges(s, P, grav);
   for (int ii = 0; ii < n1; ii++) {
     double xl = c->startx[0] + ii * c->dx[0];
     double A = (grav[ii + 1] - grav[ii]) / c->dx[0];

@@ -117,6 +117,10 @@ def _load(fast=False):
     lib.orc_dt_source_control.restype = ctypes.c_double
     lib.orc_monopole_edges.argtypes = [ctypes.c_void_p, dp]
     lib.orc_monopole_edges.restype = None
+    lib.orc_gr_state.argtypes = [ctypes.c_void_p, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_int)]
+    lib.orc_gr_state.restype = ctypes.c_int
+    lib.orc_gr_edges.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_gr_edges.restype = ctypes.c_int
     lib.orc_eos_walk.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, dp]
     lib.orc_eos_walk.restype = ctypes.c_int
@@ -213,7 +217,8 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
     c.dtmax = dtmax
     c.initial_dt = initial_dt
     c.use_fixed_grav = int(any(v != 0.0 for v in g0))
-    c.use_monopole = int(bool(monopole))
+    # monopole: False/0 none, True/1 Newtonian, 2 GR effective potential
+    c.use_monopole = 2 if (monopole == 2 and monopole is not True) else int(bool(monopole))
     c.rtrans = float(rtrans)
     c.dendritic = int(bool(dendritic))
     if eos_tab is not None:
@@ -275,6 +280,24 @@ class Sim:
         out = np.zeros(self.n[0] + 1)
         self.lib.orc_monopole_edges(self.h, ptr(out))
         return out
+
+    def gr_edges(self):
+        """GR effective monopole face gravity of the current state (updates the kept TOV factor)."""
+        out = np.zeros(self.n[0] + 1)
+        if self.lib.orc_gr_edges(self.h, ptr(out)) < 0:
+            raise ValueError("no GR monopole in this configuration")
+        return out
+
+    def gr_state(self):
+        """GR effective monopole after the last stage: dict of gtov, mr (faces),
+        lapse (zones), lapse_edge (faces), iter, fail."""
+        n1 = self.n[0]
+        g, m, lc, le = np.zeros(n1 + 1), np.zeros(n1 + 1), np.zeros(n1), np.zeros(n1 + 1)
+        it = ctypes.c_int(0)
+        rc = self.lib.orc_gr_state(self.h, ptr(g), ptr(m), ptr(lc), ptr(le), ctypes.byref(it))
+        if rc < 0:
+            raise ValueError("no GR monopole in this configuration")
+        return dict(gtov=g, mr=m, lapse=lc, lapse_edge=le, iter=it.value, fail=rc)
 
     def run(self, tmax=1.0e300, max_steps=2 ** 31 - 1):
         return self.lib.orc_run(self.h, tmax, max_steps)
