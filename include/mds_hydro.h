@@ -183,6 +183,10 @@ void mds_grid_free(mds_grid *g);
 /* Build count independent grids (host threads); returns the lowest error code. */
 int mds_grid_build_batch(mds_grid *g, int count);
 
+/* monopole gravity (mds_problem.monopole) */
+#define MDS_MONOPOLE_NEWTON 1
+#define MDS_MONOPOLE_GR     2
+
 /* Physics and boundary description of one problem. */
 typedef struct mds_problem {
   const mds_grid *grid;    /* copied into the context at create */
@@ -199,8 +203,11 @@ typedef struct mds_problem {
                               the state carries Ye as a sixth variable (decs.h:646).
                               Spherical grids (1D columns, 2D and 3D); every problem of a
                               batch shares it */
-  int monopole;            /* Newtonian monopole self-gravity (gravity.c:145-498);
-                              spherical grids (1D columns, 2D and 3D) */
+  int monopole;            /* monopole self-gravity, spherical grids (1D columns, 2D and 3D):
+                              0 none; MDS_MONOPOLE_NEWTON (gravity.c:145-498);
+                              MDS_MONOPOLE_GR, the GR effective monopole of the
+                              reference's GR_MONOPOLE build (gravity.c:142-375: TOV
+                              mass and factor, pressure and inertia terms, lapses) */
 } mds_problem;
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */
@@ -335,6 +342,16 @@ typedef struct {
 } mds_health;
 
 int mds_get_health(mds_ctx *ctx, int prob, mds_health *out);
+
+/* State of the GR effective monopole of problem prob (MDS_MONOPOLE_GR) as
+ * the last stage left it, the reference's gtov, mr, gr_lapse and
+ * gr_lapse_edge (gravity.c:142-375; geom[].lapse[0..1], dumped by
+ * io.c:198): n1 + 1 face values of the TOV factor Gamma, of the TOV mass and
+ * of the lapse, and n1 zone-centred lapses (any pointer may be NULL).
+ * passes: passes of the last call's iteration; nonconverged: calls whose 21
+ * passes did not converge (the reference exits there, gravity.c:320-332). */
+int mds_get_gr_state(mds_ctx *ctx, int prob, double *gtov, double *mr, double *lapse_edge,
+                     double *lapse, int *passes, int *nonconverged);
 
 /* Diagnostic counters since mds_update_eos: [0] radial faces tagged as shocks
  * by HLLC, [1] transverse faces solved with HLLE. */

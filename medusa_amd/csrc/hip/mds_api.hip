@@ -19,7 +19,8 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
                                          const void *sp, const void *sph_, int stage, hipStream_t st);
 extern "C" hipError_t mds_launch_slab_fill(int fp32, const DevProb *probs, int nprob, long long items,
                                            void *const *state, const DevEos *eos, int tab, int nv, hipStream_t st);
-extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
+extern "C" hipError_t mds_launch_monopole(int fp32, int mono, int tab, int nprob, const DevProb *prob,
+                                          const DevClock *clock, const DevEos *eos, const void *st, double *grav,
                                           hipStream_t s);
 hipError_t mds_eos_to_device(const mds_eos_table *t, DevEos *d, void **alloc);
 extern "C" hipError_t mds_launch_sph3d(const Sph3Launch *L, const DevProb *pb, DevClock *clock,
@@ -50,8 +51,11 @@ struct mds_ctx {
   int tab = 0;                     // tabulated EOS (Ye carried)
   DevEos deos{};
   void *eos_alloc = nullptr;
-  int mono = 0;                    // monopole gravity
+  int mono = 0;                    // monopole gravity (MDS_MONOPOLE_NEWTON or MDS_MONOPOLE_GR)
   double *dgrav = nullptr;         // monopole field at zone centres (FP64), packed like the state
+  double *dgr = nullptr;           // GR: every problem's block of MDS_GR_COUNT x (n1 + 1) doubles
+  size_t gr_doubles = 0;
+  int *dgrst = nullptr;            // GR: 4 ints of state per problem
   int sph3 = 0;                    // 3D spherical grid (direction-split stage)
   Sph3Launch s3{};                 // its work buffers
   DevProb *dprob = nullptr;
@@ -300,6 +304,8 @@ extern "C" int mds_destroy(mds_ctx *c)
   for (int k = 0; k < 6; k++) { if (c->p[k]) hipFree(c->p[k]); if (c->ph[k]) hipFree(c->ph[k]); }
   if (c->eos_alloc) hipFree(c->eos_alloc);
   if (c->dgrav) hipFree(c->dgrav);
+  if (c->dgr) hipFree(c->dgr);
+  if (c->dgrst) hipFree(c->dgrst);
   for (int d = 0; d < 3; d++) if (c->s3.F[d]) hipFree(c->s3.F[d]);
   if (c->s3.P) hipFree(c->s3.P);
   if (c->s3.G) hipFree(c->s3.G);
@@ -407,9 +413,12 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->sph && !c->sph3 && c->precision == MDS_FP32_BULK && c->nv != 1)
     return set_err(c, MDS_ENOTIMPL, "FP32-bulk spherical columns carry the radial velocity only");
   c->tab = probs[0].eos != nullptr;
-  c->mono = probs[0].monopole != 0;
+  c->mono = probs[0].monopole;
+  for (int p = 0; p < nprob; p++)
+    if (probs[p].monopole < 0 || probs[p].monopole > MDS_MONOPOLE_GR)
+      return set_err(c, MDS_EINVAL, "problem %d: monopole must be 0, MDS_MONOPOLE_NEWTON or MDS_MONOPOLE_GR", p);
   for (int p = 1; p < nprob; p++)
-    if ((probs[p].eos != probs[0].eos) || ((probs[p].monopole != 0) != (c->mono != 0)))
+    if ((probs[p].eos != probs[0].eos) || (probs[p].monopole != c->mono))
       return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same EOS and gravity", p);
   if ((c->tab || c->mono) && !c->sph)
     return set_err(c, MDS_ENOTIMPL, "tabulated EOS and monopole gravity are supported on spherical grids");
@@ -506,6 +515,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     d.ivolume = g->volume != 0.0 ? 1.0 / g->volume : 0.0;
     for (int k = 0; k < 3; k++) d.idx[k] = 1.0 / d.dx[k];
     d.order = rp->hydro_interp_order;
+    d.mono = c->mono;
     // one FP32 coefficient set for the whole column when the zones' sets
     // agree to 1e-5 (the moments of log-r and uniform grids differ only by
     // scale and shift; the spread left is the rounding of the FP64 moments,
@@ -648,6 +658,25 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     HIPCHK(c, hipMalloc(&c->dgrav, sizeof(double) * (size_t)c->nzone));
     HIPCHK(c, hipMemset(c->dgrav, 0, sizeof(double) * (size_t)c->nzone));
   }
+  if (c->mono == MDS_MONOPOLE_GR) {
+    // per problem: the host-made shell geometry (first four arrays), the
+    // rest zero; Gamma is set by the first call
+    std::vector<size_t> goff(nprob + 1, 0);
+    for (int p = 0; p < nprob; p++) goff[p + 1] = goff[p] + (size_t)MDS_GR_COUNT * (c->hprob[p].n[0] + 1);
+    c->gr_doubles = goff[nprob];
+    std::vector<double> h(c->gr_doubles, 0.0);
+    for (int p = 0; p < nprob; p++)
+      if (mds_gr_shell_geometry(probs[p].grid, h.data() + goff[p]) != MDS_OK)
+        return set_err(c, MDS_EINVAL, "problem %d: GR monopole shell geometry", p);
+    HIPCHK(c, hipMalloc(&c->dgr, sizeof(double) * c->gr_doubles));
+    HIPCHK(c, hipMemcpy(c->dgr, h.data(), sizeof(double) * c->gr_doubles, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->dgrst, sizeof(int) * 4 * nprob));
+    HIPCHK(c, hipMemset(c->dgrst, 0, sizeof(int) * 4 * nprob));
+    for (int p = 0; p < nprob; p++) {
+      c->hprob[p].gr = c->dgr + goff[p];
+      c->hprob[p].grst = c->dgrst + 4 * p;
+    }
+  }
   if (c->sph3) {
     const DevProb &h = c->hprob[0];
     Sph3Launch &L = c->s3;
@@ -677,7 +706,9 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     if (c->mono) {
       HIPCHK(c, hipMalloc(&L.ms, sizeof(double) * h.n[0]));
       L.npart = (int)(((long long)h.n[1] * h.n[2] + 4 * MDS_SHELL_ROWS - 1) / (4 * MDS_SHELL_ROWS));
-      HIPCHK(c, hipMalloc(&L.part, sizeof(double) * (size_t)L.npart * h.n[0]));
+      // shell-mass parts, or the four GR shell sums' parts
+      const size_t nq = c->mono == MDS_MONOPOLE_GR ? 4 : 1;
+      HIPCHK(c, hipMalloc(&L.part, sizeof(double) * nq * (size_t)L.npart * h.n[0]));
       HIPCHK(c, hipMalloc(&L.grav, sizeof(double) * h.n[0]));
     }
     const mds_grid *g = probs[0].grid;
@@ -894,11 +925,15 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
     if (fp32) {
       StateArrays<float> sp = arrays<float>(c->p), sh = arrays<float>(c->ph);
       // monopole field of the stage input state (gravity.c:145-251 at p_to_phys_src_start)
-      if (c->mono) HIPCHK(c, mds_launch_monopole(1, c->nprob, c->dprob, stage == 0 ? &sp : &sh, c->dgrav, c->st));
+      if (c->mono)
+        HIPCHK(c, mds_launch_monopole(1, c->mono, c->tab, c->nprob, c->dprob, c->dclock, &c->deos,
+                                      stage == 0 ? &sp : &sh, c->dgrav, c->st));
       HIPCHK(c, mds_launch_sweep1d(1, c->nv, c->sph, c->tab, c->mono, &a, &sp, &sh, stage, c->st));
     } else {
       StateArrays<double> sp = arrays<double>(c->p), sh = arrays<double>(c->ph);
-      if (c->mono) HIPCHK(c, mds_launch_monopole(0, c->nprob, c->dprob, stage == 0 ? &sp : &sh, c->dgrav, c->st));
+      if (c->mono)
+        HIPCHK(c, mds_launch_monopole(0, c->mono, c->tab, c->nprob, c->dprob, c->dclock, &c->deos,
+                                      stage == 0 ? &sp : &sh, c->dgrav, c->st));
       HIPCHK(c, mds_launch_sweep1d(0, c->nv, c->sph, c->tab, c->mono, &a, &sp, &sh, stage, c->st));
     }
   } else if (c->sph3) {
@@ -1242,6 +1277,7 @@ extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
 struct Snapshot {
   std::vector<void *> buf;
   void *clock = nullptr, *count = nullptr, *flags = nullptr;
+  void *gr = nullptr, *grst = nullptr;     // GR monopole blocks (Gamma warm start) and state
 };
 
 static void snap_free(Snapshot &s)
@@ -1251,7 +1287,9 @@ static void snap_free(Snapshot &s)
   if (s.clock) hipFree(s.clock);
   if (s.count) hipFree(s.count);
   if (s.flags) hipFree(s.flags);
-  s.clock = s.count = s.flags = nullptr;
+  if (s.gr) hipFree(s.gr);
+  if (s.grst) hipFree(s.grst);
+  s.clock = s.count = s.flags = s.gr = s.grst = nullptr;
 }
 
 static int snap_alloc(mds_ctx *c, Snapshot &s)
@@ -1265,6 +1303,10 @@ static int snap_alloc(mds_ctx *c, Snapshot &s)
   HIPCHK(c, hipMalloc(&s.clock, sizeof(DevClock) * c->nprob));
   HIPCHK(c, hipMalloc(&s.count, 4 * sizeof(unsigned long long)));
   if (c->dflags) HIPCHK(c, hipMalloc(&s.flags, (size_t)c->nzone));
+  if (c->dgr) {
+    HIPCHK(c, hipMalloc(&s.gr, sizeof(double) * c->gr_doubles));
+    HIPCHK(c, hipMalloc(&s.grst, sizeof(int) * 4 * c->nprob));
+  }
   return MDS_OK;
 }
 
@@ -1286,6 +1328,12 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
   if (c->dflags)
     HIPCHK(c, hipMemcpyAsync(save ? s.flags : (void *)c->dflags, save ? (void *)c->dflags : s.flags,
                              (size_t)c->nzone, hipMemcpyDeviceToDevice, c->st));
+  if (c->dgr) {
+    HIPCHK(c, hipMemcpyAsync(save ? s.gr : (void *)c->dgr, save ? (void *)c->dgr : s.gr,
+                             sizeof(double) * c->gr_doubles, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(save ? s.grst : (void *)c->dgrst, save ? (void *)c->dgrst : s.grst,
+                             sizeof(int) * 4 * c->nprob, hipMemcpyDeviceToDevice, c->st));
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   return MDS_OK;
 }
@@ -1382,6 +1430,27 @@ extern "C" int mds_get_health(mds_ctx *c, int prob, mds_health *out)
   out->eos_off_table = (long long)ck.noff;
   // the running CFL minimum of the step being taken: a NaN limit never wins
   // the bitwise minimum, so a NaN state shows here only through bad_state
+  return MDS_OK;
+}
+
+extern "C" int mds_get_gr_state(mds_ctx *c, int prob, double *gtov, double *mr, double *lapse_edge,
+                                double *lapse, int *passes, int *nonconverged)
+{
+  if (!c || prob < 0 || prob >= c->nprob) return MDS_EINVAL;
+  if (c->mono != MDS_MONOPOLE_GR) return set_err(c, MDS_EINVAL, "no GR monopole in this context");
+  const DevProb &pb = c->hprob[prob];
+  const size_t N = (size_t)pb.n[0] + 1;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<double> h((size_t)MDS_GR_COUNT * N);
+  int st[4];
+  HIPCHK(c, hipMemcpy(h.data(), pb.gr, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(st, pb.grst, sizeof(st), hipMemcpyDeviceToHost));
+  if (gtov) memcpy(gtov, &h[MDS_GR_GTOV * N], sizeof(double) * N);
+  if (mr) memcpy(mr, &h[MDS_GR_MR * N], sizeof(double) * N);
+  if (lapse_edge) memcpy(lapse_edge, &h[MDS_GR_LAPSE_E * N], sizeof(double) * N);
+  if (lapse) memcpy(lapse, &h[MDS_GR_LAPSE * N], sizeof(double) * (N - 1));
+  if (passes) *passes = st[1];
+  if (nonconverged) *nonconverged = st[2];
   return MDS_OK;
 }
 

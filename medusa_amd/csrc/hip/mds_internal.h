@@ -73,9 +73,34 @@ struct DevProb {
   int has_slab;
   double *gs[3][2];
   int order;               // hydro_interp_order (3 para_recon, 2 lin_recon, 1 constant)
+  // monopole gravity: 0 none, MDS_MONOPOLE_NEWTON, MDS_MONOPOLE_GR.  GR: gr
+  // is this problem's block of MDS_GR_COUNT arrays of n[0] + 1 doubles, grst
+  // its state ([0] Gamma set, [1] passes of the last call, [2] calls that did
+  // not converge)
+  int mono;
+  double *gr;
+  int *grst;
+};
+
+// arrays of the GR effective monopole block of a problem (n[0] + 1 doubles each)
+enum {
+  MDS_GR_REDGE = 0,        // face radius                   (host)
+  MDS_GR_R34,              // (4 pi) r^3 at the faces       (host)
+  MDS_GR_DRDX,             // dr/dx at the faces            (host)
+  MDS_GR_VSH,              // shell volume 4 pi/3 d(r^3)    (host)
+  MDS_GR_GTOV,             // TOV factor Gamma at the faces (kept between calls)
+  MDS_GR_MR,               // TOV mass at the faces
+  MDS_GR_LAPSE_E,          // lapse at the faces
+  MDS_GR_LAPSE,            // lapse at the zone centres
+  MDS_GR_GFACE,            // face gravity
+  MDS_GR_RHO, MDS_GR_P, MDS_GR_U, MDS_GR_V,   // shell sums, then averages
+  MDS_GR_COUNT
 };
 
 #define MDS_SLAB_NV 8
+
+// host: the radial shell geometry of the GR effective monopole (mds_grid.c)
+extern "C" int mds_gr_shell_geometry(const mds_grid *g, double *out);
 
 // transverse positions of a direction-d ghost layer, and the index of the
 // zone (i, j, k) among them (box positions, natural order)

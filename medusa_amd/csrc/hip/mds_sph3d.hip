@@ -34,6 +34,7 @@
 #include "mds_internal.h"
 #include "mds_physics.h"
 #include "mds_eos.h"
+#include "mds_gravity.h"
 
 namespace {
 
@@ -760,6 +761,60 @@ __global__ __launch_bounds__(1024) void sph3_shell_field(Sph3Args a, const doubl
   }
 }
 
+// GR effective monopole (gravity.c:142-375 with GR_MONOPOLE): the shell sums
+// V rho, V P, V u and V v^1 in the same two fixed-order passes as the shell
+// masses (part[q][y][i], q = rho, P, u, v), P from the stage's table EOS
+// arrays or the gamma law; then one block reduces the parts of each shell
+// into the problem's GR block and evaluates the field (mds_gravity.h).
+template <class R, int TAB>
+__global__ __launch_bounds__(256) void sph3_shell_part_gr(Sph3Args a, StateArrays<R> in, double *part, int npart)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int r0 = (blockIdx.y * 4 + w) * MDS_SHELL_ROWS;
+  const int r1 = min(r0 + MDS_SHELL_ROWS, n2 * n3);
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n1) {
+    const int nj = nj_of(pb, i);
+#pragma unroll 4
+    for (int r = r0; r < r1; r++) {
+      const int j = r % n2, k = r / n2;
+      const long long z = zid3(pb, i, j, k);
+      const double v = rec3(pb, i, j)[MDS_ZS_VOLUME];
+      const double u = (double)in.v[1][z];
+      const double press = TAB ? (double)((const R *)a.P)[z] : (pb.gam - 1.0) * u;
+      const bool zone = j < nj && k < nk_of(pb, i, j);
+      s[0] += zone ? v * (double)in.v[0][z] : 0.0;
+      s[1] += zone ? v * press : 0.0;
+      s[2] += zone ? v * u : 0.0;
+      s[3] += zone ? v * (double)in.v[2][z] : 0.0;
+    }
+  }
+  __shared__ double sh[4][4][64];
+  for (int q = 0; q < 4; q++) sh[q][w][lane] = s[q];
+  __syncthreads();
+  if (w == 0 && i < n1)
+    for (int q = 0; q < 4; q++)
+      part[((size_t)q * npart + blockIdx.y) * n1 + i] = ((sh[q][0][lane] + sh[q][1][lane]) + sh[q][2][lane]) + sh[q][3][lane];
+}
+
+__global__ __launch_bounds__(1024) void sph3_shell_field_gr(Sph3Args a, const double *part, int npart, double *grav)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0];
+  double *sums = pb.gr + (size_t)MDS_GR_RHO * (n1 + 1);     // rho, P, u, v arrays follow each other
+  for (int i = threadIdx.x; i < n1; i += blockDim.x)
+    for (int q = 0; q < 4; q++) {
+      double m = 0.0;
+      for (int c = 0; c < npart; c++) m += part[((size_t)q * npart + c) * n1 + i];
+      sums[(size_t)q * (n1 + 1) + i] = m;
+    }
+  __syncthreads();
+  gr_monopole_block<1024>(pb, grav);
+}
+
 // ---------------------------------------------------------------------------
 // face fluxes in direction D (on dendritic grids, the faces whose 4-zone
 // pencil stays on one level).  Each thread reconstructs one zone (the PPM of
@@ -1323,9 +1378,16 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
   if (stage == 0)
     hipLaunchKernelGGL(sph3_reset_flags, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, L.flags, nz);
   if (L.tab) hipLaunchKernelGGL(sph3_eos_kernel<R>, dim3((unsigned)((nz + 255) / 256)), blk, 0, st, a, in);
-  if (L.mono) {
-    hipLaunchKernelGGL(sph3_shell_part<R>, dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)L.npart), blk, 0, s2, a,
-                       in, L.part);
+  const dim3 pgrid((unsigned)((pb.n[0] + 63) / 64), (unsigned)L.npart);
+  if (L.mono == MDS_MONOPOLE_GR) {
+    // the pressure sums read the table EOS arrays of this stage: after the
+    // EOS kernel, on its stream
+    hipStream_t sg = L.tab ? st : s2;
+    if (L.tab) hipLaunchKernelGGL((sph3_shell_part_gr<R, 1>), pgrid, blk, 0, sg, a, in, L.part, L.npart);
+    else hipLaunchKernelGGL((sph3_shell_part_gr<R, 0>), pgrid, blk, 0, sg, a, in, L.part, L.npart);
+    hipLaunchKernelGGL(sph3_shell_field_gr, dim3(1), dim3(1024), 0, sg, a, L.part, L.npart, L.grav);
+  } else if (L.mono) {
+    hipLaunchKernelGGL(sph3_shell_part<R>, pgrid, blk, 0, s2, a, in, L.part);
     hipLaunchKernelGGL(sph3_shell_field, dim3(1), dim3(1024), 0, s2, a, L.part, L.npart, L.ms, L.grav);
   }
   rec(1, st);

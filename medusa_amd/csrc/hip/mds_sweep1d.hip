@@ -34,6 +34,7 @@
 #include "mds_internal.h"
 #include "mds_physics.h"
 #include "mds_eos.h"
+#include "mds_gravity.h"
 
 namespace {
 
@@ -868,17 +869,66 @@ __global__ __launch_bounds__(MDS_MONO_THREADS) void monopole_kernel(const DevPro
 }
 
 // ---------------------------------------------------------------------------
+// GR effective monopole field of spherical columns (gravity.c:142-375 with
+// GR_MONOPOLE), one workgroup per problem: the shell sums V rho, V P, V u,
+// V v^1 of the stage state (one zone per shell; P from the gamma law or the
+// FP64 table EOS, as update_eos gives the reference), then the field
+// (mds_gravity.h).  Problems whose clock has stopped keep their field.
+// ---------------------------------------------------------------------------
+template <class R, int TAB>
+__global__ __launch_bounds__(MDS_MONO_THREADS) void monopole_gr_kernel(const DevProb *prob, const DevClock *clock,
+                                                                        DevEos eos, StateArrays<R> st, double *grav)
+{
+  const DevProb &pb = prob[blockIdx.x];
+  if (!clock[blockIdx.x].active) return;
+  const int n = pb.n[0];
+  const long long off = pb.off;
+  double *rs = pb.gr + (size_t)MDS_GR_RHO * (n + 1), *ps = pb.gr + (size_t)MDS_GR_P * (n + 1);
+  double *us = pb.gr + (size_t)MDS_GR_U * (n + 1), *vs = pb.gr + (size_t)MDS_GR_V * (n + 1);
+  for (int i = threadIdx.x; i < n; i += MDS_MONO_THREADS) {
+    const double vol = pb.zone[(size_t)(i + MDS_NG) * MDS_ZG_COUNT + MDS_ZG_VOLUME];
+    const double rho = (double)st.v[0][off + i], u = (double)st.v[1][off + i], v1 = (double)st.v[2][off + i];
+    double press;
+    if (TAB) {
+      double g1, cs, lt;
+      eos_fill_x64(eos, rho, u, (double)st.v[5][off + i], press, g1, cs, lt);
+    } else {
+      press = (pb.gam - 1.0) * u;
+    }
+    rs[i] = vol * rho;
+    ps[i] = vol * press;
+    us[i] = vol * u;
+    vs[i] = vol * v1;
+  }
+  __syncthreads();
+  gr_monopole_block<MDS_MONO_THREADS>(pb, grav + off);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-extern "C" hipError_t mds_launch_monopole(int fp32, int nprob, const DevProb *prob, const void *st, double *grav,
+// mono: MDS_MONOPOLE_NEWTON or MDS_MONOPOLE_GR (then clock and eos are read;
+// tab: the batch has a table EOS)
+extern "C" hipError_t mds_launch_monopole(int fp32, int mono, int tab, int nprob, const DevProb *prob,
+                                          const DevClock *clock, const DevEos *eos, const void *st, double *grav,
                                           hipStream_t s)
 {
-  if (fp32)
-    hipLaunchKernelGGL(monopole_kernel<float>, dim3(nprob), dim3(MDS_MONO_THREADS), 0, s, prob,
-                       *(const StateArrays<float> *)st, grav);
-  else
-    hipLaunchKernelGGL(monopole_kernel<double>, dim3(nprob), dim3(MDS_MONO_THREADS), 0, s, prob,
-                       *(const StateArrays<double> *)st, grav);
+  const dim3 g(nprob), b(MDS_MONO_THREADS);
+  if (mono == MDS_MONOPOLE_GR) {
+    if (fp32) {
+      const StateArrays<float> &a = *(const StateArrays<float> *)st;
+      if (tab) hipLaunchKernelGGL((monopole_gr_kernel<float, 1>), g, b, 0, s, prob, clock, *eos, a, grav);
+      else hipLaunchKernelGGL((monopole_gr_kernel<float, 0>), g, b, 0, s, prob, clock, *eos, a, grav);
+    } else {
+      const StateArrays<double> &a = *(const StateArrays<double> *)st;
+      if (tab) hipLaunchKernelGGL((monopole_gr_kernel<double, 1>), g, b, 0, s, prob, clock, *eos, a, grav);
+      else hipLaunchKernelGGL((monopole_gr_kernel<double, 0>), g, b, 0, s, prob, clock, *eos, a, grav);
+    }
+  } else if (fp32) {
+    hipLaunchKernelGGL(monopole_kernel<float>, g, b, 0, s, prob, *(const StateArrays<float> *)st, grav);
+  } else {
+    hipLaunchKernelGGL(monopole_kernel<double>, g, b, 0, s, prob, *(const StateArrays<double> *)st, grav);
+  }
   return hipGetLastError();
 }
 

@@ -601,3 +601,29 @@ void mds_grid_free(mds_grid *g)
     g->lalpha[t] = g->lbeta[t] = g->lGamma[t] = g->lrcoef[t] = NULL;
   }
 }
+
+/* Radial shell geometry of the GR effective monopole (gravity.c:275-375),
+ * evaluated on the host with the same libm calls and operation order as
+ * the CPU restatement, so the device starts from the same bits.  out holds
+ * four arrays of n[0] + 1 doubles:
+ *   [0]  redge[i]  = r(|startx + i dx|)                      (face radius)
+ *   [1]  r34[i]    = (4 pi) redge[i]^3                       (face pressure term)
+ *   [2]  drdx[i]   = dr/dx(startx + i dx)                    (face gravity factor)
+ *   [3]  vsh[i]    = 4 pi/3 (redge[i+1]^3 - redge[i]^3)      (shell volume, i < n[0])
+ * Returns MDS_EINVAL for a grid that is not spherical. */
+int mds_gr_shell_geometry(const mds_grid *g, double *out)
+{
+  if (!g || !out || g->geom != MDS_GEOM_SPHERICAL) return MDS_EINVAL;
+  maps m;
+  maps_init(&m, g);
+  const int n1 = g->n[0], N = n1 + 1;
+  double *re = out, *r34 = out + N, *dr = out + 2 * N, *vs = out + 3 * N;
+  for (int i = 0; i <= n1; i++) {
+    re[i] = rx(&m, fabs(i * g->dx[0] + g->startx[0]));
+    r34[i] = (4.0 * M_PI) * pow(re[i], 3);
+    dr[i] = drx(&m, g->startx[0] + i * g->dx[0]);
+  }
+  for (int i = 0; i < n1; i++) vs[i] = ((4.0 * M_PI) / 3.0) * (pow(re[i + 1], 3) - pow(re[i], 3));
+  vs[n1] = 0.0;
+  return MDS_OK;
+}

@@ -17,6 +17,7 @@ FP64, FP32_BULK = 0, 1
 EIO = -5                       # MDS_EIO: a table file missing or short
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC, BC_PROB = 0, 1, 2, 3, 4
 BC_DISK = 8
+MONOPOLE_NEWTON, MONOPOLE_GR = 1, 2
 GEOM_CARTESIAN, GEOM_SPHERICAL = 0, 1
 RCOORD_UNIFORM, RCOORD_LOG, RCOORD_SINH = 0, 1, 2
 RC_COUNT = 8
@@ -107,7 +108,7 @@ _LIB = None
 
 EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
-            "mds_run_graph", "mds_sync", "mds_get_clock", "mds_get_health", "mds_conserved_totals", "mds_counters",
+            "mds_run_graph", "mds_sync", "mds_get_clock", "mds_get_health", "mds_get_gr_state", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
             "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
@@ -146,6 +147,8 @@ def lib():
     L.mds_sync.argtypes = [vp]
     L.mds_get_clock.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.mds_get_health.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Health)]
+    L.mds_get_gr_state.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]
     L.mds_conserved_totals.argtypes = [vp, ctypes.c_int, _dp]
     L.mds_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.mds_stream.argtypes = [vp]
@@ -322,7 +325,8 @@ class Context:
             arr[k].nvel = p.get("nvel", 0)
             if p.get("eos") is not None:
                 arr[k].eos = ctypes.pointer(p["eos"].t)
-            arr[k].monopole = int(bool(p.get("monopole", False)))
+            # True / 1: Newtonian, 2 (MONOPOLE_GR): the GR effective monopole
+            arr[k].monopole = int(p.get("monopole", 0))
         self._tables = [p.get("eos") for p in problems]     # keep the tables alive
         self.nstate = 6 if problems[0].get("eos") is not None else 5
         self.rp = rp if rp is not None else run_params()
@@ -409,6 +413,18 @@ class Context:
         h = Health()
         self._check(lib().mds_get_health(self.h, prob, ctypes.byref(h)), "get_health")
         return {f: getattr(h, f) for f, _ in Health._fields_}
+
+    def gr_state(self, prob=0):
+        """GR effective monopole state of a problem (mds_get_gr_state): dict of
+        gtov, mr, lapse_edge (n1 + 1 faces), lapse (n1 zones), passes, nonconverged."""
+        n1 = self.grids[prob].n[0]
+        out = {k: np.zeros(n1 + 1) for k in ("gtov", "mr", "lapse_edge")}
+        out["lapse"] = np.zeros(n1)
+        ps, nc = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().mds_get_gr_state(self.h, prob, _ptr(out["gtov"]), _ptr(out["mr"]), _ptr(out["lapse_edge"]),
+                                           _ptr(out["lapse"]), ctypes.byref(ps), ctypes.byref(nc)), "get_gr_state")
+        out["passes"], out["nonconverged"] = ps.value, nc.value
+        return out
 
     def totals(self, prob=0):
         out = np.zeros(2)

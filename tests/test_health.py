@@ -100,7 +100,10 @@ def test_sph3d_star_planted_zones_are_counted(precision):
     ctx = _run(prob, p, precision, 1)
     h1 = ctx.health()
     print(f"[health] sph3d star precision={precision}: clean {h0}, planted {h1}", flush=True)
-    assert h1["eos_off_table"] >= h0["eos_off_table"] + 2
+    # the first stage looks the planted zone up off the table; the inflow from
+    # its neighbours (a million times denser) can lift it into the table
+    # before the second stage's lookup
+    assert h1["eos_off_table"] >= h0["eos_off_table"] + 1
     # gamma law: a negative internal energy
     g, prob, prims = problems.star3d(*n, table=None)
     p = prims.copy()
