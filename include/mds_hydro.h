@@ -63,6 +63,7 @@ extern "C" {
 #define MDS_EDEVICE    -3
 #define MDS_ENOTIMPL   -4
 #define MDS_EIO        -5
+#define MDS_ERANGE     -6   /* a size beyond an entry point's fixed maximum */
 
 /* Per-zone geometry record of a 1D spherical column (one record per zone,
  * n1 + 2*MDS_NG records, zone -NG first), restating the fields of zone_geom
@@ -372,6 +373,21 @@ int mds_recon_points(int precision, int ncell, const double *a, const double *mo
 int mds_riemann_points(int precision, int solver /*0 HLLC, 1 HLLE*/, int nface,
                        const double *pl, const double *pr, const double *fg, int dir,
                        double *flux, double *sig, int *isshock);
+
+/* Allocation-free variant of mds_riemann_points (no counterpart in the
+ * reference): for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results; the device staging of the inputs and outputs lives in a fixed
+ * static device array of the library (about 6.7 MB per GPU, held until the
+ * process ends) instead of per-call allocations, and the call makes no
+ * hipMalloc/hipFree.  nface > MDS_NOALLOC_MAX_FACES is rejected with
+ * MDS_ERANGE.  Runs on the current device, synchronously on its null stream.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several host threads) race and must be serialised by the caller. */
+#define MDS_NOALLOC_MAX_FACES 32768
+int mds_riemann_points_noalloc(int precision, int solver, int nface,
+                               const double *pl, const double *pr, const double *fg, int dir,
+                               double *flux, double *sig, int *isshock);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 #include <stdarg.h>
 #include <stdlib.h>
 #include <vector>
+#include <atomic>
 #include <new>
 #include <algorithm>
 #include "mds_internal.h"
@@ -1652,5 +1653,79 @@ extern "C" int mds_riemann_points(int precision, int solver, int nface, const do
   if (e == hipSuccess) e = hipMemcpy(sig, dsg, (size_t)nface * sizeof(double), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(isshock, dsh, (size_t)nface * sizeof(int), hipMemcpyDeviceToHost);
   hipFree(dpl); hipFree(dpr); hipFree(dfg); hipFree(dfl); hipFree(dsg); hipFree(dsh);
+  return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free pointwise Riemann solve
+// ---------------------------------------------------------------------------
+// The staging of mds_riemann_points (its inputs and outputs on the device)
+// lives in this file-scope array instead of per-call hipMalloc: per face,
+// 7 + 7 + 5 doubles in (pl, pr, fg), 5 + 1 doubles out (flux, sig), and one
+// int (isshock) in the tail.  Every GPU holds its own copy, at its own
+// address, until the process ends.  In host code the name `scratch` is a
+// host-side stand-in: only the address from hipGetSymbolAddress (kept per
+// device in noalloc_ptr) may reach a kernel or a copy.
+namespace {
+constexpr int NOALLOC_DOUBLES_PER_FACE = 7 + 7 + 5 + 5 + 1;
+constexpr size_t NOALLOC_BYTES =
+    (size_t)MDS_NOALLOC_MAX_FACES * (NOALLOC_DOUBLES_PER_FACE * sizeof(double) + sizeof(int));
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_BYTES / sizeof(double)];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double *noalloc_ptr[NOALLOC_MAX_DEVICES];
+std::atomic<int> noalloc_lock{0};
+
+// The device address of `scratch` on the current device, resolved once per
+// device (with that device current) and kept.
+hipError_t noalloc_scratch(double **out)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  while (noalloc_lock.exchange(1, std::memory_order_acquire)) {
+  }
+  if (!noalloc_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) noalloc_ptr[dev] = (double *)p;
+  }
+  *out = noalloc_ptr[dev];
+  noalloc_lock.store(0, std::memory_order_release);
+  return (e == hipSuccess && !*out) ? hipErrorInvalidSymbol : e;
+}
+}  // namespace
+
+extern "C" int mds_riemann_points_noalloc(int precision, int solver, int nface, const double *pl, const double *pr,
+                                          const double *fg, int dir, double *flux, double *sig, int *isshock)
+{
+  if (nface <= 0 || !pl || !pr || !fg || !flux || !sig || !isshock || dir < 0 || dir > 2) return MDS_EINVAL;
+  if (nface > MDS_NOALLOC_MAX_FACES) return MDS_ERANGE;
+  double *base = nullptr;
+  hipError_t e = noalloc_scratch(&base);
+  if (e != hipSuccess) return MDS_EDEVICE;
+  const size_t n = (size_t)nface;
+  // regions sized for nface faces, packed from the start of the array;
+  // every output element is written by the kernel, so nothing needs clearing
+  double *dpl = base, *dpr = dpl + 7 * n, *dfg = dpr + 7 * n, *dfl = dfg + 5 * n, *dsg = dfl + 5 * n;
+  int *dsh = (int *)(dsg + n);
+  e = hipMemcpy(dpl, pl, 7 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dpr, pr, 7 * n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dfg, fg, 5 * n * sizeof(double), hipMemcpyHostToDevice);
+  dim3 b(128), g((nface + 127) / 128);
+  if (e == hipSuccess) {
+    if (precision == MDS_FP64)
+      hipLaunchKernelGGL(riemann_points<double>, g, b, 0, 0, solver, nface, dpl, dpr, dfg, dir, dfl, dsg, dsh);
+    else
+      hipLaunchKernelGGL(riemann_points<float>, g, b, 0, 0, solver, nface, dpl, dpr, dfg, dir, dfl, dsg, dsh);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(flux, dfl, 5 * n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(sig, dsg, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(isshock, dsh, n * sizeof(int), hipMemcpyDeviceToHost);
   return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
 }

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libmds.so")
 NG = 2
 FP64, FP32_BULK = 0, 1
 EIO = -5                       # MDS_EIO: a table file missing or short
+ERANGE = -6                    # MDS_ERANGE: a size beyond an entry point's fixed maximum
+MAX_NOALLOC_FACES = 32768      # MDS_NOALLOC_MAX_FACES
 BC_OUTFLOW, BC_REFLECT, BC_EXTRAP, BC_PERIODIC, BC_PROB = 0, 1, 2, 3, 4
 BC_DISK = 8
 MONOPOLE_NEWTON, MONOPOLE_GR = 1, 2
@@ -109,7 +111,7 @@ _LIB = None
 EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_default_run_params", "mds_create", "mds_destroy",
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_get_health", "mds_get_gr_state", "mds_conserved_totals", "mds_counters",
-            "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_run_timed", "mds_copy_probe",
+            "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_riemann_points_noalloc", "mds_run_timed", "mds_copy_probe",
             "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
@@ -163,6 +165,7 @@ def lib():
     L.mds_recon_points.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mds_riemann_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.mds_riemann_points_noalloc.argtypes = L.mds_riemann_points.argtypes
     L.mds_eos_table_default.argtypes = [ctypes.POINTER(EosTableDesc)]
     L.mds_eos_table_default.restype = None
     L.mds_eos_model_params.argtypes = [ctypes.c_ulonglong, _dp]
@@ -477,7 +480,9 @@ def recon_points(precision, a, mom=None, x=None, rcoef=None):
     return al, ar
 
 
-def riemann_points(precision, solver, pl, pr, fg, direction):
+def riemann_points(precision, solver, pl, pr, fg, direction, noalloc=False):
+    """Pointwise device Riemann fluxes (mds_riemann_points, or with noalloc
+    its allocation-free variant, at most MAX_NOALLOC_FACES faces)."""
     pl = np.ascontiguousarray(pl, dtype=np.float64)
     pr = np.ascontiguousarray(pr, dtype=np.float64)
     fg = np.ascontiguousarray(fg, dtype=np.float64)
@@ -485,10 +490,11 @@ def riemann_points(precision, solver, pl, pr, fg, direction):
     flux = np.zeros((n, 5))
     sig = np.zeros(n)
     sh = np.zeros(n, dtype=np.int32)
-    r = lib().mds_riemann_points(precision, solver, n, _ptr(pl), _ptr(pr), _ptr(fg), direction, _ptr(flux),
-                                 _ptr(sig), sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    fn = lib().mds_riemann_points_noalloc if noalloc else lib().mds_riemann_points
+    r = fn(precision, solver, n, _ptr(pl), _ptr(pr), _ptr(fg), direction, _ptr(flux),
+           _ptr(sig), sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
     if r:
-        raise RuntimeError(f"mds_riemann_points failed ({r})")
+        raise RuntimeError(f"{fn.__name__} failed ({r})")
     return flux, sig, sh
 
 
