@@ -259,6 +259,17 @@ int mds_download_prims(mds_ctx *ctx, int prob, double *prims, int nvars);
 typedef void (*mds_prob_bounds_fn)(int i, int j, int k, double *p, void *user);
 int mds_set_prob_bounds(mds_ctx *ctx, int prob, mds_prob_bounds_fn fn, void *user);
 
+/* A PROB hook that reads the global time, as the reference's Noh problems do
+ * (problems/init_spherical_noh.c prob_bounds: rho = (1 + t/r)^2): fn(i, j, k,
+ * t, p, user) fills the same ghost zones as above at the time the
+ * reference's fill sees -- t in the first RK stage and t + dt in the second
+ * (bc.c:717-721).  The library keeps the hook: every step of mds_run and
+ * mds_run_graph then computes dt on the device, waits for it, evaluates the
+ * hook on the host at both times and copies the values in before each stage,
+ * so such a context steps synchronously (one host round trip per step). */
+typedef void (*mds_prob_bounds_t_fn)(int i, int j, int k, double t, double *p, void *user);
+int mds_set_prob_bounds_t(mds_ctx *ctx, int prob, mds_prob_bounds_t_fn fn, void *user);
+
 /* Initial EOS fill and first dt estimate input (main.c:181, eos.c:4-18);
  * also resets t, the step counter and the dt history of every problem. */
 int mds_update_eos(mds_ctx *ctx);

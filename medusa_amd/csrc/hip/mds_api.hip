@@ -82,6 +82,12 @@ struct mds_ctx {
   int has_slab = 0;                     // some problem has a PROB or EXTRAP side
   long long slab_items = 0;             // most ghost zones of PROB/EXTRAP sides in one problem
   std::vector<int> prob_pending;        // per problem: PROB sides whose values were never set
+  // time-dependent PROB hooks (mds_set_prob_bounds_t), per problem, and
+  // pinned host staging of their values per side and stage
+  std::vector<mds_prob_bounds_t_fn> tdep_fn;
+  std::vector<void *> tdep_user;
+  std::vector<double *> tdep_buf;                 // [prob][3 dirs][2 sides][2 stages], or null
+  int ntdep = 0;
   std::vector<int> hdj, hdk;            // dendritic grid: dj and dk with ghosts (the grid's layout)
   hipGraph_t graph[2] = {nullptr, nullptr};        // one step, per buffer parity
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -140,7 +146,9 @@ __global__ void dt_policy_kernel(DevClock *ck, int nprob, RunParams rp)
   if (c.t + 1 * dt > c.tmax) dt = (1.000000001 * (c.tmax - c.t)) / 1;
   if (dt < rp.dtmin) dt = rp.dtmin;
   c.dt = dt;
-  c.t += dt;
+  // the reference's second-stage fill adds dt to the global t and takes it
+  // off again (bc.c:717-721, 1417), then the main loop adds dt (main.c:263)
+  c.t = ((c.t + dt) - dt) + dt;
   c.istep += 1;
   c.mindt = dt_key(1.0e100);   // step() resets min_dt (step.c:49)
   ck[p] = c;
@@ -306,6 +314,7 @@ extern "C" int mds_destroy(mds_ctx *c)
   if (c->eos_alloc) hipFree(c->eos_alloc);
   if (c->dgrav) hipFree(c->dgrav);
   if (c->dgr) hipFree(c->dgr);
+  for (double *b : c->tdep_buf) if (b) hipHostFree(b);
   if (c->dgrst) hipFree(c->dgrst);
   for (int d = 0; d < 3; d++) if (c->s3.F[d]) hipFree(c->s3.F[d]);
   if (c->s3.P) hipFree(c->s3.P);
@@ -778,17 +787,20 @@ extern "C" int mds_create(const mds_problem *probs, int nprob, const mds_run_par
   return MDS_OK;
 }
 
-extern "C" int mds_set_prob_bounds(mds_ctx *c, int prob, mds_prob_bounds_fn fn, void *user)
+// The values of every PROB side of problem prob from eval(i, j, k, q):
+// out(d, side) names the host buffer of side (d, side) (MDS_NG * MDS_SLAB_NV
+// * T doubles, entry-major), or null to skip it.
+template <class Eval, class Out>
+static void prob_fill_host(mds_ctx *c, int prob, Eval eval, Out out)
 {
-  if (!c || prob < 0 || prob >= c->nprob || !fn) return MDS_EINVAL;
   const DevProb &pb = c->hprob[prob];
-  HIPCHK(c, hipStreamSynchronize(c->st));
   for (int d = 0; d < pb.ndim; d++)
     for (int side = 0; side < 2; side++) {
       const int code = side ? pb.bc_hi[d] : pb.bc_lo[d];
       if (code != MDS_BC_PROB) continue;
+      double *h = out(d, side);
+      if (!h) continue;
       const long long T = slab_T(pb, d);
-      std::vector<double> h((size_t)MDS_NG * MDS_SLAB_NV * T, 0.0);
       const int lo1 = d == 0 ? 1 : 0, lo2 = d == 2 ? 1 : 2;   // transverse directions in index order
       for (int g = 0; g < MDS_NG; g++) {
         const int c0 = side ? pb.n[d] + g : g - MDS_NG;
@@ -807,7 +819,7 @@ extern "C" int mds_set_prob_bounds(mds_ctx *c, int prob, mds_prob_bounds_fn fn, 
             if (idx[2] >= pb.n[2] / dk) continue;
           }
           double q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          fn(pb.ndim > 0 ? idx[0] : 0, pb.ndim > 1 ? idx[1] : 0, pb.ndim > 2 ? idx[2] : 0, q, user);
+          eval(pb.ndim > 0 ? idx[0] : 0, pb.ndim > 1 ? idx[1] : 0, pb.ndim > 2 ? idx[2] : 0, q);
           for (int v = 0; v < 6; v++) {
             if (v >= 2 + c->nv && v < 5) continue;   // velocity components the state does not carry
             if (v == 5 && !c->tab) continue;
@@ -815,8 +827,57 @@ extern "C" int mds_set_prob_bounds(mds_ctx *c, int prob, mds_prob_bounds_fn fn, 
           }
         }
       }
-      HIPCHK(c, hipMemcpy(pb.gs[d][side], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+}
+
+extern "C" int mds_set_prob_bounds(mds_ctx *c, int prob, mds_prob_bounds_fn fn, void *user)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !fn) return MDS_EINVAL;
+  const DevProb &pb = c->hprob[prob];
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<double> h[3][2];
+  prob_fill_host(c, prob, [&](int i, int j, int k, double *q) { fn(i, j, k, q, user); },
+                 [&](int d, int side) {
+                   h[d][side].assign((size_t)MDS_NG * MDS_SLAB_NV * slab_T(pb, d), 0.0);
+                   return h[d][side].data();
+                 });
+  for (int d = 0; d < 3; d++)
+    for (int side = 0; side < 2; side++)
+      if (!h[d][side].empty())
+        HIPCHK(c, hipMemcpy(pb.gs[d][side], h[d][side].data(), h[d][side].size() * sizeof(double),
+                            hipMemcpyHostToDevice));
+  if (!c->tdep_fn.empty() && c->tdep_fn[prob]) { c->tdep_fn[prob] = nullptr; c->ntdep--; }
+  c->prob_pending[prob] = 0;
+  return MDS_OK;
+}
+
+extern "C" int mds_set_prob_bounds_t(mds_ctx *c, int prob, mds_prob_bounds_t_fn fn, void *user)
+{
+  if (!c || prob < 0 || prob >= c->nprob || !fn) return MDS_EINVAL;
+  const DevProb &pb = c->hprob[prob];
+  bool any = false;
+  for (int d = 0; d < pb.ndim; d++) any = any || pb.bc_lo[d] == MDS_BC_PROB || pb.bc_hi[d] == MDS_BC_PROB;
+  if (!any) return set_err(c, MDS_EINVAL, "problem %d has no PROB side", prob);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->tdep_fn.empty()) {
+    c->tdep_fn.assign(c->nprob, nullptr);
+    c->tdep_user.assign(c->nprob, nullptr);
+    c->tdep_buf.assign((size_t)c->nprob * 12, nullptr);
+  }
+  for (int d = 0; d < pb.ndim; d++)
+    for (int side = 0; side < 2; side++) {
+      if ((side ? pb.bc_hi[d] : pb.bc_lo[d]) != MDS_BC_PROB) continue;
+      for (int st = 0; st < 2; st++) {
+        double *&b = c->tdep_buf[(size_t)prob * 12 + (d * 2 + side) * 2 + st];
+        if (!b)
+          HIPCHK(c, hipHostMalloc((void **)&b, sizeof(double) * MDS_NG * MDS_SLAB_NV * slab_T(pb, d),
+                                  hipHostMallocDefault));
+        memset(b, 0, sizeof(double) * MDS_NG * MDS_SLAB_NV * slab_T(pb, d));
+      }
+    }
+  if (!c->tdep_fn[prob]) c->ntdep++;
+  c->tdep_fn[prob] = fn;
+  c->tdep_user[prob] = user;
   c->prob_pending[prob] = 0;
   return MDS_OK;
 }
@@ -1050,9 +1111,56 @@ static int enqueue_step(mds_ctx *c)
   return rc;
 }
 
+// One step of a context with time-dependent PROB hooks: the dt policy on
+// the device and a wait for it, the hooks on the host at the times the
+// reference's fills see (t and t + dt, bc.c:717-721), and their values
+// copied in on the stream before each stage's ghost-layer refill.
+static int step_tdep(mds_ctx *c)
+{
+  const int fp32 = c->precision == MDS_FP32_BULK;
+  std::vector<DevClock> ck0(c->nprob), ck1(c->nprob);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(ck0.data(), c->dclock, sizeof(DevClock) * c->nprob, hipMemcpyDeviceToHost));
+  enqueue_dt(c);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(ck1.data(), c->dclock, sizeof(DevClock) * c->nprob, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  auto buf = [&](int p, int d, int side, int st) { return c->tdep_buf[(size_t)p * 12 + (d * 2 + side) * 2 + st]; };
+  for (int p = 0; p < c->nprob; p++) {
+    if (!c->tdep_fn[p] || !ck1[p].active) continue;
+    const double tt[2] = {ck0[p].t, ck0[p].t + ck1[p].dt};
+    for (int st = 0; st < 2; st++)
+      prob_fill_host(c, p, [&](int i, int j, int k, double *q) { c->tdep_fn[p](i, j, k, tt[st], q, c->tdep_user[p]); },
+                     [&](int d, int side) { return buf(p, d, side, st); });
+  }
+  for (int stage = 0; stage < 2; stage++) {
+    for (int p = 0; p < c->nprob; p++) {
+      if (!c->tdep_fn[p] || !ck1[p].active) continue;
+      const DevProb &pb = c->hprob[p];
+      for (int d = 0; d < pb.ndim; d++)
+        for (int side = 0; side < 2; side++)
+          if ((side ? pb.bc_hi[d] : pb.bc_lo[d]) == MDS_BC_PROB)
+            HIPCHK(c, hipMemcpyAsync(pb.gs[d][side], buf(p, d, side, stage),
+                                     sizeof(double) * MDS_NG * MDS_SLAB_NV * slab_T(pb, d), hipMemcpyHostToDevice,
+                                     c->st));
+    }
+    int rc = enqueue_stage(c, stage, fp32);
+    if (rc) return rc;
+  }
+  finish_step(c);
+  return MDS_OK;
+}
+
 extern "C" int mds_run(mds_ctx *c, int nsteps)
 {
   if (!c || nsteps < 0) return MDS_EINVAL;
+  if (c->ntdep > 0) {
+    for (int s = 0; s < nsteps; s++) {
+      int rc = step_tdep(c);
+      if (rc) return rc;
+    }
+    return MDS_OK;
+  }
   for (int s = 0; s < nsteps; s++) {
     int rc = enqueue_step(c);
     if (rc) return rc;
@@ -1063,6 +1171,7 @@ extern "C" int mds_run(mds_ctx *c, int nsteps)
 extern "C" int mds_run_graph(mds_ctx *c, int nsteps)
 {
   if (!c || nsteps < 0) return MDS_EINVAL;
+  if (c->ntdep > 0) return mds_run(c, nsteps);     // host hooks between the stages: no graph
   for (int s = 0; s < nsteps; s++) {
     // one graph per buffer parity: the two-stage 1D step alternates p and ph
     const int k = c->fuse1d ? c->par : 0;
@@ -1085,6 +1194,7 @@ extern "C" int mds_run_graph(mds_ctx *c, int nsteps)
 extern "C" int mds_run_timed(mds_ctx *c, int nsteps, double ms[3])
 {
   if (!c || nsteps < 1 || !ms) return MDS_EINVAL;
+  if (c->ntdep > 0) return set_err(c, MDS_EINVAL, "time-dependent PROB hooks step with mds_run only");
   std::vector<hipEvent_t> ev(3 * (size_t)nsteps + 1, nullptr);
   int rc = MDS_OK;
   for (auto &e : ev) {
@@ -1342,7 +1452,9 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
-  if (c->sph3) { if (best_ms) *best_ms = 0.0; return MDS_OK; }   // fixed shape: one thread per face/zone
+  // fixed shapes: 3D spherical grids (one thread per face/zone), and
+  // contexts whose time-dependent PROB hooks run on the host every step
+  if (c->sph3 || c->ntdep > 0) { if (best_ms) *best_ms = 0.0; return MDS_OK; }
   // candidates: (a, b, c) = 1D (two-stage launch, persistent grid, -);
   // 2D (rows per wavefront, -, -); 3D (rows per workgroup, planes, halo mode)
   struct Shape { int a, b, c; };

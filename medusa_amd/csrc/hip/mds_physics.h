@@ -506,10 +506,14 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   if (ss < sl) sl = 1.001f * ss;
   float iscale = CART ? 1.0f : frcp(g.scale);
   sig = fmaxf(fabsf(sl), fabsf(sr)) * iscale;
-  if (sl > 0.0f) { pflux_f32<NV, CART, NS>(L, g, dir, flux); return isshock; }
-  if (sr < 0.0f) { pflux_f32<NV, CART, NS>(Rt, g, dir, flux); return isshock; }
+  // One evaluation for every face, no divergent exits: the physical flux of
+  // the upwind state (the left state where sl > 0, the right where sr < 0,
+  // else the side of the contact), plus the star-state term where neither
+  // wave is supersonic (riemann.c:437-495).  The term is selected, not
+  // weighted, so an undefined star state of a supersonic face stays out.
+  const bool sup_l = sl > 0.0f, sup_r = sr < 0.0f, star = !(sup_l || sup_r);
+  const bool left = sup_l || (!sup_r && ss > 0.0f);
   float u[6], fx[6], Us[6];
-  bool left = ss > 0.0f;
   const PState<float, NV> K = pick_state(left, L, Rt);
   float sk = left ? sl : sr, uk = left ? u_l : u_r;
   p_uflux_f32<NV, CART>(K, g, dir, u, fx);
@@ -521,7 +525,7 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
     float vs = (d == dir) ? ss * iscale : K.v[d];
     Us[2 + d] = (CART ? vs : g.gcov[d] * vs) * Us[0];
   }
-  for (int q = 0; q < 2 + NV; q++) flux[q] = fx[q] + slr * (Us[q] - u[q]);
+  for (int q = 0; q < 2 + NV; q++) flux[q] = star ? fx[q] + slr * (Us[q] - u[q]) : fx[q];
   if (NS) flux[MDS_YI] = flux[0] * K.ye;     // passive scalar: upwind by the contact
   return isshock;
 }

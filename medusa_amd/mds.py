@@ -112,7 +112,7 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_last_error", "mds_upload_prims", "mds_download_prims", "mds_update_eos", "mds_run",
             "mds_run_graph", "mds_sync", "mds_get_clock", "mds_get_health", "mds_get_gr_state", "mds_conserved_totals", "mds_counters",
             "mds_stream", "mds_recon_points", "mds_riemann_points", "mds_riemann_points_noalloc", "mds_run_timed", "mds_copy_probe",
-            "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds",
+            "mds_valu_probe", "mds_set_tile", "mds_get_tile", "mds_autotune", "mds_set_prob_bounds", "mds_set_prob_bounds_t",
             "mds_splitmix64", "mds_uniform", "mds_init_sod", "mds_init_kh", "mds_init_rt",
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
@@ -162,6 +162,7 @@ def lib():
     L.mds_get_tile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.mds_autotune.argtypes = [vp, ctypes.c_int, _dp]
     L.mds_set_prob_bounds.argtypes = [vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    L.mds_set_prob_bounds_t.argtypes = [vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.mds_recon_points.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mds_riemann_points.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
                                      _dp, _dp, ctypes.POINTER(ctypes.c_int)]
@@ -308,6 +309,8 @@ def run_params(cfl=0.5, tmax=1.0e300, **kw):
 # mds_prob_bounds_fn: (i, j, k, p, user)
 PROB_BOUNDS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                   ctypes.c_void_p)
+PROB_BOUNDS_T_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 class Context:
@@ -361,6 +364,21 @@ class Context:
         cb = PROB_BOUNDS_FN(_cb)
         self._check(lib().mds_set_prob_bounds(self.h, prob, ctypes.cast(cb, ctypes.c_void_p), None),
                     "set_prob_bounds")
+
+    def set_prob_bounds_t(self, fn, prob=0):
+        """Time-dependent PROB ghost values: fn(i, j, k, t) -> (rho, u, v1, v2,
+        v3[, Ye]) at the time the reference's fill sees (t in the first RK
+        stage, t + dt in the second); the context then steps synchronously."""
+        def _cb(i, j, k, t, p, user):
+            vals = fn(i, j, k, t)
+            for v in range(min(len(vals), 6)):
+                p[v] = float(vals[v])
+        cb = PROB_BOUNDS_T_FN(_cb)
+        if not hasattr(self, "_tdep_keep"):
+            self._tdep_keep = {}
+        self._tdep_keep[prob] = cb                     # the library calls it every step
+        self._check(lib().mds_set_prob_bounds_t(self.h, prob, ctypes.cast(cb, ctypes.c_void_p), None),
+                    "set_prob_bounds_t")
 
     def _shape(self, prob):
         g = self.grids[prob]

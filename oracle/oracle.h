@@ -87,9 +87,17 @@ typedef struct {
   int dendritic;       /* 3D spherical: dendritic angular coarsening (utils.c:181-590) */
   /* PROB boundaries: the problem's prob_bounds(i, j, k, p) (e.g.
    * problems/init_rt_3d.c:223-236), called for every ghost zone of a PROB side
-   * with p holding nvars entries; time-independent here */
+   * with p holding nvars entries */
   void (*prob_bounds)(int i, int j, int k, double *p, void *user);
   void *prob_user;
+  /* or a hook that reads the global time, as problems/init_spherical_noh.c
+   * does: called with the time the reference's fill sees, t in the first
+   * stage and t + dt in the second (bc.c:717-721) */
+  void (*prob_bounds_t)(int i, int j, int k, double t, double *p, void *user);
+  /* multipole gravity (SPHERICAL_MULTIPOLE_GRAV, gravity.c:499-1110): the
+   * l = 1..multipole_lmax terms of the potential on top of the monopole
+   * (use_monopole must be set); 0 = none.  2D grids take m = 0 only */
+  int multipole_lmax;
 } orc_config;
 
 typedef struct orc_sim orc_sim;
@@ -111,6 +119,12 @@ int orc_eos_walk(const orc_eos_tab *t, double rho, double u, double ye, double t
 /* eos_stellar_collapse.c:294-369 on the same interpolant: log10 T of the root */
 int orc_eos_root(const orc_eos_tab *t, double rho, double u, double ye, double lt_guess, double tol,
                  double *lt_out, int *newton_iters);
+
+/* ---- multipole gravity pieces (gravity.c:499-697) ---- */
+/* real spherical harmonics at (th, phi), LM index l (l + 1) + m, m in [-l, l] */
+void orc_ylms(double *ylm, double th, double phi, int lmax, int mmax);
+/* their integrals times sin(th) over th in [thmin, thmax], phi in [phimin, phimax] */
+void orc_ylm_ints(double *slm, double thmin, double thmax, double phimin, double phimax, int lmax, int mmax);
 
 /* ---- pointwise kernels (reconstruct.c, riemann.c, fluid.c) ---- */
 void orc_para_recon(const double *a, double *al, double *ar,
@@ -170,6 +184,16 @@ int orc_gr_state(const orc_sim *s, double *gtov, double *mr, double *lapse, doub
 /* GR effective monopole face gravity (n1+1 values) of the current state, after
  * an EOS fill of it; updates the kept TOV factor like a stage does. */
 int orc_gr_edges(orc_sim *s, double *grav);
+/* multipole potential at the zone corners of the current state, after the
+ * radial fluxes of a stage are known or not (the potential needs no flux):
+ * phi[(ii * (n2 + 1) + jj) * (n3 + 1) + kk] for corner (ii, jj, kk) of
+ * level-local indices, ii in [0, n1], jj in [0, nj(ii)], kk in [0, nk(ii,
+ * jj)] (gravity.c:848-906); -1 without multipole gravity */
+int orc_multipole_phi(orc_sim *s, double *phi);
+/* tests only: the multipole momentum sources -rho dPhi/dx^d of the current
+ * state with no mass fluxes (src[3 * zone + d], zones i fastest, box
+ * positions; the energy source needs fluxes and is not returned) */
+int orc_multipole_sources(orc_sim *s, double *src);
 /* dendritic levels: nj[n1], nk[n1*n2] (0 beyond nj[i]); returns maxjlev*256 + maxklev */
 int orc_dendritic_levels(const orc_sim *s, int *nj, int *nk);
 /* theta (dir 0) / phi (dir 1) moments of refinement level lev (length n + 2 NG), NULL if none */

@@ -548,7 +548,8 @@ static void build_geometry_spherical1d(orc_sim *s)
 orc_sim *orc_create(const orc_config *cfg)
 {
   for (int d = 0; d < cfg->ndim; d++)
-    if ((cfg->bc_lo[d] == ORC_BC_PROB || cfg->bc_hi[d] == ORC_BC_PROB) && !cfg->prob_bounds) return NULL;
+    if ((cfg->bc_lo[d] == ORC_BC_PROB || cfg->bc_hi[d] == ORC_BC_PROB) && !cfg->prob_bounds && !cfg->prob_bounds_t)
+      return NULL;
   orc_sim *s = calloc(1, sizeof(*s));
   s->c = *cfg;
   for (int d = 0; d < 3; d++) {
@@ -672,6 +673,16 @@ long orc_hlle_face_count(const orc_sim *s) { return s->nhlle; }
 /* ------------------------------------------------------------------ */
 /* bc.c:705-1421 for uniform (non-dendritic) grids.  Direction d fills its own
  * ghost slabs over interior zones of the other directions only. */
+
+/* one PROB ghost zone from the problem's hook: the time-dependent form sees
+ * the global t as the reference's fill sets it (bc.c:717-721) */
+static void prob_hook(const orc_sim *s, int i, int j, int k, double *dst)
+{
+  const orc_config *c = &s->c;
+  if (c->prob_bounds_t) c->prob_bounds_t(i, j, k, s->t, dst, c->prob_user);
+  else c->prob_bounds(i, j, k, dst, c->prob_user);
+}
+
 static void fill_ghost(orc_sim *s, double *P, int d, int side)
 {
   const orc_config *c = &s->c;
@@ -731,7 +742,7 @@ static void fill_ghost(orc_sim *s, double *P, int d, int side)
         }
         double *dst = &P[zid(s, idx[0], idx[1], idx[2]) * nv];
         if (prob) {
-          c->prob_bounds(idx[0], idx[1], idx[2], dst, c->prob_user);
+          prob_hook(s, idx[0], idx[1], idx[2], dst);
         } else if (extrap) {
           int i0, i1;
           double w;
@@ -895,7 +906,7 @@ static void fill_ghost_dend(orc_sim *s, double *P)
           const int origin = code == ORC_BC_SPHERICAL_ORIGIN && !side;
           double *dst = &P[zid(s, i, j, k) * nv];
           if (code == ORC_BC_PROB) {
-            c->prob_bounds(i, j, k, dst, c->prob_user);
+            prob_hook(s, i, j, k, dst);
             continue;
           }
           if (code == ORC_BC_EXTRAP) {
@@ -1600,6 +1611,352 @@ static void monopole(orc_sim *s, const double *P)
   free(grav);
 }
 
+/* ------------------------------------------------------------------ */
+/* Multipole gravity (gravity.c:499-1110, SPHERICAL_MULTIPOLE_GRAV): the
+ * l = 1..lmax terms of the potential, added after the monopole
+ * (gravity.c:1137-1140, 1162-1165).  Index of (l, m): l (l + 1) + m. */
+#define ORC_LM(l, m) ((l) * ((l) + 1) + (m))
+#define ORC_MP_LMAXC 15                 /* MULTIPOLE_LMAX, decs.h:38 */
+#define ORC_SQRT2 1.41421356237309504880
+
+/* n! / m! (gravity.c:499-516) */
+static double mp_factorial_ratio(int n, int m)
+{
+  double fact = 1.0;
+  while (n > m) { fact *= n; n--; }
+  while (m > n) { fact /= m; m--; }
+  return fact;
+}
+
+/* associated Legendre P[l,m](cos th), m >= 0, with the Condon-Shortley
+ * phase: P[m,m] from (2m-1)!! (1-x^2)^(m/2), P[m+1,m], then the three-term
+ * recurrence in l (gravity.c:579-607) */
+static void mp_plms(double *plm, double th, int lmax, int mmax)
+{
+  const double cth = cos(th);
+  for (int m = 0; m <= mmax; m++) {
+    const double somx2 = sqrt((1.0 - cth) * (1.0 + cth));
+    double fact = 1.0, pmm = 1.0;
+    for (int i = 1; i <= m; i++) {
+      pmm = ((-pmm) * fact) * somx2;
+      fact = fact + 2.0;
+    }
+    plm[ORC_LM(m, m)] = pmm;
+    if (m != lmax) {
+      plm[ORC_LM(m + 1, m)] = (cth * ((2.0 * m) + 1.0)) * pmm;
+      for (int l = m + 2; l <= lmax; l++)
+        plm[ORC_LM(l, m)] = (((cth * ((2.0 * l) - 1.0)) * plm[ORC_LM(l - 1, m)]) - ((l + m - 1) * plm[ORC_LM(l - 2, m)])) /
+                            (l - m);
+    }
+  }
+}
+
+/* the real harmonics: sqrt((2l+1)/(4pi) (l-|m|)!/(l+|m|)!) P[l,|m|], times
+ * sqrt(2) cos(m phi) for m > 0 and sqrt(2) sin(|m| phi) for m < 0
+ * (gravity.c:518-577) */
+void orc_ylms(double *ylm, double th, double phi, int lmax, int mmax)
+{
+  mp_plms(ylm, th, lmax, mmax);
+  for (int m = 0; m <= mmax; m++) {
+    const double smphi = sin(m * phi), cmphi = cos(m * phi);
+    const double norm_fact = (m == 0) ? 1 : ORC_SQRT2;
+    for (int l = m; l <= lmax; l++) {
+      const double norm = norm_fact * sqrt((((2.0 * l) + 1.0) / (4.0 * M_PI)) * mp_factorial_ratio(l - m, l + m));
+      ylm[ORC_LM(l, m)] *= norm;
+      if (m > 0) {
+        ylm[ORC_LM(l, -m)] = ylm[ORC_LM(l, m)] * smphi;
+        ylm[ORC_LM(l, m)] *= cmphi;
+      }
+    }
+  }
+}
+
+/* the integral of Y[l,m] sin(th) over a zone (gravity.c:609-697): the theta
+ * integrals of P[l,m] by their recurrences, the phi integrals of cos / sin */
+void orc_ylm_ints(double *slm, double thmin, double thmax, double phimin, double phimax, int lmax, int mmax)
+{
+  double plmp[ORC_MP_LMAXC * ORC_MP_LMAXC], plmm[ORC_MP_LMAXC * ORC_MP_LMAXC];
+  const double cmin = cos(thmin), cmax = cos(thmax);
+  mp_plms(plmm, thmin, lmax, mmax);
+  mp_plms(plmp, thmax, lmax, mmax);
+  for (int m = 0; m <= mmax; m++) {
+    if (m > 1) {
+      slm[ORC_LM(m, m)] = (((m * (2 * m - 1) * (2 * m - 3)) * slm[ORC_LM(m - 2, m - 2)] - (cmax * plmp[ORC_LM(m, m)])) +
+                           (cmin * plmm[ORC_LM(m, m)])) / (m + 1.0);
+    } else if (m == 0) {
+      slm[ORC_LM(0, 0)] = cmin - cmax;
+    } else {
+      slm[ORC_LM(1, 1)] = 0.5 * ((((cmax * sqrt(1 - (cmax * cmax))) - thmax) - (cmin * sqrt(1 - (cmin * cmin)))) + thmin);
+    }
+    if (m != lmax) {
+      slm[ORC_LM(m + 1, m)] = ((((1 - (cmax * cmax)) * plmp[ORC_LM(m, m)]) - ((1 - (cmin * cmin)) * plmm[ORC_LM(m, m)])) *
+                               (2 * m + 1)) / (m + 2.0);
+      for (int l = m + 2; l <= lmax; l++)
+        slm[ORC_LM(l, m)] = ((((l - 2) * (l + m - 1)) * slm[ORC_LM(l - 2, m)]) +
+                             ((2 * l - 1) * (((1 - (cmax * cmax)) * plmp[ORC_LM(l - 1, m)]) -
+                                             ((1 - (cmin * cmin)) * plmm[ORC_LM(l - 1, m)])))) /
+                            ((l + 1) * (l - m));
+    }
+  }
+  for (int m = 0; m <= mmax; m++) {
+    double smphi, cmphi;
+    if (m == 0) {
+      smphi = phimax - phimin;
+      cmphi = 0.0;
+    } else {
+      smphi = (sin(m * phimax) - sin(m * phimin)) / m;
+      cmphi = (cos(m * phimin) - cos(m * phimax)) / m;
+    }
+    const double norm_fact = (m == 0) ? 1.0 : ORC_SQRT2;
+    for (int l = m; l <= lmax; l++) {
+      const double norm = norm_fact * sqrt((((2.0 * l) + 1.0) / (4.0 * M_PI)) * mp_factorial_ratio(l - m, l + m));
+      slm[ORC_LM(l, m)] *= norm;
+      if (m > 0) {
+        slm[ORC_LM(l, -m)] = slm[ORC_LM(l, m)] * cmphi;
+        slm[ORC_LM(l, m)] *= smphi;
+      } else {
+        slm[ORC_LM(l, 0)] *= smphi;
+      }
+    }
+  }
+}
+
+/* corner (ii, jj, kk) of the potential array */
+static long mp_corner(const orc_sim *s, int ii, int jj, int kk)
+{
+  return ((long)ii * (s->c.n[1] + 1) + jj) * (long)(s->c.n[2] + 1) + kk;
+}
+
+/* The moments C[l,m](r) (mass inside, r^(l+3) weights) and D[l,m](r) (mass
+ * outside, r^(2-l)) of the state, summed over shells in the reference's
+ * order (gravity.c:720-793 start, 822-839 cumulative sums), then the
+ * potential at every zone corner (gravity.c:848-906).  phi: the corner
+ * array (mp_corner). */
+static void multipole_phi(const orc_sim *s, const double *P, double *phi)
+{
+  const orc_config *c = &s->c;
+  const int n1 = c->n[0], nv = s->nvars;
+  const int lmax = c->multipole_lmax, mmax = c->ndim == 3 ? lmax : 0;
+  const int L1 = lmax + 1, nlm = L1 * L1;
+  sphmap m;
+  sphmap_init(&m, c->rcoord, c->rtrans);
+  double *Clm = calloc((size_t)n1 * nlm, sizeof(double)), *Dlm = calloc((size_t)n1 * nlm, sizeof(double));
+  double ylm[ORC_MP_LMAXC * ORC_MP_LMAXC];
+  for (int kk = 0; kk < c->n[2]; kk++)
+    for (int jj = 0; jj < c->n[1]; jj++)
+      for (int ii = 0; ii < n1; ii++) {
+        if (!zone_valid(s, ii, jj, kk)) continue;
+        const double rmin = r_of_x(&m, c->startx[0] + ii * c->dx[0]);
+        const double rmax = r_of_x(&m, c->startx[0] + (ii + 1) * c->dx[0]);
+        const double thmin = th_of_x(&m, c->startx[1] + (jj * djs(s, ii)) * c->dx[1]);
+        const double thmax = th_of_x(&m, c->startx[1] + ((jj + 1) * djs(s, ii)) * c->dx[1]);
+        double phimin = 0.0, phimax = 2.0 * M_PI;
+        if (c->ndim == 3) {
+          phimin = c->startx[2] + (kk * dks(s, ii, jj)) * c->dx[2];
+          phimax = c->startx[2] + ((kk + 1) * dks(s, ii, jj)) * c->dx[2];
+        }
+        orc_ylm_ints(ylm, thmin, thmax, phimin, phimax, lmax, mmax);
+        const double rho = P[zid(s, ii, jj, kk) * nv + ORC_RHO];
+        for (int l = 0; l <= lmax; l++) {
+          const double rc = (pow(rmax, l + 3) * (1.0 - pow(rmin / rmax, l + 3))) / (l + 3);
+          double rd;
+          if (l == 0) rd = 0.5 * (OSQR(rmax) - OSQR(rmin));
+          else if (l == 1) rd = rmax - rmin;
+          else if (l == 2) rd = (rmin > 1.0e-16) ? log(rmax / rmin) : 0.0;
+          else rd = (rmin > 1.0e-16) ? (pow(rmin, 2 - l) - pow(rmax, 2 - l)) / (l - 2) : 0.0;
+          const int mm = l < mmax ? l : mmax;
+          for (int q = -mm; q <= mm; q++) {
+            Clm[(size_t)ii * nlm + ORC_LM(l, q)] += (rho * ylm[ORC_LM(l, q)]) * rc;
+            Dlm[(size_t)ii * nlm + ORC_LM(l, q)] += (rho * ylm[ORC_LM(l, q)]) * rd;
+          }
+        }
+      }
+  for (int ii = 1; ii < n1; ii++)
+    for (int l = 0; l <= lmax; l++) {
+      const int mm = l < mmax ? l : mmax;
+      for (int q = -mm; q <= mm; q++) Clm[(size_t)ii * nlm + ORC_LM(l, q)] += Clm[(size_t)(ii - 1) * nlm + ORC_LM(l, q)];
+    }
+  for (int ii = n1 - 2; ii >= 0; ii--)
+    for (int l = 0; l <= lmax; l++) {
+      const int mm = l < mmax ? l : mmax;
+      for (int q = -mm; q <= mm; q++) Dlm[(size_t)ii * nlm + ORC_LM(l, q)] += Dlm[(size_t)(ii + 1) * nlm + ORC_LM(l, q)];
+    }
+  for (int ii = 0; ii <= n1; ii++) {
+    const int njc = c->ndim > 1 ? njs(s, ii) : 0;
+    for (int jj = 0; jj <= njc; jj++) {
+      const int nkc = c->ndim == 3 ? nks(s, ii, jj) : 0;
+      for (int kk = 0; kk <= nkc; kk++) {
+        const double rm = r_of_x(&m, c->startx[0] + ii * c->dx[0]);
+        const double thm = th_of_x(&m, c->startx[1] + (jj * djs(s, ii)) * c->dx[1]);
+        const double phm = c->startx[2] + (kk * dks(s, ii, jj)) * c->dx[2];
+        orc_ylms(ylm, thm, phm, lmax, mmax);
+        double v = 0.0;
+        if (rm < 1.0e-16) {
+        } else if (ii == n1) {
+          for (int l = 1; l <= lmax; l++) {
+            const double Clfac = (((-4.0 * M_PI) * GNEWT) / (2 * l + 1)) / pow(rm, l + 1);
+            const int mm = l < mmax ? l : mmax;
+            for (int q = -mm; q <= mm; q++) v += (Clfac * ylm[ORC_LM(l, q)]) * Clm[(size_t)(ii - 1) * nlm + ORC_LM(l, q)];
+          }
+        } else {
+          for (int l = 1; l <= lmax; l++) {
+            const double Clfac = (((-4.0 * M_PI) * GNEWT) / (2 * l + 1)) / pow(rm, l + 1);
+            const double Dlfac = (((-4.0 * M_PI) * GNEWT) / (2 * l + 1)) * pow(rm, l);
+            const int mm = l < mmax ? l : mmax;
+            for (int q = -mm; q <= mm; q++) {
+              v += (Clfac * ylm[ORC_LM(l, q)]) * Clm[(size_t)(ii - 1) * nlm + ORC_LM(l, q)];
+              v += (Dlfac * ylm[ORC_LM(l, q)]) * Dlm[(size_t)ii * nlm + ORC_LM(l, q)];
+            }
+          }
+        }
+        phi[mp_corner(s, ii, jj, kk)] = v;
+      }
+    }
+  }
+  free(Clm);
+  free(Dlm);
+}
+
+/* The sources of the multipole potential (gravity.c:919-1103): per zone the
+ * gradient of the corner potential along each direction, as a weighted sum
+ * over the eight corners (weights from the volume centroid, 1/2 in phi),
+ * the momentum source -rho dPhi/dx and the energy source -<rho v> dPhi/dx
+ * with <rho v> from this stage's mass fluxes per area, weighted alike. */
+static void multipole(orc_sim *s, const double *P)
+{
+  const orc_config *c = &s->c;
+  if (c->ndim == 1) return;
+  const int n1 = c->n[0], nv = s->nvars;
+  sphmap m;
+  sphmap_init(&m, c->rcoord, c->rtrans);
+  double *phi = calloc((size_t)(n1 + 1) * (c->n[1] + 1) * (c->n[2] + 1), sizeof(double));
+  multipole_phi(s, P, phi);
+  const double *F0 = s->flux[0], *F1 = s->flux[1], *F2 = s->flux[2];
+  const double sgn[2] = {-1.0, 1.0};
+  for (int ii = 0; ii < n1; ii++)
+    for (int jj = 0; jj < njs(s, ii); jj++)
+      for (int kk = 0; kk < nks(s, ii, jj); kk++) {
+        double wx[2], wy[2] = {0.5, 0.5}, wz[2] = {0.5, 0.5};
+        double dx1 = 1.0, dx2 = 1.0;
+        const double rm = r_of_x(&m, c->startx[0] + ii * c->dx[0]);
+        const double rp = r_of_x(&m, c->startx[0] + (ii + 1) * c->dx[0]);
+        const double rc = r_of_x(&m, s->mom[1][0][ii + ORC_NG] / s->mom[2][0][ii + ORC_NG]);
+        wx[0] = (rp - rc) / (rp - rm);
+        wx[1] = (rc - rm) / (rp - rm);
+        const double dx0 = c->dx[0];
+        const int lev = lev_of(djs(s, ii));
+        {
+          const double thm = th_of_x(&m, c->startx[1] + (jj * djs(s, ii)) * c->dx[1]);
+          const double thp = th_of_x(&m, c->startx[1] + ((jj + 1) * djs(s, ii)) * c->dx[1]);
+          const double thc = th_of_x(&m, lev_arr(s, 1, 0, lev)[jj] / lev_arr(s, 2, 0, lev)[jj]);
+          wy[0] = (thp - thc) / (thp - thm);
+          wy[1] = (thc - thm) / (thp - thm);
+          dx1 = djs(s, ii) * c->dx[1];
+        }
+        if (c->ndim == 3) {
+          const double phm = c->startx[2] + (kk * dks(s, ii, jj)) * c->dx[2];
+          const double php = c->startx[2] + ((kk + 1) * dks(s, ii, jj)) * c->dx[2];
+          const double phc = 0.5 * (phm + php);
+          wz[0] = (php - phc) / (php - phm);
+          wz[1] = (phc - phm) / (php - phm);
+          dx2 = dks(s, ii, jj) * c->dx[2];
+        }
+        const int nkk = c->ndim == 3 ? 2 : 1;          /* 2D: no phi corners (m = 0 only) */
+        const long z = zid(s, ii, jj, kk);
+        const double rho = P[z * nv + ORC_RHO];
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+        for (int a = 0; a <= 1; a++) {
+          const int ip = ii + a;
+          for (int b = 0; b <= 1; b++) {
+            const int jp = ((jj + b) * djs(s, ii)) / djs(s, ip);
+            for (int q = 0; q < nkk; q++) {
+              const int kp = c->ndim == 3 ? ((kk + q) * dks(s, ii, jj)) / dks(s, ip, jp) : 0;
+              const double ph = phi[mp_corner(s, ip, jp, kp)];
+              const double w2 = c->ndim == 3 ? wz[q] : 1.0;
+              d0 += ((ph * sgn[a]) * wy[b]) * w2;
+              d1 += ((ph * wx[a]) * sgn[b]) * w2;
+              if (c->ndim == 3) d2 += ((ph * wx[a]) * wy[b]) * sgn[q];
+            }
+          }
+        }
+        /* radial <rho v>: the lower face, and the faces of the next shell over this zone */
+        const double dA0 = s->geom[z].area[0] + 1.0e-16;
+        const double vx0 = F0[z * nv + ORC_RHO] / dA0;
+        const int njp = djs(s, ii) / djs(s, ii + 1), jp0 = jj * njp;
+        double dA1 = 0.0, vx1 = 0.0;
+        for (int j = jp0; j < jp0 + njp; j++) {
+          const int nkp = dks(s, ii, jj) / dks(s, ii + 1, j), kp0 = kk * nkp;
+          for (int k = kp0; k < kp0 + nkp; k++) {
+            const long zf = zid(s, ii + 1, j, k);
+            dA1 += s->geom[zf].area[0] + 1.0e-16;
+            vx1 += F0[zf * nv + ORC_RHO];
+          }
+        }
+        vx1 /= dA1;
+        const double vx = (vx0 * wx[0]) + (vx1 * wx[1]);
+        s->src[z * nv + ORC_U1] -= (rho * d0) / dx0;
+        s->src[z * nv + ORC_ETOT] -= (vx * d0) / dx0;
+        /* theta <rho v>, with the phi level rules of the faces (gravity.c:1030-1058) */
+        double a1 = 0.0, vy1 = 0.0, a0 = 0.0, vy0 = 0.0;
+        const int dk0 = dks(s, ii, jj), dkp = dks(s, ii, jj + 1), dkm = dks(s, ii, jj - 1);
+        if (dkp > dk0) {
+          a1 += (0.5 * s->geom[zid(s, ii, jj + 1, kk / 2)].area[1]) + 1.0e-16;
+          vy1 += F1[zid(s, ii, jj + 1, kk) * nv + ORC_RHO];
+        } else if (dkp < dk0) {
+          a1 += (s->geom[zid(s, ii, jj + 1, 2 * kk)].area[1] + s->geom[zid(s, ii, jj + 1, 2 * kk + 1)].area[1]) + 1.0e-16;
+          vy1 += F1[zid(s, ii, jj + 1, 2 * kk) * nv + ORC_RHO] + F1[zid(s, ii, jj + 1, 2 * kk + 1) * nv + ORC_RHO];
+        } else {
+          a1 += s->geom[zid(s, ii, jj + 1, kk)].area[1] + 1.0e-16;
+          vy1 += F1[zid(s, ii, jj + 1, kk) * nv + ORC_RHO];
+        }
+        vy1 /= a1;
+        if (dkm < dk0) {
+          a0 += s->geom[z].area[1] + 1.0e-16;
+          vy0 += F1[zid(s, ii, jj, 2 * kk) * nv + ORC_RHO] + F1[zid(s, ii, jj, 2 * kk + 1) * nv + ORC_RHO];
+        } else {
+          a0 += s->geom[z].area[1] + 1.0e-16;
+          vy0 += F1[z * nv + ORC_RHO];
+        }
+        vy0 /= a0;
+        const double vy = (vy0 * wy[0]) + (vy1 * wy[1]);
+        s->src[z * nv + ORC_U2] -= (rho * d1) / dx1;
+        s->src[z * nv + ORC_ETOT] -= (vy * d1) / dx1;
+        if (c->ndim == 3) {
+          const long z2 = zid(s, ii, jj, kk + 1);
+          const double vz0 = F2[z * nv + ORC_RHO] / (s->geom[z].area[2] + 1.0e-16);
+          const double vz1 = F2[z2 * nv + ORC_RHO] / (s->geom[z2].area[2] + 1.0e-16);
+          const double vz = (vz0 * wz[0]) + (vz1 * wz[1]);
+          s->src[z * nv + ORC_U3] -= (rho * d2) / dx2;
+          s->src[z * nv + ORC_ETOT] -= (vz * d2) / dx2;
+        }
+      }
+  free(phi);
+}
+
+int orc_multipole_sources(orc_sim *s, double *src)
+{
+  if (s->c.multipole_lmax < 1 || s->c.ndim < 2) return -1;
+  const long n = s->nall * s->nvars;
+  for (long q = 0; q < n; q++) s->src[q] = 0.0;
+  for (int d = 0; d < 3; d++)
+    for (long q = 0; q < n; q++) s->flux[d][q] = 0.0;
+  multipole(s, s->p);
+  const orc_config *c = &s->c;
+  long o = 0;
+  for (int k = 0; k < c->n[2]; k++)
+    for (int j = 0; j < c->n[1]; j++)
+      for (int i = 0; i < c->n[0]; i++, o++)
+        for (int v = 0; v < 3; v++) src[3 * o + v] = s->src[zid(s, i, j, k) * s->nvars + ORC_U1 + v];
+  return 0;
+}
+
+int orc_multipole_phi(orc_sim *s, double *phi)
+{
+  if (s->c.multipole_lmax < 1 || s->c.ndim < 2) return -1;
+  multipole_phi(s, s->p, phi);
+  return 0;
+}
+
 /* The ghost fill that starts each stage, applied to the current state; with
  * orc_zone_prims it lets a test read any ghost zone. */
 void orc_fill_ghosts(orc_sim *s)
@@ -1621,14 +1978,20 @@ int orc_zone_prims(const orc_sim *s, int i, int j, int k, double *out)
 static void stage(orc_sim *s, int st)
 {
   double *P = st == 0 ? s->p : s->ph;
+  /* the second stage's fill runs at t + dt and takes dt off again after it
+   * (bc.c:717-721, 1417: half_step_sync), so the global t of the reference
+   * becomes ((t + dt) - dt) before the main loop adds dt */
+  if (st == 1) s->t += s->dt;
   if (s->dj) fill_ghost_dend(s, P);
   else reset_boundaries(s, P);
+  if (st == 1) s->t -= s->dt;
   update_eos(s, P);
   sources(s, P);
   if (s->dj) sweep_dend(s, P);
   else
     for (int d = 0; d < s->c.ndim; d++) sweep_dir(s, P, d);
   if (s->c.use_monopole) monopole(s, P);
+  if (s->c.multipole_lmax > 0) multipole(s, P);
   if (s->dj) {
     if (st == 0) push_dend(s, s->dt, s->p, s->p, s->ph, 0);
     else push_dend(s, s->dt, s->p, s->ph, s->p, 1);

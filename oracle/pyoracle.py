@@ -55,12 +55,17 @@ class Config(ctypes.Structure):
                 ("rtrans", ctypes.c_double),
                 ("dendritic", ctypes.c_int),
                 ("prob_bounds", ctypes.c_void_p),
-                ("prob_user", ctypes.c_void_p)]
+                ("prob_user", ctypes.c_void_p),
+                ("prob_bounds_t", ctypes.c_void_p),
+                ("multipole_lmax", ctypes.c_int)]
 
 
 # prob_bounds(i, j, k, p, user): the ghost values of a PROB boundary zone
 PROB_BOUNDS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+# prob_bounds_t(i, j, k, t, p, user): the same at the time the fill sees
+PROB_BOUNDS_T_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 def _load(fast=False):
@@ -121,6 +126,15 @@ def _load(fast=False):
     lib.orc_gr_state.restype = ctypes.c_int
     lib.orc_gr_edges.argtypes = [ctypes.c_void_p, dp]
     lib.orc_gr_edges.restype = ctypes.c_int
+    lib.orc_ylms.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    lib.orc_ylms.restype = None
+    lib.orc_ylm_ints.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int]
+    lib.orc_ylm_ints.restype = None
+    lib.orc_multipole_phi.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_multipole_phi.restype = ctypes.c_int
+    lib.orc_multipole_sources.argtypes = [ctypes.c_void_p, dp]
+    lib.orc_multipole_sources.restype = ctypes.c_int
     lib.orc_eos_walk.argtypes = [ctypes.POINTER(EosTab), ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, dp]
     lib.orc_eos_walk.restype = ctypes.c_int
@@ -191,10 +205,11 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
                 geom=GEOM_CARTESIAN, rcoord=RCOORD_UNIFORM, interp_order=3,
                 g0=(0.0, 0.0, 0.0), dt_init_safety=0.1, dt_grow_safety=1.1,
                 dtmin=1.0e-200, dtmax=1.0e200, initial_dt=1.0e100, monopole=False, eos_tab=None,
-                rtrans=0.0, dendritic=False, prob_bounds=None):
+                rtrans=0.0, dendritic=False, prob_bounds=None, prob_bounds_t=None, multipole_lmax=0):
     """eos_tab: an EosTab (tabulated EOS; nvars must be 6 to carry Ye).
     rtrans: scale of the sinh radial map; dendritic: 3D spherical dendritic grid.
-    prob_bounds: f(i, j, k) -> sequence of nvars ghost values for PROB sides."""
+    prob_bounds: f(i, j, k) -> sequence of nvars ghost values for PROB sides;
+    prob_bounds_t: f(i, j, k, t) -> the same at the time the fill sees."""
     c = Config()
     c.ndim = ndim
     n = list(n) + [1] * (3 - len(n))
@@ -221,6 +236,7 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
     c.use_monopole = 2 if (monopole == 2 and monopole is not True) else int(bool(monopole))
     c.rtrans = float(rtrans)
     c.dendritic = int(bool(dendritic))
+    c.multipole_lmax = int(multipole_lmax)
     if eos_tab is not None:
         c.eos_tab = ctypes.cast(ctypes.pointer(eos_tab), ctypes.c_void_p)
         c._eos_keep = eos_tab
@@ -232,6 +248,14 @@ def make_config(ndim, n, startx, dx, bc_lo, bc_hi, gam, cfl=0.5, nvars=5,
         cb = PROB_BOUNDS_FN(_cb)
         c.prob_bounds = ctypes.cast(cb, ctypes.c_void_p)
         c._prob_keep = cb
+    if prob_bounds_t is not None:
+        def _cbt(i, j, k, t, p, user, f=prob_bounds_t, nv=nvars):
+            vals = f(i, j, k, t)
+            for v in range(nv):
+                p[v] = float(vals[v])
+        cbt = PROB_BOUNDS_T_FN(_cbt)
+        c.prob_bounds_t = ctypes.cast(cbt, ctypes.c_void_p)
+        c._prob_t_keep = cbt
     return c
 
 
@@ -287,6 +311,21 @@ class Sim:
         if self.lib.orc_gr_edges(self.h, ptr(out)) < 0:
             raise ValueError("no GR monopole in this configuration")
         return out
+
+    def multipole_phi(self):
+        """multipole potential (l >= 1) at the zone corners of the current state:
+        array [n1 + 1][n2 + 1][n3 + 1] over level-local corner indices (unused entries 0)."""
+        out = np.zeros((self.n[0] + 1) * (self.n[1] + 1) * (self.n[2] + 1))
+        if self.lib.orc_multipole_phi(self.h, ptr(out)) < 0:
+            raise ValueError("no multipole gravity in this configuration")
+        return out.reshape(self.n[0] + 1, self.n[1] + 1, self.n[2] + 1)
+
+    def multipole_sources(self):
+        """multipole momentum sources of the current state without fluxes: [3][n3][n2][n1]"""
+        out = np.zeros(3 * self.n[0] * self.n[1] * self.n[2])
+        if self.lib.orc_multipole_sources(self.h, ptr(out)) < 0:
+            raise ValueError("no multipole gravity in this configuration")
+        return np.moveaxis(out.reshape(self.n[2], self.n[1], self.n[0], 3), -1, 0)
 
     def gr_state(self):
         """GR effective monopole after the last stage: dict of gtov, mr (faces),
@@ -394,3 +433,17 @@ GEOM_GCOV = 30
 GEOM_GCON = 42
 GEOM_SCALE = 45
 GEOM_VOLUME = 57
+
+
+def ylms(th, phi, lmax, mmax):
+    """real spherical harmonics of the multipole gravity (gravity.c:518-577), index l (l + 1) + m"""
+    out = np.zeros(15 * 15)
+    lib().orc_ylms(ptr(out), th, phi, lmax, mmax)
+    return out[:(lmax + 1) ** 2]
+
+
+def ylm_ints(thmin, thmax, phimin, phimax, lmax, mmax):
+    """integrals of Y[l,m] sin(th) over a zone (gravity.c:609-697)"""
+    out = np.zeros(15 * 15)
+    lib().orc_ylm_ints(ptr(out), thmin, thmax, phimin, phimax, lmax, mmax)
+    return out[:(lmax + 1) ** 2]

