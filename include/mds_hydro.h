@@ -209,7 +209,13 @@ typedef struct mds_problem {
                               MDS_MONOPOLE_GR, the GR effective monopole of the
                               reference's GR_MONOPOLE build (gravity.c:142-375: TOV
                               mass and factor, pressure and inertia terms, lapses) */
+  int multipole_lmax;      /* multipole gravity (SPHERICAL_MULTIPOLE_GRAV, gravity.c:499-1110):
+                              the l = 1..multipole_lmax terms of the potential on top
+                              of the monopole (needs monopole != 0), at most
+                              MDS_MULTIPOLE_LMAX; 2D and 3D spherical grids (2D: m = 0
+                              only).  0 = none; the reference's default is 12 */
 } mds_problem;
+#define MDS_MULTIPOLE_LMAX 13
 
 /* Runtime parameters with the reference's names and defaults (io.c:5018-5041). */
 typedef struct mds_run_params {

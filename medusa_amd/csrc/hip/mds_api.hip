@@ -14,6 +14,7 @@
 #include "mds_physics.h"
 #include "mds_eos.h"
 #include "../../../include/mds_eos.h"
+#include "../common/mds_mp.h"
 
 
 extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int mono, const Sweep1DArgs *a,
@@ -381,6 +382,59 @@ static int build_dendritic_lists(mds_ctx *c, const mds_grid *g, Sph3Launch &L)
   return 0;
 }
 
+// Multipole gravity: the host tables on the device and the stage work arrays.
+static int build_multipole(mds_ctx *c, const mds_grid *g, int lmax, Sph3Launch &L)
+{
+  mds_mp_tab t;
+  const int rc = mds_mp_tab_build(g, lmax, &t);
+  if (rc) return set_err(c, rc, "multipole gravity tables");
+  MpDev &M = L.mp;
+  memset(&M, 0, sizeof(M));
+  M.lmax = t.lmax;
+  M.mmax = t.mmax;
+  M.nlm = t.nlm;
+  M.maxklev = t.maxklev;
+  std::vector<short2> lms;
+  for (int l = 0; l <= lmax; l++) {
+    const int mm = l < t.mmax ? l : t.mmax;
+    for (int m = -mm; m <= mm; m++) lms.push_back(make_short2((short)MDS_MP_LM(l, m), (short)l));
+  }
+  M.nlist = (int)lms.size();
+  const long long rows = (long long)g->n[1] * g->n[2];
+  M.npart = (int)((rows + 4 * MDS_MP_ROWS - 1) / (4 * MDS_MP_ROWS));
+  const int NJ = t.maxjlev + 1, NK = t.maxklev + 1, n1 = g->n[0], n2 = g->n[1], n3 = g->n[2], L1 = lmax + 1;
+  int e = MDS_OK;
+  auto up = [&](const void *h, size_t bytes, const void **d) {
+    if (e == MDS_OK) e = upload_geo(c, h, bytes, d);
+  };
+  up(lms.data(), lms.size() * sizeof(short2), (const void **)&M.lms);
+  up(t.S, sizeof(double) * (size_t)t.nS, (const void **)&M.S);
+  up(t.Soff, sizeof(long long) * NJ * NK, (const void **)&M.Soff);
+  up(t.Y, sizeof(double) * (size_t)(n2 + 1) * (n3 + 1) * t.nlm, (const void **)&M.Y);
+  up(t.rc, sizeof(double) * (size_t)n1 * L1, (const void **)&M.rc);
+  up(t.rd, sizeof(double) * (size_t)n1 * L1, (const void **)&M.rd);
+  up(t.Cf, sizeof(double) * (size_t)(n1 + 1) * L1, (const void **)&M.Cf);
+  up(t.Df, sizeof(double) * (size_t)(n1 + 1) * L1, (const void **)&M.Df);
+  up(t.rm, sizeof(double) * (size_t)(n1 + 1), (const void **)&M.rm);
+  up(t.wx, sizeof(double) * 2 * (size_t)n1, (const void **)&M.wx);
+  up(t.wy, sizeof(double) * 2 * (size_t)NJ * n2, (const void **)&M.wy);
+  up(t.wz, sizeof(double) * 2 * (size_t)NK * n3, (const void **)&M.wz);
+  mds_mp_tab_free(&t);
+  if (e) return e;
+  auto work = [&](size_t bytes, double **d) {
+    void *m = nullptr;
+    if (e == MDS_OK && hipMalloc(&m, bytes) != hipSuccess) e = set_err(c, MDS_ENOMEM, "multipole work arrays");
+    if (e == MDS_OK) { c->geo_allocs.push_back(m); hipMemset(m, 0, bytes); *d = (double *)m; }
+  };
+  work(sizeof(double) * 2 * (size_t)M.nlist * M.npart * n1, &M.part);
+  work(sizeof(double) * (size_t)M.nlist * n1, &M.C);
+  work(sizeof(double) * (size_t)M.nlist * n1, &M.D);
+  work(sizeof(double) * (size_t)(n1 + 1) * (n2 + 1) * (n3 + 1), &M.phi);
+  work(sizeof(double) * 6 * (size_t)n1 * n2 * n3, &M.src);
+  if (e == MDS_OK) L.mpl = lmax;
+  return e;
+}
+
 static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const mds_run_params *rp)
 {
   c->nprob = nprob;
@@ -430,6 +484,11 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   for (int p = 1; p < nprob; p++)
     if ((probs[p].eos != probs[0].eos) || (probs[p].monopole != c->mono))
       return set_err(c, MDS_EINVAL, "problem %d: every problem of a batch needs the same EOS and gravity", p);
+  if (probs[0].multipole_lmax < 0 || probs[0].multipole_lmax > MDS_MULTIPOLE_LMAX)
+    return set_err(c, MDS_EINVAL, "multipole_lmax must be in [0, %d]", MDS_MULTIPOLE_LMAX);
+  if (probs[0].multipole_lmax > 0 &&
+      (!c->mono || probs[0].grid->geom != MDS_GEOM_SPHERICAL || probs[0].grid->ndim < 2))
+    return set_err(c, MDS_EINVAL, "multipole gravity needs monopole gravity on a 2D or 3D spherical grid");
   if ((c->tab || c->mono) && !c->sph)
     return set_err(c, MDS_ENOTIMPL, "tabulated EOS and monopole gravity are supported on spherical grids");
   if (c->tab) {
@@ -725,6 +784,10 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     if (g->dendritic) {
       L.dend = 1;
       int rc1 = build_dendritic_lists(c, g, L);
+      if (rc1) return rc1;
+    }
+    if (probs[0].multipole_lmax > 0) {
+      int rc1 = build_multipole(c, g, probs[0].multipole_lmax, L);
       if (rc1) return rc1;
     }
   }

@@ -184,6 +184,28 @@ struct Sweep3DArgs {
   int slab;                // the general instances: PROB/EXTRAP ghost layers, hydro_interp_order < 3
 };
 
+// multipole gravity of a 3D/2D spherical grid (gravity.c:499-1110): the
+// host-made grid tables (mds_mp.h) on the device and the stage work arrays
+struct MpDev {
+  int lmax, mmax, nlm, maxklev;
+  int nlist;               // (l, m) pairs evaluated: l = 0..lmax, |m| <= min(l, mmax), in that order
+  int npart;               // row parts of the moment pass
+  const short2 *lms;       // per pair: (lm index, l)
+  const double *S;         // zone integrals of Y sin(th) per level pair and level-local (j, k)
+  const long long *Soff;
+  const double *Y;         // harmonics at the corner angles
+  const double *rc, *rd;   // moment weights per shell and l
+  const double *Cf, *Df;   // potential factors per corner radius and l
+  const double *rm;        // corner radii
+  const double *wx, *wy, *wz;   // centroid weights of the corner differences
+  double *part;            // [pair][part][n1][2]: moment parts
+  double *C, *D;           // [pair][n1]: moments inside / outside, then their running sums
+  double *phi;             // [(n1 + 1) (n2 + 1) (n3 + 1)]: potential at the corners (level-local indices)
+  double *src;             // [6][zones]: -rho dPhi/dx and -<rho v> dPhi/dx per direction
+};
+
+#define MDS_MP_ROWS 128    // (j, k) rows per wavefront in the multipole moment pass
+
 // buffers of the direction-split 3D spherical stage
 struct Sph3Launch {
   int fp32, tab, mono;
@@ -201,6 +223,8 @@ struct Sph3Launch {
   int dend;                // dendritic grid
   const int4 *irr[2];      // dendritic: radial and theta faces whose pencils cross a level change
   long long nirr[2];       // (i, j, k, 0) per face; k is the face's own phi index
+  int mpl;                 // multipole gravity: lmax (0 = none)
+  MpDev mp;
 };
 
 #define MDS_SHELL_ROWS 32  // (j, k) rows per wavefront in the 3D shell-mass pass

@@ -35,6 +35,7 @@
 #include "mds_physics.h"
 #include "mds_eos.h"
 #include "mds_gravity.h"
+#include "../common/mds_mp.h"
 
 namespace {
 
@@ -49,6 +50,7 @@ struct Sph3Args {
   const void *P, *G, *CS;     // table EOS arrays (state precision)
   unsigned short *tg;         // each zone's temperature bracket of the last EOS call
   const double *grav;         // monopole field per radial shell, or null
+  const double *mpsrc;        // multipole sources [6][zones], or null
 };
 
 MDS_DEV long long zid3(const DevProb &pb, int i, int j, int k)
@@ -816,6 +818,221 @@ __global__ __launch_bounds__(1024) void sph3_shell_field_gr(Sph3Args a, const do
 }
 
 // ---------------------------------------------------------------------------
+// multipole gravity (gravity.c:499-1110, SPHERICAL_MULTIPOLE_GRAV): the
+// l >= 1 terms of the potential on top of the monopole.  Grid-only factors
+// (zone integrals of the harmonics, harmonics at the corner angles, radial
+// weights and factors, centroid weights) come from host tables made in the
+// restatement's operation order; the kernels form the moments of the stage
+// state, their running sums, the potential at every zone corner and the
+// corner-difference sources, which the update subtracts after the monopole's.
+// ---------------------------------------------------------------------------
+// Moment parts: thread i of wave w of block (x, y, z) sums, over rows
+// r = (4 y + w) MDS_MP_ROWS + t of shell i (r = (j, k) = (r % n2, r / n2)),
+// rho S[l,m](zone) times the shell's rc and rd for the pair z; the block adds
+// its four waves in order.
+template <class R>
+__global__ __launch_bounds__(256) void sph3_mp_part(Sph3Args a, MpDev M, StateArrays<R> in)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const short2 lm = M.lms[blockIdx.z];
+  const int L1 = M.lmax + 1;
+  const int r0 = (blockIdx.y * 4 + w) * MDS_MP_ROWS;
+  const int r1 = min(r0 + MDS_MP_ROWS, n2 * n3);
+  double c = 0.0, d = 0.0;
+  if (i < n1) {
+    const int lj = dend_lj(pb, i), nj = n2 >> lj;
+    const double wc = M.rc[i * L1 + lm.y], wd = M.rd[i * L1 + lm.y];
+    for (int r = r0; r < r1; r++) {
+      const int j = r % n2, k = r / n2;
+      if (j >= nj) continue;
+      const int lk = dend_lk(pb, i, j);
+      if (k >= (n3 >> lk)) continue;
+      const double S = M.S[(M.Soff[lj * (M.maxklev + 1) + lk] + (long long)j * (n3 >> lk) + k) * M.nlm + lm.x];
+      const double rs = (double)in.v[0][zid3(pb, i, j, k)] * S;
+      c += rs * wc;
+      d += rs * wd;
+    }
+  }
+  __shared__ double sh[2][4][64];
+  sh[0][w][lane] = c;
+  sh[1][w][lane] = d;
+  __syncthreads();
+  if (w == 0 && i < n1) {
+    double *p = M.part + (((size_t)blockIdx.z * M.npart + blockIdx.y) * n1 + i) * 2;
+    p[0] = ((sh[0][0][lane] + sh[0][1][lane]) + sh[0][2][lane]) + sh[0][3][lane];
+    p[1] = ((sh[1][0][lane] + sh[1][1][lane]) + sh[1][2][lane]) + sh[1][3][lane];
+  }
+}
+
+// each shell's moments from its parts, in part order
+__global__ __launch_bounds__(256) void sph3_mp_sum(Sph3Args a, MpDev M)
+{
+  const int n1 = a.pb.n[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
+  if (i >= n1) return;
+  double c = 0.0, d = 0.0;
+  for (int y = 0; y < M.npart; y++) {
+    const double *p = M.part + (((size_t)z * M.npart + y) * n1 + i) * 2;
+    c += p[0];
+    d += p[1];
+  }
+  M.C[(size_t)z * n1 + i] = c;
+  M.D[(size_t)z * n1 + i] = d;
+}
+
+// running sums: C over the shells inside, D over the shells outside (gravity.c:822-839)
+__global__ void sph3_mp_scan(Sph3Args a, MpDev M)
+{
+  const int n1 = a.pb.n[0];
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= M.nlist) return;
+  double *C = M.C + (size_t)z * n1, *D = M.D + (size_t)z * n1;
+  for (int i = 1; i < n1; i++) C[i] += C[i - 1];
+  for (int i = n1 - 2; i >= 0; i--) D[i] += D[i + 1];
+}
+
+// the potential at every zone corner (gravity.c:848-906), one thread per box corner
+__global__ __launch_bounds__(256) void sph3_mp_phi(Sph3Args a, MpDev M)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)(n1 + 1) * (n2 + 1) * (n3 + 1)) return;
+  const int ii = (int)(q % (n1 + 1)), jj = (int)((q / (n1 + 1)) % (n2 + 1)), kk = (int)(q / ((long long)(n1 + 1) * (n2 + 1)));
+  const int lj = dend_lj(pb, ii);
+  if (jj > (n2 >> lj)) return;
+  const int lk = dend_lk(pb, ii, jj);
+  if (pb.ndim == 3 ? kk > (n3 >> lk) : kk != 0) return;
+  const int J = jj << lj, K = pb.ndim == 3 ? kk << lk : 0;
+  const double *Y = M.Y + ((size_t)J * (n3 + 1) + K) * M.nlm;
+  const int L1 = M.lmax + 1;
+  double v = 0.0;
+  const double rm = M.rm[ii];
+  if (rm < 1.0e-16) {
+  } else {
+    int z = 1;                                   // pair index: the l = 0 pair comes first
+    for (int l = 1; l <= M.lmax; l++) {
+      const double Cl = M.Cf[ii * L1 + l], Dl = M.Df[ii * L1 + l];
+      const int mm = l < M.mmax ? l : M.mmax;
+      for (int m = -mm; m <= mm; m++, z++) {
+        const double y = Y[MDS_MP_LM(l, m)];
+        // no mass inside the inner radius of a grid that does not reach the origin
+        if (ii > 0) v += (Cl * y) * M.C[(size_t)z * n1 + ii - 1];
+        if (ii < n1) v += (Dl * y) * M.D[(size_t)z * n1 + ii];
+      }
+    }
+  }
+  M.phi[((size_t)ii * (n2 + 1) + jj) * (n3 + 1) + kk] = v;
+}
+
+// Corner-difference sources (gravity.c:919-1103): per zone the potential's
+// difference along each direction over its eight corners, weighted by the
+// volume centroid (1/2 in phi), times -rho / dx and -<rho v> / dx, with
+// <rho v> from this stage's mass fluxes per area (the level rules of the
+// theta faces included), weighted alike.
+template <class R>
+__global__ __launch_bounds__(256) void sph3_mp_src(Sph3Args a, MpDev M, StateArrays<R> in)
+{
+  const DevProb &pb = a.pb;
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const long long nz = (long long)n1 * n2 * n3;
+  const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const int i = (int)(z % n1), j = (int)((z / n1) % n2), k = (int)(z / ((long long)n1 * n2));
+  const int li = dend_lj(pb, i);
+  if (j >= (n2 >> li)) return;
+  const int lk0 = dend_lk(pb, i, j);
+  if (k >= (n3 >> lk0)) return;
+  const bool d3 = pb.ndim == 3;
+  const double *phi = M.phi;
+  auto corner = [&](int ii, int jj, int kk) { return phi[((size_t)ii * (n2 + 1) + jj) * (n3 + 1) + kk]; };
+  const double wx[2] = {M.wx[2 * i], M.wx[2 * i + 1]};
+  const double wy[2] = {M.wy[2 * (li * n2 + j)], M.wy[2 * (li * n2 + j) + 1]};
+  double wz[2] = {0.5, 0.5};
+  if (d3) { wz[0] = M.wz[2 * (lk0 * n3 + k)]; wz[1] = M.wz[2 * (lk0 * n3 + k) + 1]; }
+  const double sgn[2] = {-1.0, 1.0};
+  const double dx0 = pb.dx[0], dx1 = (double)(1 << li) * pb.dx[1], dx2 = (double)(1 << lk0) * pb.dx[2];
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int ai = 0; ai <= 1; ai++) {
+    const int ip = i + ai, lip = dend_lj(pb, ip);
+    for (int b = 0; b <= 1; b++) {
+      const int jp = rescale(j + b, li, lip);
+      for (int qk = 0; qk < (d3 ? 2 : 1); qk++) {
+        const int kp = d3 ? rescale(k + qk, lk0, dend_lk(pb, ip, jp)) : 0;
+        const double ph = corner(ip, jp, kp);
+        const double w2 = d3 ? wz[qk] : 1.0;
+        d0 += ((ph * sgn[ai]) * wy[b]) * w2;
+        d1 += ((ph * wx[ai]) * sgn[b]) * w2;
+        if (d3) d2 += ((ph * wx[ai]) * wy[b]) * sgn[qk];
+      }
+    }
+  }
+  auto mflux = [&](int D, long long f, double area) -> double {
+    const R x = ((const R *)a.F[D])[f];                 // slot 0: mass
+    if constexpr (sizeof(R) == 8) return x;
+    else return area * (double)x;
+  };
+  const double rho = (double)in.v[0][z];
+  // radial <rho v>: the lower face, and the next shell's faces over this zone
+  const double ain = rec3(pb, i, j)[MDS_ZS_AREA0];
+  const double vx0 = mflux(0, fid3(pb, 0, i, j, k), ain) / (ain + 1.0e-16);
+  const int lo = dend_lj(pb, i + 1);
+  const int njp = li > lo ? 1 << (li - lo) : 1, jp0 = rescale(j, li, lo);
+  double dA1 = 0.0, vx1 = 0.0;
+  for (int jj = jp0; jj < jp0 + njp; jj++) {
+    const int l = dend_lk(pb, i + 1, jj);
+    const int nkp = l <= lk0 ? 1 << (lk0 - l) : 0;
+    const double ar = rec3(pb, i + 1, jj)[MDS_ZS_AREA0];
+    for (int kk = k * nkp; kk < k * nkp + nkp; kk++) {
+      dA1 += ar + 1.0e-16;
+      vx1 += mflux(0, fid3(pb, 0, i + 1, jj, kk), ar);
+    }
+  }
+  vx1 /= dA1;
+  const double vx = (vx0 * wx[0]) + (vx1 * wx[1]);
+  double *S = M.src;
+  S[0 * nz + z] = (rho * d0) / dx0;
+  S[1 * nz + z] = (vx * d0) / dx0;
+  // theta <rho v> with the phi level rules of its faces (gravity.c:1030-1058)
+  const int lkp = dend_lk(pb, i, j + 1), lkm = dend_lk(pb, i, j - 1);
+  const double a1 = rec3(pb, i, j)[MDS_ZS_AREA1], a1p = rec3(pb, i, j + 1)[MDS_ZS_AREA1];
+  double A1 = 0.0, vy1 = 0.0, A0 = 0.0, vy0 = 0.0;
+  if (lkp > lk0) {
+    A1 += (0.5 * a1p) + 1.0e-16;
+    vy1 += mflux(1, fid3(pb, 1, i, j + 1, k), a1p);
+  } else if (lkp < lk0) {
+    A1 += (a1p + a1p) + 1.0e-16;
+    vy1 += mflux(1, fid3(pb, 1, i, j + 1, 2 * k), a1p) + mflux(1, fid3(pb, 1, i, j + 1, 2 * k + 1), a1p);
+  } else {
+    A1 += a1p + 1.0e-16;
+    vy1 += mflux(1, fid3(pb, 1, i, j + 1, k), a1p);
+  }
+  vy1 /= A1;
+  if (lkm < lk0) {
+    A0 += a1 + 1.0e-16;
+    vy0 += mflux(1, fid3(pb, 1, i, j, 2 * k), a1) + mflux(1, fid3(pb, 1, i, j, 2 * k + 1), a1);
+  } else {
+    A0 += a1 + 1.0e-16;
+    vy0 += mflux(1, fid3(pb, 1, i, j, k), a1);
+  }
+  vy0 /= A0;
+  const double vy = (vy0 * wy[0]) + (vy1 * wy[1]);
+  S[2 * nz + z] = (rho * d1) / dx1;
+  S[3 * nz + z] = (vy * d1) / dx1;
+  if (d3) {
+    const double a2 = rec3(pb, i, j)[MDS_ZS_AREA2];
+    const double vz0 = mflux(2, fid3(pb, 2, i, j, k), a2) / (a2 + 1.0e-16);
+    const double vz1 = mflux(2, fid3(pb, 2, i, j, k + 1), a2) / (a2 + 1.0e-16);
+    const double vz = (vz0 * wz[0]) + (vz1 * wz[1]);
+    S[4 * nz + z] = (rho * d2) / dx2;
+    S[5 * nz + z] = (vz * d2) / dx2;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // face fluxes in direction D (on dendritic grids, the faces whose 4-zone
 // pencil stays on one level).  Each thread reconstructs one zone (the PPM of
 // a zone serves its two faces); the face between zones z-1 and z takes the
@@ -1301,6 +1518,18 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
     const double be = pb.mom[0][3 * (i + MDS_NG) + 1], ga = pb.mom[0][3 * (i + MDS_NG) + 2];
     src[1] += ((A * be) / ga + B) * gz;
   }
+  if (MONO == 2) {
+    // multipole gravity after the monopole (gravity.c:1162-1165, 985-1103)
+    const long long nz = (long long)n1 * n2 * n3;
+    src[2] -= a.mpsrc[0 * nz + zr];
+    src[1] -= a.mpsrc[1 * nz + zr];
+    src[3] -= a.mpsrc[2 * nz + zr];
+    src[1] -= a.mpsrc[3 * nz + zr];
+    if (pb.ndim == 3) {
+      src[4] -= a.mpsrc[4 * nz + zr];
+      src[1] -= a.mpsrc[5 * nz + zr];
+    }
+  }
   double U[6], Uh[6];
   p_to_u_x64<3, false>(qn[0], qn[1], &qn[2], gcov, U);
   p_to_u_x64<3, false>(q[0], q[1], &q[2], gcov, Uh);
@@ -1436,7 +1665,20 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
 #undef MDS_FLUX
 #undef MDS_FLUX_IRR
   const dim3 grd((unsigned)((nz + 255) / 256));
-  if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  if (L.mpl) {
+    // multipole gravity: moments of the stage state, the corner potential and
+    // the sources, after this stage's mass fluxes (all directions) are known
+    const MpDev &M = L.mp;
+    hipLaunchKernelGGL(sph3_mp_part<R>, dim3((unsigned)((pb.n[0] + 63) / 64), (unsigned)M.npart, (unsigned)M.nlist),
+                       blk, 0, st, a, M, in);
+    hipLaunchKernelGGL(sph3_mp_sum, dim3((unsigned)((pb.n[0] + 255) / 256), (unsigned)M.nlist), blk, 0, st, a, M);
+    hipLaunchKernelGGL(sph3_mp_scan, dim3((unsigned)((M.nlist + 63) / 64)), dim3(64), 0, st, a, M);
+    const long long nc = (long long)(pb.n[0] + 1) * (pb.n[1] + 1) * (pb.n[2] + 1);
+    hipLaunchKernelGGL(sph3_mp_phi, dim3((unsigned)((nc + 255) / 256)), blk, 0, st, a, M);
+    hipLaunchKernelGGL(sph3_mp_src<R>, grd, blk, 0, st, a, M, in);
+    if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 2, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+    else hipLaunchKernelGGL((sph3_update<R, 0, 2, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+  } else if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
   else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
   else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
   else hipLaunchKernelGGL((sph3_update<R, 0, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
@@ -1458,6 +1700,7 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
   a.P = L.P; a.G = L.G; a.CS = L.CS;
   a.tg = L.tg;
   a.grav = L.grav;
+  a.mpsrc = L.mpl ? L.mp.src : nullptr;
   const hipError_t e = L.dend ? launch_stage<R, 1>(L, a, sp, sph, stage, st) : launch_stage<R, 0>(L, a, sp, sph, stage, st);
   if (e != hipSuccess) return e;
   return hipGetLastError();

@@ -627,3 +627,224 @@ int mds_gr_shell_geometry(const mds_grid *g, double *out)
   vs[n1] = 0.0;
   return MDS_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Multipole gravity tables (gravity.c:499-1110) */
+#include "../common/mds_mp.h"
+#define MP_LMAXC 15
+
+static double mp_factorial_ratio(int n, int m)
+{
+  double fact = 1.0;
+  while (n > m) { fact *= n; n--; }
+  while (m > n) { fact /= m; m--; }
+  return fact;
+}
+
+/* associated Legendre P[l,m](cos th), Condon-Shortley phase (gravity.c:579-607) */
+static void mp_plms(double *plm, double th, int lmax, int mmax)
+{
+  const double cth = cos(th);
+  for (int m = 0; m <= mmax; m++) {
+    const double somx2 = sqrt((1.0 - cth) * (1.0 + cth));
+    double fact = 1.0, pmm = 1.0;
+    for (int i = 1; i <= m; i++) {
+      pmm = ((-pmm) * fact) * somx2;
+      fact = fact + 2.0;
+    }
+    plm[MDS_MP_LM(m, m)] = pmm;
+    if (m != lmax) {
+      plm[MDS_MP_LM(m + 1, m)] = (cth * ((2.0 * m) + 1.0)) * pmm;
+      for (int l = m + 2; l <= lmax; l++)
+        plm[MDS_MP_LM(l, m)] = (((cth * ((2.0 * l) - 1.0)) * plm[MDS_MP_LM(l - 1, m)]) -
+                                ((l + m - 1) * plm[MDS_MP_LM(l - 2, m)])) / (l - m);
+    }
+  }
+}
+
+static double mp_norm(int l, int m)
+{
+  const double norm_fact = (m == 0) ? 1.0 : 1.41421356237309504880;
+  return norm_fact * sqrt((((2.0 * l) + 1.0) / (4.0 * M_PI)) * mp_factorial_ratio(l - m, l + m));
+}
+
+/* real harmonics at (th, phi) (gravity.c:518-577) */
+static void mp_ylms(double *ylm, double th, double phi, int lmax, int mmax)
+{
+  mp_plms(ylm, th, lmax, mmax);
+  for (int m = 0; m <= mmax; m++) {
+    const double smphi = sin(m * phi), cmphi = cos(m * phi);
+    for (int l = m; l <= lmax; l++) {
+      ylm[MDS_MP_LM(l, m)] *= mp_norm(l, m);
+      if (m > 0) {
+        ylm[MDS_MP_LM(l, -m)] = ylm[MDS_MP_LM(l, m)] * smphi;
+        ylm[MDS_MP_LM(l, m)] *= cmphi;
+      }
+    }
+  }
+}
+
+/* zone integrals of Y[l,m] sin(th) (gravity.c:609-697) */
+static void mp_ylm_ints(double *slm, double thmin, double thmax, double phimin, double phimax, int lmax, int mmax)
+{
+  double plmp[MP_LMAXC * MP_LMAXC], plmm[MP_LMAXC * MP_LMAXC];
+  const double cmin = cos(thmin), cmax = cos(thmax);
+  mp_plms(plmm, thmin, lmax, mmax);
+  mp_plms(plmp, thmax, lmax, mmax);
+  for (int m = 0; m <= mmax; m++) {
+    if (m > 1) {
+      slm[MDS_MP_LM(m, m)] = (((m * (2 * m - 1) * (2 * m - 3)) * slm[MDS_MP_LM(m - 2, m - 2)] -
+                               (cmax * plmp[MDS_MP_LM(m, m)])) + (cmin * plmm[MDS_MP_LM(m, m)])) / (m + 1.0);
+    } else if (m == 0) {
+      slm[MDS_MP_LM(0, 0)] = cmin - cmax;
+    } else {
+      slm[MDS_MP_LM(1, 1)] = 0.5 * ((((cmax * sqrt(1 - (cmax * cmax))) - thmax) - (cmin * sqrt(1 - (cmin * cmin)))) + thmin);
+    }
+    if (m != lmax) {
+      slm[MDS_MP_LM(m + 1, m)] = ((((1 - (cmax * cmax)) * plmp[MDS_MP_LM(m, m)]) -
+                                   ((1 - (cmin * cmin)) * plmm[MDS_MP_LM(m, m)])) * (2 * m + 1)) / (m + 2.0);
+      for (int l = m + 2; l <= lmax; l++)
+        slm[MDS_MP_LM(l, m)] = ((((l - 2) * (l + m - 1)) * slm[MDS_MP_LM(l - 2, m)]) +
+                                ((2 * l - 1) * (((1 - (cmax * cmax)) * plmp[MDS_MP_LM(l - 1, m)]) -
+                                                ((1 - (cmin * cmin)) * plmm[MDS_MP_LM(l - 1, m)])))) /
+                               ((l + 1) * (l - m));
+    }
+  }
+  for (int m = 0; m <= mmax; m++) {
+    double smphi, cmphi;
+    if (m == 0) {
+      smphi = phimax - phimin;
+      cmphi = 0.0;
+    } else {
+      smphi = (sin(m * phimax) - sin(m * phimin)) / m;
+      cmphi = (cos(m * phimin) - cos(m * phimax)) / m;
+    }
+    for (int l = m; l <= lmax; l++) {
+      slm[MDS_MP_LM(l, m)] *= mp_norm(l, m);
+      if (m > 0) {
+        slm[MDS_MP_LM(l, -m)] = slm[MDS_MP_LM(l, m)] * cmphi;
+        slm[MDS_MP_LM(l, m)] *= smphi;
+      } else {
+        slm[MDS_MP_LM(l, 0)] *= smphi;
+      }
+    }
+  }
+}
+
+void mds_mp_tab_free(mds_mp_tab *t)
+{
+  if (!t) return;
+  free(t->S); free(t->Soff); free(t->Y); free(t->rc); free(t->rd); free(t->Cf); free(t->Df);
+  free(t->rm); free(t->wx); free(t->wy); free(t->wz);
+  memset(t, 0, sizeof(*t));
+}
+
+int mds_mp_tab_build(const mds_grid *g, int lmax, mds_mp_tab *t)
+{
+  if (!g || !t || g->geom != MDS_GEOM_SPHERICAL || g->ndim < 2 || lmax < 1 || lmax > MDS_MP_LMAX) return MDS_EINVAL;
+  memset(t, 0, sizeof(*t));
+  maps m;
+  maps_init(&m, g);
+  const double GN = 6.67384e-8;
+  const int n1 = g->n[0], n2 = g->n[1], n3 = g->n[2], L1 = lmax + 1;
+  t->lmax = lmax;
+  t->mmax = g->ndim == 3 ? lmax : 0;
+  t->nlm = L1 * L1;
+  t->n1 = n1; t->n2 = n2; t->n3 = n3; t->ndim = g->ndim;
+  t->maxjlev = g->dj ? g->maxjlev : 0;
+  t->maxklev = g->dk ? g->maxklev : 0;
+  const int NJ = t->maxjlev + 1, NK = t->maxklev + 1, nlm = t->nlm;
+  t->Soff = calloc((size_t)NJ * NK, sizeof(long long));
+  long long tot = 0;
+  for (int lj = 0; lj < NJ; lj++)
+    for (int lk = 0; lk < NK; lk++) {
+      t->Soff[lj * NK + lk] = tot;
+      tot += (long long)(n2 >> lj) * (n3 >> lk);
+    }
+  t->nS = tot * nlm;
+  t->S = calloc((size_t)t->nS, sizeof(double));
+  t->Y = calloc((size_t)(n2 + 1) * (n3 + 1) * nlm, sizeof(double));
+  t->rc = calloc((size_t)n1 * L1, sizeof(double));
+  t->rd = calloc((size_t)n1 * L1, sizeof(double));
+  t->Cf = calloc((size_t)(n1 + 1) * L1, sizeof(double));
+  t->Df = calloc((size_t)(n1 + 1) * L1, sizeof(double));
+  t->rm = calloc((size_t)n1 + 1, sizeof(double));
+  t->wx = calloc((size_t)2 * n1, sizeof(double));
+  t->wy = calloc((size_t)2 * NJ * n2, sizeof(double));
+  t->wz = calloc((size_t)2 * NK * n3, sizeof(double));
+  if (!t->Soff || !t->S || !t->Y || !t->rc || !t->rd || !t->Cf || !t->Df || !t->rm || !t->wx || !t->wy || !t->wz) {
+    mds_mp_tab_free(t);
+    return MDS_ENOMEM;
+  }
+  double buf[MP_LMAXC * MP_LMAXC];
+  /* zone integrals (gravity.c:739-756): theta edges of the row's level, phi
+   * edges of the column's level; 2D grids integrate phi over [0, 2 pi] */
+  for (int lj = 0; lj < NJ; lj++)
+    for (int lk = 0; lk < NK; lk++)
+      for (int j = 0; j < (n2 >> lj); j++)
+        for (int k = 0; k < (n3 >> lk); k++) {
+          const double thmin = thx(&m, g->startx[1] + (j * (1 << lj)) * g->dx[1]);
+          const double thmax = thx(&m, g->startx[1] + ((j + 1) * (1 << lj)) * g->dx[1]);
+          double phimin = 0.0, phimax = 2.0 * M_PI;
+          if (g->ndim == 3) {
+            phimin = g->startx[2] + (k * (1 << lk)) * g->dx[2];
+            phimax = g->startx[2] + ((k + 1) * (1 << lk)) * g->dx[2];
+          }
+          mp_ylm_ints(buf, thmin, thmax, phimin, phimax, lmax, t->mmax);
+          memcpy(t->S + (t->Soff[lj * NK + lk] + (long long)j * (n3 >> lk) + k) * nlm, buf, sizeof(double) * nlm);
+        }
+  /* corner harmonics (gravity.c:869-873) */
+  for (int J = 0; J <= n2; J++)
+    for (int K = 0; K <= n3; K++) {
+      const double thm = thx(&m, g->startx[1] + J * g->dx[1]);
+      const double phm = g->startx[2] + K * g->dx[2];
+      mp_ylms(buf, thm, phm, lmax, t->mmax);
+      memcpy(t->Y + ((size_t)J * (n3 + 1) + K) * nlm, buf, sizeof(double) * nlm);
+    }
+  /* radial weights of the moments (gravity.c:758-766) */
+  for (int i = 0; i < n1; i++) {
+    const double rmin = rx(&m, g->startx[0] + i * g->dx[0]), rmax = rx(&m, g->startx[0] + (i + 1) * g->dx[0]);
+    for (int l = 0; l <= lmax; l++) {
+      t->rc[i * L1 + l] = (pow(rmax, l + 3) * (1.0 - pow(rmin / rmax, l + 3))) / (l + 3);
+      double rd;
+      if (l == 0) rd = 0.5 * ((rmax * rmax) - (rmin * rmin));
+      else if (l == 1) rd = rmax - rmin;
+      else if (l == 2) rd = (rmin > 1.0e-16) ? log(rmax / rmin) : 0.0;
+      else rd = (rmin > 1.0e-16) ? (pow(rmin, 2 - l) - pow(rmax, 2 - l)) / (l - 2) : 0.0;
+      t->rd[i * L1 + l] = rd;
+    }
+    /* radial centroid weights (gravity.c:932-939) */
+    const double rc = rx(&m, g->beta[0][i + NG] / g->Gamma[0][i + NG]);
+    t->wx[2 * i] = (rmax - rc) / (rmax - rmin);
+    t->wx[2 * i + 1] = (rc - rmin) / (rmax - rmin);
+  }
+  /* corner factors (gravity.c:877-904) */
+  for (int ii = 0; ii <= n1; ii++) {
+    const double rm = rx(&m, g->startx[0] + ii * g->dx[0]);
+    t->rm[ii] = rm;
+    for (int l = 1; l <= lmax; l++) {
+      t->Cf[ii * L1 + l] = (((-4.0 * M_PI) * GN) / (2 * l + 1)) / pow(rm, l + 1);
+      t->Df[ii * L1 + l] = (((-4.0 * M_PI) * GN) / (2 * l + 1)) * pow(rm, l);
+    }
+  }
+  /* theta and phi centroid weights per level (gravity.c:944-963) */
+  for (int lj = 0; lj < NJ; lj++)
+    for (int j = 0; j < (n2 >> lj); j++) {
+      const double thm = thx(&m, g->startx[1] + (j * (1 << lj)) * g->dx[1]);
+      const double thp = thx(&m, g->startx[1] + ((j + 1) * (1 << lj)) * g->dx[1]);
+      const double *be = g->lbeta[0] ? g->lbeta[0] + (size_t)lj * (n2 + 2 * NG) : g->beta[1];
+      const double *ga = g->lGamma[0] ? g->lGamma[0] + (size_t)lj * (n2 + 2 * NG) : g->Gamma[1];
+      const double thc = thx(&m, be[j + NG] / ga[j + NG]);
+      t->wy[2 * (lj * n2 + j)] = (thp - thc) / (thp - thm);
+      t->wy[2 * (lj * n2 + j) + 1] = (thc - thm) / (thp - thm);
+    }
+  for (int lk = 0; lk < NK; lk++)
+    for (int k = 0; k < (n3 >> lk); k++) {
+      const double phm = g->startx[2] + (k * (1 << lk)) * g->dx[2];
+      const double php = g->startx[2] + ((k + 1) * (1 << lk)) * g->dx[2];
+      const double phc = 0.5 * (phm + php);
+      t->wz[2 * (lk * n3 + k)] = (php - phc) / (php - phm);
+      t->wz[2 * (lk * n3 + k) + 1] = (phc - phm) / (php - phm);
+    }
+  return MDS_OK;
+}

@@ -85,7 +85,8 @@ class Problem(ctypes.Structure):
                 ("g0", ctypes.c_double * 3),
                 ("nvel", ctypes.c_int),
                 ("eos", ctypes.POINTER(EosTableDesc)),
-                ("monopole", ctypes.c_int)]
+                ("monopole", ctypes.c_int),
+                ("multipole_lmax", ctypes.c_int)]
 
 
 class RunParams(ctypes.Structure):
@@ -333,6 +334,7 @@ class Context:
                 arr[k].eos = ctypes.pointer(p["eos"].t)
             # True / 1: Newtonian, 2 (MONOPOLE_GR): the GR effective monopole
             arr[k].monopole = int(p.get("monopole", 0))
+            arr[k].multipole_lmax = int(p.get("multipole_lmax", 0))
         self._tables = [p.get("eos") for p in problems]     # keep the tables alive
         self.nstate = 6 if problems[0].get("eos") is not None else 5
         self.rp = rp if rp is not None else run_params()

@@ -1804,7 +1804,10 @@ static void multipole_phi(const orc_sim *s, const double *P, double *phi)
             const double Dlfac = (((-4.0 * M_PI) * GNEWT) / (2 * l + 1)) * pow(rm, l);
             const int mm = l < mmax ? l : mmax;
             for (int q = -mm; q <= mm; q++) {
-              v += (Clfac * ylm[ORC_LM(l, q)]) * Clm[(size_t)(ii - 1) * nlm + ORC_LM(l, q)];
+              /* the inner radius of a grid that does not reach the origin:
+               * no mass inside it (the reference reads Clm[ii - 1] before
+               * the array there, gravity.c:897) */
+              if (ii > 0) v += (Clfac * ylm[ORC_LM(l, q)]) * Clm[(size_t)(ii - 1) * nlm + ORC_LM(l, q)];
               v += (Dlfac * ylm[ORC_LM(l, q)]) * Dlm[(size_t)ii * nlm + ORC_LM(l, q)];
             }
           }

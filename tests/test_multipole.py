@@ -229,3 +229,95 @@ def test_multipole_needs_a_grid_beyond_1d():
     sim = po.Sim(cfg)
     with pytest.raises(ValueError):
         sim.multipole_phi()
+
+
+# ---------------------------------------------------------------------------
+# device parity: stars with monopole + multipole gravity against the oracle
+# ---------------------------------------------------------------------------
+def _otab(T):
+    t = T.t
+    return po.eos_tab(dict(nr=t.nr, nt=t.nt, ny=t.ny, lr0=t.lr0, lr1=t.lr1, lt0=t.lt0, lt1=t.lt1, ye0=t.ye0,
+                           ye1=t.ye1), T.data)
+
+
+def _star(kind, use_table, lmax):
+    from medusa_amd import mds, problems
+    T = mds.EosTable() if use_table else None
+    if kind == "dend":
+        n = (32, 16, 32)
+        g = problems.dend3d_grid(*n, dr0=2.0e6, rout=2.0e9)
+        bc_lo = list(problems.SPH3D_BC_LO)
+        bc_lo[0] = mds.BC_SPHERICAL_ORIGIN
+        prob = dict(grid=g, bc_lo=bc_lo, bc_hi=list(problems.SPH3D_BC_HI), gam=4.0 / 3.0, monopole=True, eos=T)
+        prims = mds.init_polytrope(g, table=T, sigma=0.05) * g.valid_mask()[None]
+        valid = g.valid_mask()
+    elif kind == "2d":
+        n = (48, 16)
+        x0, x1 = math.log(1.0e5), math.log(2.0e9)
+        g = mds.GridBuilder(2, [n[0], n[1]], [x0, -1.0], [(x1 - x0) / n[0], 2.0 / n[1]],
+                            geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_LOG)
+        prob = dict(grid=g, bc_lo=[mds.BC_REFLECT, mds.BC_SPHERICAL_ORIGIN],
+                    bc_hi=[mds.BC_OUTFLOW, mds.BC_SPHERICAL_ORIGIN], gam=4.0 / 3.0, monopole=True, eos=T)
+        prims = mds.init_polytrope(g, table=T, sigma=0.05)
+        valid = np.ones(prims[0].shape, bool)
+    else:
+        n = (32, 8, 8)
+        g, prob, prims = problems.star3d(*n, table=T, sigma=0.05)
+        valid = np.ones(prims[0].shape, bool)
+    prob = dict(prob, multipole_lmax=lmax)
+    return T, g, prob, prims, valid
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,use_table,precision,lmax", [("uniform", False, 0, 12), ("dend", True, 0, 6),
+                                                           ("dend", True, 1, 6), ("2d", False, 0, 12)])
+def test_multipole_star_matches_oracle(kind, use_table, precision, lmax):
+    from medusa_amd import mds
+    from parity_util import oracle_config
+    from test_dendritic import star_sound_speed
+    T, g, prob, prims, valid = _star(kind, use_table, lmax)
+    nsteps = 10 if precision == 0 else 20
+    print(f"[multipole] {kind} star {g.n} table={use_table} precision={precision} lmax={lmax}, {nsteps} steps",
+          flush=True)
+    ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+    ctx.upload(prims)
+    ctx.update_eos()
+    ctx.run(nsteps, graph=True)
+    dev = ctx.download()
+    kw = dict(nvars=6, monopole=True, multipole_lmax=lmax)
+    if T is not None:
+        kw["eos_tab"] = _otab(T)
+    sim = po.Sim(oracle_config(prob, 0.5, **kw))
+    sim.set_prims(prims)
+    sim.run(max_steps=nsteps)
+    ref = sim.get_prims()
+    zs = g.zone()[2:-2, 2:-2]
+    w = np.broadcast_to(zs[..., mds.ZS_VOLUME][None], ref[0].shape) * valid
+    rel = lambda a, b: float(np.sum(np.abs(a - b) * w) / np.sum(np.abs(b) * w))
+    err = {"rho": rel(dev[0], ref[0]), "u": rel(dev[1], ref[1])}
+    cs = star_sound_speed(ref, g, T, 4.0 / 3.0, valid)
+    for d in range(3 if g.ndim == 3 else 2):
+        err[f"v{d + 1}/cs"] = float(np.sum(np.abs(dev[2 + d] - ref[2 + d]) * zs[None, :, :, mds.ZS_SCALE0 + d] * w)
+                                    / np.sum(cs * w))
+    t, dt, ist = ctx.clock()
+    print(f"[multipole]   L1 {err}; t {t!r} vs {sim.t!r}", flush=True)
+    tol = 1e-12 if precision == 0 else 1e-5
+    for k, v in err.items():
+        assert v <= tol, (k, v)
+
+
+@pytest.mark.gpu
+def test_multipole_changes_the_flow():
+    """guard: the same star with the monopole only moves differently"""
+    from medusa_amd import mds
+    T, g, prob, prims, valid = _star("uniform", False, 12)
+    outs = []
+    for lmax in (0, 12):
+        ctx = mds.Context([dict(prob, multipole_lmax=lmax)], precision=0, rp=mds.run_params(cfl=0.5))
+        ctx.upload(prims)
+        ctx.update_eos()
+        ctx.run(10)
+        outs.append(ctx.download())
+    dv = float(np.max(np.abs(outs[1][3] - outs[0][3])))         # theta velocity
+    print(f"[multipole] max |dv_theta| monopole vs + multipole: {dv:.3e}", flush=True)
+    assert dv > 0.0
