@@ -319,8 +319,10 @@ def main():
                           "GB/s": 2.0 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
         return
     if args.valu_probe:
-        print(json.dumps({"valu_wave_instr_per_s": {"fp32": mds.valu_probe_rate(False),
-                                                    "fp64": mds.valu_probe_rate(True)}}), flush=True)
+        rates = {"fp32": mds.valu_probe_rate(False), "fp64": mds.valu_probe_rate(True)}
+        for k in ("rcp32", "rcp64", "sqrt32", "sqrt64"):
+            rates[k] = mds.valu_probe_rate(kind=k)
+        print(json.dumps({"valu_wave_instr_per_s": rates}), flush=True)
         return
     tiles = [int(t) for t in args.tiles.split(",")] if args.tiles else None
 

@@ -325,11 +325,13 @@ int mds_run_timed(mds_ctx *ctx, int nsteps, double ms[3]);
 int mds_copy_probe(long long nbytes, int reps, double *ms);
 
 /* VALU issue rate of the device (no counterpart in the reference): eight
- * independent FP32 (fp64 = 0) or FP64 fused multiply-add chains per lane over
- * a grid of 16 workgroups per compute unit, reps launches; returns wavefront
- * FMA instructions per second.  The capacity against which the stage
- * kernels' SQ_INSTS_VALU counts are read. */
-int mds_valu_probe(int fp64, int reps, double *wave_instr_per_s);
+ * independent chains per lane over a grid of 16 workgroups per compute unit,
+ * reps launches; returns wavefront instructions per second.  kind 0: FP32
+ * fused multiply-add, 1: FP64 fused multiply-add (the capacity against which
+ * the stage kernels' SQ_INSTS_VALU counts are read), 2/3: FP32/FP64
+ * reciprocal, 4/5: FP32/FP64 square root (the cost of the transcendental
+ * instructions beside the FMA). */
+int mds_valu_probe(int kind, int reps, double *wave_instr_per_s);
 
 /* Clock of a problem: time, last dt, steps taken (reference globals t, dt, istep). */
 int mds_get_clock(mds_ctx *ctx, int prob, double *t, double *dt, int *istep);

@@ -1340,37 +1340,49 @@ extern "C" int mds_copy_probe(long long nbytes, int reps, double *ms)
   return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
 }
 
-// eight independent FMA chains per lane: the VALU issue rate without memory
-template <class T>
+// eight independent chains per lane: the VALU issue rate without memory.
+// OP 0: fused multiply-add; 1: reciprocal (v_rcp_f32 / v_rcp_f64); 2: square
+// root (v_sqrt_f32 / v_sqrt_f64).  The reciprocal and root chains stay
+// bounded (x -> 1/x alternates, x -> sqrt(x) tends to 1).
+__device__ __forceinline__ float probe_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double probe_rcp(double x) { return __builtin_amdgcn_rcp(x); }
+__device__ __forceinline__ float probe_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double probe_sqrt(double x) { return __builtin_amdgcn_sqrt(x); }
+
+template <class T, int OP>
 __global__ __launch_bounds__(256) void valu_probe_kernel(T *out, int iters, T a, T b)
 {
   T x[8];
-  for (int k = 0; k < 8; k++) x[k] = (T)(threadIdx.x * 1.0e-3 + k);
+  for (int k = 0; k < 8; k++) x[k] = (T)(threadIdx.x * 1.0e-3 + k + 1);
   for (int i = 0; i < iters; i++) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) x[k] = fma(x[k], a, b);   // one v_fma per chain (the library builds with contraction off)
+    for (int k = 0; k < 8; k++) {
+      if constexpr (OP == 0) x[k] = fma(x[k], a, b);   // one v_fma per chain (the library builds with contraction off)
+      else if constexpr (OP == 1) x[k] = probe_rcp(x[k]);
+      else x[k] = probe_sqrt(x[k]);
+    }
   }
   T s = 0;
   for (int k = 0; k < 8; k++) s += x[k];
   out[(long long)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
-template <class T>
+template <class T, int OP>
 static hipError_t valu_probe(int reps, double *rate)
 {
   int dev = 0, ncu = 256;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  const int blocks = ncu * 16, iters = 4000;
+  const int blocks = ncu * 16, iters = OP == 0 ? 4000 : 1000;
   T *d = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipMalloc(&d, sizeof(T) * (size_t)blocks * 256);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) hipLaunchKernelGGL(valu_probe_kernel<T>, dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
+  if (e == hipSuccess) hipLaunchKernelGGL((valu_probe_kernel<T, OP>), dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
   if (e == hipSuccess) e = hipEventRecord(e0, 0);
   for (int r = 0; r < reps && e == hipSuccess; r++)
-    hipLaunchKernelGGL(valu_probe_kernel<T>, dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
+    hipLaunchKernelGGL((valu_probe_kernel<T, OP>), dim3(blocks), dim3(256), 0, 0, d, iters, (T)0.999, (T)1e-3);
   if (e == hipSuccess) e = hipEventRecord(e1, 0);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float f = 0.0f;
@@ -1382,10 +1394,18 @@ static hipError_t valu_probe(int reps, double *rate)
   return e;
 }
 
-extern "C" int mds_valu_probe(int fp64, int reps, double *wave_instr_per_s)
+extern "C" int mds_valu_probe(int kind, int reps, double *wave_instr_per_s)
 {
-  if (reps < 1 || !wave_instr_per_s) return MDS_EINVAL;
-  const hipError_t e = fp64 ? valu_probe<double>(reps, wave_instr_per_s) : valu_probe<float>(reps, wave_instr_per_s);
+  if (reps < 1 || !wave_instr_per_s || kind < 0 || kind > 5) return MDS_EINVAL;
+  hipError_t e = hipSuccess;
+  switch (kind) {
+    case 0: e = valu_probe<float, 0>(reps, wave_instr_per_s); break;
+    case 1: e = valu_probe<double, 0>(reps, wave_instr_per_s); break;
+    case 2: e = valu_probe<float, 1>(reps, wave_instr_per_s); break;
+    case 3: e = valu_probe<double, 1>(reps, wave_instr_per_s); break;
+    case 4: e = valu_probe<float, 2>(reps, wave_instr_per_s); break;
+    default: e = valu_probe<double, 2>(reps, wave_instr_per_s); break;
+  }
   return e == hipSuccess ? MDS_OK : MDS_EDEVICE;
 }
 

@@ -410,14 +410,24 @@ MDS_DEV void para_recon_f32(float qm, float q0, float qp, const ReconGeo32 &g, f
   ar = r;
 }
 
-template <bool HLLC_FLOOR>
+// EAGER: both pressure reciprocals up front, serving the ratio and both
+// shock factors (faster where shock factors are common, as on Cartesian
+// boxes); otherwise the shock factors divide only where a wavefront needs them
+template <bool HLLC_FLOOR, bool EAGER = false>
 MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l, float G_r,
                              float u_l, float u_r, float cs_l, float cs_r,
                              float &sl, float &sr, float &ss, float &p_s, int &isshock)
 {
 #pragma clang fp contract(fast)
   float p_max = fmaxf(p_l, p_r), p_min = fminf(p_l, p_r);
-  float ipmin = frcp(p_min);
+  float ipl = 0.0f, ipr = 0.0f, ipmin;
+  if constexpr (EAGER) {
+    ipl = frcp(p_l);
+    ipr = frcp(p_r);
+    ipmin = p_l <= p_r ? ipl : ipr;
+  } else {
+    ipmin = frcp(p_min);
+  }
   float pratio = p_max * ipmin;
   isshock = ((p_max - p_min) * ipmin > 0.5f) ? 1 : 0;
   p_s = 0.5f * (p_l + p_r) - 0.125f * (u_r - u_l) * (rl + rr) * (cs_l + cs_r);
@@ -425,7 +435,7 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
   if (pratio < 2.0f && p_s >= p_min && p_s <= p_max) {
   } else if (p_s < p_min) {
     float G_avg = 0.5f * (G_l + G_r);
-    float plr = fpow(p_l * frcp(p_r), (G_avg - 1.0f) * frcp(2.0f * G_avg));
+    float plr = fpow(p_l * (EAGER ? ipr : frcp(p_r)), (G_avg - 1.0f) * frcp(2.0f * G_avg));
     float icl = frcp(cs_l), icr = frcp(cs_r);
     float s0 = (plr * u_l * icl + u_r * icr + 2.0f * (plr - 1.0f) * frcp(G_avg - 1.0f)) * frcp(plr * icl + icr);
     p_s = 0.5f * (p_l * fpow(1.0f + (G_l - 1.0f) * 0.5f * icl * (u_l - s0), 2.0f * G_l * frcp(G_l - 1.0f))
@@ -445,8 +455,8 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
     p_s = (g_l * p_l + g_r * p_r - (u_r - u_l)) * frcp(g_l + g_r);
     p_s = fmaxf(p_s, 0.0f);
   }
-  float q_l = (p_s <= p_l) ? 1.0f : fsqrt(1.0f + (G_l + 1.0f) * frcp(2.0f * G_l) * (p_s * frcp(p_l) - 1.0f));
-  float q_r = (p_s <= p_r) ? 1.0f : fsqrt(1.0f + (G_r + 1.0f) * frcp(2.0f * G_r) * (p_s * frcp(p_r) - 1.0f));
+  float q_l = (p_s <= p_l) ? 1.0f : fsqrt(1.0f + (G_l + 1.0f) * frcp(2.0f * G_l) * (p_s * (EAGER ? ipl : frcp(p_l)) - 1.0f));
+  float q_r = (p_s <= p_r) ? 1.0f : fsqrt(1.0f + (G_r + 1.0f) * frcp(2.0f * G_r) * (p_s * (EAGER ? ipr : frcp(p_r)) - 1.0f));
   sl = u_l - cs_l * q_l;
   sr = u_r + cs_r * q_r;
   ss = (p_r - p_l + rl * u_l * (sl - u_l) - rr * u_r * (sr - u_r)) * frcp(rl * (sl - u_l) - rr * (sr - u_r));
@@ -501,7 +511,7 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
   float sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_f32<true>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_f32<true, CART>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
   if (ss > sr) sr = 1.001f * ss;
   if (ss < sl) sl = 1.001f * ss;
   float iscale = CART ? 1.0f : frcp(g.scale);
@@ -541,7 +551,7 @@ MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
   float sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_f32<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_f32<false, CART>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
   float ul[6], ur[6], fl[6], fr[6];
   p_uflux_f32<NV, CART, NS>(L, g, dir, ul, fl);
   p_uflux_f32<NV, CART, NS>(Rt, g, dir, ur, fr);

@@ -1532,15 +1532,25 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
   }
   double U[6], Uh[6];
   p_to_u_x64<3, false>(qn[0], qn[1], &qn[2], gcov, U);
-  p_to_u_x64<3, false>(q[0], q[1], &q[2], gcov, Uh);
   U[MDS_YI] = TAB ? U[0] * qn[5] : 0.0;
-  Uh[MDS_YI] = TAB ? Uh[0] * q[5] : 0.0;
+  // FP32-bulk stage 0 combines 1 U^n + 0 U* with U* = U^n itself: no second
+  // conversion (the FP64 path keeps the reference's combination as written)
+  const bool short0 = sizeof(R) == 4 && stage == 0;
+  if (!short0) {
+    p_to_u_x64<3, false>(q[0], q[1], &q[2], gcov, Uh);
+    Uh[MDS_YI] = TAB ? Uh[0] * q[5] : 0.0;
+  }
   const double lim_src = dt_source_control_x64<3>(qn[1], &qn[2], U, src);
   const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
+  if (short0) {
 #pragma unroll
-  for (int v = 0; v < NSLOT; v++) {
-    const double ddt = (divF[v] / vol) - src[v];
-    U[v] = ((pf * U[v]) + (phf * Uh[v])) - ((ff * dt) * ddt);
+    for (int v = 0; v < NSLOT; v++) U[v] = U[v] - dt * ((divF[v] / vol) - src[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NSLOT; v++) {
+      const double ddt = (divF[v] / vol) - src[v];
+      U[v] = ((pf * U[v]) + (phf * Uh[v])) - ((ff * dt) * ddt);
+    }
   }
   double ro, uo, vo[3];
   u_to_p_x64<3, false>(U, gcon, ro, uo, vo);

@@ -561,11 +561,14 @@ MDS_DEV void conserved_f32(const Geo32 &g, const Prim<float> &q, double U0[6], P
   U0[MDS_YI] = TAB ? U0[0] * qn.ye : 0.0;
 }
 
-// CFL and source limits of one FP32-bulk lane: the smallest of its faces'
-// dx / |signal speed|, its zone's estimate_dt bound and its source limit,
-// as one FP32 candidate, reduced once per wavefront.
+// CFL and source limits of one FP32-bulk lane.  The faces' |signal speed| /
+// dx and the zone's estimate_dt bound are rates (1 / time), kept as their
+// maximum; the source limits are times, kept as their minimum.  The lane's
+// candidate min(1 / rate, dts) is formed once, when the wavefront reduces it:
+// one reciprocal per lane instead of one per face and zone.
 struct Lim32 {
-  float dtc = __builtin_huge_valf();
+  float rate = 0.0f;
+  float dts = __builtin_huge_valf();
   unsigned nsh = 0;         // shock-tagged owned faces (wave-uniform)
   unsigned nbad = 0;        // owned zone updates with an unphysical state (wave-uniform)
   unsigned noff = 0;        // table EOS evaluations of owned zones off the table (wave-uniform)
@@ -579,7 +582,7 @@ MDS_DEV float wave_min_f32(float v)
 
 MDS_DEV void commit_limits32(DevClock *ck, unsigned long long *counters, int lane, const Lim32 &lim)
 {
-  const float w = wave_min_f32(lim.dtc);
+  const float w = wave_min_f32(fminf(frcp(lim.rate), lim.dts));
   if (lane == 0) {
     if (w < __builtin_huge_valf()) atomicMin(&ck->mindt, dt_key((double)w));
     if (lim.nsh) atomicAdd(&counters[0], (unsigned long long)lim.nsh);
@@ -645,7 +648,7 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
   int isshock = 0;
   if (g.area >= 1.0e-16) isshock = hllc_f32<NV, !SPH, TAB>(L, Rt, g.fg, 0, Ff, sig);
   else sig = 1.0e-30f;
-  if (r.face) lim.dtc = fminf(lim.dtc, (float)pb.dx[0] * frcp(fabsf(sig)));
+  if (r.face) lim.rate = fmaxf(lim.rate, fabsf(sig) * (float)pb.idx[0]);
   lim.nsh += __popcll(__ballot(r.count && isshock));
   double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < 2 + NV; k++) F[k] = g.area * (double)Ff[k];
@@ -674,21 +677,44 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
     src[2] += drho * gz;
     src[1] += (vl + ((vr - vl) * pb.idx[0]) * (double)g.dcen) * gz;
   }
-  const double gcov[3] = {g.gcov0, 1.0, 1.0};
-  double U[6], Uh[6];
+  double U[6];
   for (int k = 0; k < 6; k++) U[k] = U0[k];
-  p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
-  Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
-  lim.dtc = fminf(lim.dtc, (float)dt_source_control_fast<NV>(qn.uu, qn.v, U, src));
-  const double pf = stage == 0 ? 1.0 : 0.5, phf = stage == 0 ? 0.0 : 0.5, ff = stage == 0 ? 1.0 : 0.5;
-  for (int k = 0; k < 2 + NV; k++) {
-    double ddt = (Fp[k] - F[k]) * g.ivol - src[k];
-    U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
+  {
+    // dt_source_control (step.c:205-223) in FP32: there is no mass source
+    // here, and the energy source is non-zero only with gravity (a uniform
+    // branch per problem), so most zones spend one reciprocal on the limit
+    const float s1 = fabsf((float)src[1]);
+    float mom = 0.0f;
+    for (int d = 0; d < NV; d++) mom += (float)qn.v[d] * (float)src[2 + d];
+    float t = fminf(0.02f * (float)U[0], 0.02f * (float)U[1]) * 1.0e16f;
+    if (MONO || pb.g0[0] != 0.0) t = fminf(0.02f * (float)U[0] * 1.0e16f, 0.02f * (float)U[1] * frcp(s1 + 1.0e-16f));
+    t = fminf(t, (2.0f * (float)qn.uu) * frcp(fabsf(s1 - fabsf(mom)) + 1.0e-16f));
+    lim.dts = fminf(lim.dts, t);
   }
-  if (TAB) {
-    const int k = MDS_YI;
-    double ddt = (Fp[k] - F[k]) * g.ivol;
-    U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
+  if (stage == 0) {
+    // U* = U^n - dt L(U^n): the stage state is U^n itself, so the RK
+    // combination 1 U^n + 0 U* needs no second conversion (same bits for
+    // finite states)
+    for (int k = 0; k < 2 + NV; k++) {
+      double ddt = (Fp[k] - F[k]) * g.ivol - src[k];
+      U[k] = U[k] - dt * ddt;
+    }
+    if (TAB) U[MDS_YI] = U[MDS_YI] - dt * ((Fp[MDS_YI] - F[MDS_YI]) * g.ivol);
+  } else {
+    const double gcov[3] = {g.gcov0, 1.0, 1.0};
+    double Uh[6];
+    p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
+    Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
+    const double pf = 0.5, phf = 0.5, ff = 0.5;
+    for (int k = 0; k < 2 + NV; k++) {
+      double ddt = (Fp[k] - F[k]) * g.ivol - src[k];
+      U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
+    }
+    if (TAB) {
+      const int k = MDS_YI;
+      double ddt = (Fp[k] - F[k]) * g.ivol;
+      U[k] = pf * U[k] + phf * Uh[k] - (ff * dt) * ddt;
+    }
   }
   // u -> p (fluid.c:27-53): v^d = gcon[d] m_d / rho with gcon[0] = 1 / gcov0,
   // one reciprocal of rho gcov0 for the radial column
@@ -704,11 +730,13 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
   for (int d = 0; d < NV; d++) o.v[d] = (float)vout[d];
   if (TAB) o.ye = (float)(U[MDS_YI] * (SPH ? rg * g.gcov0 : rg));
   if (stage == 1) {
-    // estimate_dt zone bound (step.c:177-187) with 1 / scale[0][0] = rsqrt(gcov0)
-    if (!TAB) cs = fsqrt(gam * press * frcp(q.rho));
-    const float isc0 = SPH ? frsqrt((float)g.gcov0) : 1.0f;
-    const float loc = (fabsf((float)vout[0]) + cs * isc0) * (float)pb.idx[0];
-    lim.dtc = fminf(lim.dtc, frcp(loc));
+    // estimate_dt zone bound (step.c:177-187) with 1 / scale[0][0] = rsqrt(gcov0):
+    // the gamma-law sound speed over the scale factor as one root
+    float csi;
+    if (!TAB) csi = fsqrt(gam * press * frcp(SPH ? q.rho * (float)g.gcov0 : q.rho));
+    else csi = SPH ? cs * frsqrt((float)g.gcov0) : cs;
+    const float loc = (fabsf((float)vout[0]) + csi) * (float)pb.idx[0];
+    lim.rate = fmaxf(lim.rate, loc);
   }
   return o;
 }

@@ -478,10 +478,15 @@ def copy_probe_ms(nbytes, reps=20):
     return float(out[0])
 
 
-def valu_probe_rate(fp64=False, reps=5):
-    """Wavefront FMA instructions per second of the device (FP32 or FP64)."""
+VALU_PROBE_KINDS = {"fma32": 0, "fma64": 1, "rcp32": 2, "rcp64": 3, "sqrt32": 4, "sqrt64": 5}
+
+
+def valu_probe_rate(fp64=False, reps=5, kind=None):
+    """Wavefront instructions per second of the device: FMA (FP32 or FP64), or
+    the chain named by kind (a VALU_PROBE_KINDS key)."""
     out = np.zeros(1)
-    if lib().mds_valu_probe(int(bool(fp64)), reps, _ptr(out)):
+    k = int(bool(fp64)) if kind is None else VALU_PROBE_KINDS[kind]
+    if lib().mds_valu_probe(k, reps, _ptr(out)):
         raise RuntimeError("mds_valu_probe failed")
     return float(out[0])
 
