@@ -371,10 +371,15 @@ def main():
     if args.workload in ("sod1d", "ensemble"):
         # the launch the autotuner (or --tiles) chose: both RK stages in one, or one per stage
         two = head["launches"] == 1
-        base = {"sod1d": ("sweep1d_f32<1,false>", "sweep1d_x64<1,false>"),
-                "ensemble": ("sweep1d_f32<1,true>", "sweep1d_x64<3,true>")}[args.workload]
+        base = {"sod1d": ("sweep1d_f32<1,false,0>", "sweep1d_x64<1,false,0>"),
+                "ensemble": ("sweep1d_f32<1,true,0>", "sweep1d_x64<3,true,0>")}[args.workload]
         kname = base[0 if head_name == "fp32" else 1]
-        if two:
+        if two and head_name == "fp32":
+            # template <NV, SPH, TAB, zones per lane>
+            z = 2 if head["tiles"][3] == 2 else 1
+            kname = kname.replace("sweep1d_", "sweep1d2_").replace(",0>", f",0,{z}>") + \
+                f" (both RK stages per launch, {z} zone{'s' if z > 1 else ''} per lane)"
+        elif two:
             kname = kname.replace("sweep1d_", "sweep1d2_") + " (both RK stages per launch)"
     elif args.workload == "rt3d":
         kname = kname.replace("TY", str(head["tiles"][1]))

@@ -297,7 +297,9 @@ int mds_sync(mds_ctx *ctx);
                                     hydro_interp_order < 3) */
 #define MDS_TILE_KC3D    2       /* planes marched by one workgroup in the 3D sweep */
 #define MDS_TILE_FUSE1D  3       /* 1D: 1 = both RK stages of a step in one launch (default
-                                    when no problem is periodic in x), 0 = one launch per stage */
+                                    when no problem is periodic in x), 0 = one launch per stage,
+                                    2 = both stages in one launch with two zones per lane
+                                    (FP32-bulk only) */
 #define MDS_TILE_GRID1D  4       /* 1D grid: 0 = one wavefront per segment of the segment table,
                                     1 = persistent (the workgroups the device holds at once walk
                                     the whole table) */

@@ -66,8 +66,10 @@ struct mds_ctx {
   int nseg = 0;
   int2 *dseg2 = nullptr;           // 1D segment table of the two-stage kernel
   int nseg2 = 0;
+  int2 *dseg3 = nullptr;           // the same for two zones per lane (FP32-bulk)
+  int nseg3 = 0;
   int can_fuse1d = 0;              // 1D and no problem periodic in x
-  int fuse1d = 0;                  // use the two-stage 1D kernel (p -> ph, then swap)
+  int fuse1d = 0;                  // use the two-stage 1D kernel (p -> ph, then swap); 2: two zones per lane
   int par = 0;                     // buffer parity: which allocation p currently names
   unsigned char *dflags = nullptr;
   unsigned long long *dcount = nullptr;
@@ -332,6 +334,7 @@ extern "C" int mds_destroy(mds_ctx *c)
   if (c->dclock) hipFree(c->dclock);
   if (c->dseg) hipFree(c->dseg);
   if (c->dseg2) hipFree(c->dseg2);
+  if (c->dseg3) hipFree(c->dseg3);
   if (c->dflags) hipFree(c->dflags);
   if (c->dcount) hipFree(c->dcount);
   if (c->dzstart) hipFree(c->dzstart);
@@ -816,6 +819,12 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     c->nseg2 = (int)seg.size();
     HIPCHK(c, hipMalloc(&c->dseg2, sizeof(int2) * seg.size()));
     HIPCHK(c, hipMemcpy(c->dseg2, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
+    seg.clear();
+    for (int p : order)
+      for (int s = 0; s < c->hprob[p].n[0]; s += MDS_SEG1D2Z) seg.push_back(make_int2(p, s));
+    c->nseg3 = (int)seg.size();
+    HIPCHK(c, hipMalloc(&c->dseg3, sizeof(int2) * seg.size()));
+    HIPCHK(c, hipMemcpy(c->dseg3, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice));
     // the two-stage kernel needs every ghost source zone inside the wave, and
     // no field that depends on the whole column between the stages
     c->can_fuse1d = !c->mono && !c->has_slab && rp->hydro_interp_order == 3;
@@ -1036,6 +1045,7 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
   }
   if (c->ndim == 1) {
     Sweep1DArgs a;
+    a.zpl = 1;
     a.seg = c->dseg;
     a.nseg = c->nseg;
     a.prob = c->dprob;
@@ -1116,8 +1126,9 @@ static int enqueue_stage(mds_ctx *c, int stage, int fp32)
 static int enqueue_fused1d(mds_ctx *c)
 {
   Sweep1DArgs a;
-  a.seg = c->dseg2;
-  a.nseg = c->nseg2;
+  a.zpl = c->fuse1d == 2 ? 2 : 1;
+  a.seg = a.zpl == 2 ? c->dseg3 : c->dseg2;
+  a.nseg = a.zpl == 2 ? c->nseg3 : c->nseg2;
   a.prob = c->dprob;
   a.clock = c->dclock;
   a.counters = c->dcount;
@@ -1432,9 +1443,11 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
     if (value < 1 || value > 4096) return set_err(c, MDS_EINVAL, "planes per workgroup out of range: %d", value);
     slot = &c->kc3d;
   } else if (param == MDS_TILE_FUSE1D) {
-    if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "two-stage 1D switch must be 0 or 1: %d", value);
+    if (value < 0 || value > 2) return set_err(c, MDS_EINVAL, "two-stage 1D switch must be 0, 1 or 2: %d", value);
     if (value && !c->can_fuse1d)
       return set_err(c, MDS_EINVAL, "the two-stage 1D kernel needs 1D problems without periodic x boundaries");
+    if (value == 2 && c->precision != MDS_FP32_BULK)
+      return set_err(c, MDS_EINVAL, "two zones per lane is an FP32-bulk launch shape");
     slot = &c->fuse1d;
   } else if (param == MDS_TILE_HALO3D) {
     if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "3D halo mode must be 0 or 1: %d", value);
@@ -1546,6 +1559,7 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
     for (int g : {0, 1}) {
       cand.push_back({0, g, 0});
       if (c->can_fuse1d) cand.push_back({1, g, 0});
+      if (c->can_fuse1d && c->precision == MDS_FP32_BULK) cand.push_back({2, g, 0});
     }
   } else if (c->ndim == 2) {
     for (int v : {4, 6, 8, 12, 16, 24, 32, 48, 64})

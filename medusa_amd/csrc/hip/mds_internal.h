@@ -162,6 +162,7 @@ struct Sweep1DArgs {
   int ncu;                 // compute units of the device
   int persist;             // 1: persistent grid (resident workgroups walk the segment table)
   int gen;                 // the general instances: PROB/EXTRAP ghost layers, hydro_interp_order < 3
+  int zpl;                 // two-stage FP32-bulk kernel: zones per lane (1 or 2)
 };
 
 struct Sweep2DArgs {
@@ -230,3 +231,4 @@ struct Sph3Launch {
 #define MDS_SHELL_ROWS 32  // (j, k) rows per wavefront in the 3D shell-mass pass
 #define MDS_SEG1D 60       // output zones per 64-lane wave in the 1D stage sweep
 #define MDS_SEG1D2 56      // output zones per 64-lane wave in the 1D two-stage sweep
+#define MDS_SEG1D2Z 120    // the same with two zones per lane (FP32-bulk)

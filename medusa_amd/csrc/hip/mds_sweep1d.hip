@@ -138,6 +138,44 @@ MDS_DEV void ghost_permute(int c, int n, int c0, int lo, int hi, Prim<R> &q)
   if (NV > 2 && (flip & 4)) q.v[2] = -q.v[2];
 }
 
+// the same with Z zones per lane (zone c0 - 2 NG + Z lane + k in slot k):
+// every slot of the source lane is fetched, the source zone's slot kept
+template <int NV, int TAB, int Z, class R>
+MDS_DEV void ghost_permute(const int (&cz)[Z], int n, int c0, int lo, int hi, Prim<R> (&q)[Z])
+{
+  if constexpr (Z == 1) {
+    ghost_permute<NV, TAB>(cz[0], n, c0, lo, hi, q[0]);
+  } else {
+    int src_lane[Z], slot[Z], flip[Z];
+#pragma unroll
+    for (int k = 0; k < Z; k++) {
+      const int s = bc_src(cz[k], n, lo, hi, flip[k]);
+      int ps = s - c0 + 2 * MDS_NG;
+      ps = ps < 0 ? 0 : (ps > 64 * Z - 1 ? 64 * Z - 1 : ps);
+      src_lane[k] = ps / Z;
+      slot[k] = ps % Z;
+    }
+    Prim<R> out[Z];
+#pragma unroll
+    for (int k = 0; k < Z; k++) {
+#pragma unroll
+      for (int m = 0; m < Z; m++) {
+        Prim<R> v;
+        v.rho = __shfl(q[m].rho, src_lane[k], 64);
+        v.uu = __shfl(q[m].uu, src_lane[k], 64);
+        for (int d = 0; d < NV; d++) v.v[d] = __shfl(q[m].v[d], src_lane[k], 64);
+        v.ye = TAB ? __shfl(q[m].ye, src_lane[k], 64) : q[m].ye;
+        if (m == 0 || slot[k] == m) out[k] = v;
+      }
+      for (int d = NV; d < 3; d++) out[k].v[d] = q[k].v[d];
+      if (flip[k] & 1) out[k].v[0] = -out[k].v[0];
+      if (NV > 2 && (flip[k] & 4)) out[k].v[2] = -out[k].v[2];
+    }
+#pragma unroll
+    for (int k = 0; k < Z; k++) q[k] = out[k];
+  }
+}
+
 template <int NV, int TAB, class R>
 MDS_DEV Prim<R> load_prims(const StateArrays<R> &in, long long off, int s, int flip)
 {
@@ -591,74 +629,16 @@ MDS_DEV void commit_limits32(DevClock *ck, unsigned long long *counters, int lan
   }
 }
 
-// One FP32-bulk RK stage of one lane: EOS table interpolation,
-// reconstruction and Riemann fluxes in FP32 (per unit area), area-weighted
-// fluxes, sources, the update and u -> p in FP64.  With the gamma law the
-// face pressures are (gamma - 1) times the face energies: para_recon is
-// homogeneous of degree one, so this is the reference's separate pressure
-// reconstruction (reconstruct.c:246-266 over P) up to rounding.
+// The update of one zone from its face fluxes (FP64, area-weighted): sources
+// (geometric, fixed or monopole gravity), the RK combination, u -> p and the
+// zone's CFL and source limits.
 template <int NV, bool SPH, int TAB, int MONO>
-MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 &g, int stage, double dt,
-                              const Prim<float> &q, const double U0[6], const Prim<double> &qn, double gz,
-                              Roles r, Lim32 &lim)
+MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage, double dt, const Prim<float> &q,
+                                    const double U0[6], const Prim<double> &qn, double gz, float press, double dpress,
+                                    float cs, const double F[6], const double Fp[6], double area_hi, Lim32 &lim)
 {
   const float gam = (float)pb.gam;
-  float press, gam1, cs = 0.0f;
-  double dpress;
-  if (TAB) {
-    double lt;
-    eos_fill_f32(eos, (double)q.rho, (double)q.uu, (double)q.ye, dpress, gam1, cs, lt);
-    press = (float)dpress;
-    lim.noff += (unsigned)__popcll(__ballot(r.own && eos_off_table(eos, (double)q.rho, (double)q.ye, lt)));
-  } else {
-    press = (gam - 1.0f) * q.uu;
-    dpress = press;
-    gam1 = gam;
-  }
-  constexpr int NQ = TAB ? 5 + NV : 2 + NV;
-  float al[NQ], ar[NQ];
-  {
-    float qv[NQ];
-    qv[0] = q.rho; qv[1] = q.uu;
-    for (int d = 0; d < NV; d++) qv[2 + d] = q.v[d];
-    if (TAB) { qv[2 + NV] = q.ye; qv[3 + NV] = press; qv[4 + NV] = gam1; }
-#pragma unroll
-    for (int k = 0; k < NQ; k++) {
-      float qm = lane_from_below(qv[k]);
-      float qp = lane_from_above(qv[k]);
-      para_recon_f32(qm, qv[k], qp, g.rg, al[k], ar[k]);
-    }
-  }
-  PState<float, NV> L, Rt;
-  L.rho = lane_from_below(ar[0]);  Rt.rho = al[0];
-  L.uu = lane_from_below(ar[1]);   Rt.uu = al[1];
-  for (int d = 0; d < NV; d++) { L.v[d] = lane_from_below(ar[2 + d]); Rt.v[d] = al[2 + d]; }
-  for (int d = NV; d < 3; d++) { L.v[d] = 0.0f; Rt.v[d] = 0.0f; }
-  if (TAB) {
-    L.ye = lane_from_below(ar[2 + NV]); Rt.ye = al[2 + NV];
-    L.P = lane_from_below(ar[3 + NV]);  Rt.P = al[3 + NV];
-    L.G = lane_from_below(ar[4 + NV]);  Rt.G = al[4 + NV];
-  } else {
-    L.ye = Rt.ye = 0.0f;
-    L.P = (gam - 1.0f) * L.uu; Rt.P = (gam - 1.0f) * Rt.uu;
-    L.G = gam; Rt.G = gam;
-  }
-
-  float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, sig = 1.0f;
-  int isshock = 0;
-  if (g.area >= 1.0e-16) isshock = hllc_f32<NV, !SPH, TAB>(L, Rt, g.fg, 0, Ff, sig);
-  else sig = 1.0e-30f;
-  if (r.face) lim.rate = fmaxf(lim.rate, fabsf(sig) * (float)pb.idx[0]);
-  lim.nsh += __popcll(__ballot(r.count && isshock));
-  double F[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < 2 + NV; k++) F[k] = g.area * (double)Ff[k];
-  if (TAB) F[MDS_YI] = g.area * (double)Ff[MDS_YI];
-  for (int k = 0; k < 2 + NV; k++) Fp[k] = lane_from_above(F[k]);
-  if (TAB) Fp[MDS_YI] = lane_from_above(F[MDS_YI]);
-  const double area_hi = MONO ? lane_from_above(g.area) : 0.0;
-
   Prim<float> o = q;
-  if (!r.out) return o;
   const double drho = q.rho;
   double dv[3] = {q.v[0], q.v[1], q.v[2]};
   double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -741,6 +721,100 @@ MDS_DEV Prim<float> stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 
   return o;
 }
 
+// One FP32-bulk RK stage of the Z consecutive zones of one lane: EOS table
+// interpolation, reconstruction and Riemann fluxes in FP32 (per unit area),
+// area-weighted fluxes, sources, the update and u -> p in FP64.  A zone's
+// neighbours are the lane's own zones, or the edge zone of the next lane
+// below or above by a DPP row shift; with Z = 2 a wavefront carries 128
+// zones, and the shifts, the wave reductions and the per-lane set-up serve
+// two zones each.  With the gamma law the face pressures are (gamma - 1)
+// times the face energies: para_recon is homogeneous of degree one, so this
+// is the reference's separate pressure reconstruction (reconstruct.c:246-266
+// over P) up to rounding.
+template <int NV, bool SPH, int TAB, int MONO, int Z>
+MDS_DEV void stage_f32(const DevProb &pb, const DevEos &eos, const Geo32 (&g)[Z], int stage, double dt,
+                       const Prim<float> (&q)[Z], const double (&U0)[Z][6], const Prim<double> (&qn)[Z],
+                       const double (&gz)[Z], const Roles (&r)[Z], Lim32 &lim, Prim<float> (&o)[Z])
+{
+  const float gam = (float)pb.gam;
+  constexpr int NQ = TAB ? 5 + NV : 2 + NV;
+  float press[Z], cs[Z], qv[Z][NQ];
+  double dpress[Z];
+#pragma unroll
+  for (int k = 0; k < Z; k++) {
+    float gam1;
+    cs[k] = 0.0f;
+    if (TAB) {
+      double lt;
+      eos_fill_f32(eos, (double)q[k].rho, (double)q[k].uu, (double)q[k].ye, dpress[k], gam1, cs[k], lt);
+      press[k] = (float)dpress[k];
+      lim.noff += (unsigned)__popcll(__ballot(r[k].own && eos_off_table(eos, (double)q[k].rho, (double)q[k].ye, lt)));
+    } else {
+      press[k] = (gam - 1.0f) * q[k].uu;
+      dpress[k] = press[k];
+      gam1 = gam;
+    }
+    qv[k][0] = q[k].rho; qv[k][1] = q[k].uu;
+    for (int d = 0; d < NV; d++) qv[k][2 + d] = q[k].v[d];
+    if (TAB) { qv[k][2 + NV] = q[k].ye; qv[k][3 + NV] = press[k]; qv[k][4 + NV] = gam1; }
+  }
+  float al[Z][NQ], ar[Z][NQ];
+#pragma unroll
+  for (int j = 0; j < NQ; j++) {
+    const float below = lane_from_below(qv[Z - 1][j]);
+    const float above = lane_from_above(qv[0][j]);
+#pragma unroll
+    for (int k = 0; k < Z; k++)
+      para_recon_f32(k > 0 ? qv[k - 1][j] : below, qv[k][j], k < Z - 1 ? qv[k + 1][j] : above, g[k].rg,
+                     al[k][j], ar[k][j]);
+  }
+  double F[Z][6], Fp[Z][6], area_hi[Z];
+#pragma unroll
+  for (int k = 0; k < Z; k++) {
+    // the lower face of zone k: the right state of the zone below, the left state of zone k
+    float arl[NQ];
+    for (int j = 0; j < NQ; j++) arl[j] = k > 0 ? ar[k - 1][j] : lane_from_below(ar[Z - 1][j]);
+    PState<float, NV> L, Rt;
+    L.rho = arl[0];  Rt.rho = al[k][0];
+    L.uu = arl[1];   Rt.uu = al[k][1];
+    for (int d = 0; d < NV; d++) { L.v[d] = arl[2 + d]; Rt.v[d] = al[k][2 + d]; }
+    for (int d = NV; d < 3; d++) { L.v[d] = 0.0f; Rt.v[d] = 0.0f; }
+    if (TAB) {
+      L.ye = arl[2 + NV]; Rt.ye = al[k][2 + NV];
+      L.P = arl[3 + NV];  Rt.P = al[k][3 + NV];
+      L.G = arl[4 + NV];  Rt.G = al[k][4 + NV];
+    } else {
+      L.ye = Rt.ye = 0.0f;
+      L.P = (gam - 1.0f) * L.uu; Rt.P = (gam - 1.0f) * Rt.uu;
+      L.G = gam; Rt.G = gam;
+    }
+    float Ff[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, sig = 1.0f;
+    int isshock = 0;
+    if (g[k].area >= 1.0e-16) isshock = hllc_f32<NV, !SPH, TAB>(L, Rt, g[k].fg, 0, Ff, sig);
+    else sig = 1.0e-30f;
+    if (r[k].face) lim.rate = fmaxf(lim.rate, fabsf(sig) * (float)pb.idx[0]);
+    lim.nsh += __popcll(__ballot(r[k].count && isshock));
+    for (int v = 0; v < 6; v++) F[k][v] = 0.0;
+    for (int v = 0; v < 2 + NV; v++) F[k][v] = g[k].area * (double)Ff[v];
+    if (TAB) F[k][MDS_YI] = g[k].area * (double)Ff[MDS_YI];
+  }
+#pragma unroll
+  for (int k = 0; k < Z; k++) {
+    // the upper face of zone k: the lower face of the zone above
+    for (int v = 0; v < 6; v++) Fp[k][v] = 0.0;
+    for (int v = 0; v < 2 + NV; v++) Fp[k][v] = k < Z - 1 ? F[k + 1][v] : lane_from_above(F[0][v]);
+    if (TAB) Fp[k][MDS_YI] = k < Z - 1 ? F[k + 1][MDS_YI] : lane_from_above(F[0][MDS_YI]);
+    area_hi[k] = MONO ? (k < Z - 1 ? g[k + 1].area : lane_from_above(g[0].area)) : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < Z; k++) {
+    o[k] = q[k];
+    if (r[k].out)
+      o[k] = zone_update_f32<NV, SPH, TAB, MONO>(pb, g[k], stage, dt, q[k], U0[k], qn[k], gz[k], press[k], dpress[k],
+                                                 cs[k], F[k], Fp[k], area_hi[k], lim);
+  }
+}
+
 }  // namespace
 
 template <int NV, bool SPH, int TAB, int MONO, int GEN>
@@ -769,35 +843,41 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     int flip;
     const int s = bc_src<GEN != 0>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<float> q = load_zone<NV, TAB, GEN != 0>(pb, in, s, flip);
-    Geo32 g;
-    load_geo32<SPH>(pb, c, n, g);
-    Roles r;
-    r.face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
-    r.count = r.face && (lane <= 61 || c == n);
-    r.out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
-    r.own = r.out;
+    Geo32 g[1];
+    load_geo32<SPH>(pb, c, n, g[0]);
+    Roles r[1];
+    r[0].face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
+    r[0].count = r[0].face && (lane <= 61 || c == n);
+    r[0].out = (lane >= 2) && (lane <= 61) && (c >= 0) && (c < n);
+    r[0].own = r[0].out;
     Prim<float> qnf = q;
-    if (stage == 1 && r.out) qnf = load_prims<NV, TAB>(sp, off, c, 0);
-    double U0[6];
-    Prim<double> qn;
-    conserved_f32<NV, SPH, TAB>(g, qnf, U0, qn);
-    const double gz = (MONO && r.out) ? a.grav[off + c] : 0.0;
+    if (stage == 1 && r[0].out) qnf = load_prims<NV, TAB>(sp, off, c, 0);
+    double U0[1][6];
+    Prim<double> qn[1];
+    conserved_f32<NV, SPH, TAB>(g[0], qnf, U0[0], qn[0]);
+    const double gz[1] = {(MONO && r[0].out) ? a.grav[off + c] : 0.0};
 
     Lim32 lim;
-    const Prim<float> o = stage_f32<NV, SPH, TAB, MONO>(pb, a.eos, g, stage, dt, q, U0, qn, gz, r, lim);
-    lim.nbad += bad_state_lanes(r.out, o.rho, o.uu, !TAB);
-    if (r.out) {
+    const Prim<float> qa[1] = {q};
+    Prim<float> o[1];
+    stage_f32<NV, SPH, TAB, MONO, 1>(pb, a.eos, g, stage, dt, qa, U0, qn, gz, r, lim, o);
+    lim.nbad += bad_state_lanes(r[0].out, o[0].rho, o[0].uu, !TAB);
+    if (r[0].out) {
       StateArrays<float> &out = (stage == 0) ? sph : sp;
-      store_prims<NV, TAB>(out, off, c, o);
+      store_prims<NV, TAB>(out, off, c, o[0]);
     }
     commit_limits32(ck, a.counters, lane, lim);
   }
 }
 
-template <int NV, bool SPH, int TAB>
+// Z zones per lane: the wave holds zones c0 - 4 + p, p = Z lane + k in
+// [0, 64 Z); stage 0 is valid for p in [2, 64 Z - 3], stage 1 for the owned
+// p in [4, 64 Z - 5] (64 Z - 8 zones per segment).
+template <int NV, bool SPH, int TAB, int Z>
 __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<float> in,
                                                     StateArrays<float> out)
 {
+  constexpr int W = 64 * Z, SEG = W - 4 * MDS_NG;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nw = (int)((gridDim.x * blockDim.x) >> 6);
@@ -810,34 +890,54 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
     DevClock *ck = &a.clock[sg.x];
     const int n = pb.n[0];
     const int c0 = sg.y;
-    const int c = c0 - 2 * MDS_NG + lane;
     const long long off = pb.off;
-    const bool own = (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2) && (c < n);
+    int cz[Z];
+    bool own[Z];
+#pragma unroll
+    for (int k = 0; k < Z; k++) {
+      const int p = Z * lane + k;
+      cz[k] = c0 - 2 * MDS_NG + p;
+      own[k] = (p >= 2 * MDS_NG) && (p < 2 * MDS_NG + SEG) && (cz[k] < n);
+    }
     if (!ck->active) {
-      if (own) store_prims<NV, TAB>(out, off, c, load_prims<NV, TAB>(in, off, c, 0));
+#pragma unroll
+      for (int k = 0; k < Z; k++)
+        if (own[k]) store_prims<NV, TAB>(out, off, cz[k], load_prims<NV, TAB>(in, off, cz[k], 0));
       continue;
     }
     const double dt = ck->dt;
-    int flip;
-    const int s = bc_src<false>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
-    const Prim<float> q = load_zone<NV, TAB, false>(pb, in, s, flip);
-    Geo32 g;
-    load_geo32<SPH>(pb, c, n, g);
-    double U0[6];
-    Prim<double> qn;
-    conserved_f32<NV, SPH, TAB>(g, q, U0, qn);
-
-    const bool inside_f = (c >= 0) && (c <= n), inside_z = (c >= 0) && (c < n);
-    const bool owned_face = inside_f && (lane >= 2 * MDS_NG) && (lane < 2 * MDS_NG + MDS_SEG1D2 || c == n);
+    Prim<float> q[Z];
+    Geo32 g[Z];
+    double U0[Z][6];
+    Prim<double> qn[Z];
+    Roles r0[Z], r1[Z];
+#pragma unroll
+    for (int k = 0; k < Z; k++) {
+      int flip;
+      const int s = bc_src<false>(cz[k], n, pb.bc_lo[0], pb.bc_hi[0], flip);
+      q[k] = load_zone<NV, TAB, false>(pb, in, s, flip);
+      load_geo32<SPH>(pb, cz[k], n, g[k]);
+      conserved_f32<NV, SPH, TAB>(g[k], q[k], U0[k], qn[k]);
+      const int p = Z * lane + k;
+      const bool inside_f = (cz[k] >= 0) && (cz[k] <= n), inside_z = (cz[k] >= 0) && (cz[k] < n);
+      const bool owned_face = inside_f && (p >= 2 * MDS_NG) && (p < 2 * MDS_NG + SEG || cz[k] == n);
+      r0[k] = Roles{inside_f && p >= 2 && p <= W - 2, owned_face, inside_z && p >= 2 && p <= W - 3, own[k]};
+      r1[k] = Roles{inside_f && p >= 4 && p <= W - 4, owned_face, inside_z && p >= 4 && p <= W - 5, own[k]};
+    }
+    const double gz[Z] = {};
     Lim32 lim;
-    const Roles r0{inside_f && lane >= 2 && lane <= 62, owned_face, inside_z && lane >= 2 && lane <= 61, own};
-    Prim<float> h = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 0, dt, q, U0, qn, 0.0, r0, lim);
-    lim.nbad += bad_state_lanes(own, h.rho, h.uu, !TAB);
-    ghost_permute<NV, TAB>(c, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
-    const Roles r1{inside_f && lane >= 4 && lane <= 60, owned_face, inside_z && lane >= 4 && lane <= 59, own};
-    const Prim<float> o = stage_f32<NV, SPH, TAB, 0>(pb, a.eos, g, 1, dt, h, U0, qn, 0.0, r1, lim);
-    lim.nbad += bad_state_lanes(own, o.rho, o.uu, !TAB);
-    if (own) store_prims<NV, TAB>(out, off, c, o);
+    Prim<float> h[Z], o[Z];
+    stage_f32<NV, SPH, TAB, 0, Z>(pb, a.eos, g, 0, dt, q, U0, qn, gz, r0, lim, h);
+#pragma unroll
+    for (int k = 0; k < Z; k++) lim.nbad += bad_state_lanes(own[k], h[k].rho, h[k].uu, !TAB);
+    // boundary ghosts of the intermediate state, in the waves that hold some
+    if (c0 - 2 * MDS_NG < 0 || c0 - 2 * MDS_NG + W > n) ghost_permute<NV, TAB, Z>(cz, n, c0, pb.bc_lo[0], pb.bc_hi[0], h);
+    stage_f32<NV, SPH, TAB, 0, Z>(pb, a.eos, g, 1, dt, h, U0, qn, gz, r1, lim, o);
+#pragma unroll
+    for (int k = 0; k < Z; k++) {
+      lim.nbad += bad_state_lanes(own[k], o[k].rho, o[k].uu, !TAB);
+      if (own[k]) store_prims<NV, TAB>(out, off, cz[k], o[k]);
+    }
     commit_limits32(ck, a.counters, lane, lim);
   }
 }
@@ -971,7 +1071,7 @@ static int resident_blocks(K k, int block)
 
 // stage 0/1: single-stage kernels on (p, ph); stage -1: the two-stage kernel
 // with sp = input buffer and sph_ = output buffer (a->seg must then be the
-// MDS_SEG1D2 segment table).  tab: tabulated EOS with Ye; mono: monopole
+// MDS_SEG1D2 segment table, or the MDS_SEG1D2Z table when a->zpl is 2).  tab: tabulated EOS with Ye; mono: monopole
 // gravity (single-stage kernels only, a->grav filled for the stage input).
 extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int mono, const Sweep1DArgs *a,
                                          const void *sp, const void *sph_, int stage, hipStream_t st)
@@ -1009,11 +1109,16 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
   } else {
     const StateArrays<float> &p = *(const StateArrays<float> *)sp;
     const StateArrays<float> &h = *(const StateArrays<float> *)sph_;
-    if (stage < 0) {
-      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1);
-      else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0);
-      else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0);
-      else MDS_L2(sweep1d2_f32, 3, false, 0);
+    if (stage < 0 && a->zpl == 2) {
+      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 2);
+      else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0, 2);
+      else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0, 2);
+      else MDS_L2(sweep1d2_f32, 3, false, 0, 2);
+    } else if (stage < 0) {
+      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 1);
+      else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0, 1);
+      else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0, 1);
+      else MDS_L2(sweep1d2_f32, 3, false, 0, 1);
     } else {
       if (sph && tab && mono) MDS_L1(sweep1d_f32, 1, true, 1, 1);
       else if (sph && tab) MDS_L1(sweep1d_f32, 1, true, 1, 0);

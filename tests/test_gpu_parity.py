@@ -351,6 +351,12 @@ def test_results_independent_of_launch_shape(precision):
         assert np.array_equal(o, outs3[0])
 
 
+def _fuse_shapes(precision):
+    """1D launch shapes: one launch per stage, both stages in one launch, and
+    (FP32-bulk) both stages with two zones per lane"""
+    return (0, 1, 2) if precision == mds.FP32_BULK else (0, 1)
+
+
 def _batch_ctx(ens, precision, fuse, tmax=None, grid=0):
     rp = mds.run_params(cfl=0.5) if tmax is None else mds.run_params(cfl=0.5, tmax=tmax)
     ctx = mds.Context([e[1] for e in ens], precision=precision, rp=rp)
@@ -373,7 +379,7 @@ def test_persistent_1d_grid_matches_per_segment_grid(precision):
     nseg = sum(int(r[0]) for r in par) // 56
     say(f"persistent grid precision={precision} problems={len(ens)} two-stage segments~{nseg}")
     outs = []
-    for fuse in (0, 1):
+    for fuse in _fuse_shapes(precision):
         for grid in (0, 1):
             ctx = _batch_ctx(ens, precision, fuse, grid=grid)
             ctx.run(3)
@@ -390,15 +396,17 @@ def test_persistent_1d_grid_matches_per_segment_grid(precision):
 
 @pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
 def test_two_stage_1d_kernel_matches_single_stage(precision):
-    """The two-stage 1D kernel (one launch per step, buffers swapped) gives
-    the same bits as the one-launch-per-stage kernels, over odd step counts,
-    graph replays of both buffer parities and segment boundaries."""
+    """The two-stage 1D kernel (one launch per step, buffers swapped), with one
+    or (FP32-bulk) two zones per lane, gives the same bits as the
+    one-launch-per-stage kernels, over odd step counts, graph replays of both
+    buffer parities and segment boundaries."""
     par, ens = problems.ensemble(5, 60, 300, seed=77)
     g, prob, prims = problems.sod(224)          # 224 = 4 two-stage segments exactly
+    g2, prob2, prims2 = problems.sod(240)       # 240 = 2 two-zone segments exactly
     ens = list(ens)
-    for case, batch in (("ensemble", ens), ("sod224", [(g, prob, prims)])):
+    for case, batch in (("ensemble", ens), ("sod224", [(g, prob, prims)]), ("sod240", [(g2, prob2, prims2)])):
         outs = []
-        for fuse in (0, 1):
+        for fuse in _fuse_shapes(precision):
             say(f"{case} precision={precision} two-stage={fuse}")
             ctx = _batch_ctx(batch, precision, fuse)
             ctx.run(7)
@@ -407,10 +415,11 @@ def test_two_stage_1d_kernel_matches_single_stage(precision):
             ctx.run(1, graph=True)
             outs.append(([ctx.download(k) for k in range(len(batch))],
                          [ctx.clock(k) for k in range(len(batch))], ctx.counters()))
-        for a, b in zip(outs[0][0], outs[1][0]):
-            assert np.array_equal(a, b)
-        assert outs[0][1] == outs[1][1]
-        assert outs[0][2] == outs[1][2]
+        for o in outs[1:]:
+            for a, b in zip(outs[0][0], o[0]):
+                assert np.array_equal(a, b)
+            assert outs[0][1] == o[1]
+            assert outs[0][2] == o[2]
 
 
 def test_batched_columns_fp64_match_oracle_through_freeze():

@@ -22,8 +22,8 @@ def _run(prob, prims, precision, steps, graph=False):
     return ctx
 
 
-@pytest.mark.parametrize("precision", [mds.FP64, mds.FP32_BULK])
-@pytest.mark.parametrize("fuse", [0, 1])
+@pytest.mark.parametrize("precision,fuse", [(mds.FP64, 0), (mds.FP64, 1), (mds.FP32_BULK, 0), (mds.FP32_BULK, 1),
+                                            (mds.FP32_BULK, 2)])
 def test_sod_negative_internal_energy_is_counted(precision, fuse):
     g, prob, prims = problems.sod(256)
     for bad in (False, True):
