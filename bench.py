@@ -128,6 +128,8 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
     if tiles:
         for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D, mds.TILE_GRID1D,
                          mds.TILE_HALO3D), tiles):
+            if t == mds.TILE_FUSE1D and v == 2 and precision != mds.FP32_BULK:
+                v = 1           # two zones per lane is an FP32-bulk shape
             if t not in (mds.TILE_FUSE1D, mds.TILE_GRID1D) or ctx.ndim == 1:
                 ctx.set_tile(t, v)
         tuned_ms = None
