@@ -374,6 +374,17 @@ static int build_dendritic_lists(mds_ctx *c, const mds_grid *g, Sph3Launch &L)
       for (int k = 0; k < n3 / dkf; k++) th.push_back(make_int4(i, j, k, 0));
     }
   }
+  // order each list by the storage address of its faces' zones, (k, j, i)
+  // with i fastest: a wavefront's faces then share the cache lines of the
+  // i-fastest state arrays (the faces of one row of a theta level change lie
+  // side by side in i; radial faces of neighbouring shells share rows)
+  auto by_address = [](const int4 &a, const int4 &b) {
+    if (a.z != b.z) return a.z < b.z;
+    if (a.y != b.y) return a.y < b.y;
+    return a.x < b.x;
+  };
+  std::sort(rad.begin(), rad.end(), by_address);
+  std::sort(th.begin(), th.end(), by_address);
   std::vector<int4> *lists[2] = {&rad, &th};
   for (int t = 0; t < 2; t++) {
     L.nirr[t] = (long long)lists[t]->size();

@@ -1324,15 +1324,39 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
 // ---------------------------------------------------------------------------
 // zone update: sources, flux divergence, RK combination, u -> p, dt limits
 // ---------------------------------------------------------------------------
+// Zones to threads: a workgroup takes 64 consecutive i (one wavefront each)
+// at four consecutive phi positions k (four consecutive theta rows j on
+// grids with fewer than four phi zones).  The four wavefronts read the same
+// (i, j) records, and a phi face is the upper face of one and the lower face
+// of the next, so those reads come from the compute unit's cache instead of
+// once per zone from memory.
+MDS_DEV void update_zone_of(const DevProb &pb, int &i, int &j, int &k, bool &inbox)
+{
+  const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  const int nti = (n1 + 63) >> 6, w = threadIdx.x >> 6;
+  const int b = blockIdx.x, ti = b % nti, r = b / nti;
+  i = (ti << 6) + (threadIdx.x & 63);
+  if (n3 >= 4) {
+    j = r % n2;
+    k = ((r / n2) << 2) + w;
+  } else {
+    const int njg = (n2 + 3) >> 2;
+    j = ((r % njg) << 2) + w;
+    k = r / njg;
+  }
+  inbox = i < n1 && j < n2 && k < n3;
+}
+
 template <class R, int TAB, int MONO, int DEND>
 __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R> sp, StateArrays<R> sph, int stage)
 {
   const DevProb &pb = a.pb;
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
-  const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool inbox = z < (long long)n1 * n2 * n3;
-  const long long zl = inbox ? z : 0;
-  const int i0 = (int)(zl % n1), j0 = (int)((zl / n1) % n2), k0 = (int)(zl / ((long long)n1 * n2));
+  int iz, jz, kz;
+  bool inbox;
+  update_zone_of(pb, iz, jz, kz, inbox);
+  const long long zl = inbox ? zid3(pb, iz, jz, kz) : 0;
+  const int i0 = inbox ? iz : 0, j0 = inbox ? jz : 0, k0 = inbox ? kz : 0;
   // level relations with the neighbours (dendritic grids; all 0 otherwise),
   // read first from the small level tables; lq is the outer shell's level of
   // row j, its row when li == lo
@@ -1675,6 +1699,10 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
 #undef MDS_FLUX
 #undef MDS_FLUX_IRR
   const dim3 grd((unsigned)((nz + 255) / 256));
+  // the update's grid: 64 i by four phi positions (theta rows below four phi zones) per workgroup
+  const unsigned nti = (unsigned)((pb.n[0] + 63) / 64);
+  const dim3 gupd(pb.n[2] >= 4 ? nti * (unsigned)pb.n[1] * (unsigned)((pb.n[2] + 3) / 4)
+                               : nti * (unsigned)((pb.n[1] + 3) / 4) * (unsigned)pb.n[2]);
   if (L.mpl) {
     // multipole gravity: moments of the stage state, the corner potential and
     // the sources, after this stage's mass fluxes (all directions) are known
@@ -1686,12 +1714,12 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
     const long long nc = (long long)(pb.n[0] + 1) * (pb.n[1] + 1) * (pb.n[2] + 1);
     hipLaunchKernelGGL(sph3_mp_phi, dim3((unsigned)((nc + 255) / 256)), blk, 0, st, a, M);
     hipLaunchKernelGGL(sph3_mp_src<R>, grd, blk, 0, st, a, M, in);
-    if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 2, DEND>), grd, blk, 0, st, a, sp, sph, stage);
-    else hipLaunchKernelGGL((sph3_update<R, 0, 2, DEND>), grd, blk, 0, st, a, sp, sph, stage);
-  } else if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
-  else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
-  else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1, DEND>), grd, blk, 0, st, a, sp, sph, stage);
-  else hipLaunchKernelGGL((sph3_update<R, 0, 0, DEND>), grd, blk, 0, st, a, sp, sph, stage);
+    if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 2, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
+    else hipLaunchKernelGGL((sph3_update<R, 0, 2, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
+  } else if (L.tab && L.mono) hipLaunchKernelGGL((sph3_update<R, 1, 1, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
+  else if (L.tab) hipLaunchKernelGGL((sph3_update<R, 1, 0, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
+  else if (L.mono) hipLaunchKernelGGL((sph3_update<R, 0, 1, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
+  else hipLaunchKernelGGL((sph3_update<R, 0, 0, DEND>), gupd, blk, 0, st, a, sp, sph, stage);
   return e;
 }
 
