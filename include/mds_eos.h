@@ -65,6 +65,17 @@ void mds_eos_burrows_default(mds_eos_table *t);
  * Returns MDS_EIO when the file is missing or short. */
 int mds_eos_table_load_burrows(const char *path, mds_eos_table *t);
 
+/* The table at a given temperature (host): eos_given_rtx
+ * (eos/burrows/eos_stuff.c:1020-1044) through eos(eos_input_rt)
+ * (:1189-1215) and findthis (:253-313).  The records are trilinear in
+ * (log10 rho, log10 T, Ye) with the reference's weights and summation
+ * order; temp is in the table's temperature unit.  out = {u = rho e
+ * [erg/cm^3] (u_given_rtx, eos_collapse.c:172-181), P [dyn/cm^2], Gamma_1,
+ * cs = sqrt(Gamma_1 P / rho)}.  rho, temp and Ye outside the table are
+ * clamped to its edges (the reference reads beyond its arrays there).
+ * Returns MDS_EINVAL for a non-positive rho or temp. */
+int mds_eos_rtx(const mds_eos_table *t, double rho, double temp, double ye, double out[4]);
+
 /* Default synthetic table dimensions: 180 x 180 x 50 over log10 rho in
  * [3,15], log10 T in [9,12], Ye in [0.05,0.55]. */
 void mds_eos_table_default(mds_eos_table *t);

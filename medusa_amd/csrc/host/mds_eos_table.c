@@ -172,3 +172,56 @@ void mds_eos_table_free(mds_eos_table *t)
   free(t->data);
   t->data = NULL;
 }
+
+/* eos_given_rtx / findthis (eos_stuff.c:253-313, 1189-1215): node indices
+ * and fractions along each axis, the eight trilinear weights, and each
+ * record summed in the reference's corner order */
+int mds_eos_rtx(const mds_eos_table *t, double rho, double temp, double ye, double out[4])
+{
+  if (!t || !t->data || !out || !(rho > 0.0) || !(temp > 0.0)) return MDS_EINVAL;
+  const double dr = (t->lr1 - t->lr0) / (t->nr - 1);
+  const double dtt = (t->lt1 - t->lt0) / (t->nt - 1);
+  const double dy = (t->ye1 - t->ye0) / (t->ny - 1);
+  double rl = log10(rho), tl = log10(temp);
+  rl = rl < t->lr0 ? t->lr0 : (rl > t->lr1 ? t->lr1 : rl);
+  tl = tl < t->lt0 ? t->lt0 : (tl > t->lt1 ? t->lt1 : tl);
+  ye = ye < t->ye0 ? t->ye0 : (ye > t->ye1 ? t->ye1 : ye);
+  int jr = (int)((rl - t->lr0) / dr), jt = (int)((tl - t->lt0) / dtt), jy = (int)((ye - t->ye0) / dy);
+  jr = jr > t->nr - 1 ? t->nr - 1 : jr;
+  jt = jt > t->nt - 1 ? t->nt - 1 : jt;
+  jy = jy > t->ny - 1 ? t->ny - 1 : jy;
+  int jrp = jr + 1, jtp = jt + 1, jyp = jy + 1;
+  double drho = (rl - (t->lr0 + jr * dr)) / dr;
+  double dtemp = (tl - (t->lt0 + jt * dtt)) / dtt;
+  double dye = (ye - (t->ye0 + jy * dy)) / dy;
+  if (jy == t->ny - 1) { jyp = jy; dye = 0.0; }
+  if (jr == t->nr - 1) { jrp = jr; drho = 0.0; }
+  if (jt == t->nt - 1) { jtp = jt; dtemp = 0.0; }
+  double c[8];
+  c[0] = (1.0 - drho) * (1.0 - dtemp) * (1.0 - dye);
+  c[1] = (1.0 - drho) * (1.0 - dtemp) * dye;
+  c[2] = drho * (1.0 - dtemp) * (1.0 - dye);
+  c[3] = drho * (1.0 - dtemp) * dye;
+  c[4] = (1.0 - drho) * dtemp * (1.0 - dye);
+  c[5] = (1.0 - drho) * dtemp * dye;
+  c[6] = drho * dtemp * (1.0 - dye);
+  c[7] = drho * dtemp * dye;
+  const int cy[8] = {jy, jyp, jy, jyp, jy, jyp, jy, jyp};
+  const int cr[8] = {jr, jr, jrp, jrp, jr, jr, jrp, jrp};
+  const int ct[8] = {jt, jt, jt, jt, jtp, jtp, jtp, jtp};
+  double f[MDS_EOS_NFIELD];
+  for (int rec = 0; rec < MDS_EOS_NFIELD; rec++) {
+    double v = 0.0;
+    for (int k = 0; k < 8; k++)
+      v += c[k] * (double)t->data[(((size_t)rec * t->ny + cy[k]) * t->nr + cr[k]) * t->nt + ct[k]];
+    f[rec] = v;
+  }
+  const double es = t->e_scale > 0.0 ? t->e_scale : 1.0, ps = t->p_scale > 0.0 ? t->p_scale : 1.0;
+  const double eint = (f[MDS_EOS_E] + t->e_shift) * es;        /* eos_stuff.c:1204 */
+  const double P = exp(f[MDS_EOS_LNP]) * ps;                  /* :312, 1209 */
+  out[0] = rho * eint;
+  out[1] = P;
+  out[2] = f[MDS_EOS_GAM1];
+  out[3] = sqrt(out[2] * P / rho);
+  return MDS_OK;
+}

@@ -118,7 +118,7 @@ EXPORTED = ["mds_grid_build", "mds_grid_build_batch", "mds_grid_free", "mds_defa
             "mds_init_sph_riemann", "mds_ensemble_params",
             "mds_eos_table_default", "mds_eos_model_params", "mds_eos_model_eval", "mds_eos_table_build",
             "mds_eos_table_free", "mds_eos_points", "mds_init_polytrope", "mds_eos_burrows_default",
-            "mds_eos_table_load_burrows"]
+            "mds_eos_table_load_burrows", "mds_eos_rtx"]
 
 
 def lib():
@@ -182,6 +182,7 @@ def lib():
     L.mds_eos_table_free.restype = None
     L.mds_eos_points.argtypes = [ctypes.c_int, ctypes.POINTER(EosTableDesc), ctypes.c_int, _dp, _dp, _dp,
                                  _dp, _dp, _dp, _dp]
+    L.mds_eos_rtx.argtypes = [ctypes.POINTER(EosTableDesc), ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
     _LIB = L
     return L
 
@@ -652,6 +653,14 @@ def eos_model_eval(prm, rho, temp, ye):
     out = np.zeros(3)
     prm = np.ascontiguousarray(prm, dtype=np.float64)
     lib().mds_eos_model_eval(_ptr(prm), rho, temp, ye, _ptr(out))
+    return out
+
+
+def eos_rtx(table, rho, temp, ye):
+    """Host table at a given temperature (mds_eos_rtx): (u, P, Gamma_1, cs)."""
+    out = np.zeros(4)
+    if lib().mds_eos_rtx(ctypes.byref(table.t), float(rho), float(temp), float(ye), _ptr(out)):
+        raise RuntimeError("mds_eos_rtx failed")
     return out
 
 

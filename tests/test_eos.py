@@ -203,3 +203,52 @@ def test_device_eos_matches_oracle(table, precision):
           f"cs {rel[:, 1].max():.2e} T {rel[:, 2].max():.2e} Gamma1 {rel[:, 3].max():.2e}", flush=True)
     for k in range(4):
         assert rel[:, k].max() <= (1e-13 if k == 2 else tol), k
+
+
+# ---------------------------------------------------------------------------
+# The table at a given temperature (mds_eos_rtx, the restart reader's
+# u_given_rtx): exact at the nodes, linear in log T between them, and the
+# walk's inverse.
+# ---------------------------------------------------------------------------
+def test_rtx_at_nodes_is_the_node_record(table):
+    lr, lt, ye = table.nodes()
+    rng = np.random.default_rng(5)
+    d = table.data
+    for _ in range(200):
+        jy, jr, jt = rng.integers(table.t.ny - 1), rng.integers(table.t.nr - 1), rng.integers(table.t.nt - 1)
+        rho, T = 10.0 ** lr[jr], 10.0 ** lt[jt]
+        u, P, g1, cs = mds.eos_rtx(table, rho, T, ye[jy])
+        # log10(10^x) and the node fractions carry a few ulps: the node values to 1e-12
+        np.testing.assert_allclose(u, rho * float(d[0, jy, jr, jt]), rtol=1e-12)
+        np.testing.assert_allclose(P, np.exp(float(d[1, jy, jr, jt])), rtol=1e-12)
+        np.testing.assert_allclose(g1, float(d[2, jy, jr, jt]), rtol=1e-12)
+        np.testing.assert_allclose(cs, np.sqrt(g1 * P / rho), rtol=1e-15)
+
+
+def test_rtx_is_linear_in_log_temperature_between_nodes(table):
+    lr, lt, ye = table.nodes()
+    rng = np.random.default_rng(6)
+    d = table.data.astype(np.float64)
+    for _ in range(200):
+        jy, jr, jt = rng.integers(table.t.ny - 1), rng.integers(table.t.nr - 1), rng.integers(table.t.nt - 1)
+        f = rng.uniform(0.05, 0.95)
+        rho, T = 10.0 ** lr[jr], 10.0 ** (lt[jt] + f * (lt[jt + 1] - lt[jt]))
+        u = mds.eos_rtx(table, rho, T, ye[jy])[0]
+        want = rho * ((1 - f) * d[0, jy, jr, jt] + f * d[0, jy, jr, jt + 1])
+        np.testing.assert_allclose(u, want, rtol=1e-10)
+
+
+def test_rtx_inverts_the_walk(table):
+    """u(rho, T, Ye) from the forward table, then the oracle's walk from u:
+    the walk lands on T (both use the same piecewise-linear e(log T))."""
+    ot = _otab(table)
+    lr, lt, ye = table.nodes()
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        rl = rng.uniform(lr[1], lr[-2])
+        tl = rng.uniform(lt[1], lt[-2])
+        y = rng.uniform(ye[1], ye[-2])
+        u = mds.eos_rtx(table, 10.0 ** rl, 10.0 ** tl, y)[0]
+        out, st = po.eos_walk(ot, 10.0 ** rl, u, y, 0.0)
+        assert st == 0
+        assert abs(np.log10(out[2]) - tl) < 1e-9
