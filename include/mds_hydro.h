@@ -197,7 +197,8 @@ typedef struct mds_problem {
   int nvel;                /* velocity components carried; 0 = auto: ndim, except FP64
                               spherical columns, which carry 3 (the connection terms give
                               the transverse components round-off sources); FP32-bulk
-                              spherical columns carry the radial component only.  3D grids
+                              spherical columns carry the radial component by default, or
+                              all three with nvel = 3 (rotating columns).  3D grids
                               and 2D/3D spherical grids always carry 3: any other explicit
                               value is rejected with MDS_EINVAL */
   const struct mds_eos_table *eos;  /* tabulated EOS (mds_eos.h), NULL = gamma law; with a table

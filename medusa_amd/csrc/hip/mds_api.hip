@@ -488,8 +488,8 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
   if (c->sph3) c->nv = 3;
   if (c->sph && c->precision == MDS_FP64 && c->nv != 3)
     return set_err(c, MDS_EINVAL, "FP64 spherical columns carry three velocity components");
-  if (c->sph && !c->sph3 && c->precision == MDS_FP32_BULK && c->nv != 1)
-    return set_err(c, MDS_ENOTIMPL, "FP32-bulk spherical columns carry the radial velocity only");
+  if (c->sph && !c->sph3 && c->precision == MDS_FP32_BULK && c->nv != 1 && c->nv != 3)
+    return set_err(c, MDS_EINVAL, "FP32-bulk spherical columns carry the radial velocity (nvel 1) or all three (nvel 3)");
   c->tab = probs[0].eos != nullptr;
   c->mono = probs[0].monopole;
   for (int p = 0; p < nprob; p++)

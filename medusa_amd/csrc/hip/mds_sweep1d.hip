@@ -547,14 +547,25 @@ struct Geo32 {
   FaceGeo<float> fg;        // FP32 face metric for the Riemann solve (unit area)
   double ivol, gcov0, cp;
   float c000, dcen;
+  const double *zg;         // spherical columns with three velocity components: the FP64 zone record
 };
 
-template <bool SPH>
+template <bool SPH, int NV = 1>
 MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
 {
   const int q = clamp_q(c, n);
   g.fg.area = 1.0f;
   g.fg.gcov[1] = g.fg.gcov[2] = 1.0f;
+  g.zg = nullptr;
+  if (SPH && NV == 3) {
+    // the transverse components need the zone's gcov[0][1..2], gcon[1..2]
+    // and connection coefficients, and the face's gcov[1][1..2]: read from
+    // the FP64 record (rotating columns; the radial-only path keeps its
+    // 48-byte record)
+    g.zg = pb.zone + (size_t)q * MDS_ZG_COUNT;
+    g.fg.gcov[1] = (float)g.zg[MDS_ZG_GCOVF1];
+    g.fg.gcov[2] = (float)g.zg[MDS_ZG_GCOVF2];
+  }
   if (SPH) {
     const ZoneF32 &z = pb.z32[q];
     g.area = z.area;
@@ -594,7 +605,8 @@ MDS_DEV void conserved_f32(const Geo32 &g, const Prim<float> &q, double U0[6], P
 {
   qn.rho = q.rho; qn.uu = q.uu; qn.ye = q.ye;
   for (int d = 0; d < 3; d++) qn.v[d] = q.v[d];
-  const double gcov[3] = {g.gcov0, 1.0, 1.0};
+  const bool rot = SPH && NV == 3;
+  const double gcov[3] = {g.gcov0, rot ? g.zg[MDS_ZG_GCOV01] : 1.0, rot ? g.zg[MDS_ZG_GCOV02] : 1.0};
   p_to_u_x64<NV, !SPH>(qn.rho, qn.uu, qn.v, gcov, U0);
   U0[MDS_YI] = TAB ? U0[0] * qn.ye : 0.0;
 }
@@ -642,7 +654,28 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
   const double drho = q.rho;
   double dv[3] = {q.v[0], q.v[1], q.v[2]};
   double src[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (SPH) {
+  if (SPH && NV == 3) {
+    // rotating column: T[l][m] = rho v^m v_l + P delta, the 11 non-zero
+    // conn[l][m][n] in the reference's (l, n) order (rad_fluid.c:195-214)
+    const double *zg = g.zg;
+    const double gc[3] = {g.gcov0, zg[MDS_ZG_GCOV01], zg[MDS_ZG_GCOV02]};
+    double T[3][3];
+    for (int l = 0; l < 3; l++) {
+      for (int m = 0; m < 3; m++) T[l][m] = (drho * dv[m]) * (gc[l] * dv[l]);
+      T[l][l] += dpress;
+    }
+    src[2] += T[0][0] * zg[MDS_ZG_C000];
+    src[2] += T[1][1] * zg[MDS_ZG_C101];
+    src[2] += T[2][2] * zg[MDS_ZG_C202];
+    src[3] += T[0][1] * zg[MDS_ZG_C011];
+    src[3] += T[1][0] * zg[MDS_ZG_C110];
+    src[3] += T[1][1] * zg[MDS_ZG_C111];
+    src[3] += T[2][2] * zg[MDS_ZG_C212];
+    src[4] += T[0][2] * zg[MDS_ZG_C022];
+    src[4] += T[1][2] * zg[MDS_ZG_C122];
+    src[4] += T[2][0] * zg[MDS_ZG_C220];
+    src[4] += T[2][1] * zg[MDS_ZG_C221];
+  } else if (SPH) {
     // radial column, v^1 only: T[0][0] conn000 + P (conn101 + conn202)
     src[2] = ((drho * dv[0]) * (g.gcov0 * dv[0])) * (double)g.c000 + dpress * g.cp;
   }
@@ -681,7 +714,8 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
     }
     if (TAB) U[MDS_YI] = U[MDS_YI] - dt * ((Fp[MDS_YI] - F[MDS_YI]) * g.ivol);
   } else {
-    const double gcov[3] = {g.gcov0, 1.0, 1.0};
+    const bool rot = SPH && NV == 3;
+    const double gcov[3] = {g.gcov0, rot ? g.zg[MDS_ZG_GCOV01] : 1.0, rot ? g.zg[MDS_ZG_GCOV02] : 1.0};
     double Uh[6];
     p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
     Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
@@ -701,7 +735,10 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
   double vout[3] = {0.0, 0.0, 0.0}, ke = 0.0;
   const double rg = drcp(SPH ? U[0] * g.gcov0 : U[0]);
   for (int d = 0; d < NV; d++) {
-    vout[d] = U[2 + d] * ((SPH && d == 0) ? rg : (SPH ? rg * g.gcov0 : rg));
+    // v^d = gcon[d] m_d / rho: gcon[0] = 1 / gcov0 is in rg; rg gcov0 = 1 / rho
+    double f = (SPH && d == 0) ? rg : (SPH ? rg * g.gcov0 : rg);
+    if (SPH && NV == 3 && d > 0) f = (rg * g.gcov0) * g.zg[MDS_ZG_GCON0 + d];
+    vout[d] = U[2 + d] * f;
     ke += U[2 + d] * vout[d];
   }
   const double uout = U[1] - 0.5 * ke;
@@ -844,7 +881,7 @@ __global__ __launch_bounds__(256) void sweep1d_f32(Sweep1DArgs a, StateArrays<fl
     const int s = bc_src<GEN != 0>(c, n, pb.bc_lo[0], pb.bc_hi[0], flip);
     const Prim<float> q = load_zone<NV, TAB, GEN != 0>(pb, in, s, flip);
     Geo32 g[1];
-    load_geo32<SPH>(pb, c, n, g[0]);
+    load_geo32<SPH, NV>(pb, c, n, g[0]);
     Roles r[1];
     r[0].face = (lane >= 2) && (lane <= 62) && (c >= 0) && (c <= n);
     r[0].count = r[0].face && (lane <= 61 || c == n);
@@ -916,7 +953,7 @@ __global__ __launch_bounds__(256) void sweep1d2_f32(Sweep1DArgs a, StateArrays<f
       int flip;
       const int s = bc_src<false>(cz[k], n, pb.bc_lo[0], pb.bc_hi[0], flip);
       q[k] = load_zone<NV, TAB, false>(pb, in, s, flip);
-      load_geo32<SPH>(pb, cz[k], n, g[k]);
+      load_geo32<SPH, NV>(pb, cz[k], n, g[k]);
       conserved_f32<NV, SPH, TAB>(g[k], q[k], U0[k], qn[k]);
       const int p = Z * lane + k;
       const bool inside_f = (cz[k] >= 0) && (cz[k] <= n), inside_z = (cz[k] >= 0) && (cz[k] < n);
@@ -1109,16 +1146,26 @@ extern "C" hipError_t mds_launch_sweep1d(int fp32, int nv, int sph, int tab, int
   } else {
     const StateArrays<float> &p = *(const StateArrays<float> *)sp;
     const StateArrays<float> &h = *(const StateArrays<float> *)sph_;
+    // spherical columns carry v^1 (nv 1) or all three components (nv 3, rotating columns)
     if (stage < 0 && a->zpl == 2) {
-      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 2);
+      if (sph && nv == 3 && tab) MDS_L2(sweep1d2_f32, 3, true, 1, 2);
+      else if (sph && nv == 3) MDS_L2(sweep1d2_f32, 3, true, 0, 2);
+      else if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 2);
       else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0, 2);
       else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0, 2);
       else MDS_L2(sweep1d2_f32, 3, false, 0, 2);
     } else if (stage < 0) {
-      if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 1);
+      if (sph && nv == 3 && tab) MDS_L2(sweep1d2_f32, 3, true, 1, 1);
+      else if (sph && nv == 3) MDS_L2(sweep1d2_f32, 3, true, 0, 1);
+      else if (sph && tab) MDS_L2(sweep1d2_f32, 1, true, 1, 1);
       else if (sph) MDS_L2(sweep1d2_f32, 1, true, 0, 1);
       else if (nv == 1) MDS_L2(sweep1d2_f32, 1, false, 0, 1);
       else MDS_L2(sweep1d2_f32, 3, false, 0, 1);
+    } else if (sph && nv == 3) {
+      if (tab && mono) MDS_L1(sweep1d_f32, 3, true, 1, 1);
+      else if (tab) MDS_L1(sweep1d_f32, 3, true, 1, 0);
+      else if (mono) MDS_L1(sweep1d_f32, 3, true, 0, 1);
+      else MDS_L1(sweep1d_f32, 3, true, 0, 0);
     } else {
       if (sph && tab && mono) MDS_L1(sweep1d_f32, 1, true, 1, 1);
       else if (sph && tab) MDS_L1(sweep1d_f32, 1, true, 1, 0);
