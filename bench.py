@@ -127,7 +127,7 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
     ctx.update_eos()
     if tiles:
         for t, v in zip((mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D, mds.TILE_GRID1D,
-                         mds.TILE_HALO3D), tiles):
+                         mds.TILE_HALO3D, mds.TILE_UPD3S), tiles):
             if t == mds.TILE_FUSE1D and v == 2 and precision != mds.FP32_BULK:
                 v = 1           # two zones per lane is an FP32-bulk shape
             if t not in (mds.TILE_FUSE1D, mds.TILE_GRID1D) or ctx.ndim == 1:
@@ -157,7 +157,7 @@ def run_path(mds, W, precision, steps, warmup, tiles=None):
                 stage_ms=(st1_ms / steps, st2_ms / steps), mass_drift=float(drift), energy_drift=float(edrift),
                 counters=ctx.counters(), tuned_ms=tuned_ms,
                 tiles=[ctx.get_tile(t) for t in (mds.TILE_ROWS2D, mds.TILE_TY3D, mds.TILE_KC3D, mds.TILE_FUSE1D,
-                                                 mds.TILE_GRID1D, mds.TILE_HALO3D)])
+                                                 mds.TILE_GRID1D, mds.TILE_HALO3D, mds.TILE_UPD3S)])
 
 
 def cpu_baseline(name, W, budget_s):
@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--valu-probe", action="store_true",
                     help="print the device's FMA issue rate (wavefront instructions/s) and exit")
     ap.add_argument("--tiles", default=None,
-                    help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D[,HALO3D]]] launch shape to use instead of autotuning "
+                    help="ROWS2D,TY3D,KC3D[,FUSE1D[,GRID1D[,HALO3D[,UPD3S]]]] launch shape to use instead of autotuning "
                          "(profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=None, metavar="BYTES",
                     help="only run the device copy kernel over BYTES (counter calibration) and exit")

@@ -306,6 +306,9 @@ int mds_sync(mds_ctx *ctx);
                                     the whole table) */
 #define MDS_TILE_HALO3D  5       /* 3D: 0 = the four halo rows of a tile on wavefronts of their own,
                                     1 = loaded by the output wavefronts beside their rows */
+#define MDS_TILE_UPD3S   6       /* 2D/3D spherical update workgroup: 0 = 256 zones in storage
+                                    order, 1 = 64 radial zones x 4 theta rows, 2 = 64 x 2 rows x
+                                    2 phi positions, 3 = 64 x 4 phi positions (default) */
 int mds_set_tile(mds_ctx *ctx, int param, int value);
 int mds_get_tile(mds_ctx *ctx, int param, int *value);
 

@@ -768,6 +768,7 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
     L.fp32 = c->precision == MDS_FP32_BULK;
     L.tab = c->tab;
     L.mono = c->mono;
+    L.upd = 3;
     const int nslot = c->tab ? 6 : 5;
     for (int d = 0; d < 3; d++) {
       L.nf[d] = (long long)(h.n[0] + (d == 0)) * (h.n[1] + (d == 1)) * (h.n[2] + (d == 2));
@@ -1469,6 +1470,9 @@ extern "C" int mds_set_tile(mds_ctx *c, int param, int value)
   } else if (param == MDS_TILE_GRID1D) {
     if (value != 0 && value != 1) return set_err(c, MDS_EINVAL, "1D grid mode must be 0 or 1: %d", value);
     slot = &c->grid1d;
+  } else if (param == MDS_TILE_UPD3S) {
+    if (value < 0 || value > 3) return set_err(c, MDS_EINVAL, "spherical update shape must be 0 to 3: %d", value);
+    slot = &c->s3.upd;
   } else {
     return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
   }
@@ -1488,6 +1492,7 @@ extern "C" int mds_get_tile(mds_ctx *c, int param, int *value)
   if (param == MDS_TILE_FUSE1D) { *value = c->fuse1d; return MDS_OK; }
   if (param == MDS_TILE_GRID1D) { *value = c->grid1d; return MDS_OK; }
   if (param == MDS_TILE_HALO3D) { *value = c->halo3d; return MDS_OK; }
+  if (param == MDS_TILE_UPD3S) { *value = c->s3.upd; return MDS_OK; }
   return set_err(c, MDS_EINVAL, "unknown tile parameter %d", param);
 }
 
@@ -1559,14 +1564,17 @@ static int snap_copy(mds_ctx *c, Snapshot &s, bool save)
 extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
 {
   if (!c || nsteps < 1) return MDS_EINVAL;
-  // fixed shapes: 3D spherical grids (one thread per face/zone), and
-  // contexts whose time-dependent PROB hooks run on the host every step
-  if (c->sph3 || c->ntdep > 0) { if (best_ms) *best_ms = 0.0; return MDS_OK; }
+  // fixed shapes: contexts whose time-dependent PROB hooks run on the host
+  // every step
+  if (c->ntdep > 0) { if (best_ms) *best_ms = 0.0; return MDS_OK; }
   // candidates: (a, b, c) = 1D (two-stage launch, persistent grid, -);
-  // 2D (rows per wavefront, -, -); 3D (rows per workgroup, planes, halo mode)
+  // 2D (rows per wavefront, -, -); 3D (rows per workgroup, planes, halo mode);
+  // 2D/3D spherical (update workgroup shape, -, -)
   struct Shape { int a, b, c; };
   std::vector<Shape> cand;
-  if (c->ndim == 1) {
+  if (c->sph3) {
+    for (int v : {0, 1, 2, 3}) cand.push_back({v, 0, 0});
+  } else if (c->ndim == 1) {
     for (int g : {0, 1}) {
       cand.push_back({0, g, 0});
       if (c->can_fuse1d) cand.push_back({1, g, 0});
@@ -1584,14 +1592,16 @@ extern "C" int mds_autotune(mds_ctx *c, int nsteps, double *best_ms)
           if (kc <= c->hprob[0].n[2] && (!gen3d(c) || (ty == 4 && hs == 0))) cand.push_back({ty, kc, hs});
   }
   auto apply = [&](const Shape &v) {
-    if (c->ndim == 1) { c->fuse1d = v.a; c->grid1d = v.b; }
+    if (c->sph3) c->s3.upd = v.a;
+    else if (c->ndim == 1) { c->fuse1d = v.a; c->grid1d = v.b; }
     else if (c->ndim == 2) c->chunk2d = v.a;
     else { c->ty3d = v.a; c->kc3d = v.b; c->halo3d = v.c; }
   };
   Snapshot s;
   int rc = snap_alloc(c, s);
   if (rc == MDS_OK) rc = snap_copy(c, s, true);
-  Shape best = c->ndim == 1 ? Shape{c->fuse1d, c->grid1d, 0}
+  Shape best = c->sph3 ? Shape{c->s3.upd, 0, 0}
+               : c->ndim == 1 ? Shape{c->fuse1d, c->grid1d, 0}
                : c->ndim == 2 ? Shape{c->chunk2d, 0, 0} : Shape{c->ty3d, c->kc3d, c->halo3d};
   double bms = 1e300;
   for (const Shape &v : cand) {

@@ -226,6 +226,7 @@ struct Sph3Launch {
   long long nirr[2];       // (i, j, k, 0) per face; k is the face's own phi index
   int mpl;                 // multipole gravity: lmax (0 = none)
   MpDev mp;
+  int upd;                 // update workgroup shape (MDS_TILE_UPD3S)
 };
 
 #define MDS_SHELL_ROWS 32  // (j, k) rows per wavefront in the 3D shell-mass pass

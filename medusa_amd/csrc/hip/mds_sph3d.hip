@@ -51,6 +51,7 @@ struct Sph3Args {
   unsigned short *tg;         // each zone's temperature bracket of the last EOS call
   const double *grav;         // monopole field per radial shell, or null
   const double *mpsrc;        // multipole sources [6][zones], or null
+  int upd;                    // update workgroup shape (update_zone_of)
 };
 
 MDS_DEV long long zid3(const DevProb &pb, int i, int j, int k)
@@ -1324,27 +1325,54 @@ __global__ __launch_bounds__(256) void sph3_flux_irr(Sph3Args a, StateArrays<R> 
 // ---------------------------------------------------------------------------
 // zone update: sources, flux divergence, RK combination, u -> p, dt limits
 // ---------------------------------------------------------------------------
-// Zones to threads: a workgroup takes 64 consecutive i (one wavefront each)
-// at four consecutive phi positions k (four consecutive theta rows j on
-// grids with fewer than four phi zones).  The four wavefronts read the same
-// (i, j) records, and a phi face is the upper face of one and the lower face
-// of the next, so those reads come from the compute unit's cache instead of
-// once per zone from memory.
-MDS_DEV void update_zone_of(const DevProb &pb, int &i, int &j, int &k, bool &inbox)
+// Zones to threads, by the workgroup shape (MDS_TILE_UPD3S): 0, 256
+// consecutive zones in storage order; 1, 64 consecutive i at four theta rows
+// j; 2, 64 i at two rows and two phi positions; 3, 64 i at four phi
+// positions k (shape 1 on grids with fewer than four phi zones).  In shapes
+// 1-3 the wavefronts of a workgroup read neighbouring records and faces
+// (the same (i, j) records and a phi face twice in shape 3), which then come
+// from the compute unit's cache instead of once per zone from memory.
+MDS_DEV void update_zone_of(const DevProb &pb, int shape, int &i, int &j, int &k, bool &inbox)
 {
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
+  if (shape == 0) {
+    const long long z = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    inbox = z < (long long)n1 * n2 * n3;
+    const long long zl = inbox ? z : 0;
+    i = (int)(zl % n1);
+    j = (int)((zl / n1) % n2);
+    k = (int)(zl / ((long long)n1 * n2));
+    return;
+  }
+  if (shape == 3 && n3 < 4) shape = 1;
+  if (shape == 2 && n3 < 2) shape = 1;
   const int nti = (n1 + 63) >> 6, w = threadIdx.x >> 6;
   const int b = blockIdx.x, ti = b % nti, r = b / nti;
   i = (ti << 6) + (threadIdx.x & 63);
-  if (n3 >= 4) {
+  if (shape == 3) {
     j = r % n2;
     k = ((r / n2) << 2) + w;
+  } else if (shape == 2) {
+    const int njg = (n2 + 1) >> 1;
+    j = ((r % njg) << 1) + (w & 1);
+    k = ((r / njg) << 1) + (w >> 1);
   } else {
     const int njg = (n2 + 3) >> 2;
     j = ((r % njg) << 2) + w;
     k = r / njg;
   }
   inbox = i < n1 && j < n2 && k < n3;
+}
+
+// workgroups of the update's grid for a shape (the same rules)
+static unsigned update_blocks(const DevProb &pb, int shape)
+{
+  const long long n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2], nti = (n1 + 63) / 64;
+  if (shape == 0) return (unsigned)((n1 * n2 * n3 + 255) / 256);
+  if ((shape == 3 && n3 < 4) || (shape == 2 && n3 < 2)) shape = 1;
+  if (shape == 3) return (unsigned)(nti * n2 * ((n3 + 3) / 4));
+  if (shape == 2) return (unsigned)(nti * ((n2 + 1) / 2) * ((n3 + 1) / 2));
+  return (unsigned)(nti * ((n2 + 3) / 4) * n3);
 }
 
 template <class R, int TAB, int MONO, int DEND>
@@ -1354,7 +1382,7 @@ __global__ __launch_bounds__(256, 3) void sph3_update(Sph3Args a, StateArrays<R>
   const int n1 = pb.n[0], n2 = pb.n[1], n3 = pb.n[2];
   int iz, jz, kz;
   bool inbox;
-  update_zone_of(pb, iz, jz, kz, inbox);
+  update_zone_of(pb, a.upd, iz, jz, kz, inbox);
   const long long zl = inbox ? zid3(pb, iz, jz, kz) : 0;
   const int i0 = inbox ? iz : 0, j0 = inbox ? jz : 0, k0 = inbox ? kz : 0;
   // level relations with the neighbours (dendritic grids; all 0 otherwise),
@@ -1699,10 +1727,7 @@ static hipError_t launch_stage(const Sph3Launch &L, const Sph3Args &a, const Sta
 #undef MDS_FLUX
 #undef MDS_FLUX_IRR
   const dim3 grd((unsigned)((nz + 255) / 256));
-  // the update's grid: 64 i by four phi positions (theta rows below four phi zones) per workgroup
-  const unsigned nti = (unsigned)((pb.n[0] + 63) / 64);
-  const dim3 gupd(pb.n[2] >= 4 ? nti * (unsigned)pb.n[1] * (unsigned)((pb.n[2] + 3) / 4)
-                               : nti * (unsigned)((pb.n[1] + 3) / 4) * (unsigned)pb.n[2]);
+  const dim3 gupd(update_blocks(pb, a.upd));
   if (L.mpl) {
     // multipole gravity: moments of the stage state, the corner potential and
     // the sources, after this stage's mass fluxes (all directions) are known
@@ -1739,6 +1764,7 @@ static hipError_t stage_impl(const Sph3Launch &L, const DevProb &pb, DevClock *c
   a.tg = L.tg;
   a.grav = L.grav;
   a.mpsrc = L.mpl ? L.mp.src : nullptr;
+  a.upd = L.upd;
   const hipError_t e = L.dend ? launch_stage<R, 1>(L, a, sp, sph, stage, st) : launch_stage<R, 0>(L, a, sp, sph, stage, st);
   if (e != hipSuccess) return e;
   return hipGetLastError();

@@ -29,6 +29,7 @@ TILE_KC3D = 2
 TILE_FUSE1D = 3
 TILE_GRID1D = 4
 TILE_HALO3D = 5
+TILE_UPD3S = 6
 ZG_COUNT = 24
 ZG_VOLUME, ZG_SCALE0 = 0, 12            # 1D spherical record fields (MDS_ZG_*)
 # 3D spherical per-(i,j) record (MDS_ZS_*)

@@ -290,6 +290,35 @@ def test_dendritic_blast_matches_oracle(precision):
         assert hl[0] == sim.shock_faces() and hl[1] == sim.hlle_faces()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_update_workgroup_shapes_agree(precision):
+    """The spherical update's workgroup shapes (MDS_TILE_UPD3S 0-3) give the
+    same bits on a dendritic blast crossing theta and phi level changes,
+    and the autotuner (which tries them) leaves the state as it found it."""
+    from medusa_amd import mds
+    n = (24, 16, 32)
+    g, prob = _dend_problem(n)
+    prims = _blast_prims(n, g)
+    outs = []
+    for shape in (0, 1, 2, 3):
+        print(f"[dend] update shape {shape} precision={precision}", flush=True)
+        ctx = mds.Context([prob], precision=precision, rp=mds.run_params(cfl=0.5))
+        ctx.set_tile(mds.TILE_UPD3S, shape)
+        assert ctx.get_tile(mds.TILE_UPD3S) == shape
+        ctx.upload(prims)
+        ctx.update_eos()
+        if shape == 3:
+            ctx.autotune(2)
+            ctx.set_tile(mds.TILE_UPD3S, shape)
+        ctx.run(5)
+        ctx.run(3, graph=True)
+        outs.append((ctx.download(), ctx.clock(), ctx.counters()))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0], o[0])
+        assert outs[0][1] == o[1] and outs[0][2] == o[2]
+
+
 def star_sound_speed(ref, g, T, gam, valid):
     """zone sound speed of an oracle state (gamma law, or the oracle's table walk)"""
     if T is None:
