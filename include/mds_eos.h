@@ -73,7 +73,8 @@ int mds_eos_table_load_burrows(const char *path, mds_eos_table *t);
  * [erg/cm^3] (u_given_rtx, eos_collapse.c:172-181), P [dyn/cm^2], Gamma_1,
  * cs = sqrt(Gamma_1 P / rho)}.  rho, temp and Ye outside the table are
  * clamped to its edges (the reference reads beyond its arrays there).
- * Returns MDS_EINVAL for a non-positive rho or temp. */
+ * Returns MDS_EINVAL for a non-positive rho or temp, or a table with fewer
+ * than two nodes along an axis. */
 int mds_eos_rtx(const mds_eos_table *t, double rho, double temp, double ye, double out[4]);
 
 /* Default synthetic table dimensions: 180 x 180 x 50 over log10 rho in

@@ -178,7 +178,8 @@ void mds_eos_table_free(mds_eos_table *t)
  * record summed in the reference's corner order */
 int mds_eos_rtx(const mds_eos_table *t, double rho, double temp, double ye, double out[4])
 {
-  if (!t || !t->data || !out || !(rho > 0.0) || !(temp > 0.0)) return MDS_EINVAL;
+  if (!t || !t->data || !out || !(rho > 0.0) || !(temp > 0.0) || t->nr < 2 || t->nt < 2 || t->ny < 2)
+    return MDS_EINVAL;
   const double dr = (t->lr1 - t->lr0) / (t->nr - 1);
   const double dtt = (t->lt1 - t->lt0) / (t->nt - 1);
   const double dy = (t->ye1 - t->ye0) / (t->ny - 1);
