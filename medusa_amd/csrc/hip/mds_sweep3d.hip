@@ -82,7 +82,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   };
 
   Cell<R, NV> Z[5];                 // zw rows: planes k-2 .. k+2
-  R Fxp_cur[5], Fxm_cur[5], Fzm[5];
+  R Fxm_cur[5], Fzm[5];            // x flux at the lower face; the upper face's comes from the lane above when used
   Cell<R, NV> zl_cur, zr_cur;
   int tag_cur = 1;
   if (zw) {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
-    for (int q = 0; q < 2 + NV; q++) { Fxp_cur[q] = lane_from_above(Fx[q]); Fxm_cur[q] = Fx[q]; }
+    for (int q = 0; q < 2 + NV; q++) Fxm_cur[q] = Fx[q];
     typename ReconGeoT<R>::type rg;
     Cell<R, NV> zl_prev, zr_prev;
     load_rgeo<R>(pb, 2, k0 - 1, rg);
@@ -118,7 +118,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
 
   for (int k = k0; k < k1; k++) {
     Cell<R, NV> me;
-    R Fxp_next[5], Fxm_next[5], Fzp[5];
+    R Fxm_next[5], Fzp[5];
     Cell<R, NV> zl_next, zr_next;
     int tag_next = 1, mytag = 1;
     if (zw) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
-        for (int q = 0; q < 2 + NV; q++) { Fxp_next[q] = lane_from_above(F2[q]); Fxm_next[q] = F2[q]; }
+        for (int q = 0; q < 2 + NV; q++) Fxm_next[q] = F2[q];
       } else {
         tag_next = zone_tag(xshock_only<R, SLAB>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
       }
@@ -192,6 +192,10 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     }
     __syncthreads();
 
+    // the upper x face of this zone is the lower face of the lane above
+    // (a DPP shift, taken here where every lane of the wavefront is active)
+    R Fxp_cur[5];
+    for (int q = 0; q < 2 + NV; q++) Fxp_cur[q] = lane_from_above(Fxm_cur[q]);
     bool bad_live = false;
     double bad_ro = 1.0, bad_uo = 0.0;
     if (col_out) {
@@ -274,7 +278,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       for (int q = 0; q < 4; q++) Z[q] = Z[q + 1];
       zl_cur = zl_next; zr_cur = zr_next;
       tag_cur = tag_next;
-      for (int q = 0; q < 2 + NV; q++) { Fzm[q] = Fzp[q]; Fxp_cur[q] = Fxp_next[q]; Fxm_cur[q] = Fxm_next[q]; }
+      for (int q = 0; q < 2 + NV; q++) { Fzm[q] = Fzp[q]; Fxm_cur[q] = Fxm_next[q]; }
     }
   }
 
@@ -385,7 +389,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   };
 
   Cell<R, NV> Z[5];                 // zw rows: planes k-2 .. k+2
-  R Fxp_cur[5], Fxm_cur[5], Fzm[5];
+  R Fxm_cur[5], Fzm[5];            // x flux at the lower face; the upper face's comes from the lane above when used
   Cell<R, NV> zl_cur, zr_cur;
   int tag_cur = 1;
   if (zw) {
@@ -399,7 +403,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
-    for (int q = 0; q < 2 + NV; q++) { Fxp_cur[q] = lane_from_above(Fx[q]); Fxm_cur[q] = Fx[q]; }
+    for (int q = 0; q < 2 + NV; q++) Fxm_cur[q] = Fx[q];
     typename ReconGeoT<R>::type rg;
     Cell<R, NV> zl_prev, zr_prev;
     load_rgeo<R>(pb, 2, k0 - 1, rg);
@@ -421,7 +425,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
 
   for (int k = k0; k < k1; k++) {
     Cell<R, NV> me;
-    R Fxp_next[5], Fxm_next[5], Fzp[5];
+    R Fxm_next[5], Fzp[5];
     Cell<R, NV> zl_next, zr_next;
     int tag_next = 1, mytag = 1;
     if (zw) {
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
-        for (int q = 0; q < 2 + NV; q++) { Fxp_next[q] = lane_from_above(F2[q]); Fxm_next[q] = F2[q]; }
+        for (int q = 0; q < 2 + NV; q++) Fxm_next[q] = F2[q];
       } else {
         tag_next = zone_tag(xshock_only<R, SLAB>(Z[3].P, pextra(k + 1), rgx)) | s1tag(k + 1);
       }
@@ -515,6 +519,10 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     }
     __syncthreads();
 
+    // the upper x face of this zone is the lower face of the lane above
+    // (a DPP shift, taken here where every lane of the wavefront is active)
+    R Fxp_cur[5];
+    for (int q = 0; q < 2 + NV; q++) Fxp_cur[q] = lane_from_above(Fxm_cur[q]);
     bool bad_live = false;
     double bad_ro = 1.0, bad_uo = 0.0;
     if (col_out) {
@@ -597,7 +605,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
       for (int q = 0; q < 4; q++) Z[q] = Z[q + 1];
       zl_cur = zl_next; zr_cur = zr_next;
       tag_cur = tag_next;
-      for (int q = 0; q < 2 + NV; q++) { Fzm[q] = Fzp[q]; Fxp_cur[q] = Fxp_next[q]; Fxm_cur[q] = Fxm_next[q]; }
+      for (int q = 0; q < 2 + NV; q++) { Fzm[q] = Fzp[q]; Fxm_cur[q] = Fxm_next[q]; }
     }
   }
 
