@@ -618,6 +618,42 @@ static int create_impl(mds_ctx *c, const mds_problem *probs, int nprob, const md
       }
       d.rcu = uni ? 1 : 0;
     }
+    // the transverse metric ratios and connection coefficients of a
+    // spherical column, when every zone shares them to 1e-10 (a log-r column)
+    d.tcu = 0;
+    if (c->sph && !c->sph3 && g->zone) {
+      const int nzk = g->n[0] + 2 * MDS_NG;
+      const int cf[11] = {MDS_ZG_C000, MDS_ZG_C101, MDS_ZG_C202, MDS_ZG_C011, MDS_ZG_C110, MDS_ZG_C111,
+                          MDS_ZG_C212, MDS_ZG_C022, MDS_ZG_C122, MDS_ZG_C220, MDS_ZG_C221};
+      auto val = [&](int q, int t) -> double {
+        const double *z = g->zone + (size_t)q * MDS_ZG_COUNT;
+        switch (t) {
+          case MDS_TC_G1: return z[MDS_ZG_GCOV01] / z[MDS_ZG_GCOV00];
+          case MDS_TC_G2: return z[MDS_ZG_GCOV02] / z[MDS_ZG_GCOV00];
+          case MDS_TC_GI1: return z[MDS_ZG_GCON1] * z[MDS_ZG_GCOV00];
+          case MDS_TC_GI2: return z[MDS_ZG_GCON2] * z[MDS_ZG_GCOV00];
+          case MDS_TC_F1: return z[MDS_ZG_GCOVF1] / z[MDS_ZG_GCOVF0];
+          case MDS_TC_F2: return z[MDS_ZG_GCOVF2] / z[MDS_ZG_GCOVF0];
+          default: return z[cf[t - MDS_TC_C000]];
+        }
+      };
+      // zone values over the interior zones (the only ones updated), the
+      // lower-face ratios over the faces 0 .. n (ghost records carry no
+      // connection coefficients)
+      bool uni = true;
+      for (int t = 0; t < MDS_TC_COUNT && uni; t++) {
+        const double ref = val(MDS_NG, t);
+        double scale = fabs(ref);
+        for (int tt = MDS_TC_C000; tt < MDS_TC_COUNT && t >= MDS_TC_C000; tt++) scale = fmax(scale, fabs(val(MDS_NG, tt)));
+        const int q1 = (t == MDS_TC_F1 || t == MDS_TC_F2) ? nzk - MDS_NG : nzk - MDS_NG - 1;
+        for (int q = MDS_NG; q <= q1 && uni; q++) {
+          const double v = val(q, t);
+          uni = std::isfinite(v) && fabs(v - ref) <= 1.0e-10 * scale;
+        }
+        d.tcu_c[t] = ref;
+      }
+      d.tcu = uni ? 1 : 0;
+    }
     // ghost layers of the PROB and EXTRAP sides
     d.has_slab = 0;
     {

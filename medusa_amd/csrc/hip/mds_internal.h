@@ -33,6 +33,14 @@ struct DevEos {
 };
 
 // One problem as the kernels see it.
+// DevProb::tcu_c: gcov[0][1..2] / gcov[0][0], gcon[1..2] * gcov[0][0], face
+// gcov[1][1..2] / gcov[1][0], then the 11 non-zero connection coefficients
+enum {
+  MDS_TC_G1 = 0, MDS_TC_G2, MDS_TC_GI1, MDS_TC_GI2, MDS_TC_F1, MDS_TC_F2,
+  MDS_TC_C000, MDS_TC_C101, MDS_TC_C202, MDS_TC_C011, MDS_TC_C110, MDS_TC_C111, MDS_TC_C212,
+  MDS_TC_C022, MDS_TC_C122, MDS_TC_C220, MDS_TC_C221, MDS_TC_COUNT
+};
+
 struct DevProb {
   long long off;           // first interior zone in the packed state arrays
   int n[3];                // interior zones
@@ -52,6 +60,12 @@ struct DevProb {
   // Cartesian grid are the same up to scale and shift in every zone
   int rcu;
   float rcu_c[MDS_RC_COUNT];
+  // FP32-bulk spherical columns with three velocity components: the
+  // transverse metric over the radial one and the connection coefficients
+  // shared by every zone (1: tcu_c holds them, MDS_TC_*) -- constant on a
+  // log-r column, whose geometry is self-similar; else read per zone
+  int tcu;
+  double tcu_c[MDS_TC_COUNT];
   const double *mom[3];    // per direction (n[d]+2NG)*3: alpha, beta, Gamma per zone
   const float *rc[3];      // per direction (n[d]+2NG)*MDS_RC_COUNT
   const double *zone;      // spherical columns: (n[0]+2NG) x MDS_ZG_COUNT records; 3D spherical:

@@ -547,8 +547,23 @@ struct Geo32 {
   FaceGeo<float> fg;        // FP32 face metric for the Riemann solve (unit area)
   double ivol, gcov0, cp;
   float c000, dcen;
-  const double *zg;         // spherical columns with three velocity components: the FP64 zone record
+  // spherical columns with three velocity components: the zone's transverse
+  // gcov[0][1..2], gcon[1..2] gcov[0][0], and its connection coefficients
+  // (MDS_TC_C* order) -- the problem's shared constants on a log-r column,
+  // else the FP64 zone record
+  double gc1, gc2, gi1, gi2;
+  const double *zg;
+  const double *tc;
 };
+
+// connection coefficient t (MDS_TC_C000 ..) of a rotating column's zone
+MDS_DEV double conn_of(const Geo32 &g, int t)
+{
+  if (g.tc) return g.tc[t];
+  const int map[MDS_TC_COUNT] = {0, 0, 0, 0, 0, 0, MDS_ZG_C000, MDS_ZG_C101, MDS_ZG_C202, MDS_ZG_C011, MDS_ZG_C110,
+                                 MDS_ZG_C111, MDS_ZG_C212, MDS_ZG_C022, MDS_ZG_C122, MDS_ZG_C220, MDS_ZG_C221};
+  return g.zg[map[t]];
+}
 
 template <bool SPH, int NV = 1>
 MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
@@ -557,15 +572,8 @@ MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
   g.fg.area = 1.0f;
   g.fg.gcov[1] = g.fg.gcov[2] = 1.0f;
   g.zg = nullptr;
-  if (SPH && NV == 3) {
-    // the transverse components need the zone's gcov[0][1..2], gcon[1..2]
-    // and connection coefficients, and the face's gcov[1][1..2]: read from
-    // the FP64 record (rotating columns; the radial-only path keeps its
-    // 48-byte record)
-    g.zg = pb.zone + (size_t)q * MDS_ZG_COUNT;
-    g.fg.gcov[1] = (float)g.zg[MDS_ZG_GCOVF1];
-    g.fg.gcov[2] = (float)g.zg[MDS_ZG_GCOVF2];
-  }
+  g.tc = nullptr;
+  g.gc1 = g.gc2 = g.gi1 = g.gi2 = 1.0;
   if (SPH) {
     const ZoneF32 &z = pb.z32[q];
     g.area = z.area;
@@ -576,6 +584,28 @@ MDS_DEV void load_geo32(const DevProb &pb, int c, int n, Geo32 &g)
     g.cp = z.cp;
     g.c000 = z.c000;
     g.dcen = z.dcen;
+    if (NV == 3) {
+      // rotating columns: the transverse metric and connection coefficients,
+      // shared by the problem on a log-r column (uniform branch), else from
+      // the FP64 zone record (the radial-only path keeps its 48-byte record)
+      if (pb.tcu) {
+        g.tc = pb.tcu_c;
+        g.gc1 = g.gcov0 * pb.tcu_c[MDS_TC_G1];
+        g.gc2 = g.gcov0 * pb.tcu_c[MDS_TC_G2];
+        g.gi1 = pb.tcu_c[MDS_TC_GI1];
+        g.gi2 = pb.tcu_c[MDS_TC_GI2];
+        g.fg.gcov[1] = g.fg.gcov[0] * (float)pb.tcu_c[MDS_TC_F1];
+        g.fg.gcov[2] = g.fg.gcov[0] * (float)pb.tcu_c[MDS_TC_F2];
+      } else {
+        g.zg = pb.zone + (size_t)q * MDS_ZG_COUNT;
+        g.gc1 = g.zg[MDS_ZG_GCOV01];
+        g.gc2 = g.zg[MDS_ZG_GCOV02];
+        g.gi1 = g.zg[MDS_ZG_GCON1] * g.gcov0;
+        g.gi2 = g.zg[MDS_ZG_GCON2] * g.gcov0;
+        g.fg.gcov[1] = (float)g.zg[MDS_ZG_GCOVF1];
+        g.fg.gcov[2] = (float)g.zg[MDS_ZG_GCOVF2];
+      }
+    }
   } else {
     g.area = pb.area[0];
     g.fg.scale = 1.0f;
@@ -606,7 +636,7 @@ MDS_DEV void conserved_f32(const Geo32 &g, const Prim<float> &q, double U0[6], P
   qn.rho = q.rho; qn.uu = q.uu; qn.ye = q.ye;
   for (int d = 0; d < 3; d++) qn.v[d] = q.v[d];
   const bool rot = SPH && NV == 3;
-  const double gcov[3] = {g.gcov0, rot ? g.zg[MDS_ZG_GCOV01] : 1.0, rot ? g.zg[MDS_ZG_GCOV02] : 1.0};
+  const double gcov[3] = {g.gcov0, rot ? g.gc1 : 1.0, rot ? g.gc2 : 1.0};
   p_to_u_x64<NV, !SPH>(qn.rho, qn.uu, qn.v, gcov, U0);
   U0[MDS_YI] = TAB ? U0[0] * qn.ye : 0.0;
 }
@@ -657,24 +687,23 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
   if (SPH && NV == 3) {
     // rotating column: T[l][m] = rho v^m v_l + P delta, the 11 non-zero
     // conn[l][m][n] in the reference's (l, n) order (rad_fluid.c:195-214)
-    const double *zg = g.zg;
-    const double gc[3] = {g.gcov0, zg[MDS_ZG_GCOV01], zg[MDS_ZG_GCOV02]};
+    const double gc[3] = {g.gcov0, g.gc1, g.gc2};
     double T[3][3];
     for (int l = 0; l < 3; l++) {
       for (int m = 0; m < 3; m++) T[l][m] = (drho * dv[m]) * (gc[l] * dv[l]);
       T[l][l] += dpress;
     }
-    src[2] += T[0][0] * zg[MDS_ZG_C000];
-    src[2] += T[1][1] * zg[MDS_ZG_C101];
-    src[2] += T[2][2] * zg[MDS_ZG_C202];
-    src[3] += T[0][1] * zg[MDS_ZG_C011];
-    src[3] += T[1][0] * zg[MDS_ZG_C110];
-    src[3] += T[1][1] * zg[MDS_ZG_C111];
-    src[3] += T[2][2] * zg[MDS_ZG_C212];
-    src[4] += T[0][2] * zg[MDS_ZG_C022];
-    src[4] += T[1][2] * zg[MDS_ZG_C122];
-    src[4] += T[2][0] * zg[MDS_ZG_C220];
-    src[4] += T[2][1] * zg[MDS_ZG_C221];
+    src[2] += T[0][0] * conn_of(g, MDS_TC_C000);
+    src[2] += T[1][1] * conn_of(g, MDS_TC_C101);
+    src[2] += T[2][2] * conn_of(g, MDS_TC_C202);
+    src[3] += T[0][1] * conn_of(g, MDS_TC_C011);
+    src[3] += T[1][0] * conn_of(g, MDS_TC_C110);
+    src[3] += T[1][1] * conn_of(g, MDS_TC_C111);
+    src[3] += T[2][2] * conn_of(g, MDS_TC_C212);
+    src[4] += T[0][2] * conn_of(g, MDS_TC_C022);
+    src[4] += T[1][2] * conn_of(g, MDS_TC_C122);
+    src[4] += T[2][0] * conn_of(g, MDS_TC_C220);
+    src[4] += T[2][1] * conn_of(g, MDS_TC_C221);
   } else if (SPH) {
     // radial column, v^1 only: T[0][0] conn000 + P (conn101 + conn202)
     src[2] = ((drho * dv[0]) * (g.gcov0 * dv[0])) * (double)g.c000 + dpress * g.cp;
@@ -715,7 +744,7 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
     if (TAB) U[MDS_YI] = U[MDS_YI] - dt * ((Fp[MDS_YI] - F[MDS_YI]) * g.ivol);
   } else {
     const bool rot = SPH && NV == 3;
-    const double gcov[3] = {g.gcov0, rot ? g.zg[MDS_ZG_GCOV01] : 1.0, rot ? g.zg[MDS_ZG_GCOV02] : 1.0};
+    const double gcov[3] = {g.gcov0, rot ? g.gc1 : 1.0, rot ? g.gc2 : 1.0};
     double Uh[6];
     p_to_u_x64<NV, !SPH>(drho, (double)q.uu, dv, gcov, Uh);
     Uh[MDS_YI] = TAB ? Uh[0] * (double)q.ye : 0.0;
@@ -737,7 +766,8 @@ MDS_DEV Prim<float> zone_update_f32(const DevProb &pb, const Geo32 &g, int stage
   for (int d = 0; d < NV; d++) {
     // v^d = gcon[d] m_d / rho: gcon[0] = 1 / gcov0 is in rg; rg gcov0 = 1 / rho
     double f = (SPH && d == 0) ? rg : (SPH ? rg * g.gcov0 : rg);
-    if (SPH && NV == 3 && d > 0) f = (rg * g.gcov0) * g.zg[MDS_ZG_GCON0 + d];
+    // rotating columns: gcon[d] / rho = rg gcov0 gcon[d] = rg gi_d
+    if (SPH && NV == 3 && d > 0) f = rg * (d == 1 ? g.gi1 : g.gi2);
     vout[d] = U[2 + d] * f;
     ke += U[2 + d] * vout[d];
   }

@@ -112,3 +112,36 @@ def test_rotating_star_column_table_monopole_fp32_matches_oracle():
     assert np.max(np.abs(ref[4] * scale[:, 2])) > 1.0e8
     for k, v in err.items():
         assert v <= 1e-5, (k, v)
+
+
+def test_rotating_uniform_radius_column_reads_the_zone_records():
+    """On a uniform-r column the connection coefficients change from zone to
+    zone, so the rotating FP32 path reads them from the zone records (on a
+    log-r column they are one set per problem): the same parity bound."""
+    n, nsteps = 200, 60
+    g = mds.GridBuilder(1, [n], [0.0], [1.0], geom=mds.GEOM_SPHERICAL, rcoord=mds.RCOORD_UNIFORM)
+    x = (np.arange(n) + 0.5) / n
+    prims = np.zeros((5, 1, 1, n))
+    prims[0, 0, 0] = 1.0 + 0.5 * np.exp(-((x - 0.3) / 0.05) ** 2)
+    prims[1, 0, 0] = 1.0 + 9.0 * (x < 0.35)
+    prims[2, 0, 0] = 0.05 * np.sin(3 * x)
+    zs = g.zone()[2:-2]
+    scale = np.sqrt(zs[:, 2:5])
+    prims[3, 0, 0] = 0.2 * np.exp(-((x - 0.5) / 0.1) ** 2) / scale[:, 1]
+    prims[4, 0, 0] = 0.5 * x * np.exp(-x) / scale[:, 2]
+    prob = dict(grid=g, bc_lo=[mds.BC_SPHERICAL_ORIGIN], bc_hi=[mds.BC_OUTFLOW], gam=5.0 / 3.0, nvel=3)
+    print(f"[rotating] FP32-bulk nvel=3 uniform-r column n={n} from the origin, {nsteps} steps", flush=True)
+    dev = _run(prob, prims, nsteps).download()[:, 0, 0]
+    sim = po.Sim(oracle_config(prob, 0.5, nvars=5))
+    sim.set_prims(prims)
+    sim.run(max_steps=nsteps)
+    ref = sim.get_prims()[:, 0, 0]
+    w = zs[:, mds.ZG_VOLUME]
+    rel = lambda a, b: float(np.sum(np.abs(a - b) * w) / np.sum(np.abs(b) * w))
+    cs = np.sqrt(5.0 / 3.0 * (2.0 / 3.0) * ref[1] / ref[0])
+    err = {"rho": rel(dev[0], ref[0]), "u": rel(dev[1], ref[1])}
+    for d in range(3):
+        err[f"v{d + 1}/cs"] = float(np.sum(np.abs(dev[2 + d] - ref[2 + d]) * scale[:, d] * w) / np.sum(cs * w))
+    print(f"[rotating]   L1 {err}", flush=True)
+    for k, v in err.items():
+        assert v <= 1e-5, (k, v)
