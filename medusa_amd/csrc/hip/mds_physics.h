@@ -410,13 +410,43 @@ MDS_DEV void para_recon_f32(float qm, float q0, float qp, const ReconGeo32 &g, f
   ar = r;
 }
 
+// Gamma-only factors of the FP32 wave-speed estimate, each formed exactly as
+// wave_speeds_f32 forms it from a side's Gamma_1 (with G_l = G_r = G, G_avg
+// is G itself).  A gamma law needs one set per launch: gam_f32_uniform makes
+// it once and keeps it in scalar registers, and the shock and rarefaction
+// branches then take no reciprocal of Gamma per face.
+struct GamF32 {
+  float ravg;  // (G - 1) rcp(2 G): exponent of the two-rarefaction pressure ratio
+  float igm1;  // rcp(G - 1)
+  float hgm1;  // (G - 1) / 2
+  float pex;   // 2 G rcp(G - 1): power of the two-rarefaction star pressure
+  float gp1;   // G + 1
+  float bgm;   // (G - 1) rcp(G + 1): the two-shock b / p
+  float kq;    // (G + 1) rcp(2 G): the shock factor's slope
+};
+MDS_DEV float uniform_f32(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+MDS_DEV GamF32 gam_f32_uniform(float G)
+{
+  GamF32 k;
+  k.ravg = uniform_f32((G - 1.0f) * frcp(2.0f * G));
+  k.igm1 = uniform_f32(frcp(G - 1.0f));
+  k.hgm1 = uniform_f32((G - 1.0f) * 0.5f);
+  k.pex = uniform_f32(2.0f * G * frcp(G - 1.0f));
+  k.gp1 = uniform_f32(G + 1.0f);
+  k.bgm = uniform_f32((G - 1.0f) * frcp(G + 1.0f));
+  k.kq = uniform_f32((G + 1.0f) * frcp(2.0f * G));
+  return k;
+}
+
 // EAGER: both pressure reciprocals up front, serving the ratio and both
 // shock factors (faster where shock factors are common, as on Cartesian
-// boxes); otherwise the shock factors divide only where a wavefront needs them
-template <bool HLLC_FLOOR, bool EAGER = false>
+// boxes); otherwise the shock factors divide only where a wavefront needs them.
+// UNIG: G_l = G_r, with its factors in *gk (a gamma law).
+template <bool HLLC_FLOOR, bool EAGER = false, bool UNIG = false>
 MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l, float G_r,
                              float u_l, float u_r, float cs_l, float cs_r,
-                             float &sl, float &sr, float &ss, float &p_s, int &isshock)
+                             float &sl, float &sr, float &ss, float &p_s, int &isshock,
+                             const GamF32 *gk = nullptr)
 {
 #pragma clang fp contract(fast)
   float p_max = fmaxf(p_l, p_r), p_min = fminf(p_l, p_r);
@@ -435,17 +465,21 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
   if (pratio < 2.0f && p_s >= p_min && p_s <= p_max) {
   } else if (p_s < p_min) {
     float G_avg = 0.5f * (G_l + G_r);
-    float plr = fpow(p_l * (EAGER ? ipr : frcp(p_r)), (G_avg - 1.0f) * frcp(2.0f * G_avg));
+    const float ex = UNIG ? gk->ravg : (G_avg - 1.0f) * frcp(2.0f * G_avg);
+    float plr = fpow(p_l * (EAGER ? ipr : frcp(p_r)), ex);
     float icl = frcp(cs_l), icr = frcp(cs_r);
-    float s0 = (plr * u_l * icl + u_r * icr + 2.0f * (plr - 1.0f) * frcp(G_avg - 1.0f)) * frcp(plr * icl + icr);
-    p_s = 0.5f * (p_l * fpow(1.0f + (G_l - 1.0f) * 0.5f * icl * (u_l - s0), 2.0f * G_l * frcp(G_l - 1.0f))
-                  + p_r * fpow(1.0f + (G_r - 1.0f) * 0.5f * icr * (s0 - u_r), 2.0f * G_r * frcp(G_r - 1.0f)));
+    const float igm1 = UNIG ? gk->igm1 : frcp(G_avg - 1.0f);
+    float s0 = (plr * u_l * icl + u_r * icr + 2.0f * (plr - 1.0f) * igm1) * frcp(plr * icl + icr);
+    const float hl = UNIG ? gk->hgm1 : (G_l - 1.0f) * 0.5f, hr = UNIG ? gk->hgm1 : (G_r - 1.0f) * 0.5f;
+    const float el = UNIG ? gk->pex : 2.0f * G_l * frcp(G_l - 1.0f);
+    const float er = UNIG ? gk->pex : 2.0f * G_r * frcp(G_r - 1.0f);
+    p_s = 0.5f * (p_l * fpow(1.0f + hl * icl * (u_l - s0), el) + p_r * fpow(1.0f + hr * icr * (s0 - u_r), er));
     p_s = fmaxf(p_s, 0.0f);
   } else {
-    float a_l = 2.0f * frcp((G_l + 1.0f) * rl);
-    float a_r = 2.0f * frcp((G_r + 1.0f) * rr);
-    float b_l = (G_l - 1.0f) * frcp(G_l + 1.0f) * p_l;
-    float b_r = (G_r - 1.0f) * frcp(G_r + 1.0f) * p_r;
+    float a_l = 2.0f * frcp((UNIG ? gk->gp1 : G_l + 1.0f) * rl);
+    float a_r = 2.0f * frcp((UNIG ? gk->gp1 : G_r + 1.0f) * rr);
+    float b_l = (UNIG ? gk->bgm : (G_l - 1.0f) * frcp(G_l + 1.0f)) * p_l;
+    float b_r = (UNIG ? gk->bgm : (G_r - 1.0f) * frcp(G_r + 1.0f)) * p_r;
     // sqrt(a / (p + b)) as sqrt(a) / sqrt(p + b): the quotient itself falls
     // below the FP32 normal range at stellar-core densities and pressures
     // (2 / (2.3 rho) / (p + b) near 1e-38 at rho 1e10, p 1e28), where the
@@ -455,8 +489,10 @@ MDS_DEV void wave_speeds_f32(float rl, float rr, float p_l, float p_r, float G_l
     p_s = (g_l * p_l + g_r * p_r - (u_r - u_l)) * frcp(g_l + g_r);
     p_s = fmaxf(p_s, 0.0f);
   }
-  float q_l = (p_s <= p_l) ? 1.0f : fsqrt(1.0f + (G_l + 1.0f) * frcp(2.0f * G_l) * (p_s * (EAGER ? ipl : frcp(p_l)) - 1.0f));
-  float q_r = (p_s <= p_r) ? 1.0f : fsqrt(1.0f + (G_r + 1.0f) * frcp(2.0f * G_r) * (p_s * (EAGER ? ipr : frcp(p_r)) - 1.0f));
+  const float kl = UNIG ? gk->kq : (G_l + 1.0f) * frcp(2.0f * G_l);
+  const float kr = UNIG ? gk->kq : (G_r + 1.0f) * frcp(2.0f * G_r);
+  float q_l = (p_s <= p_l) ? 1.0f : fsqrt(1.0f + kl * (p_s * (EAGER ? ipl : frcp(p_l)) - 1.0f));
+  float q_r = (p_s <= p_r) ? 1.0f : fsqrt(1.0f + kr * (p_s * (EAGER ? ipr : frcp(p_r)) - 1.0f));
   sl = u_l - cs_l * q_l;
   sr = u_r + cs_r * q_r;
   ss = (p_r - p_l + rl * u_l * (sl - u_l) - rr * u_r * (sr - u_r)) * frcp(rl * (sl - u_l) - rr * (sr - u_r));
@@ -499,10 +535,11 @@ MDS_DEV void p_uflux_f32(const PState<float, NV> &p, const FaceGeo<float> &g, in
   if (NS) { u[MDS_YI] = p.rho * p.ye; f[MDS_YI] = f[0] * p.ye; }
 }
 
-// HLLC in FP32; flux per unit area.  Returns isshock.
-template <int NV, bool CART, int NS = 0>
+// HLLC in FP32; flux per unit area.  Returns isshock.  UNIG: a gamma law,
+// whose wave-speed factors *gk carries.
+template <int NV, bool CART, int NS = 0, bool UNIG = false>
 MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
-                     const FaceGeo<float> &g, int dir, float flux[6], float &sig)
+                     const FaceGeo<float> &g, int dir, float flux[6], float &sig, const GamF32 *gk = nullptr)
 {
 #pragma clang fp contract(fast)
   float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -511,7 +548,8 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
   float sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_f32<true, CART>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_f32<true, CART, UNIG>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s,
+                                    isshock, gk);
   if (ss > sr) sr = 1.001f * ss;
   if (ss < sl) sl = 1.001f * ss;
   float iscale = CART ? 1.0f : frcp(g.scale);
@@ -540,9 +578,9 @@ MDS_DEV int hllc_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   return isshock;
 }
 
-template <int NV, bool CART, int NS = 0>
+template <int NV, bool CART, int NS = 0, bool UNIG = false>
 MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
-                      const FaceGeo<float> &g, int dir, float flux[6], float &sig)
+                      const FaceGeo<float> &g, int dir, float flux[6], float &sig, const GamF32 *gk = nullptr)
 {
 #pragma clang fp contract(fast)
   float u_l = CART ? L.v[dir] : L.v[dir] * g.scale;
@@ -551,7 +589,8 @@ MDS_DEV void hlle_f32(const PState<float, NV> &L, const PState<float, NV> &Rt,
   float cs_r = fsqrt(Rt.G * Rt.P * frcp(Rt.rho));
   float sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_f32<false, CART>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_f32<false, CART, UNIG>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s,
+                                     isshock, gk);
   float ul[6], ur[6], fl[6], fr[6];
   p_uflux_f32<NV, CART, NS>(L, g, dir, ul, fl);
   p_uflux_f32<NV, CART, NS>(Rt, g, dir, ur, fr);

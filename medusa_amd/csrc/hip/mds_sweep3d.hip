@@ -53,6 +53,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   const R gam = (R)pb.gam;
   const R gm1 = (R)(pb.gam - 1.0);
   const R igm1 = (R)(1.0 / (pb.gam - 1.0));
+  const GamF32 gk = gam_f32_uniform((float)pb.gam);   // FP32 wave-speed factors (unused in FP64)
   const StateArrays<R> &in = (stage == 0) ? sp : sph;
   const bool zw = (w >= 2) && (w <= TY + 1);             // rows with a z window
   const bool row_in = (j >= 0) && (j < n2);
@@ -92,7 +93,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     else tag_prev = zone_tag(xshock_only<R, SLAB>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
     R Fx[5], sx;
     int shx;
-    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
+    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, gk, igm1, pb.area[0], Fx, sx, shx);
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
     recon_cell<R, NV, SLAB>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
     bool hl = (tag_prev | tag_cur) != 0;
     R sz;
-    solve<R, NV>(zr_prev, zl_cur, gam, pb.area[2], 2, hl, Fzm, sz);
+    solve<R, NV>(zr_prev, zl_cur, gam, gk, pb.area[2], 2, hl, Fzm, sz);
     if (col_out) {
       smax[2] = sz > smax[2] ? sz : smax[2];
       if (k0 == 0) nhlle += hl;
@@ -128,7 +129,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       } else if (k + 1 < k1) {
         R F2[5], s2;
         int sh2;
-        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
+        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, gk, igm1, pb.area[0], F2, s2, sh2);
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
@@ -141,7 +142,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       recon_cell<R, NV, SLAB>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
       bool hl = (tag_cur | tag_next) != 0;
       R sz;
-      solve<R, NV>(zr_cur, zl_next, gam, pb.area[2], 2, hl, Fzp, sz);
+      solve<R, NV>(zr_cur, zl_next, gam, gk, pb.area[2], 2, hl, Fzp, sz);
       if (col_out) {
         smax[2] = sz > smax[2] ? sz : smax[2];
         nhlle += hl;
@@ -185,7 +186,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
       yL.P = sAr[2 + NV][w - 1][lane];
       bool hl = (sTag[w][lane] | sTag[w - 1][lane]) != 0;
       R Fy[5], sy;
-      solve<R, NV>(yL, yl, gam, pb.area[1], 1, hl, Fy, sy);
+      solve<R, NV>(yL, yl, gam, gk, pb.area[1], 1, hl, Fy, sy);
       for (int q = 0; q < 2 + NV; q++) sFy[q][w][lane] = Fy[q];
       if (yface) smax[1] = sy > smax[1] ? sy : smax[1];
       if (ycount) nhlle += hl;
@@ -345,6 +346,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   const R gam = (R)pb.gam;
   const R gm1 = (R)(pb.gam - 1.0);
   const R igm1 = (R)(1.0 / (pb.gam - 1.0));
+  const GamF32 gk = gam_f32_uniform((float)pb.gam);   // FP32 wave-speed factors (unused in FP64)
   const StateArrays<R> &in = (stage == 0) ? sp : sph;
   const bool zw = true;                                  // every wavefront has a z window
   const bool row_in = j < n2;
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     else tag_prev = zone_tag(xshock_only<R, SLAB>(Z[1].P, pextra(k0 - 1), rgx)) | s1tag(k0 - 1);
     R Fx[5], sx;
     int shx;
-    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, igm1, pb.area[0], Fx, sx, shx);
+    xsweep<R, NV, SLAB>(Z[2], pextra(k0), rgx, gam, gk, igm1, pb.area[0], Fx, sx, shx);
     if (xface_ok) smax[0] = sx > smax[0] ? sx : smax[0];
     if (xface_ok && lane < 62) nshock += shx;
     tag_cur = row_in ? (zone_tag(shx) | s1tag(k0)) : 1;
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     recon_cell<R, NV, SLAB>(Z[1], Z[2], Z[3], rg, igm1, zl_cur, zr_cur);
     bool hl = (tag_prev | tag_cur) != 0;
     R sz;
-    solve<R, NV>(zr_prev, zl_cur, gam, pb.area[2], 2, hl, Fzm, sz);
+    solve<R, NV>(zr_prev, zl_cur, gam, gk, pb.area[2], 2, hl, Fzm, sz);
     if (col_out) {
       smax[2] = sz > smax[2] ? sz : smax[2];
       if (k0 == 0) nhlle += hl;
@@ -435,7 +437,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
       } else if (k + 1 < k1) {
         R F2[5], s2;
         int sh2;
-        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, igm1, pb.area[0], F2, s2, sh2);
+        xsweep<R, NV, SLAB>(Z[3], pextra(k + 1), rgx, gam, gk, igm1, pb.area[0], F2, s2, sh2);
         if (xface_ok) smax[0] = s2 > smax[0] ? s2 : smax[0];
         if (xface_ok && lane < 62) nshock += sh2;
         tag_next = zone_tag(sh2) | s1tag(k + 1);
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
       recon_cell<R, NV, SLAB>(Z[2], Z[3], Z[4], rg, igm1, zl_next, zr_next);
       bool hl = (tag_cur | tag_next) != 0;
       R sz;
-      solve<R, NV>(zr_cur, zl_next, gam, pb.area[2], 2, hl, Fzp, sz);
+      solve<R, NV>(zr_cur, zl_next, gam, gk, pb.area[2], 2, hl, Fzp, sz);
       if (col_out) {
         smax[2] = sz > smax[2] ? sz : smax[2];
         nhlle += hl;
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     {
       const bool hl = (sTag[r][lane] | sTag[r - 1][lane]) != 0;
       R Fy[5], sy;
-      solve<R, NV>(get(sAr, r - 1), yl, gam, pb.area[1], 1, hl, Fy, sy);
+      solve<R, NV>(get(sAr, r - 1), yl, gam, gk, pb.area[1], 1, hl, Fy, sy);
       for (int q = 0; q < 2 + NV; q++) sFy[q][r][lane] = Fy[q];
       if (yface) smax[1] = sy > smax[1] ? sy : smax[1];
       if (ycount) nhlle += hl;
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
     if (top) {
       const bool hl = (sTag[TY + 2][lane] | sTag[TY + 1][lane]) != 0;
       R Fy[5], sy;
-      solve<R, NV>(yr, ylh, gam, pb.area[1], 1, hl, Fy, sy);
+      solve<R, NV>(yr, ylh, gam, gk, pb.area[1], 1, hl, Fy, sy);
       for (int q = 0; q < 2 + NV; q++) sFy[q][TY + 2][lane] = Fy[q];
       if (ytop) smax[1] = sy > smax[1] ? sy : smax[1];
       if (ytopcount) nhlle += hl;

@@ -173,10 +173,11 @@ MDS_DEV void recon_cell(const Cell<R, NV> &m, const Cell<R, NV> &c, const Cell<R
 
 // Riemann solve in direction d; returns isshock.  FP64: area-weighted flux
 // in reference order.  FP32: flux per unit area (the FP64 accumulate applies
-// the area).  |signal speed| returned in ssig.
-template <class R, int NV>
-MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, double area, int d, bool use_hlle,
-                  R F[5], R &ssig)
+// the area), with the gamma law's wave-speed factors from gk (UNIG) or formed
+// per face.  |signal speed| returned in ssig.
+template <class R, int NV, bool UNIG = true>
+MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, const GamF32 &gk, double area, int d,
+                  bool use_hlle, R F[5], R &ssig)
 {
   PState<R, NV> L, Rt;
   L.rho = l.rho; L.uu = l.uu; L.P = l.P; L.G = gam;
@@ -194,8 +195,8 @@ MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, double area
   } else {
     g.area = 1.0f;
     float s = 0.0f;
-    if (use_hlle) hlle_f32<NV, true>(L, Rt, g, d, F, s);
-    else sh = hllc_f32<NV, true>(L, Rt, g, d, F, s);
+    if (use_hlle) hlle_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk);
+    else sh = hllc_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk);
     ssig = fabsf(s);
   }
   return sh;
@@ -204,9 +205,9 @@ MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, double area
 // x sweep of one row: per-lane flux through the lower x face, its shock tag
 // and |signal speed|.  Lane 63 additionally needs P of the zone above it so
 // that the tag of its own face is exact (pextra).
-template <class R, int NV, bool SLAB = false>
-MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>::type &rgx, R gam, R igm1,
-                    double area, R F[5], R &ssig, int &isshock)
+template <class R, int NV, bool SLAB = false, bool UNIG = true>
+MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>::type &rgx, R gam,
+                    const GamF32 &gk, R igm1, double area, R F[5], R &ssig, int &isshock)
 {
   const int lane = threadIdx.x & 63;
   Cell<R, NV> mm, pp, al, ar;
@@ -226,7 +227,7 @@ MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>
   L.uu = lane_from_below(ar.uu);
   for (int d = 0; d < NV; d++) L.v[d] = lane_from_below(ar.v[d]);
   L.P = lane_from_below(ar.P);
-  isshock = solve<R, NV>(L, al, gam, area, 0, false, F, ssig);
+  isshock = solve<R, NV, UNIG>(L, al, gam, gk, area, 0, false, F, ssig);
 }
 
 // shock tags only (P reconstruction + the HLLC pressure-jump test, riemann.c:374-383)
