@@ -155,11 +155,40 @@ MDS_DEV void recon_x64(double qm, double q0, double qp, const ReconGeo64 &g, dou
   else { al = q0; ar = q0; }
 }
 
-// wave-speed estimate shared by HLLC/HLLE (riemann.c:210-303, 368-429)
-template <bool HLLC_FLOOR>
+// The quotients of Gamma alone in the wave-speed estimate, for a gamma law
+// (G_l = G_r = G, so G_avg is G itself).  One set per launch, in scalar
+// registers: a quotient of the same operands has the same bits wherever it
+// is formed, so the faces' results do not change.
+struct GamF64 {
+  double ravg;  // (G - 1) / (2 G)
+  double pex;   // (2 G) / (G - 1)
+  double bgm;   // (G - 1) / (G + 1)
+  double kq;    // (G + 1) / (2 G)
+};
+MDS_DEV double uniform_f64(double x)
+{
+  const long long v = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+MDS_DEV GamF64 gam_f64_uniform(double G)
+{
+  GamF64 k;
+  k.ravg = uniform_f64((G - 1.0) / (2.0 * G));
+  k.pex = uniform_f64((2.0 * G) / (G - 1.0));
+  k.bgm = uniform_f64((G - 1.0) / (G + 1.0));
+  k.kq = uniform_f64((G + 1.0) / (2.0 * G));
+  return k;
+}
+
+// wave-speed estimate shared by HLLC/HLLE (riemann.c:210-303, 368-429);
+// UNIG: G_l = G_r, with its quotients in *gk (a gamma law)
+template <bool HLLC_FLOOR, bool UNIG = false>
 MDS_DEV void wave_speeds_x64(double rl, double rr, double p_l, double p_r, double G_l, double G_r,
                              double u_l, double u_r, double cs_l, double cs_r,
-                             double &sl, double &sr, double &ss, double &p_s, int &isshock)
+                             double &sl, double &sr, double &ss, double &p_s, int &isshock,
+                             const GamF64 *gk = nullptr)
 {
   double p_max, p_min;
   if (p_l > p_r) { p_max = p_l; p_min = p_r; } else { p_max = p_r; p_min = p_l; }
@@ -171,24 +200,28 @@ MDS_DEV void wave_speeds_x64(double rl, double rr, double p_l, double p_r, doubl
   if (pratio < 2.0 && p_s >= p_min && p_s <= p_max) {
   } else if (p_s < p_min) {
     double G_avg = 0.5 * (G_l + G_r);
-    double plr = pow(p_l / p_r, (G_avg - 1.0) / (2.0 * G_avg));
+    double plr = pow(p_l / p_r, UNIG ? gk->ravg : (G_avg - 1.0) / (2.0 * G_avg));
     double s0 = ((((plr * u_l) / cs_l) + (u_r / cs_r)) + ((2.0 * (plr - 1.0)) / (G_avg - 1.0)))
                 / ((plr / cs_l) + (1.0 / cs_r));
-    p_s = 0.5 * ((p_l * pow(1.0 + (((G_l - 1.0) / (2.0 * cs_l)) * (u_l - s0)), (2.0 * G_l) / (G_l - 1.0)))
-                 + (p_r * pow(1.0 + (((G_r - 1.0) / (2.0 * cs_r)) * (s0 - u_r)), (2.0 * G_r) / (G_r - 1.0))));
+    const double el = UNIG ? gk->pex : (2.0 * G_l) / (G_l - 1.0);
+    const double er = UNIG ? gk->pex : (2.0 * G_r) / (G_r - 1.0);
+    p_s = 0.5 * ((p_l * pow(1.0 + (((G_l - 1.0) / (2.0 * cs_l)) * (u_l - s0)), el))
+                 + (p_r * pow(1.0 + (((G_r - 1.0) / (2.0 * cs_r)) * (s0 - u_r)), er)));
     p_s = (p_s > 0.0) ? p_s : 0.0;
   } else {
     double a_l = 2.0 / ((G_l + 1.0) * rl);
     double a_r = 2.0 / ((G_r + 1.0) * rr);
-    double b_l = ((G_l - 1.0) / (G_l + 1.0)) * p_l;
-    double b_r = ((G_r - 1.0) / (G_r + 1.0)) * p_r;
+    double b_l = (UNIG ? gk->bgm : (G_l - 1.0) / (G_l + 1.0)) * p_l;
+    double b_r = (UNIG ? gk->bgm : (G_r - 1.0) / (G_r + 1.0)) * p_r;
     double g_l = sqrt(a_l / (p_s + b_l));
     double g_r = sqrt(a_r / (p_s + b_r));
     p_s = (((g_l * p_l) + (g_r * p_r)) - (u_r - u_l)) / (g_l + g_r);
     p_s = (p_s > 0.0) ? p_s : 0.0;
   }
-  double q_l = (p_s <= p_l) ? 1.0 : sqrt(1.0 + (((G_l + 1.0) / (2.0 * G_l)) * ((p_s / p_l) - 1.0)));
-  double q_r = (p_s <= p_r) ? 1.0 : sqrt(1.0 + (((G_r + 1.0) / (2.0 * G_r)) * ((p_s / p_r) - 1.0)));
+  const double kl = UNIG ? gk->kq : (G_l + 1.0) / (2.0 * G_l);
+  const double kr = UNIG ? gk->kq : (G_r + 1.0) / (2.0 * G_r);
+  double q_l = (p_s <= p_l) ? 1.0 : sqrt(1.0 + (kl * ((p_s / p_l) - 1.0)));
+  double q_r = (p_s <= p_r) ? 1.0 : sqrt(1.0 + (kr * ((p_s / p_r) - 1.0)));
   sl = u_l - (cs_l * q_l);
   sr = u_r + (cs_r * q_r);
   ss = (((p_r - p_l) + ((rl * u_l) * (sl - u_l))) - ((rr * u_r) * (sr - u_r)))
@@ -235,9 +268,9 @@ MDS_DEV void pflux_x64(const PState<double, NV> &p, const FaceGeo<double> &g, in
 }
 
 // HLLC (riemann.c:328-499).  Returns isshock.
-template <int NV, bool CART, int NS = 0>
+template <int NV, bool CART, int NS = 0, bool UNIG = false>
 MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
-                     const FaceGeo<double> &g, int dir, double flux[6], double &sig)
+                     const FaceGeo<double> &g, int dir, double flux[6], double &sig, const GamF64 *gk = nullptr)
 {
   if (g.area < 1.0e-16) {
     sig = 1.0e-200;
@@ -251,7 +284,7 @@ MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   double cs_r = sqrt((Rt.G * Rt.P) / Rt.rho);
   double sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_x64<true>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_x64<true, UNIG>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock, gk);
   if (ss > sr) sr = 1.001 * ss;
   if (ss < sl) sl = 1.001 * ss;
   double smax = (fabs(sl) > fabs(sr)) ? fabs(sl) : fabs(sr);
@@ -284,9 +317,9 @@ MDS_DEV int hllc_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
 }
 
 // HLLE (riemann.c:178-325)
-template <int NV, bool CART, int NS = 0>
+template <int NV, bool CART, int NS = 0, bool UNIG = false>
 MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
-                      const FaceGeo<double> &g, int dir, double flux[6], double &sig)
+                      const FaceGeo<double> &g, int dir, double flux[6], double &sig, const GamF64 *gk = nullptr)
 {
   if (g.area < 1.0e-16) {
     sig = 1.0e-200;
@@ -300,7 +333,7 @@ MDS_DEV void hlle_x64(const PState<double, NV> &L, const PState<double, NV> &Rt,
   double cs_r = sqrt((Rt.G * Rt.P) / Rt.rho);
   double sl, sr, ss, p_s;
   int isshock;
-  wave_speeds_x64<false>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock);
+  wave_speeds_x64<false, UNIG>(L.rho, Rt.rho, L.P, Rt.P, L.G, Rt.G, u_l, u_r, cs_l, cs_r, sl, sr, ss, p_s, isshock, gk);
   double ul[6], ur[6], fl[6], fr[6];
   p_uflux_x64<NV, CART, NS>(L, g, dir, ul, fl);
   p_uflux_x64<NV, CART, NS>(Rt, g, dir, ur, fr);
@@ -435,6 +468,19 @@ MDS_DEV GamF32 gam_f32_uniform(float G)
   k.gp1 = uniform_f32(G + 1.0f);
   k.bgm = uniform_f32((G - 1.0f) * frcp(G + 1.0f));
   k.kq = uniform_f32((G + 1.0f) * frcp(2.0f * G));
+  return k;
+}
+
+// both precisions' sets for a gamma law (a kernel uses the one it needs)
+struct GamSet {
+  GamF32 f32;
+  GamF64 f64;
+};
+MDS_DEV GamSet gam_set_uniform(double G)
+{
+  GamSet k;
+  k.f32 = gam_f32_uniform((float)G);
+  k.f64 = gam_f64_uniform(G);
   return k;
 }
 

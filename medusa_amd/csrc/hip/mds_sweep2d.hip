@@ -46,9 +46,9 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   const R gam = (R)pb.gam;
   const R gm1 = (R)(pb.gam - 1.0);
   const R igm1 = (R)(1.0 / (pb.gam - 1.0));
-  // FP32: the wave-speed factors of Gamma are formed per face here (one set
-  // per launch measured 0.8% slower on a 2048^2 Kelvin-Helmholtz box)
-  const GamF32 gk{};
+  // the wave-speed factors of Gamma are formed per face here (one set per
+  // launch measured 0.8% slower in FP32 on a 2048^2 Kelvin-Helmholtz box)
+  const GamSet gk{};
   const double area0 = pb.area[0], area1 = pb.area[1];
   const StateArrays<R> &in = (stage == 0) ? sp : sph;
   const bool col_out = (lane >= 2) && (lane <= 61) && (i >= 0) && (i < n1);

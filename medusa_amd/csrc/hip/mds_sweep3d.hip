@@ -53,7 +53,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void sweep3d(Sweep3DArgs a, StateArr
   const R gam = (R)pb.gam;
   const R gm1 = (R)(pb.gam - 1.0);
   const R igm1 = (R)(1.0 / (pb.gam - 1.0));
-  const GamF32 gk = gam_f32_uniform((float)pb.gam);   // FP32 wave-speed factors (unused in FP64)
+  const GamSet gk = gam_set_uniform(pb.gam);   // the gamma law's wave-speed factors
   const StateArrays<R> &in = (stage == 0) ? sp : sph;
   const bool zw = (w >= 2) && (w <= TY + 1);             // rows with a z window
   const bool row_in = (j >= 0) && (j < n2);
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(TY * 64) void sweep3dh(Sweep3DArgs a, StateArrays<R
   const R gam = (R)pb.gam;
   const R gm1 = (R)(pb.gam - 1.0);
   const R igm1 = (R)(1.0 / (pb.gam - 1.0));
-  const GamF32 gk = gam_f32_uniform((float)pb.gam);   // FP32 wave-speed factors (unused in FP64)
+  const GamSet gk = gam_set_uniform(pb.gam);   // the gamma law's wave-speed factors
   const StateArrays<R> &in = (stage == 0) ? sp : sph;
   const bool zw = true;                                  // every wavefront has a z window
   const bool row_in = j < n2;

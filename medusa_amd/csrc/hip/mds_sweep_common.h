@@ -173,10 +173,10 @@ MDS_DEV void recon_cell(const Cell<R, NV> &m, const Cell<R, NV> &c, const Cell<R
 
 // Riemann solve in direction d; returns isshock.  FP64: area-weighted flux
 // in reference order.  FP32: flux per unit area (the FP64 accumulate applies
-// the area), with the gamma law's wave-speed factors from gk (UNIG) or formed
-// per face.  |signal speed| returned in ssig.
+// the area).  UNIG: the gamma law's wave-speed factors come from gk instead
+// of being formed per face.  |signal speed| returned in ssig.
 template <class R, int NV, bool UNIG = true>
-MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, const GamF32 &gk, double area, int d,
+MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, const GamSet &gk, double area, int d,
                   bool use_hlle, R F[5], R &ssig)
 {
   PState<R, NV> L, Rt;
@@ -189,14 +189,14 @@ MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, const GamF3
   if constexpr (std::is_same<R, double>::value) {
     g.area = area;
     double s = 0.0;
-    if (use_hlle) hlle_x64<NV, true>(L, Rt, g, d, F, s);
-    else sh = hllc_x64<NV, true>(L, Rt, g, d, F, s);
+    if (use_hlle) hlle_x64<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk.f64);
+    else sh = hllc_x64<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk.f64);
     ssig = fabs(s);
   } else {
     g.area = 1.0f;
     float s = 0.0f;
-    if (use_hlle) hlle_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk);
-    else sh = hllc_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk);
+    if (use_hlle) hlle_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk.f32);
+    else sh = hllc_f32<NV, true, 0, UNIG>(L, Rt, g, d, F, s, &gk.f32);
     ssig = fabsf(s);
   }
   return sh;
@@ -207,7 +207,7 @@ MDS_DEV int solve(const Cell<R, NV> &l, const Cell<R, NV> &r, R gam, const GamF3
 // that the tag of its own face is exact (pextra).
 template <class R, int NV, bool SLAB = false, bool UNIG = true>
 MDS_DEV void xsweep(const Cell<R, NV> &me, R pextra, const typename ReconGeoT<R>::type &rgx, R gam,
-                    const GamF32 &gk, R igm1, double area, R F[5], R &ssig, int &isshock)
+                    const GamSet &gk, R igm1, double area, R F[5], R &ssig, int &isshock)
 {
   const int lane = threadIdx.x & 63;
   Cell<R, NV> mm, pp, al, ar;
