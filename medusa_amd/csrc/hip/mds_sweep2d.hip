@@ -63,11 +63,42 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   double locmax = 0.0, srcnum = 1.0e300, lim_src = 1.0e300;
   unsigned long long nshock = 0, nhlle = 0, nbad = 0;
 
+  // The lane's column and the row mapping's inputs, read once: the problem
+  // record sits in global memory, and after the march's stores the compiler
+  // would read its fields again for every row.  The column mapping (x) is
+  // fixed per lane, the row mapping (y) is uniform.  The general instance
+  // (PROB/EXTRAP ghost layers) keeps load_cell.
+  bool fxi, fxp;
+  const int si = bc_src2(i, n1, pb.bc_lo[0], pb.bc_hi[0], fxi);
+  const int sip = bc_src2(i + 1, n1, pb.bc_lo[0], pb.bc_hi[0], fxp);
+  const int blo1 = pb.bc_lo[1], bhi1 = pb.bc_hi[1];
+  const long long poff = pb.off;
+  auto load_row = [&](int j, Cell<R, NV> &c) {
+    if constexpr (SLAB) {
+      load_cell<R, NV, SLAB>(in, pb, i, j, 0, gm1, c);
+    } else {
+      bool fy;
+      const int sj = bc_src2(j, n2, blo1, bhi1, fy);
+      const long long idx = poff + (long long)sj * n1 + si;
+      c.rho = in.v[0][idx];
+      c.uu = in.v[1][idx];
+      for (int d = 0; d < NV; d++) c.v[d] = in.v[2 + d][idx];
+      if (fxi) c.v[0] = -c.v[0];
+      if (fy && NV > 1) c.v[1] = -c.v[1];
+      c.P = gm1 * c.uu;
+    }
+  };
   Cell<R, NV> W[5];
-  for (int k = 0; k < 4; k++) load_cell<R, NV, SLAB>(in, pb, i, j0 - 2 + k, 0, gm1, W[k]);
+  for (int k = 0; k < 4; k++) load_row(j0 - 2 + k, W[k]);
   auto pextra_of = [&](int j) -> R {
     R pe = (R)1;
-    if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, j, 0, gm1);
+    if constexpr (SLAB) {
+      if (lane == 63) pe = load_press<R, SLAB>(in, pb, i + 1, j, 0, gm1);
+    } else if (lane == 63) {
+      bool fy;
+      const int sj = bc_src2(j, n2, blo1, bhi1, fy);
+      pe = gm1 * in.v[1][poff + (long long)sj * n1 + sip];
+    }
     return pe;
   };
   auto stage1_tag = [&](int j) -> int {
@@ -110,7 +141,7 @@ __global__ __launch_bounds__(256) void sweep2d(Sweep2DArgs a, StateArrays<R> sp,
   const double ffdt = ff * dt;
   const double idx0 = 1.0 / pb.dx[0], idx1 = 1.0 / pb.dx[1];
   for (int j = j0; j < j1; j++) {
-    load_cell<R, NV, SLAB>(in, pb, i, j + 2, 0, gm1, W[4]);
+    load_row(j + 2, W[4]);
     int tag_next;
     R Fxp_next[5], Fxm_next[5];
     if (j + 1 >= n2) {
